@@ -1,0 +1,170 @@
+"""Headline benchmark: SIMP iterations/s of the 192x96x96 cantilever
+(1,769,472 elements, 5,447,811 DOF) with 4-level MG-PCG, mixed precision
+(FP32 operator and V-cycle inside FP64 refinement) against the plain FP64
+path, plus GDOF-matvec/s of the fine applyA.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 192,96,96]
+
+A step is one SIMP iteration (projection/filter, FE solve to tol 1e-6,
+energies, sensitivities, filter, OC) on device-resident data. Prints one
+JSON line.
+"""
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_solver(T, P, grid, prec, steps, warmup, levels):
+    nx, ny, nz = grid
+    if (nx, ny, nz) == (192, 96, 96):
+        cfg, fixed, F, pres, void, x0 = P.cantilever_c3(precision=prec, nlevels=levels)
+    else:
+        cfg, fixed, F, pres, void, x0 = P.cantilever(nx, ny, nz, linear_solver=T.SOLVER_MGPCG, precision=prec,
+                                                     mg_levels=levels, mg_omega=0.6,
+                                                     coarse_solver=T.COARSE_CHOLESKY, galerkin_max_ndof=0)
+        nel, _ = T.dims(nx, ny, nz)
+        x0 = P.splitmix64_uniform(nel)
+    cfg.mg_omega_cap = 1.6
+    cfg.max_iterations = 1000
+    S = T.Solver(cfg, fixed, F)
+    S.set_design(x0)
+    infos = []
+    for i in range(warmup):
+        t0 = time.perf_counter()
+        info = S.step()
+        log("warmup %d prec=%d: %.1f ms cg=%d outer=%d c=%.6g" % (i, prec, (time.perf_counter() - t0) * 1e3,
+                                                                  info.cg_iters, info.outer_iters, info.compliance))
+    t0 = time.perf_counter()
+    for i in range(steps):
+        info = S.step()
+        infos.append(info.as_dict())
+        log("step %d prec=%d: %.1f ms cg=%d outer=%d c=%.8g" % (i, prec, info.t_total_ms, info.cg_iters,
+                                                               info.outer_iters, info.compliance))
+    t1 = time.perf_counter()
+    return S, infos, (t1 - t0) / steps
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--grid", default="192,96,96")
+    ap.add_argument("--levels", type=int, default=4)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: this build has no multi-GPU path (see DESIGN.md, 'Sharding').",
+              file=sys.stderr)
+        sys.exit(2)
+    import topologyoptimization_amd as T
+    import topologyoptimization_amd.problems as P
+    grid = tuple(int(v) for v in args.grid.split(","))
+    n, name = T.device_info()
+    assert n > 0, "no HIP device"
+    log("device:", name)
+    nel, ndof = T.dims(*grid)
+
+    # mixed precision (the headline)
+    S, infos, t_mixed = run_solver(T, P, grid, T.PREC_MIXED, args.steps, args.warmup, args.levels)
+    # dominant kernel: fine-level FP32 applyA, timed with HIP events on the solver's stream
+    ms32, by32 = S.time_op(T.TIME_APPLYA, T.PREC_MIXED, 0, 50)
+    ms64, by64 = S.time_op(T.TIME_APPLYA, T.PREC_FP64, 0, 30)
+    msc, byc = S.time_op(T.TIME_COPY, T.PREC_FP64, 0, 30)
+    msv, _ = S.time_op(T.TIME_VCYCLE, T.PREC_MIXED, 0, 5)
+    S.close()
+    res = {"mixed_ms": t_mixed * 1e3, "mixed_infos": infos}
+    t_fp64 = None
+    if not args.no_fp64:
+        S64, infos64, t_fp64 = run_solver(T, P, grid, T.PREC_FP64, max(1, args.steps // 2), args.warmup, args.levels)
+        S64.close()
+        res["fp64_infos"] = infos64
+    value = 1.0 / t_mixed
+    ach = by32 / (ms32 * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": ach, "peak": 8000.0, "unit": "GB/s", "frac": ach / 8000.0,
+            "traffic": None, "kernel": "k_applyA (FP32, fine level)",
+            "algorithmic_bytes": by32, "avg_ms": ms32}
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(grid, infos)
+    fine_calls = np.mean([i["applyA_fine"] for i in infos])
+    out = {
+        "metric": "SIMP iters/s + GDOF-matvec/s at 5.4M DOF, mixed-FP32 vs FP64 MGPCG",
+        "value": value,
+        "unit": "SIMP iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_mixed * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64",
+        "data": "synthetic",
+        "config": {"workload": "cantilever %dx%dx%d hex8, %d DOF, %d-level MG-PCG V(2,2) omega 0.6 (capped), FP64 "
+                               "direct coarsest, tol 1e-6, x0~U[0.05,1] seed %d" % (grid + (ndof, args.levels, P.SEED)),
+                   "nel": nel, "ndof": ndof, "parallelism": "single GPU"},
+        "fp64_iters_per_s": (1.0 / t_fp64) if t_fp64 else None,
+        "mixed_vs_fp64": (t_fp64 / t_mixed) if t_fp64 else None,
+        "gdof_matvec_per_s": {"fp32": ndof / (ms32 * 1e-3) / 1e9, "fp64": ndof / (ms64 * 1e-3) / 1e9},
+        "applyA_ms": {"fp32": ms32, "fp64": ms64}, "vcycle_ms_fp32": msv,
+        "copy_GBps": byc / (msc * 1e-3) / 1e9,
+        "fine_applyA_per_step": fine_calls,
+        "compliance": infos[-1]["compliance"],
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "device": name,
+    }
+    print(json.dumps(out))
+
+
+def cpu_baseline(grid, infos):
+    """The oracle (single-threaded C++ restatement of the reference
+    algorithm) on the host: fine-grid applyA calls at the same grid timed for
+    about 10 s; SIMP iters/s estimated as 1 / (fine applyA per step on the
+    GPU x CPU seconds per applyA)."""
+    try:
+        from oracle import oracle as O
+        nx, ny, nz = grid
+        nel, ndof = O.dims(nx, ny, nz)
+        ke = O.k0(0.3)
+        rng = np.random.default_rng(0)
+        E = 1e-9 + rng.uniform(0.05, 1.0, nel) ** 3
+        act = np.arange(nel, dtype=np.int32)
+        p = rng.normal(size=ndof)
+        t0 = time.perf_counter()
+        calls = 0
+        while True:
+            O.applyA(nx, ny, nz, ke, E, act, p)
+            calls += 1
+            if time.perf_counter() - t0 > 8.0:
+                break
+        t_call = (time.perf_counter() - t0) / calls
+        fine = float(np.mean([i["applyA_fine"] for i in infos])) * 1.0
+        # a V-cycle costs ~ (nu1+nu2+1) * 8/7 fine applyA equivalents
+        per_step = (fine + float(np.mean([i["vcycles"] for i in infos])) * 5 * 8 / 7) * t_call
+        return {"value": 1.0 / per_step, "unit": "SIMP iters/s", "cores": 1, "kind": "port",
+                "sample": "%d oracle applyA calls at %dx%dx%d (%.2f s each, %.2f GDOF-matvec/s); per-step cost = "
+                          "GPU step's fine applyA + 5*8/7 per V-cycle applyA-equivalents" %
+                          (calls, nx, ny, nz, t_call, ndof / t_call / 1e9),
+                "host": platform.processor() or platform.machine(), "nproc": os.cpu_count()}
+    except Exception as e:  # reported, never fatal
+        return {"value": None, "error": repr(e)}
+
+
+if __name__ == "__main__":
+    main()

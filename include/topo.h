@@ -1,0 +1,279 @@
+/*
+ * topo.h -- C ABI of libtopo_hip.so, the MI355X (gfx950) implementation of the
+ * 3-D SIMP inner loop of Jozo132/TopologyOptimization.
+ *
+ * Conventions shared by every entry point
+ *   - plain C types only; every pointer is a HOST pointer unless the name says
+ *     otherwise; arrays are caller-owned;
+ *   - return value: >= 0 success (some entry points return an iteration
+ *     count, exactly like the WASM export they replace), < 0 an error code
+ *     (TOPO_E_*); topo_last_error() gives the message of the calling thread;
+ *   - numbering is the reference's:  element e = ex + ey*nelx + ez*nelx*nely,
+ *     node n = ix*(nely+1)*(nelz+1) + iy*(nelz+1) + iz, DOF = 3n + {0,1,2},
+ *     element-local node order (0,0,0),(1,0,0),(1,1,0),(0,1,0),(0,0,1),
+ *     (1,0,1),(1,1,1),(0,1,1)  (js/optimizer-worker-3d.js:1448-1494);
+ *   - every call is synchronous: results are in the caller's arrays when it
+ *     returns (the semantics of the WASM exports it replaces).
+ * Nothing here falls back to the CPU: with no usable GPU the compute entry
+ * points return TOPO_E_NODEVICE.
+ */
+#ifndef TOPO_H
+#define TOPO_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TOPO_ABI_VERSION 1
+
+/* ---- error codes ---------------------------------------------------- */
+#define TOPO_OK              0
+#define TOPO_E_ARG          -1   /* bad argument / size                      */
+#define TOPO_E_HIP          -2   /* HIP runtime or launch failure            */
+#define TOPO_E_OOM          -3   /* device allocation failed                 */
+#define TOPO_E_BREAKDOWN    -4   /* non-finite CG scalar (pAp, rz)           */
+#define TOPO_E_UNSTRUCTURED -5   /* edofs is not a structured hex8 grid      */
+#define TOPO_E_NODEVICE     -6   /* no HIP device                            */
+#define TOPO_E_STATE        -7   /* call out of order                        */
+#define TOPO_E_UNSUPPORTED  -8   /* valid request this build does not do     */
+#define TOPO_E_CANCELLED    -9   /* cancel flag observed                     */
+
+/* ---- enums ---------------------------------------------------------- */
+/* constraint positions of getFixedDOFs (js/optimizer-worker-3d.js:4436-4493) */
+enum { TOPO_FIX_LEFT = 0, TOPO_FIX_RIGHT = 1, TOPO_FIX_BOTTOM = 2,
+       TOPO_FIX_TOP = 3, TOPO_FIX_BOTTOM_CORNERS = 4, TOPO_FIX_NONE = 5 };
+/* constraintDOFs string 'all' / contains 'x','y','z' (:4429-4434) */
+enum { TOPO_DOF_X = 1, TOPO_DOF_Y = 2, TOPO_DOF_Z = 4, TOPO_DOF_ALL = 7 };
+/* force directions of getLoadVector (js/optimizer-worker-3d.js:4495-4547) */
+enum { TOPO_LOAD_DOWN = 0, TOPO_LOAD_UP = 1, TOPO_LOAD_LEFT = 2,
+       TOPO_LOAD_RIGHT = 3, TOPO_LOAD_TOP_CENTER = 4, TOPO_LOAD_NONE = 5 };
+
+enum { TOPO_SOLVER_PCG = 0,      /* Jacobi-PCG, ebePCG semantics              */
+       TOPO_SOLVER_MGPCG = 1 };  /* geometric MG-PCG, ebeMGPCG semantics      */
+enum { TOPO_PREC_FP64 = 0,       /* everything in FP64                        */
+       TOPO_PREC_MIXED = 1 };    /* FP32 operator/V-cycle inside FP64 refinement */
+enum { TOPO_COARSE_SWEEPS = 0,   /* reference: 30 extra damped-Jacobi sweeps  */
+       TOPO_COARSE_CHOLESKY = 1};/* FP64 dense Cholesky of the coarsest level */
+enum { TOPO_FILTER_SENSITIVITY = 0,  /* reference filterSensitivities       */
+       TOPO_FILTER_DENSITY = 1,      /* linear density filter (weights rmin-d) */
+       TOPO_FILTER_HELMHOLTZ = 2 };  /* PDE filter -r^2 lap(xf) + xf = x      */
+enum { TOPO_OBJ_COMPLIANCE = 0, TOPO_OBJ_MECHANISM = 1 };
+
+/* ---- solver configuration (keys of the reference `config` object) ---- */
+typedef struct topo_config {
+    int32_t nelx, nely, nelz;
+    int32_t nload;              /* load cases (columns of F); 1 in the reference      */
+    double  E0, Emin, nu;       /* youngsModulus, Emin=1e-9, poissonsRatio            */
+    double  volfrac;            /* volumeFraction                                      */
+    double  penal;              /* penaltyFactor (target)                              */
+    double  penal_start;        /* penalStart (reference default min(penal,1.5))       */
+    int32_t continuation_iters; /* continuationIters (default max(20,floor(maxIt/3)))  */
+    int32_t max_iterations;     /* maxIterations (SIMP)                                */
+    double  rmin;               /* filterRadius                                        */
+    int32_t filter_type;        /* TOPO_FILTER_*                                       */
+    int32_t use_projection;     /* useProjection (default 1)                           */
+    double  beta_max;           /* betaMax (64)                                        */
+    int32_t beta_interval;      /* betaInterval (5)                                    */
+    int32_t beta_fixed;         /* >0: constant beta (no continuation)                 */
+    double  eta;                /* projection threshold (0.5)                          */
+    int32_t linear_solver;      /* TOPO_SOLVER_*                                       */
+    int32_t precision;          /* TOPO_PREC_*                                         */
+    int32_t mg_levels;          /* 0: reference rule (floor(n/2), <=6 levels, dims>=2) */
+    int32_t mg_nu1, mg_nu2;     /* V(nu1,nu2) (2,2)                                    */
+    double  mg_omega;           /* damped-Jacobi weight (0.5)                          */
+    int32_t coarse_solver;      /* TOPO_COARSE_*                                       */
+    int32_t coarse_sweeps;      /* 30                                                  */
+    int32_t galerkin_max_ndof;  /* dense Galerkin below this ndof (3000); 0 = never    */
+    int32_t cg_max_iter;        /* 400 (capped by nfree)                               */
+    double  cg_tol;             /* > 0 fixed relative tolerance; <= 0 reference schedule 1e-3 -> 1e-8 */
+    int32_t warm_start;         /* -1 solver default (PCG warm, MGPCG cold), 0 cold, 1 warm */
+    double  move;               /* OC move limit (0.2)                                 */
+    int32_t objective;          /* TOPO_OBJ_*                                          */
+    int32_t mech_in_dof;        /* mechanism: input DOF (load applied there)           */
+    int32_t mech_out_dof;       /* mechanism: output DOF                               */
+    double  mech_k_in;          /* input spring stiffness                              */
+    double  mech_k_out;         /* output spring stiffness                             */
+    int32_t rhs_tile;           /* block-PCG RHS tile (0 = autotune)                   */
+    int32_t use_graphs;         /* capture MG-PCG iterations in hipGraphs (1 default)  */
+    double  mg_omega_cap;       /* > 0: per level omega_eff = min(mg_omega, cap / rho),
+                                   rho ~ lambda_max(D^-1 A) by 20 power iterations from
+                                   a hashed vector on free DOFs; 0 = off (reference)  */
+    int32_t reserved[8];
+} topo_config;
+
+/* Fill *cfg with the reference defaults (js/optimizer-worker-3d.js:20-64,
+ * :2056-2072, js/main.js:61-110) for an nelx x nely x nelz grid. */
+int topo_config_default(topo_config* cfg, int32_t nelx, int32_t nely, int32_t nelz);
+
+/* ---- library / device ------------------------------------------------ */
+const char* topo_version(void);
+const char* topo_last_error(void);
+/* number of HIP devices and the name of device 0 (no compute) */
+int topo_device_info(int32_t* count, char* name, int32_t name_len);
+
+/* ---- host-side problem setup (bit-exact reference numbering) ---------
+ * These run on the host, need no GPU, and are what the reference's worker
+ * computes before the solve. */
+
+/* replaces lk() js/optimizer-worker-3d.js:3404-3491 + _flattenKE :1440-1446:
+ * 24x24 row-major unit-cube hex8 stiffness, E=1, 2x2x2 Gauss. */
+int topo_k0(double nu, double* ke576);
+
+/* replaces _precomputeEdofs3D js/optimizer-worker-3d.js:1448-1494 */
+int topo_edofs(int32_t nelx, int32_t nely, int32_t nelz, int32_t* edofs /* nel*24 */);
+
+/* replaces getFixedDOFs js/optimizer-worker-3d.js:4436-4493 (+ mask build
+ * :2001-2008). Sets fixedMask[dof]=1 (does not clear). Returns #DOFs listed. */
+int topo_fixed_dofs(int32_t nelx, int32_t nely, int32_t nelz, int32_t position,
+                    int32_t dof_mask, uint8_t* fixedMask /* ndof */);
+
+/* replaces getLoadVector js/optimizer-worker-3d.js:4495-4547. forceVector may
+ * be NULL (otherwise 3 doubles: custom direction at the top-right corner).
+ * Writes F (float32, zeroed first). */
+int topo_load_vector(int32_t nelx, int32_t nely, int32_t nelz, int32_t direction,
+                     double magnitude, const double* forceVector, float* F /* ndof */);
+
+/* replaces getConstraintElements (:4552-4592) and getForceElements
+ * (:4597-4619): sets preserved[e]=1 for the elements pinned to x=1. */
+int topo_preserved_elements(int32_t nelx, int32_t nely, int32_t nelz,
+                            int32_t position, int32_t direction, uint8_t* preserved);
+
+/* replaces getLoadVectorFromPaint (:4673-4745): faces is n x 4 int32
+ * (vx,vy,vz,faceIndex; faceIndex<0 = all 8 nodes); dir is the unit force
+ * direction; magnitude spread evenly over the unique nodes. F accumulated. */
+int topo_load_paint(int32_t nelx, int32_t nely, int32_t nelz, const int32_t* faces,
+                    int32_t nfaces, const double* dir3, double magnitude, float* F);
+
+/* replaces getFixedDOFsFromPaint: same face encoding, marks fixedMask. */
+int topo_fixed_paint(int32_t nelx, int32_t nely, int32_t nelz, const int32_t* faces,
+                     int32_t nfaces, int32_t dof_mask, uint8_t* fixedMask);
+
+/* ---- WASM-op twins (drop-in for assembly/index.ts exports) -----------
+ * Same argument order and meaning as the AssemblyScript exports, with host
+ * pointers in place of linear-memory offsets. The connectivity must be a
+ * structured hex8 grid in the reference numbering (checked in full; any
+ * other edofs -> TOPO_E_UNSTRUCTURED): the device computes it instead of
+ * reading it. Each call copies its inputs to the device and its outputs
+ * back; the device-resident path is topo_solver_* below. */
+
+/* replaces applyAEbe3D assembly/index.ts:403-439 */
+int topo_applyAEbe3D(const double* ke, const int32_t* edofs, const double* evals,
+                     const int32_t* active, int32_t activeCount, const double* p,
+                     double* Ap, int32_t ndof, double* scratch);
+
+/* replaces ebePCG assembly/index.ts:504-733. Returns the CG iteration count
+ * (cgMax if not converged), like the export; U is the warm start on input.
+ * edofSize must be 24 (the 2-D quad4 path is out of scope). `work` is
+ * ignored (device workspace is owned by the library). */
+int topo_ebePCG(const double* dens, const double* ke, const int32_t* edofs,
+                const double* F, double* U, const int32_t* freedofs, int32_t nel,
+                int32_t edofSize, int32_t ndof, int32_t nfree, double Emin, double E0,
+                double penal, int32_t maxIter, double tolerance, double* work);
+
+/* Level descriptor of ebeMGPCG (assembly/index.ts:743-757): the 20 x i32
+ * record with its offsets widened to host pointers. */
+typedef struct topo_mg_level {
+    int32_t nelx, nely, nelz, ndof, nel, nfree, activeCount, hasDenseK;
+    const int32_t* edofs;      const double* evals;   const int32_t* active;
+    const uint8_t* fixedMask;  const int32_t* freeDofs; const double* invDiag;
+    const double* denseK;      double* Au; double* res; double* b; double* x;
+    double* scratch;
+} topo_mg_level;
+
+/* replaces ebeMGPCG assembly/index.ts:1076-1193 (cold start, V(2,2),
+ * omega 0.5, 30 coarsest sweeps, dense levels where hasDenseK). Returns the
+ * CG iteration count. */
+int topo_ebeMGPCG(const topo_mg_level* levels, int32_t numLevels, const double* ke,
+                  const double* F, double* U, const uint8_t* fixedMask0, int32_t ndof0,
+                  int32_t maxIter, double tolerance, double* cgWork);
+
+/* ---- device-resident SIMP solver (the fast path) --------------------- */
+typedef struct topo_solver topo_solver;
+
+typedef struct topo_step_info {
+    int32_t iteration;      /* loop counter after the step (1-based)          */
+    int32_t cg_iters;       /* Krylov iterations of this step's solve(s)      */
+    int32_t outer_iters;    /* FP64 refinement sweeps (mixed precision)       */
+    int32_t applyA_fine;    /* fine-level operator applications               */
+    int32_t vcycles;        /* V-cycles                                       */
+    int32_t oc_passes;      /* OC bisection passes                            */
+    double  compliance;     /* F^T U (summed over load cases); mechanism: u_out */
+    double  change;         /* max |x_new - x_old|                            */
+    double  penal, beta, tolerance, volume;
+    double  max_stress;     /* max_e E_e * w_e                                */
+    double  residual;       /* final relative residual of the solve           */
+    double  t_solve_ms, t_total_ms;
+} topo_step_info;
+
+/* fixedMask u8[ndof]; F float32[nload*ndof] (case-major); preserved/voidm
+ * u8[nel] or NULL. x starts at volfrac (preserved -> 1, void -> 0) as in the
+ * reference (js/optimizer-worker-3d.js:1928-2022). */
+int topo_solver_create(const topo_config* cfg, const uint8_t* fixedMask, const float* F,
+                       const uint8_t* preserved, const uint8_t* voidm, topo_solver** out);
+int topo_solver_destroy(topo_solver* s);
+/* design variables x (float32[nel], reference element order) */
+int topo_solver_set_design(topo_solver* s, const float* x);
+int topo_solver_get_design(topo_solver* s, float* x);
+/* change the SIMP iteration counter (what loop the next step is) */
+int topo_solver_set_iteration(topo_solver* s, int32_t loop);
+/* live config patch (reference updateConfig keys, :2084-2112) */
+int topo_solver_update(topo_solver* s, double penal, double rmin, double volfrac,
+                       int32_t max_iterations);
+
+/* one SIMP iteration: projection, FE solve, energies, sensitivities, filter,
+ * OC, change (js/optimizer-worker-3d.js:2124-2252). */
+int topo_solver_step(topo_solver* s, topo_step_info* info);
+
+/* fields of the last step: */
+enum { TOPO_FIELD_U = 0,        /* double[nload*ndof]                    */
+       TOPO_FIELD_XPHYS = 1,    /* float[nel]                            */
+       TOPO_FIELD_ENERGY = 2,   /* float[nel]  w_e = u_e^T k0 u_e        */
+       TOPO_FIELD_DC = 3,       /* float[nel]                            */
+       TOPO_FIELD_DCN = 4,      /* float[nel]  filtered                  */
+       TOPO_FIELD_STRESS = 5,   /* float[nel]  E_e * w_e                 */
+       TOPO_FIELD_ADJOINT = 6 };/* double[ndof] adjoint (mechanism)      */
+int topo_solver_get_field(topo_solver* s, int32_t field, void* out);
+
+/* FE() (js/optimizer-worker-3d.js:4155-4371): solve K(xPhys) U = F for the
+ * given physical densities and penalty. U (double[nload*ndof]) is the warm
+ * start when warm starting applies and receives the solution; c receives
+ * F^T U. iteration/max_iterations drive the tolerance schedule. */
+int topo_solver_fe(topo_solver* s, const float* xPhys, double penal, int32_t iteration,
+                   int32_t max_iterations, double* U, double* c, topo_step_info* info);
+
+/* ---- operator-level entry points (parity tests, profiling) ----------- */
+/* Ap = K(E) p on MG level `level` with element moduli from the last
+ * set_density/fe call, in precision prec (TOPO_PREC_FP64 or _MIXED=FP32). */
+int topo_solver_set_density(topo_solver* s, const float* xPhys, double penal);
+int topo_op_applyA(topo_solver* s, int32_t level, int32_t prec, const double* p, double* Ap);
+/* z = M^-1 r, one V-cycle from z = 0 */
+int topo_op_vcycle(topo_solver* s, int32_t prec, const double* r, double* z);
+/* energies w_e (float[nel]) of displacement U (double[ndof]) */
+int topo_op_energy(topo_solver* s, const double* U, float* w);
+/* sensitivity / density filter forward: out = H(in) (float[nel]); for the
+ * sensitivity filter x is the design (the max(1e-3,x) weighting), else NULL */
+int topo_op_filter(topo_solver* s, int32_t filter_type, const float* x, const float* in,
+                   float* out);
+/* filter adjoint (transpose): out = H^T(in) */
+int topo_op_filter_adjoint(topo_solver* s, int32_t filter_type, const float* in, float* out);
+/* OC update (js/optimizer-worker-3d.js:3066-3101); returns passes */
+int topo_op_oc(topo_solver* s, const float* x, const float* dcn, float* xnew,
+               double volfrac);
+
+/* ---- timing (HIP events on the stream the kernels run on) ------------ */
+enum { TOPO_TIME_APPLYA = 0, TOPO_TIME_VCYCLE = 1, TOPO_TIME_ENERGY = 2,
+       TOPO_TIME_FILTER = 3, TOPO_TIME_OC = 4, TOPO_TIME_COPY = 5 };
+/* run `op` reps times back to back on device-resident data and return the
+ * average duration of one launch in ms (and the algorithmic bytes of one). */
+int topo_time_op(topo_solver* s, int32_t op, int32_t prec, int32_t level, int32_t reps,
+                 double* avg_ms, double* bytes);
+
+/* cancel flag polled between CG iterations and SIMP steps (thread-safe) */
+int topo_solver_cancel(topo_solver* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TOPO_H */

@@ -1,0 +1,1056 @@
+// topo_oracle.cpp -- CPU ORACLE (test infrastructure only).
+//
+// A single-threaded FP64 C++ restatement of the reference's 3-D SIMP path
+// (Jozo132/TopologyOptimization: js/optimizer-worker-3d.js and
+// assembly/index.ts). It exists to check the HIP library and to time a CPU
+// baseline: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+// leg may load it, and never as the thing measured or shipped.
+//
+// Every function cites the reference file:line it follows. The reference
+// keeps several quantities in Float32 (x, xPhys, F, dc, dcn, energies, Hs);
+// the restatement rounds at the same points.
+//
+// Pinning (what checks each function, see tests/test_oracle.py):
+//   or_k0            symmetry, 6 rigid-body null modes, diagonal 0.47008547,
+//                    closed-form entries of the textbook hex8 stiffness
+//   or_edofs         closed-form index formulas, counts
+//   or_applyA        brute-force assembled dense K (tiny grids), rigid modes,
+//                    linearity, zero/no-active cases, symmetry
+//   or_ebePCG        dense Cholesky solve on tiny grids, U^T K U = F^T U
+//   or_mg_*          V-cycle symmetry <Ma,b>=<a,Mb>, MGPCG vs dense solve
+//   or_energy        sum_e E_e w_e = U^T K U
+//   or_sens_filter   brute-force neighbour sums
+//   or_density_filter brute force + central finite differences of compliance
+//   or_oc            volume constraint, brute-force lambda scan
+//   or_heaviside     closed form and its derivative
+// Functions of the build-only features (Cholesky coarse solve, Helmholtz
+// filter, mechanism objective, multi-load) have no reference counterpart;
+// they are pinned by the invariants named at each.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+#include <set>
+#include "../include/topo.h"
+
+typedef std::vector<double> dvec;
+
+static inline int64_t node_id(int ix, int iy, int iz, int nny, int nnz) {
+    return (int64_t)ix * nny * nnz + (int64_t)iy * nnz + iz;
+}
+
+extern "C" {
+
+// ---------------------------------------------------------------------------
+// K0: lk() js/optimizer-worker-3d.js:3404-3491, same loop and product order.
+int or_k0(double nu, double* KE) {
+    const double E = 1.0;
+    double C[6][6] = {{0}};
+    const double fact = E / ((1 + nu) * (1 - 2 * nu));
+    C[0][0] = fact * (1 - nu); C[0][1] = fact * nu; C[0][2] = fact * nu;
+    C[1][0] = fact * nu; C[1][1] = fact * (1 - nu); C[1][2] = fact * nu;
+    C[2][0] = fact * nu; C[2][1] = fact * nu; C[2][2] = fact * (1 - nu);
+    C[3][3] = fact * (1 - 2 * nu) / 2;
+    C[4][4] = fact * (1 - 2 * nu) / 2;
+    C[5][5] = fact * (1 - 2 * nu) / 2;
+    for (int i = 0; i < 576; i++) KE[i] = 0.0;
+    const double g[2] = {-1 / std::sqrt(3.0), 1 / std::sqrt(3.0)};
+    // natural coordinates of the 8 local nodes (:1448-1494 order)
+    const int sx[8] = {-1, 1, 1, -1, -1, 1, 1, -1};
+    const int sy[8] = {-1, -1, 1, 1, -1, -1, 1, 1};
+    const int sz[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++)
+            for (int k = 0; k < 2; k++) {
+                const double xi = g[i], eta = g[j], zeta = g[k];
+                double dN[8][3];
+                for (int n = 0; n < 8; n++) {
+                    // (1 +/- eta)(1 +/- zeta) etc., sign applied to the product
+                    const double fx = sx[n] > 0 ? (1 + xi) : (1 - xi);
+                    const double fy = sy[n] > 0 ? (1 + eta) : (1 - eta);
+                    const double fz = sz[n] > 0 ? (1 + zeta) : (1 - zeta);
+                    double a = fy * fz, b = fx * fz, c = fx * fy;
+                    dN[n][0] = (sx[n] > 0 ? a : -a) * 0.125;
+                    dN[n][1] = (sy[n] > 0 ? b : -b) * 0.125;
+                    dN[n][2] = (sz[n] > 0 ? c : -c) * 0.125;
+                }
+                double B[6][24] = {{0}};
+                for (int n = 0; n < 8; n++) {
+                    int col = n * 3;
+                    B[0][col] = dN[n][0];
+                    B[1][col + 1] = dN[n][1];
+                    B[2][col + 2] = dN[n][2];
+                    B[3][col] = dN[n][1];
+                    B[3][col + 1] = dN[n][0];
+                    B[4][col + 1] = dN[n][2];
+                    B[4][col + 2] = dN[n][1];
+                    B[5][col] = dN[n][2];
+                    B[5][col + 2] = dN[n][0];
+                }
+                double CB[6][24] = {{0}};
+                for (int m = 0; m < 6; m++)
+                    for (int n = 0; n < 24; n++)
+                        for (int p = 0; p < 6; p++) CB[m][n] += C[m][p] * B[p][n];
+                for (int m = 0; m < 24; m++)
+                    for (int n = 0; n < 24; n++)
+                        for (int p = 0; p < 6; p++) KE[m * 24 + n] += B[p][m] * CB[p][n] * 1.0 * 1.0;
+            }
+    return 0;
+}
+
+// _precomputeEdofs3D js/optimizer-worker-3d.js:1448-1494
+int or_edofs(int nelx, int nely, int nelz, int32_t* edof) {
+    const int nny = nely + 1, nnz = nelz + 1;
+    for (int ez = 0; ez < nelz; ez++)
+        for (int ey = 0; ey < nely; ey++)
+            for (int ex = 0; ex < nelx; ex++) {
+                int64_t e = ex + (int64_t)ey * nelx + (int64_t)ez * nelx * nely;
+                int64_t n[8] = {node_id(ex, ey, ez, nny, nnz),         node_id(ex + 1, ey, ez, nny, nnz),
+                                node_id(ex + 1, ey + 1, ez, nny, nnz), node_id(ex, ey + 1, ez, nny, nnz),
+                                node_id(ex, ey, ez + 1, nny, nnz),     node_id(ex + 1, ey, ez + 1, nny, nnz),
+                                node_id(ex + 1, ey + 1, ez + 1, nny, nnz), node_id(ex, ey + 1, ez + 1, nny, nnz)};
+                for (int a = 0; a < 8; a++)
+                    for (int d = 0; d < 3; d++) edof[e * 24 + 3 * a + d] = (int32_t)(3 * n[a] + d);
+            }
+    return 0;
+}
+
+// getFixedDOFs js/optimizer-worker-3d.js:4436-4493, _addNodeDOFs :4429-4434
+int or_fixed_dofs(int nelx, int nely, int nelz, int position, int dofmask, uint8_t* mask) {
+    const int nny = nely + 1, nnz = nelz + 1;
+    int cnt = 0;
+    auto add = [&](int64_t n) {
+        for (int d = 0; d < 3; d++)
+            if (dofmask & (1 << d)) { mask[3 * n + d] = 1; cnt++; }
+    };
+    switch (position) {
+    case TOPO_FIX_LEFT:
+        for (int j = 0; j <= nely; j++) for (int k = 0; k <= nelz; k++) add(node_id(0, j, k, nny, nnz));
+        break;
+    case TOPO_FIX_RIGHT:
+        for (int j = 0; j <= nely; j++) for (int k = 0; k <= nelz; k++) add(node_id(nelx, j, k, nny, nnz));
+        break;
+    case TOPO_FIX_BOTTOM:
+        for (int i = 0; i <= nelx; i++) for (int k = 0; k <= nelz; k++) add(node_id(i, 0, k, nny, nnz));
+        break;
+    case TOPO_FIX_TOP:
+        for (int i = 0; i <= nelx; i++) for (int k = 0; k <= nelz; k++) add(node_id(i, nely, k, nny, nnz));
+        break;
+    case TOPO_FIX_BOTTOM_CORNERS:
+        add(node_id(0, 0, 0, nny, nnz)); add(node_id(nelx, 0, 0, nny, nnz));
+        add(node_id(0, nely, 0, nny, nnz)); add(node_id(nelx, nely, 0, nny, nnz));
+        break;
+    default: break;
+    }
+    return cnt;
+}
+
+// getLoadVector js/optimizer-worker-3d.js:4495-4547 (F is Float32)
+int or_load_vector(int nelx, int nely, int nelz, int direction, double mag, const double* fv, float* F) {
+    const int nny = nely + 1, nnz = nelz + 1;
+    const int64_t ndof = 3LL * (nelx + 1) * nny * nnz;
+    for (int64_t i = 0; i < ndof; i++) F[i] = 0.f;
+    if (fv) {
+        double len = std::sqrt(fv[0] * fv[0] + fv[1] * fv[1] + fv[2] * fv[2]);
+        if (len > 0) {
+            int64_t n = node_id(nelx, nely, nelz, nny, nnz);
+            for (int d = 0; d < 3; d++) F[3 * n + d] = (float)((fv[d] / len) * mag);
+        }
+        return 0;
+    }
+    switch (direction) {
+    case TOPO_LOAD_DOWN: F[3 * node_id(nelx, nely, nelz, nny, nnz) + 1] = (float)(-mag); break;
+    case TOPO_LOAD_UP: F[3 * node_id(nelx, nely, nelz, nny, nnz) + 1] = (float)mag; break;
+    case TOPO_LOAD_LEFT: F[3 * node_id(nelx, nely / 2, nelz / 2, nny, nnz)] = (float)(-mag); break;
+    case TOPO_LOAD_RIGHT: F[3 * node_id(0, nely / 2, nelz / 2, nny, nnz)] = (float)mag; break;
+    case TOPO_LOAD_TOP_CENTER: F[3 * node_id(nelx / 2, nely / 2, nelz, nny, nnz) + 2] = (float)mag; break;
+    default: break;
+    }
+    return 0;
+}
+
+// getConstraintElements :4552-4592 and getForceElements :4597-4619
+int or_preserved(int nelx, int nely, int nelz, int position, int direction, uint8_t* pres) {
+    auto E = [&](int ex, int ey, int ez) { return (int64_t)ex + (int64_t)ey * nelx + (int64_t)ez * nelx * nely; };
+    switch (position) {
+    case TOPO_FIX_LEFT: for (int ez = 0; ez < nelz; ez++) for (int ey = 0; ey < nely; ey++) pres[E(0, ey, ez)] = 1; break;
+    case TOPO_FIX_RIGHT: for (int ez = 0; ez < nelz; ez++) for (int ey = 0; ey < nely; ey++) pres[E(nelx - 1, ey, ez)] = 1; break;
+    case TOPO_FIX_BOTTOM: for (int ez = 0; ez < nelz; ez++) for (int ex = 0; ex < nelx; ex++) pres[E(ex, 0, ez)] = 1; break;
+    case TOPO_FIX_TOP: for (int ez = 0; ez < nelz; ez++) for (int ex = 0; ex < nelx; ex++) pres[E(ex, nely - 1, ez)] = 1; break;
+    case TOPO_FIX_BOTTOM_CORNERS:
+        pres[E(0, 0, 0)] = 1; pres[E(nelx - 1, 0, 0)] = 1; pres[E(0, nely - 1, 0)] = 1; pres[E(nelx - 1, nely - 1, 0)] = 1;
+        break;
+    default: break;
+    }
+    switch (direction) {
+    case TOPO_LOAD_DOWN: case TOPO_LOAD_UP: pres[E(nelx - 1, nely - 1, nelz - 1)] = 1; break;
+    case TOPO_LOAD_LEFT: pres[E(nelx - 1, nely / 2, nelz / 2)] = 1; break;
+    case TOPO_LOAD_RIGHT: pres[E(0, nely / 2, nelz / 2)] = 1; break;
+    case TOPO_LOAD_TOP_CENTER: pres[E(nelx / 2, nely / 2, nelz - 1)] = 1; break;
+    default: break;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// _powDensity js/optimizer-worker-3d.js:293-306 (integer fast path)
+static double pow_density(double x, double p) {
+    if (std::isfinite(p) && p == std::floor(p)) {
+        switch ((int)p) {
+        case 0: return 1.0;
+        case 1: return x;
+        case 2: return x * x;
+        case 3: return x * x * x;
+        case 4: { double x2 = x * x; return x2 * x2; }
+        case 5: { double x2 = x * x; return x2 * x2 * x; }
+        case 6: { double x2 = x * x; return x2 * x2 * x2; }
+        default: break;
+        }
+    }
+    return std::pow(x, p);
+}
+
+// applyAEbe3D assembly/index.ts:403-439: Ap = sum over active e of E_e K0 u_e
+int or_applyA(int nelx, int nely, int nelz, const double* ke, const double* evals,
+              const int32_t* active, int activeCount, const double* p, double* Ap) {
+    const int64_t ndof = 3LL * (nelx + 1) * (nely + 1) * (nelz + 1);
+    const int64_t nel = (int64_t)nelx * nely * nelz;
+    std::vector<int32_t> edof(nel * 24);
+    or_edofs(nelx, nely, nelz, edof.data());
+    for (int64_t i = 0; i < ndof; i++) Ap[i] = 0.0;
+    double loc[24];
+    for (int ai = 0; ai < activeCount; ai++) {
+        int64_t e = active[ai];
+        double E = evals[e];
+        const int32_t* ed = &edof[e * 24];
+        for (int j = 0; j < 24; j++) loc[j] = p[ed[j]];
+        for (int i = 0; i < 24; i++) {
+            double s = 0.0;
+            for (int j = 0; j < 24; j++) s += ke[i * 24 + j] * loc[j];
+            Ap[ed[i]] += E * s;
+        }
+    }
+    return 0;
+}
+
+// Dense global K (test helper: brute-force assembly of sum_e E_e P_e^T K0 P_e)
+int or_assemble_dense(int nelx, int nely, int nelz, const double* ke, const double* evals, double* K) {
+    const int64_t ndof = 3LL * (nelx + 1) * (nely + 1) * (nelz + 1);
+    const int64_t nel = (int64_t)nelx * nely * nelz;
+    std::vector<int32_t> edof(nel * 24);
+    or_edofs(nelx, nely, nelz, edof.data());
+    for (int64_t i = 0; i < ndof * ndof; i++) K[i] = 0.0;
+    for (int64_t e = 0; e < nel; e++)
+        for (int i = 0; i < 24; i++)
+            for (int j = 0; j < 24; j++) K[(int64_t)edof[e * 24 + i] * ndof + edof[e * 24 + j]] += evals[e] * ke[i * 24 + j];
+    return 0;
+}
+
+// ebePCG assembly/index.ts:504-733, restated on host arrays (3-D, edofSize 24).
+// E_vals via Math.pow (:544); reduced (free-DOF) space; warm start from U.
+int or_ebePCG(const double* dens, const double* ke, int nelx, int nely, int nelz, const double* F,
+              double* U, const int32_t* freedofs, int nfree, double Emin, double E0, double penal,
+              int maxIter, double tolerance) {
+    const double EPS = 1e-12;
+    const int64_t nel = (int64_t)nelx * nely * nelz;
+    const int64_t ndof = 3LL * (nelx + 1) * (nely + 1) * (nelz + 1);
+    const double dE = E0 - Emin, skip = Emin * 1000.0;
+    dvec ev(nel);
+    std::vector<int32_t> act;
+    for (int64_t e = 0; e < nel; e++) {
+        ev[e] = Emin + std::pow(dens[e], penal) * dE;
+        if (ev[e] > skip) act.push_back((int32_t)e);
+    }
+    std::vector<int32_t> edof(nel * 24);
+    or_edofs(nelx, nely, nelz, edof.data());
+    dvec diag(ndof, 0.0);
+    for (int32_t e : act)
+        for (int i = 0; i < 24; i++) diag[edof[(int64_t)e * 24 + i]] += ev[e] * ke[i * 24 + i];
+    dvec invD(nfree), Uf(nfree), r(nfree), z(nfree), pp(nfree), Ap(nfree), pfull(ndof), apfull(ndof);
+    for (int i = 0; i < nfree; i++) {
+        double d = diag[freedofs[i]];
+        invD[i] = d > 1e-30 ? 1.0 / d : 0.0;
+    }
+    for (int i = 0; i < nfree; i++) Uf[i] = U[freedofs[i]];
+    std::fill(pfull.begin(), pfull.end(), 0.0);
+    for (int i = 0; i < nfree; i++) pfull[freedofs[i]] = Uf[i];
+    or_applyA(nelx, nely, nelz, ke, ev.data(), act.data(), (int)act.size(), pfull.data(), apfull.data());
+    for (int i = 0; i < nfree; i++) r[i] = F[freedofs[i]] - apfull[freedofs[i]];
+    double rz = 0, r0 = 0;
+    for (int i = 0; i < nfree; i++) {
+        double zi = invD[i] * r[i];
+        z[i] = zi; pp[i] = zi; rz += r[i] * zi; r0 += r[i] * r[i];
+    }
+    const double tolSq = tolerance * tolerance * (r0 > 1e-30 ? r0 : 1e-30);
+    const int cgMax = maxIter < nfree ? maxIter : nfree;
+    int ret = cgMax;
+    for (int it = 0; it < cgMax; it++) {
+        double rn = 0;
+        for (int i = 0; i < nfree; i++) rn += r[i] * r[i];
+        if (rn < tolSq) { ret = it; break; }
+        std::fill(pfull.begin(), pfull.end(), 0.0);
+        for (int i = 0; i < nfree; i++) pfull[freedofs[i]] = pp[i];
+        or_applyA(nelx, nely, nelz, ke, ev.data(), act.data(), (int)act.size(), pfull.data(), apfull.data());
+        for (int i = 0; i < nfree; i++) Ap[i] = apfull[freedofs[i]];
+        double pAp = 0;
+        for (int i = 0; i < nfree; i++) pAp += pp[i] * Ap[i];
+        double alpha = rz / (pAp + EPS);
+        double rzn = 0;
+        for (int i = 0; i < nfree; i++) {
+            double ri = r[i] - alpha * Ap[i];
+            double zi = invD[i] * ri;
+            Uf[i] += alpha * pp[i];
+            r[i] = ri; z[i] = zi; rzn += ri * zi;
+        }
+        double beta = rzn / (rz + EPS);
+        for (int i = 0; i < nfree; i++) pp[i] = z[i] + beta * pp[i];
+        rz = rzn;
+    }
+    for (int64_t i = 0; i < ndof; i++) U[i] = 0.0;
+    for (int i = 0; i < nfree; i++) U[freedofs[i]] = Uf[i];
+    return ret;
+}
+
+// ---------------------------------------------------------------------------
+// Geometric multigrid: MGPrecond3D js/optimizer-worker-3d.js:317-1025 and
+// mgApplyA/mgSmooth/mgVcycle/ebeMGPCG assembly/index.ts:759-1193.
+struct OrLevel {
+    int nelx, nely, nelz;
+    int64_t nel, ndof;
+    std::vector<int32_t> edof;
+    std::vector<uint8_t> fixed;
+    std::vector<int32_t> freed;
+    dvec E;
+    std::vector<int32_t> active;
+    dvec diag, invDiag, Au, res, b, x;
+    dvec denseK;   // ndof*ndof when assembled
+    dvec chol;     // Cholesky factor (lower, row-major) of the free-DOF block
+    std::vector<int32_t> cholMap;
+};
+
+struct OrMG {
+    std::vector<OrLevel> L;
+    dvec ke;
+    double omega = 0.5;
+    int nu1 = 2, nu2 = 2, coarseSweeps = 30;
+    int galerkinMax = 3000;
+    int coarseSolver = TOPO_COARSE_SWEEPS;
+    double omegaCap = 0.0;        // build-only safeguard, 0 = reference
+    std::vector<double> omegaL;   // per-level effective omega
+};
+
+static void level_init(OrLevel& l, int nx, int ny, int nz) {
+    l.nelx = nx; l.nely = ny; l.nelz = nz;
+    l.nel = (int64_t)nx * ny * nz;
+    l.ndof = 3LL * (nx + 1) * (ny + 1) * (nz + 1);
+    l.edof.resize(l.nel * 24);
+    or_edofs(nx, ny, nz, l.edof.data());
+    l.E.assign(l.nel, 0.0);
+    l.diag.assign(l.ndof, 0.0); l.invDiag.assign(l.ndof, 0.0);
+    l.Au.assign(l.ndof, 0.0); l.res.assign(l.ndof, 0.0);
+    l.b.assign(l.ndof, 0.0); l.x.assign(l.ndof, 0.0);
+}
+
+static void build_free(OrLevel& l) {  // _buildFreeDofsFromFixedMask :375-382
+    l.freed.clear();
+    for (int64_t i = 0; i < l.ndof; i++) if (!l.fixed[i]) l.freed.push_back((int32_t)i);
+}
+
+// _downsampleFixedMaskBy2 :384-406 (injection from fine node min(2c, nF-1))
+static void downsample_fixed(const OrLevel& f, OrLevel& c) {
+    const int nxF = f.nelx + 1, nyF = f.nely + 1, nzF = f.nelz + 1;
+    const int nxC = c.nelx + 1, nyC = c.nely + 1, nzC = c.nelz + 1;
+    c.fixed.assign(c.ndof, 0);
+    for (int cz = 0; cz < nzC; cz++) {
+        int fz = std::min(cz * 2, nzF - 1);
+        for (int cy = 0; cy < nyC; cy++) {
+            int fy = std::min(cy * 2, nyF - 1);
+            for (int cx = 0; cx < nxC; cx++) {
+                int fx = std::min(cx * 2, nxF - 1);
+                int64_t nF = node_id(fx, fy, fz, nyF, nzF), nC = node_id(cx, cy, cz, nyC, nzC);
+                for (int d = 0; d < 3; d++) c.fixed[3 * nC + d] = f.fixed[3 * nF + d];
+            }
+        }
+    }
+}
+
+// 1-D prolongation weights (:591-603): fine f -> coarse (c0,c1,w0,w1)
+static inline void pweights(int f, int nC, int& c0, int& c1, double& w0, double& w1) {
+    c0 = std::min(f >> 1, nC - 1);
+    c1 = std::min(c0 + 1, nC - 1);
+    double t = (c0 != c1 && (f & 1)) ? 0.5 : 0.0;
+    w0 = 1 - t; w1 = t;
+}
+
+static void level_applyA(const OrMG& mg, const OrLevel& l, const double* p, double* Ap) {
+    if (!l.denseK.empty()) {  // mgApplyA dense branch assembly/index.ts:777-788
+        for (int64_t i = 0; i < l.ndof; i++) {
+            double s = 0;
+            for (int64_t j = 0; j < l.ndof; j++) s += l.denseK[i * l.ndof + j] * p[j];
+            Ap[i] = s;
+        }
+        return;
+    }
+    for (int64_t i = 0; i < l.ndof; i++) Ap[i] = 0.0;
+    double loc[24];
+    for (int32_t e : l.active) {
+        double E = l.E[e];
+        const int32_t* ed = &l.edof[(int64_t)e * 24];
+        for (int j = 0; j < 24; j++) loc[j] = p[ed[j]];
+        for (int i = 0; i < 24; i++) {
+            double s = 0;
+            for (int j = 0; j < 24; j++) s += mg.ke[i * 24 + j] * loc[j];
+            Ap[ed[i]] += E * s;
+        }
+    }
+}
+
+static void set_invdiag(OrLevel& l) {
+    std::fill(l.invDiag.begin(), l.invDiag.end(), 0.0);
+    for (int32_t d : l.freed) l.invDiag[d] = l.diag[d] > 1e-30 ? 1.0 / l.diag[d] : 0.0;
+}
+
+// _updateOperatorLevel :753-796
+static void update_level0(OrMG& mg, const float* x, double penal, double E0, double Emin) {
+    OrLevel& l = mg.L[0];
+    const double dE = E0 - Emin, skip = Emin * 1000;
+    l.active.clear();
+    for (int64_t e = 0; e < l.nel; e++) {
+        l.E[e] = Emin + pow_density((double)x[e], penal) * dE;
+        if (l.E[e] > skip) l.active.push_back((int32_t)e);
+    }
+    std::fill(l.diag.begin(), l.diag.end(), 0.0);
+    for (int32_t e : l.active)
+        for (int i = 0; i < 24; i++) l.diag[l.edof[(int64_t)e * 24 + i]] += l.E[e] * mg.ke[i * 24 + i];
+    set_invdiag(l);
+}
+
+// _restrictEvalAvg :702-739 (coarse E = 2 x mean of existing children)
+static void restrict_eavg(OrMG& mg, const OrLevel& f, OrLevel& c, double Emin) {
+    const double skip = Emin * 1000;
+    c.active.clear();
+    int64_t idx = 0;
+    for (int cz = 0; cz < c.nelz; cz++)
+        for (int cy = 0; cy < c.nely; cy++)
+            for (int cx = 0; cx < c.nelx; cx++, idx++) {
+                double sum = 0; int cnt = 0;
+                for (int dz = 0; dz < 2; dz++) { int fz = 2 * cz + dz; if (fz >= f.nelz) continue;
+                    for (int dy = 0; dy < 2; dy++) { int fy = 2 * cy + dy; if (fy >= f.nely) continue;
+                        for (int dx = 0; dx < 2; dx++) { int fx = 2 * cx + dx; if (fx >= f.nelx) continue;
+                            sum += f.E[fx + (int64_t)fy * f.nelx + (int64_t)fz * f.nelx * f.nely]; cnt++; } } }
+                double E = cnt > 0 ? (sum / cnt) * 2.0 : 0;
+                c.E[idx] = E;
+                if (E > skip) c.active.push_back((int32_t)idx);
+            }
+    std::fill(c.diag.begin(), c.diag.end(), 0.0);
+    for (int32_t e : c.active)
+        for (int i = 0; i < 24; i++) c.diag[c.edof[(int64_t)e * 24 + i]] += c.E[e] * mg.ke[i * 24 + i];
+    set_invdiag(c);
+    c.denseK.clear();
+}
+
+// Prolongation as explicit per-node lists (_buildProlongMap :585-625)
+static void prolong_map(const OrLevel& f, const OrLevel& c, std::vector<int32_t>& pn, dvec& pw, std::vector<uint8_t>& pc) {
+    const int nxF = f.nelx + 1, nyF = f.nely + 1, nzF = f.nelz + 1;
+    const int nxC = c.nelx + 1, nyC = c.nely + 1, nzC = c.nelz + 1;
+    int64_t nn = (int64_t)nxF * nyF * nzF;
+    pn.assign(nn * 8, 0); pw.assign(nn * 8, 0.0); pc.assign(nn, 0);
+    for (int fz = 0; fz < nzF; fz++) {
+        int cz[2]; double wz[2]; pweights(fz, nzC, cz[0], cz[1], wz[0], wz[1]);
+        for (int fy = 0; fy < nyF; fy++) {
+            int cy[2]; double wy[2]; pweights(fy, nyC, cy[0], cy[1], wy[0], wy[1]);
+            for (int fx = 0; fx < nxF; fx++) {
+                int cx[2]; double wx[2]; pweights(fx, nxC, cx[0], cx[1], wx[0], wx[1]);
+                int64_t nF = node_id(fx, fy, fz, nyF, nzF);
+                int cnt = 0;
+                for (int dz = 0; dz < 2; dz++) for (int dy = 0; dy < 2; dy++) for (int dx = 0; dx < 2; dx++) {
+                    double w = wx[dx] * wy[dy] * wz[dz];
+                    if (w > 0) { pn[nF * 8 + cnt] = (int32_t)node_id(cx[dx], cy[dy], cz[dz], nyC, nzC); pw[nF * 8 + cnt] = w; cnt++; }
+                }
+                pc[nF] = (uint8_t)cnt;
+            }
+        }
+    }
+}
+
+static void diag_from_dense(OrLevel& l) {  // _diagFromDense :742-751
+    for (int64_t i = 0; i < l.ndof; i++) l.diag[i] = l.denseK[i * l.ndof + i];
+    set_invdiag(l);
+}
+
+// _assembleGalerkinFromElements :628-662
+static void galerkin_from_elements(OrMG& mg, const OrLevel& f, OrLevel& c) {
+    std::vector<int32_t> pn; dvec pw; std::vector<uint8_t> pc;
+    prolong_map(f, c, pn, pw, pc);
+    const int64_t n = c.ndof;
+    c.denseK.assign(n * n, 0.0);
+    for (int32_t e : f.active) {
+        double E = f.E[e];
+        for (int i = 0; i < 24; i++) {
+            int32_t fi = f.edof[(int64_t)e * 24 + i];
+            int32_t ni = fi / 3, di = fi % 3;
+            for (int j = i; j < 24; j++) {
+                double kij = E * mg.ke[i * 24 + j];
+                if (kij == 0) continue;
+                int32_t fj = f.edof[(int64_t)e * 24 + j];
+                int32_t nj = fj / 3, dj = fj % 3;
+                for (int a = 0; a < pc[ni]; a++) {
+                    int64_t cr = 3LL * pn[ni * 8 + a] + di;
+                    double wi = pw[ni * 8 + a];
+                    for (int b = 0; b < pc[nj]; b++) {
+                        int64_t cc = 3LL * pn[nj * 8 + b] + dj;
+                        double v = wi * kij * pw[nj * 8 + b];
+                        c.denseK[cr * n + cc] += v;
+                        if (i != j) c.denseK[cc * n + cr] += v;
+                    }
+                }
+            }
+        }
+    }
+    diag_from_dense(c);
+}
+
+// _assembleGalerkinFromMatrix :665-699
+static void galerkin_from_matrix(OrMG& mg, const OrLevel& f, OrLevel& c) {
+    (void)mg;
+    std::vector<int32_t> pn; dvec pw; std::vector<uint8_t> pc;
+    prolong_map(f, c, pn, pw, pc);
+    const int64_t nF = f.ndof, nC = c.ndof;
+    dvec P(nF * nC, 0.0);
+    int64_t nnF = nF / 3;
+    for (int64_t k = 0; k < nnF; k++)
+        for (int a = 0; a < pc[k]; a++)
+            for (int d = 0; d < 3; d++) P[(3 * k + d) * nC + 3LL * pn[k * 8 + a] + d] = pw[k * 8 + a];
+    dvec T(nF * nC, 0.0);
+    for (int64_t i = 0; i < nF; i++)
+        for (int64_t j = 0; j < nC; j++) {
+            double s = 0;
+            for (int64_t k = 0; k < nF; k++) s += f.denseK[i * nF + k] * P[k * nC + j];
+            T[i * nC + j] = s;
+        }
+    c.denseK.assign(nC * nC, 0.0);
+    for (int64_t i = 0; i < nC; i++)
+        for (int64_t j = 0; j < nC; j++) {
+            double s = 0;
+            for (int64_t k = 0; k < nF; k++) s += P[k * nC + i] * T[k * nC + j];
+            c.denseK[i * nC + j] = s;
+        }
+    diag_from_dense(c);
+}
+
+// Dense Cholesky of the coarsest free-DOF block (build-only feature; pinned
+// by the factor residual ||L L^T - A|| and the solve residual).
+static int coarse_cholesky(OrMG& mg, OrLevel& c) {
+    const int64_t nf = (int64_t)c.freed.size();
+    dvec A(nf * nf, 0.0);
+    if (!c.denseK.empty()) {
+        for (int64_t i = 0; i < nf; i++)
+            for (int64_t j = 0; j < nf; j++) A[i * nf + j] = c.denseK[(int64_t)c.freed[i] * c.ndof + c.freed[j]];
+    } else {
+        dvec Kd(c.ndof * c.ndof, 0.0);
+        for (int32_t e : c.active)
+            for (int i = 0; i < 24; i++)
+                for (int j = 0; j < 24; j++)
+                    Kd[(int64_t)c.edof[(int64_t)e * 24 + i] * c.ndof + c.edof[(int64_t)e * 24 + j]] += c.E[e] * mg.ke[i * 24 + j];
+        for (int64_t i = 0; i < nf; i++)
+            for (int64_t j = 0; j < nf; j++) A[i * nf + j] = Kd[(int64_t)c.freed[i] * c.ndof + c.freed[j]];
+    }
+    // free DOFs with an empty row (no active element) get a unit diagonal
+    for (int64_t i = 0; i < nf; i++)
+        if (A[i * nf + i] <= 1e-30) { for (int64_t j = 0; j < nf; j++) A[i * nf + j] = A[j * nf + i] = 0; A[i * nf + i] = 1.0; }
+    for (int64_t j = 0; j < nf; j++) {
+        double s = A[j * nf + j];
+        for (int64_t k = 0; k < j; k++) s -= A[j * nf + k] * A[j * nf + k];
+        if (!(s > 0)) return -1;
+        double d = std::sqrt(s);
+        A[j * nf + j] = d;
+        for (int64_t i = j + 1; i < nf; i++) {
+            double t = A[i * nf + j];
+            for (int64_t k = 0; k < j; k++) t -= A[i * nf + k] * A[j * nf + k];
+            A[i * nf + j] = t / d;
+        }
+    }
+    c.chol.swap(A);
+    return 0;
+}
+
+static void coarse_solve_chol(const OrLevel& c, const double* b, double* x) {
+    const int64_t nf = (int64_t)c.freed.size();
+    dvec y(nf);
+    for (int64_t i = 0; i < nf; i++) {
+        double s = b[c.freed[i]];
+        for (int64_t k = 0; k < i; k++) s -= c.chol[i * nf + k] * y[k];
+        y[i] = s / c.chol[i * nf + i];
+    }
+    for (int64_t i = nf - 1; i >= 0; i--) {
+        double s = y[i];
+        for (int64_t k = i + 1; k < nf; k++) s -= c.chol[k * nf + i] * y[k];
+        y[i] = s / c.chol[i * nf + i];
+    }
+    for (int64_t i = 0; i < c.ndof; i++) x[i] = 0.0;
+    for (int64_t i = 0; i < nf; i++) x[c.freed[i]] = y[i];
+}
+
+static void smooth(const OrMG& mg, OrLevel& l, const double* b, double* x, double omega) {  // mgSmooth :820-834
+    level_applyA(mg, l, x, l.Au.data());
+    for (int32_t d : l.freed) x[d] += omega * l.invDiag[d] * (b[d] - l.Au[d]);
+}
+
+// Build-only smoother safeguard (no reference counterpart): estimate
+// rho = lambda_max(D^-1 A) on a level by 20 power iterations from a hashed
+// pseudo-random vector on free DOFs (v_i = frac(i * 2654435761 / 2^32) - 1/2);
+// omega_eff = min(omega, cap / rho). Pinned by the eigenvalues of D^-1 A
+// from numpy on small grids.
+static double power_rho(const OrMG& mg, OrLevel& l) {
+    dvec v(l.ndof, 0.0), w(l.ndof, 0.0);
+    for (int32_t d : l.freed) v[d] = (double)(((uint64_t)d * 2654435761ULL) & 0xffffffffULL) / 4294967296.0 - 0.5;
+    double rho = 0.0;
+    for (int it = 0; it < 20; it++) {
+        double nv = 0;
+        for (int64_t i = 0; i < l.ndof; i++) nv += v[i] * v[i];
+        level_applyA(mg, l, v.data(), w.data());
+        double nw = 0;
+        for (int64_t i = 0; i < l.ndof; i++) { w[i] *= l.invDiag[i]; nw += w[i] * w[i]; }
+        if (!(nw > 0) || !(nv > 0)) return rho;
+        rho = std::sqrt(nw / nv);
+        const double s = 1.0 / std::sqrt(nw);
+        for (int64_t i = 0; i < l.ndof; i++) v[i] = w[i] * s;
+    }
+    return rho;
+}
+
+// mgVcycle assembly/index.ts:837-1048
+static void vcycle(OrMG& mg, int li, const double* b, double* x) {
+    OrLevel& l = mg.L[li];
+    const bool coarsest = li == (int)mg.L.size() - 1;
+    if (coarsest && mg.coarseSolver == TOPO_COARSE_CHOLESKY) {
+        coarse_solve_chol(l, b, x);
+        return;
+    }
+    const double om = mg.omegaL.empty() ? mg.omega : mg.omegaL[li];
+    for (int s = 0; s < mg.nu1; s++) smooth(mg, l, b, x, om);
+    level_applyA(mg, l, x, l.Au.data());
+    std::fill(l.res.begin(), l.res.end(), 0.0);
+    for (int32_t d : l.freed) l.res[d] = b[d] - l.Au[d];
+    if (coarsest) {
+        for (int s = 0; s < mg.coarseSweeps; s++) smooth(mg, l, b, x, om);
+        return;
+    }
+    OrLevel& c = mg.L[li + 1];
+    const int nxF = l.nelx + 1, nyF = l.nely + 1, nzF = l.nelz + 1;
+    const int nxC = c.nelx + 1, nyC = c.nely + 1, nzC = c.nelz + 1;
+    std::fill(c.b.begin(), c.b.end(), 0.0);
+    for (int fz = 0; fz < nzF; fz++) {
+        int cz[2]; double wz[2]; pweights(fz, nzC, cz[0], cz[1], wz[0], wz[1]);
+        for (int fy = 0; fy < nyF; fy++) {
+            int cy[2]; double wy[2]; pweights(fy, nyC, cy[0], cy[1], wy[0], wy[1]);
+            for (int fx = 0; fx < nxF; fx++) {
+                int cx[2]; double wx[2]; pweights(fx, nxC, cx[0], cx[1], wx[0], wx[1]);
+                int64_t nF = node_id(fx, fy, fz, nyF, nzF);
+                const double r0 = l.res[3 * nF], r1 = l.res[3 * nF + 1], r2 = l.res[3 * nF + 2];
+                // corner order 000,100,010,110,001,101,011,111 (:904-955)
+                for (int dz = 0; dz < 2; dz++) for (int dy = 0; dy < 2; dy++) for (int dx = 0; dx < 2; dx++) {
+                    double w = wx[dx] * wy[dy] * wz[dz];
+                    if (w > 0) {
+                        int64_t c3 = 3 * node_id(cx[dx], cy[dy], cz[dz], nyC, nzC);
+                        c.b[c3] += w * r0; c.b[c3 + 1] += w * r1; c.b[c3 + 2] += w * r2;
+                    }
+                }
+            }
+        }
+    }
+    for (int64_t i = 0; i < c.ndof; i++) if (c.fixed[i]) c.b[i] = 0;
+    std::fill(c.x.begin(), c.x.end(), 0.0);
+    vcycle(mg, li + 1, c.b.data(), c.x.data());
+    for (int fz = 0; fz < nzF; fz++) {
+        int cz[2]; double wz[2]; pweights(fz, nzC, cz[0], cz[1], wz[0], wz[1]);
+        for (int fy = 0; fy < nyF; fy++) {
+            int cy[2]; double wy[2]; pweights(fy, nyC, cy[0], cy[1], wy[0], wy[1]);
+            for (int fx = 0; fx < nxF; fx++) {
+                int cx[2]; double wx[2]; pweights(fx, nxC, cx[0], cx[1], wx[0], wx[1]);
+                int64_t nF = node_id(fx, fy, fz, nyF, nzF);
+                for (int d = 0; d < 3; d++) {
+                    if (l.fixed[3 * nF + d]) continue;
+                    double v = 0;
+                    for (int dz = 0; dz < 2; dz++) for (int dy = 0; dy < 2; dy++) for (int dx = 0; dx < 2; dx++)
+                        v += c.x[3 * node_id(cx[dx], cy[dy], cz[dz], nyC, nzC) + d] * (wx[dx] * wy[dy] * wz[dz]);
+                    x[3 * nF + d] += v;
+                }
+            }
+        }
+    }
+    for (int s = 0; s < mg.nu2; s++) smooth(mg, l, b, x, om);
+}
+
+// MG hierarchy (MGPrecond3D.ensure :408-442). nlevels<=0: reference rule.
+void* or_mg_create(int nelx, int nely, int nelz, const uint8_t* fixed0, const double* ke, int nlevels,
+                   double omega, int nu1, int nu2, int coarseSolver, int coarseSweeps, int galerkinMax) {
+    OrMG* mg = new OrMG();
+    mg->ke.assign(ke, ke + 576);
+    mg->omega = omega; mg->nu1 = nu1; mg->nu2 = nu2;
+    mg->coarseSolver = coarseSolver; mg->coarseSweeps = coarseSweeps; mg->galerkinMax = galerkinMax;
+    mg->L.emplace_back();
+    level_init(mg->L[0], nelx, nely, nelz);
+    mg->L[0].fixed.assign(fixed0, fixed0 + mg->L[0].ndof);
+    build_free(mg->L[0]);
+    const int maxL = nlevels > 0 ? nlevels : 6;
+    int cx = nelx, cy = nely, cz = nelz;
+    for (int li = 1; li < maxL; li++) {
+        int nx = cx / 2, ny = cy / 2, nz = cz / 2;
+        if (nx < 2 || ny < 2 || nz < 2) break;
+        mg->L.emplace_back();
+        OrLevel& c = mg->L.back();
+        level_init(c, nx, ny, nz);
+        downsample_fixed(mg->L[li - 1], c);
+        build_free(c);
+        cx = nx; cy = ny; cz = nz;
+    }
+    return mg;
+}
+
+int or_mg_nlevels(void* h) { return (int)((OrMG*)h)->L.size(); }
+void or_mg_destroy(void* h) { delete (OrMG*)h; }
+
+// updateFromFine :558-582 (+ Cholesky of the coarsest when requested)
+int or_mg_update(void* h, const float* x, double penal, double E0, double Emin) {
+    OrMG& mg = *(OrMG*)h;
+    update_level0(mg, x, penal, E0, Emin);
+    for (size_t li = 1; li < mg.L.size(); li++) {
+        OrLevel& c = mg.L[li];
+        if (c.ndof <= mg.galerkinMax) {
+            if (!mg.L[li - 1].denseK.empty()) galerkin_from_matrix(mg, mg.L[li - 1], c);
+            else galerkin_from_elements(mg, mg.L[li - 1], c);
+        } else {
+            restrict_eavg(mg, mg.L[li - 1], c, Emin);
+        }
+    }
+    mg.omegaL.assign(mg.L.size(), mg.omega);
+    if (mg.omegaCap > 0)
+        for (size_t li = 0; li < mg.L.size(); li++) {
+            const double rho = power_rho(mg, mg.L[li]);
+            if (rho > 0) mg.omegaL[li] = std::min(mg.omega, mg.omegaCap / rho);
+        }
+    if (mg.coarseSolver == TOPO_COARSE_CHOLESKY) return coarse_cholesky(mg, mg.L.back());
+    return 0;
+}
+
+// per-level data as the reference's serializer hands it to ebeMGPCG
+// (_solveWithWasmMGPCG js/optimizer-worker-3d.js:3690-3772)
+int or_mg_level_data(void* h, int li, double* invDiag, double* denseK, int32_t* active, int32_t* nactive) {
+    OrMG& mg = *(OrMG*)h;
+    OrLevel& l = mg.L[li];
+    if (invDiag) std::copy(l.invDiag.begin(), l.invDiag.end(), invDiag);
+    if (denseK && !l.denseK.empty()) std::copy(l.denseK.begin(), l.denseK.end(), denseK);
+    if (active) std::copy(l.active.begin(), l.active.end(), active);
+    if (nactive) *nactive = (int32_t)l.active.size();
+    return 0;
+}
+
+int or_mg_set_omega_cap(void* h, double cap) { ((OrMG*)h)->omegaCap = cap; return 0; }
+double or_mg_level_omega(void* h, int li) { return ((OrMG*)h)->omegaL[li]; }
+
+// one V-cycle z = M^-1 r from z = 0 (MGPrecond3D.apply :1021-1024)
+int or_mg_vcycle(void* h, const double* r, double* z) {
+    OrMG& mg = *(OrMG*)h;
+    for (int64_t i = 0; i < mg.L[0].ndof; i++) z[i] = 0.0;
+    vcycle(mg, 0, r, z);
+    return 0;
+}
+
+int or_mg_level_applyA(void* h, int li, const double* p, double* Ap) {
+    OrMG& mg = *(OrMG*)h;
+    level_applyA(mg, mg.L[li], p, Ap);
+    return 0;
+}
+
+int or_mg_level_info(void* h, int li, int32_t* dims4, double* E, int32_t* has_dense) {
+    OrMG& mg = *(OrMG*)h;
+    OrLevel& l = mg.L[li];
+    dims4[0] = l.nelx; dims4[1] = l.nely; dims4[2] = l.nelz; dims4[3] = (int32_t)l.active.size();
+    if (E) std::copy(l.E.begin(), l.E.end(), E);
+    if (has_dense) *has_dense = !l.denseK.empty();
+    return 0;
+}
+
+// ebeMGPCG assembly/index.ts:1076-1193 (cold start unless warm, full space)
+int or_mgpcg(void* h, const double* F, double* U, int maxIter, double tol, int warm) {
+    OrMG& mg = *(OrMG*)h;
+    OrLevel& l = mg.L[0];
+    const int64_t n = l.ndof;
+    const double EPS = 1e-12;
+    dvec r(n), z(n), p(n), Ap(n);
+    if (!warm) for (int64_t i = 0; i < n; i++) U[i] = 0.0;
+    else for (int64_t i = 0; i < n; i++) if (l.fixed[i]) U[i] = 0.0;
+    if (warm) {
+        level_applyA(mg, l, U, Ap.data());
+        for (int64_t i = 0; i < n; i++) r[i] = l.fixed[i] ? 0.0 : F[i] - Ap[i];
+    } else {
+        for (int64_t i = 0; i < n; i++) r[i] = l.fixed[i] ? 0.0 : F[i];
+    }
+    or_mg_vcycle(h, r.data(), z.data());
+    for (int64_t i = 0; i < n; i++) if (l.fixed[i]) z[i] = 0.0;
+    p = z;
+    double rz = 0, r0 = 0;
+    for (int64_t i = 0; i < n; i++) { rz += r[i] * z[i]; r0 += r[i] * r[i]; }
+    const double tolSq = tol * tol * (r0 > 1e-30 ? r0 : 1e-30);
+    for (int it = 0; it < maxIter; it++) {
+        double rn = 0;
+        for (int64_t i = 0; i < n; i++) rn += r[i] * r[i];
+        if (rn < tolSq) return it;
+        level_applyA(mg, l, p.data(), Ap.data());
+        for (int64_t i = 0; i < n; i++) if (l.fixed[i]) Ap[i] = 0.0;
+        double pAp = 0;
+        for (int64_t i = 0; i < n; i++) pAp += p[i] * Ap[i];
+        double alpha = rz / (pAp + EPS);
+        for (int64_t i = 0; i < n; i++) { U[i] += alpha * p[i]; r[i] -= alpha * Ap[i]; }
+        or_mg_vcycle(h, r.data(), z.data());
+        for (int64_t i = 0; i < n; i++) if (l.fixed[i]) z[i] = 0.0;
+        double rzn = 0;
+        for (int64_t i = 0; i < n; i++) rzn += r[i] * z[i];
+        double beta = rzn / (rz + EPS);
+        for (int64_t i = 0; i < n; i++) p[i] = z[i] + beta * p[i];
+        rz = rzn;
+    }
+    return maxIter;
+}
+
+// ---------------------------------------------------------------------------
+// element energies js/optimizer-worker-3d.js:2163-2173 and
+// _computeElementEnergyFlat :1561-1571 (f64 sum, Float32 store)
+int or_energy(int nelx, int nely, int nelz, const double* ke, const double* U, float* w) {
+    const int64_t nel = (int64_t)nelx * nely * nelz;
+    std::vector<int32_t> edof(nel * 24);
+    or_edofs(nelx, nely, nelz, edof.data());
+    double Ue[24];
+    for (int64_t e = 0; e < nel; e++) {
+        for (int i = 0; i < 24; i++) Ue[i] = U[edof[e * 24 + i]];
+        double en = 0;
+        for (int i = 0; i < 24; i++)
+            for (int j = 0; j < 24; j++) en += Ue[i] * ke[i * 24 + j] * Ue[j];
+        w[e] = (float)en;
+    }
+    return 0;
+}
+
+// _heavisideProject js/optimizer-worker-3d.js:3500-3509
+int or_heaviside(const float* x, int64_t n, double beta, double eta, float* xp) {
+    const double tb = std::tanh(beta * eta), t1 = std::tanh(beta * (1 - eta)), den = tb + t1;
+    for (int64_t i = 0; i < n; i++) xp[i] = (float)((tb + std::tanh(beta * ((double)x[i] - eta))) / den);
+    return 0;
+}
+
+// filter neighbourhood of prepareFilter js/optimizer-worker-3d.js:3006-3050
+// (window +-floor(rmin), weight max(0, rmin - dist), order x, y, z)
+static void filter_weights_ref(int nelx, int nely, int nelz, double rmin, int64_t e1i, int ix, int iy, int iz,
+                               std::vector<std::pair<int64_t, double>>& out) {
+    (void)e1i;
+    out.clear();
+    const int r = (int)std::floor(rmin);
+    for (int k = std::max(ix - r, 0); k <= std::min(ix + r, nelx - 1); k++)
+        for (int l = std::max(iy - r, 0); l <= std::min(iy + r, nely - 1); l++)
+            for (int m = std::max(iz - r, 0); m <= std::min(iz + r, nelz - 1); m++) {
+                double dist = std::sqrt((double)((ix - k) * (ix - k) + (iy - l) * (iy - l) + (iz - m) * (iz - m)));
+                if (dist <= rmin) out.push_back({k + (int64_t)l * nelx + (int64_t)m * nelx * nely, std::max(0.0, rmin - dist)});
+            }
+}
+
+// filterSensitivities :3052-3064 with Hs (Float32, :3043-3047)
+int or_sens_filter(int nelx, int nely, int nelz, double rmin, const float* x, const float* dc, float* dcn) {
+    const int64_t nel = (int64_t)nelx * nely * nelz;
+    std::vector<float> Hs(nel, 0.f);
+    for (int64_t e = 0; e < nel; e++) dcn[e] = 0.f;
+    std::vector<std::pair<int64_t, double>> nb;
+    // H entries are visited in prepareFilter's order: e1 by (i,j,m) x-outer
+    for (int i = 0; i < nelx; i++)
+        for (int j = 0; j < nely; j++)
+            for (int m = 0; m < nelz; m++) {
+                int64_t e1 = i + (int64_t)j * nelx + (int64_t)m * nelx * nely;
+                filter_weights_ref(nelx, nely, nelz, rmin, e1, i, j, m, nb);
+                for (auto& q : nb) {
+                    Hs[e1] = (float)((double)Hs[e1] + q.second);
+                    dcn[e1] = (float)((double)dcn[e1] + q.second * (double)x[q.first] * (double)dc[q.first]);
+                }
+            }
+    for (int64_t e = 0; e < nel; e++)
+        dcn[e] = (float)((double)dcn[e] / ((double)Hs[e] * std::max(1e-3, (double)x[e])));
+    return 0;
+}
+
+// density filter, applyDensityFilter assembly/index.ts:224-273 (window
+// ceil(r), weight r - dist, normalised), Float32 store.
+// transpose=1 gives the chain rule dc_x[j] = sum_i w_ij in[i] / Sw_i.
+int or_density_filter(int nelx, int nely, int nelz, double radius, const float* in, float* out, int transpose) {
+    const int64_t nel = (int64_t)nelx * nely * nelz;
+    const int r = (int)std::ceil(radius);
+    const double r2 = radius * radius;
+    std::vector<double> sw(nel, 0.0), acc(nel, 0.0);
+    for (int ez = 0; ez < nelz; ez++)
+        for (int ey = 0; ey < nely; ey++)
+            for (int ex = 0; ex < nelx; ex++) {
+                int64_t o = ex + (int64_t)ey * nelx + (int64_t)ez * nelx * nely;
+                double s = 0, w = 0;
+                for (int kz = std::max(0, ez - r); kz <= std::min(nelz - 1, ez + r); kz++)
+                    for (int ky = std::max(0, ey - r); ky <= std::min(nely - 1, ey + r); ky++)
+                        for (int kx = std::max(0, ex - r); kx <= std::min(nelx - 1, ex + r); kx++) {
+                            double dx = ex - kx, dy = ey - ky, dz = ez - kz;
+                            double d2 = dx * dx + dy * dy + dz * dz;
+                            if (d2 <= r2) {
+                                double wt = radius - std::sqrt(d2);
+                                s += wt * (double)in[kx + (int64_t)ky * nelx + (int64_t)kz * nelx * nely];
+                                w += wt;
+                            }
+                        }
+                sw[o] = w;
+                acc[o] = s;
+            }
+    if (!transpose) {
+        for (int64_t e = 0; e < nel; e++) out[e] = (float)(acc[e] / sw[e]);
+        return 0;
+    }
+    // transpose: out[j] = sum_i w_ij in[i]/sw_i (w symmetric)
+    for (int ez = 0; ez < nelz; ez++)
+        for (int ey = 0; ey < nely; ey++)
+            for (int ex = 0; ex < nelx; ex++) {
+                int64_t o = ex + (int64_t)ey * nelx + (int64_t)ez * nelx * nely;
+                double s = 0;
+                for (int kz = std::max(0, ez - r); kz <= std::min(nelz - 1, ez + r); kz++)
+                    for (int ky = std::max(0, ey - r); ky <= std::min(nely - 1, ey + r); ky++)
+                        for (int kx = std::max(0, ex - r); kx <= std::min(nelx - 1, ex + r); kx++) {
+                            double dx = ex - kx, dy = ey - ky, dz = ez - kz;
+                            double d2 = dx * dx + dy * dy + dz * dz;
+                            if (d2 <= r2) {
+                                int64_t i = kx + (int64_t)ky * nelx + (int64_t)kz * nelx * nely;
+                                s += (radius - std::sqrt(d2)) * (double)in[i] / sw[i];
+                            }
+                        }
+                out[o] = (float)s;
+            }
+    return 0;
+}
+
+// OC js/optimizer-worker-3d.js:3066-3101 (Float32 xnew, f64 volume sum)
+int or_oc(int64_t nel, const float* x, double volfrac, const float* dc, const uint8_t* pres, const uint8_t* voidm,
+          double move, float* xnew) {
+    double l1 = 0, l2 = 1e9;
+    int passes = 0;
+    auto jsmax = [](double a, double b) { return (std::isnan(a) || std::isnan(b)) ? NAN : (a > b ? a : b); };
+    auto jsmin = [](double a, double b) { return (std::isnan(a) || std::isnan(b)) ? NAN : (a < b ? a : b); };
+    while ((l2 - l1) / (l2 + l1) > 1e-3) {
+        double lmid = 0.5 * (l2 + l1);
+        for (int64_t i = 0; i < nel; i++) {
+            if (pres && pres[i]) xnew[i] = 1.0f;
+            else if (voidm && voidm[i]) xnew[i] = 0.0f;
+            else {
+                double Be = -(double)dc[i] / lmid;
+                double xi = x[i];
+                xnew[i] = (float)jsmax(0.0, jsmax(xi - move, jsmin(1.0, jsmin(xi + move, xi * std::sqrt(Be)))));
+            }
+        }
+        double s = 0;
+        for (int64_t i = 0; i < nel; i++) s += xnew[i];
+        if (s > volfrac * nel) l1 = lmid; else l2 = lmid;
+        passes++;
+    }
+    return passes;
+}
+
+// ---------------------------------------------------------------------------
+// SIMP loop: optimize() js/optimizer-worker-3d.js:1923-2352 restricted to the
+// hot path (no AMR/manufacturing/UI). Runs `niter` iterations (or stops at
+// change <= 0.01 like :2074). Outputs per-iteration compliance and the final
+// design. Solver per cfg->linear_solver: PCG = ebePCG with warm start
+// (:3572-3629); MGPCG = ebeMGPCG cold start (:3642-3807) with the MG options.
+int or_simp_run(const topo_config* cfg, const uint8_t* fixed, const float* F32, const uint8_t* pres,
+                const uint8_t* voidm, int niter, float* x_io, double* comp_out, double* change_out,
+                int32_t* cg_out, double* U_out) {
+    const int nelx = cfg->nelx, nely = cfg->nely, nelz = cfg->nelz;
+    const int64_t nel = (int64_t)nelx * nely * nelz;
+    const int64_t ndof = 3LL * (nelx + 1) * (nely + 1) * (nelz + 1);
+    dvec ke(576);
+    or_k0(cfg->nu, ke.data());
+    std::vector<int32_t> freed;
+    for (int64_t i = 0; i < ndof; i++) if (!fixed[i]) freed.push_back((int32_t)i);
+    dvec F(ndof);
+    for (int64_t i = 0; i < ndof; i++) F[i] = F32[i];
+    std::vector<float> x(x_io, x_io + nel), xnew(nel), xold(nel), xPhys(nel), w(nel), dc(nel), dcn(nel), xf(nel);
+    dvec U(ndof, 0.0);
+    void* mg = nullptr;
+    if (cfg->linear_solver == TOPO_SOLVER_MGPCG)
+    {
+        mg = or_mg_create(nelx, nely, nelz, fixed, ke.data(), cfg->mg_levels, cfg->mg_omega, cfg->mg_nu1, cfg->mg_nu2,
+                          cfg->coarse_solver, cfg->coarse_sweeps, cfg->galerkin_max_ndof);
+        or_mg_set_omega_cap(mg, cfg->mg_omega_cap);
+    }
+    const double penalTarget = cfg->penal, penalStart = cfg->penal_start;
+    const int contIters = cfg->continuation_iters;
+    const bool proj = cfg->use_projection != 0;
+    const int maxIt = cfg->max_iterations;
+    double change = 1.0;
+    int loop = 0;
+    std::vector<double> sw;
+    while (change > 0.01 && loop < maxIt && loop < niter) {
+        loop++;
+        xold = x;
+        const double penal = penalStart + (penalTarget - penalStart) * std::min(1.0, (double)(loop - 1) / contIters);
+        double beta = 1;
+        if (proj) beta = cfg->beta_fixed > 0 ? cfg->beta_fixed
+                                             : std::min(cfg->beta_max, std::pow(2.0, std::floor((double)(loop - 1) / cfg->beta_interval)));
+        // design -> physical density
+        if (cfg->filter_type == TOPO_FILTER_DENSITY) {
+            or_density_filter(nelx, nely, nelz, cfg->rmin, x.data(), xf.data(), 0);
+        } else {
+            xf = x;
+        }
+        if (proj) or_heaviside(xf.data(), nel, beta, cfg->eta, xPhys.data());
+        else xPhys = xf;
+        // FE (:4155)
+        const double progress = maxIt > 1 ? (double)(loop - 1) / (maxIt - 1) : 1.0;
+        const double tol = cfg->cg_tol > 0 ? cfg->cg_tol
+                                           : std::exp(std::log(1e-3) + progress * (std::log(1e-8) - std::log(1e-3)));
+        int its;
+        if (cfg->linear_solver == TOPO_SOLVER_PCG) {
+            dvec dens(xPhys.begin(), xPhys.end());
+            if (!(loop > 1 && cfg->warm_start != 0)) std::fill(U.begin(), U.end(), 0.0);
+            int maxI = std::min((int)freed.size(), cfg->cg_max_iter);
+            its = or_ebePCG(dens.data(), ke.data(), nelx, nely, nelz, F.data(), U.data(), freed.data(), (int)freed.size(),
+                            cfg->Emin, cfg->E0, penal, maxI, tol);
+        } else {
+            or_mg_update(mg, xPhys.data(), penal, cfg->E0, cfg->Emin);
+            int maxI = std::min((int)freed.size(), cfg->cg_max_iter);
+            its = or_mgpcg(mg, F.data(), U.data(), maxI, tol, cfg->warm_start == 1 && loop > 1);
+        }
+        double c = 0;
+        for (int64_t i = 0; i < ndof; i++) c += F[i] * U[i];
+        or_energy(nelx, nely, nelz, ke.data(), U.data(), w.data());
+        // sensitivities :2175-2192
+        const double denom = 2 * std::tanh(beta * 0.5);
+        for (int64_t e = 0; e < nel; e++) {
+            double dphys = -penal * std::pow((double)xPhys[e], penal - 1) * cfg->E0 * (double)w[e];
+            double d = 1.0;
+            if (proj) {
+                double th = std::tanh(beta * ((double)xf[e] - 0.5));
+                d = beta * (1 - th * th) / denom;
+            }
+            dc[e] = (float)(dphys * d);
+        }
+        if (cfg->filter_type == TOPO_FILTER_DENSITY) {
+            or_density_filter(nelx, nely, nelz, cfg->rmin, dc.data(), dcn.data(), 1);
+        } else {
+            or_sens_filter(nelx, nely, nelz, cfg->rmin, x.data(), dc.data(), dcn.data());
+        }
+        or_oc(nel, x.data(), cfg->volfrac, dcn.data(), pres, voidm, cfg->move, xnew.data());
+        change = 0;
+        for (int64_t i = 0; i < nel; i++) change = std::max(change, (double)std::fabs(xnew[i] - xold[i]));
+        std::swap(x, xnew);
+        if (comp_out) comp_out[loop - 1] = c;
+        if (change_out) change_out[loop - 1] = change;
+        if (cg_out) cg_out[loop - 1] = its;
+    }
+    std::copy(x.begin(), x.end(), x_io);
+    if (U_out) std::copy(U.begin(), U.end(), U_out);
+    if (mg) or_mg_destroy(mg);
+    return loop;
+}
+
+}  // extern "C"

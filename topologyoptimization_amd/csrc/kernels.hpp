@@ -1,0 +1,1022 @@
+// kernels.hpp -- device kernels of libtopo_hip (gfx950). Included by exactly
+// one translation unit (solver.hip).
+//
+// Layout in HBM
+//   DOF vectors    reference order 3n+d, n = (ix*nny + iy)*nnz + iz
+//   element arrays "zyx" order ei = (ex*nely + ey)*nelz + ez, so that
+//                  consecutive lanes (consecutive iz) read consecutive
+//                  elements; converted from/to the reference order
+//                  (x fastest) at the host boundary.
+#pragma once
+#include "common.hpp"
+
+namespace topo {
+
+// K0 (24x24 row-major, FP64 and FP32 copies) and the filter stencil live in
+// per-solver device buffers passed by pointer; every lane reads the same
+// address, so the loads are scalar.
+
+// ---------------------------------------------------------------------------
+// small helpers
+__device__ __forceinline__ int64_t gtid() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
+__device__ __forceinline__ int64_t gstride() { return (int64_t)gridDim.x * blockDim.x; }
+
+// local index of the element node with offsets (a,b,c) in {0,1}^3
+// (order (0,0,0),(1,0,0),(1,1,0),(0,1,0),(0,0,1),(1,0,1),(1,1,1),(0,1,1))
+__host__ __device__ __forceinline__ int local_node(int a, int b, int c) {
+    return c * 4 + (b ? (a ? 2 : 3) : (a ? 1 : 0));
+}
+__host__ __device__ __forceinline__ void local_offsets(int L, int& a, int& b, int& c) {
+    c = L >> 2;
+    const int q = L & 3;
+    a = (q == 1 || q == 2);
+    b = (q >= 2);
+}
+
+
+// JS Math.max / Math.min semantics (NaN propagates)
+__device__ __forceinline__ double js_max(double a, double b) { return (a != a || b != b) ? __builtin_nan("") : (a > b ? a : b); }
+__device__ __forceinline__ double js_min(double a, double b) { return (a != a || b != b) ? __builtin_nan("") : (a < b ? a : b); }
+
+// ---------------------------------------------------------------------------
+// element order conversions
+__global__ void k_ref_to_zyx(const float* __restrict__ in, float* __restrict__ out, Grid g) {
+    for (int64_t i = gtid(); i < g.nel(); i += gstride()) {
+        const int64_t ez = i % g.nelz, t = i / g.nelz, ey = t % g.nely, ex = t / g.nely;
+        out[i] = in[ex + g.nelx * (ey + (int64_t)g.nely * ez)];
+    }
+}
+__global__ void k_zyx_to_ref(const float* __restrict__ in, float* __restrict__ out, Grid g) {
+    for (int64_t i = gtid(); i < g.nel(); i += gstride()) {
+        const int64_t ez = i % g.nelz, t = i / g.nelz, ey = t % g.nely, ex = t / g.nely;
+        out[ex + g.nelx * (ey + (int64_t)g.nely * ez)] = in[i];
+    }
+}
+__global__ void k_u8_ref_to_zyx(const uint8_t* __restrict__ in, uint8_t* __restrict__ out, Grid g) {
+    for (int64_t i = gtid(); i < g.nel(); i += gstride()) {
+        const int64_t ez = i % g.nelz, t = i / g.nelz, ey = t % g.nely, ex = t / g.nely;
+        out[i] = in[ex + g.nelx * (ey + (int64_t)g.nely * ez)];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Element moduli E_e = Emin + x^p (E0 - Emin).
+// powmode 0: _powDensity fast path for integer p (js/optimizer-worker-3d.js:293-306)
+// powmode 1: Math.pow as in ebePCG (assembly/index.ts:544)
+__device__ __forceinline__ double pow_density(double x, double p, int powmode) {
+#pragma clang fp contract(off)
+    if (powmode == 0 && p == __builtin_floor(p) && p >= 0 && p <= 6) {
+        switch ((int)p) {
+        case 0: return 1.0;
+        case 1: return x;
+        case 2: return x * x;
+        case 3: return x * x * x;
+        case 4: { double x2 = x * x; return x2 * x2; }
+        case 5: { double x2 = x * x; return x2 * x2 * x; }
+        case 6: { double x2 = x * x; return x2 * x2 * x2; }
+        }
+    }
+    return pow(x, p);
+}
+
+template <typename TX>
+__global__ void k_moduli(const TX* __restrict__ xphys, double penal, double E0, double Emin, int powmode,
+                         double* __restrict__ E, float* __restrict__ E32, int64_t nel) {
+#pragma clang fp contract(off)
+    const double dE = E0 - Emin;
+    for (int64_t i = gtid(); i < nel; i += gstride()) {
+        const double v = Emin + pow_density((double)xphys[i], penal, powmode) * dE;
+        E[i] = v;
+        if (E32) E32[i] = (float)v;
+    }
+}
+
+// Jacobi diagonal: diag[3n+d] = sum over active elements around n of
+// E_e K0_ii, added in ascending reference element order like
+// _updateOperatorLevel (js/optimizer-worker-3d.js:773-784); invDiag = 1/d
+// on free DOFs with d > 1e-30, else 0 (:787-795).
+__global__ void k_invdiag(const double* __restrict__ E, double thr, const uint8_t* __restrict__ fixed,
+                          double* __restrict__ invD, float* __restrict__ invD32, Grid g, const double* __restrict__ ke) {
+#pragma clang fp contract(off)
+    const int nny = g.nny(), nnz = g.nnz();
+    for (int64_t n = gtid(); n < g.nnodes(); n += gstride()) {
+        const int iz = (int)(n % nnz);
+        const int64_t t = n / nnz;
+        const int iy = (int)(t % nny), ix = (int)(t / nny);
+        double acc[3] = {0.0, 0.0, 0.0};
+        for (int c = 1; c >= 0; c--) {
+            const int ez = iz - c;
+            if (ez < 0 || ez >= g.nelz) continue;
+            for (int b = 1; b >= 0; b--) {
+                const int ey = iy - b;
+                if (ey < 0 || ey >= g.nely) continue;
+                for (int a = 1; a >= 0; a--) {
+                    const int ex = ix - a;
+                    if (ex < 0 || ex >= g.nelx) continue;
+                    const double Ee = E[((int64_t)ex * g.nely + ey) * g.nelz + ez];
+                    if (!(Ee > thr)) continue;
+                    const int L = local_node(a, b, c);
+                    for (int d = 0; d < 3; d++) acc[d] += Ee * ke[(3 * L + d) * 25];
+                }
+            }
+        }
+        for (int d = 0; d < 3; d++) {
+            const int64_t i = 3 * n + d;
+            const double v = (!fixed[i] && acc[d] > 1e-30) ? 1.0 / acc[d] : 0.0;
+            invD[i] = v;
+            if (invD32) invD32[i] = (float)v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// applyA, reference-order gather ("exact" kernel).
+// Ap[3n+d] = sum over active elements e around n, in ascending reference
+// element order, of E_e * (sum_j K0[3L+d][j] p[edof_e[j]]) with j ascending:
+// the addition sequence of applyAEbe3D (assembly/index.ts:403-439), without
+// FMA contraction, so FP64 results are identical to the reference's.
+// mode: 0 plain Ap; 1 masked (Ap = 0 on fixed DOFs); 2 residual
+// out = b - Ap on free DOFs, 0 on fixed; 3 damped-Jacobi sweep
+// out = x + omega*invD*(b - Ap) on free DOFs, x on fixed.
+template <typename T, typename TE>
+__global__ void __launch_bounds__(256) k_applyA_exact(const T* __restrict__ p, T* __restrict__ out,
+                                                      const TE* __restrict__ E, double thr, Grid g,
+                                                      const uint8_t* __restrict__ fixed, int mode,
+                                                      const T* __restrict__ b, const T* __restrict__ invD,
+                                                      T omega, const int* gate, const T* __restrict__ ke) {
+#pragma clang fp contract(off)
+    if (gate && *gate) return;
+    const int nny = g.nny(), nnz = g.nnz();
+    const int64_t nyz = (int64_t)nny * nnz;
+    for (int64_t n = gtid(); n < g.nnodes(); n += gstride()) {
+        const int iz = (int)(n % nnz);
+        const int64_t t = n / nnz;
+        const int iy = (int)(t % nny), ix = (int)(t / nny);
+        T acc[3] = {T(0), T(0), T(0)};
+        for (int c = 1; c >= 0; c--) {
+            const int ez = iz - c;
+            if (ez < 0 || ez >= g.nelz) continue;
+            for (int bb = 1; bb >= 0; bb--) {
+                const int ey = iy - bb;
+                if (ey < 0 || ey >= g.nely) continue;
+                for (int a = 1; a >= 0; a--) {
+                    const int ex = ix - a;
+                    if (ex < 0 || ex >= g.nelx) continue;
+                    const TE Ee = E[((int64_t)ex * g.nely + ey) * g.nelz + ez];
+                    if (!((double)Ee > thr)) continue;
+                    const int L = local_node(a, bb, c);
+                    const int64_t n0 = ((int64_t)ex * nny + ey) * nnz + ez;
+                    T loc[24];
+#pragma unroll
+                    for (int m = 0; m < 8; m++) {
+                        int ma, mb, mc;
+                        local_offsets(m, ma, mb, mc);
+                        const int64_t nm = n0 + ma * nyz + mb * nnz + mc;
+                        loc[3 * m] = p[3 * nm];
+                        loc[3 * m + 1] = p[3 * nm + 1];
+                        loc[3 * m + 2] = p[3 * nm + 2];
+                    }
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        const int row = (3 * L + d) * 24;
+                        T s = T(0);
+#pragma unroll
+                        for (int j = 0; j < 24; j++) s += ke[row + j] * loc[j];
+                        acc[d] += (T)Ee * s;
+                    }
+                }
+            }
+        }
+        for (int d = 0; d < 3; d++) {
+            const int64_t i = 3 * n + d;
+            T v = acc[d];
+            if (mode == 1) {
+                if (fixed[i]) v = T(0);
+            } else if (mode == 2) {
+                v = fixed[i] ? T(0) : b[i] - v;
+            } else if (mode == 3) {
+                const T r = b[i] - v;
+                v = fixed[i] ? p[i] : p[i] + omega * invD[i] * r;
+            }
+            out[i] = v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic reductions. Each block reduces its grid-stride share in a
+// fixed tree, writes partial[block]; the last block to finish (ticket
+// counter) adds the partials in block order and runs the epilogue. Results
+// depend only on n and the launch shape, never on timing.
+constexpr int kRedBlocks = 512;
+constexpr int kRedThreads = 256;
+
+template <int NV>
+__device__ __forceinline__ void block_reduce(double (&v)[NV], double* sh /* NV*kRedThreads/64 */) {
+    // wave64 tree with fixed shuffle pattern, then the 4 wave sums in order
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        double x = v[k];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        v[k] = x;
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int k = 0; k < NV; k++) sh[k * (kRedThreads / 64) + wid] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int k = 0; k < NV; k++) {
+            double s = 0;
+            for (int w = 0; w < kRedThreads / 64; w++) s += sh[k * (kRedThreads / 64) + w];
+            v[k] = s;
+        }
+}
+
+// returns true in thread 0 of the last block, with tot[] = the grid sums
+template <int NV>
+__device__ __forceinline__ bool grid_reduce(double (&v)[NV], double* partials, unsigned* ticket, double (&tot)[NV]) {
+    __shared__ double sh[NV * (kRedThreads / 64)];
+    __shared__ bool last;
+    block_reduce<NV>(v, sh);
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < NV; k++) partials[k * kRedBlocks + blockIdx.x] = v[k];
+        __threadfence();
+        const unsigned t = atomicAdd(ticket, 1u);
+        last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return false;
+    if (threadIdx.x == 0) {
+        __threadfence();
+        for (int k = 0; k < NV; k++) {
+            double s = 0;
+            for (unsigned bI = 0; bI < gridDim.x; bI++) s += __builtin_nontemporal_load(&partials[k * kRedBlocks + bI]);
+            tot[k] = s;
+        }
+        *ticket = 0u;   // ready for the next launch on this stream
+        return true;
+    }
+    return false;
+}
+
+// CG scalars living on the device (no host round trip per iteration)
+struct CGState {
+    double rz, rz_new, pAp, alpha, beta;
+    double rn;        // ||r||^2 at the last update
+    double r0;        // ||r0||^2
+    double tolSq;
+    double tol;
+    int done;         // converged: later update kernels are no-ops
+    int iters;        // iterations completed
+    int maxit;
+    int nonfinite;
+};
+
+// rn = ||r||^2 and (optionally) rz = r.z; used at CG start
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads) k_cg_start(const T* __restrict__ r, const T* __restrict__ z, int64_t n,
+                                                         CGState* st, double* partials, unsigned* ticket) {
+    double v[2] = {0.0, 0.0};
+    for (int64_t i = gtid(); i < n; i += gstride()) {
+        const double ri = (double)r[i];
+        v[0] += ri * ri;
+        v[1] += ri * (double)z[i];
+    }
+    double tot[2];
+    if (grid_reduce<2>(v, partials, ticket, tot)) {
+        st->r0 = tot[0];
+        st->rn = tot[0];
+        st->rz = tot[1];
+        st->tolSq = st->tol * st->tol * (tot[0] > 1e-30 ? tot[0] : 1e-30);
+        st->done = (st->maxit <= 0) || (tot[0] < st->tolSq);
+        st->iters = 0;
+        st->nonfinite = 0;
+    }
+}
+
+// pAp = p.Ap -> alpha = rz/(pAp + 1e-12) (ebePCG :695-699)
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads) k_cg_alpha(const T* __restrict__ p, const T* __restrict__ Ap, int64_t n,
+                                                         CGState* st, double* partials, unsigned* ticket) {
+    if (st->done) return;
+    double v[1] = {0.0};
+    for (int64_t i = gtid(); i < n; i += gstride()) v[0] += (double)p[i] * (double)Ap[i];
+    double tot[1];
+    if (grid_reduce<1>(v, partials, ticket, tot)) {
+        st->pAp = tot[0];
+        st->alpha = st->rz / (tot[0] + 1e-12);
+        if (!isfinite(st->alpha)) st->nonfinite = 1;
+    }
+}
+
+// Jacobi variant: U += alpha p; r -= alpha Ap; z = invD r; rz_new, ||r||^2;
+// then beta = rz_new/(rz + 1e-12) (ebePCG :701-719). The convergence test
+// of the next iteration (:645-650) is folded in: done = ||r||^2 < tolSq.
+template <typename T, typename TU>
+__global__ void __launch_bounds__(kRedThreads) k_cg_update_jacobi(TU* __restrict__ U, T* __restrict__ r, T* __restrict__ z,
+                                                                 const T* __restrict__ p, const T* __restrict__ Ap,
+                                                                 const T* __restrict__ invD, int64_t n, CGState* st,
+                                                                 double* partials, unsigned* ticket) {
+    if (st->done) return;
+    const double alpha = st->alpha;
+    double v[2] = {0.0, 0.0};
+    for (int64_t i = gtid(); i < n; i += gstride()) {
+        const T ri = (T)((double)r[i] - alpha * (double)Ap[i]);
+        const T zi = invD[i] * ri;
+        U[i] = (TU)((double)U[i] + alpha * (double)p[i]);
+        r[i] = ri;
+        z[i] = zi;
+        v[0] += (double)ri * (double)zi;
+        v[1] += (double)ri * (double)ri;
+    }
+    double tot[2];
+    if (grid_reduce<2>(v, partials, ticket, tot)) {
+        st->rz_new = tot[0];
+        st->rn = tot[1];
+        st->beta = tot[0] / (st->rz + 1e-12);
+        st->rz = tot[0];
+        st->iters += 1;
+        if (!isfinite(tot[0]) || !isfinite(tot[1])) st->nonfinite = 1;
+        st->done = (tot[1] < st->tolSq) || (st->iters >= st->maxit) || st->nonfinite;
+    }
+}
+
+// MG variant part 1: U += alpha p; r -= alpha Ap (ebeMGPCG :1158-1166) and
+// ||r||^2 for the next convergence test.
+template <typename T, typename TU>
+__global__ void __launch_bounds__(kRedThreads) k_cg_update_ur(TU* __restrict__ U, T* __restrict__ r, const T* __restrict__ p,
+                                                             const T* __restrict__ Ap, int64_t n, CGState* st,
+                                                             double* partials, unsigned* ticket) {
+    if (st->done) return;
+    const double alpha = st->alpha;
+    double v[1] = {0.0};
+    for (int64_t i = gtid(); i < n; i += gstride()) {
+        U[i] = (TU)((double)U[i] + alpha * (double)p[i]);
+        const T ri = (T)((double)r[i] - alpha * (double)Ap[i]);
+        r[i] = ri;
+        v[0] += (double)ri * (double)ri;
+    }
+    double tot[1];
+    if (grid_reduce<1>(v, partials, ticket, tot)) {
+        st->rn = tot[0];
+        st->iters += 1;
+        if (!isfinite(tot[0])) st->nonfinite = 1;
+    }
+}
+
+// MG variant part 2 (after z = M^-1 r with fixed DOFs zeroed):
+// rz_new = r.z, beta = rz_new/(rz + 1e-12) (:1174-1187); the convergence
+// flag is set here so the p-update of a converged iteration still runs,
+// exactly like the reference (which tests at the top of the next iteration).
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads) k_cg_rz(const T* __restrict__ r, const T* __restrict__ z, int64_t n,
+                                                      CGState* st, double* partials, unsigned* ticket) {
+    if (st->done) return;
+    double v[1] = {0.0};
+    for (int64_t i = gtid(); i < n; i += gstride()) v[0] += (double)r[i] * (double)z[i];
+    double tot[1];
+    if (grid_reduce<1>(v, partials, ticket, tot)) {
+        st->rz_new = tot[0];
+        st->beta = tot[0] / (st->rz + 1e-12);
+        st->rz = tot[0];
+        if (!isfinite(tot[0])) st->nonfinite = 1;
+    }
+}
+
+// p = z + beta p; afterwards evaluate the convergence flag
+template <typename T>
+__global__ void k_cg_pupdate(T* __restrict__ p, const T* __restrict__ z, int64_t n, const CGState* st) {
+    if (st->done) return;
+    const T beta = (T)st->beta;
+    for (int64_t i = gtid(); i < n; i += gstride()) p[i] = z[i] + beta * p[i];
+}
+__global__ void k_cg_check(CGState* st) {
+    if (st->done) return;
+    st->done = (st->rn < st->tolSq) || (st->iters >= st->maxit) || st->nonfinite;
+}
+
+// ---------------------------------------------------------------------------
+// vector helpers
+template <typename T>
+__global__ void k_fill(T* __restrict__ v, T val, int64_t n, const int* gate) {
+    if (gate && *gate) return;
+    for (int64_t i = gtid(); i < n; i += gstride()) v[i] = val;
+}
+template <typename TD, typename TS>
+__global__ void k_convert(TD* __restrict__ d, const TS* __restrict__ s, int64_t n) {
+    for (int64_t i = gtid(); i < n; i += gstride()) d[i] = (TD)s[i];
+}
+template <typename T>
+__global__ void k_mask(T* __restrict__ v, const uint8_t* __restrict__ fixed, int64_t n) {
+    for (int64_t i = gtid(); i < n; i += gstride())
+        if (fixed[i]) v[i] = T(0);
+}
+// r = masked F (cold start of ebeMGPCG :1104-1112)
+template <typename T>
+__global__ void k_masked_copy(T* __restrict__ d, const double* __restrict__ s, const uint8_t* __restrict__ fixed, int64_t n) {
+    for (int64_t i = gtid(); i < n; i += gstride()) d[i] = fixed[i] ? T(0) : (T)s[i];
+}
+// z = invD r
+template <typename T>
+__global__ void k_jacobi_apply(T* __restrict__ z, const T* __restrict__ r, const T* __restrict__ invD, int64_t n) {
+    for (int64_t i = gtid(); i < n; i += gstride()) z[i] = invD[i] * r[i];
+}
+// U += s * d (refinement step, FP64 += FP32)
+template <typename TS>
+__global__ void k_axpy_refine(double* __restrict__ U, const TS* __restrict__ d, double scale, int64_t n) {
+    for (int64_t i = gtid(); i < n; i += gstride()) U[i] += scale * (double)d[i];
+}
+
+__global__ void k_add_f(float* __restrict__ a, const float* __restrict__ b, int64_t n) {
+    for (int64_t i = gtid(); i < n; i += gstride()) a[i] = a[i] + b[i];
+}
+
+// dot products into a device double (deterministic)
+template <typename TA, typename TB>
+__global__ void __launch_bounds__(kRedThreads) k_dot(const TA* __restrict__ a, const TB* __restrict__ b, int64_t n,
+                                                    double* out, double* partials, unsigned* ticket) {
+    double v[1] = {0.0};
+    for (int64_t i = gtid(); i < n; i += gstride()) v[0] += (double)a[i] * (double)b[i];
+    double tot[1];
+    if (grid_reduce<1>(v, partials, ticket, tot)) *out = tot[0];
+}
+
+// scaled residual for the mixed inner solve: rs = (float)(r / s), s = ||r||_inf-ish scale
+template <typename T>
+__global__ void k_scale_to(T* __restrict__ d, const double* __restrict__ s, const double* scale, int64_t n) {
+    const double inv = 1.0 / *scale;
+    for (int64_t i = gtid(); i < n; i += gstride()) d[i] = (T)(s[i] * inv);
+}
+
+// power iteration for lambda_max(D^-1 A): v0 hashed on free DOFs, nv = ||v0||^2
+__global__ void k_power_init(double* __restrict__ v, const uint8_t* __restrict__ fixed, int64_t n, double* sc) {
+    for (int64_t i = gtid(); i < n; i += gstride())
+        v[i] = fixed[i] ? 0.0 : (double)(((uint64_t)i * 2654435761ULL) & 0xffffffffULL) / 4294967296.0 - 0.5;
+    if (gtid() == 0) sc[0] = -1.0;   // nv computed by the first scale step
+}
+// w *= invD; nw = ||w||^2; rho = sqrt(nw/nv) (sc: [nv, nw, rho, 1/sqrt(nw)])
+__global__ void __launch_bounds__(kRedThreads) k_power_scale(double* __restrict__ w, const double* __restrict__ invD,
+                                                            int64_t n, double* sc, double* partials, unsigned* ticket) {
+    double v[1] = {0.0};
+    for (int64_t i = gtid(); i < n; i += gstride()) {
+        const double t = w[i] * invD[i];
+        w[i] = t;
+        v[0] += t * t;
+    }
+    double tot[1];
+    if (grid_reduce<1>(v, partials, ticket, tot)) {
+        sc[1] = tot[0];
+        sc[3] = tot[0] > 0 ? 1.0 / sqrt(tot[0]) : 0.0;
+    }
+}
+__global__ void k_power_norm(double* __restrict__ v, const double* __restrict__ w, int64_t n, const double* sc) {
+    // v holds the previous iterate: nv = ||v||^2 is 1 after the first step
+    const double s = sc[3];
+    for (int64_t i = gtid(); i < n; i += gstride()) v[i] = w[i] * s;
+}
+
+// ---------------------------------------------------------------------------
+// Multigrid transfer (trilinear, clamped at odd/boundary nodes like
+// _buildProlongMap js/optimizer-worker-3d.js:591-603).
+__device__ __forceinline__ void pw1d(int f, int nC, int& c0, int& c1, double& w0, double& w1) {
+    c0 = min(f >> 1, nC - 1);
+    c1 = min(c0 + 1, nC - 1);
+    const double t = (c0 != c1 && (f & 1)) ? 0.5 : 0.0;
+    w0 = 1 - t;
+    w1 = t;
+}
+
+// bC = P^T res, gathered per coarse node; contributions are added in the
+// reference's visiting order (fine fz, fy, fx ascending; one corner each,
+// mgVcycle assembly/index.ts:868-961), then fixed coarse DOFs zeroed (:964-967).
+template <typename T>
+__global__ void k_restrict(const T* __restrict__ res, T* __restrict__ bC, Grid gf, Grid gc,
+                           const uint8_t* __restrict__ fixedC, const int* gate) {
+#pragma clang fp contract(off)
+    if (gate && *gate) return;
+    const int nxF = gf.nnx(), nyF = gf.nny(), nzF = gf.nnz();
+    const int nxC = gc.nnx(), nyC = gc.nny(), nzC = gc.nnz();
+    for (int64_t nc = gtid(); nc < gc.nnodes(); nc += gstride()) {
+        const int cz = (int)(nc % nzC);
+        const int64_t t = nc / nzC;
+        const int cy = (int)(t % nyC), cx = (int)(t / nyC);
+        // per-axis contributing fine indices (ascending) and weights
+        int fl[3][6];
+        double wl[3][6];
+        int cnt[3] = {0, 0, 0};
+        const int cc[3] = {cx, cy, cz};
+        const int nF[3] = {nxF, nyF, nzF};
+        const int nC[3] = {nxC, nyC, nzC};
+        for (int ax = 0; ax < 3; ax++) {
+            const int lo = max(0, 2 * cc[ax] - 1), hi = min(nF[ax] - 1, 2 * cc[ax] + 4);
+            for (int f = lo; f <= hi; f++) {
+                int c0, c1;
+                double w0, w1;
+                pw1d(f, nC[ax], c0, c1, w0, w1);
+                double w = 0;
+                if (c0 == cc[ax] && w0 > 0) w = w0;
+                else if (c1 == cc[ax] && w1 > 0) w = w1;
+                if (w > 0 && cnt[ax] < 6) {
+                    fl[ax][cnt[ax]] = f;
+                    wl[ax][cnt[ax]] = w;
+                    cnt[ax]++;
+                }
+            }
+        }
+        T acc[3] = {T(0), T(0), T(0)};
+        for (int iz = 0; iz < cnt[2]; iz++)
+            for (int iy = 0; iy < cnt[1]; iy++)
+                for (int ix = 0; ix < cnt[0]; ix++) {
+                    const double w = (wl[0][ix] * wl[1][iy]) * wl[2][iz];
+                    const int64_t nf = ((int64_t)fl[0][ix] * nyF + fl[1][iy]) * nzF + fl[2][iz];
+                    for (int d = 0; d < 3; d++) acc[d] += (T)w * res[3 * nf + d];
+                }
+        for (int d = 0; d < 3; d++) bC[3 * nc + d] = fixedC[3 * nc + d] ? T(0) : acc[d];
+    }
+}
+
+// x += P xC on free fine DOFs (mgVcycle :977-1042): the 8 corner terms are
+// summed left to right in the order 000,100,010,110,001,101,011,111.
+template <typename T>
+__global__ void k_prolong_add(T* __restrict__ x, const T* __restrict__ xC, Grid gf, Grid gc,
+                              const uint8_t* __restrict__ fixedF, const int* gate) {
+#pragma clang fp contract(off)
+    if (gate && *gate) return;
+    const int nyF = gf.nny(), nzF = gf.nnz();
+    const int nyC = gc.nny(), nzC = gc.nnz(), nxC = gc.nnx();
+    for (int64_t nf = gtid(); nf < gf.nnodes(); nf += gstride()) {
+        const int fz = (int)(nf % nzF);
+        const int64_t t = nf / nzF;
+        const int fy = (int)(t % nyF), fx = (int)(t / nyF);
+        int cx[2], cy[2], cz[2];
+        double wx[2], wy[2], wz[2];
+        pw1d(fx, nxC, cx[0], cx[1], wx[0], wx[1]);
+        pw1d(fy, nyC, cy[0], cy[1], wy[0], wy[1]);
+        pw1d(fz, nzC, cz[0], cz[1], wz[0], wz[1]);
+        for (int d = 0; d < 3; d++) {
+            const int64_t i = 3 * nf + d;
+            if (fixedF[i]) continue;
+            T v = T(0);
+            bool first = true;
+            for (int dz = 0; dz < 2; dz++)
+                for (int dy = 0; dy < 2; dy++)
+                    for (int dx = 0; dx < 2; dx++) {
+                        const double w = (wx[dx] * wy[dy]) * wz[dz];
+                        const int64_t nc = ((int64_t)cx[dx] * nyC + cy[dy]) * nzC + cz[dz];
+                        const T term = xC[3 * nc + d] * (T)w;
+                        v = first ? term : v + term;
+                        first = false;
+                    }
+            x[i] = x[i] + v;
+        }
+    }
+}
+
+// coarse moduli: E_C = 2 * mean of the existing fine children (dz,dy,dx
+// order), _restrictEvalAvg js/optimizer-worker-3d.js:702-724
+__global__ void k_coarsen_E(const double* __restrict__ Ef, double* __restrict__ Ec, float* __restrict__ Ec32,
+                            Grid gf, Grid gc) {
+#pragma clang fp contract(off)
+    for (int64_t i = gtid(); i < gc.nel(); i += gstride()) {
+        const int cz = (int)(i % gc.nelz);
+        const int64_t t = i / gc.nelz;
+        const int cy = (int)(t % gc.nely), cx = (int)(t / gc.nely);
+        double sum = 0;
+        int cnt = 0;
+        for (int dz = 0; dz < 2; dz++) {
+            const int fz = 2 * cz + dz;
+            if (fz >= gf.nelz) continue;
+            for (int dy = 0; dy < 2; dy++) {
+                const int fy = 2 * cy + dy;
+                if (fy >= gf.nely) continue;
+                for (int dx = 0; dx < 2; dx++) {
+                    const int fx = 2 * cx + dx;
+                    if (fx >= gf.nelx) continue;
+                    sum += Ef[((int64_t)fx * gf.nely + fy) * gf.nelz + fz];
+                    cnt++;
+                }
+            }
+        }
+        const double v = cnt > 0 ? (sum / cnt) * 2.0 : 0.0;
+        Ec[i] = v;
+        if (Ec32) Ec32[i] = (float)v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// dense levels (Galerkin operators, ndof <= galerkin_max_ndof)
+// y = K x, one wave per row, or a damped-Jacobi sweep / residual with it
+template <typename T>
+__global__ void __launch_bounds__(256) k_dense_apply(const T* __restrict__ K, const T* __restrict__ xin, T* __restrict__ out,
+                                                     int64_t n, const uint8_t* __restrict__ fixed, int mode,
+                                                     const T* __restrict__ b, const T* __restrict__ invD, T omega,
+                                                     const int* gate) {
+    if (gate && *gate) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (row >= n) return;
+    double s = 0;
+    for (int64_t j = lane; j < n; j += 64) s += (double)K[row * n + j] * (double)xin[j];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) {
+        T v = (T)s;
+        if (mode == 1) {
+            if (fixed[row]) v = T(0);
+        } else if (mode == 2) {
+            v = fixed[row] ? T(0) : b[row] - v;
+        } else if (mode == 3) {
+            const T r = b[row] - v;
+            v = fixed[row] ? xin[row] : xin[row] + omega * invD[row] * r;
+        }
+        out[row] = v;
+    }
+}
+
+__global__ void k_dense_invdiag(const double* __restrict__ K, int64_t n, const uint8_t* __restrict__ fixed,
+                                double* __restrict__ invD, float* __restrict__ invD32) {
+    for (int64_t i = gtid(); i < n; i += gstride()) {
+        const double d = K[i * n + i];
+        const double v = (!fixed[i] && d > 1e-30) ? 1.0 / d : 0.0;
+        invD[i] = v;
+        if (invD32) invD32[i] = (float)v;
+    }
+}
+
+// Galerkin P^T K_f P from fine elements (_assembleGalerkinFromElements
+// js/optimizer-worker-3d.js:628-662); one thread per (element, i, j),
+// FP64 atomics into the dense coarse matrix.
+__global__ void k_galerkin_elems(const double* __restrict__ Ef, double thr, Grid gf, Grid gc, double* __restrict__ Kc,
+                                 const double* __restrict__ ke) {
+    const int nyF = gf.nny(), nzF = gf.nnz();
+    const int nxC = gc.nnx(), nyC = gc.nny(), nzC = gc.nnz();
+    const int64_t ndofC = gc.ndof();
+    const int64_t total = gf.nel() * 576;
+    for (int64_t w = gtid(); w < total; w += gstride()) {
+        const int64_t e = w / 576;
+        const int ij = (int)(w % 576), i = ij / 24, j = ij % 24;
+        const double Ee = Ef[e];
+        if (!(Ee > thr)) continue;
+        const double kij = Ee * ke[i * 24 + j];
+        if (kij == 0) continue;
+        const int ez = (int)(e % gf.nelz);
+        const int64_t t = e / gf.nelz;
+        const int ey = (int)(t % gf.nely), ex = (int)(t / gf.nely);
+        int a, b, c;
+        local_offsets(i / 3, a, b, c);
+        const int fi[3] = {ex + a, ey + b, ez + c};
+        local_offsets(j / 3, a, b, c);
+        const int fj[3] = {ex + a, ey + b, ez + c};
+        const int di = i % 3, dj = j % 3;
+        int ci[3][2], cj[3][2];
+        double wi[3][2], wj[3][2];
+        const int nC[3] = {nxC, nyC, nzC};
+        for (int ax = 0; ax < 3; ax++) {
+            pw1d(fi[ax], nC[ax], ci[ax][0], ci[ax][1], wi[ax][0], wi[ax][1]);
+            pw1d(fj[ax], nC[ax], cj[ax][0], cj[ax][1], wj[ax][0], wj[ax][1]);
+        }
+        (void)nyF; (void)nzF;
+        for (int qa = 0; qa < 8; qa++) {
+            const double wa = wi[0][qa & 1] * wi[1][(qa >> 1) & 1] * wi[2][qa >> 2];
+            if (!(wa > 0)) continue;
+            const int64_t cr = 3 * (((int64_t)ci[0][qa & 1] * nyC + ci[1][(qa >> 1) & 1]) * nzC + ci[2][qa >> 2]) + di;
+            for (int qb = 0; qb < 8; qb++) {
+                const double wb = wj[0][qb & 1] * wj[1][(qb >> 1) & 1] * wj[2][qb >> 2];
+                if (!(wb > 0)) continue;
+                const int64_t cc = 3 * (((int64_t)cj[0][qb & 1] * nyC + cj[1][(qb >> 1) & 1]) * nzC + cj[2][qb >> 2]) + dj;
+                atomicAdd(&Kc[cr * ndofC + cc], wa * kij * wb);
+            }
+        }
+    }
+}
+
+// T = K_f P  (nF x nC), then Kc = P^T T (_assembleGalerkinFromMatrix :665-699)
+__device__ __forceinline__ int prolong_entries(int64_t fdof, Grid gf, Grid gc, int64_t* cdof, double* w) {
+    const int nyF = gf.nny(), nzF = gf.nnz();
+    const int64_t nfn = fdof / 3;
+    const int d = (int)(fdof % 3);
+    const int fz = (int)(nfn % nzF);
+    const int64_t t = nfn / nzF;
+    const int fy = (int)(t % nyF), fx = (int)(t / nyF);
+    int cx[2], cy[2], cz[2];
+    double wx[2], wy[2], wz[2];
+    pw1d(fx, gc.nnx(), cx[0], cx[1], wx[0], wx[1]);
+    pw1d(fy, gc.nny(), cy[0], cy[1], wy[0], wy[1]);
+    pw1d(fz, gc.nnz(), cz[0], cz[1], wz[0], wz[1]);
+    int cnt = 0;
+    for (int q = 0; q < 8; q++) {
+        const double ww = wx[q & 1] * wy[(q >> 1) & 1] * wz[q >> 2];
+        if (ww > 0) {
+            cdof[cnt] = 3 * (((int64_t)cx[q & 1] * gc.nny() + cy[(q >> 1) & 1]) * gc.nnz() + cz[q >> 2]) + d;
+            w[cnt] = ww;
+            cnt++;
+        }
+    }
+    return cnt;
+}
+// T (nF x nC) zeroed first; T[i][c] += K_f[i][k] * P[k][c]
+__global__ void k_galerkin_KP(const double* __restrict__ Kf, int64_t nF, Grid gf, Grid gc, double* __restrict__ T) {
+    const int64_t nC = gc.ndof();
+    for (int64_t w = gtid(); w < nF * nF; w += gstride()) {
+        const int64_t i = w / nF, k = w % nF;
+        const double kik = Kf[i * nF + k];
+        if (kik == 0) continue;
+        int64_t cd[8];
+        double ww[8];
+        const int cnt = prolong_entries(k, gf, gc, cd, ww);
+        for (int q = 0; q < cnt; q++) atomicAdd(&T[i * nC + cd[q]], kik * ww[q]);
+    }
+}
+// Kc[c][j] += P[k][c] * T[k][j]
+__global__ void k_galerkin_PtT(const double* __restrict__ T, int64_t nF, Grid gf, Grid gc, double* __restrict__ Kc) {
+    const int64_t nC = gc.ndof();
+    for (int64_t w = gtid(); w < nF * nC; w += gstride()) {
+        const int64_t k = w / nC, j = w % nC;
+        const double tkj = T[k * nC + j];
+        if (tkj == 0) continue;
+        int64_t cd[8];
+        double ww[8];
+        const int cnt = prolong_entries(k, gf, gc, cd, ww);
+        for (int q = 0; q < cnt; q++) atomicAdd(&Kc[cd[q] * nC + j], ww[q] * tkj);
+    }
+}
+// dense K of an EbE level (for the coarsest Cholesky): K[r][c] += E K0
+__global__ void k_assemble_dense(const double* __restrict__ E, double thr, Grid g, double* __restrict__ K,
+                                 const double* __restrict__ ke) {
+    const int nny = g.nny(), nnz = g.nnz();
+    const int64_t ndof = g.ndof();
+    for (int64_t w = gtid(); w < g.nel() * 576; w += gstride()) {
+        const int64_t e = w / 576;
+        const int ij = (int)(w % 576), i = ij / 24, j = ij % 24;
+        const double Ee = E[e];
+        if (!(Ee > thr)) continue;
+        const int ez = (int)(e % g.nelz);
+        const int64_t t = e / g.nelz;
+        const int ey = (int)(t % g.nely), ex = (int)(t / g.nely);
+        int a, b, c;
+        local_offsets(i / 3, a, b, c);
+        const int64_t ni = ((int64_t)(ex + a) * nny + (ey + b)) * nnz + (ez + c);
+        local_offsets(j / 3, a, b, c);
+        const int64_t nj = ((int64_t)(ex + a) * nny + (ey + b)) * nnz + (ez + c);
+        atomicAdd(&K[(3 * ni + i % 3) * ndof + 3 * nj + j % 3], Ee * ke[i * 24 + j]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// SIMP element kernels (zyx element order)
+
+// Heaviside projection _heavisideProject js/optimizer-worker-3d.js:3500-3509
+__global__ void k_heaviside(const float* __restrict__ x, float* __restrict__ xp, int64_t n, double beta, double eta) {
+#pragma clang fp contract(off)
+    const double tb = tanh(beta * eta), t1 = tanh(beta * (1 - eta)), den = tb + t1;
+    for (int64_t i = gtid(); i < n; i += gstride()) xp[i] = (float)((tb + tanh(beta * ((double)x[i] - eta))) / den);
+}
+
+// energies w_e = u_e^T K0 u_e (f64 sum, Float32 store; :2163-2173 with
+// _computeElementEnergyFlat :1561-1571), stress = E_e w_e, and the
+// compliance sensitivity dc (:2175-2192):
+//   dc = -p xPhys^(p-1) E0 w * (proj ? beta (1 - tanh^2(beta(xd - eta))) / (2 tanh(beta/2)) : 1)
+// where xd is the projected field's argument (x, or the density-filtered x).
+__global__ void k_energy_sens(const double* __restrict__ U, const float* __restrict__ xphys, const float* __restrict__ xd,
+                              double penal, double E0, double Emin, int use_proj, double beta, double eta, Grid g,
+                              float* __restrict__ w, float* __restrict__ stress, float* __restrict__ dc,
+                              const double* __restrict__ ke) {
+#pragma clang fp contract(off)
+    const int nny = g.nny(), nnz = g.nnz();
+    const int64_t nyz = (int64_t)nny * nnz;
+    const double denom = 2 * tanh(beta * 0.5);
+    for (int64_t e = gtid(); e < g.nel(); e += gstride()) {
+        const int ez = (int)(e % g.nelz);
+        const int64_t t = e / g.nelz;
+        const int ey = (int)(t % g.nely), ex = (int)(t / g.nely);
+        const int64_t n0 = ((int64_t)ex * nny + ey) * nnz + ez;
+        double u[24];
+        for (int m = 0; m < 8; m++) {
+            int a, b, c;
+            local_offsets(m, a, b, c);
+            const int64_t nm = n0 + a * nyz + b * nnz + c;
+            u[3 * m] = U[3 * nm];
+            u[3 * m + 1] = U[3 * nm + 1];
+            u[3 * m + 2] = U[3 * nm + 2];
+        }
+        double en = 0;
+        for (int i = 0; i < 24; i++)
+            for (int j = 0; j < 24; j++) en += u[i] * ke[i * 24 + j] * u[j];
+        const float wf = (float)en;
+        w[e] = wf;
+        const double xp = (double)xphys[e];
+        const double stiff = Emin + pow(xp, penal) * (E0 - Emin);
+        if (stress) stress[e] = (float)(stiff * (double)wf);
+        const double dphys = -penal * pow(xp, penal - 1) * E0 * (double)wf;
+        double dd = 1.0;
+        if (use_proj) {
+            const double th = tanh(beta * ((double)xd[e] - eta));
+            dd = beta * (1 - th * th) / denom;
+        }
+        dc[e] = (float)(dphys * dd);
+    }
+}
+
+// Sensitivity filter (filterSensitivities js/optimizer-worker-3d.js:3052-3064
+// over the prepareFilter neighbourhood :3006-3050): for each element the
+// neighbours are visited x-major, then y, then z, accumulating in Float32
+// with FP64 products; Hs is precomputed the same way.
+struct FilterStencil {
+    int r;            // window half width
+    int count;        // number of (dx,dy,dz) with dist <= rmin
+    int off[125][3];
+    double w[125];    // rmin - dist
+};
+
+__global__ void k_filter_hs(float* __restrict__ Hs, Grid g, const FilterStencil* __restrict__ st) {
+#pragma clang fp contract(off)
+    for (int64_t e = gtid(); e < g.nel(); e += gstride()) {
+        const int ez = (int)(e % g.nelz);
+        const int64_t t = e / g.nelz;
+        const int ey = (int)(t % g.nely), ex = (int)(t / g.nely);
+        float hs = 0.f;
+        for (int q = 0; q < st->count; q++) {
+            const int kx = ex + st->off[q][0], ky = ey + st->off[q][1], kz = ez + st->off[q][2];
+            if (kx < 0 || kx >= g.nelx || ky < 0 || ky >= g.nely || kz < 0 || kz >= g.nelz) continue;
+            hs = (float)((double)hs + st->w[q]);
+        }
+        Hs[e] = hs;
+    }
+}
+
+__global__ void k_sens_filter(const float* __restrict__ x, const float* __restrict__ dc, const float* __restrict__ Hs,
+                              float* __restrict__ dcn, Grid g, const FilterStencil* __restrict__ st) {
+#pragma clang fp contract(off)
+    for (int64_t e = gtid(); e < g.nel(); e += gstride()) {
+        const int ez = (int)(e % g.nelz);
+        const int64_t t = e / g.nelz;
+        const int ey = (int)(t % g.nely), ex = (int)(t / g.nely);
+        float acc = 0.f;
+        for (int q = 0; q < st->count; q++) {
+            const int kx = ex + st->off[q][0], ky = ey + st->off[q][1], kz = ez + st->off[q][2];
+            if (kx < 0 || kx >= g.nelx || ky < 0 || ky >= g.nely || kz < 0 || kz >= g.nelz) continue;
+            const int64_t j = ((int64_t)kx * g.nely + ky) * g.nelz + kz;
+            acc = (float)((double)acc + st->w[q] * (double)x[j] * (double)dc[j]);
+        }
+        const double xe = (double)x[e];
+        dcn[e] = (float)((double)acc / ((double)Hs[e] * (xe > 1e-3 ? xe : 1e-3)));
+    }
+}
+
+// Density filter (applyDensityFilter assembly/index.ts:224-273: window
+// ceil(r), weight r - dist, normalised; neighbours z-major). FP64 sums.
+// sw[e] = sum of weights. transpose=1: out[j] = sum_i w_ij in[i] / sw[i].
+__global__ void k_density_sw(double* __restrict__ sw, Grid g, const FilterStencil* __restrict__ st) {
+#pragma clang fp contract(off)
+    for (int64_t e = gtid(); e < g.nel(); e += gstride()) {
+        const int ez = (int)(e % g.nelz);
+        const int64_t t = e / g.nelz;
+        const int ey = (int)(t % g.nely), ex = (int)(t / g.nely);
+        double s = 0;
+        for (int q = 0; q < st->count; q++) {
+            const int kx = ex + st->off[q][0], ky = ey + st->off[q][1], kz = ez + st->off[q][2];
+            if (kx < 0 || kx >= g.nelx || ky < 0 || ky >= g.nely || kz < 0 || kz >= g.nelz) continue;
+            s += st->w[q];
+        }
+        sw[e] = s;
+    }
+}
+__global__ void k_density_filter(const float* __restrict__ in, float* __restrict__ out, const double* __restrict__ sw,
+                                 Grid g, int transpose, const FilterStencil* __restrict__ st) {
+#pragma clang fp contract(off)
+    for (int64_t e = gtid(); e < g.nel(); e += gstride()) {
+        const int ez = (int)(e % g.nelz);
+        const int64_t t = e / g.nelz;
+        const int ey = (int)(t % g.nely), ex = (int)(t / g.nely);
+        double s = 0;
+        for (int q = 0; q < st->count; q++) {
+            const int kx = ex + st->off[q][0], ky = ey + st->off[q][1], kz = ez + st->off[q][2];
+            if (kx < 0 || kx >= g.nelx || ky < 0 || ky >= g.nely || kz < 0 || kz >= g.nelz) continue;
+            const int64_t j = ((int64_t)kx * g.nely + ky) * g.nelz + kz;
+            s += transpose ? st->w[q] * (double)in[j] / sw[j] : st->w[q] * (double)in[j];
+        }
+        out[e] = transpose ? (float)s : (float)(s / sw[e]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// OC bisection (OC js/optimizer-worker-3d.js:3066-3101) evaluated for a
+// whole subtree of bisection steps per pass: the K = 2^depth - 1 midpoints
+// of the next `depth` steps are evaluated in one sweep over the elements,
+// then the path through the tree is walked on the device. The sequence of
+// lmid values is exactly the reference's.
+constexpr int kOcDepth = 5;
+constexpr int kOcCand = (1 << kOcDepth) - 1;
+
+struct OCState {
+    double l1, l2;
+    double lam[kOcCand];   // candidate midpoints (tree in BFS order)
+    int nlam;
+    int passes;            // bisection steps taken
+    int finished;
+    double target;         // volfrac * nel
+    double vol_last;
+    double lmid_last;      // midpoint of the last executed step
+};
+
+__device__ __forceinline__ float oc_xnew(float xf, float dcf, double lmid, double move, uint8_t pres, uint8_t vd) {
+#pragma clang fp contract(off)
+    if (pres) return 1.0f;
+    if (vd) return 0.0f;
+    const double Be = -(double)dcf / lmid;
+    const double xi = (double)xf;
+    return (float)js_max(0.0, js_max(xi - move, js_min(1.0, js_min(xi + move, xi * sqrt(Be)))));
+}
+
+// fill the candidate tree for the next steps from [l1, l2]
+__global__ void k_oc_plan(OCState* st) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (st->finished) { st->nlam = 0; return; }
+    // node k (BFS, root 0) covers interval [lo, hi]; its midpoint is lam[k]
+    double lo[kOcCand], hi[kOcCand];
+    lo[0] = st->l1;
+    hi[0] = st->l2;
+    for (int k = 0; k < kOcCand; k++) {
+        const double m = 0.5 * (hi[k] + lo[k]);
+        st->lam[k] = m;
+        const int lc = 2 * k + 1, rc = 2 * k + 2;
+        if (lc < kOcCand) { lo[lc] = m; hi[lc] = hi[k]; }      // sum > target: l1 = lmid
+        if (rc < kOcCand) { lo[rc] = lo[k]; hi[rc] = m; }      // else l2 = lmid
+    }
+    st->nlam = kOcCand;
+}
+
+__global__ void __launch_bounds__(kRedThreads) k_oc_eval(const float* __restrict__ x, const float* __restrict__ dc,
+                                                        const uint8_t* __restrict__ pres, const uint8_t* __restrict__ vd,
+                                                        double move, int64_t n, OCState* st, double* partials,
+                                                        unsigned* ticket) {
+    if (st->finished) return;
+    double v[kOcCand];
+    double lam[kOcCand];
+#pragma unroll
+    for (int k = 0; k < kOcCand; k++) { v[k] = 0.0; lam[k] = st->lam[k]; }
+    for (int64_t i = gtid(); i < n; i += gstride()) {
+        const float xf = x[i], dcf = dc[i];
+        const uint8_t pr = pres ? pres[i] : 0, vv = vd ? vd[i] : 0;
+#pragma unroll
+        for (int k = 0; k < kOcCand; k++) v[k] += (double)oc_xnew(xf, dcf, lam[k], move, pr, vv);
+    }
+    double tot[kOcCand];
+    if (grid_reduce<kOcCand>(v, partials, ticket, tot)) {
+        // walk the tree with the reference's loop condition
+        int k = 0;
+        while (k < kOcCand) {
+            if (!((st->l2 - st->l1) / (st->l2 + st->l1) > 1e-3)) { st->finished = 1; break; }
+            st->vol_last = tot[k];
+            st->lmid_last = st->lam[k];
+            if (tot[k] > st->target) { st->l1 = st->lam[k]; k = 2 * k + 1; }
+            else { st->l2 = st->lam[k]; k = 2 * k + 2; }
+            st->passes++;
+        }
+        if (!((st->l2 - st->l1) / (st->l2 + st->l1) > 1e-3)) st->finished = 1;
+    }
+}
+
+// final xnew with the last lmid of the reference loop (the xnew left in the
+// array by the last executed pass)
+__global__ void k_oc_final(const float* __restrict__ x, const float* __restrict__ dc, const uint8_t* __restrict__ pres,
+                           const uint8_t* __restrict__ vd, double move, int64_t n, const double* lmid_last,
+                           float* __restrict__ xnew) {
+    const double lm = *lmid_last;
+    for (int64_t i = gtid(); i < n; i += gstride())
+        xnew[i] = oc_xnew(x[i], dc[i], lm, move, pres ? pres[i] : 0, vd ? vd[i] : 0);
+}
+
+// change = max |xnew - xold|, volume = sum xnew
+__global__ void __launch_bounds__(kRedThreads) k_change(const float* __restrict__ a, const float* __restrict__ b, int64_t n,
+                                                       double* out2, double* partials, unsigned* ticket) {
+    __shared__ double sh[kRedThreads / 64];
+    __shared__ bool last;
+    double m = 0.0, s = 0.0;
+    for (int64_t i = gtid(); i < n; i += gstride()) {
+        const float bv = b ? b[i] : 0.0f;
+        m = fmax(m, (double)fabsf(a[i] - bv));
+        s += (double)a[i];
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+    double vs[1] = {s};
+    double tot[1];
+    // max via a second small reduction in shared memory
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) sh[wid] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mm = 0;
+        for (int w = 0; w < kRedThreads / 64; w++) mm = fmax(mm, sh[w]);
+        partials[2 * kRedBlocks + blockIdx.x] = mm;
+    }
+    __syncthreads();
+    (void)last;
+    if (grid_reduce<1>(vs, partials, ticket, tot)) {
+        double mm = 0;
+        for (unsigned bI = 0; bI < gridDim.x; bI++) mm = fmax(mm, partials[2 * kRedBlocks + bI]);
+        out2[0] = mm;
+        out2[1] = tot[0];
+    }
+}
+
+}  // namespace topo

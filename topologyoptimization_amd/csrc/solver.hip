@@ -1,0 +1,1612 @@
+// solver.hip -- device-resident SIMP solver of libtopo_hip and the C ABI of
+// include/topo.h (except the host-only setup functions in host_setup.cpp).
+//
+// Structure
+//   Ctx        stream, reduction scratch, device CG/OC state, allocations
+//   MGH        geometric multigrid hierarchy (level 0 = the fine grid);
+//              used by Jacobi-PCG too (level 0's diagonal)
+//   Solver     the topo_solver handle: design fields, U, CG vectors, SIMP step
+// Precision: T = double (FP64 path) or float (inner solve of the mixed path);
+// CG scalars are always FP64 and live on the device.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <type_traits>
+#include <vector>
+#include "kernels.hpp"
+
+namespace topo {
+
+static inline int nblk(int64_t n, int bs = 256) {
+    int64_t b = (n + bs - 1) / bs;
+    if (b < 1) b = 1;
+    if (b > (1 << 20)) b = 1 << 20;
+    return (int)b;
+}
+
+
+// ---------------------------------------------------------------------------
+struct Ctx {
+    hipStream_t s = nullptr;
+    int device = 0;
+    double* partials = nullptr;   // 32 * kRedBlocks
+    unsigned* ticket = nullptr;
+    CGState* st = nullptr;        // device CG state
+    CGState* h_st = nullptr;      // pinned mirror
+    OCState* oc = nullptr;
+    OCState* h_oc = nullptr;
+    double* dsc = nullptr;        // 16 device doubles of scratch
+    double* h_sc = nullptr;       // pinned mirror
+    std::vector<void*> allocs;
+
+    Ctx() {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+            set_error("no HIP device available");
+            throw TopoError{TOPO_E_NODEVICE};
+        }
+        TOPO_HIP(hipGetDevice(&device));
+        TOPO_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        partials = alloc<double>(32 * kRedBlocks);
+        ticket = alloc<unsigned>(4);
+        st = alloc<CGState>(1);
+        oc = alloc<OCState>(1);
+        dsc = alloc<double>(16);
+        TOPO_HIP(hipHostMalloc((void**)&h_st, sizeof(CGState), hipHostMallocDefault));
+        TOPO_HIP(hipHostMalloc((void**)&h_oc, sizeof(OCState), hipHostMallocDefault));
+        TOPO_HIP(hipHostMalloc((void**)&h_sc, 16 * sizeof(double), hipHostMallocDefault));
+        TOPO_HIP(hipMemsetAsync(ticket, 0, 4 * sizeof(unsigned), s));
+        TOPO_HIP(hipMemsetAsync(st, 0, sizeof(CGState), s));
+        TOPO_HIP(hipStreamSynchronize(s));
+    }
+    ~Ctx() {
+        if (s) (void)hipStreamSynchronize(s);
+        for (void* p : allocs) (void)hipFree(p);
+        if (h_st) (void)hipHostFree(h_st);
+        if (h_oc) (void)hipHostFree(h_oc);
+        if (h_sc) (void)hipHostFree(h_sc);
+        if (s) (void)hipStreamDestroy(s);
+    }
+    template <typename T>
+    T* alloc(int64_t n) {
+        void* p = nullptr;
+        if (n <= 0) n = 1;
+        TOPO_HIP(hipMalloc(&p, (size_t)n * sizeof(T)));
+        allocs.push_back(p);
+        return (T*)p;
+    }
+    void free_one(void* p) {
+        if (!p) return;
+        auto it = std::find(allocs.begin(), allocs.end(), p);
+        if (it != allocs.end()) {
+            (void)hipFree(p);
+            allocs.erase(it);
+        }
+    }
+    void sync() { TOPO_HIP(hipStreamSynchronize(s)); }
+    void check_launch() { TOPO_HIP(hipGetLastError()); }
+    // read the device CG state
+    const CGState& read_st() {
+        TOPO_HIP(hipMemcpyAsync(h_st, st, sizeof(CGState), hipMemcpyDeviceToHost, s));
+        sync();
+        return *h_st;
+    }
+    double read_d(const double* dptr) {
+        TOPO_HIP(hipMemcpyAsync(h_sc, dptr, sizeof(double), hipMemcpyDeviceToHost, s));
+        sync();
+        return h_sc[0];
+    }
+};
+
+// K0 in device memory (FP64 + FP32 copies)
+struct DevKe {
+    double* ke = nullptr;
+    float* ke32 = nullptr;
+    void set(Ctx& c, const double* hke) {
+        if (!ke) {
+            ke = c.alloc<double>(576);
+            ke32 = c.alloc<float>(576);
+        }
+        float k32[576];
+        for (int i = 0; i < 576; i++) k32[i] = (float)hke[i];
+        TOPO_HIP(hipMemcpyAsync(ke, hke, 576 * sizeof(double), hipMemcpyHostToDevice, c.s));
+        TOPO_HIP(hipMemcpyAsync(ke32, k32, 576 * sizeof(float), hipMemcpyHostToDevice, c.s));
+        c.sync();
+    }
+};
+
+// ---------------------------------------------------------------------------
+struct MGLevel {
+    Grid g;
+    int64_t ndof = 0;
+    double* E = nullptr;
+    float* E32 = nullptr;
+    bool ownE = true;
+    uint8_t* fixed = nullptr;
+    double* invD = nullptr;
+    float* invD32 = nullptr;
+    bool dense = false;
+    double* K = nullptr;
+    float* K32 = nullptr;
+    double *b = nullptr, *x = nullptr, *x2 = nullptr, *res = nullptr;
+    float *b32 = nullptr, *x32 = nullptr, *x232 = nullptr, *res32 = nullptr;
+    // direct coarse solve: explicit inverse of the free block
+    bool chol = false;
+    int64_t nf = 0;
+    int32_t* freeIdx = nullptr;
+    double* Kinv = nullptr;
+    float* Kinv32 = nullptr;
+    double* work = nullptr;
+};
+
+template <typename T> struct Sel;
+template <> struct Sel<double> {
+    static double* b(MGLevel& l) { return l.b; }
+    static double* x(MGLevel& l) { return l.x; }
+    static double* x2(MGLevel& l) { return l.x2; }
+    static double* res(MGLevel& l) { return l.res; }
+    static double* invD(MGLevel& l) { return l.invD; }
+    static double* K(MGLevel& l) { return l.K; }
+    static double* Kinv(MGLevel& l) { return l.Kinv; }
+    static const double* E(MGLevel& l) { return l.E; }
+    typedef double TE;
+};
+template <> struct Sel<float> {
+    static float* b(MGLevel& l) { return l.b32; }
+    static float* x(MGLevel& l) { return l.x32; }
+    static float* x2(MGLevel& l) { return l.x232; }
+    static float* res(MGLevel& l) { return l.res32; }
+    static float* invD(MGLevel& l) { return l.invD32; }
+    static float* K(MGLevel& l) { return l.K32; }
+    static float* Kinv(MGLevel& l) { return l.Kinv32; }
+    static const float* E(MGLevel& l) { return l.E32; }
+    typedef float TE;
+};
+
+// x = omega * invD * b (first sweep from x = 0), gated
+template <typename T>
+__global__ void k_jacobi_first(T* __restrict__ x, const T* __restrict__ b, const T* __restrict__ invD, T omega,
+                               int64_t n, const int* gate) {
+    if (gate && *gate) return;
+    for (int64_t i = gtid(); i < n; i += gstride()) x[i] = T(0) + omega * invD[i] * b[i];
+}
+// coarse direct solve with the explicit inverse of the free block:
+// x[free[i]] = sum_j Kinv[i][j] b[free[j]], x = 0 on fixed DOFs
+template <typename T>
+__global__ void __launch_bounds__(256) k_inv_apply(const T* __restrict__ Kinv, const int32_t* __restrict__ freeIdx,
+                                                   int64_t nf, const T* __restrict__ b, T* __restrict__ x,
+                                                   const int* gate) {
+    if (gate && *gate) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (row >= nf) return;
+    double s = 0;
+    for (int64_t j = lane; j < nf; j += 64) s += (double)Kinv[row * nf + j] * (double)b[freeIdx[j]];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) x[freeIdx[row]] = (T)s;
+}
+template <typename T>
+__global__ void k_zero_fixed_or_all(T* __restrict__ x, int64_t n, const int* gate) {
+    if (gate && *gate) return;
+    for (int64_t i = gtid(); i < n; i += gstride()) x[i] = T(0);
+}
+
+// dense SPD inverse by Gauss-Jordan without pivoting (SPD): one step k.
+// A is nf x nf in place; step k uses the pivot row k.
+__global__ void k_gj_scale_row(double* A, int64_t n, int64_t k) {
+    // pivot row scaled by 1/akk, column k of the row becomes 1/akk
+    const double akk = A[k * n + k];
+    const double inv = 1.0 / akk;
+    for (int64_t j = gtid(); j < n; j += gstride()) {
+        if (j == k) continue;
+        A[k * n + j] *= inv;
+    }
+}
+__global__ void k_gj_eliminate(double* A, int64_t n, int64_t k) {
+    // rows i != k: A[i][j] -= A[i][k] * A[k][j] for j != k; A[i][k] = -A[i][k]/akk
+    const double akk = A[k * n + k];
+    for (int64_t w = gtid(); w < n * n; w += gstride()) {
+        const int64_t i = w / n, j = w % n;
+        if (i == k || j == k) continue;
+        A[i * n + j] -= A[i * n + k] * A[k * n + j];
+    }
+    (void)akk;
+}
+__global__ void k_gj_fix_col(double* A, int64_t n, int64_t k) {
+    const double akk = A[k * n + k];
+    for (int64_t i = gtid(); i < n; i += gstride()) {
+        if (i == k) continue;
+        A[i * n + k] = -A[i * n + k] / akk;
+    }
+}
+__global__ void k_gj_pivot(double* A, int64_t n, int64_t k) {
+    if (gtid() == 0) A[k * n + k] = 1.0 / A[k * n + k];
+}
+__global__ void k_extract_free(const double* __restrict__ K, int64_t n, const int32_t* __restrict__ freeIdx, int64_t nf,
+                               double* __restrict__ A) {
+    for (int64_t w = gtid(); w < nf * nf; w += gstride()) {
+        const int64_t i = w / nf, j = w % nf;
+        double v = K[(int64_t)freeIdx[i] * n + freeIdx[j]];
+        A[w] = v;
+    }
+}
+// rows/cols of free DOFs without stiffness get a unit diagonal
+__global__ void k_regularize(double* A, int64_t nf) {
+    for (int64_t i = gtid(); i < nf; i += gstride())
+        if (!(A[i * nf + i] > 1e-30)) {
+            for (int64_t j = 0; j < nf; j++) { A[i * nf + j] = 0; A[j * nf + i] = 0; }
+            A[i * nf + i] = 1.0;
+        }
+}
+
+struct MGH {
+    std::vector<MGLevel> L;
+    double omega = 0.5;
+    int nu1 = 2, nu2 = 2, coarse_sweeps = 30, coarse_solver = TOPO_COARSE_SWEEPS;
+    int galerkin_max = 3000;
+    double omega_cap = 0.0;
+    std::vector<double> omegaL;
+    double thr = 1e-6;   // skip threshold 1000 * Emin
+    bool want32 = false;
+    DevKe* K0 = nullptr;
+    template <typename T> const T* ke() const;
+
+    // build levels: factor-2 coarsening of MGPrecond3D.ensure
+    // (js/optimizer-worker-3d.js:408-442); nlevels <= 0 = reference rule
+    void build(Ctx& c, Grid g0, const uint8_t* dfixed0, int nlevels, bool fp32) {
+        want32 = fp32;
+        L.clear();
+        MGLevel l0;
+        l0.g = g0;
+        l0.ndof = g0.ndof();
+        l0.fixed = const_cast<uint8_t*>(dfixed0);
+        L.push_back(l0);
+        const int maxL = nlevels > 0 ? nlevels : 6;
+        int cx = g0.nelx, cy = g0.nely, cz = g0.nelz;
+        for (int li = 1; li < maxL; li++) {
+            const int nx = cx / 2, ny = cy / 2, nz = cz / 2;
+            if (nx < 2 || ny < 2 || nz < 2) break;
+            MGLevel l;
+            l.g.nelx = nx; l.g.nely = ny; l.g.nelz = nz;
+            l.ndof = l.g.ndof();
+            L.push_back(l);
+            cx = nx; cy = ny; cz = nz;
+        }
+        // fixed masks by injection (_downsampleFixedMaskBy2 :384-406), done on
+        // the host from the fine mask
+        std::vector<uint8_t> hf(L[0].ndof);
+        TOPO_HIP(hipMemcpy(hf.data(), dfixed0, L[0].ndof, hipMemcpyDeviceToHost));
+        for (size_t li = 1; li < L.size(); li++) {
+            const Grid& gf = L[li - 1].g;
+            const Grid& gc = L[li].g;
+            std::vector<uint8_t> hc(L[li].ndof, 0);
+            for (int ccz = 0; ccz < gc.nnz(); ccz++) {
+                const int fz = std::min(ccz * 2, gf.nnz() - 1);
+                for (int ccy = 0; ccy < gc.nny(); ccy++) {
+                    const int fy = std::min(ccy * 2, gf.nny() - 1);
+                    for (int ccx = 0; ccx < gc.nnx(); ccx++) {
+                        const int fx = std::min(ccx * 2, gf.nnx() - 1);
+                        const int64_t nF = ((int64_t)fx * gf.nny() + fy) * gf.nnz() + fz;
+                        const int64_t nC = ((int64_t)ccx * gc.nny() + ccy) * gc.nnz() + ccz;
+                        for (int d = 0; d < 3; d++) hc[3 * nC + d] = hf[3 * nF + d];
+                    }
+                }
+            }
+            L[li].fixed = c.alloc<uint8_t>(L[li].ndof);
+            TOPO_HIP(hipMemcpy(L[li].fixed, hc.data(), L[li].ndof, hipMemcpyHostToDevice));
+            hf.swap(hc);
+        }
+        for (size_t li = 0; li < L.size(); li++) {
+            MGLevel& l = L[li];
+            if (li > 0) {
+                l.E = c.alloc<double>(l.g.nel());
+                if (fp32) l.E32 = c.alloc<float>(l.g.nel());
+            }
+            l.invD = c.alloc<double>(l.ndof);
+            l.b = c.alloc<double>(l.ndof);
+            l.x = c.alloc<double>(l.ndof);
+            l.x2 = c.alloc<double>(l.ndof);
+            l.res = c.alloc<double>(l.ndof);
+            if (fp32) {
+                l.invD32 = c.alloc<float>(l.ndof);
+                l.b32 = c.alloc<float>(l.ndof);
+                l.x32 = c.alloc<float>(l.ndof);
+                l.x232 = c.alloc<float>(l.ndof);
+                l.res32 = c.alloc<float>(l.ndof);
+            }
+        }
+    }
+
+    // per-density operator update (updateFromFine :558-582); level 0's E is
+    // set by the caller.
+    void update(Ctx& c) {
+        MGLevel& l0 = L[0];
+        hipLaunchKernelGGL(k_invdiag, dim3(nblk(l0.g.nnodes())), dim3(256), 0, c.s, l0.E, thr, l0.fixed, l0.invD,
+                           l0.invD32, l0.g, (const double*)K0->ke);
+        c.check_launch();
+        for (size_t li = 1; li < L.size(); li++) {
+            MGLevel& f = L[li - 1];
+            MGLevel& l = L[li];
+            if (l.ndof <= galerkin_max) {
+                if (!l.K) {
+                    l.K = c.alloc<double>(l.ndof * l.ndof);
+                    if (want32) l.K32 = c.alloc<float>(l.ndof * l.ndof);
+                }
+                l.dense = true;
+                hipLaunchKernelGGL(k_fill<double>, dim3(nblk(l.ndof * l.ndof)), dim3(256), 0, c.s, l.K, 0.0,
+                                   l.ndof * l.ndof, (const int*)nullptr);
+                if (!f.dense) {
+                    hipLaunchKernelGGL(k_galerkin_elems, dim3(nblk(f.g.nel() * 576)), dim3(256), 0, c.s, f.E, thr, f.g,
+                                       l.g, l.K, (const double*)K0->ke);
+                } else {
+                    double* T = c.alloc<double>(f.ndof * l.ndof);
+                    hipLaunchKernelGGL(k_fill<double>, dim3(nblk(f.ndof * l.ndof)), dim3(256), 0, c.s, T, 0.0,
+                                       f.ndof * l.ndof, (const int*)nullptr);
+                    hipLaunchKernelGGL(k_galerkin_KP, dim3(nblk(f.ndof * f.ndof)), dim3(256), 0, c.s, f.K, f.ndof, f.g,
+                                       l.g, T);
+                    hipLaunchKernelGGL(k_galerkin_PtT, dim3(nblk(f.ndof * l.ndof)), dim3(256), 0, c.s, T, f.ndof, f.g,
+                                       l.g, l.K);
+                    c.sync();
+                    c.free_one(T);
+                }
+                hipLaunchKernelGGL(k_dense_invdiag, dim3(nblk(l.ndof)), dim3(256), 0, c.s, l.K, l.ndof, l.fixed, l.invD,
+                                   l.invD32);
+                if (want32)
+                    hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(l.ndof * l.ndof)), dim3(256), 0, c.s, l.K32,
+                                       l.K, l.ndof * l.ndof);
+            } else {
+                l.dense = false;
+                hipLaunchKernelGGL(k_coarsen_E, dim3(nblk(l.g.nel())), dim3(256), 0, c.s, f.E, l.E, l.E32, f.g, l.g);
+                hipLaunchKernelGGL(k_invdiag, dim3(nblk(l.g.nnodes())), dim3(256), 0, c.s, l.E, thr, l.fixed, l.invD,
+                                   l.invD32, l.g, (const double*)K0->ke);
+            }
+            c.check_launch();
+        }
+        omegaL.assign(L.size(), omega);
+        if (omega_cap > 0)
+            for (size_t li = 0; li < L.size(); li++) {
+                const double rho = power_rho(c, (int)li);
+                if (rho > 0) omegaL[li] = std::min(omega, omega_cap / rho);
+            }
+        if (coarse_solver == TOPO_COARSE_CHOLESKY) factor_coarsest(c);
+    }
+
+    // smoother safeguard: rho ~ lambda_max(D^-1 A) of level li by 20 power
+    // iterations from the hashed vector v_i = frac(i*2654435761/2^32) - 1/2 on
+    // free DOFs (the oracle's power_rho); FP64.
+    double power_rho(Ctx& c, int li) {
+        MGLevel& l = L[li];
+        hipLaunchKernelGGL(k_power_init, dim3(nblk(l.ndof)), dim3(256), 0, c.s, l.x, (const uint8_t*)l.fixed, l.ndof,
+                           c.dsc + 8);
+        for (int it = 0; it < 20; it++) {
+            apply<double>(c, li, l.x, l.x2, 0, nullptr, nullptr, 0.0);
+            hipLaunchKernelGGL(k_power_scale, dim3(kRedBlocks), dim3(kRedThreads), 0, c.s, l.x2,
+                               (const double*)l.invD, l.ndof, c.dsc + 8, c.partials, c.ticket);
+            hipLaunchKernelGGL(k_power_norm, dim3(nblk(l.ndof)), dim3(256), 0, c.s, l.x, (const double*)l.x2, l.ndof,
+                               (const double*)(c.dsc + 8));
+        }
+        c.check_launch();
+        // v is unit-norm after the first step, so rho = ||D^-1 A v||
+        return std::sqrt(c.read_d(c.dsc + 9));
+    }
+
+    // FP64 direct solve of the coarsest level: the free-DOF block is
+    // inverted once per density update (SPD Gauss-Jordan on the device); the
+    // V-cycle then applies the inverse with one GEMV.
+    void factor_coarsest(Ctx& c) {
+        MGLevel& l = L.back();
+        if (!l.freeIdx) {
+            std::vector<uint8_t> hf(l.ndof);
+            TOPO_HIP(hipMemcpy(hf.data(), l.fixed, l.ndof, hipMemcpyDeviceToHost));
+            std::vector<int32_t> fr;
+            for (int64_t i = 0; i < l.ndof; i++)
+                if (!hf[i]) fr.push_back((int32_t)i);
+            l.nf = (int64_t)fr.size();
+            TOPO_CHECK(l.nf > 0 && l.nf <= 20000, TOPO_E_UNSUPPORTED,
+                       "coarsest level has %lld free DOFs; the direct solve supports 1..20000", (long long)l.nf);
+            l.freeIdx = c.alloc<int32_t>(l.nf);
+            TOPO_HIP(hipMemcpy(l.freeIdx, fr.data(), l.nf * sizeof(int32_t), hipMemcpyHostToDevice));
+            l.Kinv = c.alloc<double>(l.nf * l.nf);
+            if (want32) l.Kinv32 = c.alloc<float>(l.nf * l.nf);
+        }
+        double* Kd = l.K;
+        double* tmp = nullptr;
+        if (!l.dense) {
+            tmp = c.alloc<double>(l.ndof * l.ndof);
+            hipLaunchKernelGGL(k_fill<double>, dim3(nblk(l.ndof * l.ndof)), dim3(256), 0, c.s, tmp, 0.0,
+                               l.ndof * l.ndof, (const int*)nullptr);
+            hipLaunchKernelGGL(k_assemble_dense, dim3(nblk(l.g.nel() * 576)), dim3(256), 0, c.s, l.E, thr, l.g, tmp,
+                               (const double*)K0->ke);
+            Kd = tmp;
+        }
+        const int64_t nf = l.nf;
+        hipLaunchKernelGGL(k_extract_free, dim3(nblk(nf * nf)), dim3(256), 0, c.s, Kd, l.ndof, l.freeIdx, nf, l.Kinv);
+        hipLaunchKernelGGL(k_regularize, dim3(nblk(nf)), dim3(256), 0, c.s, l.Kinv, nf);
+        for (int64_t k = 0; k < nf; k++) {
+            hipLaunchKernelGGL(k_gj_scale_row, dim3(nblk(nf)), dim3(256), 0, c.s, l.Kinv, nf, k);
+            hipLaunchKernelGGL(k_gj_eliminate, dim3(nblk(nf * nf)), dim3(256), 0, c.s, l.Kinv, nf, k);
+            hipLaunchKernelGGL(k_gj_fix_col, dim3(nblk(nf)), dim3(256), 0, c.s, l.Kinv, nf, k);
+            hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(64), 0, c.s, l.Kinv, nf, k);
+        }
+        c.check_launch();
+        if (want32)
+            hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(nf * nf)), dim3(256), 0, c.s, l.Kinv32, l.Kinv, nf * nf);
+        if (tmp) {
+            c.sync();
+            c.free_one(tmp);
+        }
+        l.chol = true;
+    }
+
+    // out = A in (mode 0), masked (1), b - A in (2), Jacobi sweep (3)
+    template <typename T>
+    void apply(Ctx& c, int li, const T* in, T* out, int mode, const T* b, const int* gate, double om) {
+        MGLevel& l = L[li];
+        if (l.dense) {
+            hipLaunchKernelGGL(k_dense_apply<T>, dim3(nblk(l.ndof * 64)), dim3(256), 0, c.s, Sel<T>::K(l), in, out, l.ndof,
+                               l.fixed, mode, b, Sel<T>::invD(l), (T)om, gate);
+        } else {
+            typedef typename Sel<T>::TE TE;
+            hipLaunchKernelGGL((k_applyA_exact<T, TE>), dim3(nblk(l.g.nnodes())), dim3(256), 0, c.s, in, out,
+                               (const TE*)Sel<T>::E(l), thr, l.g, l.fixed, mode, b, Sel<T>::invD(l), (T)om, gate,
+                               ke<T>());
+        }
+    }
+
+    // one V-cycle (mgVcycle assembly/index.ts:837-1048) for A x = b from
+    // x = 0 on level li; returns the buffer holding x.
+    template <typename T>
+    T* vcycle(Ctx& c, int li, const T* b, const int* gate) {
+        MGLevel& l = L[li];
+        T* x = Sel<T>::x(l);
+        T* x2 = Sel<T>::x2(l);
+        const bool coarsest = li == (int)L.size() - 1;
+        if (coarsest && l.chol) {
+            hipLaunchKernelGGL(k_zero_fixed_or_all<T>, dim3(nblk(l.ndof)), dim3(256), 0, c.s, x, l.ndof, gate);
+            hipLaunchKernelGGL(k_inv_apply<T>, dim3(nblk(l.nf * 64)), dim3(256), 0, c.s, (const T*)Sel<T>::Kinv(l),
+                               l.freeIdx, l.nf, b, x, gate);
+            return x;
+        }
+        const int nsweep_pre = coarsest ? nu1 + coarse_sweeps : nu1;
+        const double om = omegaL.empty() ? omega : omegaL[li];
+        if (nsweep_pre > 0) {
+            hipLaunchKernelGGL(k_jacobi_first<T>, dim3(nblk(l.ndof)), dim3(256), 0, c.s, x, b, (const T*)Sel<T>::invD(l),
+                               (T)om, l.ndof, gate);
+        } else {
+            hipLaunchKernelGGL(k_fill<T>, dim3(nblk(l.ndof)), dim3(256), 0, c.s, x, T(0), l.ndof, gate);
+        }
+        for (int s = 1; s < nsweep_pre; s++) {
+            apply<T>(c, li, x, x2, 3, b, gate, om);
+            std::swap(x, x2);
+        }
+        if (coarsest) return x;
+        apply<T>(c, li, x, Sel<T>::res(l), 2, b, gate, om);
+        MGLevel& lc = L[li + 1];
+        hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(lc.g.nnodes())), dim3(256), 0, c.s, (const T*)Sel<T>::res(l),
+                           Sel<T>::b(lc), l.g, lc.g, lc.fixed, gate);
+        T* xc = vcycle<T>(c, li + 1, Sel<T>::b(lc), gate);
+        hipLaunchKernelGGL(k_prolong_add<T>, dim3(nblk(l.g.nnodes())), dim3(256), 0, c.s, x, (const T*)xc, l.g, lc.g,
+                           l.fixed, gate);
+        for (int s = 0; s < nu2; s++) {
+            apply<T>(c, li, x, x2, 3, b, gate, om);
+            std::swap(x, x2);
+        }
+        return x;
+    }
+};
+
+template <> const double* MGH::ke<double>() const { return K0->ke; }
+template <> const float* MGH::ke<float>() const { return K0->ke32; }
+
+// ---------------------------------------------------------------------------
+// CG drivers. Host polls the device state every `check` iterations; update
+// kernels are no-ops once converged, so the iterate equals the one a
+// per-iteration test would stop at.
+struct CGVecs {
+    void *r = nullptr, *z = nullptr, *p = nullptr, *Ap = nullptr;
+};
+
+struct CGResult {
+    int iters = 0;
+    int nonfinite = 0;
+    int vcycles = 0;
+    int applyA = 0;
+};
+
+static void cg_set(Ctx& c, double tol, int maxit) {
+    CGState h;
+    std::memset(&h, 0, sizeof h);
+    h.tol = tol;
+    h.maxit = maxit;
+    *c.h_st = h;
+    TOPO_HIP(hipMemcpyAsync(c.st, c.h_st, sizeof(CGState), hipMemcpyHostToDevice, c.s));
+}
+
+// Jacobi-PCG on the fine level (ebePCG assembly/index.ts:504-733 in full
+// space with fixed DOFs masked): U is the warm start (TU precision), rhs in T.
+template <typename T, typename TU>
+static CGResult pcg_jacobi(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVecs& v, double tol, int maxit,
+                           volatile int* cancel) {
+    MGLevel& l = mg.L[0];
+    const int64_t n = l.ndof;
+    T* r = (T*)v.r;
+    T* z = (T*)v.z;
+    T* p = (T*)v.p;
+    T* Ap = (T*)v.Ap;
+    CGResult res;
+    if (warm) {
+        // r = rhs - A U on free DOFs (U in T precision for the operator)
+        if (std::is_same<T, TU>::value) {
+            mg.apply<T>(c, 0, (const T*)U, r, 2, rhs, nullptr, 0.0);
+        } else {
+            hipLaunchKernelGGL((k_convert<T, TU>), dim3(nblk(n)), dim3(256), 0, c.s, z, (const TU*)U, n);
+            mg.apply<T>(c, 0, z, r, 2, rhs, nullptr, 0.0);
+        }
+        res.applyA++;
+    } else {
+        hipLaunchKernelGGL(k_fill<TU>, dim3(nblk(n)), dim3(256), 0, c.s, U, TU(0), n, (const int*)nullptr);
+        hipLaunchKernelGGL((k_convert<T, T>), dim3(nblk(n)), dim3(256), 0, c.s, r, rhs, n);
+        hipLaunchKernelGGL(k_mask<T>, dim3(nblk(n)), dim3(256), 0, c.s, r, l.fixed, n);
+    }
+    hipLaunchKernelGGL(k_jacobi_apply<T>, dim3(nblk(n)), dim3(256), 0, c.s, z, (const T*)r, (const T*)Sel<T>::invD(l), n);
+    hipLaunchKernelGGL((k_convert<T, T>), dim3(nblk(n)), dim3(256), 0, c.s, p, (const T*)z, n);
+    cg_set(c, tol, maxit);
+    hipLaunchKernelGGL(k_cg_start<T>, dim3(kRedBlocks), dim3(kRedThreads), 0, c.s, (const T*)r, (const T*)z, n, c.st,
+                       c.partials, c.ticket);
+    c.check_launch();
+    const int* gate = &c.st->done;
+    const int check = 8;
+    for (;;) {
+        const CGState& h = c.read_st();
+        if (h.done) {
+            res.iters = h.iters;
+            res.nonfinite = h.nonfinite;
+            break;
+        }
+        if (cancel && *cancel) throw TopoError{TOPO_E_CANCELLED};
+        for (int k = 0; k < check; k++) {
+            mg.apply<T>(c, 0, p, Ap, 1, nullptr, gate, 0.0);
+            hipLaunchKernelGGL(k_cg_alpha<T>, dim3(kRedBlocks), dim3(kRedThreads), 0, c.s, (const T*)p, (const T*)Ap, n,
+                               c.st, c.partials, c.ticket);
+            hipLaunchKernelGGL((k_cg_update_jacobi<T, TU>), dim3(kRedBlocks), dim3(kRedThreads), 0, c.s, U, r, z,
+                               (const T*)p, (const T*)Ap, (const T*)Sel<T>::invD(l), n, c.st, c.partials, c.ticket);
+            hipLaunchKernelGGL(k_cg_pupdate<T>, dim3(nblk(n)), dim3(256), 0, c.s, p, (const T*)z, n, c.st);
+        }
+        c.check_launch();
+    }
+    res.applyA += res.iters;
+    return res;
+}
+
+// MG-PCG on the fine level (ebeMGPCG assembly/index.ts:1076-1193).
+template <typename T, typename TU>
+static CGResult pcg_mg(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVecs& v, double tol, int maxit,
+                       volatile int* cancel) {
+    MGLevel& l = mg.L[0];
+    const int64_t n = l.ndof;
+    T* r = (T*)v.r;
+    T* z = (T*)v.z;
+    T* p = (T*)v.p;
+    T* Ap = (T*)v.Ap;
+    CGResult res;
+    if (warm) {
+        if (std::is_same<T, TU>::value) {
+            mg.apply<T>(c, 0, (const T*)U, r, 2, rhs, nullptr, 0.0);
+        } else {
+            hipLaunchKernelGGL((k_convert<T, TU>), dim3(nblk(n)), dim3(256), 0, c.s, z, (const TU*)U, n);
+            mg.apply<T>(c, 0, z, r, 2, rhs, nullptr, 0.0);
+        }
+        hipLaunchKernelGGL(k_mask<TU>, dim3(nblk(n)), dim3(256), 0, c.s, U, l.fixed, n);
+        res.applyA++;
+    } else {
+        hipLaunchKernelGGL(k_fill<TU>, dim3(nblk(n)), dim3(256), 0, c.s, U, TU(0), n, (const int*)nullptr);
+        hipLaunchKernelGGL((k_convert<T, T>), dim3(nblk(n)), dim3(256), 0, c.s, r, rhs, n);
+        hipLaunchKernelGGL(k_mask<T>, dim3(nblk(n)), dim3(256), 0, c.s, r, l.fixed, n);
+    }
+    T* zz = mg.vcycle<T>(c, 0, r, nullptr);
+    res.vcycles++;
+    hipLaunchKernelGGL((k_convert<T, T>), dim3(nblk(n)), dim3(256), 0, c.s, z, (const T*)zz, n);
+    hipLaunchKernelGGL(k_mask<T>, dim3(nblk(n)), dim3(256), 0, c.s, z, l.fixed, n);
+    hipLaunchKernelGGL((k_convert<T, T>), dim3(nblk(n)), dim3(256), 0, c.s, p, (const T*)z, n);
+    cg_set(c, tol, maxit);
+    hipLaunchKernelGGL(k_cg_start<T>, dim3(kRedBlocks), dim3(kRedThreads), 0, c.s, (const T*)r, (const T*)z, n, c.st,
+                       c.partials, c.ticket);
+    c.check_launch();
+    const int* gate = &c.st->done;
+    const int check = 4;
+    for (;;) {
+        const CGState& h = c.read_st();
+        if (h.done) {
+            res.iters = h.iters;
+            res.nonfinite = h.nonfinite;
+            break;
+        }
+        if (cancel && *cancel) throw TopoError{TOPO_E_CANCELLED};
+        for (int k = 0; k < check; k++) {
+            mg.apply<T>(c, 0, p, Ap, 1, nullptr, gate, 0.0);
+            hipLaunchKernelGGL(k_cg_alpha<T>, dim3(kRedBlocks), dim3(kRedThreads), 0, c.s, (const T*)p, (const T*)Ap, n,
+                               c.st, c.partials, c.ticket);
+            hipLaunchKernelGGL((k_cg_update_ur<T, TU>), dim3(kRedBlocks), dim3(kRedThreads), 0, c.s, U, r, (const T*)p,
+                               (const T*)Ap, n, c.st, c.partials, c.ticket);
+            T* zk = mg.vcycle<T>(c, 0, r, gate);
+            hipLaunchKernelGGL((k_convert<T, T>), dim3(nblk(n)), dim3(256), 0, c.s, z, (const T*)zk, n);
+            hipLaunchKernelGGL(k_mask<T>, dim3(nblk(n)), dim3(256), 0, c.s, z, l.fixed, n);
+            hipLaunchKernelGGL(k_cg_rz<T>, dim3(kRedBlocks), dim3(kRedThreads), 0, c.s, (const T*)r, (const T*)z, n,
+                               c.st, c.partials, c.ticket);
+            hipLaunchKernelGGL(k_cg_pupdate<T>, dim3(nblk(n)), dim3(256), 0, c.s, p, (const T*)z, n, c.st);
+            hipLaunchKernelGGL(k_cg_check, dim3(1), dim3(1), 0, c.s, c.st);
+        }
+        c.check_launch();
+    }
+    res.applyA += res.iters;
+    res.vcycles += res.iters;
+    return res;
+}
+
+// ---------------------------------------------------------------------------
+struct Solver {
+    topo_config cfg;
+    Grid g;
+    int64_t nel = 0, ndof = 0;
+    std::unique_ptr<Ctx> c;
+    MGH mg;
+    uint8_t* fixed = nullptr;
+    double* F = nullptr;          // nload * ndof
+    uint8_t* pres = nullptr;      // zyx, or null
+    uint8_t* voidm = nullptr;
+    float *x = nullptr, *xold = nullptr, *xnew = nullptr, *xphys = nullptr, *xf = nullptr;
+    float *dc = nullptr, *dcn = nullptr, *w = nullptr, *stress = nullptr, *Hs = nullptr, *tmpf = nullptr;
+    double* sw = nullptr;
+    double* E = nullptr;          // fine moduli (level 0)
+    float* E32 = nullptr;
+    double* U = nullptr;          // nload * ndof
+    double* adj = nullptr;
+    bool have_U = false;
+    // CG work
+    CGVecs v64, v32;
+    double* r64 = nullptr;
+    float* d32 = nullptr;
+    float* rhs32 = nullptr;
+    int64_t nfree = 0;
+    int loop = 0;
+    double last_c = 0;
+    volatile int cancel_flag = 0;
+    bool stencil_ready = false;
+    DevKe K0;
+    FilterStencil* stencil = nullptr;
+
+    void init(const topo_config* cfgin, const uint8_t* hfixed, const float* hF, const uint8_t* hpres,
+              const uint8_t* hvoid) {
+        cfg = *cfgin;
+        TOPO_CHECK(cfg.nelx >= 1 && cfg.nely >= 1 && cfg.nelz >= 1, TOPO_E_ARG, "grid dims must be >= 1");
+        TOPO_CHECK(cfg.nload >= 1 && cfg.nload <= 64, TOPO_E_ARG, "nload must be in 1..64");
+        TOPO_CHECK(cfg.linear_solver == TOPO_SOLVER_PCG || cfg.linear_solver == TOPO_SOLVER_MGPCG, TOPO_E_ARG,
+                   "unknown linear_solver %d", cfg.linear_solver);
+        TOPO_CHECK(cfg.precision == TOPO_PREC_FP64 || cfg.precision == TOPO_PREC_MIXED, TOPO_E_ARG,
+                   "unknown precision %d", cfg.precision);
+        TOPO_CHECK(hfixed && hF, TOPO_E_ARG, "fixedMask and F are required");
+        TOPO_CHECK(cfg.objective == TOPO_OBJ_COMPLIANCE, TOPO_E_UNSUPPORTED, "objective %d not built", cfg.objective);
+        g.nelx = cfg.nelx; g.nely = cfg.nely; g.nelz = cfg.nelz;
+        nel = g.nel();
+        ndof = g.ndof();
+        c.reset(new Ctx());
+        Ctx& C = *c;
+        {
+            double hke[576];
+            k0_matrix(cfg.nu, hke);
+            K0.set(C, hke);
+        }
+        mg.K0 = &K0;
+        stencil = C.alloc<FilterStencil>(1);
+        fixed = C.alloc<uint8_t>(ndof);
+        TOPO_HIP(hipMemcpy(fixed, hfixed, ndof, hipMemcpyHostToDevice));
+        nfree = 0;
+        for (int64_t i = 0; i < ndof; i++) nfree += hfixed[i] ? 0 : 1;
+        F = C.alloc<double>(ndof * cfg.nload);
+        {
+            std::vector<double> tmp(ndof * cfg.nload);
+            for (int64_t i = 0; i < ndof * cfg.nload; i++) tmp[i] = (double)hF[i];
+            TOPO_HIP(hipMemcpy(F, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice));
+        }
+        uint8_t* tmpu = C.alloc<uint8_t>(nel);
+        if (hpres) {
+            pres = C.alloc<uint8_t>(nel);
+            TOPO_HIP(hipMemcpy(tmpu, hpres, nel, hipMemcpyHostToDevice));
+            hipLaunchKernelGGL(k_u8_ref_to_zyx, dim3(nblk(nel)), dim3(256), 0, C.s, tmpu, pres, g);
+            C.sync();
+        }
+        if (hvoid) {
+            voidm = C.alloc<uint8_t>(nel);
+            TOPO_HIP(hipMemcpy(tmpu, hvoid, nel, hipMemcpyHostToDevice));
+            hipLaunchKernelGGL(k_u8_ref_to_zyx, dim3(nblk(nel)), dim3(256), 0, C.s, tmpu, voidm, g);
+            C.sync();
+        }
+        C.free_one(tmpu);
+        x = C.alloc<float>(nel); xold = C.alloc<float>(nel); xnew = C.alloc<float>(nel);
+        xphys = C.alloc<float>(nel); xf = C.alloc<float>(nel);
+        dc = C.alloc<float>(nel); dcn = C.alloc<float>(nel); w = C.alloc<float>(nel);
+        stress = C.alloc<float>(nel); Hs = C.alloc<float>(nel); tmpf = C.alloc<float>(nel);
+        sw = C.alloc<double>(nel);
+        E = C.alloc<double>(nel);
+        const bool mixed = cfg.precision == TOPO_PREC_MIXED;
+        if (mixed) E32 = C.alloc<float>(nel);
+        U = C.alloc<double>(ndof * cfg.nload);
+        hipLaunchKernelGGL(k_fill<double>, dim3(nblk(ndof * cfg.nload)), dim3(256), 0, C.s, U, 0.0, ndof * cfg.nload,
+                           (const int*)nullptr);
+        v64.r = C.alloc<double>(ndof); v64.z = C.alloc<double>(ndof);
+        v64.p = C.alloc<double>(ndof); v64.Ap = C.alloc<double>(ndof);
+        if (mixed) {
+            v32.r = C.alloc<float>(ndof); v32.z = C.alloc<float>(ndof);
+            v32.p = C.alloc<float>(ndof); v32.Ap = C.alloc<float>(ndof);
+            r64 = C.alloc<double>(ndof);
+            d32 = C.alloc<float>(ndof);
+            rhs32 = C.alloc<float>(ndof);
+        }
+        // multigrid hierarchy (level 0 always exists: its diagonal is the
+        // Jacobi preconditioner)
+        mg.omega = cfg.mg_omega;
+        mg.nu1 = cfg.mg_nu1;
+        mg.nu2 = cfg.mg_nu2;
+        mg.coarse_sweeps = cfg.coarse_sweeps;
+        mg.coarse_solver = cfg.coarse_solver;
+        mg.galerkin_max = cfg.galerkin_max_ndof;
+        mg.thr = cfg.Emin * 1000.0;
+        mg.omega_cap = cfg.mg_omega_cap;
+        const int nlev = cfg.linear_solver == TOPO_SOLVER_MGPCG ? cfg.mg_levels : 1;
+        mg.build(C, g, fixed, cfg.linear_solver == TOPO_SOLVER_MGPCG ? nlev : 1, mixed);
+        mg.L[0].E = E;
+        mg.L[0].E32 = E32;
+        // filter stencil + normalisers
+        setup_filter(cfg.rmin);
+        // initial design: volfrac, preserved -> 1, void -> 0 (:1928-2022)
+        std::vector<float> hx(nel, (float)cfg.volfrac);
+        if (hpres) for (int64_t i = 0; i < nel; i++) if (hpres[i]) hx[i] = 1.0f;
+        if (hvoid) for (int64_t i = 0; i < nel; i++) if (hvoid[i]) hx[i] = 0.0f;
+        set_design(hx.data());
+        C.sync();
+    }
+
+    void setup_filter(double rmin) {
+        FilterStencil st;
+        std::memset(&st, 0, sizeof st);
+        if (cfg.filter_type == TOPO_FILTER_DENSITY) {
+            // applyDensityFilter window ceil(r), d^2 <= r^2, order z, y, x
+            const int r = (int)std::ceil(rmin);
+            TOPO_CHECK(r <= 2, TOPO_E_UNSUPPORTED, "density filter radius > 2 not supported");
+            st.r = r;
+            const double r2 = rmin * rmin;
+            for (int kz = -r; kz <= r; kz++)
+                for (int ky = -r; ky <= r; ky++)
+                    for (int kx = -r; kx <= r; kx++) {
+                        const double d2 = (double)(kx * kx + ky * ky + kz * kz);
+                        if (d2 <= r2) {
+                            st.off[st.count][0] = kx; st.off[st.count][1] = ky; st.off[st.count][2] = kz;
+                            st.w[st.count] = rmin - std::sqrt(d2);
+                            st.count++;
+                        }
+                    }
+        } else {
+            // prepareFilter window +-floor(rmin), dist <= rmin, order x, y, z
+            const int r = (int)std::floor(rmin);
+            TOPO_CHECK(r <= 2, TOPO_E_UNSUPPORTED, "sensitivity filter radius >= 3 not supported");
+            st.r = r;
+            for (int kx = -r; kx <= r; kx++)
+                for (int ky = -r; ky <= r; ky++)
+                    for (int kz = -r; kz <= r; kz++) {
+                        const double dist = std::sqrt((double)(kx * kx + ky * ky + kz * kz));
+                        if (dist <= rmin) {
+                            st.off[st.count][0] = kx; st.off[st.count][1] = ky; st.off[st.count][2] = kz;
+                            st.w[st.count] = std::max(0.0, rmin - dist);
+                            st.count++;
+                        }
+                    }
+        }
+        Ctx& C = *c;
+        TOPO_HIP(hipMemcpyAsync(stencil, &st, sizeof st, hipMemcpyHostToDevice, C.s));
+        if (cfg.filter_type == TOPO_FILTER_DENSITY)
+            hipLaunchKernelGGL(k_density_sw, dim3(nblk(nel)), dim3(256), 0, C.s, sw, g, (const FilterStencil*)stencil);
+        else
+            hipLaunchKernelGGL(k_filter_hs, dim3(nblk(nel)), dim3(256), 0, C.s, Hs, g, (const FilterStencil*)stencil);
+        C.sync();
+        C.check_launch();
+        stencil_ready = true;
+    }
+
+    void set_design(const float* hx) {
+        Ctx& C = *c;
+        TOPO_HIP(hipMemcpyAsync(tmpf, hx, nel * sizeof(float), hipMemcpyHostToDevice, C.s));
+        hipLaunchKernelGGL(k_ref_to_zyx, dim3(nblk(nel)), dim3(256), 0, C.s, tmpf, x, g);
+        C.sync();
+    }
+    void get_field_f(const float* dsrc, float* hout) {
+        Ctx& C = *c;
+        hipLaunchKernelGGL(k_zyx_to_ref, dim3(nblk(nel)), dim3(256), 0, C.s, dsrc, tmpf, g);
+        TOPO_HIP(hipMemcpyAsync(hout, tmpf, nel * sizeof(float), hipMemcpyDeviceToHost, C.s));
+        C.sync();
+    }
+
+    // E and the operators of all levels from the physical densities (zyx)
+    void set_density_dev(const float* dxphys, double penal) {
+        Ctx& C = *c;
+        // ebePCG uses Math.pow (assembly/index.ts:544); the MG path _powDensity
+        const int powmode = cfg.linear_solver == TOPO_SOLVER_PCG ? 1 : 0;
+        hipLaunchKernelGGL(k_moduli<float>, dim3(nblk(nel)), dim3(256), 0, C.s, dxphys, penal, cfg.E0, cfg.Emin, powmode, E,
+                           E32, nel);
+        C.check_launch();
+        mg.update(C);
+    }
+
+    double tolerance_for(int iteration, int maxIt) const {
+        if (cfg.cg_tol > 0) return cfg.cg_tol;
+        // CG tolerance schedule (js/optimizer-worker-3d.js:3597-3602)
+        const double progress = maxIt > 1 ? (double)(iteration - 1) / (maxIt - 1) : 1.0;
+        return std::exp(std::log(1e-3) + progress * (std::log(1e-8) - std::log(1e-3)));
+    }
+
+    // solve K U_k = F_k for every load case (FE() :4155-4371)
+    void fe(int iteration, int maxIt, topo_step_info* info) {
+        Ctx& C = *c;
+        const double tol = tolerance_for(iteration, maxIt);
+        const int64_t nfree_cap = cfg.cg_max_iter;
+        const bool mgp = cfg.linear_solver == TOPO_SOLVER_MGPCG;
+        // warm start: PCG warm (:3589-3595), WASM MGPCG cold (:1101)
+        bool warm = iteration > 1 && have_U;
+        if (cfg.warm_start == 0) warm = false;
+        if (cfg.warm_start == -1 && mgp) warm = false;
+        // maxIter = min(nfree, MAX_CG_ITERATIONS) (:3609, :3779)
+        const int maxit = (int)std::min<int64_t>(nfree_cap, nfree);
+        CGResult tot;
+        int outer = 0;
+        for (int k = 0; k < cfg.nload; k++) {
+            double* Uk = U + (int64_t)k * ndof;
+            const double* Fk = F + (int64_t)k * ndof;
+            if (cfg.precision == TOPO_PREC_FP64) {
+                CGResult r = mgp ? pcg_mg<double, double>(C, mg, Fk, Uk, warm, v64, tol, maxit, &cancel_flag)
+                                 : pcg_jacobi<double, double>(C, mg, Fk, Uk, warm, v64, tol, maxit, &cancel_flag);
+                TOPO_CHECK(!r.nonfinite, TOPO_E_BREAKDOWN, "non-finite CG scalar");
+                tot.iters += r.iters; tot.vcycles += r.vcycles; tot.applyA += r.applyA;
+            } else {
+                CGResult r = solve_mixed(Fk, Uk, warm, tol, maxit, mgp, &outer);
+                tot.iters += r.iters; tot.vcycles += r.vcycles; tot.applyA += r.applyA;
+            }
+        }
+        have_U = true;
+        if (info) {
+            info->cg_iters = tot.iters;
+            info->vcycles = tot.vcycles;
+            info->applyA_fine = tot.applyA;
+            info->outer_iters = outer;
+            info->tolerance = tol;
+        }
+    }
+
+    // FP64 iterative refinement around an FP32 inner solve: the outer
+    // residual r = F - K U, its norm and the stopping test
+    // ||r||^2 < tol^2 ||r0||^2 are FP64; the inner MG-/Jacobi-PCG runs on FP32
+    // vectors and an FP32 operator with FP64 CG scalars.
+    CGResult solve_mixed(const double* Fk, double* Uk, bool warm, double tol, int maxit, bool mgp, int* outer_out) {
+        Ctx& C = *c;
+        CGResult tot;
+        if (!warm)
+            hipLaunchKernelGGL(k_fill<double>, dim3(nblk(ndof)), dim3(256), 0, C.s, Uk, 0.0, ndof, (const int*)nullptr);
+        else
+            hipLaunchKernelGGL(k_mask<double>, dim3(nblk(ndof)), dim3(256), 0, C.s, Uk, fixed, ndof);
+        double* r = r64;
+        mg.apply<double>(C, 0, Uk, r, 2, Fk, nullptr, 0.0);
+        tot.applyA++;
+        hipLaunchKernelGGL((k_dot<double, double>), dim3(kRedBlocks), dim3(kRedThreads), 0, C.s, r, r, ndof, C.dsc,
+                           C.partials, C.ticket);
+        const double rn0 = C.read_d(C.dsc);
+        const double tolSq = tol * tol * (rn0 > 1e-30 ? rn0 : 1e-30);
+        double rn = rn0;
+        const double eta = std::min(1e-1, std::max(1e-4, std::sqrt(tol)));
+        int outer = 0;
+        while (rn >= tolSq && outer < 20 && tot.iters < maxit) {
+            const double scale = std::sqrt(rn);
+            TOPO_HIP(hipMemcpyAsync(C.dsc + 1, &scale, sizeof(double), hipMemcpyHostToDevice, C.s));
+            hipLaunchKernelGGL(k_scale_to<float>, dim3(nblk(ndof)), dim3(256), 0, C.s, rhs32, r, C.dsc + 1, ndof);
+            // inner solve A d = r/scale from d = 0
+            CGResult ir = mgp ? pcg_mg<float, float>(C, mg, rhs32, d32, false, v32, eta, maxit - tot.iters, &cancel_flag)
+                              : pcg_jacobi<float, float>(C, mg, rhs32, d32, false, v32, eta, maxit - tot.iters,
+                                                         &cancel_flag);
+            hipLaunchKernelGGL(k_axpy_refine<float>, dim3(nblk(ndof)), dim3(256), 0, C.s, Uk, (const float*)d32, scale,
+                               ndof);
+            tot.iters += ir.iters; tot.vcycles += ir.vcycles; tot.applyA += ir.applyA;
+            mg.apply<double>(C, 0, Uk, r, 2, Fk, nullptr, 0.0);
+            tot.applyA++;
+            hipLaunchKernelGGL((k_dot<double, double>), dim3(kRedBlocks), dim3(kRedThreads), 0, C.s, r, r, ndof, C.dsc,
+                               C.partials, C.ticket);
+            rn = C.read_d(C.dsc);
+            outer++;
+            TOPO_CHECK(std::isfinite(rn), TOPO_E_BREAKDOWN, "non-finite residual in mixed refinement");
+            if (ir.iters == 0) break;
+        }
+        *outer_out += outer;
+        return tot;
+    }
+
+    double compliance() {
+        Ctx& C = *c;
+        hipLaunchKernelGGL((k_dot<double, double>), dim3(kRedBlocks), dim3(kRedThreads), 0, C.s, F, U, ndof * cfg.nload,
+                           C.dsc + 2, C.partials, C.ticket);
+        return C.read_d(C.dsc + 2);
+    }
+
+    int oc_run(const float* dx, const float* ddcn, float* dxnew, double volfrac, double* vol) {
+        Ctx& C = *c;
+        OCState h;
+        std::memset(&h, 0, sizeof h);
+        h.l1 = 0;
+        h.l2 = 1e9;
+        h.target = volfrac * (double)nel;
+        *C.h_oc = h;
+        TOPO_HIP(hipMemcpyAsync(C.oc, C.h_oc, sizeof(OCState), hipMemcpyHostToDevice, C.s));
+        for (int guard = 0; guard < 100; guard++) {
+            for (int k = 0; k < 3; k++) {
+                hipLaunchKernelGGL(k_oc_plan, dim3(1), dim3(64), 0, C.s, C.oc);
+                hipLaunchKernelGGL(k_oc_eval, dim3(kRedBlocks), dim3(kRedThreads), 0, C.s, dx, ddcn, pres, voidm,
+                                   cfg.move, nel, C.oc, C.partials, C.ticket);
+            }
+            C.check_launch();
+            TOPO_HIP(hipMemcpyAsync(C.h_oc, C.oc, sizeof(OCState), hipMemcpyDeviceToHost, C.s));
+            C.sync();
+            if (C.h_oc->finished) break;
+        }
+        TOPO_CHECK(C.h_oc->finished, TOPO_E_BREAKDOWN, "OC bisection did not terminate");
+        hipLaunchKernelGGL(k_oc_final, dim3(nblk(nel)), dim3(256), 0, C.s, dx, ddcn, pres, voidm, cfg.move, nel,
+                           &C.oc->lmid_last, dxnew);
+        C.check_launch();
+        if (vol) *vol = C.h_oc->vol_last;
+        return C.h_oc->passes;
+    }
+
+    void filter_fwd(int type, const float* dxd, const float* din, float* dout) {
+        Ctx& C = *c;
+        if (type == TOPO_FILTER_SENSITIVITY)
+            hipLaunchKernelGGL(k_sens_filter, dim3(nblk(nel)), dim3(256), 0, C.s, dxd, din, Hs, dout, g,
+                               (const FilterStencil*)stencil);
+        else
+            hipLaunchKernelGGL(k_density_filter, dim3(nblk(nel)), dim3(256), 0, C.s, din, dout, sw, g, 0,
+                               (const FilterStencil*)stencil);
+        C.check_launch();
+    }
+
+    // one SIMP iteration (optimize() loop body js/optimizer-worker-3d.js:2124-2252)
+    void step(topo_step_info* info) {
+        Ctx& C = *c;
+        auto t0 = std::chrono::steady_clock::now();
+        topo_step_info loc;
+        std::memset(&loc, 0, sizeof loc);
+        if (cancel_flag) throw TopoError{TOPO_E_CANCELLED};
+        loop++;
+        TOPO_HIP(hipMemcpyAsync(xold, x, nel * sizeof(float), hipMemcpyDeviceToDevice, C.s));
+        const double penal = cfg.penal_start + (cfg.penal - cfg.penal_start) *
+                                                   std::min(1.0, (double)(loop - 1) / std::max(1, cfg.continuation_iters));
+        const bool proj = cfg.use_projection != 0;
+        double beta = 1.0;
+        if (proj)
+            beta = cfg.beta_fixed > 0 ? (double)cfg.beta_fixed
+                                      : std::min(cfg.beta_max, std::pow(2.0, std::floor((double)(loop - 1) / cfg.beta_interval)));
+        const float* xd = x;
+        if (cfg.filter_type == TOPO_FILTER_DENSITY) {
+            filter_fwd(TOPO_FILTER_DENSITY, nullptr, x, xf);
+            xd = xf;
+        }
+        const float* xp = xd;
+        if (proj) {
+            hipLaunchKernelGGL(k_heaviside, dim3(nblk(nel)), dim3(256), 0, C.s, xd, xphys, nel, beta, cfg.eta);
+            xp = xphys;
+        } else if (xd != xphys) {
+            TOPO_HIP(hipMemcpyAsync(xphys, xd, nel * sizeof(float), hipMemcpyDeviceToDevice, C.s));
+        }
+        set_density_dev(xp, penal);
+        auto ts = std::chrono::steady_clock::now();
+        fe(loop, cfg.max_iterations, &loc);
+        C.sync();
+        auto te = std::chrono::steady_clock::now();
+        last_c = compliance();
+        // energies + sensitivities (first load case carries the energies for
+        // nload=1; for several cases the sensitivities are summed)
+        for (int k = 0; k < cfg.nload; k++) {
+            hipLaunchKernelGGL(k_energy_sens, dim3(nblk(nel)), dim3(256), 0, C.s, U + (int64_t)k * ndof, xp, xd, penal,
+                               cfg.E0, cfg.Emin, (int)proj, beta, cfg.eta, g, w, stress, k == 0 ? dc : tmpf,
+                               (const double*)K0.ke);
+            if (k > 0) hipLaunchKernelGGL(k_add_f, dim3(nblk(nel)), dim3(256), 0, C.s, dc, (const float*)tmpf, nel);
+        }
+        C.check_launch();
+        if (cfg.filter_type == TOPO_FILTER_DENSITY)
+            hipLaunchKernelGGL(k_density_filter, dim3(nblk(nel)), dim3(256), 0, C.s, dc, dcn, sw, g, 1,
+                               (const FilterStencil*)stencil);
+        else
+            hipLaunchKernelGGL(k_sens_filter, dim3(nblk(nel)), dim3(256), 0, C.s, x, dc, Hs, dcn, g,
+                               (const FilterStencil*)stencil);
+        C.check_launch();
+        double vol = 0;
+        loc.oc_passes = oc_run(x, dcn, xnew, cfg.volfrac, &vol);
+        hipLaunchKernelGGL(k_change, dim3(kRedBlocks), dim3(kRedThreads), 0, C.s, xnew, xold, nel, C.dsc + 4, C.partials,
+                           C.ticket);
+        C.check_launch();
+        TOPO_HIP(hipMemcpyAsync(C.h_sc, C.dsc, 8 * sizeof(double), hipMemcpyDeviceToHost, C.s));
+        C.sync();
+        const double change = C.h_sc[4];
+        const double volume = C.h_sc[5];
+        std::swap(x, xnew);
+        // max stress
+        hipLaunchKernelGGL(k_change, dim3(kRedBlocks), dim3(kRedThreads), 0, C.s, stress, (const float*)nullptr, nel,
+                           C.dsc + 6, C.partials, C.ticket);
+        TOPO_HIP(hipMemcpyAsync(C.h_sc, C.dsc, 8 * sizeof(double), hipMemcpyDeviceToHost, C.s));
+        C.sync();
+        auto t1 = std::chrono::steady_clock::now();
+        loc.iteration = loop;
+        loc.compliance = last_c;
+        loc.change = change;
+        loc.penal = penal;
+        loc.beta = beta;
+        loc.volume = volume / (double)nel;
+        loc.max_stress = C.h_sc[6];
+        loc.t_solve_ms = std::chrono::duration<double, std::milli>(te - ts).count();
+        loc.t_total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        if (info) *info = loc;
+    }
+};
+
+}  // namespace topo
+
+using namespace topo;
+
+struct topo_solver {
+    Solver s;
+};
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int topo_solver_create(const topo_config* cfg, const uint8_t* fixedMask, const float* F, const uint8_t* preserved,
+                       const uint8_t* voidm, topo_solver** out) {
+    TOPO_API_BEGIN
+    TOPO_CHECK(cfg && out, TOPO_E_ARG, "topo_solver_create: null argument");
+    std::unique_ptr<topo_solver> h(new topo_solver());
+    h->s.init(cfg, fixedMask, F, preserved, voidm);
+    *out = h.release();
+    return TOPO_OK;
+    TOPO_API_END
+}
+
+int topo_solver_destroy(topo_solver* s) {
+    delete s;
+    return TOPO_OK;
+}
+
+int topo_solver_set_design(topo_solver* h, const float* x) {
+    TOPO_API_BEGIN
+    TOPO_CHECK(h && x, TOPO_E_ARG, "null argument");
+    h->s.set_design(x);
+    return TOPO_OK;
+    TOPO_API_END
+}
+
+int topo_solver_get_design(topo_solver* h, float* x) {
+    TOPO_API_BEGIN
+    TOPO_CHECK(h && x, TOPO_E_ARG, "null argument");
+    h->s.get_field_f(h->s.x, x);
+    return TOPO_OK;
+    TOPO_API_END
+}
+
+int topo_solver_set_iteration(topo_solver* h, int32_t loop) {
+    TOPO_API_BEGIN
+    TOPO_CHECK(h && loop >= 0, TOPO_E_ARG, "bad argument");
+    h->s.loop = loop;
+    return TOPO_OK;
+    TOPO_API_END
+}
+
+int topo_solver_update(topo_solver* h, double penal, double rmin, double volfrac, int32_t max_iterations) {
+    TOPO_API_BEGIN
+    TOPO_CHECK(h, TOPO_E_ARG, "null solver");
+    Solver& S = h->s;
+    if (penal > 0) S.cfg.penal = penal;
+    if (volfrac > 0) S.cfg.volfrac = volfrac;
+    if (max_iterations > 0) S.cfg.max_iterations = max_iterations;
+    if (rmin > 0 && rmin != S.cfg.rmin) {
+        S.cfg.rmin = rmin;
+        S.setup_filter(rmin);
+    }
+    return TOPO_OK;
+    TOPO_API_END
+}
+
+int topo_solver_step(topo_solver* h, topo_step_info* info) {
+    TOPO_API_BEGIN
+    TOPO_CHECK(h, TOPO_E_ARG, "null solver");
+    h->s.step(info);
+    return TOPO_OK;
+    TOPO_API_END
+}
+
+int topo_solver_get_field(topo_solver* h, int32_t field, void* out) {
+    TOPO_API_BEGIN
+    TOPO_CHECK(h && out, TOPO_E_ARG, "null argument");
+    Solver& S = h->s;
+    switch (field) {
+    case TOPO_FIELD_U:
+        TOPO_HIP(hipMemcpyAsync(out, S.U, S.ndof * S.cfg.nload * sizeof(double), hipMemcpyDeviceToHost, S.c->s));
+        S.c->sync();
+        break;
+    case TOPO_FIELD_XPHYS: S.get_field_f(S.xphys, (float*)out); break;
+    case TOPO_FIELD_ENERGY: S.get_field_f(S.w, (float*)out); break;
+    case TOPO_FIELD_DC: S.get_field_f(S.dc, (float*)out); break;
+    case TOPO_FIELD_DCN: S.get_field_f(S.dcn, (float*)out); break;
+    case TOPO_FIELD_STRESS: S.get_field_f(S.stress, (float*)out); break;
+    default: TOPO_CHECK(false, TOPO_E_ARG, "unknown field %d", field);
+    }
+    return TOPO_OK;
+    TOPO_API_END
+}
+
+int topo_solver_set_density(topo_solver* h, const float* xPhys, double penal) {
+    TOPO_API_BEGIN
+    TOPO_CHECK(h && xPhys, TOPO_E_ARG, "null argument");
+    Solver& S = h->s;
+    TOPO_HIP(hipMemcpyAsync(S.tmpf, xPhys, S.nel * sizeof(float), hipMemcpyHostToDevice, S.c->s));
+    hipLaunchKernelGGL(k_ref_to_zyx, dim3(nblk(S.nel)), dim3(256), 0, S.c->s, S.tmpf, S.xphys, S.g);
+    S.set_density_dev(S.xphys, penal);
+    S.c->sync();
+    return TOPO_OK;
+    TOPO_API_END
+}
+
+int topo_solver_fe(topo_solver* h, const float* xPhys, double penal, int32_t iteration, int32_t max_iterations,
+                   double* Uio, double* cout, topo_step_info* info) {
+    TOPO_API_BEGIN
+    TOPO_CHECK(h && xPhys && Uio, TOPO_E_ARG, "null argument");
+    Solver& S = h->s;
+    TOPO_HIP(hipMemcpyAsync(S.tmpf, xPhys, S.nel * sizeof(float), hipMemcpyHostToDevice, S.c->s));
+    hipLaunchKernelGGL(k_ref_to_zyx, dim3(nblk(S.nel)), dim3(256), 0, S.c->s, S.tmpf, S.xphys, S.g);
+    S.set_density_dev(S.xphys, penal);
+    TOPO_HIP(hipMemcpyAsync(S.U, Uio, S.ndof * S.cfg.nload * sizeof(double), hipMemcpyHostToDevice, S.c->s));
+    S.have_U = true;
+    topo_step_info loc;
+    std::memset(&loc, 0, sizeof loc);
+    S.fe(iteration, max_iterations, &loc);
+    TOPO_HIP(hipMemcpyAsync(Uio, S.U, S.ndof * S.cfg.nload * sizeof(double), hipMemcpyDeviceToHost, S.c->s));
+    S.c->sync();
+    const double cc = S.compliance();
+    if (cout) *cout = cc;
+    loc.compliance = cc;
+    if (info) *info = loc;
+    return TOPO_OK;
+    TOPO_API_END
+}
+
+int topo_op_applyA(topo_solver* h, int32_t level, int32_t prec, const double* p, double* Ap) {
+    TOPO_API_BEGIN
+    TOPO_CHECK(h && p && Ap, TOPO_E_ARG, "null argument");
+    Solver& S = h->s;
+    TOPO_CHECK(level >= 0 && level < (int)S.mg.L.size(), TOPO_E_ARG, "level %d out of range", level);
+    TOPO_CHECK(prec == TOPO_PREC_FP64 || (prec == TOPO_PREC_MIXED && S.cfg.precision == TOPO_PREC_MIXED), TOPO_E_ARG,
+               "FP32 operator needs a mixed-precision solver");
+    MGLevel& l = S.mg.L[level];
+    Ctx& C = *S.c;
+    if (prec == TOPO_PREC_FP64) {
+        TOPO_HIP(hipMemcpyAsync(l.x, p, l.ndof * sizeof(double), hipMemcpyHostToDevice, C.s));
+        S.mg.apply<double>(C, level, l.x, l.x2, 0, nullptr, nullptr, 0.0);
+        TOPO_HIP(hipMemcpyAsync(Ap, l.x2, l.ndof * sizeof(double), hipMemcpyDeviceToHost, C.s));
+    } else {
+        TOPO_HIP(hipMemcpyAsync(l.x, p, l.ndof * sizeof(double), hipMemcpyHostToDevice, C.s));
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(l.ndof)), dim3(256), 0, C.s, l.x32, l.x, l.ndof);
+        S.mg.apply<float>(C, level, l.x32, l.x232, 0, nullptr, nullptr, 0.0);
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(nblk(l.ndof)), dim3(256), 0, C.s, l.x2, l.x232, l.ndof);
+        TOPO_HIP(hipMemcpyAsync(Ap, l.x2, l.ndof * sizeof(double), hipMemcpyDeviceToHost, C.s));
+    }
+    C.sync();
+    return TOPO_OK;
+    TOPO_API_END
+}
+
+int topo_op_vcycle(topo_solver* h, int32_t prec, const double* r, double* z) {
+    TOPO_API_BEGIN
+    TOPO_CHECK(h && r && z, TOPO_E_ARG, "null argument");
+    Solver& S = h->s;
+    Ctx& C = *S.c;
+    MGLevel& l = S.mg.L[0];
+    double* rd = (double*)S.v64.r;
+    TOPO_HIP(hipMemcpyAsync(rd, r, l.ndof * sizeof(double), hipMemcpyHostToDevice, C.s));
+    if (prec == TOPO_PREC_FP64) {
+        double* zz = S.mg.vcycle<double>(C, 0, rd, nullptr);
+        TOPO_HIP(hipMemcpyAsync(z, zz, l.ndof * sizeof(double), hipMemcpyDeviceToHost, C.s));
+    } else {
+        TOPO_CHECK(S.cfg.precision == TOPO_PREC_MIXED, TOPO_E_ARG, "FP32 V-cycle needs a mixed-precision solver");
+        float* r32 = (float*)S.v32.r;
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(l.ndof)), dim3(256), 0, C.s, r32, rd, l.ndof);
+        float* zz = S.mg.vcycle<float>(C, 0, r32, nullptr);
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(nblk(l.ndof)), dim3(256), 0, C.s, (double*)S.v64.z, zz,
+                           l.ndof);
+        TOPO_HIP(hipMemcpyAsync(z, S.v64.z, l.ndof * sizeof(double), hipMemcpyDeviceToHost, C.s));
+    }
+    C.sync();
+    return TOPO_OK;
+    TOPO_API_END
+}
+
+int topo_op_energy(topo_solver* h, const double* Uh, float* wout) {
+    TOPO_API_BEGIN
+    TOPO_CHECK(h && Uh && wout, TOPO_E_ARG, "null argument");
+    Solver& S = h->s;
+    Ctx& C = *S.c;
+    double* Ud = (double*)S.v64.z;
+    TOPO_HIP(hipMemcpyAsync(Ud, Uh, S.ndof * sizeof(double), hipMemcpyHostToDevice, C.s));
+    hipLaunchKernelGGL(k_energy_sens, dim3(nblk(S.nel)), dim3(256), 0, C.s, Ud, S.xphys, S.xphys, 3.0, S.cfg.E0,
+                       S.cfg.Emin, 0, 1.0, 0.5, S.g, S.w, (float*)nullptr, S.dc, (const double*)S.K0.ke);
+    S.get_field_f(S.w, wout);
+    return TOPO_OK;
+    TOPO_API_END
+}
+
+int topo_op_filter(topo_solver* h, int32_t type, const float* xh, const float* in, float* out) {
+    TOPO_API_BEGIN
+    TOPO_CHECK(h && in && out, TOPO_E_ARG, "null argument");
+    Solver& S = h->s;
+    TOPO_CHECK(type == (S.cfg.filter_type == TOPO_FILTER_DENSITY ? TOPO_FILTER_DENSITY : TOPO_FILTER_SENSITIVITY),
+               TOPO_E_ARG, "filter type %d does not match the solver's filter", type);
+    TOPO_CHECK(type != TOPO_FILTER_SENSITIVITY || xh, TOPO_E_ARG, "sensitivity filter needs x");
+    Ctx& C = *S.c;
+    if (xh) {
+        TOPO_HIP(hipMemcpyAsync(S.tmpf, xh, S.nel * sizeof(float), hipMemcpyHostToDevice, C.s));
+        hipLaunchKernelGGL(k_ref_to_zyx, dim3(nblk(S.nel)), dim3(256), 0, C.s, S.tmpf, S.xnew, S.g);
+    }
+    TOPO_HIP(hipMemcpyAsync(S.tmpf, in, S.nel * sizeof(float), hipMemcpyHostToDevice, C.s));
+    hipLaunchKernelGGL(k_ref_to_zyx, dim3(nblk(S.nel)), dim3(256), 0, C.s, S.tmpf, S.dc, S.g);
+    S.filter_fwd(type, S.xnew, S.dc, S.dcn);
+    S.get_field_f(S.dcn, out);
+    return TOPO_OK;
+    TOPO_API_END
+}
+
+int topo_op_filter_adjoint(topo_solver* h, int32_t type, const float* in, float* out) {
+    TOPO_API_BEGIN
+    TOPO_CHECK(h && in && out, TOPO_E_ARG, "null argument");
+    Solver& S = h->s;
+    TOPO_CHECK(type == TOPO_FILTER_DENSITY && S.cfg.filter_type == TOPO_FILTER_DENSITY, TOPO_E_ARG,
+               "adjoint defined for the density filter");
+    Ctx& C = *S.c;
+    TOPO_HIP(hipMemcpyAsync(S.tmpf, in, S.nel * sizeof(float), hipMemcpyHostToDevice, C.s));
+    hipLaunchKernelGGL(k_ref_to_zyx, dim3(nblk(S.nel)), dim3(256), 0, C.s, S.tmpf, S.dc, S.g);
+    hipLaunchKernelGGL(k_density_filter, dim3(nblk(S.nel)), dim3(256), 0, C.s, S.dc, S.dcn, S.sw, S.g, 1,
+                       (const FilterStencil*)S.stencil);
+    S.get_field_f(S.dcn, out);
+    return TOPO_OK;
+    TOPO_API_END
+}
+
+int topo_op_oc(topo_solver* h, const float* xh, const float* dcnh, float* xnewh, double volfrac) {
+    TOPO_API_BEGIN
+    TOPO_CHECK(h && xh && dcnh && xnewh, TOPO_E_ARG, "null argument");
+    Solver& S = h->s;
+    Ctx& C = *S.c;
+    TOPO_HIP(hipMemcpyAsync(S.tmpf, xh, S.nel * sizeof(float), hipMemcpyHostToDevice, C.s));
+    hipLaunchKernelGGL(k_ref_to_zyx, dim3(nblk(S.nel)), dim3(256), 0, C.s, S.tmpf, S.xold, S.g);
+    TOPO_HIP(hipMemcpyAsync(S.tmpf, dcnh, S.nel * sizeof(float), hipMemcpyHostToDevice, C.s));
+    hipLaunchKernelGGL(k_ref_to_zyx, dim3(nblk(S.nel)), dim3(256), 0, C.s, S.tmpf, S.dc, S.g);
+    const int passes = S.oc_run(S.xold, S.dc, S.xnew, volfrac, nullptr);
+    S.get_field_f(S.xnew, xnewh);
+    return passes;
+    TOPO_API_END
+}
+
+int topo_time_op(topo_solver* h, int32_t op, int32_t prec, int32_t level, int32_t reps, double* avg_ms, double* bytes) {
+    TOPO_API_BEGIN
+    TOPO_CHECK(h && avg_ms && reps > 0, TOPO_E_ARG, "bad argument");
+    Solver& S = h->s;
+    Ctx& C = *S.c;
+    TOPO_CHECK(level >= 0 && level < (int)S.mg.L.size(), TOPO_E_ARG, "level out of range");
+    MGLevel& l = S.mg.L[level];
+    const bool f32 = prec == TOPO_PREC_MIXED;
+    TOPO_CHECK(!f32 || S.cfg.precision == TOPO_PREC_MIXED, TOPO_E_ARG, "FP32 timing needs a mixed solver");
+    hipEvent_t e0, e1;
+    TOPO_HIP(hipEventCreate(&e0));
+    TOPO_HIP(hipEventCreate(&e1));
+    double by = 0;
+    auto run = [&]() {
+        switch (op) {
+        case TOPO_TIME_APPLYA:
+            if (f32) S.mg.apply<float>(C, level, l.x32, l.x232, 0, nullptr, nullptr, 0.0);
+            else S.mg.apply<double>(C, level, l.x, l.x2, 0, nullptr, nullptr, 0.0);
+            break;
+        case TOPO_TIME_VCYCLE:
+            if (f32) S.mg.vcycle<float>(C, 0, S.mg.L[0].b32, nullptr);
+            else S.mg.vcycle<double>(C, 0, S.mg.L[0].b, nullptr);
+            break;
+        case TOPO_TIME_ENERGY:
+            hipLaunchKernelGGL(k_energy_sens, dim3(nblk(S.nel)), dim3(256), 0, C.s, S.U, S.xphys, S.xphys, 3.0, S.cfg.E0,
+                               S.cfg.Emin, 0, 1.0, 0.5, S.g, S.w, S.stress, S.dc, (const double*)S.K0.ke);
+            break;
+        case TOPO_TIME_FILTER:
+            S.filter_fwd(S.cfg.filter_type == TOPO_FILTER_DENSITY ? TOPO_FILTER_DENSITY : TOPO_FILTER_SENSITIVITY, S.x,
+                         S.dc, S.dcn);
+            break;
+        case TOPO_TIME_COPY:
+            hipLaunchKernelGGL((k_convert<double, double>), dim3(nblk(l.ndof)), dim3(256), 0, C.s, l.x2, (const double*)l.x,
+                               l.ndof);
+            break;
+        default: TOPO_CHECK(false, TOPO_E_ARG, "unknown op %d", op);
+        }
+    };
+    // algorithmic bytes of one launch
+    const double sv = f32 ? 4.0 : 8.0;
+    if (op == TOPO_TIME_APPLYA) by = 2.0 * l.ndof * sv + (double)l.g.nel() * sv;
+    else if (op == TOPO_TIME_COPY) by = 2.0 * l.ndof * 8.0;
+    else if (op == TOPO_TIME_ENERGY) by = (double)S.ndof * 8 + (double)S.nel * 4 * 4;
+    else if (op == TOPO_TIME_FILTER) by = (double)S.nel * 4 * 3;
+    run();
+    C.sync();
+    TOPO_HIP(hipEventRecord(e0, C.s));
+    for (int i = 0; i < reps; i++) run();
+    TOPO_HIP(hipEventRecord(e1, C.s));
+    TOPO_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    TOPO_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = ms / reps;
+    if (bytes) *bytes = by;
+    return TOPO_OK;
+    TOPO_API_END
+}
+
+int topo_solver_cancel(topo_solver* h) {
+    if (!h) return TOPO_E_ARG;
+    h->s.cancel_flag = 1;
+    return TOPO_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// WASM-op twins. One process-wide context, serialised by a mutex (the
+// AssemblyScript exports are single-threaded and not re-entrant either).
+#include <mutex>
+
+namespace topo {
+
+struct TwinCtx {
+    std::unique_ptr<Ctx> c;
+    DevKe K0;
+    double last_ke[576];
+    bool have_ke = false;
+    // structured-grid check cache: pointer, size and a sample of entries
+    const int32_t* ed_ptr = nullptr;
+    int64_t ed_ndof = 0;
+    int32_t ed_sample[64];
+    Grid ed_grid;
+
+    void ensure() {
+        if (!c) c.reset(new Ctx());
+    }
+    void set_ke(const double* ke) {
+        if (!have_ke || std::memcmp(ke, last_ke, sizeof last_ke) != 0) {
+            K0.set(*c, ke);
+            std::memcpy(last_ke, ke, sizeof last_ke);
+            have_ke = true;
+        }
+    }
+    Grid grid_of(const int32_t* ed, int64_t ndof) {
+        TOPO_CHECK(ed, TOPO_E_ARG, "edofs is null");
+        int32_t smp[64];
+        bool cached = ed == ed_ptr && ndof == ed_ndof;
+        if (cached) {
+            const int64_t nel = ed_grid.nel();
+            for (int i = 0; i < 64; i++) {
+                smp[i] = ed[(int64_t)(((double)i / 63.0) * (nel * 24 - 1))];
+                if (smp[i] != ed_sample[i]) cached = false;
+            }
+        }
+        if (cached) return ed_grid;
+        Grid gg;
+        TOPO_CHECK(edofs_structured(ed, ndof, &gg), TOPO_E_UNSTRUCTURED,
+                   "edofs is not a structured hex8 grid in the reference numbering");
+        ed_ptr = ed;
+        ed_ndof = ndof;
+        ed_grid = gg;
+        const int64_t nel = gg.nel();
+        for (int i = 0; i < 64; i++) ed_sample[i] = ed[(int64_t)(((double)i / 63.0) * (nel * 24 - 1))];
+        return gg;
+    }
+};
+
+static std::mutex g_twin_mu;
+static TwinCtx g_twin;
+
+// E in zyx order with inactive elements (not in `active`) set to 0
+static std::vector<double> masked_E_zyx(const Grid& gg, const double* evals, const int32_t* active, int32_t cnt) {
+    const int64_t nel = gg.nel();
+    std::vector<double> Ez(nel, 0.0);
+    for (int32_t a = 0; a < cnt; a++) {
+        const int64_t e = active[a];
+        TOPO_CHECK(e >= 0 && e < nel, TOPO_E_ARG, "active element %lld out of range", (long long)e);
+        const int64_t ex = e % gg.nelx, t = e / gg.nelx, ey = t % gg.nely, ez = t / gg.nely;
+        Ez[(ex * gg.nely + ey) * gg.nelz + ez] = evals[e];
+    }
+    return Ez;
+}
+
+}  // namespace topo
+
+extern "C" {
+
+int topo_applyAEbe3D(const double* ke, const int32_t* edofs, const double* evals, const int32_t* active,
+                     int32_t activeCount, const double* p, double* Ap, int32_t ndof, double* scratch) {
+    (void)scratch;
+    TOPO_API_BEGIN
+    TOPO_CHECK(ke && evals && p && Ap && (activeCount == 0 || active) && activeCount >= 0 && ndof > 0, TOPO_E_ARG,
+               "topo_applyAEbe3D: bad argument");
+    std::lock_guard<std::mutex> lk(g_twin_mu);
+    TwinCtx& T = g_twin;
+    T.ensure();
+    Ctx& C = *T.c;
+    const Grid gg = T.grid_of(edofs, ndof);
+    T.set_ke(ke);
+    std::vector<double> Ez = masked_E_zyx(gg, evals, active, activeCount);
+    double* dE = C.alloc<double>(gg.nel());
+    double* dp = C.alloc<double>(ndof);
+    double* dAp = C.alloc<double>(ndof);
+    TOPO_HIP(hipMemcpyAsync(dE, Ez.data(), Ez.size() * sizeof(double), hipMemcpyHostToDevice, C.s));
+    TOPO_HIP(hipMemcpyAsync(dp, p, ndof * sizeof(double), hipMemcpyHostToDevice, C.s));
+    hipLaunchKernelGGL((k_applyA_exact<double, double>), dim3(nblk(gg.nnodes())), dim3(256), 0, C.s, (const double*)dp,
+                       dAp, (const double*)dE, 0.0, gg, (const uint8_t*)nullptr, 0, (const double*)nullptr,
+                       (const double*)nullptr, 0.0, (const int*)nullptr, (const double*)T.K0.ke);
+    C.check_launch();
+    TOPO_HIP(hipMemcpyAsync(Ap, dAp, ndof * sizeof(double), hipMemcpyDeviceToHost, C.s));
+    C.sync();
+    C.free_one(dE);
+    C.free_one(dp);
+    C.free_one(dAp);
+    return TOPO_OK;
+    TOPO_API_END
+}
+
+int topo_ebePCG(const double* dens, const double* ke, const int32_t* edofs, const double* Fh, double* Uh,
+                const int32_t* freedofs, int32_t nel, int32_t edofSize, int32_t ndof, int32_t nfree, double Emin,
+                double E0, double penal, int32_t maxIter, double tolerance, double* work) {
+    (void)work;
+    TOPO_API_BEGIN
+    TOPO_CHECK(dens && ke && Fh && Uh && (nfree == 0 || freedofs) && ndof > 0 && nel > 0 && nfree >= 0 && nfree <= ndof,
+               TOPO_E_ARG, "topo_ebePCG: bad argument");
+    TOPO_CHECK(edofSize == 24, TOPO_E_UNSUPPORTED, "topo_ebePCG: edofSize %d (only the 3-D hex8 path, 24)", edofSize);
+    std::lock_guard<std::mutex> lk(g_twin_mu);
+    TwinCtx& T = g_twin;
+    T.ensure();
+    Ctx& C = *T.c;
+    const Grid gg = T.grid_of(edofs, ndof);
+    TOPO_CHECK(gg.nel() == nel, TOPO_E_ARG, "nel %d does not match edofs (%lld)", nel, (long long)gg.nel());
+    T.set_ke(ke);
+    std::vector<uint8_t> fixedh(ndof, 1);
+    for (int32_t i = 0; i < nfree; i++) {
+        TOPO_CHECK(freedofs[i] >= 0 && freedofs[i] < ndof, TOPO_E_ARG, "freedofs entry out of range");
+        fixedh[freedofs[i]] = 0;
+    }
+    // densities to zyx order
+    std::vector<double> dz(nel);
+    for (int64_t e = 0; e < nel; e++) {
+        const int64_t ex = e % gg.nelx, t = e / gg.nelx, ey = t % gg.nely, ez = t / gg.nely;
+        dz[(ex * gg.nely + ey) * gg.nelz + ez] = dens[e];
+    }
+    std::vector<void*> tmp;
+    auto A = [&](size_t bytes) {
+        void* q = C.alloc<uint8_t>((int64_t)bytes);
+        tmp.push_back(q);
+        return q;
+    };
+    uint8_t* dfixed = (uint8_t*)A(ndof);
+    double* ddens = (double*)A(nel * sizeof(double));
+    double* dE = (double*)A(nel * sizeof(double));
+    double* dF = (double*)A(ndof * sizeof(double));
+    double* dU = (double*)A(ndof * sizeof(double));
+    CGVecs v;
+    v.r = A(ndof * sizeof(double)); v.z = A(ndof * sizeof(double));
+    v.p = A(ndof * sizeof(double)); v.Ap = A(ndof * sizeof(double));
+    TOPO_HIP(hipMemcpyAsync(dfixed, fixedh.data(), ndof, hipMemcpyHostToDevice, C.s));
+    TOPO_HIP(hipMemcpyAsync(ddens, dz.data(), nel * sizeof(double), hipMemcpyHostToDevice, C.s));
+    TOPO_HIP(hipMemcpyAsync(dF, Fh, ndof * sizeof(double), hipMemcpyHostToDevice, C.s));
+    TOPO_HIP(hipMemcpyAsync(dU, Uh, ndof * sizeof(double), hipMemcpyHostToDevice, C.s));
+    // E via Math.pow (assembly/index.ts:541-550)
+    hipLaunchKernelGGL(k_moduli<double>, dim3(nblk(nel)), dim3(256), 0, C.s, (const double*)ddens, penal, E0, Emin, 1, dE,
+                       (float*)nullptr, (int64_t)nel);
+    // the warm start reads only free DOFs (Uf = U[free], :583-587)
+    hipLaunchKernelGGL(k_mask<double>, dim3(nblk(ndof)), dim3(256), 0, C.s, dU, (const uint8_t*)dfixed, (int64_t)ndof);
+    MGH mg;
+    mg.K0 = &T.K0;
+    mg.thr = Emin * 1000.0;
+    mg.build(C, gg, dfixed, 1, false);
+    mg.L[0].E = dE;
+    mg.update(C);
+    const int cgMax = maxIter < nfree ? maxIter : nfree;
+    CGResult r = pcg_jacobi<double, double>(C, mg, dF, dU, true, v, tolerance, cgMax, nullptr);
+    TOPO_HIP(hipMemcpyAsync(Uh, dU, ndof * sizeof(double), hipMemcpyDeviceToHost, C.s));
+    C.sync();
+    for (MGLevel& l : mg.L) {
+        if (l.fixed != dfixed) C.free_one(l.fixed);
+        C.free_one(l.invD); C.free_one(l.b); C.free_one(l.x); C.free_one(l.x2); C.free_one(l.res);
+    }
+    for (void* q : tmp) C.free_one(q);
+    TOPO_CHECK(!r.nonfinite, TOPO_E_BREAKDOWN, "non-finite CG scalar");
+    return r.iters;
+    TOPO_API_END
+}
+
+int topo_ebeMGPCG(const topo_mg_level* levels, int32_t numLevels, const double* ke, const double* Fh, double* Uh,
+                  const uint8_t* fixedMask0, int32_t ndof0, int32_t maxIter, double tolerance, double* cgWork) {
+    (void)cgWork;
+    TOPO_API_BEGIN
+    TOPO_CHECK(levels && numLevels >= 1 && ke && Fh && Uh && fixedMask0 && ndof0 > 0, TOPO_E_ARG,
+               "topo_ebeMGPCG: bad argument");
+    std::lock_guard<std::mutex> lk(g_twin_mu);
+    TwinCtx& T = g_twin;
+    T.ensure();
+    Ctx& C = *T.c;
+    T.set_ke(ke);
+    std::vector<void*> tmp;
+    auto A = [&](size_t bytes) {
+        void* q = C.alloc<uint8_t>((int64_t)bytes);
+        tmp.push_back(q);
+        return q;
+    };
+    MGH mg;
+    mg.K0 = &T.K0;
+    mg.thr = 0.0;   // inactive elements carry E = 0
+    mg.omega = 0.5; mg.nu1 = 2; mg.nu2 = 2; mg.coarse_sweeps = 30;
+    for (int li = 0; li < numLevels; li++) {
+        const topo_mg_level& d = levels[li];
+        MGLevel l;
+        l.g.nelx = d.nelx; l.g.nely = d.nely; l.g.nelz = d.nelz;
+        l.ndof = l.g.ndof();
+        TOPO_CHECK(d.ndof == l.ndof && d.nel == l.g.nel(), TOPO_E_ARG, "level %d: ndof/nel inconsistent with dims", li);
+        TOPO_CHECK(li > 0 || d.ndof == ndof0, TOPO_E_ARG, "level 0 ndof != ndof0");
+        TOPO_CHECK(d.fixedMask && d.invDiag, TOPO_E_ARG, "level %d: fixedMask/invDiag missing", li);
+        l.fixed = (uint8_t*)A(l.ndof);
+        TOPO_HIP(hipMemcpyAsync(l.fixed, li == 0 ? fixedMask0 : d.fixedMask, l.ndof, hipMemcpyHostToDevice, C.s));
+        l.invD = (double*)A(l.ndof * sizeof(double));
+        TOPO_HIP(hipMemcpyAsync(l.invD, d.invDiag, l.ndof * sizeof(double), hipMemcpyHostToDevice, C.s));
+        if (d.hasDenseK) {
+            TOPO_CHECK(d.denseK, TOPO_E_ARG, "level %d: denseK missing", li);
+            l.dense = true;
+            l.K = (double*)A(l.ndof * l.ndof * sizeof(double));
+            TOPO_HIP(hipMemcpyAsync(l.K, d.denseK, l.ndof * l.ndof * sizeof(double), hipMemcpyHostToDevice, C.s));
+        } else {
+            const Grid gg = T.grid_of(d.edofs, l.ndof);
+            TOPO_CHECK(gg.nelx == d.nelx && gg.nely == d.nely && gg.nelz == d.nelz, TOPO_E_UNSTRUCTURED,
+                       "level %d: edofs does not match its dims", li);
+            TOPO_CHECK(d.evals && (d.activeCount == 0 || d.active), TOPO_E_ARG, "level %d: evals/active missing", li);
+            std::vector<double> Ez = masked_E_zyx(gg, d.evals, d.active, d.activeCount);
+            l.E = (double*)A(Ez.size() * sizeof(double));
+            TOPO_HIP(hipMemcpyAsync(l.E, Ez.data(), Ez.size() * sizeof(double), hipMemcpyHostToDevice, C.s));
+            C.sync();
+        }
+        l.b = (double*)A(l.ndof * sizeof(double));
+        l.x = (double*)A(l.ndof * sizeof(double));
+        l.x2 = (double*)A(l.ndof * sizeof(double));
+        l.res = (double*)A(l.ndof * sizeof(double));
+        mg.L.push_back(l);
+    }
+    for (int li = 1; li < numLevels; li++)
+        TOPO_CHECK(mg.L[li].g.nelx == mg.L[li - 1].g.nelx / 2 && mg.L[li].g.nely == mg.L[li - 1].g.nely / 2 &&
+                       mg.L[li].g.nelz == mg.L[li - 1].g.nelz / 2,
+                   TOPO_E_UNSUPPORTED, "level %d is not the factor-2 coarsening of level %d", li, li - 1);
+    double* dF = (double*)A(ndof0 * sizeof(double));
+    double* dU = (double*)A(ndof0 * sizeof(double));
+    CGVecs v;
+    v.r = A(ndof0 * sizeof(double)); v.z = A(ndof0 * sizeof(double));
+    v.p = A(ndof0 * sizeof(double)); v.Ap = A(ndof0 * sizeof(double));
+    TOPO_HIP(hipMemcpyAsync(dF, Fh, ndof0 * sizeof(double), hipMemcpyHostToDevice, C.s));
+    CGResult r = pcg_mg<double, double>(C, mg, dF, dU, false, v, tolerance, maxIter, nullptr);
+    TOPO_HIP(hipMemcpyAsync(Uh, dU, ndof0 * sizeof(double), hipMemcpyDeviceToHost, C.s));
+    C.sync();
+    for (void* q : tmp) C.free_one(q);
+    TOPO_CHECK(!r.nonfinite, TOPO_E_BREAKDOWN, "non-finite CG scalar");
+    return r.iters;
+    TOPO_API_END
+}
+
+}  // extern "C"

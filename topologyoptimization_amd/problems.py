@@ -1,0 +1,119 @@
+"""Problem generators: the reference's presets and the benchmark workloads.
+
+All index arithmetic uses the reference numbering (node z fastest, element x
+fastest). The cantilever, bridge and inverter workloads have no preset in the
+reference; their boundary conditions are defined here and documented with
+the seed of the random initial densities.
+"""
+import numpy as np
+
+from . import (DOF_ALL, FIX_BOTTOM_CORNERS, FIX_LEFT, LOAD_TOP_CENTER, PREC_FP64, SOLVER_MGPCG, SOLVER_PCG,
+               COARSE_CHOLESKY, FILTER_SENSITIVITY, default_config, dims, fixed_dofs, load_vector, preserved_elements)
+
+SEED = 20261203
+
+
+def splitmix64_uniform(n, seed=SEED, lo=0.05, hi=1.0):
+    """x_i = lo + (hi-lo) * ((r_i >> 11) * 2^-53), r_i the splitmix64 stream
+    of `seed`, rounded to float32, in element order e = ex + ey*nelx + ez*nelx*nely."""
+    out = np.empty(n, np.float64)
+    mask = (1 << 64) - 1
+    s = seed & mask
+    # vectorised splitmix64: state_i = seed + (i+1)*golden
+    gamma = np.uint64(0x9E3779B97F4A7C15)
+    idx = np.arange(1, n + 1, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(s) + idx * gamma
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    out[:] = (z >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+    return (lo + (hi - lo) * out).astype(np.float32)
+
+
+def node_index(nely, nelz, ix, iy, iz):
+    return (ix * (nely + 1) + iy) * (nelz + 1) + iz
+
+
+def cube_reference(n=5, **kw):
+    """Plain reference method on an n^3 cube (the reference's own preset):
+    4 bottom corners fixed in xyz, unit -z point force at the top-centre node
+    ('top-center' with forceMagnitude -1), preserved constraint/force
+    elements, volfrac 0.3, p = 3 from the first iteration, sensitivity
+    filter rmin 1.5, OC move 0.2, projection off, Jacobi-PCG FP64 tol 1e-8."""
+    cfg = default_config(n, n, n, volfrac=0.3, penal=3.0, penal_start=3.0, rmin=1.5, use_projection=0,
+                         linear_solver=SOLVER_PCG, precision=PREC_FP64, cg_tol=1e-8, max_iterations=30,
+                         filter_type=FILTER_SENSITIVITY)
+    for k, v in kw.items():
+        setattr(cfg, k, v)
+    fixed = fixed_dofs(n, n, n, FIX_BOTTOM_CORNERS, DOF_ALL)
+    F = load_vector(n, n, n, LOAD_TOP_CENTER, -1.0)
+    pres = preserved_elements(n, n, n, FIX_BOTTOM_CORNERS, LOAD_TOP_CENTER)
+    nel, _ = dims(n, n, n)
+    x0 = np.full(nel, np.float32(cfg.volfrac), np.float32)
+    x0[pres > 0] = 1.0
+    return cfg, fixed, F, pres, None, x0
+
+
+def cantilever(nelx, nely, nelz, **kw):
+    """Cantilever: x = 0 face clamped (all DOFs); -z load of total 1.0 spread
+    uniformly (float32 -1/(nely+1) per node) over the nodes of the bottom edge
+    of the x = max face (ix = nelx, iz = 0, iy = 0..nely). volfrac 0.12,
+    p = 3, sensitivity filter rmin 1.5, projection off, no preserved
+    elements."""
+    cfg = default_config(nelx, nely, nelz, volfrac=0.12, penal=3.0, penal_start=3.0, rmin=1.5, use_projection=0,
+                         max_iterations=20, cg_tol=1e-6)
+    for k, v in kw.items():
+        setattr(cfg, k, v)
+    nel, ndof = dims(nelx, nely, nelz)
+    fixed = fixed_dofs(nelx, nely, nelz, FIX_LEFT, DOF_ALL)
+    F = np.zeros(ndof, np.float32)
+    per = np.float32(-1.0 / (nely + 1))
+    for iy in range(nely + 1):
+        F[3 * node_index(nely, nelz, nelx, iy, 0) + 2] = per
+    x0 = np.full(nel, np.float32(cfg.volfrac), np.float32)
+    return cfg, fixed, F, None, None, x0
+
+
+def cantilever_c2(precision=PREC_FP64, **kw):
+    """96x48x48 cantilever, Jacobi-PCG, tol 1e-6, 20 SIMP iterations."""
+    return cantilever(96, 48, 48, linear_solver=SOLVER_PCG, precision=precision, **kw)
+
+
+def cantilever_c3(precision=PREC_FP64, nlevels=4, **kw):
+    """192x96x96 cantilever (5,447,811 DOF), 4-level MG-PCG, V(2,2) damped
+    Jacobi omega 0.6, FP64 direct solve of the coarsest level, tol 1e-6,
+    initial densities U[0.05, 1] from splitmix64(SEED)."""
+    cfg, fixed, F, pres, void, _ = cantilever(192, 96, 96, linear_solver=SOLVER_MGPCG, precision=precision,
+                                             mg_levels=nlevels, mg_omega=0.6, coarse_solver=COARSE_CHOLESKY,
+                                             galerkin_max_ndof=0, **kw)
+    nel, _ = dims(192, 96, 96)
+    return cfg, fixed, F, pres, void, splitmix64_uniform(nel)
+
+
+def bridge_c4(**kw):
+    """128x64x64 bridge: the two bottom edges along y at x = 0 and x = nelx
+    (z = 0) pinned in xyz; 4 load cases, each a -z load of total 1.0 on an
+    8x8-node patch of the top face (z = nelz) centred in y at x = 0.2, 0.4,
+    0.6, 0.8 of the span."""
+    nelx, nely, nelz = 128, 64, 64
+    cfg = default_config(nelx, nely, nelz, nload=4, volfrac=0.12, penal=3.0, penal_start=3.0, rmin=1.5,
+                         use_projection=0, max_iterations=10, cg_tol=1e-6, linear_solver=SOLVER_MGPCG)
+    for k, v in kw.items():
+        setattr(cfg, k, v)
+    nel, ndof = dims(nelx, nely, nelz)
+    fixed = np.zeros(ndof, np.uint8)
+    for ix in (0, nelx):
+        for iy in range(nely + 1):
+            n = node_index(nely, nelz, ix, iy, 0)
+            fixed[3 * n:3 * n + 3] = 1
+    F = np.zeros((4, ndof), np.float32)
+    for k, frac in enumerate((0.2, 0.4, 0.6, 0.8)):
+        cx = int(round(frac * nelx))
+        cy = nely // 2
+        per = np.float32(-1.0 / 64)
+        for ix in range(cx - 4, cx + 4):
+            for iy in range(cy - 4, cy + 4):
+                F[k, 3 * node_index(nely, nelz, ix, iy, nelz) + 2] = per
+    x0 = np.full(nel, np.float32(cfg.volfrac), np.float32)
+    return cfg, fixed, F.ravel(), None, None, x0
