@@ -97,6 +97,9 @@ typedef struct topo_config {
     double  mech_k_out;         /* output spring stiffness                             */
     int32_t rhs_tile;           /* block-PCG RHS tile (0 = autotune)                   */
     int32_t use_graphs;         /* capture MG-PCG iterations in hipGraphs (1 default)  */
+    int32_t exact_apply;        /* 1: operator applies in the reference's summation order
+                                   (FP64 results bit-identical to applyAEbe3D), 0: fast
+                                   Hadamard x-march kernel (default)                   */
     double  mg_omega_cap;       /* > 0: per level omega_eff = min(mg_omega, cap / rho),
                                    rho ~ lambda_max(D^-1 A) by 20 power iterations from
                                    a hashed vector on free DOFs; 0 = off (reference)  */

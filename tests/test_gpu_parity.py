@@ -19,11 +19,12 @@ def _solver(T, cfg, fixed=None, F=None):
 
 @pytest.mark.parametrize("dims", GRIDS)
 def test_applyA_fp64_bitexact(gpu, oracle, dims):
+    """reference-order kernel: FP64 identical to applyAEbe3D's sums"""
     T = gpu
     rng = np.random.default_rng(11)
     nx, ny, nz = dims
     nel, ndof = T.dims(*dims)
-    cfg = T.default_config(nx, ny, nz, linear_solver=T.SOLVER_PCG, precision=T.PREC_MIXED)
+    cfg = T.default_config(nx, ny, nz, linear_solver=T.SOLVER_PCG, precision=T.PREC_MIXED, exact_apply=1)
     S = _solver(T, cfg)
     x = rng.uniform(0.0, 1.0, nel).astype(np.float32)
     x[rng.uniform(size=nel) < 0.2] = 0.0           # some elements below the skip threshold
@@ -37,6 +38,36 @@ def test_applyA_fp64_bitexact(gpu, oracle, dims):
     # FP32 operator
     Ap32 = S.applyA(p, prec=T.PREC_MIXED)
     assert np.abs(Ap32 - ref).max() <= 2e-6 * np.abs(ref).max()
+    S.close()
+
+
+FAST_GRIDS = [(1, 1, 1), (3, 3, 3), (6, 4, 3), (4, 7, 5), (9, 2, 3), (20, 13, 70), (7, 40, 11), (33, 9, 130)]
+
+
+@pytest.mark.parametrize("dims", FAST_GRIDS)
+def test_applyA_fast_kernel(gpu, oracle, dims):
+    """Hadamard x-march kernel (all modes) against the oracle: FP64 within
+    1e-14 and FP32 within 2e-6 of max |Ap|; grids cover several workgroups
+    along (y,z) and x, odd sizes and nelz+1 > 64 (lane-0 wave crossings)."""
+    T = gpu
+    rng = np.random.default_rng(21)
+    nx, ny, nz = dims
+    nel, ndof = T.dims(*dims)
+    cfg = T.default_config(nx, ny, nz, linear_solver=T.SOLVER_PCG, precision=T.PREC_MIXED)
+    fixed = (rng.uniform(size=ndof) < 0.1).astype(np.uint8)
+    S = T.Solver(cfg, fixed, np.zeros(ndof, np.float32))
+    x = rng.uniform(0.0, 1.0, nel).astype(np.float32)
+    x[rng.uniform(size=nel) < 0.2] = 0.0
+    S.set_density(x, 3.0)
+    p = rng.normal(size=ndof)
+    E = 1e-9 + x.astype(np.float64) ** 3 * (1 - 1e-9)
+    act = np.flatnonzero(E > 1e-6).astype(np.int32)
+    ref = oracle.applyA(nx, ny, nz, oracle.k0(0.3), E, act, p)
+    scale = np.abs(ref).max()
+    Ap = S.applyA(p)
+    assert np.abs(Ap - ref).max() <= 1e-14 * scale, np.abs(Ap - ref).max() / scale
+    Ap32 = S.applyA(p, prec=T.PREC_MIXED)
+    assert np.abs(Ap32 - ref).max() <= 2e-6 * scale
     S.close()
 
 

@@ -1,0 +1,272 @@
+// kernels_xm.hpp -- fast matrix-free hex8 stiffness apply for gfx950.
+//
+// The element operator is applied in the Hadamard (parity) basis of the 8
+// element nodes: with H the 2x2x2 sign transform per displacement component,
+// K0 = H^T B H / 64 where B is block diagonal over the 8 reflection-parity
+// classes (cube symmetry of the isotropic unit-cube element): two 3x3
+// classes of the form (a-b) I + b 1 1^T, three 2x2 classes (a translation
+// member is zero) and three classes made of a 2x2 shear pair plus a scalar.
+// That is ~40 multiply-adds per element instead of 576, plus two butterfly
+// transforms.
+//
+// Work decomposition ("x-march"): a workgroup owns NT consecutive nodes of
+// the flattened (y,z) plane (j = iy*nnz + iz, consecutive in memory) and
+// marches along x over CX node planes. Per step it loads one node plane
+// (coalesced), stages it in LDS for the y/z neighbours, computes the element
+// plane behind it, and completes one node plane of the output:
+//   - the y/z butterflies of a node-plane face are computed once and reused
+//     by the two element planes sharing it (carried in registers);
+//   - the element's output for its trailing face is carried, in the face
+//     basis, to the next step and added before the inverse y/z butterflies;
+//   - node sums over the 4 faces around a node: the z neighbour by a lane
+//     shuffle (wave boundaries through LDS), the y neighbour through LDS.
+// Every node value is produced by exactly one thread with a fixed order of
+// additions: results are deterministic (no atomics).
+#pragma once
+#include "kernels.hpp"
+
+namespace topo {
+
+template <typename T>
+struct HadCoef {
+    T am0, b0;            // class (0,0,0): y_i = am0 v_i + b0 (v0+v1+v2)
+    T am7, b7;            // class (1,1,1)
+    T oa[3], oc[3], ob[3];  // class e_k: 2x2 [[oa, oc], [oc, ob]]
+    T ta[3], tc[3], tb[3];  // class ~e_k: shear pair 2x2
+    T tf[3];                // class ~e_k: scalar (1,1,1) member
+};
+
+// face arrays f[qy][qz][d] flattened as f[(qy*2 + qz)*3 + d]
+#define FI(qy, qz, d) (((qy) * 2 + (qz)) * 3 + (d))
+
+template <typename T>
+__device__ __forceinline__ void yz_forward(const T (&n00)[3], const T (&n01)[3], const T (&n10)[3], const T (&n11)[3],
+                                           T (&f)[12]) {
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const T s0 = n00[d] + n01[d], d0 = n00[d] - n01[d];
+        const T s1 = n10[d] + n11[d], d1 = n10[d] - n11[d];
+        f[FI(0, 0, d)] = s0 + s1;
+        f[FI(1, 0, d)] = s0 - s1;
+        f[FI(0, 1, d)] = d0 + d1;
+        f[FI(1, 1, d)] = d0 - d1;
+    }
+}
+
+// A = u^[qx=0], Bq = u^[qx=1] (in: transformed displacements; out: the
+// block-diagonal operator applied, scaled by E; translation members -> 0)
+template <typename T>
+__device__ __forceinline__ void had_block(T (&A)[12], T (&Bq)[12], const HadCoef<T>& c, T E) {
+    // class (0,0,0)
+    {
+        const T v0 = Bq[FI(0, 0, 0)], v1 = A[FI(1, 0, 1)], v2 = A[FI(0, 1, 2)];
+        const T s = c.b0 * (v0 + v1 + v2);
+        Bq[FI(0, 0, 0)] = E * (c.am0 * v0 + s);
+        A[FI(1, 0, 1)] = E * (c.am0 * v1 + s);
+        A[FI(0, 1, 2)] = E * (c.am0 * v2 + s);
+    }
+    // class (1,1,1)
+    {
+        const T v0 = A[FI(1, 1, 0)], v1 = Bq[FI(0, 1, 1)], v2 = Bq[FI(1, 0, 2)];
+        const T s = c.b7 * (v0 + v1 + v2);
+        A[FI(1, 1, 0)] = E * (c.am7 * v0 + s);
+        Bq[FI(0, 1, 1)] = E * (c.am7 * v1 + s);
+        Bq[FI(1, 0, 2)] = E * (c.am7 * v2 + s);
+    }
+    // classes e_k: members (d ascending, d != k); translation -> 0
+    {
+        T v1 = Bq[FI(1, 0, 1)], v2 = Bq[FI(0, 1, 2)];
+        Bq[FI(1, 0, 1)] = E * (c.oa[0] * v1 + c.oc[0] * v2);
+        Bq[FI(0, 1, 2)] = E * (c.oc[0] * v1 + c.ob[0] * v2);
+        v1 = Bq[FI(1, 0, 0)];
+        v2 = A[FI(1, 1, 2)];
+        Bq[FI(1, 0, 0)] = E * (c.oa[1] * v1 + c.oc[1] * v2);
+        A[FI(1, 1, 2)] = E * (c.oc[1] * v1 + c.ob[1] * v2);
+        v1 = Bq[FI(0, 1, 0)];
+        v2 = A[FI(1, 1, 1)];
+        Bq[FI(0, 1, 0)] = E * (c.oa[2] * v1 + c.oc[2] * v2);
+        A[FI(1, 1, 1)] = E * (c.oc[2] * v1 + c.ob[2] * v2);
+        A[FI(0, 0, 0)] = T(0);
+        A[FI(0, 0, 1)] = T(0);
+        A[FI(0, 0, 2)] = T(0);
+    }
+    // classes ~e_k: scalar (1,1,1) member d = k, shear pair (d ascending)
+    {
+        Bq[FI(1, 1, 0)] = E * (c.tf[0] * Bq[FI(1, 1, 0)]);
+        T v1 = A[FI(0, 1, 1)], v2 = A[FI(1, 0, 2)];
+        A[FI(0, 1, 1)] = E * (c.ta[0] * v1 + c.tc[0] * v2);
+        A[FI(1, 0, 2)] = E * (c.tc[0] * v1 + c.tb[0] * v2);
+        Bq[FI(1, 1, 1)] = E * (c.tf[1] * Bq[FI(1, 1, 1)]);
+        v1 = A[FI(0, 1, 0)];
+        v2 = Bq[FI(0, 0, 2)];
+        A[FI(0, 1, 0)] = E * (c.ta[1] * v1 + c.tc[1] * v2);
+        Bq[FI(0, 0, 2)] = E * (c.tc[1] * v1 + c.tb[1] * v2);
+        Bq[FI(1, 1, 2)] = E * (c.tf[2] * Bq[FI(1, 1, 2)]);
+        v1 = A[FI(1, 0, 0)];
+        v2 = Bq[FI(0, 0, 1)];
+        A[FI(1, 0, 0)] = E * (c.ta[2] * v1 + c.tc[2] * v2);
+        Bq[FI(0, 0, 1)] = E * (c.tc[2] * v1 + c.tb[2] * v2);
+    }
+}
+
+// dynamic LDS: s_in[(NT + H) * 3] | s_y[NT * 3] | s_w[(NT/64) * 6]
+template <int NT>
+__host__ __device__ inline size_t xm_lds_bytes(int nnz, size_t elt) {
+    return ((size_t)(NT + nnz + 1) * 3 + (size_t)NT * 3 + (NT / 64) * 6) * elt;
+}
+
+// mode as k_applyA_exact: 0 Ap, 1 masked, 2 b - Ap (free) / 0 (fixed),
+// 3 Jacobi sweep x + omega invD (b - Ap) (free) / x (fixed)
+template <typename T, typename TE, int NT>
+__global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __restrict__ out,
+                                                  const TE* __restrict__ E, double thr, Grid g,
+                                                  const uint8_t* __restrict__ fixed, int mode,
+                                                  const T* __restrict__ bvec, const T* __restrict__ invD, T omega,
+                                                  const int* gate, HadCoef<T> co, int CX) {
+    if (gate && *gate) return;
+    extern __shared__ __align__(16) unsigned char xm_smem[];
+    const int nnz = g.nnz(), nny = g.nny(), nnx = g.nnx();
+    const int nyz = nny * nnz;
+    const int H = nnz + 1;
+    T* s_in = reinterpret_cast<T*>(xm_smem);
+    T* s_y = s_in + (NT + H) * 3;
+    T* s_w = s_y + NT * 3;
+    const int t = threadIdx.x;
+    const int lane = t & 63, wv = t >> 6;
+    const int outs = NT - H;
+    const int jbase = (int)blockIdx.x * outs - H;
+    const int j = jbase + t;
+    const bool jv = (j >= 0) && (j < nyz);
+    const int iy = jv ? j / nnz : 0;
+    const int iz = jv ? j - iy * nnz : 0;
+    const bool elem_yz = jv && iy < g.nely && iz < g.nelz;
+    const bool is_out = (t >= H) && jv;
+    const int jh = jbase + NT + t;                 // halo node loaded by t < H
+    const bool jhv = (t < H) && (jh >= 0) && (jh < nyz);
+    const int X0 = (int)blockIdx.y * CX;
+    const int X1 = min(X0 + CX, nnx);
+    const int64_t plane = (int64_t)nyz * 3;
+    // element (ex, iy, iz) in zyx order: (ex*nely + iy)*nelz + iz
+    const int64_t erow = (int64_t)iy * g.nelz + iz;
+    const int64_t eplane = (int64_t)g.nely * g.nelz;
+
+    T fprev[12], carry[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) { fprev[k] = T(0); carry[k] = T(0); }
+
+    // load node plane ix into LDS and return this thread's face transform
+    auto load_face = [&](int ix, T (&f)[12]) {
+        T own[3] = {T(0), T(0), T(0)};
+        if (ix >= 0 && ix < nnx) {
+            const T* src = p + (int64_t)ix * plane;
+            if (jv) { own[0] = src[3 * j]; own[1] = src[3 * j + 1]; own[2] = src[3 * j + 2]; }
+            T h0 = T(0), h1 = T(0), h2 = T(0);
+            if (jhv) { h0 = src[3 * jh]; h1 = src[3 * jh + 1]; h2 = src[3 * jh + 2]; }
+            if (t < H) { s_in[3 * (NT + t)] = h0; s_in[3 * (NT + t) + 1] = h1; s_in[3 * (NT + t) + 2] = h2; }
+        } else if (t < H) {
+            s_in[3 * (NT + t)] = T(0); s_in[3 * (NT + t) + 1] = T(0); s_in[3 * (NT + t) + 2] = T(0);
+        }
+        s_in[3 * t] = own[0]; s_in[3 * t + 1] = own[1]; s_in[3 * t + 2] = own[2];
+        __syncthreads();
+        T n01[3], n10[3], n11[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            n01[d] = s_in[3 * (t + 1) + d];
+            n10[d] = s_in[3 * (t + nnz) + d];
+            n11[d] = s_in[3 * (t + nnz + 1) + d];
+        }
+        yz_forward<T>(own, n01, n10, n11, f);
+    };
+
+    load_face(X0 - 1, fprev);
+    for (int ex = X0 - 1; ex < X1; ex++) {
+        T A[12], Bq[12];
+        {
+            T fnew[12];
+            __syncthreads();   // everyone done reading s_in of the previous plane
+            load_face(ex + 1, fnew);
+#pragma unroll
+            for (int k = 0; k < 12; k++) {
+                A[k] = fprev[k] + fnew[k];
+                Bq[k] = fprev[k] - fnew[k];
+                fprev[k] = fnew[k];
+            }
+        }
+        T Ee = T(0);
+        if (elem_yz && ex >= 0 && ex < g.nelx) {
+            const TE ev = E[(int64_t)ex * eplane + erow];
+            if ((double)ev > thr) Ee = (T)ev;
+        }
+        had_block<T>(A, Bq, co, Ee);
+        // inverse x butterfly: face at x=0 (plane ex) and x=1 (plane ex+1)
+        T nv[12];
+#pragma unroll
+        for (int k = 0; k < 12; k++) {
+            const T f0 = A[k] + Bq[k];
+            const T f1 = A[k] - Bq[k];
+            nv[k] = carry[k] + f0;
+            carry[k] = f1;
+        }
+        // inverse y/z butterflies of the face at node plane ex -> 4 node values
+        T q00[3], q01[3], q10[3], q11[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const T a00 = nv[FI(0, 0, d)] + nv[FI(1, 0, d)];   // y = 0, qz = 0
+            const T a01 = nv[FI(0, 1, d)] + nv[FI(1, 1, d)];   // y = 0, qz = 1
+            const T a10 = nv[FI(0, 0, d)] - nv[FI(1, 0, d)];   // y = 1, qz = 0
+            const T a11 = nv[FI(0, 1, d)] - nv[FI(1, 1, d)];   // y = 1, qz = 1
+            q00[d] = a00 + a01;
+            q01[d] = a00 - a01;
+            q10[d] = a10 + a11;
+            q11[d] = a10 - a11;
+        }
+        // z neighbour: node (iy, iz) gets face (iy, iz-1)'s z=1 values
+        T r01[3], r11[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            r01[d] = __shfl_up(q01[d], 1, 64);
+            r11[d] = __shfl_up(q11[d], 1, 64);
+        }
+        if (lane == 63) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) { s_w[wv * 6 + d] = q01[d]; s_w[wv * 6 + 3 + d] = q11[d]; }
+        }
+        __syncthreads();
+        if (lane == 0) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                r01[d] = wv > 0 ? s_w[(wv - 1) * 6 + d] : T(0);
+                r11[d] = wv > 0 ? s_w[(wv - 1) * 6 + 3 + d] : T(0);
+            }
+        }
+        T c0[3], c1[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            c0[d] = q00[d] + r01[d];
+            c1[d] = q10[d] + r11[d];
+            s_y[3 * t + d] = c1[d];
+        }
+        __syncthreads();
+        if (is_out && ex >= X0) {
+            const int64_t base = (int64_t)ex * plane + 3 * (int64_t)j;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                T v = c0[d] + s_y[3 * (t - nnz) + d];
+                const int64_t i = base + d;
+                if (mode == 1) {
+                    if (fixed[i]) v = T(0);
+                } else if (mode == 2) {
+                    v = fixed[i] ? T(0) : bvec[i] - v;
+                } else if (mode == 3) {
+                    const T r = bvec[i] - v;
+                    v = fixed[i] ? p[i] : p[i] + omega * invD[i] * r;
+                }
+                out[i] = v;
+            }
+        }
+    }
+}
+
+#undef FI
+
+}  // namespace topo

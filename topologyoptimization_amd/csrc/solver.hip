@@ -16,6 +16,7 @@
 #include <type_traits>
 #include <vector>
 #include "kernels.hpp"
+#include "kernels_xm.hpp"
 
 namespace topo {
 
@@ -100,11 +101,98 @@ struct Ctx {
     }
 };
 
-// K0 in device memory (FP64 + FP32 copies)
+// Block coefficients of K0 in the Hadamard basis (see kernels_xm.hpp):
+// B = H K0 H^T / 64 with modes q = (qx,qy,qz), index 3*(4qx + 2qy + qz) + d.
+// Returns false when K0 does not have the cube-symmetric block structure
+// (then the reference-order kernel is used).
+static bool had_coefs(const double* K, HadCoef<double>* c) {
+    static const int off[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+    double Hm[24][24] = {{0}};
+    for (int qx = 0; qx < 2; qx++)
+        for (int qy = 0; qy < 2; qy++)
+            for (int qz = 0; qz < 2; qz++) {
+                const int qi = qx * 4 + qy * 2 + qz;
+                for (int m = 0; m < 8; m++) {
+                    const int sgn = ((qx * off[m][0] + qy * off[m][1] + qz * off[m][2]) & 1) ? -1 : 1;
+                    for (int d = 0; d < 3; d++) Hm[3 * qi + d][3 * m + d] = sgn;
+                }
+            }
+    double HK[24][24], B[24][24];
+    for (int i = 0; i < 24; i++)
+        for (int j = 0; j < 24; j++) {
+            double s = 0;
+            for (int k = 0; k < 24; k++) s += Hm[i][k] * K[k * 24 + j];
+            HK[i][j] = s;
+        }
+    double bmax = 0;
+    for (int i = 0; i < 24; i++)
+        for (int j = 0; j < 24; j++) {
+            double s = 0;
+            for (int k = 0; k < 24; k++) s += HK[i][k] * Hm[j][k];
+            B[i][j] = s / 64.0;
+            bmax = std::max(bmax, std::fabs(B[i][j]));
+        }
+    auto I = [](int qx, int qy, int qz, int d) { return 3 * (qx * 4 + qy * 2 + qz) + d; };
+    bool used[24][24] = {{false}};
+    const double tol = 1e-12 * bmax;
+    bool ok = true;
+    auto tri = [&](const int m[3], double& am, double& b) {
+        const double a = B[m[0]][m[0]];
+        b = B[m[0]][m[1]];
+        am = a - b;
+        for (int x = 0; x < 3; x++)
+            for (int y = 0; y < 3; y++) {
+                used[m[x]][m[y]] = true;
+                if (std::fabs(B[m[x]][m[y]] - (x == y ? a : b)) > tol) ok = false;
+            }
+    };
+    const int t0[3] = {I(1, 0, 0, 0), I(0, 1, 0, 1), I(0, 0, 1, 2)};
+    const int t7[3] = {I(0, 1, 1, 0), I(1, 0, 1, 1), I(1, 1, 0, 2)};
+    tri(t0, c->am0, c->b0);
+    tri(t7, c->am7, c->b7);
+    const int one[3][2] = {{I(1, 1, 0, 1), I(1, 0, 1, 2)}, {I(1, 1, 0, 0), I(0, 1, 1, 2)}, {I(1, 0, 1, 0), I(0, 1, 1, 1)}};
+    const int two[3][2] = {{I(0, 0, 1, 1), I(0, 1, 0, 2)}, {I(0, 0, 1, 0), I(1, 0, 0, 2)}, {I(0, 1, 0, 0), I(1, 0, 0, 1)}};
+    for (int k = 0; k < 3; k++) {
+        const int a = one[k][0], b = one[k][1];
+        c->oa[k] = B[a][a]; c->oc[k] = B[a][b]; c->ob[k] = B[b][b];
+        used[a][a] = used[a][b] = used[b][a] = used[b][b] = true;
+        if (std::fabs(B[a][b] - B[b][a]) > tol) ok = false;
+        const int a2 = two[k][0], b2 = two[k][1];
+        c->ta[k] = B[a2][a2]; c->tc[k] = B[a2][b2]; c->tb[k] = B[b2][b2];
+        used[a2][a2] = used[a2][b2] = used[b2][a2] = used[b2][b2] = true;
+        if (std::fabs(B[a2][b2] - B[b2][a2]) > tol) ok = false;
+        const int f = I(1, 1, 1, k);
+        c->tf[k] = B[f][f];
+        used[f][f] = true;
+    }
+    for (int i = 0; i < 24; i++)
+        for (int j = 0; j < 24; j++)
+            if (!used[i][j] && std::fabs(B[i][j]) > tol) ok = false;
+    return ok;
+}
+
+template <typename TD>
+static HadCoef<TD> had_cast(const HadCoef<double>& c) {
+    HadCoef<TD> r;
+    r.am0 = (TD)c.am0; r.b0 = (TD)c.b0; r.am7 = (TD)c.am7; r.b7 = (TD)c.b7;
+    for (int k = 0; k < 3; k++) {
+        r.oa[k] = (TD)c.oa[k]; r.oc[k] = (TD)c.oc[k]; r.ob[k] = (TD)c.ob[k];
+        r.ta[k] = (TD)c.ta[k]; r.tc[k] = (TD)c.tc[k]; r.tb[k] = (TD)c.tb[k];
+        r.tf[k] = (TD)c.tf[k];
+    }
+    return r;
+}
+
+// K0 in device memory (FP64 + FP32 copies) and its Hadamard-block coefficients
 struct DevKe {
     double* ke = nullptr;
     float* ke32 = nullptr;
+    HadCoef<double> hc64;
+    HadCoef<float> hc32;
+    bool had_ok = false;
     void set(Ctx& c, const double* hke) {
+        had_ok = had_coefs(hke, &hc64);
+        hc32 = had_cast<float>(hc64);
         if (!ke) {
             ke = c.alloc<double>(576);
             ke32 = c.alloc<float>(576);
@@ -250,8 +338,10 @@ struct MGH {
     std::vector<double> omegaL;
     double thr = 1e-6;   // skip threshold 1000 * Emin
     bool want32 = false;
+    bool exact = false;   // reference-order kernel everywhere
     DevKe* K0 = nullptr;
     template <typename T> const T* ke() const;
+    template <typename T> HadCoef<T> had() const;
 
     // build levels: factor-2 coarsening of MGPrecond3D.ensure
     // (js/optimizer-worker-3d.js:408-442); nlevels <= 0 = reference rule
@@ -449,9 +539,23 @@ struct MGH {
                                l.fixed, mode, b, Sel<T>::invD(l), (T)om, gate);
         } else {
             typedef typename Sel<T>::TE TE;
-            hipLaunchKernelGGL((k_applyA_exact<T, TE>), dim3(nblk(l.g.nnodes())), dim3(256), 0, c.s, in, out,
-                               (const TE*)Sel<T>::E(l), thr, l.g, l.fixed, mode, b, Sel<T>::invD(l), (T)om, gate,
-                               ke<T>());
+            constexpr int NT = 512;
+            const int H = l.g.nnz() + 1;
+            if (!exact && K0->had_ok && 2 * H <= NT) {
+                const int nyz = l.g.nny() * l.g.nnz();
+                const int gx = (nyz + (NT - H) - 1) / (NT - H);
+                int CX = (int)((int64_t)l.g.nnx() * gx / 1024);
+                CX = std::max(2, std::min(16, CX));
+                const int gy = (l.g.nnx() + CX - 1) / CX;
+                const size_t lds = xm_lds_bytes<NT>(l.g.nnz(), sizeof(T));
+                hipLaunchKernelGGL((k_applyA_xm<T, TE, NT>), dim3(gx, gy), dim3(NT), lds, c.s, in, out,
+                                   (const TE*)Sel<T>::E(l), thr, l.g, (const uint8_t*)l.fixed, mode, b,
+                                   (const T*)Sel<T>::invD(l), (T)om, gate, had<T>(), CX);
+            } else {
+                hipLaunchKernelGGL((k_applyA_exact<T, TE>), dim3(nblk(l.g.nnodes())), dim3(256), 0, c.s, in, out,
+                                   (const TE*)Sel<T>::E(l), thr, l.g, l.fixed, mode, b, Sel<T>::invD(l), (T)om, gate,
+                                   ke<T>());
+            }
         }
     }
 
@@ -499,6 +603,8 @@ struct MGH {
 
 template <> const double* MGH::ke<double>() const { return K0->ke; }
 template <> const float* MGH::ke<float>() const { return K0->ke32; }
+template <> HadCoef<double> MGH::had<double>() const { return K0->hc64; }
+template <> HadCoef<float> MGH::had<float>() const { return K0->hc32; }
 
 // ---------------------------------------------------------------------------
 // CG drivers. Host polls the device state every `check` iterations; update
@@ -754,6 +860,7 @@ struct Solver {
         mg.galerkin_max = cfg.galerkin_max_ndof;
         mg.thr = cfg.Emin * 1000.0;
         mg.omega_cap = cfg.mg_omega_cap;
+        mg.exact = cfg.exact_apply != 0;
         const int nlev = cfg.linear_solver == TOPO_SOLVER_MGPCG ? cfg.mg_levels : 1;
         mg.build(C, g, fixed, cfg.linear_solver == TOPO_SOLVER_MGPCG ? nlev : 1, mixed);
         mg.L[0].E = E;
@@ -1519,6 +1626,7 @@ int topo_ebePCG(const double* dens, const double* ke, const int32_t* edofs, cons
     MGH mg;
     mg.K0 = &T.K0;
     mg.thr = Emin * 1000.0;
+    mg.exact = true;   // reference summation order (drop-in)
     mg.build(C, gg, dfixed, 1, false);
     mg.L[0].E = dE;
     mg.update(C);
@@ -1556,6 +1664,7 @@ int topo_ebeMGPCG(const topo_mg_level* levels, int32_t numLevels, const double* 
     MGH mg;
     mg.K0 = &T.K0;
     mg.thr = 0.0;   // inactive elements carry E = 0
+    mg.exact = true;
     mg.omega = 0.5; mg.nu1 = 2; mg.nu2 = 2; mg.coarse_sweeps = 30;
     for (int li = 0; li < numLevels; li++) {
         const topo_mg_level& d = levels[li];
