@@ -559,10 +559,18 @@ static int coarse_cholesky(OrMG& mg, OrLevel& c) {
     // free DOFs with an empty row (no active element) get a unit diagonal
     for (int64_t i = 0; i < nf; i++)
         if (A[i * nf + i] <= 1e-30) { for (int64_t j = 0; j < nf; j++) A[i * nf + j] = A[j * nf + i] = 0; A[i * nf + i] = 1.0; }
+    // a pivot below 1e-11 of the DOF's diagonal marks a DOF dependent on
+    // the earlier ones (a mechanism after voided elements): it is dropped
+    // (zero row/column of the factor's inverse), as in the HIP library
+    dvec d0(nf);
+    for (int64_t j = 0; j < nf; j++) d0[j] = A[j * nf + j];
     for (int64_t j = 0; j < nf; j++) {
         double s = A[j * nf + j];
         for (int64_t k = 0; k < j; k++) s -= A[j * nf + k] * A[j * nf + k];
-        if (!(s > 0)) return -1;
+        if (!(s > 1e-11 * d0[j])) {
+            for (int64_t i = j; i < nf; i++) A[i * nf + j] = 0.0;
+            continue;
+        }
         double d = std::sqrt(s);
         A[j * nf + j] = d;
         for (int64_t i = j + 1; i < nf; i++) {
@@ -581,12 +589,12 @@ static void coarse_solve_chol(const OrLevel& c, const double* b, double* x) {
     for (int64_t i = 0; i < nf; i++) {
         double s = b[c.freed[i]];
         for (int64_t k = 0; k < i; k++) s -= c.chol[i * nf + k] * y[k];
-        y[i] = s / c.chol[i * nf + i];
+        y[i] = c.chol[i * nf + i] != 0.0 ? s / c.chol[i * nf + i] : 0.0;
     }
     for (int64_t i = nf - 1; i >= 0; i--) {
         double s = y[i];
         for (int64_t k = i + 1; k < nf; k++) s -= c.chol[k * nf + i] * y[k];
-        y[i] = s / c.chol[i * nf + i];
+        y[i] = c.chol[i * nf + i] != 0.0 ? s / c.chol[i * nf + i] : 0.0;
     }
     for (int64_t i = 0; i < c.ndof; i++) x[i] = 0.0;
     for (int64_t i = 0; i < nf; i++) x[c.freed[i]] = y[i];

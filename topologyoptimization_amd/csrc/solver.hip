@@ -281,36 +281,48 @@ __global__ void k_zero_fixed_or_all(T* __restrict__ x, int64_t n, const int* gat
     for (int64_t i = gtid(); i < n; i += gstride()) x[i] = T(0);
 }
 
-// dense SPD inverse by Gauss-Jordan without pivoting (SPD): one step k.
-// A is nf x nf in place; step k uses the pivot row k.
-__global__ void k_gj_scale_row(double* A, int64_t n, int64_t k) {
-    // pivot row scaled by 1/akk, column k of the row becomes 1/akk
+// Dense symmetric positive (semi-)definite inverse by Gauss-Jordan without
+// pivoting, one launch per step k. A pivot below 1e-11 times the DOF's
+// original diagonal marks a DOF that is (numerically) dependent on the ones
+// eliminated before it -- a mechanism of the coarse operator after voided
+// elements disconnect parts of the structure: its row and column are
+// dropped (zero in the result), so the result is the inverse on the
+// remaining subspace.
+__global__ void k_gj_scale_row(double* A, int64_t n, int64_t k, const double* d0) {
     const double akk = A[k * n + k];
-    const double inv = 1.0 / akk;
+    const bool skip = !(akk > 1e-11 * d0[k]);
+    const double inv = skip ? 0.0 : 1.0 / akk;
     for (int64_t j = gtid(); j < n; j += gstride()) {
         if (j == k) continue;
         A[k * n + j] *= inv;
     }
 }
-__global__ void k_gj_eliminate(double* A, int64_t n, int64_t k) {
-    // rows i != k: A[i][j] -= A[i][k] * A[k][j] for j != k; A[i][k] = -A[i][k]/akk
+__global__ void k_gj_eliminate(double* A, int64_t n, int64_t k, const double* d0) {
+    // rows i != k: A[i][j] -= A[i][k] * A[k][j] for j != k (row k already scaled)
     const double akk = A[k * n + k];
+    if (!(akk > 1e-11 * d0[k])) return;
     for (int64_t w = gtid(); w < n * n; w += gstride()) {
         const int64_t i = w / n, j = w % n;
         if (i == k || j == k) continue;
         A[i * n + j] -= A[i * n + k] * A[k * n + j];
     }
-    (void)akk;
 }
-__global__ void k_gj_fix_col(double* A, int64_t n, int64_t k) {
+__global__ void k_gj_fix_col(double* A, int64_t n, int64_t k, const double* d0) {
     const double akk = A[k * n + k];
+    const bool skip = !(akk > 1e-11 * d0[k]);
     for (int64_t i = gtid(); i < n; i += gstride()) {
         if (i == k) continue;
-        A[i * n + k] = -A[i * n + k] / akk;
+        A[i * n + k] = skip ? 0.0 : -A[i * n + k] / akk;
     }
 }
-__global__ void k_gj_pivot(double* A, int64_t n, int64_t k) {
-    if (gtid() == 0) A[k * n + k] = 1.0 / A[k * n + k];
+__global__ void k_gj_pivot(double* A, int64_t n, int64_t k, const double* d0) {
+    if (gtid() == 0) {
+        const double akk = A[k * n + k];
+        A[k * n + k] = (akk > 1e-11 * d0[k]) ? 1.0 / akk : 0.0;
+    }
+}
+__global__ void k_diag_copy(const double* A, int64_t n, double* d0) {
+    for (int64_t i = gtid(); i < n; i += gstride()) d0[i] = A[i * n + i];
 }
 __global__ void k_extract_free(const double* __restrict__ K, int64_t n, const int32_t* __restrict__ freeIdx, int64_t nf,
                                double* __restrict__ A) {
@@ -514,12 +526,16 @@ struct MGH {
         const int64_t nf = l.nf;
         hipLaunchKernelGGL(k_extract_free, dim3(nblk(nf * nf)), dim3(256), 0, c.s, Kd, l.ndof, l.freeIdx, nf, l.Kinv);
         hipLaunchKernelGGL(k_regularize, dim3(nblk(nf)), dim3(256), 0, c.s, l.Kinv, nf);
+        double* d0 = c.alloc<double>(nf);
+        hipLaunchKernelGGL(k_diag_copy, dim3(nblk(nf)), dim3(256), 0, c.s, (const double*)l.Kinv, nf, d0);
         for (int64_t k = 0; k < nf; k++) {
-            hipLaunchKernelGGL(k_gj_scale_row, dim3(nblk(nf)), dim3(256), 0, c.s, l.Kinv, nf, k);
-            hipLaunchKernelGGL(k_gj_eliminate, dim3(nblk(nf * nf)), dim3(256), 0, c.s, l.Kinv, nf, k);
-            hipLaunchKernelGGL(k_gj_fix_col, dim3(nblk(nf)), dim3(256), 0, c.s, l.Kinv, nf, k);
-            hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(64), 0, c.s, l.Kinv, nf, k);
+            hipLaunchKernelGGL(k_gj_scale_row, dim3(nblk(nf)), dim3(256), 0, c.s, l.Kinv, nf, k, (const double*)d0);
+            hipLaunchKernelGGL(k_gj_eliminate, dim3(nblk(nf * nf)), dim3(256), 0, c.s, l.Kinv, nf, k, (const double*)d0);
+            hipLaunchKernelGGL(k_gj_fix_col, dim3(nblk(nf)), dim3(256), 0, c.s, l.Kinv, nf, k, (const double*)d0);
+            hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(64), 0, c.s, l.Kinv, nf, k, (const double*)d0);
         }
+        c.sync();
+        c.free_one(d0);
         c.check_launch();
         if (want32)
             hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(nf * nf)), dim3(256), 0, c.s, l.Kinv32, l.Kinv, nf * nf);
