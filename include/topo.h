@@ -97,6 +97,9 @@ typedef struct topo_config {
     double  mech_k_out;         /* output spring stiffness                             */
     int32_t rhs_tile;           /* block-PCG RHS tile (0 = autotune)                   */
     int32_t use_graphs;         /* capture MG-PCG iterations in hipGraphs (1 default)  */
+    double  skip_factor;        /* elements with E <= skip_factor*Emin are left out of the
+                                   operator (reference: 1000); 0 keeps every element so K
+                                   stays SPD after voids disconnect parts of the design */
     int32_t exact_apply;        /* 1: operator applies in the reference's summation order
                                    (FP64 results bit-identical to applyAEbe3D), 0: fast
                                    Hadamard x-march kernel (default)                   */

@@ -38,6 +38,8 @@ def lib():
         _lib.or_mgpcg.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_int]
         _lib.or_mg_nlevels.argtypes = [C.c_void_p]
         _lib.or_mg_set_omega_cap.argtypes = [C.c_void_p, C.c_double]
+        _lib.or_mg_set_skip.argtypes = [C.c_void_p, C.c_double]
+        _lib.or_set_pcg_skip_factor.argtypes = [C.c_double]
         _lib.or_mg_level_data.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         _lib.or_mg_level_omega.argtypes = [C.c_void_p, C.c_int]
         _lib.or_mg_level_omega.restype = C.c_double
@@ -132,13 +134,14 @@ class MG:
     """MGPrecond3D / ebeMGPCG restatement (see topo_oracle.cpp)."""
 
     def __init__(self, nelx, nely, nelz, fixed, nu=0.3, nlevels=0, omega=0.5, nu1=2, nu2=2, coarse=0,
-                 coarse_sweeps=30, galerkin_max=3000, omega_cap=0.0):
+                 coarse_sweeps=30, galerkin_max=3000, omega_cap=0.0, skip=1e-6):
         self.ke = np.ascontiguousarray(k0(nu).ravel())
         self.fixed = np.ascontiguousarray(fixed, np.uint8)
         self.nel, self.ndof = dims(nelx, nely, nelz)
         self.h = lib().or_mg_create(nelx, nely, nelz, _p(self.fixed), _p(self.ke), nlevels, omega, nu1, nu2, coarse,
                                     coarse_sweeps, galerkin_max)
         lib().or_mg_set_omega_cap(self.h, omega_cap)
+        lib().or_mg_set_skip(self.h, skip)
 
     def level_omega(self, li):
         return lib().or_mg_level_omega(self.h, li)

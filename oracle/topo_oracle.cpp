@@ -249,13 +249,16 @@ int or_assemble_dense(int nelx, int nely, int nelz, const double* ke, const doub
 
 // ebePCG assembly/index.ts:504-733, restated on host arrays (3-D, edofSize 24).
 // E_vals via Math.pow (:544); reduced (free-DOF) space; warm start from U.
+static double g_pcg_skip_factor = 1000.0;
+int or_set_pcg_skip_factor(double f) { g_pcg_skip_factor = f; return 0; }
+
 int or_ebePCG(const double* dens, const double* ke, int nelx, int nely, int nelz, const double* F,
               double* U, const int32_t* freedofs, int nfree, double Emin, double E0, double penal,
               int maxIter, double tolerance) {
     const double EPS = 1e-12;
     const int64_t nel = (int64_t)nelx * nely * nelz;
     const int64_t ndof = 3LL * (nelx + 1) * (nely + 1) * (nelz + 1);
-    const double dE = E0 - Emin, skip = Emin * 1000.0;
+    const double dE = E0 - Emin, skip = g_pcg_skip_factor > 0 ? Emin * g_pcg_skip_factor : -1.0;
     dvec ev(nel);
     std::vector<int32_t> act;
     for (int64_t e = 0; e < nel; e++) {
@@ -337,6 +340,7 @@ struct OrMG {
     int galerkinMax = 3000;
     int coarseSolver = TOPO_COARSE_SWEEPS;
     double omegaCap = 0.0;        // build-only safeguard, 0 = reference
+    double skip = 1e-6;           // skipThreshold (Emin * 1000 in the reference)
     std::vector<double> omegaL;   // per-level effective omega
 };
 
@@ -414,7 +418,7 @@ static void set_invdiag(OrLevel& l) {
 // _updateOperatorLevel :753-796
 static void update_level0(OrMG& mg, const float* x, double penal, double E0, double Emin) {
     OrLevel& l = mg.L[0];
-    const double dE = E0 - Emin, skip = Emin * 1000;
+    const double dE = E0 - Emin, skip = mg.skip;
     l.active.clear();
     for (int64_t e = 0; e < l.nel; e++) {
         l.E[e] = Emin + pow_density((double)x[e], penal) * dE;
@@ -428,7 +432,8 @@ static void update_level0(OrMG& mg, const float* x, double penal, double E0, dou
 
 // _restrictEvalAvg :702-739 (coarse E = 2 x mean of existing children)
 static void restrict_eavg(OrMG& mg, const OrLevel& f, OrLevel& c, double Emin) {
-    const double skip = Emin * 1000;
+    (void)Emin;
+    const double skip = mg.skip;
     c.active.clear();
     int64_t idx = 0;
     for (int cz = 0; cz < c.nelz; cz++)
@@ -721,6 +726,8 @@ int or_mg_nlevels(void* h) { return (int)((OrMG*)h)->L.size(); }
 void or_mg_destroy(void* h) { delete (OrMG*)h; }
 
 // updateFromFine :558-582 (+ Cholesky of the coarsest when requested)
+int or_mg_set_skip(void* h, double skip) { ((OrMG*)h)->skip = skip; return 0; }
+
 int or_mg_update(void* h, const float* x, double penal, double E0, double Emin) {
     OrMG& mg = *(OrMG*)h;
     update_level0(mg, x, penal, E0, Emin);
@@ -989,6 +996,7 @@ int or_simp_run(const topo_config* cfg, const uint8_t* fixed, const float* F32, 
         mg = or_mg_create(nelx, nely, nelz, fixed, ke.data(), cfg->mg_levels, cfg->mg_omega, cfg->mg_nu1, cfg->mg_nu2,
                           cfg->coarse_solver, cfg->coarse_sweeps, cfg->galerkin_max_ndof);
         or_mg_set_omega_cap(mg, cfg->mg_omega_cap);
+        or_mg_set_skip(mg, cfg->skip_factor > 0 ? cfg->Emin * cfg->skip_factor : -1.0);
     }
     const double penalTarget = cfg->penal, penalStart = cfg->penal_start;
     const int contIters = cfg->continuation_iters;
@@ -1018,6 +1026,7 @@ int or_simp_run(const topo_config* cfg, const uint8_t* fixed, const float* F32, 
                                            : std::exp(std::log(1e-3) + progress * (std::log(1e-8) - std::log(1e-3)));
         int its;
         if (cfg->linear_solver == TOPO_SOLVER_PCG) {
+            or_set_pcg_skip_factor(cfg->skip_factor);
             dvec dens(xPhys.begin(), xPhys.end());
             if (!(loop > 1 && cfg->warm_start != 0)) std::fill(U.begin(), U.end(), 0.0);
             int maxI = std::min((int)freed.size(), cfg->cg_max_iter);
@@ -1055,6 +1064,7 @@ int or_simp_run(const topo_config* cfg, const uint8_t* fixed, const float* F32, 
         if (change_out) change_out[loop - 1] = change;
         if (cg_out) cg_out[loop - 1] = its;
     }
+    or_set_pcg_skip_factor(1000.0);
     std::copy(x.begin(), x.end(), x_io);
     if (U_out) std::copy(U.begin(), U.end(), U_out);
     if (mg) or_mg_destroy(mg);

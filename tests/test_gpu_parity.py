@@ -24,7 +24,9 @@ def test_applyA_fp64_bitexact(gpu, oracle, dims):
     rng = np.random.default_rng(11)
     nx, ny, nz = dims
     nel, ndof = T.dims(*dims)
-    cfg = T.default_config(nx, ny, nz, linear_solver=T.SOLVER_PCG, precision=T.PREC_MIXED, exact_apply=1)
+    # MG-PCG config: E via _powDensity (x*x*x), the reference's MG path
+    cfg = T.default_config(nx, ny, nz, linear_solver=T.SOLVER_MGPCG, mg_levels=1, precision=T.PREC_MIXED,
+                           exact_apply=1)
     S = _solver(T, cfg)
     x = rng.uniform(0.0, 1.0, nel).astype(np.float32)
     x[rng.uniform(size=nel) < 0.2] = 0.0           # some elements below the skip threshold
@@ -252,14 +254,39 @@ def test_mg_cholesky_capped_vs_oracle(gpu, oracle):
     cfg, fixed, F, pres, void, x0 = P.cantilever(nx, ny, nz, linear_solver=T.SOLVER_MGPCG, mg_levels=4, mg_omega=0.6,
                                                  coarse_solver=T.COARSE_CHOLESKY, galerkin_max_ndof=0,
                                                  mg_omega_cap=1.6, cg_tol=1e-6)
+    assert cfg.skip_factor == 0.0
     nel, ndof = T.dims(nx, ny, nz)
     x = P.splitmix64_uniform(nel)
     with T.Solver(cfg, fixed, F) as S:
         U, c, info = S.fe(x, 3.0)
-    mg = oracle.MG(nx, ny, nz, fixed, nlevels=4, omega=0.6, coarse=1, galerkin_max=0, omega_cap=1.6)
+    mg = oracle.MG(nx, ny, nz, fixed, nlevels=4, omega=0.6, coarse=1, galerkin_max=0, omega_cap=1.6, skip=-1.0)
     mg.update(x, 3.0)
     Uo, it = mg.solve(F.astype(np.float64), maxIter=400, tol=1e-6)
     co = float(F.astype(np.float64) @ Uo)
     assert it < 400 and info.cg_iters < 400
     assert abs(info.cg_iters - it) <= 2, (info.cg_iters, it)
     assert abs(c - co) <= 1e-6 * abs(co)
+
+
+def test_c3_style_simp_steps_vs_oracle(gpu, oracle):
+    """Three SIMP steps of the C3 recipe (random densities, 4-level MG-PCG,
+    omega 0.6 capped, FP64 direct coarse solve, every element kept) on a
+    32x16x16 grid: FP64 and mixed compliance against the FP64 oracle."""
+    T = gpu
+    import topologyoptimization_amd.problems as P
+    nx, ny, nz = 32, 16, 16
+    nel, ndof = T.dims(nx, ny, nz)
+    x0 = P.splitmix64_uniform(nel)
+    mk = lambda prec: P.cantilever(nx, ny, nz, linear_solver=T.SOLVER_MGPCG, mg_levels=4, mg_omega=0.6,
+                                   coarse_solver=T.COARSE_CHOLESKY, galerkin_max_ndof=0, precision=prec)
+    cfg, fixed, F, pres, void, _ = mk(T.PREC_FP64)
+    ref = oracle.simp_run(cfg, fixed, F, None, None, 3, x0)
+    for prec, tol in ((T.PREC_FP64, 1e-6), (T.PREC_MIXED, 1e-4)):
+        cfg, fixed, F, pres, void, _ = mk(prec)
+        with T.Solver(cfg, fixed, F) as S:
+            S.set_design(x0)
+            comp = np.array([S.step().compliance for _ in range(3)])
+            x = S.design()
+        rel = np.abs(comp - ref["compliance"]) / np.abs(ref["compliance"])
+        assert rel.max() <= tol, (prec, rel)
+        assert np.abs(x - ref["x"]).max() <= 1e-3

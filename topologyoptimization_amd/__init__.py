@@ -50,7 +50,7 @@ class Config(C.Structure):
         ("warm_start", C.c_int32), ("move", C.c_double), ("objective", C.c_int32),
         ("mech_in_dof", C.c_int32), ("mech_out_dof", C.c_int32),
         ("mech_k_in", C.c_double), ("mech_k_out", C.c_double),
-        ("rhs_tile", C.c_int32), ("use_graphs", C.c_int32), ("exact_apply", C.c_int32),
+        ("rhs_tile", C.c_int32), ("use_graphs", C.c_int32), ("skip_factor", C.c_double), ("exact_apply", C.c_int32),
         ("mg_omega_cap", C.c_double),
         ("reserved", C.c_int32 * 8),
     ]

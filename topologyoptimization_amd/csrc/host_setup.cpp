@@ -184,6 +184,7 @@ int topo_config_default(topo_config* c, int32_t nelx, int32_t nely, int32_t nelz
     c->mech_k_in = 1.0; c->mech_k_out = 0.001;
     c->rhs_tile = 0;
     c->use_graphs = 1;
+    c->skip_factor = 1000.0;          // skipThreshold = Emin * 1000
     return TOPO_OK;
 }
 

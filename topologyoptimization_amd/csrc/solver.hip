@@ -874,7 +874,9 @@ struct Solver {
         mg.coarse_sweeps = cfg.coarse_sweeps;
         mg.coarse_solver = cfg.coarse_solver;
         mg.galerkin_max = cfg.galerkin_max_ndof;
-        mg.thr = cfg.Emin * 1000.0;
+        // skip threshold (skipThreshold = Emin * 1000, js/optimizer-worker-3d.js:762);
+        // -1 keeps every element (E >= Emin > 0)
+        mg.thr = cfg.skip_factor > 0 ? cfg.Emin * cfg.skip_factor : -1.0;
         mg.omega_cap = cfg.mg_omega_cap;
         mg.exact = cfg.exact_apply != 0;
         const int nlev = cfg.linear_solver == TOPO_SOLVER_MGPCG ? cfg.mg_levels : 1;

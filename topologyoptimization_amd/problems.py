@@ -60,9 +60,13 @@ def cantilever(nelx, nely, nelz, **kw):
     uniformly (float32 -1/(nely+1) per node) over the nodes of the bottom edge
     of the x = max face (ix = nelx, iz = 0, iy = 0..nely). volfrac 0.12,
     p = 3, sensitivity filter rmin 1.5, projection off, no preserved
-    elements."""
+    elements. Every element stays in the operator (skip_factor 0): with
+    voided elements removed, disconnected parts make K singular and the CG
+    iterates diverge along their rigid modes. MG smoothing weights are capped
+    at 1.6 / lambda_max(D^-1 A) (damped Jacobi at omega 0.6 is unstable for
+    these density contrasts)."""
     cfg = default_config(nelx, nely, nelz, volfrac=0.12, penal=3.0, penal_start=3.0, rmin=1.5, use_projection=0,
-                         max_iterations=20, cg_tol=1e-6)
+                         max_iterations=20, cg_tol=1e-6, skip_factor=0.0, mg_omega_cap=1.6)
     for k, v in kw.items():
         setattr(cfg, k, v)
     nel, ndof = dims(nelx, nely, nelz)
@@ -98,7 +102,8 @@ def bridge_c4(**kw):
     0.6, 0.8 of the span."""
     nelx, nely, nelz = 128, 64, 64
     cfg = default_config(nelx, nely, nelz, nload=4, volfrac=0.12, penal=3.0, penal_start=3.0, rmin=1.5,
-                         use_projection=0, max_iterations=10, cg_tol=1e-6, linear_solver=SOLVER_MGPCG)
+                         use_projection=0, max_iterations=10, cg_tol=1e-6, linear_solver=SOLVER_MGPCG,
+                         skip_factor=0.0, mg_omega_cap=1.6)
     for k, v in kw.items():
         setattr(cfg, k, v)
     nel, ndof = dims(nelx, nely, nelz)
