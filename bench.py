@@ -51,8 +51,9 @@ def run_solver(T, P, grid, prec, steps, warmup, levels):
     for i in range(steps):
         info = S.step()
         infos.append(info.as_dict())
-        log("step %d prec=%d: %.1f ms cg=%d outer=%d c=%.8g" % (i, prec, info.t_total_ms, info.cg_iters,
-                                                               info.outer_iters, info.compliance))
+        log("step %d prec=%d: %.1f ms (setup %.1f solve %.1f post %.1f) cg=%d outer=%d vcyc=%d c=%.8g" % (
+            i, prec, info.t_total_ms, info.t_setup_ms, info.t_solve_ms, info.t_post_ms, info.cg_iters,
+            info.outer_iters, info.vcycles, info.compliance))
     t1 = time.perf_counter()
     return S, infos, (t1 - t0) / steps
 

@@ -211,6 +211,8 @@ typedef struct topo_step_info {
     double  max_stress;     /* max_e E_e * w_e                                */
     double  residual;       /* final relative residual of the solve           */
     double  t_solve_ms, t_total_ms;
+    double  t_setup_ms;     /* operators of all levels from the densities   */
+    double  t_post_ms;      /* energies, sensitivities, filter, OC, change  */
 } topo_step_info;
 
 /* fixedMask u8[ndof]; F float32[nload*ndof] (case-major); preserved/voidm

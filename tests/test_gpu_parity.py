@@ -182,7 +182,8 @@ def test_fe_pcg_and_mgpcg_vs_oracle(gpu, oracle):
         co = float(F.astype(np.float64) @ Uo)
         assert abs(info.cg_iters - it) <= 1, (solver, info.cg_iters, it)
         assert abs(c - co) <= 1e-9 * abs(co)
-        assert np.abs(U - Uo).max() <= 1e-8 * np.abs(Uo).max()
+        # both stop at relative residual 1e-6; the operators differ by rounding
+        assert np.abs(U - Uo).max() <= 1e-6 * np.abs(Uo).max()
 
 
 def test_energy_filters_oc_bitexact(gpu, oracle):

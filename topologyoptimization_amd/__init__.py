@@ -68,6 +68,7 @@ class StepInfo(C.Structure):
         ("compliance", C.c_double), ("change", C.c_double), ("penal", C.c_double), ("beta", C.c_double),
         ("tolerance", C.c_double), ("volume", C.c_double), ("max_stress", C.c_double),
         ("residual", C.c_double), ("t_solve_ms", C.c_double), ("t_total_ms", C.c_double),
+        ("t_setup_ms", C.c_double), ("t_post_ms", C.c_double),
     ]
 
     def as_dict(self):

@@ -1126,7 +1126,9 @@ struct Solver {
         } else if (xd != xphys) {
             TOPO_HIP(hipMemcpyAsync(xphys, xd, nel * sizeof(float), hipMemcpyDeviceToDevice, C.s));
         }
+        auto tsu = std::chrono::steady_clock::now();
         set_density_dev(xp, penal);
+        C.sync();
         auto ts = std::chrono::steady_clock::now();
         fe(loop, cfg.max_iterations, &loc);
         C.sync();
@@ -1173,6 +1175,8 @@ struct Solver {
         loc.max_stress = C.h_sc[6];
         loc.t_solve_ms = std::chrono::duration<double, std::milli>(te - ts).count();
         loc.t_total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        loc.t_setup_ms = std::chrono::duration<double, std::milli>(ts - tsu).count();
+        loc.t_post_ms = std::chrono::duration<double, std::milli>(t1 - te).count();
         if (info) *info = loc;
     }
 };
