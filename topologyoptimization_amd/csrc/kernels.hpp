@@ -143,9 +143,11 @@ __global__ void __launch_bounds__(256) k_applyA_exact(const T* __restrict__ p, T
                                                       const TE* __restrict__ E, double thr, Grid g,
                                                       const uint8_t* __restrict__ fixed, int mode,
                                                       const T* __restrict__ b, const T* __restrict__ invD,
-                                                      T omega, const int* gate, const T* __restrict__ ke) {
+                                                      const double* __restrict__ omp, const int* gate,
+                                                      const T* __restrict__ ke) {
 #pragma clang fp contract(off)
     if (gate && *gate) return;
+    const T omega = omp ? (T)*omp : T(0);
     const int nny = g.nny(), nnz = g.nnz();
     const int64_t nyz = (int64_t)nny * nnz;
     for (int64_t n = gtid(); n < g.nnodes(); n += gstride()) {
@@ -232,7 +234,9 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV], double* sh /* NV*k
         }
 }
 
-// returns true in thread 0 of the last block, with tot[] = the grid sums
+// returns true in thread 0 of the last block, with tot[] = the grid sums.
+// The last block adds the partials with all its threads: thread t takes
+// blocks t, t+256, ... in order, then the fixed wave/block tree.
 template <int NV>
 __device__ __forceinline__ bool grid_reduce(double (&v)[NV], double* partials, unsigned* ticket, double (&tot)[NV]) {
     __shared__ double sh[NV * (kRedThreads / 64)];
@@ -246,13 +250,19 @@ __device__ __forceinline__ bool grid_reduce(double (&v)[NV], double* partials, u
     }
     __syncthreads();
     if (!last) return false;
+    __threadfence();
+    double w[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        double s = 0;
+        for (unsigned bI = threadIdx.x; bI < gridDim.x; bI += blockDim.x)
+            s += __hip_atomic_load(&partials[k * kRedBlocks + bI], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        w[k] = s;
+    }
+    __syncthreads();   // sh is reused
+    block_reduce<NV>(w, sh);
     if (threadIdx.x == 0) {
-        __threadfence();
-        for (int k = 0; k < NV; k++) {
-            double s = 0;
-            for (unsigned bI = 0; bI < gridDim.x; bI++) s += __builtin_nontemporal_load(&partials[k * kRedBlocks + bI]);
-            tot[k] = s;
-        }
+        for (int k = 0; k < NV; k++) tot[k] = w[k];
         *ticket = 0u;   // ready for the next launch on this stream
         return true;
     }
@@ -609,9 +619,10 @@ __global__ void k_coarsen_E(const double* __restrict__ Ef, double* __restrict__ 
 template <typename T>
 __global__ void __launch_bounds__(256) k_dense_apply(const T* __restrict__ K, const T* __restrict__ xin, T* __restrict__ out,
                                                      int64_t n, const uint8_t* __restrict__ fixed, int mode,
-                                                     const T* __restrict__ b, const T* __restrict__ invD, T omega,
-                                                     const int* gate) {
+                                                     const T* __restrict__ b, const T* __restrict__ invD,
+                                                     const double* __restrict__ omp, const int* gate) {
     if (gate && *gate) return;
+    const T omega = omp ? (T)*omp : T(0);
     const int lane = threadIdx.x & 63;
     const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     if (row >= n) return;
@@ -1012,10 +1023,10 @@ __global__ void __launch_bounds__(kRedThreads) k_change(const float* __restrict_
     __syncthreads();
     (void)last;
     if (grid_reduce<1>(vs, partials, ticket, tot)) {
+        out2[1] = tot[0];
         double mm = 0;
         for (unsigned bI = 0; bI < gridDim.x; bI++) mm = fmax(mm, partials[2 * kRedBlocks + bI]);
         out2[0] = mm;
-        out2[1] = tot[0];
     }
 }
 

@@ -121,9 +121,11 @@ template <typename T, typename TE, int NT>
 __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __restrict__ out,
                                                   const TE* __restrict__ E, double thr, Grid g,
                                                   const uint8_t* __restrict__ fixed, int mode,
-                                                  const T* __restrict__ bvec, const T* __restrict__ invD, T omega,
-                                                  const int* gate, HadCoef<T> co, int CX) {
+                                                  const T* __restrict__ bvec, const T* __restrict__ invD,
+                                                  const double* __restrict__ omp, const int* gate, HadCoef<T> co,
+                                                  int CX) {
     if (gate && *gate) return;
+    const T omega = omp ? (T)*omp : T(0);
     extern __shared__ __align__(16) unsigned char xm_smem[];
     const int nnz = g.nnz(), nny = g.nny(), nnx = g.nnx();
     const int nyz = nny * nnz;

@@ -41,6 +41,32 @@ struct Ctx {
     double* dsc = nullptr;        // 16 device doubles of scratch
     double* h_sc = nullptr;       // pinned mirror
     std::vector<void*> allocs;
+    bool use_graphs = false;
+    struct GraphEntry {
+        std::vector<const void*> key;
+        hipGraphExec_t exec;
+    };
+    std::vector<GraphEntry> graphs;
+
+    hipGraphExec_t graph_find(const std::vector<const void*>& key) {
+        for (auto& g : graphs)
+            if (g.key == key) return g.exec;
+        return nullptr;
+    }
+    // capture `body` on the stream into a cached executable graph
+    template <typename F>
+    hipGraphExec_t graph_capture(const std::vector<const void*>& key, F&& body) {
+        hipGraphExec_t ex = graph_find(key);
+        if (ex) return ex;
+        TOPO_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        body();
+        hipGraph_t gph = nullptr;
+        TOPO_HIP(hipStreamEndCapture(s, &gph));
+        TOPO_HIP(hipGraphInstantiate(&ex, gph, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(gph);
+        graphs.push_back({key, ex});
+        return ex;
+    }
 
     Ctx() {
         int n = 0;
@@ -64,6 +90,7 @@ struct Ctx {
     }
     ~Ctx() {
         if (s) (void)hipStreamSynchronize(s);
+        for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
         for (void* p : allocs) (void)hipFree(p);
         if (h_st) (void)hipHostFree(h_st);
         if (h_oc) (void)hipHostFree(h_oc);
@@ -255,9 +282,10 @@ template <> struct Sel<float> {
 
 // x = omega * invD * b (first sweep from x = 0), gated
 template <typename T>
-__global__ void k_jacobi_first(T* __restrict__ x, const T* __restrict__ b, const T* __restrict__ invD, T omega,
-                               int64_t n, const int* gate) {
+__global__ void k_jacobi_first(T* __restrict__ x, const T* __restrict__ b, const T* __restrict__ invD,
+                               const double* __restrict__ omp, int64_t n, const int* gate) {
     if (gate && *gate) return;
+    const T omega = (T)*omp;
     for (int64_t i = gtid(); i < n; i += gstride()) x[i] = T(0) + omega * invD[i] * b[i];
 }
 // coarse direct solve with the explicit inverse of the free block:
@@ -324,6 +352,119 @@ __global__ void k_gj_pivot(double* A, int64_t n, int64_t k, const double* d0) {
 __global__ void k_diag_copy(const double* A, int64_t n, double* d0) {
     for (int64_t i = gtid(); i < n; i += gstride()) d0[i] = A[i * n + i];
 }
+// Blocked form of the same Gauss-Jordan inverse (panels of 64 pivots):
+//   D   = A_kk^-1            diagonal block, one workgroup, in LDS
+//   A_kj <- D A_kj           row panel
+//   A_ij <- A_ij - A_ik A_kj rank-64 update of everything outside the panel
+//   A_ik <- -A_ik D          column panel
+// with the same pivot-dropping rule inside D. 4 launches per panel.
+constexpr int kGJB = 64;
+
+__global__ void __launch_bounds__(256) k_bgj_diag(double* A, int64_t n, int64_t k0, int kb, const double* d0) {
+    __shared__ double D[kGJB][kGJB + 1];
+    __shared__ double rowk[kGJB], colk[kGJB];
+    __shared__ int keep;
+    const int t = threadIdx.x;
+    for (int e = t; e < kb * kb; e += 256) D[e / kb][e % kb] = A[(k0 + e / kb) * n + k0 + e % kb];
+    __syncthreads();
+    for (int p = 0; p < kb; p++) {
+        if (t == 0) keep = D[p][p] > 1e-11 * d0[k0 + p];
+        __syncthreads();
+        const double app = D[p][p];
+        if (t < kb) { rowk[t] = D[p][t]; colk[t] = D[t][p]; }
+        __syncthreads();
+        if (keep) {
+            const double inv = 1.0 / app;
+            for (int e = t; e < kb * kb; e += 256) {
+                const int i = e / kb, j = e % kb;
+                if (i == p && j == p) D[i][j] = inv;
+                else if (i == p) D[i][j] = rowk[j] * inv;
+                else if (j == p) D[i][j] = -colk[i] * inv;
+                else D[i][j] = D[i][j] - colk[i] * rowk[j] * inv;
+            }
+        } else {
+            for (int e = t; e < kb * kb; e += 256) {
+                const int i = e / kb, j = e % kb;
+                if (i == p || j == p) D[i][j] = 0.0;
+            }
+        }
+        __syncthreads();
+    }
+    for (int e = t; e < kb * kb; e += 256) A[(k0 + e / kb) * n + k0 + e % kb] = D[e / kb][e % kb];
+}
+
+// row panel: A[k0+r][j] <- sum_q D[r][q] A[k0+q][j] for j outside the panel;
+// one workgroup per 64 columns
+__global__ void __launch_bounds__(256) k_bgj_row(double* A, int64_t n, int64_t k0, int kb) {
+    __shared__ double D[kGJB][kGJB + 1];
+    __shared__ double R[kGJB][kGJB + 1];
+    const int t = threadIdx.x;
+    const int64_t j0 = (int64_t)blockIdx.x * kGJB;
+    if (j0 == k0) return;
+    const int jb = (int)min<int64_t>(kGJB, n - j0);
+    for (int e = t; e < kb * kb; e += 256) D[e / kb][e % kb] = A[(k0 + e / kb) * n + k0 + e % kb];
+    for (int e = t; e < kb * jb; e += 256) R[e / jb][e % jb] = A[(k0 + e / jb) * n + j0 + e % jb];
+    __syncthreads();
+    for (int e = t; e < kb * jb; e += 256) {
+        const int r = e / jb, c = e % jb;
+        double s = 0;
+        for (int q = 0; q < kb; q++) s += D[r][q] * R[q][c];
+        A[(k0 + r) * n + j0 + c] = s;
+    }
+}
+
+// rank-kb update of the tiles outside the panel: C[i][j] -= Acol[i][q] Arow[q][j];
+// 64x64 tile per workgroup, 4x4 outputs per thread
+__global__ void __launch_bounds__(256) k_bgj_update(double* A, int64_t n, int64_t k0, int kb) {
+    const int64_t i0 = (int64_t)blockIdx.y * kGJB, j0 = (int64_t)blockIdx.x * kGJB;
+    if (i0 == k0 || j0 == k0) return;
+    __shared__ double Cs[kGJB][kGJB + 1];   // column panel tile: rows i0.., cols k0..
+    __shared__ double Rs[kGJB][kGJB + 1];   // row panel tile: rows k0.., cols j0..
+    const int t = threadIdx.x;
+    const int ib = (int)min<int64_t>(kGJB, n - i0), jb = (int)min<int64_t>(kGJB, n - j0);
+    for (int e = t; e < kGJB * kGJB; e += 256) {
+        const int r = e / kGJB, c = e % kGJB;
+        Cs[r][c] = (r < ib && c < kb) ? A[(i0 + r) * n + k0 + c] : 0.0;
+        Rs[r][c] = (r < kb && c < jb) ? A[(k0 + r) * n + j0 + c] : 0.0;
+    }
+    __syncthreads();
+    const int tr = (t / 16) * 4, tc = (t % 16) * 4;
+    double acc[4][4] = {{0}};
+    for (int q = 0; q < kb; q++) {
+        double a[4], b[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) { a[u] = Cs[tr + u][q]; b[u] = Rs[q][tc + u]; }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) acc[u][v] += a[u] * b[v];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+            if (tr + u < ib && tc + v < jb) A[(i0 + tr + u) * n + j0 + tc + v] -= acc[u][v];
+}
+
+// column panel: A[i][k0+c] <- -sum_q A[i][k0+q] D[q][c] for i outside the panel
+__global__ void __launch_bounds__(256) k_bgj_col(double* A, int64_t n, int64_t k0, int kb) {
+    __shared__ double D[kGJB][kGJB + 1];
+    __shared__ double Cq[kGJB][kGJB + 1];
+    const int t = threadIdx.x;
+    const int64_t i0 = (int64_t)blockIdx.x * kGJB;
+    if (i0 == k0) return;
+    const int ib = (int)min<int64_t>(kGJB, n - i0);
+    for (int e = t; e < kb * kb; e += 256) D[e / kb][e % kb] = A[(k0 + e / kb) * n + k0 + e % kb];
+    for (int e = t; e < ib * kb; e += 256) Cq[e / kb][e % kb] = A[(i0 + e / kb) * n + k0 + e % kb];
+    __syncthreads();
+    for (int e = t; e < ib * kb; e += 256) {
+        const int r = e / kb, c = e % kb;
+        double s = 0;
+        for (int q = 0; q < kb; q++) s += Cq[r][q] * D[q][c];
+        A[(i0 + r) * n + k0 + c] = -s;
+    }
+}
+
 __global__ void k_extract_free(const double* __restrict__ K, int64_t n, const int32_t* __restrict__ freeIdx, int64_t nf,
                                double* __restrict__ A) {
     for (int64_t w = gtid(); w < nf * nf; w += gstride()) {
@@ -348,6 +489,7 @@ struct MGH {
     int galerkin_max = 3000;
     double omega_cap = 0.0;
     std::vector<double> omegaL;
+    double* d_omega = nullptr;    // per-level weights on the device (read by the kernels)
     double thr = 1e-6;   // skip threshold 1000 * Emin
     bool want32 = false;
     bool exact = false;   // reference-order kernel everywhere
@@ -472,8 +614,18 @@ struct MGH {
                 const double rho = power_rho(c, (int)li);
                 if (rho > 0) omegaL[li] = std::min(omega, omega_cap / rho);
             }
+        upload_omega(c);
         if (coarse_solver == TOPO_COARSE_CHOLESKY) factor_coarsest(c);
     }
+
+    void upload_omega(Ctx& c) {
+        if (!d_omega) d_omega = c.alloc<double>(16);
+        double h[16] = {0};
+        for (size_t li = 0; li < L.size() && li < 16; li++) h[li] = omegaL.empty() ? omega : omegaL[li];
+        TOPO_HIP(hipMemcpyAsync(d_omega, h, sizeof h, hipMemcpyHostToDevice, c.s));
+        c.sync();
+    }
+    const double* omp(int li) const { return d_omega ? d_omega + li : nullptr; }
 
     // smoother safeguard: rho ~ lambda_max(D^-1 A) of level li by 20 power
     // iterations from the hashed vector v_i = frac(i*2654435761/2^32) - 1/2 on
@@ -483,7 +635,7 @@ struct MGH {
         hipLaunchKernelGGL(k_power_init, dim3(nblk(l.ndof)), dim3(256), 0, c.s, l.x, (const uint8_t*)l.fixed, l.ndof,
                            c.dsc + 8);
         for (int it = 0; it < 20; it++) {
-            apply<double>(c, li, l.x, l.x2, 0, nullptr, nullptr, 0.0);
+            apply<double>(c, li, l.x, l.x2, 0, nullptr, nullptr);
             hipLaunchKernelGGL(k_power_scale, dim3(kRedBlocks), dim3(kRedThreads), 0, c.s, l.x2,
                                (const double*)l.invD, l.ndof, c.dsc + 8, c.partials, c.ticket);
             hipLaunchKernelGGL(k_power_norm, dim3(nblk(l.ndof)), dim3(256), 0, c.s, l.x, (const double*)l.x2, l.ndof,
@@ -528,11 +680,13 @@ struct MGH {
         hipLaunchKernelGGL(k_regularize, dim3(nblk(nf)), dim3(256), 0, c.s, l.Kinv, nf);
         double* d0 = c.alloc<double>(nf);
         hipLaunchKernelGGL(k_diag_copy, dim3(nblk(nf)), dim3(256), 0, c.s, (const double*)l.Kinv, nf, d0);
-        for (int64_t k = 0; k < nf; k++) {
-            hipLaunchKernelGGL(k_gj_scale_row, dim3(nblk(nf)), dim3(256), 0, c.s, l.Kinv, nf, k, (const double*)d0);
-            hipLaunchKernelGGL(k_gj_eliminate, dim3(nblk(nf * nf)), dim3(256), 0, c.s, l.Kinv, nf, k, (const double*)d0);
-            hipLaunchKernelGGL(k_gj_fix_col, dim3(nblk(nf)), dim3(256), 0, c.s, l.Kinv, nf, k, (const double*)d0);
-            hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(64), 0, c.s, l.Kinv, nf, k, (const double*)d0);
+        const int nt = (int)((nf + kGJB - 1) / kGJB);
+        for (int64_t k0 = 0; k0 < nf; k0 += kGJB) {
+            const int kb = (int)std::min<int64_t>(kGJB, nf - k0);
+            hipLaunchKernelGGL(k_bgj_diag, dim3(1), dim3(256), 0, c.s, l.Kinv, nf, k0, kb, (const double*)d0);
+            hipLaunchKernelGGL(k_bgj_row, dim3(nt), dim3(256), 0, c.s, l.Kinv, nf, k0, kb);
+            hipLaunchKernelGGL(k_bgj_update, dim3(nt, nt), dim3(256), 0, c.s, l.Kinv, nf, k0, kb);
+            hipLaunchKernelGGL(k_bgj_col, dim3(nt), dim3(256), 0, c.s, l.Kinv, nf, k0, kb);
         }
         c.sync();
         c.free_one(d0);
@@ -548,11 +702,12 @@ struct MGH {
 
     // out = A in (mode 0), masked (1), b - A in (2), Jacobi sweep (3)
     template <typename T>
-    void apply(Ctx& c, int li, const T* in, T* out, int mode, const T* b, const int* gate, double om) {
+    void apply(Ctx& c, int li, const T* in, T* out, int mode, const T* b, const int* gate) {
         MGLevel& l = L[li];
+        const double* om = mode == 3 ? omp(li) : nullptr;
         if (l.dense) {
             hipLaunchKernelGGL(k_dense_apply<T>, dim3(nblk(l.ndof * 64)), dim3(256), 0, c.s, Sel<T>::K(l), in, out, l.ndof,
-                               l.fixed, mode, b, Sel<T>::invD(l), (T)om, gate);
+                               l.fixed, mode, b, Sel<T>::invD(l), om, gate);
         } else {
             typedef typename Sel<T>::TE TE;
             constexpr int NT = 512;
@@ -566,10 +721,10 @@ struct MGH {
                 const size_t lds = xm_lds_bytes<NT>(l.g.nnz(), sizeof(T));
                 hipLaunchKernelGGL((k_applyA_xm<T, TE, NT>), dim3(gx, gy), dim3(NT), lds, c.s, in, out,
                                    (const TE*)Sel<T>::E(l), thr, l.g, (const uint8_t*)l.fixed, mode, b,
-                                   (const T*)Sel<T>::invD(l), (T)om, gate, had<T>(), CX);
+                                   (const T*)Sel<T>::invD(l), om, gate, had<T>(), CX);
             } else {
                 hipLaunchKernelGGL((k_applyA_exact<T, TE>), dim3(nblk(l.g.nnodes())), dim3(256), 0, c.s, in, out,
-                                   (const TE*)Sel<T>::E(l), thr, l.g, l.fixed, mode, b, Sel<T>::invD(l), (T)om, gate,
+                                   (const TE*)Sel<T>::E(l), thr, l.g, l.fixed, mode, b, Sel<T>::invD(l), om, gate,
                                    ke<T>());
             }
         }
@@ -590,19 +745,18 @@ struct MGH {
             return x;
         }
         const int nsweep_pre = coarsest ? nu1 + coarse_sweeps : nu1;
-        const double om = omegaL.empty() ? omega : omegaL[li];
         if (nsweep_pre > 0) {
             hipLaunchKernelGGL(k_jacobi_first<T>, dim3(nblk(l.ndof)), dim3(256), 0, c.s, x, b, (const T*)Sel<T>::invD(l),
-                               (T)om, l.ndof, gate);
+                               omp(li), l.ndof, gate);
         } else {
             hipLaunchKernelGGL(k_fill<T>, dim3(nblk(l.ndof)), dim3(256), 0, c.s, x, T(0), l.ndof, gate);
         }
         for (int s = 1; s < nsweep_pre; s++) {
-            apply<T>(c, li, x, x2, 3, b, gate, om);
+            apply<T>(c, li, x, x2, 3, b, gate);
             std::swap(x, x2);
         }
         if (coarsest) return x;
-        apply<T>(c, li, x, Sel<T>::res(l), 2, b, gate, om);
+        apply<T>(c, li, x, Sel<T>::res(l), 2, b, gate);
         MGLevel& lc = L[li + 1];
         hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(lc.g.nnodes())), dim3(256), 0, c.s, (const T*)Sel<T>::res(l),
                            Sel<T>::b(lc), l.g, lc.g, lc.fixed, gate);
@@ -610,7 +764,7 @@ struct MGH {
         hipLaunchKernelGGL(k_prolong_add<T>, dim3(nblk(l.g.nnodes())), dim3(256), 0, c.s, x, (const T*)xc, l.g, lc.g,
                            l.fixed, gate);
         for (int s = 0; s < nu2; s++) {
-            apply<T>(c, li, x, x2, 3, b, gate, om);
+            apply<T>(c, li, x, x2, 3, b, gate);
             std::swap(x, x2);
         }
         return x;
@@ -661,10 +815,10 @@ static CGResult pcg_jacobi(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVe
     if (warm) {
         // r = rhs - A U on free DOFs (U in T precision for the operator)
         if (std::is_same<T, TU>::value) {
-            mg.apply<T>(c, 0, (const T*)U, r, 2, rhs, nullptr, 0.0);
+            mg.apply<T>(c, 0, (const T*)U, r, 2, rhs, nullptr);
         } else {
             hipLaunchKernelGGL((k_convert<T, TU>), dim3(nblk(n)), dim3(256), 0, c.s, z, (const TU*)U, n);
-            mg.apply<T>(c, 0, z, r, 2, rhs, nullptr, 0.0);
+            mg.apply<T>(c, 0, z, r, 2, rhs, nullptr);
         }
         res.applyA++;
     } else {
@@ -680,6 +834,19 @@ static CGResult pcg_jacobi(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVe
     c.check_launch();
     const int* gate = &c.st->done;
     const int check = 8;
+    auto body = [&]() {
+        for (int k = 0; k < check; k++) {
+            mg.apply<T>(c, 0, p, Ap, 1, nullptr, gate);
+            hipLaunchKernelGGL(k_cg_alpha<T>, dim3(kRedBlocks), dim3(kRedThreads), 0, c.s, (const T*)p, (const T*)Ap, n,
+                               c.st, c.partials, c.ticket);
+            hipLaunchKernelGGL((k_cg_update_jacobi<T, TU>), dim3(kRedBlocks), dim3(kRedThreads), 0, c.s, U, r, z,
+                               (const T*)p, (const T*)Ap, (const T*)Sel<T>::invD(l), n, c.st, c.partials, c.ticket);
+            hipLaunchKernelGGL(k_cg_pupdate<T>, dim3(nblk(n)), dim3(256), 0, c.s, p, (const T*)z, n, c.st);
+        }
+    };
+    hipGraphExec_t ex = nullptr;
+    if (c.use_graphs)
+        ex = c.graph_capture({(const void*)&pcg_jacobi<T, TU>, U, r, z, p, Ap, rhs, &mg}, body);
     for (;;) {
         const CGState& h = c.read_st();
         if (h.done) {
@@ -688,14 +855,8 @@ static CGResult pcg_jacobi(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVe
             break;
         }
         if (cancel && *cancel) throw TopoError{TOPO_E_CANCELLED};
-        for (int k = 0; k < check; k++) {
-            mg.apply<T>(c, 0, p, Ap, 1, nullptr, gate, 0.0);
-            hipLaunchKernelGGL(k_cg_alpha<T>, dim3(kRedBlocks), dim3(kRedThreads), 0, c.s, (const T*)p, (const T*)Ap, n,
-                               c.st, c.partials, c.ticket);
-            hipLaunchKernelGGL((k_cg_update_jacobi<T, TU>), dim3(kRedBlocks), dim3(kRedThreads), 0, c.s, U, r, z,
-                               (const T*)p, (const T*)Ap, (const T*)Sel<T>::invD(l), n, c.st, c.partials, c.ticket);
-            hipLaunchKernelGGL(k_cg_pupdate<T>, dim3(nblk(n)), dim3(256), 0, c.s, p, (const T*)z, n, c.st);
-        }
+        if (ex) TOPO_HIP(hipGraphLaunch(ex, c.s));
+        else body();
         c.check_launch();
     }
     res.applyA += res.iters;
@@ -715,10 +876,10 @@ static CGResult pcg_mg(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVecs& 
     CGResult res;
     if (warm) {
         if (std::is_same<T, TU>::value) {
-            mg.apply<T>(c, 0, (const T*)U, r, 2, rhs, nullptr, 0.0);
+            mg.apply<T>(c, 0, (const T*)U, r, 2, rhs, nullptr);
         } else {
             hipLaunchKernelGGL((k_convert<T, TU>), dim3(nblk(n)), dim3(256), 0, c.s, z, (const TU*)U, n);
-            mg.apply<T>(c, 0, z, r, 2, rhs, nullptr, 0.0);
+            mg.apply<T>(c, 0, z, r, 2, rhs, nullptr);
         }
         hipLaunchKernelGGL(k_mask<TU>, dim3(nblk(n)), dim3(256), 0, c.s, U, l.fixed, n);
         res.applyA++;
@@ -738,16 +899,9 @@ static CGResult pcg_mg(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVecs& 
     c.check_launch();
     const int* gate = &c.st->done;
     const int check = 4;
-    for (;;) {
-        const CGState& h = c.read_st();
-        if (h.done) {
-            res.iters = h.iters;
-            res.nonfinite = h.nonfinite;
-            break;
-        }
-        if (cancel && *cancel) throw TopoError{TOPO_E_CANCELLED};
+    auto body = [&]() {
         for (int k = 0; k < check; k++) {
-            mg.apply<T>(c, 0, p, Ap, 1, nullptr, gate, 0.0);
+            mg.apply<T>(c, 0, p, Ap, 1, nullptr, gate);
             hipLaunchKernelGGL(k_cg_alpha<T>, dim3(kRedBlocks), dim3(kRedThreads), 0, c.s, (const T*)p, (const T*)Ap, n,
                                c.st, c.partials, c.ticket);
             hipLaunchKernelGGL((k_cg_update_ur<T, TU>), dim3(kRedBlocks), dim3(kRedThreads), 0, c.s, U, r, (const T*)p,
@@ -760,6 +914,20 @@ static CGResult pcg_mg(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVecs& 
             hipLaunchKernelGGL(k_cg_pupdate<T>, dim3(nblk(n)), dim3(256), 0, c.s, p, (const T*)z, n, c.st);
             hipLaunchKernelGGL(k_cg_check, dim3(1), dim3(1), 0, c.s, c.st);
         }
+    };
+    hipGraphExec_t ex = nullptr;
+    if (c.use_graphs)
+        ex = c.graph_capture({(const void*)&pcg_mg<T, TU>, U, r, z, p, Ap, rhs, &mg}, body);
+    for (;;) {
+        const CGState& h = c.read_st();
+        if (h.done) {
+            res.iters = h.iters;
+            res.nonfinite = h.nonfinite;
+            break;
+        }
+        if (cancel && *cancel) throw TopoError{TOPO_E_CANCELLED};
+        if (ex) TOPO_HIP(hipGraphLaunch(ex, c.s));
+        else body();
         c.check_launch();
     }
     res.applyA += res.iters;
@@ -815,6 +983,7 @@ struct Solver {
         ndof = g.ndof();
         c.reset(new Ctx());
         Ctx& C = *c;
+        C.use_graphs = cfg.use_graphs != 0;
         {
             double hke[576];
             k0_matrix(cfg.nu, hke);
@@ -1019,7 +1188,7 @@ struct Solver {
         else
             hipLaunchKernelGGL(k_mask<double>, dim3(nblk(ndof)), dim3(256), 0, C.s, Uk, fixed, ndof);
         double* r = r64;
-        mg.apply<double>(C, 0, Uk, r, 2, Fk, nullptr, 0.0);
+        mg.apply<double>(C, 0, Uk, r, 2, Fk, nullptr);
         tot.applyA++;
         hipLaunchKernelGGL((k_dot<double, double>), dim3(kRedBlocks), dim3(kRedThreads), 0, C.s, r, r, ndof, C.dsc,
                            C.partials, C.ticket);
@@ -1039,7 +1208,7 @@ struct Solver {
             hipLaunchKernelGGL(k_axpy_refine<float>, dim3(nblk(ndof)), dim3(256), 0, C.s, Uk, (const float*)d32, scale,
                                ndof);
             tot.iters += ir.iters; tot.vcycles += ir.vcycles; tot.applyA += ir.applyA;
-            mg.apply<double>(C, 0, Uk, r, 2, Fk, nullptr, 0.0);
+            mg.apply<double>(C, 0, Uk, r, 2, Fk, nullptr);
             tot.applyA++;
             hipLaunchKernelGGL((k_dot<double, double>), dim3(kRedBlocks), dim3(kRedThreads), 0, C.s, r, r, ndof, C.dsc,
                                C.partials, C.ticket);
@@ -1321,12 +1490,12 @@ int topo_op_applyA(topo_solver* h, int32_t level, int32_t prec, const double* p,
     Ctx& C = *S.c;
     if (prec == TOPO_PREC_FP64) {
         TOPO_HIP(hipMemcpyAsync(l.x, p, l.ndof * sizeof(double), hipMemcpyHostToDevice, C.s));
-        S.mg.apply<double>(C, level, l.x, l.x2, 0, nullptr, nullptr, 0.0);
+        S.mg.apply<double>(C, level, l.x, l.x2, 0, nullptr, nullptr);
         TOPO_HIP(hipMemcpyAsync(Ap, l.x2, l.ndof * sizeof(double), hipMemcpyDeviceToHost, C.s));
     } else {
         TOPO_HIP(hipMemcpyAsync(l.x, p, l.ndof * sizeof(double), hipMemcpyHostToDevice, C.s));
         hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(l.ndof)), dim3(256), 0, C.s, l.x32, l.x, l.ndof);
-        S.mg.apply<float>(C, level, l.x32, l.x232, 0, nullptr, nullptr, 0.0);
+        S.mg.apply<float>(C, level, l.x32, l.x232, 0, nullptr, nullptr);
         hipLaunchKernelGGL((k_convert<double, float>), dim3(nblk(l.ndof)), dim3(256), 0, C.s, l.x2, l.x232, l.ndof);
         TOPO_HIP(hipMemcpyAsync(Ap, l.x2, l.ndof * sizeof(double), hipMemcpyDeviceToHost, C.s));
     }
@@ -1441,8 +1610,8 @@ int topo_time_op(topo_solver* h, int32_t op, int32_t prec, int32_t level, int32_
     auto run = [&]() {
         switch (op) {
         case TOPO_TIME_APPLYA:
-            if (f32) S.mg.apply<float>(C, level, l.x32, l.x232, 0, nullptr, nullptr, 0.0);
-            else S.mg.apply<double>(C, level, l.x, l.x2, 0, nullptr, nullptr, 0.0);
+            if (f32) S.mg.apply<float>(C, level, l.x32, l.x232, 0, nullptr, nullptr);
+            else S.mg.apply<double>(C, level, l.x, l.x2, 0, nullptr, nullptr);
             break;
         case TOPO_TIME_VCYCLE:
             if (f32) S.mg.vcycle<float>(C, 0, S.mg.L[0].b32, nullptr);
@@ -1585,7 +1754,7 @@ int topo_applyAEbe3D(const double* ke, const int32_t* edofs, const double* evals
     TOPO_HIP(hipMemcpyAsync(dp, p, ndof * sizeof(double), hipMemcpyHostToDevice, C.s));
     hipLaunchKernelGGL((k_applyA_exact<double, double>), dim3(nblk(gg.nnodes())), dim3(256), 0, C.s, (const double*)dp,
                        dAp, (const double*)dE, 0.0, gg, (const uint8_t*)nullptr, 0, (const double*)nullptr,
-                       (const double*)nullptr, 0.0, (const int*)nullptr, (const double*)T.K0.ke);
+                       (const double*)nullptr, (const double*)nullptr, (const int*)nullptr, (const double*)T.K0.ke);
     C.check_launch();
     TOPO_HIP(hipMemcpyAsync(Ap, dAp, ndof * sizeof(double), hipMemcpyDeviceToHost, C.s));
     C.sync();
@@ -1721,6 +1890,7 @@ int topo_ebeMGPCG(const topo_mg_level* levels, int32_t numLevels, const double* 
         l.res = (double*)A(l.ndof * sizeof(double));
         mg.L.push_back(l);
     }
+    mg.upload_omega(C);
     for (int li = 1; li < numLevels; li++)
         TOPO_CHECK(mg.L[li].g.nelx == mg.L[li - 1].g.nelx / 2 && mg.L[li].g.nely == mg.L[li - 1].g.nely / 2 &&
                        mg.L[li].g.nelz == mg.L[li - 1].g.nelz / 2,
