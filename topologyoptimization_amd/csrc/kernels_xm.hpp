@@ -18,8 +18,9 @@
 //     by the two element planes sharing it (carried in registers);
 //   - the element's output for its trailing face is carried, in the face
 //     basis, to the next step and added before the inverse y/z butterflies;
-//   - node sums over the 4 faces around a node: the z neighbour by a lane
-//     shuffle (wave boundaries through LDS), the y neighbour through LDS.
+//   - node sums over the 4 faces around a node through one LDS exchange;
+//   - the next node plane is prefetched into registers while the current
+//     element plane is computed (2 barriers per plane).
 // Every node value is produced by exactly one thread with a fixed order of
 // additions: results are deterministic (no atomics).
 #pragma once
@@ -109,10 +110,10 @@ __device__ __forceinline__ void had_block(T (&A)[12], T (&Bq)[12], const HadCoef
     }
 }
 
-// dynamic LDS: s_in[(NT + H) * 3] | s_y[NT * 3] | s_w[(NT/64) * 6]
+// dynamic LDS: s_in[(NT + H) * 3] | s_q[NT * 9]
 template <int NT>
 __host__ __device__ inline size_t xm_lds_bytes(int nnz, size_t elt) {
-    return ((size_t)(NT + nnz + 1) * 3 + (size_t)NT * 3 + (NT / 64) * 6) * elt;
+    return ((size_t)(NT + nnz + 1) * 3 + (size_t)NT * 9) * elt;
 }
 
 // mode as k_applyA_exact: 0 Ap, 1 masked, 2 b - Ap (free) / 0 (fixed),
@@ -131,10 +132,8 @@ __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __
     const int nyz = nny * nnz;
     const int H = nnz + 1;
     T* s_in = reinterpret_cast<T*>(xm_smem);
-    T* s_y = s_in + (NT + H) * 3;
-    T* s_w = s_y + NT * 3;
+    T* s_q = s_in + (NT + H) * 3;       // per thread: q01[3] q10[3] q11[3]
     const int t = threadIdx.x;
-    const int lane = t & 63, wv = t >> 6;
     const int outs = NT - H;
     const int jbase = (int)blockIdx.x * outs - H;
     const int j = jbase + t;
@@ -152,23 +151,34 @@ __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __
     const int64_t erow = (int64_t)iy * g.nelz + iz;
     const int64_t eplane = (int64_t)g.nely * g.nelz;
 
-    T fprev[12], carry[12];
+    // register prefetch of one node plane (own node + halo node)
+    T pf[3], ph[3];
+    auto fetch = [&](int ix) {
 #pragma unroll
-    for (int k = 0; k < 12; k++) { fprev[k] = T(0); carry[k] = T(0); }
-
-    // load node plane ix into LDS and return this thread's face transform
-    auto load_face = [&](int ix, T (&f)[12]) {
-        T own[3] = {T(0), T(0), T(0)};
+        for (int d = 0; d < 3; d++) { pf[d] = T(0); ph[d] = T(0); }
         if (ix >= 0 && ix < nnx) {
             const T* src = p + (int64_t)ix * plane;
-            if (jv) { own[0] = src[3 * j]; own[1] = src[3 * j + 1]; own[2] = src[3 * j + 2]; }
-            T h0 = T(0), h1 = T(0), h2 = T(0);
-            if (jhv) { h0 = src[3 * jh]; h1 = src[3 * jh + 1]; h2 = src[3 * jh + 2]; }
-            if (t < H) { s_in[3 * (NT + t)] = h0; s_in[3 * (NT + t) + 1] = h1; s_in[3 * (NT + t) + 2] = h2; }
-        } else if (t < H) {
-            s_in[3 * (NT + t)] = T(0); s_in[3 * (NT + t) + 1] = T(0); s_in[3 * (NT + t) + 2] = T(0);
+            if (jv) {
+#pragma unroll
+                for (int d = 0; d < 3; d++) pf[d] = src[3 * j + d];
+            }
+            if (jhv) {
+#pragma unroll
+                for (int d = 0; d < 3; d++) ph[d] = src[3 * jh + d];
+            }
         }
-        s_in[3 * t] = own[0]; s_in[3 * t + 1] = own[1]; s_in[3 * t + 2] = own[2];
+    };
+    // stage the prefetched plane in LDS, prefetch plane ixn, and return this
+    // thread's face transform of the staged plane
+    auto stage_face = [&](int ixn, T (&f)[12]) {
+        T own[3] = {pf[0], pf[1], pf[2]};
+#pragma unroll
+        for (int d = 0; d < 3; d++) s_in[3 * t + d] = own[d];
+        if (t < H) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) s_in[3 * (NT + t) + d] = ph[d];
+        }
+        fetch(ixn);
         __syncthreads();
         T n01[3], n10[3], n11[3];
 #pragma unroll
@@ -180,13 +190,20 @@ __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __
         yz_forward<T>(own, n01, n10, n11, f);
     };
 
-    load_face(X0 - 1, fprev);
+    T fprev[12], carry[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) carry[k] = T(0);
+    fetch(X0 - 1);
+    stage_face(X0, fprev);
     for (int ex = X0 - 1; ex < X1; ex++) {
         T A[12], Bq[12];
         {
             T fnew[12];
-            __syncthreads();   // everyone done reading s_in of the previous plane
-            load_face(ex + 1, fnew);
+            // reads of s_in for the previous plane finished before the last
+            // barrier of the previous step; only the first step follows the
+            // pre-loop staging directly
+            if (ex == X0 - 1) __syncthreads();
+            stage_face(ex + 2, fnew);
 #pragma unroll
             for (int k = 0; k < 12; k++) {
                 A[k] = fprev[k] + fnew[k];
@@ -210,7 +227,7 @@ __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __
             carry[k] = f1;
         }
         // inverse y/z butterflies of the face at node plane ex -> 4 node values
-        T q00[3], q01[3], q10[3], q11[3];
+        T q00[3];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             const T a00 = nv[FI(0, 0, d)] + nv[FI(1, 0, d)];   // y = 0, qz = 0
@@ -218,42 +235,19 @@ __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __
             const T a10 = nv[FI(0, 0, d)] - nv[FI(1, 0, d)];   // y = 1, qz = 0
             const T a11 = nv[FI(0, 1, d)] - nv[FI(1, 1, d)];   // y = 1, qz = 1
             q00[d] = a00 + a01;
-            q01[d] = a00 - a01;
-            q10[d] = a10 + a11;
-            q11[d] = a10 - a11;
-        }
-        // z neighbour: node (iy, iz) gets face (iy, iz-1)'s z=1 values
-        T r01[3], r11[3];
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            r01[d] = __shfl_up(q01[d], 1, 64);
-            r11[d] = __shfl_up(q11[d], 1, 64);
-        }
-        if (lane == 63) {
-#pragma unroll
-            for (int d = 0; d < 3; d++) { s_w[wv * 6 + d] = q01[d]; s_w[wv * 6 + 3 + d] = q11[d]; }
+            s_q[9 * t + d] = a00 - a01;          // (y,z) = (0,1)
+            s_q[9 * t + 3 + d] = a10 + a11;      // (1,0)
+            s_q[9 * t + 6 + d] = a10 - a11;      // (1,1)
         }
         __syncthreads();
-        if (lane == 0) {
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                r01[d] = wv > 0 ? s_w[(wv - 1) * 6 + d] : T(0);
-                r11[d] = wv > 0 ? s_w[(wv - 1) * 6 + 3 + d] : T(0);
-            }
-        }
-        T c0[3], c1[3];
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            c0[d] = q00[d] + r01[d];
-            c1[d] = q10[d] + r11[d];
-            s_y[3 * t + d] = c1[d];
-        }
-        __syncthreads();
+        // node (iy, iz) = face(iy,iz).q00 + face(iy,iz-1).q01 + face(iy-1,iz).q10
+        //               + face(iy-1,iz-1).q11 (faces beyond the grid carry zeros)
         if (is_out && ex >= X0) {
             const int64_t base = (int64_t)ex * plane + 3 * (int64_t)j;
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                T v = c0[d] + s_y[3 * (t - nnz) + d];
+                T v = q00[d] + s_q[9 * (t - 1) + d];
+                v = v + (s_q[9 * (t - nnz) + 3 + d] + s_q[9 * (t - nnz - 1) + 6 + d]);
                 const int64_t i = base + d;
                 if (mode == 1) {
                     if (fixed[i]) v = T(0);

@@ -362,32 +362,41 @@ constexpr int kGJB = 64;
 
 __global__ void __launch_bounds__(256) k_bgj_diag(double* A, int64_t n, int64_t k0, int kb, const double* d0) {
     __shared__ double D[kGJB][kGJB + 1];
-    __shared__ double rowk[kGJB], colk[kGJB];
-    __shared__ int keep;
+    __shared__ double rowk[kGJB], colk[kGJB], thr[kGJB];
     const int t = threadIdx.x;
-    for (int e = t; e < kb * kb; e += 256) D[e / kb][e % kb] = A[(k0 + e / kb) * n + k0 + e % kb];
+    for (int e = t; e < kGJB * kGJB; e += 256) {
+        const int i = e >> 6, j = e & 63;
+        D[i][j] = (i < kb && j < kb) ? A[(k0 + i) * n + k0 + j] : (i == j ? 1.0 : 0.0);
+    }
+    if (t < kGJB) thr[t] = t < kb ? 1e-11 * d0[k0 + t] : 0.0;
     __syncthreads();
+    // each thread owns 16 entries of the 64x64 block: rows r0..r0+3, cols c0..c0+3
+    const int r0 = (t >> 4) * 4, c0 = (t & 15) * 4;
     for (int p = 0; p < kb; p++) {
-        if (t == 0) keep = D[p][p] > 1e-11 * d0[k0 + p];
-        __syncthreads();
         const double app = D[p][p];
-        if (t < kb) { rowk[t] = D[p][t]; colk[t] = D[t][p]; }
+        const bool keep = app > thr[p];
+        if (t < kGJB) { rowk[t] = D[p][t]; colk[t] = D[t][p]; }
         __syncthreads();
-        if (keep) {
-            const double inv = 1.0 / app;
-            for (int e = t; e < kb * kb; e += 256) {
-                const int i = e / kb, j = e % kb;
-                if (i == p && j == p) D[i][j] = inv;
-                else if (i == p) D[i][j] = rowk[j] * inv;
-                else if (j == p) D[i][j] = -colk[i] * inv;
-                else D[i][j] = D[i][j] - colk[i] * rowk[j] * inv;
+        const double inv = keep ? 1.0 / app : 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int i = r0 + u, j = c0 + v;
+                double val = D[i][j];
+                if (!keep) {
+                    if (i == p || j == p) val = 0.0;
+                } else if (i == p && j == p) {
+                    val = inv;
+                } else if (i == p) {
+                    val = rowk[j] * inv;
+                } else if (j == p) {
+                    val = -colk[i] * inv;
+                } else {
+                    val = val - colk[i] * rowk[j] * inv;
+                }
+                D[i][j] = val;
             }
-        } else {
-            for (int e = t; e < kb * kb; e += 256) {
-                const int i = e / kb, j = e % kb;
-                if (i == p || j == p) D[i][j] = 0.0;
-            }
-        }
         __syncthreads();
     }
     for (int e = t; e < kb * kb; e += 256) A[(k0 + e / kb) * n + k0 + e % kb] = D[e / kb][e % kb];
@@ -493,6 +502,8 @@ struct MGH {
     double thr = 1e-6;   // skip threshold 1000 * Emin
     bool want32 = false;
     bool exact = false;   // reference-order kernel everywhere
+    int xm_nt = 0;        // x-march workgroup size (0 auto, 512, 1024)
+    int xm_cx = 0;        // x-march planes per workgroup (0 auto)
     DevKe* K0 = nullptr;
     template <typename T> const T* ke() const;
     template <typename T> HadCoef<T> had() const;
@@ -710,18 +721,25 @@ struct MGH {
                                l.fixed, mode, b, Sel<T>::invD(l), om, gate);
         } else {
             typedef typename Sel<T>::TE TE;
-            constexpr int NT = 512;
             const int H = l.g.nnz() + 1;
+            const int NT = (xm_nt == 1024 || (xm_nt == 0 && 4 * H > 512)) ? 1024 : 512;
             if (!exact && K0->had_ok && 2 * H <= NT) {
                 const int nyz = l.g.nny() * l.g.nnz();
                 const int gx = (nyz + (NT - H) - 1) / (NT - H);
-                int CX = (int)((int64_t)l.g.nnx() * gx / 1024);
-                CX = std::max(2, std::min(16, CX));
+                int CX = xm_cx > 0 ? xm_cx : (int)((int64_t)l.g.nnx() * gx * (NT / 512) / 1024);
+                CX = std::max(2, std::min(32, CX));
                 const int gy = (l.g.nnx() + CX - 1) / CX;
-                const size_t lds = xm_lds_bytes<NT>(l.g.nnz(), sizeof(T));
-                hipLaunchKernelGGL((k_applyA_xm<T, TE, NT>), dim3(gx, gy), dim3(NT), lds, c.s, in, out,
-                                   (const TE*)Sel<T>::E(l), thr, l.g, (const uint8_t*)l.fixed, mode, b,
-                                   (const T*)Sel<T>::invD(l), om, gate, had<T>(), CX);
+                if (NT == 512) {
+                    const size_t lds = xm_lds_bytes<512>(l.g.nnz(), sizeof(T));
+                    hipLaunchKernelGGL((k_applyA_xm<T, TE, 512>), dim3(gx, gy), dim3(512), lds, c.s, in, out,
+                                       (const TE*)Sel<T>::E(l), thr, l.g, (const uint8_t*)l.fixed, mode, b,
+                                       (const T*)Sel<T>::invD(l), om, gate, had<T>(), CX);
+                } else {
+                    const size_t lds = xm_lds_bytes<1024>(l.g.nnz(), sizeof(T));
+                    hipLaunchKernelGGL((k_applyA_xm<T, TE, 1024>), dim3(gx, gy), dim3(1024), lds, c.s, in, out,
+                                       (const TE*)Sel<T>::E(l), thr, l.g, (const uint8_t*)l.fixed, mode, b,
+                                       (const T*)Sel<T>::invD(l), om, gate, had<T>(), CX);
+                }
             } else {
                 hipLaunchKernelGGL((k_applyA_exact<T, TE>), dim3(nblk(l.g.nnodes())), dim3(256), 0, c.s, in, out,
                                    (const TE*)Sel<T>::E(l), thr, l.g, l.fixed, mode, b, Sel<T>::invD(l), om, gate,
@@ -1048,6 +1066,8 @@ struct Solver {
         mg.thr = cfg.skip_factor > 0 ? cfg.Emin * cfg.skip_factor : -1.0;
         mg.omega_cap = cfg.mg_omega_cap;
         mg.exact = cfg.exact_apply != 0;
+        if (const char* e = getenv("TOPO_XM_NT")) mg.xm_nt = atoi(e);
+        if (const char* e = getenv("TOPO_XM_CX")) mg.xm_cx = atoi(e);
         const int nlev = cfg.linear_solver == TOPO_SOLVER_MGPCG ? cfg.mg_levels : 1;
         mg.build(C, g, fixed, cfg.linear_solver == TOPO_SOLVER_MGPCG ? nlev : 1, mixed);
         mg.L[0].E = E;
