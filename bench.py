@@ -67,7 +67,10 @@ def main():
     ap.add_argument("--levels", type=int, default=4)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-kernel", action="store_true")
     args = ap.parse_args()
+    if args.profile_kernel:
+        return profile_kernels(args)
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (see DESIGN.md, 'Sharding').",
               file=sys.stderr)
@@ -97,11 +100,11 @@ def main():
     value = 1.0 / t_mixed
     ach = by32 / (ms32 * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": ach, "peak": 8000.0, "unit": "GB/s", "frac": ach / 8000.0,
-            "traffic": None, "kernel": "k_applyA (FP32, fine level)",
+            "traffic": measured_traffic(grid), "kernel": "k_applyA_xm<float> (fine level)",
             "algorithmic_bytes": by32, "avg_ms": ms32}
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(grid, infos)
+        cpu = cpu_baseline(grid, infos, args.levels)
     fine_calls = np.mean([i["applyA_fine"] for i in infos])
     out = {
         "metric": "SIMP iters/s + GDOF-matvec/s at 5.4M DOF, mixed-FP32 vs FP64 MGPCG",
@@ -133,38 +136,81 @@ def main():
     print(json.dumps(out))
 
 
-def cpu_baseline(grid, infos):
-    """The oracle (single-threaded C++ restatement of the reference
-    algorithm) on the host: fine-grid applyA calls at the same grid timed for
-    about 10 s; SIMP iters/s estimated as 1 / (fine applyA per step on the
-    GPU x CPU seconds per applyA)."""
+def measured_traffic(grid):
+    """HBM bytes per fine FP32 applyA launch from the committed rocprofv3
+    counter runs (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
+    calibration), or None when no summary exists for this grid."""
+    path = os.path.join(ROOT, "profiles", "applyA_pmc_%dx%dx%d.json" % grid)
+    try:
+        with open(path) as f:
+            return json.load(f)["applyA_fp32"]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def cpu_baseline(grid, infos, levels):
+    """CPU baseline: the oracle (single-threaded C++ restatement of the
+    reference algorithm, -O2) on the host, on a bounded sample of the same
+    workload: the MG hierarchy setup for the step's densities, one V-cycle
+    and one fine applyA at the same grid, each timed once (~15 s in all).
+    SIMP iters/s = 1 / (setup + n_cg x (applyA + V-cycle)), n_cg = the mean
+    CG iteration count of the GPU steps (same algorithm and tolerance)."""
     try:
         from oracle import oracle as O
+        import topologyoptimization_amd.problems as P
         nx, ny, nz = grid
         nel, ndof = O.dims(nx, ny, nz)
+        cfg, fixed, F, _, _, _ = P.cantilever(nx, ny, nz)
+        x = P.splitmix64_uniform(nel)
         ke = O.k0(0.3)
-        rng = np.random.default_rng(0)
-        E = 1e-9 + rng.uniform(0.05, 1.0, nel) ** 3
+        E = 1e-9 + x.astype(np.float64) ** 3 * (1 - 1e-9)
         act = np.arange(nel, dtype=np.int32)
-        p = rng.normal(size=ndof)
+        p = np.random.default_rng(0).normal(size=ndof)
         t0 = time.perf_counter()
-        calls = 0
-        while True:
-            O.applyA(nx, ny, nz, ke, E, act, p)
-            calls += 1
-            if time.perf_counter() - t0 > 8.0:
-                break
-        t_call = (time.perf_counter() - t0) / calls
-        fine = float(np.mean([i["applyA_fine"] for i in infos])) * 1.0
-        # a V-cycle costs ~ (nu1+nu2+1) * 8/7 fine applyA equivalents
-        per_step = (fine + float(np.mean([i["vcycles"] for i in infos])) * 5 * 8 / 7) * t_call
+        O.applyA(nx, ny, nz, ke, E, act, p)
+        t_ap = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        mg = O.MG(nx, ny, nz, fixed, nlevels=levels, omega=0.6, coarse=1, galerkin_max=0, omega_cap=1.6, skip=-1.0)
+        mg.update(x, 3.0)
+        t_setup = time.perf_counter() - t0
+        r = F.astype(np.float64).copy()
+        r[fixed > 0] = 0
+        t0 = time.perf_counter()
+        mg.vcycle(r)
+        t_vc = time.perf_counter() - t0
+        n_cg = float(np.mean([i["cg_iters"] for i in infos]))
+        per_step = t_setup + n_cg * (t_ap + t_vc)
         return {"value": 1.0 / per_step, "unit": "SIMP iters/s", "cores": 1, "kind": "port",
-                "sample": "%d oracle applyA calls at %dx%dx%d (%.2f s each, %.2f GDOF-matvec/s); per-step cost = "
-                          "GPU step's fine applyA + 5*8/7 per V-cycle applyA-equivalents" %
-                          (calls, nx, ny, nz, t_call, ndof / t_call / 1e9),
+                "sample": "oracle at %dx%dx%d: MG setup %.2f s + 1 V-cycle %.2f s + 1 fine applyA %.2f s "
+                          "(%.4f GDOF-matvec/s), scaled by %.1f CG iterations per step" %
+                          (nx, ny, nz, t_setup, t_vc, t_ap, ndof / t_ap / 1e9, n_cg),
+                "gdof_matvec_per_s": ndof / t_ap / 1e9,
                 "host": platform.processor() or platform.machine(), "nproc": os.cpu_count()}
     except Exception as e:  # reported, never fatal
         return {"value": None, "error": repr(e)}
+
+
+def profile_kernels(args):
+    """Fixed kernel workload for rocprofv3 counter runs: 20 fine applyA in
+    FP32, 20 in FP64 and 20 plain FP64 copies (the calibration kernel) at the
+    benchmark grid."""
+    import topologyoptimization_amd as T
+    import topologyoptimization_amd.problems as P
+    grid = tuple(int(v) for v in args.grid.split(","))
+    if grid == (192, 96, 96):
+        cfg, fixed, F, pres, void, x0 = P.cantilever_c3(precision=T.PREC_MIXED, nlevels=args.levels)
+    else:
+        cfg, fixed, F, pres, void, x0 = P.cantilever(*grid, precision=T.PREC_MIXED)
+        x0 = P.splitmix64_uniform(T.dims(*grid)[0])
+    S = T.Solver(cfg, fixed, F)
+    S.set_density(x0, 3.0)
+    out = {}
+    for name, op, prec in (("applyA_fp32", T.TIME_APPLYA, T.PREC_MIXED), ("applyA_fp64", T.TIME_APPLYA, T.PREC_FP64),
+                           ("copy_fp64", T.TIME_COPY, T.PREC_FP64)):
+        ms, by = S.time_op(op, prec, 0, 20)
+        out[name] = {"avg_ms": ms, "algorithmic_bytes": by}
+    S.close()
+    print(json.dumps(out))
 
 
 if __name__ == "__main__":
