@@ -124,8 +124,17 @@ __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __
                                                   const uint8_t* __restrict__ fixed, int mode,
                                                   const T* __restrict__ bvec, const T* __restrict__ invD,
                                                   const double* __restrict__ omp, const int* gate, HadCoef<T> co,
-                                                  int CX) {
+                                                  int CX, int gx, int gy) {
     if (gate && *gate) return;
+    // XCD-aware work order: hardware places workgroup L on XCD L % 8; work
+    // item w = (L % 8) * ceil(N/8) + L / 8 gives each XCD a contiguous range
+    // of (column, x-chunk) items with the x-chunks of a column adjacent, so
+    // the node plane two chunks share is read once into that XCD's L2.
+    const int nwork = gx * gy;
+    const int per = (nwork + 7) / 8;
+    const int w = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+    if (w >= nwork) return;
+    const int bx = w / gy, by = w - bx * gy;
     const T omega = omp ? (T)*omp : T(0);
     extern __shared__ __align__(16) unsigned char xm_smem[];
     const int nnz = g.nnz(), nny = g.nny(), nnx = g.nnx();
@@ -135,7 +144,7 @@ __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __
     T* s_q = s_in + (NT + H) * 3;       // per thread: q01[3] q10[3] q11[3]
     const int t = threadIdx.x;
     const int outs = NT - H;
-    const int jbase = (int)blockIdx.x * outs - H;
+    const int jbase = bx * outs - H;
     const int j = jbase + t;
     const bool jv = (j >= 0) && (j < nyz);
     const int iy = jv ? j / nnz : 0;
@@ -144,7 +153,7 @@ __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __
     const bool is_out = (t >= H) && jv;
     const int jh = jbase + NT + t;                 // halo node loaded by t < H
     const bool jhv = (t < H) && (jh >= 0) && (jh < nyz);
-    const int X0 = (int)blockIdx.y * CX;
+    const int X0 = by * CX;
     const int X1 = min(X0 + CX, nnx);
     const int64_t plane = (int64_t)nyz * 3;
     // element (ex, iy, iz) in zyx order: (ex*nely + iy)*nelz + iz

@@ -17,6 +17,7 @@
 #include <vector>
 #include "kernels.hpp"
 #include "kernels_xm.hpp"
+#include <rocblas/rocblas.h>
 
 namespace topo {
 
@@ -47,6 +48,14 @@ struct Ctx {
         hipGraphExec_t exec;
     };
     std::vector<GraphEntry> graphs;
+    rocblas_handle blas = nullptr;
+    rocblas_handle blas_handle() {
+        if (!blas) {
+            TOPO_CHECK(rocblas_create_handle(&blas) == rocblas_status_success, TOPO_E_HIP, "rocblas_create_handle failed");
+            TOPO_CHECK(rocblas_set_stream(blas, s) == rocblas_status_success, TOPO_E_HIP, "rocblas_set_stream failed");
+        }
+        return blas;
+    }
 
     hipGraphExec_t graph_find(const std::vector<const void*>& key) {
         for (auto& g : graphs)
@@ -91,6 +100,7 @@ struct Ctx {
     ~Ctx() {
         if (s) (void)hipStreamSynchronize(s);
         for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
+        if (blas) (void)rocblas_destroy_handle(blas);
         for (void* p : allocs) (void)hipFree(p);
         if (h_st) (void)hipHostFree(h_st);
         if (h_oc) (void)hipHostFree(h_oc);
@@ -455,6 +465,19 @@ __global__ void __launch_bounds__(256) k_bgj_update(double* A, int64_t n, int64_
             if (tr + u < ib && tc + v < jb) A[(i0 + tr + u) * n + j0 + tc + v] -= acc[u][v];
 }
 
+// operands of the rank-kb update as one plain GEMM A -= C R: C = A[:, panel]
+// with the panel rows zeroed (n x kb), R = A[panel, :] with the panel
+// columns zeroed (kb x n), so rows and columns of the panel are untouched
+__global__ void k_bgj_panels(const double* __restrict__ A, int64_t n, int64_t k0, int kb, double* __restrict__ Cp,
+                             double* __restrict__ Rp) {
+    for (int64_t w = gtid(); w < n * kb; w += gstride()) {
+        const int64_t i = w / kb, q = w % kb;
+        Cp[w] = (i >= k0 && i < k0 + kb) ? 0.0 : A[i * n + k0 + q];
+        const int64_t r = w / n, j = w % n;
+        Rp[w] = (j >= k0 && j < k0 + kb) ? 0.0 : A[(k0 + r) * n + j];
+    }
+}
+
 // column panel: A[i][k0+c] <- -sum_q A[i][k0+q] D[q][c] for i outside the panel
 __global__ void __launch_bounds__(256) k_bgj_col(double* A, int64_t n, int64_t k0, int kb) {
     __shared__ double D[kGJB][kGJB + 1];
@@ -692,12 +715,35 @@ struct MGH {
         double* d0 = c.alloc<double>(nf);
         hipLaunchKernelGGL(k_diag_copy, dim3(nblk(nf)), dim3(256), 0, c.s, (const double*)l.Kinv, nf, d0);
         const int nt = (int)((nf + kGJB - 1) / kGJB);
+        // large coarse levels: the rank-64 update as a plain rocBLAS DGEMM
+        const bool use_blas = nf > 1024;
+        double *Cp = nullptr, *Rp = nullptr;
+        if (use_blas) {
+            Cp = c.alloc<double>(nf * kGJB);
+            Rp = c.alloc<double>(nf * kGJB);
+        }
         for (int64_t k0 = 0; k0 < nf; k0 += kGJB) {
             const int kb = (int)std::min<int64_t>(kGJB, nf - k0);
             hipLaunchKernelGGL(k_bgj_diag, dim3(1), dim3(256), 0, c.s, l.Kinv, nf, k0, kb, (const double*)d0);
             hipLaunchKernelGGL(k_bgj_row, dim3(nt), dim3(256), 0, c.s, l.Kinv, nf, k0, kb);
-            hipLaunchKernelGGL(k_bgj_update, dim3(nt, nt), dim3(256), 0, c.s, l.Kinv, nf, k0, kb);
+            if (use_blas) {
+                hipLaunchKernelGGL(k_bgj_panels, dim3(nblk(nf * kb)), dim3(256), 0, c.s, (const double*)l.Kinv, nf, k0,
+                                   kb, Cp, Rp);
+                // row-major A -= C R  <=>  column-major A^T -= R^T C^T
+                const double alpha = -1.0, beta = 1.0;
+                TOPO_CHECK(rocblas_dgemm(c.blas_handle(), rocblas_operation_none, rocblas_operation_none, (rocblas_int)nf,
+                                         (rocblas_int)nf, kb, &alpha, Rp, (rocblas_int)nf, Cp, kb, &beta, l.Kinv,
+                                         (rocblas_int)nf) == rocblas_status_success,
+                           TOPO_E_HIP, "rocblas_dgemm failed");
+            } else {
+                hipLaunchKernelGGL(k_bgj_update, dim3(nt, nt), dim3(256), 0, c.s, l.Kinv, nf, k0, kb);
+            }
             hipLaunchKernelGGL(k_bgj_col, dim3(nt), dim3(256), 0, c.s, l.Kinv, nf, k0, kb);
+        }
+        if (use_blas) {
+            c.sync();
+            c.free_one(Cp);
+            c.free_one(Rp);
         }
         c.sync();
         c.free_one(d0);
@@ -731,14 +777,14 @@ struct MGH {
                 const int gy = (l.g.nnx() + CX - 1) / CX;
                 if (NT == 512) {
                     const size_t lds = xm_lds_bytes<512>(l.g.nnz(), sizeof(T));
-                    hipLaunchKernelGGL((k_applyA_xm<T, TE, 512>), dim3(gx, gy), dim3(512), lds, c.s, in, out,
-                                       (const TE*)Sel<T>::E(l), thr, l.g, (const uint8_t*)l.fixed, mode, b,
-                                       (const T*)Sel<T>::invD(l), om, gate, had<T>(), CX);
+                    hipLaunchKernelGGL((k_applyA_xm<T, TE, 512>), dim3(8 * ((gx * gy + 7) / 8)), dim3(512), lds, c.s,
+                                       in, out, (const TE*)Sel<T>::E(l), thr, l.g, (const uint8_t*)l.fixed, mode, b,
+                                       (const T*)Sel<T>::invD(l), om, gate, had<T>(), CX, gx, gy);
                 } else {
                     const size_t lds = xm_lds_bytes<1024>(l.g.nnz(), sizeof(T));
-                    hipLaunchKernelGGL((k_applyA_xm<T, TE, 1024>), dim3(gx, gy), dim3(1024), lds, c.s, in, out,
-                                       (const TE*)Sel<T>::E(l), thr, l.g, (const uint8_t*)l.fixed, mode, b,
-                                       (const T*)Sel<T>::invD(l), om, gate, had<T>(), CX);
+                    hipLaunchKernelGGL((k_applyA_xm<T, TE, 1024>), dim3(8 * ((gx * gy + 7) / 8)), dim3(1024), lds, c.s,
+                                       in, out, (const TE*)Sel<T>::E(l), thr, l.g, (const uint8_t*)l.fixed, mode, b,
+                                       (const T*)Sel<T>::invD(l), om, gate, had<T>(), CX, gx, gy);
                 }
             } else {
                 hipLaunchKernelGGL((k_applyA_exact<T, TE>), dim3(nblk(l.g.nnodes())), dim3(256), 0, c.s, in, out,
