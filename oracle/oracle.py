@@ -258,3 +258,77 @@ def simp_run(cfg, fixed, F, pres, void, niter, x0):
     n = lib().or_simp_run(C.byref(cfg), _p(fixed), _p(F), _p(pres), _p(void), niter, _p(x), _p(comp), _p(change),
                           _p(cg), _p(U))
     return dict(iterations=n, x=x, compliance=comp[:n], change=change[:n], cg=cg[:n], U=U)
+
+
+# ---- painted supports / loads (pure Python; small grids) --------------------
+# FACE_NODE_INDICES js/optimizer-worker-3d.js:49-56: face -> corners of the
+# 8-node list (n0..n7 in the reference's local order).
+_FACE_NODES = ((0, 4, 7, 3), (5, 1, 2, 6), (0, 1, 5, 4), (3, 7, 6, 2), (0, 3, 2, 1), (4, 5, 6, 7))
+
+
+def _paint_node_order(nelx, nely, nelz, keys):
+    """Nodes of painted voxel faces in first-seen order (JS Set order),
+    following js/optimizer-worker-3d.js:4631-4660 / :4700-4731: key
+    "vx,vy,vz[,face]", a face outside 0..5 or absent -> all 8 corners,
+    node kept when 0 <= n < (nelx+1)(nely+1)(nelz+1)."""
+    nny, nnz = nely + 1, nelz + 1
+    maxn = (nelx + 1) * nny * nnz
+    seen, order = set(), []
+    for key in keys:
+        parts = str(key).split(",")
+        if len(parts) < 3:
+            continue
+        try:
+            vx, vy, vz = int(parts[0]), int(parts[1]), int(parts[2])
+        except ValueError:
+            continue
+        N = lambda i, j, k: i * nny * nnz + j * nnz + k  # noqa: E731
+        alln = (N(vx, vy, vz), N(vx + 1, vy, vz), N(vx + 1, vy + 1, vz), N(vx, vy + 1, vz),
+                N(vx, vy, vz + 1), N(vx + 1, vy, vz + 1), N(vx + 1, vy + 1, vz + 1), N(vx, vy + 1, vz + 1))
+        fi = -1
+        if len(parts) >= 4:
+            try:
+                fi = int(parts[3])
+            except ValueError:
+                fi = -1
+        nodes = [alln[i] for i in _FACE_NODES[fi]] if 0 <= fi < 6 else alln
+        for n in nodes:
+            if 0 <= n < maxn and n not in seen:
+                seen.add(n)
+                order.append(n)
+    return order
+
+
+def fixed_paint(nelx, nely, nelz, keys, mode="all"):
+    """getFixedDOFsFromPaint js/optimizer-worker-3d.js:4625-4666 as a mask."""
+    _, ndof = dims(nelx, nely, nelz)
+    mask = np.zeros(ndof, np.uint8)
+    for n in _paint_node_order(nelx, nely, nelz, keys):
+        for d, ch in enumerate("xyz"):
+            if mode == "all" or ch in mode:
+                mask[3 * n + d] = 1
+    return mask
+
+
+def load_paint(nelx, nely, nelz, keys, direction="down", magnitude=1.0, force_vector=None):
+    """getLoadVectorFromPaint js/optimizer-worker-3d.js:4673-4745: unit
+    direction from forceVector or the direction name, magnitude / #nodes per
+    unique node, accumulated into a float32 vector."""
+    _, ndof = dims(nelx, nely, nelz)
+    F = np.zeros(ndof, np.float32)
+    fx = fy = fz = 0.0
+    if force_vector is not None and len(force_vector) >= 3:
+        ln = float(np.sqrt(force_vector[0] ** 2 + force_vector[1] ** 2 + force_vector[2] ** 2))
+        if ln > 0:
+            fx, fy, fz = force_vector[0] / ln, force_vector[1] / ln, force_vector[2] / ln
+    else:
+        fx, fy, fz = {"up": (0, 1, 0), "left": (-1, 0, 0), "right": (1, 0, 0),
+                      "top-center": (0, 0, 1)}.get(direction, (0, -1, 0))
+    nodes = _paint_node_order(nelx, nely, nelz, keys)
+    if nodes:
+        per = magnitude / len(nodes)
+        for n in nodes:
+            F[3 * n] = np.float32(float(F[3 * n]) + fx * per)
+            F[3 * n + 1] = np.float32(float(F[3 * n + 1]) + fy * per)
+            F[3 * n + 2] = np.float32(float(F[3 * n + 2]) + fz * per)
+    return F

@@ -1,0 +1,230 @@
+"""The Node.js host (topologyoptimization_amd/node): the N-API addon, the
+reference-config -> problem translation, TopologySolver's worker protocol
+and the WASM-shaped facade. Each case runs a small script under tests/js in
+a node subprocess and checks its JSON output against the oracle."""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import topologyoptimization_amd as T
+import topologyoptimization_amd.problems as P
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+JS = os.path.join(ROOT, "tests", "js")
+NODE = shutil.which("node")
+ADDON = os.path.join(ROOT, "topologyoptimization_amd", "node", "topo_addon.node")
+
+pytestmark = pytest.mark.skipif(NODE is None, reason="node is not installed")
+
+
+def run_js(script, arg, timeout=120):
+    p = subprocess.run([NODE, os.path.join(JS, script), json.dumps(arg)], stdout=subprocess.PIPE,
+                       stderr=subprocess.PIPE, text=True, timeout=timeout)
+    assert p.returncode == 0, p.stderr[-2000:]
+    return json.loads(p.stdout.strip().splitlines()[-1])
+
+
+def _mask(idx, n):
+    m = np.zeros(n, np.uint8)
+    m[np.asarray(idx, np.int64)] = 1
+    return m
+
+
+def _dense_F(Fd, n):
+    F = np.zeros(n, np.float32)
+    for k, v in Fd.items():
+        F[int(k)] = v
+    return F
+
+
+def test_addon_exports():
+    assert os.path.exists(ADDON), "addon not built"
+    src = ("const a=require(%r);process.stdout.write(JSON.stringify({k:Object.keys(a).sort(),v:a.version(),"
+           "k0:Array.from(a.k0(0.3,new Float64Array(576)))}))" % ADDON)
+    p = subprocess.run([NODE, "-e", src], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=60)
+    assert p.returncode == 0, p.stderr
+    out = json.loads(p.stdout)
+    for f in ("applyAEbe3D", "ebePCG", "ebeMGPCG", "createSolver", "solverStep", "solverFe", "solverEnergy",
+              "fixedDofs", "loadVector", "preservedElements", "fixedPaint", "loadPaint", "configDefault"):
+        assert f in out["k"], f
+    assert "gfx950" in out["v"]
+    assert np.array_equal(np.array(out["k0"]), T.k0(0.3).ravel())
+
+
+PRESETS = [
+    ("bottom-corners", "all", "top-center", -1.0),
+    ("left", "all", "down", 2.0),
+    ("right", "xz", "left", 1.0),
+    ("bottom", "y", "up", 0.5),
+    ("top", "all", "right", 3.0),
+]
+
+
+@pytest.mark.parametrize("pos,dofs,direction,mag", PRESETS)
+def test_preset_problem_matches_oracle(oracle, pos, dofs, direction, mag):
+    nx, ny, nz = 6, 4, 5
+    nel, ndof = T.dims(nx, ny, nz)
+    out = run_js("problem_dump.js", {"model": {"nx": nx, "ny": ny, "nz": nz},
+                                     "config": {"constraintPosition": pos, "constraintDOFs": dofs,
+                                                "forceDirection": direction, "forceMagnitude": mag,
+                                                "volumeFraction": 0.3, "penaltyFactor": 3, "filterRadius": 1.5,
+                                                "maxIterations": 60}})
+    P_ = {"left": 0, "right": 1, "bottom": 2, "top": 3, "bottom-corners": 4}[pos]
+    D_ = {"down": 0, "up": 1, "left": 2, "right": 3, "top-center": 4}[direction]
+    mask = 7 if dofs == "all" else sum(b for c, b in (("x", 1), ("y", 2), ("z", 4)) if c in dofs)
+    assert np.array_equal(_mask(out["fixed"], ndof), oracle.fixed_dofs(nx, ny, nz, P_, mask))
+    assert np.array_equal(_dense_F(out["F"], ndof), oracle.load_vector(nx, ny, nz, D_, mag))
+    assert np.array_equal(_mask(out["preserved"], nel), oracle.preserved(nx, ny, nz, P_, D_))
+    c = out["config"]
+    assert (c["volfrac"], c["penal"], c["rmin"], c["max_iterations"]) == (0.3, 3, 1.5, 60)
+    # penalStart / continuationIters defaults of js/optimizer-worker-3d.js:2057-2059
+    assert c["penal_start"] == 1.5 and c["continuation_iters"] == 20
+    assert c["use_projection"] == 1
+
+
+def test_painted_problem_matches_oracle(oracle):
+    nx, ny, nz = 5, 4, 3
+    nel, ndof = T.dims(nx, ny, nz)
+    pc = ["0,0,0,0", "0,1,0,0", "0,1,1", "0,3,2,9", "bad", "1,2"]
+    pf = ["4,3,2,1", "4,2,2,1", "4,2,2,3"]
+    keep = ["2,2,2", "9,9,9"]
+    for fv in (None, [1.0, -2.0, 0.5]):
+        cfg = {"constraintPosition": "left", "constraintDOFs": "xy", "forceDirection": "right", "forceMagnitude": 2.5,
+               "paintedConstraints": pc, "paintedForces": pf, "paintedKeep": keep, "volumeFraction": 0.4}
+        if fv:
+            cfg["forceVector"] = fv
+        out = run_js("problem_dump.js", {"model": {"nx": nx, "ny": ny, "nz": nz}, "config": cfg})
+        assert np.array_equal(_mask(out["fixed"], ndof), oracle.fixed_paint(nx, ny, nz, pc, "xy"))
+        assert np.array_equal(_dense_F(out["F"], ndof),
+                              oracle.load_paint(nx, ny, nz, pf, "right", 2.5, fv))
+        pres = np.zeros(nel, np.uint8)
+        for key in pc + pf + keep:
+            parts = key.split(",")
+            if len(parts) < 3:
+                continue
+            vx, vy, vz = (int(v) for v in parts[:3])
+            if 0 <= vx < nx and 0 <= vy < ny and 0 <= vz < nz:
+                pres[vx + vy * nx + vz * nx * ny] = 1
+        assert np.array_equal(_mask(out["preserved"], nel), pres)
+
+
+def test_oracle_paint_known_answer(oracle):
+    # two +X faces sharing an edge -> 6 unique nodes, 1/6 of the load each;
+    # all-corner key and an out-of-grid corner (dropped by the node bound)
+    F = oracle.load_paint(4, 4, 4, ["3,1,1,1", "3,2,1,1"], "top-center", 6.0)
+    nzi = np.flatnonzero(F)
+    assert len(nzi) == 6 and np.all(F[nzi] == 1.0) and np.all(nzi % 3 == 2)
+    m = oracle.fixed_paint(2, 2, 2, ["1,1,1"], "z")
+    assert m.sum() == 8 and np.all(np.flatnonzero(m) % 3 == 2)
+    assert oracle.fixed_paint(2, 2, 2, ["2,2,2"], "all").sum() == 3  # only node (2,2,2) is in range
+
+
+def test_constrain_to_solid_void(oracle):
+    nx, ny, nz = 4, 3, 2
+    nel, _ = T.dims(nx, ny, nz)
+    el = [1 if (i % 3) else 0 for i in range(nel)]
+    out = run_js("problem_dump.js", {"model": {"nx": nx, "ny": ny, "nz": nz, "elements": el,
+                                               "meshType": "blended-curvature"},
+                                     "config": {"constraintPosition": "left", "forceDirection": "down",
+                                                "forceMagnitude": 1, "constrainToSolid": True,
+                                                "volumeFraction": 0.3}})
+    assert out["void"] == [i for i in range(nel) if el[i] == 0]
+    x0 = np.array(out["x0"], np.float32)
+    pres = _mask(out["preserved"], nel)
+    exp = np.where(np.array(el) > 0, 1.0, 0.0).astype(np.float32)
+    exp[pres > 0] = 1.0
+    exp[np.array(el) == 0] = 0.0
+    assert np.array_equal(x0, exp)
+
+
+# ---- GPU: the TopologySolver host end to end --------------------------------
+def _c1_js_config(n):
+    return {"model": {"nx": n, "ny": n, "nz": n, "type": "cube"},
+            "config": {"solver": "cg", "volumeFraction": 0.3, "penaltyFactor": 3, "penalStart": 3,
+                       "filterRadius": 1.5, "useProjection": False, "maxIterations": 30,
+                       "constraintPosition": "bottom-corners", "constraintDOFs": "all",
+                       "forceDirection": "top-center", "forceMagnitude": -1.0, "cgTol": 1e-8,
+                       "precision": "fp64", "filterType": "sensitivity"}}
+
+
+@pytest.mark.gpu
+def test_topology_solver_c1_matches_oracle(gpu, oracle):
+    """TopologySolver.optimize on the reference's 5^3 cube preset, through
+    the worker and the N-API addon, against the oracle's SIMP loop."""
+    n = 5
+    out = run_js("run_optimize.js", _c1_js_config(n))
+    assert out["ok"], out
+    cfg, fixed, F, pres, void, x0 = P.cube_reference(n)
+    ref = oracle.simp_run(cfg, fixed, F, pres, void, 30, x0)
+    assert out["iterations"] == ref["iterations"]
+    comp = np.array(out["compliance"])
+    rel = np.abs(comp - ref["compliance"]) / np.abs(ref["compliance"])
+    assert rel.max() <= 1e-6, rel.max()
+    assert np.abs(np.array(out["densities"], np.float32) - ref["x"]).max() <= 1e-3
+
+
+@pytest.mark.gpu
+def test_topology_solver_controls(gpu):
+    """pause -> 'paused' + resume, updateConfig(maxIterations), then cancel
+    rejects with 'Optimization cancelled' (lib/topology-solver.js:456-511)."""
+    arg = _c1_js_config(6)
+    arg["config"]["maxIterations"] = 200
+    arg["actions"] = {"2": "pause", "3": {"maxIterations": 6}}
+    out = run_js("run_optimize.js", arg)
+    assert out["ok"], out
+    assert ["paused", 2] in out["events"] and ["resume", 2] in out["events"]
+    assert out["iterations"] <= 6 and len(out["compliance"]) == out["iterations"]
+    arg["actions"] = {"2": "cancel"}
+    arg["config"]["maxIterations"] = 200
+    out = run_js("run_optimize.js", arg)
+    assert not out["ok"] and out["error"] == "Optimization cancelled"
+    assert 2 <= len(out["compliance"]) <= 4
+
+
+@pytest.mark.gpu
+def test_topology_solver_fea_only(gpu, oracle):
+    n = 4
+    nel, ndof = T.dims(n, n, n)
+    arg = _c1_js_config(n)
+    arg["config"]["solver"] = "fea"
+    rng = np.random.default_rng(3)
+    el = rng.uniform(0.2, 1.0, nel).round(3)
+    arg["model"]["elements"] = el.tolist()
+    out = run_js("run_optimize.js", arg)
+    assert out["ok"] and out["feaOnly"], out
+    cfg, fixed, F, pres, void, x0 = P.cube_reference(n)
+    dens = el.astype(np.float32).astype(np.float64)
+    free = np.flatnonzero(fixed == 0).astype(np.int32)
+    U = np.zeros(ndof)
+    oracle.ebePCG(dens, oracle.k0(0.3), n, n, n, F.astype(np.float64), U, free, 1e-9, 1.0, 3.0, 400, 1e-8)
+    c_ref = float(F.astype(np.float64) @ U)
+    assert abs(out["finalCompliance"] - c_ref) <= 1e-6 * abs(c_ref)
+
+
+@pytest.mark.gpu
+def test_wasm_facade_matches_oracle(gpu, oracle):
+    """applyAEbe3D / ebePCG called by byte offset on the facade's
+    WebAssembly.Memory: applyA bit-exact, PCG within solver tolerance."""
+    n = 6
+    nel, ndof = T.dims(n, n, n)
+    rng = np.random.default_rng(21)
+    evals = rng.uniform(0.1, 1.0, nel)
+    evals[rng.random(nel) < 0.2] = 0.0
+    p = rng.normal(size=ndof)
+    fixed = oracle.fixed_dofs(n, n, n, 4)
+    F = oracle.load_vector(n, n, n, 4, -1.0).astype(np.float64)
+    dens = rng.uniform(0.2, 1.0, nel)
+    free = np.flatnonzero(fixed == 0).astype(np.int32)
+    out = run_js("facade_apply.js", {"n": n, "evals": evals.tolist(), "p": p.tolist(), "dens": dens.tolist(),
+                                     "F": F.tolist(), "free": free.tolist()})
+    active = np.flatnonzero(evals > 0).astype(np.int32)
+    ref = oracle.applyA(n, n, n, oracle.k0(0.3), evals, active, p)
+    assert np.array_equal(np.array(out["Ap"]), ref)
+    U = np.zeros(ndof)
+    it = oracle.ebePCG(dens, oracle.k0(0.3), n, n, n, F, U, free, 1e-9, 1.0, 3.0, 400, 1e-8)
+    assert abs(out["iters"] - it) <= 1
+    assert np.abs(np.array(out["U"]) - U).max() <= 1e-6 * np.abs(U).max()
