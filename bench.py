@@ -1,7 +1,8 @@
 """Headline benchmark: SIMP iterations/s of the 192x96x96 cantilever
 (1,769,472 elements, 5,447,811 DOF) with 4-level MG-PCG, mixed precision
-(FP32 operator and V-cycle inside FP64 refinement) against the plain FP64
-path, plus GDOF-matvec/s of the fine applyA.
+(FP32 operator and V-cycle inside FP64 refinement) against the same
+MG-PCG in FP64 and against the plain FP64 method (Jacobi-PCG), at matched
+compliance, plus GDOF-matvec/s of the fine applyA.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 192,96,96]
 
@@ -27,7 +28,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_solver(T, P, grid, prec, steps, warmup, levels):
+def run_solver(T, P, grid, prec, steps, warmup, levels, plain=False):
+    """plain: the reference's plain method on the same problem -- Jacobi-PCG
+    in FP64, with the 400-iteration cap lifted so it reaches the same
+    tolerance (the compliance is then comparable)."""
     nx, ny, nz = grid
     if (nx, ny, nz) == (192, 96, 96):
         cfg, fixed, F, pres, void, x0 = P.cantilever_c3(precision=prec, nlevels=levels)
@@ -39,6 +43,10 @@ def run_solver(T, P, grid, prec, steps, warmup, levels):
         x0 = P.splitmix64_uniform(nel)
     cfg.mg_omega_cap = 1.6
     cfg.max_iterations = 1000
+    if plain:
+        cfg.linear_solver = T.SOLVER_PCG
+        cfg.precision = T.PREC_FP64
+        cfg.cg_max_iter = 100000
     S = T.Solver(cfg, fixed, F)
     S.set_design(x0)
     infos = []
@@ -67,6 +75,7 @@ def main():
     ap.add_argument("--levels", type=int, default=4)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--profile-kernel", action="store_true")
     args = ap.parse_args()
     if args.profile_kernel:
@@ -95,9 +104,12 @@ def main():
     res = {"mixed_ms": t_mixed * 1e3, "mixed_infos": infos}
     t_fp64 = None
     if not args.no_fp64:
-        S64, infos64, t_fp64 = run_solver(T, P, grid, T.PREC_FP64, max(1, args.steps // 2), args.warmup, args.levels)
+        S64, infos64, t_fp64 = run_solver(T, P, grid, T.PREC_FP64, args.steps, args.warmup, args.levels)
         S64.close()
-        res["fp64_infos"] = infos64
+    t_plain = None
+    if not args.no_plain:
+        Sp, infos_p, t_plain = run_solver(T, P, grid, T.PREC_FP64, 1, args.warmup, args.levels, plain=True)
+        Sp.close()
     value = 1.0 / t_mixed
     ach = by32 / (ms32 * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": ach, "peak": 8000.0, "unit": "GB/s", "frac": ach / 8000.0,
@@ -125,6 +137,11 @@ def main():
                    "nel": nel, "ndof": ndof, "parallelism": "single GPU"},
         "fp64_iters_per_s": (1.0 / t_fp64) if t_fp64 else None,
         "mixed_vs_fp64": (t_fp64 / t_mixed) if t_fp64 else None,
+        "compliance_rel_vs_fp64": _rel(infos, infos64) if t_fp64 else None,
+        "plain_fp64_pcg_iters_per_s": (1.0 / t_plain) if t_plain else None,
+        "plain_fp64_pcg_cg_iters": infos_p[0]["cg_iters"] if t_plain else None,
+        "mixed_vs_plain_fp64_pcg": (t_plain / t_mixed) if t_plain else None,
+        "compliance_rel_vs_plain": _rel(infos, infos_p) if t_plain else None,
         "gdof_matvec_per_s": {"fp32": ndof / (ms32 * 1e-3) / 1e9, "fp64": ndof / (ms64 * 1e-3) / 1e9},
         "applyA_ms": {"fp32": ms32, "fp64": ms64}, "vcycle_ms_fp32": msv,
         "copy_GBps": byc / (msc * 1e-3) / 1e9,
@@ -135,6 +152,13 @@ def main():
         "device": name,
     }
     print(json.dumps(out))
+
+
+def _rel(a, b):
+    """max relative compliance difference over the steps both legs ran (both
+    start from the same design)."""
+    n = min(len(a), len(b))
+    return max(abs(a[i]["compliance"] - b[i]["compliance"]) / abs(b[i]["compliance"]) for i in range(n))
 
 
 def measured_traffic(grid):

@@ -18,6 +18,7 @@
 #include "kernels.hpp"
 #include "kernels_xm.hpp"
 #include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 namespace topo {
 
@@ -259,8 +260,11 @@ struct MGLevel {
     float *b32 = nullptr, *x32 = nullptr, *x232 = nullptr, *res32 = nullptr;
     // direct coarse solve: explicit inverse of the free block
     bool chol = false;
-    int64_t nf = 0;
+    int64_t nf = 0, nfp = 0;       // free DOFs, padded to a multiple of 4
     int32_t* freeIdx = nullptr;
+    int32_t* freeOf = nullptr;     // DOF -> free index or -1
+    double* bf = nullptr;          // gathered right-hand side (padded)
+    float* bf32 = nullptr;
     double* Kinv = nullptr;
     float* Kinv32 = nullptr;
     double* work = nullptr;
@@ -275,6 +279,7 @@ template <> struct Sel<double> {
     static double* invD(MGLevel& l) { return l.invD; }
     static double* K(MGLevel& l) { return l.K; }
     static double* Kinv(MGLevel& l) { return l.Kinv; }
+    static double* bf(MGLevel& l) { return l.bf; }
     static const double* E(MGLevel& l) { return l.E; }
     typedef double TE;
 };
@@ -286,6 +291,7 @@ template <> struct Sel<float> {
     static float* invD(MGLevel& l) { return l.invD32; }
     static float* K(MGLevel& l) { return l.K32; }
     static float* Kinv(MGLevel& l) { return l.Kinv32; }
+    static float* bf(MGLevel& l) { return l.bf32; }
     static const float* E(MGLevel& l) { return l.E32; }
     typedef float TE;
 };
@@ -298,20 +304,52 @@ __global__ void k_jacobi_first(T* __restrict__ x, const T* __restrict__ b, const
     const T omega = (T)*omp;
     for (int64_t i = gtid(); i < n; i += gstride()) x[i] = T(0) + omega * invD[i] * b[i];
 }
-// coarse direct solve with the explicit inverse of the free block:
-// x[free[i]] = sum_j Kinv[i][j] b[free[j]], x = 0 on fixed DOFs
+// coarse direct solve with the explicit inverse of the free block (nf free
+// DOFs, padded to ld = nfp, a multiple of 4, so rows are 16-byte aligned):
+// bf[j] = b[free[j]] (zero for the padding), then x[free[i]] = sum_j
+// Kinv[i][j] bf[j]; x = 0 on fixed DOFs.
+template <typename T>
+__global__ void k_gather_free(const T* __restrict__ b, const int32_t* __restrict__ freeIdx, int64_t nf, int64_t nfp,
+                              T* __restrict__ bf, const int* gate) {
+    if (gate && *gate) return;
+    for (int64_t j = gtid(); j < nfp; j += gstride()) bf[j] = j < nf ? b[freeIdx[j]] : T(0);
+}
+template <typename T> struct Vec4;
+template <> struct Vec4<float> { typedef float4 type; static constexpr int n = 4; };
+template <> struct Vec4<double> { typedef double2 type; static constexpr int n = 2; };
+// one wave per 4 consecutive rows; each lane streams 16-byte vectors of the
+// four rows against the same slice of bf (read once per wave from L2)
 template <typename T>
 __global__ void __launch_bounds__(256) k_inv_apply(const T* __restrict__ Kinv, const int32_t* __restrict__ freeIdx,
-                                                   int64_t nf, const T* __restrict__ b, T* __restrict__ x,
-                                                   const int* gate) {
+                                                   int64_t nf, int64_t nfp, const T* __restrict__ bf,
+                                                   T* __restrict__ x, const int* gate) {
     if (gate && *gate) return;
+    typedef typename Vec4<T>::type V;
+    constexpr int VN = Vec4<T>::n;
     const int lane = threadIdx.x & 63;
-    const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    if (row >= nf) return;
-    double s = 0;
-    for (int64_t j = lane; j < nf; j += 64) s += (double)Kinv[row * nf + j] * (double)b[freeIdx[j]];
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-    if (lane == 0) x[freeIdx[row]] = (T)s;
+    const int64_t row0 = (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 4;
+    if (row0 >= nf) return;
+    T acc[4] = {T(0), T(0), T(0), T(0)};
+    const V* bv = (const V*)bf;
+    const int64_t nv = nfp / VN;
+    const V* r0 = (const V*)(Kinv + row0 * nfp);
+    const int nr = (int)std::min<int64_t>(4, nf - row0);
+    for (int64_t j = lane; j < nv; j += 64) {
+        const V bj = bv[j];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            if (r < nr) {
+                const V a = r0[r * nv + j];
+                if constexpr (VN == 4) acc[r] += a.x * bj.x + a.y * bj.y + a.z * bj.z + a.w * bj.w;
+                else acc[r] += a.x * bj.x + a.y * bj.y;
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+        for (int off = 32; off > 0; off >>= 1) acc[r] += __shfl_down(acc[r], off, 64);
+    if (lane == 0)
+        for (int r = 0; r < nr; r++) x[freeIdx[row0 + r]] = acc[r];
 }
 template <typename T>
 __global__ void k_zero_fixed_or_all(T* __restrict__ x, int64_t n, const int* gate) {
@@ -498,20 +536,68 @@ __global__ void __launch_bounds__(256) k_bgj_col(double* A, int64_t n, int64_t k
 }
 
 __global__ void k_extract_free(const double* __restrict__ K, int64_t n, const int32_t* __restrict__ freeIdx, int64_t nf,
-                               double* __restrict__ A) {
+                               int64_t ld, double* __restrict__ A) {
     for (int64_t w = gtid(); w < nf * nf; w += gstride()) {
         const int64_t i = w / nf, j = w % nf;
-        double v = K[(int64_t)freeIdx[i] * n + freeIdx[j]];
-        A[w] = v;
+        A[i * ld + j] = K[(int64_t)freeIdx[i] * n + freeIdx[j]];
     }
 }
-// rows/cols of free DOFs without stiffness get a unit diagonal
+// Free-DOF block of the assembled operator sum_e E_e P_e^T K0 P_e, written
+// row by row without atomics: entry (a, b) gathers, in a fixed order, the
+// <= 8 elements that hold both nodes (E in zyx element order, reference node
+// numbering). freeOf maps a DOF to its free index or -1. A must be zero
+// outside the 81 neighbour columns of each row.
+__global__ void k_assemble_free(const double* __restrict__ E, double thr, Grid g, const int32_t* __restrict__ freeIdx,
+                                const int32_t* __restrict__ freeOf, int64_t nf, int64_t ld,
+                                const double* __restrict__ ke, double* __restrict__ A) {
+    const int nny = g.nny(), nnz = g.nnz();
+    for (int64_t w = gtid(); w < nf * 81; w += gstride()) {
+        const int64_t fa = w / 81;
+        const int q = (int)(w % 81), nb = q / 3, d2 = q % 3;
+        const int dx = nb % 3 - 1, dy = (nb / 3) % 3 - 1, dz = nb / 9 - 1;
+        const int64_t a = freeIdx[fa];
+        const int64_t na = a / 3;
+        const int d1 = (int)(a % 3);
+        const int iz = (int)(na % nnz);
+        const int64_t t = na / nnz;
+        const int iy = (int)(t % nny), ix = (int)(t / nny);
+        const int jx = ix + dx, jy = iy + dy, jz = iz + dz;
+        if (jx < 0 || jx > g.nelx || jy < 0 || jy > g.nely || jz < 0 || jz > g.nelz) continue;
+        const int64_t bdof = 3 * (((int64_t)jx * nny + jy) * nnz + jz) + d2;
+        const int32_t fb = freeOf[bdof];
+        if (fb < 0) continue;
+        double s = 0.0;
+        for (int ex = max(max(ix, jx) - 1, 0); ex <= min(min(ix, jx), g.nelx - 1); ex++)
+            for (int ey = max(max(iy, jy) - 1, 0); ey <= min(min(iy, jy), g.nely - 1); ey++)
+                for (int ez = max(max(iz, jz) - 1, 0); ez <= min(min(iz, jz), g.nelz - 1); ez++) {
+                    const double Ee = E[((int64_t)ex * g.nely + ey) * g.nelz + ez];
+                    if (!(Ee > thr)) continue;
+                    // local node = 4*c + {(0,0):0, (1,0):1, (1,1):2, (0,1):3}[(a,b)]
+                    const int ai = ix - ex, bi = iy - ey, ci = iz - ez;
+                    const int aj = jx - ex, bj = jy - ey, cj = jz - ez;
+                    const int li = 4 * ci + (bi ? (ai ? 2 : 3) : ai);
+                    const int lj = 4 * cj + (bj ? (aj ? 2 : 3) : aj);
+                    s += Ee * ke[(3 * li + d1) * 24 + 3 * lj + d2];
+                }
+        A[fa * ld + fb] = s;
+    }
+}
+// rows/cols of free DOFs without stiffness (and the padding rows) get a unit
+// diagonal
 __global__ void k_regularize(double* A, int64_t nf) {
     for (int64_t i = gtid(); i < nf; i += gstride())
         if (!(A[i * nf + i] > 1e-30)) {
             for (int64_t j = 0; j < nf; j++) { A[i * nf + j] = 0; A[j * nf + i] = 0; }
             A[i * nf + i] = 1.0;
         }
+}
+// copy the triangle a LAPACK-style inverse left filled (column-major lower =
+// row-major upper) to the other one
+__global__ void k_symmetrize_upper(double* A, int64_t n) {
+    for (int64_t w = gtid(); w < n * n; w += gstride()) {
+        const int64_t r = w / n, c = w % n;
+        if (r > c) A[w] = A[c * n + r];
+    }
 }
 
 struct MGH {
@@ -681,41 +767,88 @@ struct MGH {
     }
 
     // FP64 direct solve of the coarsest level: the free-DOF block is
-    // inverted once per density update (SPD Gauss-Jordan on the device); the
-    // V-cycle then applies the inverse with one GEMV.
+    // assembled (deterministically, no atomics), factored by Cholesky
+    // (rocSOLVER dpotrf) and inverted (dpotri) once per density update; the
+    // V-cycle then applies the inverse with one GEMV. A block that is not
+    // positive definite (voided elements left out of the operator make
+    // mechanisms) falls back to Gauss-Jordan with pivot dropping.
+    void coarse_block(Ctx& c, MGLevel& l) {
+        const int64_t nf = l.nf, ld = l.nfp;
+        hipLaunchKernelGGL(k_fill<double>, dim3(nblk(ld * ld)), dim3(256), 0, c.s, l.Kinv, 0.0, ld * ld,
+                           (const int*)nullptr);
+        if (l.dense)
+            hipLaunchKernelGGL(k_extract_free, dim3(nblk(nf * nf)), dim3(256), 0, c.s, (const double*)l.K, l.ndof,
+                               l.freeIdx, nf, ld, l.Kinv);
+        else
+            hipLaunchKernelGGL(k_assemble_free, dim3(nblk(nf * 81)), dim3(256), 0, c.s, (const double*)l.E, thr, l.g,
+                               l.freeIdx, l.freeOf, nf, ld, (const double*)K0->ke, l.Kinv);
+        hipLaunchKernelGGL(k_regularize, dim3(nblk(ld)), dim3(256), 0, c.s, l.Kinv, ld);
+    }
     void factor_coarsest(Ctx& c) {
         MGLevel& l = L.back();
         if (!l.freeIdx) {
             std::vector<uint8_t> hf(l.ndof);
             TOPO_HIP(hipMemcpy(hf.data(), l.fixed, l.ndof, hipMemcpyDeviceToHost));
-            std::vector<int32_t> fr;
+            std::vector<int32_t> fr, of(l.ndof, -1);
             for (int64_t i = 0; i < l.ndof; i++)
-                if (!hf[i]) fr.push_back((int32_t)i);
+                if (!hf[i]) {
+                    of[i] = (int32_t)fr.size();
+                    fr.push_back((int32_t)i);
+                }
             l.nf = (int64_t)fr.size();
             TOPO_CHECK(l.nf > 0 && l.nf <= 20000, TOPO_E_UNSUPPORTED,
                        "coarsest level has %lld free DOFs; the direct solve supports 1..20000", (long long)l.nf);
+            l.nfp = (l.nf + 3) / 4 * 4;
             l.freeIdx = c.alloc<int32_t>(l.nf);
             TOPO_HIP(hipMemcpy(l.freeIdx, fr.data(), l.nf * sizeof(int32_t), hipMemcpyHostToDevice));
-            l.Kinv = c.alloc<double>(l.nf * l.nf);
-            if (want32) l.Kinv32 = c.alloc<float>(l.nf * l.nf);
+            l.freeOf = c.alloc<int32_t>(l.ndof);
+            TOPO_HIP(hipMemcpy(l.freeOf, of.data(), l.ndof * sizeof(int32_t), hipMemcpyHostToDevice));
+            l.Kinv = c.alloc<double>(l.nfp * l.nfp);
+            l.bf = c.alloc<double>(l.nfp);
+            if (want32) {
+                l.Kinv32 = c.alloc<float>(l.nfp * l.nfp);
+                l.bf32 = c.alloc<float>(l.nfp);
+            }
         }
-        double* Kd = l.K;
-        double* tmp = nullptr;
-        if (!l.dense) {
-            tmp = c.alloc<double>(l.ndof * l.ndof);
-            hipLaunchKernelGGL(k_fill<double>, dim3(nblk(l.ndof * l.ndof)), dim3(256), 0, c.s, tmp, 0.0,
-                               l.ndof * l.ndof, (const int*)nullptr);
-            hipLaunchKernelGGL(k_assemble_dense, dim3(nblk(l.g.nel() * 576)), dim3(256), 0, c.s, l.E, thr, l.g, tmp,
-                               (const double*)K0->ke);
-            Kd = tmp;
+        const int64_t n = l.nfp;
+        coarse_block(c, l);
+        bool done = false;
+        if (n > 256) {
+            int* dinfo = (int*)(c.dsc + 14);   // device scratch word
+            rocblas_handle h = c.blas_handle();
+            TOPO_CHECK(rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)n, l.Kinv, (rocblas_int)n, dinfo) ==
+                           rocblas_status_success, TOPO_E_HIP, "rocsolver_dpotrf failed");
+            int info = 0;
+            TOPO_HIP(hipMemcpyAsync(&c.h_sc[0], dinfo, sizeof(int), hipMemcpyDeviceToHost, c.s));
+            c.sync();
+            std::memcpy(&info, &c.h_sc[0], sizeof(int));
+            if (info == 0) {
+                TOPO_CHECK(rocsolver_dpotri(h, rocblas_fill_lower, (rocblas_int)n, l.Kinv, (rocblas_int)n, dinfo) ==
+                               rocblas_status_success, TOPO_E_HIP, "rocsolver_dpotri failed");
+                TOPO_HIP(hipMemcpyAsync(&c.h_sc[0], dinfo, sizeof(int), hipMemcpyDeviceToHost, c.s));
+                c.sync();
+                std::memcpy(&info, &c.h_sc[0], sizeof(int));
+                if (info == 0) {
+                    hipLaunchKernelGGL(k_symmetrize_upper, dim3(nblk(n * n)), dim3(256), 0, c.s, l.Kinv, n);
+                    done = true;
+                }
+            }
+            c.sync();
+            if (!done) coarse_block(c, l);   // the factorization overwrote the block
         }
-        const int64_t nf = l.nf;
-        hipLaunchKernelGGL(k_extract_free, dim3(nblk(nf * nf)), dim3(256), 0, c.s, Kd, l.ndof, l.freeIdx, nf, l.Kinv);
-        hipLaunchKernelGGL(k_regularize, dim3(nblk(nf)), dim3(256), 0, c.s, l.Kinv, nf);
+        if (!done) gauss_jordan(c, l.Kinv, n);
+        c.check_launch();
+        if (want32)
+            hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(n * n)), dim3(256), 0, c.s, l.Kinv32, l.Kinv, n * n);
+        l.chol = true;
+    }
+    // in-place inverse of the n x n block A (row-major, ld n) by blocked
+    // Gauss-Jordan with pivot dropping
+    void gauss_jordan(Ctx& c, double* A, int64_t nf) {
         double* d0 = c.alloc<double>(nf);
-        hipLaunchKernelGGL(k_diag_copy, dim3(nblk(nf)), dim3(256), 0, c.s, (const double*)l.Kinv, nf, d0);
+        hipLaunchKernelGGL(k_diag_copy, dim3(nblk(nf)), dim3(256), 0, c.s, (const double*)A, nf, d0);
         const int nt = (int)((nf + kGJB - 1) / kGJB);
-        // large coarse levels: the rank-64 update as a plain rocBLAS DGEMM
+        // large blocks: the rank-64 update as a plain rocBLAS DGEMM
         const bool use_blas = nf > 1024;
         double *Cp = nullptr, *Rp = nullptr;
         if (use_blas) {
@@ -724,37 +857,28 @@ struct MGH {
         }
         for (int64_t k0 = 0; k0 < nf; k0 += kGJB) {
             const int kb = (int)std::min<int64_t>(kGJB, nf - k0);
-            hipLaunchKernelGGL(k_bgj_diag, dim3(1), dim3(256), 0, c.s, l.Kinv, nf, k0, kb, (const double*)d0);
-            hipLaunchKernelGGL(k_bgj_row, dim3(nt), dim3(256), 0, c.s, l.Kinv, nf, k0, kb);
+            hipLaunchKernelGGL(k_bgj_diag, dim3(1), dim3(256), 0, c.s, A, nf, k0, kb, (const double*)d0);
+            hipLaunchKernelGGL(k_bgj_row, dim3(nt), dim3(256), 0, c.s, A, nf, k0, kb);
             if (use_blas) {
-                hipLaunchKernelGGL(k_bgj_panels, dim3(nblk(nf * kb)), dim3(256), 0, c.s, (const double*)l.Kinv, nf, k0,
+                hipLaunchKernelGGL(k_bgj_panels, dim3(nblk(nf * kb)), dim3(256), 0, c.s, (const double*)A, nf, k0,
                                    kb, Cp, Rp);
                 // row-major A -= C R  <=>  column-major A^T -= R^T C^T
                 const double alpha = -1.0, beta = 1.0;
                 TOPO_CHECK(rocblas_dgemm(c.blas_handle(), rocblas_operation_none, rocblas_operation_none, (rocblas_int)nf,
-                                         (rocblas_int)nf, kb, &alpha, Rp, (rocblas_int)nf, Cp, kb, &beta, l.Kinv,
+                                         (rocblas_int)nf, kb, &alpha, Rp, (rocblas_int)nf, Cp, kb, &beta, A,
                                          (rocblas_int)nf) == rocblas_status_success,
                            TOPO_E_HIP, "rocblas_dgemm failed");
             } else {
-                hipLaunchKernelGGL(k_bgj_update, dim3(nt, nt), dim3(256), 0, c.s, l.Kinv, nf, k0, kb);
+                hipLaunchKernelGGL(k_bgj_update, dim3(nt, nt), dim3(256), 0, c.s, A, nf, k0, kb);
             }
-            hipLaunchKernelGGL(k_bgj_col, dim3(nt), dim3(256), 0, c.s, l.Kinv, nf, k0, kb);
+            hipLaunchKernelGGL(k_bgj_col, dim3(nt), dim3(256), 0, c.s, A, nf, k0, kb);
         }
+        c.sync();
         if (use_blas) {
-            c.sync();
             c.free_one(Cp);
             c.free_one(Rp);
         }
-        c.sync();
         c.free_one(d0);
-        c.check_launch();
-        if (want32)
-            hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(nf * nf)), dim3(256), 0, c.s, l.Kinv32, l.Kinv, nf * nf);
-        if (tmp) {
-            c.sync();
-            c.free_one(tmp);
-        }
-        l.chol = true;
     }
 
     // out = A in (mode 0), masked (1), b - A in (2), Jacobi sweep (3)
@@ -804,8 +928,10 @@ struct MGH {
         const bool coarsest = li == (int)L.size() - 1;
         if (coarsest && l.chol) {
             hipLaunchKernelGGL(k_zero_fixed_or_all<T>, dim3(nblk(l.ndof)), dim3(256), 0, c.s, x, l.ndof, gate);
-            hipLaunchKernelGGL(k_inv_apply<T>, dim3(nblk(l.nf * 64)), dim3(256), 0, c.s, (const T*)Sel<T>::Kinv(l),
-                               l.freeIdx, l.nf, b, x, gate);
+            hipLaunchKernelGGL(k_gather_free<T>, dim3(nblk(l.nfp)), dim3(256), 0, c.s, b, l.freeIdx, l.nf, l.nfp,
+                               Sel<T>::bf(l), gate);
+            hipLaunchKernelGGL(k_inv_apply<T>, dim3((unsigned)((l.nf + 15) / 16)), dim3(256), 0, c.s,
+                               (const T*)Sel<T>::Kinv(l), l.freeIdx, l.nf, l.nfp, (const T*)Sel<T>::bf(l), x, gate);
             return x;
         }
         const int nsweep_pre = coarsest ? nu1 + coarse_sweeps : nu1;
@@ -1105,7 +1231,8 @@ struct Solver {
         mg.nu1 = cfg.mg_nu1;
         mg.nu2 = cfg.mg_nu2;
         mg.coarse_sweeps = cfg.coarse_sweeps;
-        mg.coarse_solver = cfg.coarse_solver;
+        // Jacobi-PCG uses level 0 for its diagonal only: no coarse factor
+        mg.coarse_solver = cfg.linear_solver == TOPO_SOLVER_MGPCG ? cfg.coarse_solver : TOPO_COARSE_SWEEPS;
         mg.galerkin_max = cfg.galerkin_max_ndof;
         // skip threshold (skipThreshold = Emin * 1000, js/optimizer-worker-3d.js:762);
         // -1 keeps every element (E >= Emin > 0)
