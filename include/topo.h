@@ -58,7 +58,8 @@ enum { TOPO_COARSE_SWEEPS = 0,   /* reference: 30 extra damped-Jacobi sweeps  */
        TOPO_COARSE_CHOLESKY = 1};/* FP64 dense Cholesky of the coarsest level */
 enum { TOPO_FILTER_SENSITIVITY = 0,  /* reference filterSensitivities       */
        TOPO_FILTER_DENSITY = 1,      /* linear density filter (weights rmin-d) */
-       TOPO_FILTER_HELMHOLTZ = 2 };  /* PDE filter -r^2 lap(xf) + xf = x      */
+       TOPO_FILTER_HELMHOLTZ = 2 };  /* PDE filter -r^2 lap(xf) + xf = x on the
+                                        nodes, r = rmin / (2 sqrt 3)         */
 enum { TOPO_OBJ_COMPLIANCE = 0, TOPO_OBJ_MECHANISM = 1 };
 
 /* ---- solver configuration (keys of the reference `config` object) ---- */
@@ -90,7 +91,7 @@ typedef struct topo_config {
     double  cg_tol;             /* > 0 fixed relative tolerance; <= 0 reference schedule 1e-3 -> 1e-8 */
     int32_t warm_start;         /* -1 solver default (PCG warm, MGPCG cold), 0 cold, 1 warm */
     double  move;               /* OC move limit (0.2)                                 */
-    int32_t objective;          /* TOPO_OBJ_*                                          */
+    int32_t objective;          /* TOPO_OBJ_* (mechanism: nload 1, F = input force)    */
     int32_t mech_in_dof;        /* mechanism: input DOF (load applied there)           */
     int32_t mech_out_dof;       /* mechanism: output DOF                               */
     double  mech_k_in;          /* input spring stiffness                              */
@@ -243,6 +244,28 @@ enum { TOPO_FIELD_U = 0,        /* double[nload*ndof]                    */
        TOPO_FIELD_STRESS = 5,   /* float[nel]  E_e * w_e                 */
        TOPO_FIELD_ADJOINT = 6 };/* double[ndof] adjoint (mechanism)      */
 int topo_solver_get_field(topo_solver* s, int32_t field, void* out);
+
+/* ---- differentiation: forward/backward pair --------------------------
+ * forward: at the current design x (set_design) and SIMP iteration counter,
+ * maps x -> filter (density or Helmholtz) -> Heaviside projection (if on) ->
+ * E = Emin + xPhys^p (E0 - Emin) -> FE solve(s) -> objective, all in FP64
+ * (the solve in the configured precision). objective = sum_k F_k^T U_k
+ * (compliance) or u_out = U[mech_out_dof] (mechanism, TOPO_OBJ_MECHANISM: the
+ * input spring mech_k_in at mech_in_dof and the output spring mech_k_out at
+ * mech_out_dof are part of K; the adjoint K lam = e_out is solved here too).
+ * volume = mean(xPhys). Replaces, for compliance, the reference's FE +
+ * compliance of js/optimizer-worker-3d.js:2124-2162 (no reference counterpart
+ * for the mechanism or the Helmholtz filter).
+ * backward: the hand-written adjoint of the last forward: dJdx and dVdx
+ * (double[nel], reference element order; dVdx may be NULL). For compliance
+ * with no filter it equals the reference's analytic sensitivity (:2175-2192)
+ * up to its use of E0 in place of E0 - Emin. */
+int topo_solver_forward(topo_solver* s, double* objective, double* volume, topo_step_info* info);
+int topo_solver_backward(topo_solver* s, double* dJdx, double* dVdx);
+/* Helmholtz filter of an element field (double[nel], reference order) for a
+ * solver created with TOPO_FILTER_HELMHOLTZ; iters receives the CG count.
+ * The map is symmetric, so it is also its own adjoint. */
+int topo_op_helmholtz(topo_solver* s, const double* in, double* out, int32_t* iters);
 
 /* FE() (js/optimizer-worker-3d.js:4155-4371): solve K(xPhys) U = F for the
  * given physical densities and penalty. U (double[nload*ndof]) is the warm

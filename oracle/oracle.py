@@ -54,6 +54,8 @@ def lib():
         _lib.or_k0.argtypes = [C.c_double, C.c_void_p]
         _lib.or_load_vector.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p, C.c_void_p]
         _lib.or_simp_run.argtypes = [C.c_void_p] * 5 + [C.c_int] + [C.c_void_p] * 5
+        _lib.or_helmholtz.argtypes = [C.c_int] * 3 + [C.c_double, C.c_void_p, C.c_void_p]
+        _lib.or_fb.argtypes = [C.c_void_p] * 4 + [C.c_int] + [C.c_void_p] * 6
     return _lib
 
 
@@ -332,3 +334,29 @@ def load_paint(nelx, nely, nelz, keys, direction="down", magnitude=1.0, force_ve
             F[3 * n + 1] = np.float32(float(F[3 * n + 1]) + fy * per)
             F[3 * n + 2] = np.float32(float(F[3 * n + 2]) + fz * per)
     return F
+
+
+# ---- differentiation path (no reference counterpart) -------------------------
+def helmholtz(nelx, nely, nelz, rmin, v):
+    """Helmholtz filter of an element field (reference element order)."""
+    v = np.ascontiguousarray(v, np.float64)
+    out = np.zeros_like(v)
+    lib().or_helmholtz(nelx, nely, nelz, rmin, _p(v), _p(out))
+    return out
+
+
+def fb(cfg, fixed, F, x, it=1):
+    """Forward/backward restatement: J, volume, dJ/dx, dV/dx (reference
+    element order), U and (mechanism) the adjoint field."""
+    nel, ndof = dims(cfg.nelx, cfg.nely, cfg.nelz)
+    x = np.ascontiguousarray(x, np.float32)
+    fixed = np.ascontiguousarray(fixed, np.uint8)
+    F = np.ascontiguousarray(F, np.float32)
+    J = C.c_double()
+    vol = C.c_double()
+    dJ = np.zeros(nel)
+    dV = np.zeros(nel)
+    U = np.zeros(ndof * cfg.nload)
+    lam = np.zeros(ndof)
+    lib().or_fb(C.byref(cfg), _p(fixed), _p(F), _p(x), it, C.byref(J), C.byref(vol), _p(dJ), _p(dV), _p(U), _p(lam))
+    return dict(J=J.value, vol=vol.value, dJ=dJ, dV=dV, U=U, lam=lam)

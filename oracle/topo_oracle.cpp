@@ -1071,4 +1071,249 @@ int or_simp_run(const topo_config* cfg, const uint8_t* fixed, const float* F32, 
     return loop;
 }
 
+// ---------------------------------------------------------------------------
+// Differentiation path (no reference counterpart; pinned by invariants in
+// tests/test_oracle.py: the Helmholtz map keeps constants and sum(x) and is
+// symmetric; it equals a dense numpy assembly of (r^2 K_L + M); central
+// finite differences of J; the adjoint identity L^T u = lam^T f).
+
+// Helmholtz filter: nodes of the grid, (r^2 K_L + M) y = T x, xf = T^T y with
+// T the element -> node map of weight 1/8; r = rmin / (2 sqrt 3). Elements
+// in the reference order (e = ex + ey nelx + ez nelx nely). CG (Jacobi) to
+// 1e-14 relative.
+int or_helmholtz(int nelx, int nely, int nelz, double rmin, const double* in, double* out) {
+    const int nnx = nelx + 1, nny = nely + 1, nnz = nelz + 1;
+    const int64_t nn = (int64_t)nnx * nny * nnz, nel = (int64_t)nelx * nely * nelz;
+    const double rp = rmin / (2.0 * std::sqrt(3.0));
+    static const int off[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+    const double m1[2][2] = {{1.0 / 3, 1.0 / 6}, {1.0 / 6, 1.0 / 3}}, k1[2][2] = {{1, -1}, {-1, 1}};
+    double A[8][8];
+    for (int i = 0; i < 8; i++)
+        for (int j = 0; j < 8; j++) {
+            const int* a = off[i];
+            const int* b = off[j];
+            const double mx = m1[a[0]][b[0]], my = m1[a[1]][b[1]], mz = m1[a[2]][b[2]];
+            A[i][j] = rp * rp * (k1[a[0]][b[0]] * my * mz + mx * k1[a[1]][b[1]] * mz + mx * my * k1[a[2]][b[2]]) +
+                      mx * my * mz;
+        }
+    auto nodes = [&](int64_t e, int64_t* n8) {
+        const int ex = (int)(e % nelx), ey = (int)((e / nelx) % nely), ez = (int)(e / ((int64_t)nelx * nely));
+        for (int m = 0; m < 8; m++) n8[m] = node_id(ex + off[m][0], ey + off[m][1], ez + off[m][2], nny, nnz);
+    };
+    auto apply = [&](const dvec& v, dvec& o) {
+        std::fill(o.begin(), o.end(), 0.0);
+        int64_t n8[8];
+        for (int64_t e = 0; e < nel; e++) {
+            nodes(e, n8);
+            for (int i = 0; i < 8; i++) {
+                double sacc = 0;
+                for (int j = 0; j < 8; j++) sacc += A[i][j] * v[n8[j]];
+                o[n8[i]] += sacc;
+            }
+        }
+    };
+    dvec b(nn, 0.0), d(nn, 0.0), y(nn, 0.0), r(nn), z(nn), pp(nn), Ap(nn);
+    int64_t n8[8];
+    for (int64_t e = 0; e < nel; e++) {
+        nodes(e, n8);
+        for (int m = 0; m < 8; m++) { b[n8[m]] += in[e] / 8.0; d[n8[m]] += A[m][m]; }
+    }
+    r = b;
+    double rz = 0, rn0 = 0;
+    for (int64_t i = 0; i < nn; i++) { z[i] = r[i] / d[i]; pp[i] = z[i]; rz += r[i] * z[i]; rn0 += r[i] * r[i]; }
+    for (int it = 0; it < 10000 && rn0 > 0; it++) {
+        apply(pp, Ap);
+        double pAp = 0;
+        for (int64_t i = 0; i < nn; i++) pAp += pp[i] * Ap[i];
+        const double al = rz / pAp;
+        double rn = 0, rz2 = 0;
+        for (int64_t i = 0; i < nn; i++) { y[i] += al * pp[i]; r[i] -= al * Ap[i]; rn += r[i] * r[i]; }
+        if (rn < 1e-28 * rn0) break;
+        for (int64_t i = 0; i < nn; i++) { z[i] = r[i] / d[i]; rz2 += r[i] * z[i]; }
+        const double be = rz2 / rz;
+        rz = rz2;
+        for (int64_t i = 0; i < nn; i++) pp[i] = z[i] + be * pp[i];
+    }
+    for (int64_t e = 0; e < nel; e++) {
+        nodes(e, n8);
+        double sacc = 0;
+        for (int m = 0; m < 8; m++) sacc += y[n8[m]];
+        out[e] = sacc / 8.0;
+    }
+    return 0;
+}
+
+// K(E) + springs, full space, fixed DOFs masked: Jacobi-PCG to 1e-14
+// relative from zero (test solver of the forward/backward restatement)
+static int fb_solve(int nelx, int nely, int nelz, const double* ke, const dvec& E, const uint8_t* fixed,
+                    const int64_t* sdof, const double* sk, int ns, const dvec& rhs, dvec& u) {
+    const int64_t ndof = 3LL * (nelx + 1) * (nely + 1) * (nelz + 1), nel = (int64_t)nelx * nely * nelz;
+    std::vector<int32_t> act(nel);
+    for (int64_t e = 0; e < nel; e++) act[e] = (int32_t)e;
+    auto apply = [&](const dvec& v, dvec& o) {
+        or_applyA(nelx, nely, nelz, ke, E.data(), act.data(), (int)nel, v.data(), o.data());
+        for (int s = 0; s < ns; s++) o[sdof[s]] += sk[s] * v[sdof[s]];
+        for (int64_t i = 0; i < ndof; i++) if (fixed[i]) o[i] = 0;
+    };
+    dvec diag(ndof, 0.0);
+    {   // diagonal: sum_e E_e K0_ii
+        std::vector<int32_t> ed(nel * 24);
+        or_edofs(nelx, nely, nelz, ed.data());
+        for (int64_t e = 0; e < nel; e++)
+            for (int i = 0; i < 24; i++) diag[ed[e * 24 + i]] += E[e] * ke[i * 24 + i];
+        for (int s = 0; s < ns; s++) diag[sdof[s]] += sk[s];
+    }
+    u.assign(ndof, 0.0);
+    dvec r(rhs), z(ndof), p(ndof), Ap(ndof);
+    double rz = 0, rn0 = 0;
+    for (int64_t i = 0; i < ndof; i++) {
+        if (fixed[i]) r[i] = 0;
+        z[i] = fixed[i] ? 0.0 : r[i] / diag[i];
+        p[i] = z[i];
+        rz += r[i] * z[i];
+        rn0 += r[i] * r[i];
+    }
+    int it = 0;
+    for (; it < 200000 && rn0 > 0; it++) {
+        apply(p, Ap);
+        double pAp = 0;
+        for (int64_t i = 0; i < ndof; i++) pAp += p[i] * Ap[i];
+        const double al = rz / pAp;
+        double rn = 0, rz2 = 0;
+        for (int64_t i = 0; i < ndof; i++) { u[i] += al * p[i]; r[i] -= al * Ap[i]; rn += r[i] * r[i]; }
+        if (rn < 1e-28 * rn0) break;
+        for (int64_t i = 0; i < ndof; i++) { z[i] = fixed[i] ? 0.0 : r[i] / diag[i]; rz2 += r[i] * z[i]; }
+        const double be = rz2 / rz;
+        rz = rz2;
+        for (int64_t i = 0; i < ndof; i++) p[i] = z[i] + be * p[i];
+    }
+    return it;
+}
+
+// Forward/backward of the design map and objective (restates
+// topo_solver_forward/backward): x (float, reference element order) ->
+// density filter (filter_type DENSITY) or Helmholtz (HELMHOLTZ) -> Heaviside
+// (use_projection, beta = beta_fixed or the schedule at iteration it) ->
+// E = Emin + xp^p (E0 - Emin) (penal from the continuation at it) -> solve ->
+// J = sum F^T U (compliance) or U[out] (mechanism, springs at the ports,
+// adjoint K lam = e_out). Gradients dJ/dx, d mean(xp)/dx by the chain rule.
+int or_fb(const topo_config* cfg, const uint8_t* fixed, const float* F32, const float* x, int it, double* J,
+          double* vol, double* dJdx, double* dVdx, double* U_out, double* lam_out) {
+    const int nelx = cfg->nelx, nely = cfg->nely, nelz = cfg->nelz;
+    const int64_t nel = (int64_t)nelx * nely * nelz;
+    const int64_t ndof = 3LL * (nelx + 1) * (nely + 1) * (nelz + 1);
+    const double penal = cfg->penal_start + (cfg->penal - cfg->penal_start) *
+                                                std::min(1.0, (double)(it - 1) / std::max(1, cfg->continuation_iters));
+    double beta = 1;
+    if (cfg->use_projection)
+        beta = cfg->beta_fixed > 0 ? cfg->beta_fixed
+                                   : std::min(cfg->beta_max, std::pow(2.0, std::floor((double)(it - 1) / cfg->beta_interval)));
+    const double eta = cfg->eta;
+    dvec xd(x, x + nel), xf(nel), xp(nel);
+    // density filter in FP64 (same weights as or_density_filter)
+    auto dfilter = [&](const dvec& in, dvec& out, int transpose) {
+        const double rmin = cfg->rmin;
+        const int R = (int)std::ceil(rmin);
+        dvec sw(nel, 0.0);
+        for (int64_t e = 0; e < nel; e++) {
+            const int ex = (int)(e % nelx), ey = (int)((e / nelx) % nely), ez = (int)(e / ((int64_t)nelx * nely));
+            for (int kz = -R; kz <= R; kz++)
+                for (int ky = -R; ky <= R; ky++)
+                    for (int kx = -R; kx <= R; kx++) {
+                        const double d2 = kx * kx + ky * ky + kz * kz;
+                        const int jx = ex + kx, jy = ey + ky, jz = ez + kz;
+                        if (d2 > rmin * rmin || jx < 0 || jx >= nelx || jy < 0 || jy >= nely || jz < 0 || jz >= nelz) continue;
+                        sw[e] += rmin - std::sqrt(d2);
+                    }
+        }
+        for (int64_t e = 0; e < nel; e++) {
+            const int ex = (int)(e % nelx), ey = (int)((e / nelx) % nely), ez = (int)(e / ((int64_t)nelx * nely));
+            double acc = 0;
+            for (int kz = -R; kz <= R; kz++)
+                for (int ky = -R; ky <= R; ky++)
+                    for (int kx = -R; kx <= R; kx++) {
+                        const double d2 = kx * kx + ky * ky + kz * kz;
+                        const int jx = ex + kx, jy = ey + ky, jz = ez + kz;
+                        if (d2 > rmin * rmin || jx < 0 || jx >= nelx || jy < 0 || jy >= nely || jz < 0 || jz >= nelz) continue;
+                        const int64_t j = jx + (int64_t)nelx * (jy + (int64_t)nely * jz);
+                        acc += transpose ? (rmin - std::sqrt(d2)) * in[j] / sw[j] : (rmin - std::sqrt(d2)) * in[j];
+                    }
+            out[e] = transpose ? acc : acc / sw[e];
+        }
+    };
+    if (cfg->filter_type == TOPO_FILTER_DENSITY) dfilter(xd, xf, 0);
+    else if (cfg->filter_type == TOPO_FILTER_HELMHOLTZ) or_helmholtz(nelx, nely, nelz, cfg->rmin, xd.data(), xf.data());
+    else xf = xd;
+    const double tb = std::tanh(beta * eta), den = tb + std::tanh(beta * (1 - eta));
+    for (int64_t e = 0; e < nel; e++)
+        xp[e] = cfg->use_projection ? (tb + std::tanh(beta * (xf[e] - eta))) / den : xf[e];
+    dvec E(nel);
+    for (int64_t e = 0; e < nel; e++) E[e] = cfg->Emin + std::pow(xp[e], penal) * (cfg->E0 - cfg->Emin);
+    dvec ke(576);
+    or_k0(cfg->nu, ke.data());
+    const bool mech = cfg->objective == TOPO_OBJ_MECHANISM;
+    int64_t sdof[2];
+    double sk[2];
+    int ns = 0;
+    if (mech) {
+        if (cfg->mech_k_in > 0) { sdof[ns] = cfg->mech_in_dof; sk[ns] = cfg->mech_k_in; ns++; }
+        if (cfg->mech_k_out > 0) { sdof[ns] = cfg->mech_out_dof; sk[ns] = cfg->mech_k_out; ns++; }
+    }
+    dvec g(nel, 0.0), u, lam;
+    double Jv = 0;
+    std::vector<int32_t> ed(nel * 24);
+    or_edofs(nelx, nely, nelz, ed.data());
+    auto accumulate = [&](const dvec& uu, const dvec& ll) {
+        for (int64_t e = 0; e < nel; e++) {
+            double en = 0;
+            for (int i = 0; i < 24; i++) {
+                double r = 0;
+                for (int j = 0; j < 24; j++) r += ke[i * 24 + j] * uu[ed[e * 24 + j]];
+                en += ll[ed[e * 24 + i]] * r;
+            }
+            g[e] += -en * penal * std::pow(xp[e], penal - 1) * (cfg->E0 - cfg->Emin);
+        }
+    };
+    for (int k = 0; k < cfg->nload; k++) {
+        dvec f(ndof);
+        for (int64_t i = 0; i < ndof; i++) f[i] = F32[(int64_t)k * ndof + i];
+        fb_solve(nelx, nely, nelz, ke.data(), E, fixed, sdof, sk, ns, f, u);
+        if (U_out) std::copy(u.begin(), u.end(), U_out + (int64_t)k * ndof);
+        if (mech) {
+            dvec L(ndof, 0.0);
+            L[cfg->mech_out_dof] = 1.0;
+            fb_solve(nelx, nely, nelz, ke.data(), E, fixed, sdof, sk, ns, L, lam);
+            if (lam_out) std::copy(lam.begin(), lam.end(), lam_out);
+            Jv = u[cfg->mech_out_dof];
+            accumulate(u, lam);
+        } else {
+            for (int64_t i = 0; i < ndof; i++) Jv += f[i] * u[i];
+            accumulate(u, u);
+        }
+    }
+    dvec gv(nel, 1.0 / (double)nel);
+    if (cfg->use_projection)
+        for (int64_t e = 0; e < nel; e++) {
+            const double th = std::tanh(beta * (xf[e] - eta));
+            const double dH = beta * (1 - th * th) / den;
+            g[e] *= dH;
+            gv[e] *= dH;
+        }
+    dvec t(nel);
+    if (cfg->filter_type == TOPO_FILTER_DENSITY) {
+        dfilter(g, t, 1); g = t;
+        dfilter(gv, t, 1); gv = t;
+    } else if (cfg->filter_type == TOPO_FILTER_HELMHOLTZ) {
+        or_helmholtz(nelx, nely, nelz, cfg->rmin, g.data(), t.data()); g = t;
+        or_helmholtz(nelx, nely, nelz, cfg->rmin, gv.data(), t.data()); gv = t;
+    }
+    double vs = 0;
+    for (int64_t e = 0; e < nel; e++) vs += xp[e];
+    if (J) *J = Jv;
+    if (vol) *vol = vs / (double)nel;
+    if (dJdx) std::copy(g.begin(), g.end(), dJdx);
+    if (dVdx) std::copy(gv.begin(), gv.end(), dVdx);
+    return 0;
+}
+
 }  // extern "C"

@@ -91,7 +91,7 @@ SYMBOLS = [
     "topo_solver_set_design", "topo_solver_get_design", "topo_solver_set_iteration", "topo_solver_update",
     "topo_solver_step", "topo_solver_get_field", "topo_solver_fe", "topo_solver_set_density", "topo_op_applyA",
     "topo_op_vcycle", "topo_op_energy", "topo_op_filter", "topo_op_filter_adjoint", "topo_op_oc", "topo_time_op",
-    "topo_solver_cancel",
+    "topo_solver_cancel", "topo_solver_forward", "topo_solver_backward", "topo_op_helmholtz",
 ]
 
 
@@ -136,6 +136,9 @@ def lib():
     L.topo_op_oc.argtypes = [vp, vp, vp, vp, dbl]
     L.topo_time_op.argtypes = [vp, i32, i32, i32, i32, vp, vp]
     L.topo_solver_cancel.argtypes = [vp]
+    L.topo_solver_forward.argtypes = [vp, vp, vp, vp]
+    L.topo_solver_backward.argtypes = [vp, vp, vp]
+    L.topo_op_helmholtz.argtypes = [vp, vp, vp, vp]
     _lib = L
     return L
 
@@ -309,6 +312,8 @@ class Solver:
     def field(self, which):
         if which == FIELD_U:
             out = np.zeros(self.ndof * self.cfg.nload)
+        elif which == FIELD_ADJOINT:
+            out = np.zeros(self.ndof)
         else:
             out = np.zeros(self.nel, np.float32)
         _check(lib().topo_solver_get_field(self.h, which, _p(out)))
@@ -373,3 +378,25 @@ class Solver:
 
     def cancel(self):
         lib().topo_solver_cancel(self.h)
+
+    def forward(self):
+        """objective, mean(xPhys), StepInfo of the FP64 forward pass at the current design"""
+        J = C.c_double()
+        vol = C.c_double()
+        info = StepInfo()
+        _check(lib().topo_solver_forward(self.h, C.byref(J), C.byref(vol), C.byref(info)))
+        return J.value, vol.value, info
+
+    def backward(self):
+        """dJ/dx and d mean(xPhys)/dx (float64, reference element order)"""
+        dJ = np.zeros(self.nel)
+        dV = np.zeros(self.nel)
+        _check(lib().topo_solver_backward(self.h, _p(dJ), _p(dV)))
+        return dJ, dV
+
+    def helmholtz(self, v):
+        v = np.ascontiguousarray(v, np.float64)
+        out = np.zeros_like(v)
+        it = C.c_int32()
+        _check(lib().topo_op_helmholtz(self.h, _p(v), _p(out), C.byref(it)))
+        return out, it.value

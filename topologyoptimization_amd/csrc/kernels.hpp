@@ -40,6 +40,20 @@ __device__ __forceinline__ double js_min(double a, double b) { return (a != a ||
 
 // ---------------------------------------------------------------------------
 // element order conversions
+template <typename T>
+__global__ void k_ref_to_zyx_t(const T* __restrict__ in, T* __restrict__ out, Grid g) {
+    for (int64_t i = gtid(); i < g.nel(); i += gstride()) {
+        const int64_t ez = i % g.nelz, t = i / g.nelz, ey = t % g.nely, ex = t / g.nely;
+        out[i] = in[ex + g.nelx * (ey + (int64_t)g.nely * ez)];
+    }
+}
+template <typename T>
+__global__ void k_zyx_to_ref_t(const T* __restrict__ in, T* __restrict__ out, Grid g) {
+    for (int64_t i = gtid(); i < g.nel(); i += gstride()) {
+        const int64_t ez = i % g.nelz, t = i / g.nelz, ey = t % g.nely, ex = t / g.nely;
+        out[ex + g.nelx * (ey + (int64_t)g.nely * ez)] = in[i];
+    }
+}
 __global__ void k_ref_to_zyx(const float* __restrict__ in, float* __restrict__ out, Grid g) {
     for (int64_t i = gtid(); i < g.nel(); i += gstride()) {
         const int64_t ez = i % g.nelz, t = i / g.nelz, ey = t % g.nely, ex = t / g.nely;
@@ -446,10 +460,12 @@ template <typename TA, typename TB>
 __global__ void __launch_bounds__(kRedThreads) k_dot(const TA* __restrict__ a, const TB* __restrict__ b, int64_t n,
                                                     double* out, double* partials, unsigned* ticket) {
     double v[1] = {0.0};
-    for (int64_t i = gtid(); i < n; i += gstride()) v[0] += (double)a[i] * (double)b[i];
+    for (int64_t i = gtid(); i < n; i += gstride()) v[0] += (double)a[i] * (b ? (double)b[i] : 1.0);
     double tot[1];
     if (grid_reduce<1>(v, partials, ticket, tot)) *out = tot[0];
 }
+
+// (b == nullptr: sum of a)
 
 // scaled residual for the mixed inner solve: rs = (float)(r / s), s = ||r||_inf-ish scale
 template <typename T>
@@ -938,6 +954,21 @@ __device__ __forceinline__ float oc_xnew(float xf, float dcf, double lmid, doubl
     return (float)js_max(0.0, js_max(xi - move, js_min(1.0, js_min(xi + move, xi * sqrt(Be)))));
 }
 
+// OC update of the compliant mechanism (the damped OC of the classic
+// mechanism-synthesis codes; no reference counterpart): B_e = max(1e-10,
+// -dc_e / lmid), x_new = clamp(x B^0.3, x -+ move, [0, 1]).
+__device__ __forceinline__ float oc_xnew_mech(float xf, float dcf, double lmid, double move, uint8_t pres, uint8_t vd) {
+    if (pres) return 1.0f;
+    if (vd) return 0.0f;
+    const double Be = fmax(1e-10, -(double)dcf / lmid);
+    const double xi = (double)xf;
+    return (float)fmax(0.0, fmax(xi - move, fmin(1.0, fmin(xi + move, xi * pow(Be, 0.3)))));
+}
+__device__ __forceinline__ float oc_xnew_any(int mech, float xf, float dcf, double lmid, double move, uint8_t pres,
+                                             uint8_t vd) {
+    return mech ? oc_xnew_mech(xf, dcf, lmid, move, pres, vd) : oc_xnew(xf, dcf, lmid, move, pres, vd);
+}
+
 // fill the candidate tree for the next steps from [l1, l2]
 __global__ void k_oc_plan(OCState* st) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -959,7 +990,7 @@ __global__ void k_oc_plan(OCState* st) {
 __global__ void __launch_bounds__(kRedThreads) k_oc_eval(const float* __restrict__ x, const float* __restrict__ dc,
                                                         const uint8_t* __restrict__ pres, const uint8_t* __restrict__ vd,
                                                         double move, int64_t n, OCState* st, double* partials,
-                                                        unsigned* ticket) {
+                                                        unsigned* ticket, int mech) {
     if (st->finished) return;
     double v[kOcCand];
     double lam[kOcCand];
@@ -969,7 +1000,7 @@ __global__ void __launch_bounds__(kRedThreads) k_oc_eval(const float* __restrict
         const float xf = x[i], dcf = dc[i];
         const uint8_t pr = pres ? pres[i] : 0, vv = vd ? vd[i] : 0;
 #pragma unroll
-        for (int k = 0; k < kOcCand; k++) v[k] += (double)oc_xnew(xf, dcf, lam[k], move, pr, vv);
+        for (int k = 0; k < kOcCand; k++) v[k] += (double)oc_xnew_any(mech, xf, dcf, lam[k], move, pr, vv);
     }
     double tot[kOcCand];
     if (grid_reduce<kOcCand>(v, partials, ticket, tot)) {
@@ -991,10 +1022,10 @@ __global__ void __launch_bounds__(kRedThreads) k_oc_eval(const float* __restrict
 // array by the last executed pass)
 __global__ void k_oc_final(const float* __restrict__ x, const float* __restrict__ dc, const uint8_t* __restrict__ pres,
                            const uint8_t* __restrict__ vd, double move, int64_t n, const double* lmid_last,
-                           float* __restrict__ xnew) {
+                           float* __restrict__ xnew, int mech) {
     const double lm = *lmid_last;
     for (int64_t i = gtid(); i < n; i += gstride())
-        xnew[i] = oc_xnew(x[i], dc[i], lm, move, pres ? pres[i] : 0, vd ? vd[i] : 0);
+        xnew[i] = oc_xnew_any(mech, x[i], dc[i], lm, move, pres ? pres[i] : 0, vd ? vd[i] : 0);
 }
 
 // change = max |xnew - xold|, volume = sum xnew

@@ -17,6 +17,7 @@
 #include <vector>
 #include "kernels.hpp"
 #include "kernels_xm.hpp"
+#include "kernels_adj.hpp"
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
@@ -614,6 +615,7 @@ struct MGH {
     int xm_nt = 0;        // x-march workgroup size (0 auto, 512, 1024)
     int xm_cx = 0;        // x-march planes per workgroup (0 auto)
     DevKe* K0 = nullptr;
+    Springs springs{};    // mechanism ports on the finest level
     template <typename T> const T* ke() const;
     template <typename T> HadCoef<T> had() const;
 
@@ -689,6 +691,8 @@ struct MGH {
         MGLevel& l0 = L[0];
         hipLaunchKernelGGL(k_invdiag, dim3(nblk(l0.g.nnodes())), dim3(256), 0, c.s, l0.E, thr, l0.fixed, l0.invD,
                            l0.invD32, l0.g, (const double*)K0->ke);
+        if (springs.n > 0)
+            hipLaunchKernelGGL(k_spring_diag, dim3(1), dim3(64), 0, c.s, l0.invD, l0.invD32, springs);
         c.check_launch();
         for (size_t li = 1; li < L.size(); li++) {
             MGLevel& f = L[li - 1];
@@ -916,6 +920,9 @@ struct MGH {
                                    ke<T>());
             }
         }
+        if (li == 0 && springs.n > 0)
+            hipLaunchKernelGGL(k_spring<T>, dim3(1), dim3(64), 0, c.s, in, out, springs, mode, (const uint8_t*)l.fixed,
+                               (const T*)Sel<T>::invD(l), om, gate);
     }
 
     // one V-cycle (mgVcycle assembly/index.ts:837-1048) for A x = b from
@@ -1156,6 +1163,14 @@ struct Solver {
     bool stencil_ready = false;
     DevKe K0;
     FilterStencil* stencil = nullptr;
+    // design map and adjoint in FP64 (Helmholtz filter, forward/backward)
+    double *xd64 = nullptr, *xf64 = nullptr, *xp64 = nullptr, *g64 = nullptr, *gv64 = nullptr, *t64 = nullptr;
+    double *hb = nullptr, *hy = nullptr, *hr = nullptr, *hz = nullptr, *hp = nullptr, *hAp = nullptr, *hinvD = nullptr;
+    HelmKe helm{};
+    double* Lvec = nullptr;       // mechanism: unit vector at the output DOF
+    double fb_penal = 3, fb_beta = 1;
+    bool fb_ready = false;        // a forward pass has run at the current design
+    int helm_iters = 0;
 
     void init(const topo_config* cfgin, const uint8_t* hfixed, const float* hF, const uint8_t* hpres,
               const uint8_t* hvoid) {
@@ -1167,7 +1182,10 @@ struct Solver {
         TOPO_CHECK(cfg.precision == TOPO_PREC_FP64 || cfg.precision == TOPO_PREC_MIXED, TOPO_E_ARG,
                    "unknown precision %d", cfg.precision);
         TOPO_CHECK(hfixed && hF, TOPO_E_ARG, "fixedMask and F are required");
-        TOPO_CHECK(cfg.objective == TOPO_OBJ_COMPLIANCE, TOPO_E_UNSUPPORTED, "objective %d not built", cfg.objective);
+        TOPO_CHECK(cfg.objective == TOPO_OBJ_COMPLIANCE || cfg.objective == TOPO_OBJ_MECHANISM, TOPO_E_ARG,
+                   "unknown objective %d", cfg.objective);
+        TOPO_CHECK(cfg.filter_type >= TOPO_FILTER_SENSITIVITY && cfg.filter_type <= TOPO_FILTER_HELMHOLTZ, TOPO_E_ARG,
+                   "unknown filter_type %d", cfg.filter_type);
         g.nelx = cfg.nelx; g.nely = cfg.nely; g.nelz = cfg.nelz;
         nel = g.nel();
         ndof = g.ndof();
@@ -1180,6 +1198,17 @@ struct Solver {
             K0.set(C, hke);
         }
         mg.K0 = &K0;
+        if (cfg.objective == TOPO_OBJ_MECHANISM) {
+            TOPO_CHECK(cfg.nload == 1, TOPO_E_ARG, "the mechanism objective takes one load case (the input force)");
+            const int64_t din = cfg.mech_in_dof, dout = cfg.mech_out_dof;
+            TOPO_CHECK(din >= 0 && din < g.ndof() && dout >= 0 && dout < g.ndof() && din != dout, TOPO_E_ARG,
+                       "mechanism DOFs out of range");
+            TOPO_CHECK(!hfixed[din] && !hfixed[dout], TOPO_E_ARG, "mechanism DOFs must be free");
+            Springs sp{};
+            if (cfg.mech_k_in > 0) { sp.dof[sp.n] = din; sp.k[sp.n] = cfg.mech_k_in; sp.n++; }
+            if (cfg.mech_k_out > 0) { sp.dof[sp.n] = dout; sp.k[sp.n] = cfg.mech_k_out; sp.n++; }
+            mg.springs = sp;
+        }
         stencil = C.alloc<FilterStencil>(1);
         fixed = C.alloc<uint8_t>(ndof);
         TOPO_HIP(hipMemcpy(fixed, hfixed, ndof, hipMemcpyHostToDevice));
@@ -1213,6 +1242,21 @@ struct Solver {
         E = C.alloc<double>(nel);
         const bool mixed = cfg.precision == TOPO_PREC_MIXED;
         if (mixed) E32 = C.alloc<float>(nel);
+        xd64 = C.alloc<double>(nel); xf64 = C.alloc<double>(nel); xp64 = C.alloc<double>(nel);
+        g64 = C.alloc<double>(nel); gv64 = C.alloc<double>(nel); t64 = C.alloc<double>(nel);
+        if (cfg.filter_type == TOPO_FILTER_HELMHOLTZ) {
+            const int64_t nn = g.nnodes();
+            hb = C.alloc<double>(nn); hy = C.alloc<double>(nn); hr = C.alloc<double>(nn); hz = C.alloc<double>(nn);
+            hp = C.alloc<double>(nn); hAp = C.alloc<double>(nn); hinvD = C.alloc<double>(nn);
+        }
+        if (cfg.objective == TOPO_OBJ_MECHANISM) {
+            adj = C.alloc<double>(ndof);
+            Lvec = C.alloc<double>(ndof);
+            std::vector<double> hl(ndof, 0.0);
+            hl[cfg.mech_out_dof] = 1.0;
+            TOPO_HIP(hipMemcpy(Lvec, hl.data(), ndof * sizeof(double), hipMemcpyHostToDevice));
+            hipLaunchKernelGGL(k_fill<double>, dim3(nblk(ndof)), dim3(256), 0, C.s, adj, 0.0, ndof, (const int*)nullptr);
+        }
         U = C.alloc<double>(ndof * cfg.nload);
         hipLaunchKernelGGL(k_fill<double>, dim3(nblk(ndof * cfg.nload)), dim3(256), 0, C.s, U, 0.0, ndof * cfg.nload,
                            (const int*)nullptr);
@@ -1255,7 +1299,34 @@ struct Solver {
         C.sync();
     }
 
+    // Helmholtz filter element matrix r^2 K_L + M of the unit cube with the
+    // PDE length r = rmin / (2 sqrt 3) (the equivalence of Lazarov & Sigmund
+    // 2011 between the PDE filter and a linear filter of radius rmin)
+    void setup_helmholtz(double rmin) {
+        const double rp = rmin / (2.0 * std::sqrt(3.0));
+        const double m1[2][2] = {{1.0 / 3.0, 1.0 / 6.0}, {1.0 / 6.0, 1.0 / 3.0}};
+        const double k1[2][2] = {{1.0, -1.0}, {-1.0, 1.0}};
+        for (int i = 0; i < 8; i++)
+            for (int j = 0; j < 8; j++) {
+                int ai, bi, ci, aj, bj, cj;
+                local_offsets(i, ai, bi, ci);
+                local_offsets(j, aj, bj, cj);
+                const double mx = m1[ai][aj], my = m1[bi][bj], mz = m1[ci][cj];
+                const double kl = k1[ai][aj] * my * mz + mx * k1[bi][bj] * mz + mx * my * k1[ci][cj];
+                helm.a[i * 8 + j] = rp * rp * kl + mx * my * mz;
+            }
+        Ctx& C = *c;
+        hipLaunchKernelGGL(k_helm_invdiag, dim3(nblk(g.nnodes())), dim3(256), 0, C.s, hinvD, g, helm.a[0]);
+        C.sync();
+        C.check_launch();
+        stencil_ready = true;
+    }
+
     void setup_filter(double rmin) {
+        if (cfg.filter_type == TOPO_FILTER_HELMHOLTZ) {
+            setup_helmholtz(rmin);
+            return;
+        }
         FilterStencil st;
         std::memset(&st, 0, sizeof st);
         if (cfg.filter_type == TOPO_FILTER_DENSITY) {
@@ -1306,6 +1377,7 @@ struct Solver {
         TOPO_HIP(hipMemcpyAsync(tmpf, hx, nel * sizeof(float), hipMemcpyHostToDevice, C.s));
         hipLaunchKernelGGL(k_ref_to_zyx, dim3(nblk(nel)), dim3(256), 0, C.s, tmpf, x, g);
         C.sync();
+        fb_ready = false;
     }
     void get_field_f(const float* dsrc, float* hout) {
         Ctx& C = *c;
@@ -1334,7 +1406,6 @@ struct Solver {
 
     // solve K U_k = F_k for every load case (FE() :4155-4371)
     void fe(int iteration, int maxIt, topo_step_info* info) {
-        Ctx& C = *c;
         const double tol = tolerance_for(iteration, maxIt);
         const int64_t nfree_cap = cfg.cg_max_iter;
         const bool mgp = cfg.linear_solver == TOPO_SOLVER_MGPCG;
@@ -1346,19 +1417,10 @@ struct Solver {
         const int maxit = (int)std::min<int64_t>(nfree_cap, nfree);
         CGResult tot;
         int outer = 0;
-        for (int k = 0; k < cfg.nload; k++) {
-            double* Uk = U + (int64_t)k * ndof;
-            const double* Fk = F + (int64_t)k * ndof;
-            if (cfg.precision == TOPO_PREC_FP64) {
-                CGResult r = mgp ? pcg_mg<double, double>(C, mg, Fk, Uk, warm, v64, tol, maxit, &cancel_flag)
-                                 : pcg_jacobi<double, double>(C, mg, Fk, Uk, warm, v64, tol, maxit, &cancel_flag);
-                TOPO_CHECK(!r.nonfinite, TOPO_E_BREAKDOWN, "non-finite CG scalar");
-                tot.iters += r.iters; tot.vcycles += r.vcycles; tot.applyA += r.applyA;
-            } else {
-                CGResult r = solve_mixed(Fk, Uk, warm, tol, maxit, mgp, &outer);
-                tot.iters += r.iters; tot.vcycles += r.vcycles; tot.applyA += r.applyA;
-            }
-        }
+        for (int k = 0; k < cfg.nload; k++)
+            solve_one(F + (int64_t)k * ndof, U + (int64_t)k * ndof, warm, tol, maxit, tot, outer);
+        // the mechanism's adjoint: K lam = L (L = unit vector at the output DOF)
+        if (cfg.objective == TOPO_OBJ_MECHANISM) solve_one(Lvec, adj, warm, tol, maxit, tot, outer);
         have_U = true;
         if (info) {
             info->cg_iters = tot.iters;
@@ -1367,6 +1429,151 @@ struct Solver {
             info->outer_iters = outer;
             info->tolerance = tol;
         }
+    }
+
+    // K sol = rhs with the configured solver and precision
+    void solve_one(const double* rhs, double* sol, bool warm, double tol, int maxit, CGResult& tot, int& outer) {
+        Ctx& C = *c;
+        const bool mgp = cfg.linear_solver == TOPO_SOLVER_MGPCG;
+        if (cfg.precision == TOPO_PREC_FP64) {
+            CGResult r = mgp ? pcg_mg<double, double>(C, mg, rhs, sol, warm, v64, tol, maxit, &cancel_flag)
+                             : pcg_jacobi<double, double>(C, mg, rhs, sol, warm, v64, tol, maxit, &cancel_flag);
+            TOPO_CHECK(!r.nonfinite, TOPO_E_BREAKDOWN, "non-finite CG scalar");
+            tot.iters += r.iters; tot.vcycles += r.vcycles; tot.applyA += r.applyA;
+        } else {
+            CGResult r = solve_mixed(rhs, sol, warm, tol, maxit, mgp, &outer);
+            tot.iters += r.iters; tot.vcycles += r.vcycles; tot.applyA += r.applyA;
+        }
+    }
+
+    // Helmholtz filter of an element field (FP64, zyx): b = T in, Jacobi-PCG
+    // on (r^2 K_L + M) y = b to 1e-12 relative, out = T^T y
+    void helm_solve(const double* in, double* out) {
+        Ctx& C = *c;
+        const int64_t nn = g.nnodes();
+        hipLaunchKernelGGL(k_helm_rhs<double>, dim3(nblk(nn)), dim3(256), 0, C.s, in, hb, g);
+        hipLaunchKernelGGL(k_fill<double>, dim3(nblk(nn)), dim3(256), 0, C.s, hy, 0.0, nn, (const int*)nullptr);
+        hipLaunchKernelGGL((k_convert<double, double>), dim3(nblk(nn)), dim3(256), 0, C.s, hr, (const double*)hb, nn);
+        hipLaunchKernelGGL(k_jacobi_apply<double>, dim3(nblk(nn)), dim3(256), 0, C.s, hz, (const double*)hr,
+                           (const double*)hinvD, nn);
+        hipLaunchKernelGGL((k_convert<double, double>), dim3(nblk(nn)), dim3(256), 0, C.s, hp, (const double*)hz, nn);
+        cg_set(C, 1e-12, 5000);
+        hipLaunchKernelGGL(k_cg_start<double>, dim3(kRedBlocks), dim3(kRedThreads), 0, C.s, (const double*)hr,
+                           (const double*)hz, nn, C.st, C.partials, C.ticket);
+        const int* gate = &C.st->done;
+        for (;;) {
+            const CGState& h = C.read_st();
+            if (h.done) {
+                TOPO_CHECK(!h.nonfinite, TOPO_E_BREAKDOWN, "non-finite CG scalar in the Helmholtz filter");
+                helm_iters = h.iters;
+                break;
+            }
+            for (int k = 0; k < 8; k++) {
+                hipLaunchKernelGGL(k_helm_apply, dim3(nblk(nn)), dim3(256), 0, C.s, (const double*)hp, hAp, g, helm, 0,
+                                   (const double*)nullptr, gate);
+                hipLaunchKernelGGL(k_cg_alpha<double>, dim3(kRedBlocks), dim3(kRedThreads), 0, C.s, (const double*)hp,
+                                   (const double*)hAp, nn, C.st, C.partials, C.ticket);
+                hipLaunchKernelGGL((k_cg_update_jacobi<double, double>), dim3(kRedBlocks), dim3(kRedThreads), 0, C.s,
+                                   hy, hr, hz, (const double*)hp, (const double*)hAp, (const double*)hinvD, nn, C.st,
+                                   C.partials, C.ticket);
+                hipLaunchKernelGGL(k_cg_pupdate<double>, dim3(nblk(nn)), dim3(256), 0, C.s, hp, (const double*)hz, nn,
+                                   C.st);
+            }
+            C.check_launch();
+        }
+        hipLaunchKernelGGL(k_helm_out, dim3(nblk(nel)), dim3(256), 0, C.s, (const double*)hy, out, g);
+        C.check_launch();
+    }
+
+    // penalty continuation (:2056-2064, :2130) and beta continuation (:2133)
+    // at SIMP iteration `it`
+    void schedule(int it, double* penal, double* beta) const {
+        *penal = cfg.penal_start +
+                 (cfg.penal - cfg.penal_start) * std::min(1.0, (double)(it - 1) / std::max(1, cfg.continuation_iters));
+        *beta = 1.0;
+        if (cfg.use_projection)
+            *beta = cfg.beta_fixed > 0 ? (double)cfg.beta_fixed
+                                       : std::min(cfg.beta_max, std::pow(2.0, std::floor((double)(it - 1) / cfg.beta_interval)));
+    }
+
+    // Forward pass in FP64 at the current design x and SIMP iteration `it`:
+    // x -> filter (density or Helmholtz; the sensitivity filter acts on the
+    // gradient instead) -> Heaviside (if on) -> E -> solve(s) -> objective
+    // (compliance sum_k F_k^T U_k, or the mechanism's u_out = L^T U). Returns
+    // the objective; *vol = mean(xPhys).
+    double forward(int it, topo_step_info* info, double* vol) {
+        Ctx& C = *c;
+        schedule(it, &fb_penal, &fb_beta);
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(nblk(nel)), dim3(256), 0, C.s, xd64, (const float*)x, nel);
+        if (cfg.filter_type == TOPO_FILTER_DENSITY)
+            hipLaunchKernelGGL(k_density_filter_d, dim3(nblk(nel)), dim3(256), 0, C.s, (const double*)xd64, xf64, sw, g, 0,
+                               (const FilterStencil*)stencil);
+        else if (cfg.filter_type == TOPO_FILTER_HELMHOLTZ)
+            helm_solve(xd64, xf64);
+        else
+            hipLaunchKernelGGL((k_convert<double, double>), dim3(nblk(nel)), dim3(256), 0, C.s, xf64, (const double*)xd64,
+                               nel);
+        if (cfg.use_projection)
+            hipLaunchKernelGGL(k_heaviside_d, dim3(nblk(nel)), dim3(256), 0, C.s, (const double*)xf64, xp64, nel, fb_beta,
+                               cfg.eta);
+        else
+            hipLaunchKernelGGL((k_convert<double, double>), dim3(nblk(nel)), dim3(256), 0, C.s, xp64, (const double*)xf64,
+                               nel);
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(nel)), dim3(256), 0, C.s, xphys, (const double*)xp64, nel);
+        hipLaunchKernelGGL(k_moduli<double>, dim3(nblk(nel)), dim3(256), 0, C.s, (const double*)xp64, fb_penal, cfg.E0,
+                           cfg.Emin, 0, E, E32, nel);
+        C.check_launch();
+        mg.update(C);
+        C.sync();
+        fe(it, cfg.max_iterations, info);
+        double J;
+        if (cfg.objective == TOPO_OBJ_MECHANISM) {
+            TOPO_HIP(hipMemcpyAsync(C.h_sc, U + cfg.mech_out_dof, sizeof(double), hipMemcpyDeviceToHost, C.s));
+            C.sync();
+            J = C.h_sc[0];
+        } else {
+            J = compliance();
+        }
+        hipLaunchKernelGGL((k_dot<double, double>), dim3(kRedBlocks), dim3(kRedThreads), 0, C.s, (const double*)xp64,
+                           (const double*)nullptr, nel, C.dsc + 3, C.partials, C.ticket);
+        if (vol) *vol = C.read_d(C.dsc + 3) / (double)nel;
+        fb_ready = true;
+        return J;
+    }
+
+    // Backward pass (hand-written adjoint of forward()): g64 = dJ/dx and
+    // gv64 = d mean(xPhys)/dx, zyx, FP64.
+    void backward() {
+        Ctx& C = *c;
+        TOPO_CHECK(fb_ready, TOPO_E_ARG, "backward needs a forward pass at the current design");
+        const double* lam = cfg.objective == TOPO_OBJ_MECHANISM ? adj : nullptr;
+        for (int k = 0; k < cfg.nload; k++) {
+            const double* Uk = U + (int64_t)k * ndof;
+            hipLaunchKernelGGL(k_adj_sens, dim3(nblk(nel)), dim3(256), 0, C.s, Uk, lam ? lam : Uk, (const double*)xp64,
+                               fb_penal, cfg.E0, cfg.Emin, g, g64, k > 0 ? 1 : 0, (const double*)K0.ke);
+        }
+        hipLaunchKernelGGL(k_fill<double>, dim3(nblk(nel)), dim3(256), 0, C.s, gv64, 1.0 / (double)nel, nel,
+                           (const int*)nullptr);
+        if (cfg.use_projection) {
+            hipLaunchKernelGGL(k_heaviside_grad, dim3(nblk(nel)), dim3(256), 0, C.s, (const double*)xf64, g64, nel, fb_beta,
+                               cfg.eta);
+            hipLaunchKernelGGL(k_heaviside_grad, dim3(nblk(nel)), dim3(256), 0, C.s, (const double*)xf64, gv64, nel,
+                               fb_beta, cfg.eta);
+        }
+        for (double* v : {g64, gv64}) {
+            if (cfg.filter_type == TOPO_FILTER_DENSITY) {
+                hipLaunchKernelGGL(k_density_filter_d, dim3(nblk(nel)), dim3(256), 0, C.s, (const double*)v, t64, sw, g, 1,
+                                   (const FilterStencil*)stencil);
+                hipLaunchKernelGGL((k_convert<double, double>), dim3(nblk(nel)), dim3(256), 0, C.s, v, (const double*)t64,
+                                   nel);
+            } else if (cfg.filter_type == TOPO_FILTER_HELMHOLTZ) {
+                // the Helmholtz map is symmetric: its adjoint is itself
+                helm_solve(v, t64);
+                hipLaunchKernelGGL((k_convert<double, double>), dim3(nblk(nel)), dim3(256), 0, C.s, v, (const double*)t64,
+                                   nel);
+            }
+        }
+        C.check_launch();
     }
 
     // FP64 iterative refinement around an FP32 inner solve: the outer
@@ -1421,7 +1628,7 @@ struct Solver {
         return C.read_d(C.dsc + 2);
     }
 
-    int oc_run(const float* dx, const float* ddcn, float* dxnew, double volfrac, double* vol) {
+    int oc_run(const float* dx, const float* ddcn, float* dxnew, double volfrac, double* vol, int mech = 0) {
         Ctx& C = *c;
         OCState h;
         std::memset(&h, 0, sizeof h);
@@ -1434,7 +1641,7 @@ struct Solver {
             for (int k = 0; k < 3; k++) {
                 hipLaunchKernelGGL(k_oc_plan, dim3(1), dim3(64), 0, C.s, C.oc);
                 hipLaunchKernelGGL(k_oc_eval, dim3(kRedBlocks), dim3(kRedThreads), 0, C.s, dx, ddcn, pres, voidm,
-                                   cfg.move, nel, C.oc, C.partials, C.ticket);
+                                   cfg.move, nel, C.oc, C.partials, C.ticket, mech);
             }
             C.check_launch();
             TOPO_HIP(hipMemcpyAsync(C.h_oc, C.oc, sizeof(OCState), hipMemcpyDeviceToHost, C.s));
@@ -1443,7 +1650,7 @@ struct Solver {
         }
         TOPO_CHECK(C.h_oc->finished, TOPO_E_BREAKDOWN, "OC bisection did not terminate");
         hipLaunchKernelGGL(k_oc_final, dim3(nblk(nel)), dim3(256), 0, C.s, dx, ddcn, pres, voidm, cfg.move, nel,
-                           &C.oc->lmid_last, dxnew);
+                           &C.oc->lmid_last, dxnew, mech);
         C.check_launch();
         if (vol) *vol = C.h_oc->vol_last;
         return C.h_oc->passes;
@@ -1460,8 +1667,61 @@ struct Solver {
         C.check_launch();
     }
 
+    // SIMP iteration through the FP64 forward/backward pair: used for the
+    // mechanism objective and for the Helmholtz filter (no reference
+    // counterpart). The gradient is cast to float for the OC update (the
+    // mechanism's damped OC, or the reference OC for compliance); with the
+    // sensitivity filter it is filtered as in the reference first.
+    void step_fb(topo_step_info* info) {
+        Ctx& C = *c;
+        auto t0 = std::chrono::steady_clock::now();
+        topo_step_info loc;
+        std::memset(&loc, 0, sizeof loc);
+        if (cancel_flag) throw TopoError{TOPO_E_CANCELLED};
+        loop++;
+        TOPO_HIP(hipMemcpyAsync(xold, x, nel * sizeof(float), hipMemcpyDeviceToDevice, C.s));
+        double vphys = 0;
+        auto ts = std::chrono::steady_clock::now();
+        const double J = forward(loop, &loc, &vphys);
+        auto te = std::chrono::steady_clock::now();
+        backward();
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(nel)), dim3(256), 0, C.s, dc, (const double*)g64, nel);
+        if (cfg.filter_type == TOPO_FILTER_SENSITIVITY)
+            hipLaunchKernelGGL(k_sens_filter, dim3(nblk(nel)), dim3(256), 0, C.s, x, dc, Hs, dcn, g,
+                               (const FilterStencil*)stencil);
+        else
+            TOPO_HIP(hipMemcpyAsync(dcn, dc, nel * sizeof(float), hipMemcpyDeviceToDevice, C.s));
+        C.check_launch();
+        double vol = 0;
+        loc.oc_passes = oc_run(x, dcn, xnew, cfg.volfrac, &vol, cfg.objective == TOPO_OBJ_MECHANISM ? 1 : 0);
+        hipLaunchKernelGGL(k_change, dim3(kRedBlocks), dim3(kRedThreads), 0, C.s, xnew, xold, nel, C.dsc + 4, C.partials,
+                           C.ticket);
+        C.check_launch();
+        TOPO_HIP(hipMemcpyAsync(C.h_sc, C.dsc, 8 * sizeof(double), hipMemcpyDeviceToHost, C.s));
+        C.sync();
+        const double change = C.h_sc[4];
+        const double volume = C.h_sc[5];
+        std::swap(x, xnew);
+        fb_ready = false;
+        auto t1 = std::chrono::steady_clock::now();
+        loc.iteration = loop;
+        loc.compliance = J;
+        loc.change = change;
+        loc.penal = fb_penal;
+        loc.beta = fb_beta;
+        loc.volume = volume / (double)nel;
+        loc.t_solve_ms = std::chrono::duration<double, std::milli>(te - ts).count();
+        loc.t_total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        loc.t_post_ms = std::chrono::duration<double, std::milli>(t1 - te).count();
+        if (info) *info = loc;
+    }
+
     // one SIMP iteration (optimize() loop body js/optimizer-worker-3d.js:2124-2252)
     void step(topo_step_info* info) {
+        if (cfg.objective == TOPO_OBJ_MECHANISM || cfg.filter_type == TOPO_FILTER_HELMHOLTZ) {
+            step_fb(info);
+            return;
+        }
         Ctx& C = *c;
         auto t0 = std::chrono::steady_clock::now();
         topo_step_info loc;
@@ -1631,8 +1891,67 @@ int topo_solver_get_field(topo_solver* h, int32_t field, void* out) {
     case TOPO_FIELD_DC: S.get_field_f(S.dc, (float*)out); break;
     case TOPO_FIELD_DCN: S.get_field_f(S.dcn, (float*)out); break;
     case TOPO_FIELD_STRESS: S.get_field_f(S.stress, (float*)out); break;
+    case TOPO_FIELD_ADJOINT:
+        TOPO_CHECK(S.adj, TOPO_E_ARG, "no adjoint field (mechanism objective only)");
+        TOPO_HIP(hipMemcpyAsync(out, S.adj, S.ndof * sizeof(double), hipMemcpyDeviceToHost, S.c->s));
+        S.c->sync();
+        break;
     default: TOPO_CHECK(false, TOPO_E_ARG, "unknown field %d", field);
     }
+    return TOPO_OK;
+    TOPO_API_END
+}
+
+int topo_solver_forward(topo_solver* h, double* objective, double* volume, topo_step_info* info) {
+    TOPO_API_BEGIN
+    TOPO_CHECK(h && objective, TOPO_E_ARG, "null argument");
+    Solver& S = h->s;
+    topo_step_info loc;
+    std::memset(&loc, 0, sizeof loc);
+    double vol = 0;
+    *objective = S.forward(std::max(1, S.loop), &loc, &vol);
+    loc.compliance = *objective;
+    loc.penal = S.fb_penal;
+    loc.beta = S.fb_beta;
+    loc.volume = vol;
+    if (volume) *volume = vol;
+    if (info) *info = loc;
+    return TOPO_OK;
+    TOPO_API_END
+}
+
+int topo_solver_backward(topo_solver* h, double* dJdx, double* dVdx) {
+    TOPO_API_BEGIN
+    TOPO_CHECK(h && dJdx, TOPO_E_ARG, "null argument");
+    Solver& S = h->s;
+    Ctx& C = *S.c;
+    S.backward();
+    for (int q = 0; q < 2; q++) {
+        double* dst = q == 0 ? dJdx : dVdx;
+        if (!dst) continue;
+        hipLaunchKernelGGL(k_zyx_to_ref_t<double>, dim3(nblk(S.nel)), dim3(256), 0, C.s,
+                           (const double*)(q == 0 ? S.g64 : S.gv64), S.t64, S.g);
+        TOPO_HIP(hipMemcpyAsync(dst, S.t64, S.nel * sizeof(double), hipMemcpyDeviceToHost, C.s));
+        C.sync();
+    }
+    return TOPO_OK;
+    TOPO_API_END
+}
+
+int topo_op_helmholtz(topo_solver* h, const double* in, double* out, int32_t* iters) {
+    TOPO_API_BEGIN
+    TOPO_CHECK(h && in && out, TOPO_E_ARG, "null argument");
+    Solver& S = h->s;
+    TOPO_CHECK(S.cfg.filter_type == TOPO_FILTER_HELMHOLTZ, TOPO_E_ARG, "solver was not created with the Helmholtz filter");
+    Ctx& C = *S.c;
+    TOPO_HIP(hipMemcpyAsync(S.t64, in, S.nel * sizeof(double), hipMemcpyHostToDevice, C.s));
+    hipLaunchKernelGGL(k_ref_to_zyx_t<double>, dim3(nblk(S.nel)), dim3(256), 0, C.s, (const double*)S.t64, S.g64, S.g);
+    S.helm_solve(S.g64, S.gv64);
+    hipLaunchKernelGGL(k_zyx_to_ref_t<double>, dim3(nblk(S.nel)), dim3(256), 0, C.s, (const double*)S.gv64, S.t64, S.g);
+    TOPO_HIP(hipMemcpyAsync(out, S.t64, S.nel * sizeof(double), hipMemcpyDeviceToHost, C.s));
+    C.sync();
+    S.fb_ready = false;
+    if (iters) *iters = S.helm_iters;
     return TOPO_OK;
     TOPO_API_END
 }
@@ -1740,6 +2059,7 @@ int topo_op_filter(topo_solver* h, int32_t type, const float* xh, const float* i
     TOPO_API_BEGIN
     TOPO_CHECK(h && in && out, TOPO_E_ARG, "null argument");
     Solver& S = h->s;
+    TOPO_CHECK(S.cfg.filter_type != TOPO_FILTER_HELMHOLTZ, TOPO_E_ARG, "use topo_op_helmholtz for the Helmholtz filter");
     TOPO_CHECK(type == (S.cfg.filter_type == TOPO_FILTER_DENSITY ? TOPO_FILTER_DENSITY : TOPO_FILTER_SENSITIVITY),
                TOPO_E_ARG, "filter type %d does not match the solver's filter", type);
     TOPO_CHECK(type != TOPO_FILTER_SENSITIVITY || xh, TOPO_E_ARG, "sensitivity filter needs x");
@@ -1815,6 +2135,7 @@ int topo_time_op(topo_solver* h, int32_t op, int32_t prec, int32_t level, int32_
                                S.cfg.Emin, 0, 1.0, 0.5, S.g, S.w, S.stress, S.dc, (const double*)S.K0.ke);
             break;
         case TOPO_TIME_FILTER:
+            TOPO_CHECK(S.cfg.filter_type != TOPO_FILTER_HELMHOLTZ, TOPO_E_ARG, "no timing op for the Helmholtz filter");
             S.filter_fwd(S.cfg.filter_type == TOPO_FILTER_DENSITY ? TOPO_FILTER_DENSITY : TOPO_FILTER_SENSITIVITY, S.x,
                          S.dc, S.dcn);
             break;

@@ -7,8 +7,9 @@ the seed of the random initial densities.
 """
 import numpy as np
 
-from . import (DOF_ALL, FIX_BOTTOM_CORNERS, FIX_LEFT, LOAD_TOP_CENTER, PREC_FP64, SOLVER_MGPCG, SOLVER_PCG,
-               COARSE_CHOLESKY, FILTER_SENSITIVITY, default_config, dims, fixed_dofs, load_vector, preserved_elements)
+from . import (DOF_ALL, FIX_BOTTOM_CORNERS, FIX_LEFT, LOAD_TOP_CENTER, PREC_FP64, PREC_MIXED, SOLVER_MGPCG,
+               SOLVER_PCG, COARSE_CHOLESKY, FILTER_HELMHOLTZ, FILTER_SENSITIVITY, OBJ_MECHANISM, default_config, dims,
+               fixed_dofs, load_vector, preserved_elements)
 
 SEED = 20261203
 
@@ -122,3 +123,39 @@ def bridge_c4(**kw):
                 F[k, 3 * node_index(nely, nelz, ix, iy, nelz) + 2] = per
     x0 = np.full(nel, np.float32(cfg.volfrac), np.float32)
     return cfg, fixed, F.ravel(), None, None, x0
+
+
+def inverter(nelx=160, nely=80, nelz=80, **kw):
+    """Force-inverter compliant mechanism. Input: +x force 1.0 at the centre
+    node of the x = 0 face, input spring k_in = 1 on that DOF; output: the x
+    DOF of the centre node of the x = max face with spring k_out = 0.001; the
+    objective is that output displacement (minimised: the output moves
+    against the input). Supports (not fixed by the workload statement;
+    chosen here): every DOF of the x = 0 face nodes in the four corner
+    squares of side nely/8 x nelz/8. Helmholtz filter rmin 2 (PDE length
+    2 / (2 sqrt 3)), Heaviside beta 8, eta 0.5, volfrac 0.3, p = 3, FP32
+    MG-PCG inside FP64 refinement (4 levels, Cholesky coarsest), tol 1e-6."""
+    cfg = default_config(nelx, nely, nelz, volfrac=0.3, penal=3.0, penal_start=3.0, rmin=2.0,
+                         filter_type=FILTER_HELMHOLTZ, use_projection=1, beta_fixed=8, eta=0.5, max_iterations=20,
+                         cg_tol=1e-6, skip_factor=0.0, mg_omega_cap=1.6, objective=OBJ_MECHANISM,
+                         linear_solver=SOLVER_MGPCG, precision=PREC_MIXED, mg_levels=4, mg_omega=0.6,
+                         coarse_solver=COARSE_CHOLESKY, galerkin_max_ndof=0, mech_k_in=1.0, mech_k_out=0.001)
+    for k, v in kw.items():
+        setattr(cfg, k, v)
+    nel, ndof = dims(nelx, nely, nelz)
+    fixed = np.zeros(ndof, np.uint8)
+    sy, sz = max(1, nely // 8), max(1, nelz // 8)
+    for iy in range(nely + 1):
+        for iz in range(nelz + 1):
+            if min(iy, nely - iy) <= sy and min(iz, nelz - iz) <= sz:
+                n = node_index(nely, nelz, 0, iy, iz)
+                fixed[3 * n:3 * n + 3] = 1
+    n_in = node_index(nely, nelz, 0, nely // 2, nelz // 2)
+    n_out = node_index(nely, nelz, nelx, nely // 2, nelz // 2)
+    assert not fixed[3 * n_in], "grid too small: the input node falls in a support square"
+    cfg.mech_in_dof = 3 * n_in
+    cfg.mech_out_dof = 3 * n_out
+    F = np.zeros(ndof, np.float32)
+    F[3 * n_in] = 1.0
+    x0 = np.full(nel, np.float32(cfg.volfrac), np.float32)
+    return cfg, fixed, F, None, None, x0
