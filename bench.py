@@ -76,6 +76,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--mixed", choices=("pc", "ir"), default="pc",
+                    help="pc: FP64 MG-PCG with an FP32 V-cycle preconditioner; ir: FP64 iterative refinement "
+                         "around an FP32 inner MG-PCG")
     ap.add_argument("--profile-kernel", action="store_true")
     args = ap.parse_args()
     if args.profile_kernel:
@@ -93,7 +96,8 @@ def main():
     nel, ndof = T.dims(*grid)
 
     # mixed precision (the headline)
-    S, infos, t_mixed = run_solver(T, P, grid, T.PREC_MIXED, args.steps, args.warmup, args.levels)
+    mixed_prec = T.PREC_MIXED_PC if args.mixed == "pc" else T.PREC_MIXED
+    S, infos, t_mixed = run_solver(T, P, grid, mixed_prec, args.steps, args.warmup, args.levels)
     # compliance of the mixed path must match the FP64 path (checked below)
     # dominant kernel: fine-level FP32 applyA, timed with HIP events on the solver's stream
     ms32, by32 = S.time_op(T.TIME_APPLYA, T.PREC_MIXED, 0, 50)

@@ -53,7 +53,9 @@ enum { TOPO_LOAD_DOWN = 0, TOPO_LOAD_UP = 1, TOPO_LOAD_LEFT = 2,
 enum { TOPO_SOLVER_PCG = 0,      /* Jacobi-PCG, ebePCG semantics              */
        TOPO_SOLVER_MGPCG = 1 };  /* geometric MG-PCG, ebeMGPCG semantics      */
 enum { TOPO_PREC_FP64 = 0,       /* everything in FP64                        */
-       TOPO_PREC_MIXED = 1 };    /* FP32 operator/V-cycle inside FP64 refinement */
+       TOPO_PREC_MIXED = 1,      /* FP32 operator/V-cycle inside FP64 refinement */
+       TOPO_PREC_MIXED_PC = 2 }; /* FP64 MG-PCG, FP32 V-cycle preconditioner (MG-PCG only;
+                                    Jacobi-PCG runs in FP64)                   */
 enum { TOPO_COARSE_SWEEPS = 0,   /* reference: 30 extra damped-Jacobi sweeps  */
        TOPO_COARSE_CHOLESKY = 1};/* FP64 dense Cholesky of the coarsest level */
 enum { TOPO_FILTER_SENSITIVITY = 0,  /* reference filterSensitivities       */

@@ -18,7 +18,7 @@ FIX_LEFT, FIX_RIGHT, FIX_BOTTOM, FIX_TOP, FIX_BOTTOM_CORNERS, FIX_NONE = range(6
 DOF_X, DOF_Y, DOF_Z, DOF_ALL = 1, 2, 4, 7
 LOAD_DOWN, LOAD_UP, LOAD_LEFT, LOAD_RIGHT, LOAD_TOP_CENTER, LOAD_NONE = range(6)
 SOLVER_PCG, SOLVER_MGPCG = 0, 1
-PREC_FP64, PREC_MIXED = 0, 1
+PREC_FP64, PREC_MIXED, PREC_MIXED_PC = 0, 1, 2
 COARSE_SWEEPS, COARSE_CHOLESKY = 0, 1
 FILTER_SENSITIVITY, FILTER_DENSITY, FILTER_HELMHOLTZ = 0, 1, 2
 OBJ_COMPLIANCE, OBJ_MECHANISM = 0, 1

@@ -294,6 +294,7 @@ struct CGState {
     int iters;        // iterations completed
     int maxit;
     int nonfinite;
+    double guard;     // denominator guard: 1e-12 in the reference's PCGs, 0 elsewhere
 };
 
 // rn = ||r||^2 and (optionally) rz = r.z; used at CG start
@@ -328,7 +329,7 @@ __global__ void __launch_bounds__(kRedThreads) k_cg_alpha(const T* __restrict__ 
     double tot[1];
     if (grid_reduce<1>(v, partials, ticket, tot)) {
         st->pAp = tot[0];
-        st->alpha = st->rz / (tot[0] + 1e-12);
+        st->alpha = st->rz / (tot[0] + st->guard);
         if (!isfinite(st->alpha)) st->nonfinite = 1;
     }
 }
@@ -357,7 +358,7 @@ __global__ void __launch_bounds__(kRedThreads) k_cg_update_jacobi(TU* __restrict
     if (grid_reduce<2>(v, partials, ticket, tot)) {
         st->rz_new = tot[0];
         st->rn = tot[1];
-        st->beta = tot[0] / (st->rz + 1e-12);
+        st->beta = tot[0] / (st->rz + st->guard);
         st->rz = tot[0];
         st->iters += 1;
         if (!isfinite(tot[0]) || !isfinite(tot[1])) st->nonfinite = 1;
@@ -401,7 +402,7 @@ __global__ void __launch_bounds__(kRedThreads) k_cg_rz(const T* __restrict__ r, 
     double tot[1];
     if (grid_reduce<1>(v, partials, ticket, tot)) {
         st->rz_new = tot[0];
-        st->beta = tot[0] / (st->rz + 1e-12);
+        st->beta = tot[0] / (st->rz + st->guard);
         st->rz = tot[0];
         if (!isfinite(tot[0])) st->nonfinite = 1;
     }
@@ -428,6 +429,11 @@ __global__ void k_fill(T* __restrict__ v, T val, int64_t n, const int* gate) {
 }
 template <typename TD, typename TS>
 __global__ void k_convert(TD* __restrict__ d, const TS* __restrict__ s, int64_t n) {
+    for (int64_t i = gtid(); i < n; i += gstride()) d[i] = (TD)s[i];
+}
+template <typename TD, typename TS>
+__global__ void k_convert_gated(TD* __restrict__ d, const TS* __restrict__ s, int64_t n, const int* gate) {
+    if (gate && *gate) return;
     for (int64_t i = gtid(); i < n; i += gstride()) d[i] = (TD)s[i];
 }
 template <typename T>

@@ -265,9 +265,7 @@ struct MGLevel {
     int32_t* freeIdx = nullptr;
     int32_t* freeOf = nullptr;     // DOF -> free index or -1
     double* bf = nullptr;          // gathered right-hand side (padded)
-    float* bf32 = nullptr;
     double* Kinv = nullptr;
-    float* Kinv32 = nullptr;
     double* work = nullptr;
 };
 
@@ -280,7 +278,6 @@ template <> struct Sel<double> {
     static double* invD(MGLevel& l) { return l.invD; }
     static double* K(MGLevel& l) { return l.K; }
     static double* Kinv(MGLevel& l) { return l.Kinv; }
-    static double* bf(MGLevel& l) { return l.bf; }
     static const double* E(MGLevel& l) { return l.E; }
     typedef double TE;
 };
@@ -291,8 +288,7 @@ template <> struct Sel<float> {
     static float* res(MGLevel& l) { return l.res32; }
     static float* invD(MGLevel& l) { return l.invD32; }
     static float* K(MGLevel& l) { return l.K32; }
-    static float* Kinv(MGLevel& l) { return l.Kinv32; }
-    static float* bf(MGLevel& l) { return l.bf32; }
+
     static const float* E(MGLevel& l) { return l.E32; }
     typedef float TE;
 };
@@ -311,38 +307,34 @@ __global__ void k_jacobi_first(T* __restrict__ x, const T* __restrict__ b, const
 // Kinv[i][j] bf[j]; x = 0 on fixed DOFs.
 template <typename T>
 __global__ void k_gather_free(const T* __restrict__ b, const int32_t* __restrict__ freeIdx, int64_t nf, int64_t nfp,
-                              T* __restrict__ bf, const int* gate) {
+                              double* __restrict__ bf, const int* gate) {
     if (gate && *gate) return;
-    for (int64_t j = gtid(); j < nfp; j += gstride()) bf[j] = j < nf ? b[freeIdx[j]] : T(0);
+    for (int64_t j = gtid(); j < nfp; j += gstride()) bf[j] = j < nf ? (double)b[freeIdx[j]] : 0.0;
 }
-template <typename T> struct Vec4;
-template <> struct Vec4<float> { typedef float4 type; static constexpr int n = 4; };
-template <> struct Vec4<double> { typedef double2 type; static constexpr int n = 2; };
 // one wave per 4 consecutive rows; each lane streams 16-byte vectors of the
-// four rows against the same slice of bf (read once per wave from L2)
+// four rows against the same slice of bf (read once per wave from L2). The
+// inverse and the accumulation are FP64 in every precision; x receives the
+// result in the V-cycle's precision T.
 template <typename T>
-__global__ void __launch_bounds__(256) k_inv_apply(const T* __restrict__ Kinv, const int32_t* __restrict__ freeIdx,
-                                                   int64_t nf, int64_t nfp, const T* __restrict__ bf,
+__global__ void __launch_bounds__(256) k_inv_apply(const double* __restrict__ Kinv, const int32_t* __restrict__ freeIdx,
+                                                   int64_t nf, int64_t nfp, const double* __restrict__ bf,
                                                    T* __restrict__ x, const int* gate) {
     if (gate && *gate) return;
-    typedef typename Vec4<T>::type V;
-    constexpr int VN = Vec4<T>::n;
     const int lane = threadIdx.x & 63;
     const int64_t row0 = (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 4;
     if (row0 >= nf) return;
-    T acc[4] = {T(0), T(0), T(0), T(0)};
-    const V* bv = (const V*)bf;
-    const int64_t nv = nfp / VN;
-    const V* r0 = (const V*)(Kinv + row0 * nfp);
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const double2* bv = (const double2*)bf;
+    const int64_t nv = nfp / 2;
+    const double2* r0 = (const double2*)(Kinv + row0 * nfp);
     const int nr = (int)std::min<int64_t>(4, nf - row0);
     for (int64_t j = lane; j < nv; j += 64) {
-        const V bj = bv[j];
+        const double2 bj = bv[j];
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             if (r < nr) {
-                const V a = r0[r * nv + j];
-                if constexpr (VN == 4) acc[r] += a.x * bj.x + a.y * bj.y + a.z * bj.z + a.w * bj.w;
-                else acc[r] += a.x * bj.x + a.y * bj.y;
+                const double2 a = r0[r * nv + j];
+                acc[r] += a.x * bj.x + a.y * bj.y;
             }
         }
     }
@@ -350,7 +342,7 @@ __global__ void __launch_bounds__(256) k_inv_apply(const T* __restrict__ Kinv, c
     for (int r = 0; r < 4; r++)
         for (int off = 32; off > 0; off >>= 1) acc[r] += __shfl_down(acc[r], off, 64);
     if (lane == 0)
-        for (int r = 0; r < nr; r++) x[freeIdx[row0 + r]] = acc[r];
+        for (int r = 0; r < nr; r++) x[freeIdx[row0 + r]] = (T)acc[r];
 }
 template <typename T>
 __global__ void k_zero_fixed_or_all(T* __restrict__ x, int64_t n, const int* gate) {
@@ -809,10 +801,6 @@ struct MGH {
             TOPO_HIP(hipMemcpy(l.freeOf, of.data(), l.ndof * sizeof(int32_t), hipMemcpyHostToDevice));
             l.Kinv = c.alloc<double>(l.nfp * l.nfp);
             l.bf = c.alloc<double>(l.nfp);
-            if (want32) {
-                l.Kinv32 = c.alloc<float>(l.nfp * l.nfp);
-                l.bf32 = c.alloc<float>(l.nfp);
-            }
         }
         const int64_t n = l.nfp;
         coarse_block(c, l);
@@ -842,8 +830,6 @@ struct MGH {
         }
         if (!done) gauss_jordan(c, l.Kinv, n);
         c.check_launch();
-        if (want32)
-            hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(n * n)), dim3(256), 0, c.s, l.Kinv32, l.Kinv, n * n);
         l.chol = true;
     }
     // in-place inverse of the n x n block A (row-major, ld n) by blocked
@@ -935,10 +921,10 @@ struct MGH {
         const bool coarsest = li == (int)L.size() - 1;
         if (coarsest && l.chol) {
             hipLaunchKernelGGL(k_zero_fixed_or_all<T>, dim3(nblk(l.ndof)), dim3(256), 0, c.s, x, l.ndof, gate);
-            hipLaunchKernelGGL(k_gather_free<T>, dim3(nblk(l.nfp)), dim3(256), 0, c.s, b, l.freeIdx, l.nf, l.nfp,
-                               Sel<T>::bf(l), gate);
+            hipLaunchKernelGGL(k_gather_free<T>, dim3(nblk(l.nfp)), dim3(256), 0, c.s, b, l.freeIdx, l.nf, l.nfp, l.bf,
+                               gate);
             hipLaunchKernelGGL(k_inv_apply<T>, dim3((unsigned)((l.nf + 15) / 16)), dim3(256), 0, c.s,
-                               (const T*)Sel<T>::Kinv(l), l.freeIdx, l.nf, l.nfp, (const T*)Sel<T>::bf(l), x, gate);
+                               (const double*)l.Kinv, l.freeIdx, l.nf, l.nfp, (const double*)l.bf, x, gate);
             return x;
         }
         const int nsweep_pre = coarsest ? nu1 + coarse_sweeps : nu1;
@@ -986,13 +972,15 @@ struct CGResult {
     int nonfinite = 0;
     int vcycles = 0;
     int applyA = 0;
+    double relres = 0;   // largest final relative residual of the solves summed here
 };
 
-static void cg_set(Ctx& c, double tol, int maxit) {
+static void cg_set(Ctx& c, double tol, int maxit, double guard = 1e-12) {
     CGState h;
     std::memset(&h, 0, sizeof h);
     h.tol = tol;
     h.maxit = maxit;
+    h.guard = guard;
     *c.h_st = h;
     TOPO_HIP(hipMemcpyAsync(c.st, c.h_st, sizeof(CGState), hipMemcpyHostToDevice, c.s));
 }
@@ -1048,6 +1036,7 @@ static CGResult pcg_jacobi(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVe
         const CGState& h = c.read_st();
         if (h.done) {
             res.iters = h.iters;
+            res.relres = std::sqrt(h.rn / (h.r0 > 1e-300 ? h.r0 : 1e-300));
             res.nonfinite = h.nonfinite;
             break;
         }
@@ -1060,10 +1049,31 @@ static CGResult pcg_jacobi(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVe
     return res;
 }
 
-// MG-PCG on the fine level (ebeMGPCG assembly/index.ts:1076-1193).
-template <typename T, typename TU>
+// z = M^-1 r by one V-cycle in precision PT (T when equal; an FP32 V-cycle
+// inside an FP64 CG when PT = float: r is rounded to FP32, the V-cycle runs on
+// the FP32 operators, z comes back in FP64). Returns the buffer holding z.
+template <typename T, typename PT>
+static T* precondition(Ctx& c, MGH& mg, const T* r, T* z, CGVecs& v32, const int* gate) {
+    if constexpr (std::is_same<T, PT>::value) {
+        T* zz = mg.vcycle<T>(c, 0, r, gate);
+        hipLaunchKernelGGL((k_convert<T, T>), dim3(nblk(mg.L[0].ndof)), dim3(256), 0, c.s, z, (const T*)zz,
+                           mg.L[0].ndof);
+        return z;
+    } else {
+        const int64_t n = mg.L[0].ndof;
+        PT* r32 = (PT*)v32.r;
+        hipLaunchKernelGGL((k_convert_gated<PT, T>), dim3(nblk(n)), dim3(256), 0, c.s, r32, r, n, gate);
+        PT* zz = mg.vcycle<PT>(c, 0, r32, gate);
+        hipLaunchKernelGGL((k_convert_gated<T, PT>), dim3(nblk(n)), dim3(256), 0, c.s, z, (const PT*)zz, n, gate);
+        return z;
+    }
+}
+
+// MG-PCG on the fine level (ebeMGPCG assembly/index.ts:1076-1193); PT is the
+// precision of the V-cycle preconditioner.
+template <typename T, typename TU, typename PT = T>
 static CGResult pcg_mg(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVecs& v, double tol, int maxit,
-                       volatile int* cancel) {
+                       volatile int* cancel, CGVecs* v32 = nullptr) {
     MGLevel& l = mg.L[0];
     const int64_t n = l.ndof;
     T* r = (T*)v.r;
@@ -1085,9 +1095,8 @@ static CGResult pcg_mg(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVecs& 
         hipLaunchKernelGGL((k_convert<T, T>), dim3(nblk(n)), dim3(256), 0, c.s, r, rhs, n);
         hipLaunchKernelGGL(k_mask<T>, dim3(nblk(n)), dim3(256), 0, c.s, r, l.fixed, n);
     }
-    T* zz = mg.vcycle<T>(c, 0, r, nullptr);
+    precondition<T, PT>(c, mg, r, z, *(v32 ? v32 : &v), nullptr);
     res.vcycles++;
-    hipLaunchKernelGGL((k_convert<T, T>), dim3(nblk(n)), dim3(256), 0, c.s, z, (const T*)zz, n);
     hipLaunchKernelGGL(k_mask<T>, dim3(nblk(n)), dim3(256), 0, c.s, z, l.fixed, n);
     hipLaunchKernelGGL((k_convert<T, T>), dim3(nblk(n)), dim3(256), 0, c.s, p, (const T*)z, n);
     cg_set(c, tol, maxit);
@@ -1103,8 +1112,7 @@ static CGResult pcg_mg(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVecs& 
                                c.st, c.partials, c.ticket);
             hipLaunchKernelGGL((k_cg_update_ur<T, TU>), dim3(kRedBlocks), dim3(kRedThreads), 0, c.s, U, r, (const T*)p,
                                (const T*)Ap, n, c.st, c.partials, c.ticket);
-            T* zk = mg.vcycle<T>(c, 0, r, gate);
-            hipLaunchKernelGGL((k_convert<T, T>), dim3(nblk(n)), dim3(256), 0, c.s, z, (const T*)zk, n);
+            precondition<T, PT>(c, mg, r, z, *(v32 ? v32 : &v), gate);
             hipLaunchKernelGGL(k_mask<T>, dim3(nblk(n)), dim3(256), 0, c.s, z, l.fixed, n);
             hipLaunchKernelGGL(k_cg_rz<T>, dim3(kRedBlocks), dim3(kRedThreads), 0, c.s, (const T*)r, (const T*)z, n,
                                c.st, c.partials, c.ticket);
@@ -1114,11 +1122,12 @@ static CGResult pcg_mg(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVecs& 
     };
     hipGraphExec_t ex = nullptr;
     if (c.use_graphs)
-        ex = c.graph_capture({(const void*)&pcg_mg<T, TU>, U, r, z, p, Ap, rhs, &mg}, body);
+        ex = c.graph_capture({(const void*)&pcg_mg<T, TU, PT>, U, r, z, p, Ap, rhs, &mg}, body);
     for (;;) {
         const CGState& h = c.read_st();
         if (h.done) {
             res.iters = h.iters;
+            res.relres = std::sqrt(h.rn / (h.r0 > 1e-300 ? h.r0 : 1e-300));
             res.nonfinite = h.nonfinite;
             break;
         }
@@ -1171,6 +1180,8 @@ struct Solver {
     double fb_penal = 3, fb_beta = 1;
     bool fb_ready = false;        // a forward pass has run at the current design
     int helm_iters = 0;
+    double mixed_eta = 0;         // inner tolerance of the mixed solve (0: clamp(sqrt(tol), 1e-4, 1e-1))
+    int fallback_solves = 0;      // mixed solves finished in FP64 after the FP32 inner solve stalled
 
     void init(const topo_config* cfgin, const uint8_t* hfixed, const float* hF, const uint8_t* hpres,
               const uint8_t* hvoid) {
@@ -1179,8 +1190,8 @@ struct Solver {
         TOPO_CHECK(cfg.nload >= 1 && cfg.nload <= 64, TOPO_E_ARG, "nload must be in 1..64");
         TOPO_CHECK(cfg.linear_solver == TOPO_SOLVER_PCG || cfg.linear_solver == TOPO_SOLVER_MGPCG, TOPO_E_ARG,
                    "unknown linear_solver %d", cfg.linear_solver);
-        TOPO_CHECK(cfg.precision == TOPO_PREC_FP64 || cfg.precision == TOPO_PREC_MIXED, TOPO_E_ARG,
-                   "unknown precision %d", cfg.precision);
+        TOPO_CHECK(cfg.precision == TOPO_PREC_FP64 || cfg.precision == TOPO_PREC_MIXED ||
+                       cfg.precision == TOPO_PREC_MIXED_PC, TOPO_E_ARG, "unknown precision %d", cfg.precision);
         TOPO_CHECK(hfixed && hF, TOPO_E_ARG, "fixedMask and F are required");
         TOPO_CHECK(cfg.objective == TOPO_OBJ_COMPLIANCE || cfg.objective == TOPO_OBJ_MECHANISM, TOPO_E_ARG,
                    "unknown objective %d", cfg.objective);
@@ -1240,7 +1251,7 @@ struct Solver {
         stress = C.alloc<float>(nel); Hs = C.alloc<float>(nel); tmpf = C.alloc<float>(nel);
         sw = C.alloc<double>(nel);
         E = C.alloc<double>(nel);
-        const bool mixed = cfg.precision == TOPO_PREC_MIXED;
+        const bool mixed = cfg.precision != TOPO_PREC_FP64;
         if (mixed) E32 = C.alloc<float>(nel);
         xd64 = C.alloc<double>(nel); xf64 = C.alloc<double>(nel); xp64 = C.alloc<double>(nel);
         g64 = C.alloc<double>(nel); gv64 = C.alloc<double>(nel); t64 = C.alloc<double>(nel);
@@ -1285,6 +1296,7 @@ struct Solver {
         mg.exact = cfg.exact_apply != 0;
         if (const char* e = getenv("TOPO_XM_NT")) mg.xm_nt = atoi(e);
         if (const char* e = getenv("TOPO_XM_CX")) mg.xm_cx = atoi(e);
+        if (const char* e = getenv("TOPO_MIXED_ETA")) mixed_eta = atof(e);
         const int nlev = cfg.linear_solver == TOPO_SOLVER_MGPCG ? cfg.mg_levels : 1;
         mg.build(C, g, fixed, cfg.linear_solver == TOPO_SOLVER_MGPCG ? nlev : 1, mixed);
         mg.L[0].E = E;
@@ -1428,6 +1440,7 @@ struct Solver {
             info->applyA_fine = tot.applyA;
             info->outer_iters = outer;
             info->tolerance = tol;
+            info->residual = tot.relres;
         }
     }
 
@@ -1435,14 +1448,21 @@ struct Solver {
     void solve_one(const double* rhs, double* sol, bool warm, double tol, int maxit, CGResult& tot, int& outer) {
         Ctx& C = *c;
         const bool mgp = cfg.linear_solver == TOPO_SOLVER_MGPCG;
-        if (cfg.precision == TOPO_PREC_FP64) {
+        if (cfg.precision == TOPO_PREC_MIXED_PC && mgp) {
+            CGResult r = pcg_mg<double, double, float>(C, mg, rhs, sol, warm, v64, tol, maxit, &cancel_flag, &v32);
+            TOPO_CHECK(!r.nonfinite, TOPO_E_BREAKDOWN, "non-finite CG scalar");
+            tot.iters += r.iters; tot.vcycles += r.vcycles; tot.applyA += r.applyA;
+            tot.relres = std::max(tot.relres, r.relres);
+        } else if (cfg.precision == TOPO_PREC_FP64 || cfg.precision == TOPO_PREC_MIXED_PC) {
             CGResult r = mgp ? pcg_mg<double, double>(C, mg, rhs, sol, warm, v64, tol, maxit, &cancel_flag)
                              : pcg_jacobi<double, double>(C, mg, rhs, sol, warm, v64, tol, maxit, &cancel_flag);
             TOPO_CHECK(!r.nonfinite, TOPO_E_BREAKDOWN, "non-finite CG scalar");
             tot.iters += r.iters; tot.vcycles += r.vcycles; tot.applyA += r.applyA;
+            tot.relres = std::max(tot.relres, r.relres);
         } else {
             CGResult r = solve_mixed(rhs, sol, warm, tol, maxit, mgp, &outer);
             tot.iters += r.iters; tot.vcycles += r.vcycles; tot.applyA += r.applyA;
+            tot.relres = std::max(tot.relres, r.relres);
         }
     }
 
@@ -1457,7 +1477,7 @@ struct Solver {
         hipLaunchKernelGGL(k_jacobi_apply<double>, dim3(nblk(nn)), dim3(256), 0, C.s, hz, (const double*)hr,
                            (const double*)hinvD, nn);
         hipLaunchKernelGGL((k_convert<double, double>), dim3(nblk(nn)), dim3(256), 0, C.s, hp, (const double*)hz, nn);
-        cg_set(C, 1e-12, 5000);
+        cg_set(C, 1e-13, 5000, 0.0);
         hipLaunchKernelGGL(k_cg_start<double>, dim3(kRedBlocks), dim3(kRedThreads), 0, C.s, (const double*)hr,
                            (const double*)hz, nn, C.st, C.partials, C.ticket);
         const int* gate = &C.st->done;
@@ -1595,11 +1615,14 @@ struct Solver {
         const double rn0 = C.read_d(C.dsc);
         const double tolSq = tol * tol * (rn0 > 1e-30 ? rn0 : 1e-30);
         double rn = rn0;
-        const double eta = std::min(1e-1, std::max(1e-4, std::sqrt(tol)));
+        double eta = std::min(1e-1, std::max(1e-4, std::sqrt(tol)));
+        if (mixed_eta > 0) eta = mixed_eta;
         int outer = 0;
+        bool stalled = false;
         while (rn >= tolSq && outer < 20 && tot.iters < maxit) {
             const double scale = std::sqrt(rn);
-            TOPO_HIP(hipMemcpyAsync(C.dsc + 1, &scale, sizeof(double), hipMemcpyHostToDevice, C.s));
+            C.h_sc[7] = scale;   // pinned staging word (every read_d synchronises before it is rewritten)
+            TOPO_HIP(hipMemcpyAsync(C.dsc + 1, C.h_sc + 7, sizeof(double), hipMemcpyHostToDevice, C.s));
             hipLaunchKernelGGL(k_scale_to<float>, dim3(nblk(ndof)), dim3(256), 0, C.s, rhs32, r, C.dsc + 1, ndof);
             // inner solve A d = r/scale from d = 0
             CGResult ir = mgp ? pcg_mg<float, float>(C, mg, rhs32, d32, false, v32, eta, maxit - tot.iters, &cancel_flag)
@@ -1612,12 +1635,33 @@ struct Solver {
             tot.applyA++;
             hipLaunchKernelGGL((k_dot<double, double>), dim3(kRedBlocks), dim3(kRedThreads), 0, C.s, r, r, ndof, C.dsc,
                                C.partials, C.ticket);
+            const double rn_prev = rn;
             rn = C.read_d(C.dsc);
             outer++;
             TOPO_CHECK(std::isfinite(rn), TOPO_E_BREAKDOWN, "non-finite residual in mixed refinement");
             if (ir.iters == 0) break;
+            // an FP32 inner solve that no longer cuts the FP64 residual norm by
+            // 10x has hit its attainable accuracy (high-contrast, near-singular
+            // operators): finish in FP64
+            if (rn > 1e-2 * rn_prev) {
+                stalled = true;
+                break;
+            }
+        }
+        if (stalled && rn >= tolSq && tot.iters < maxit) {
+            // FP64 PCG warm-started from U; its tolerance is relative to the
+            // current residual, scaled to reach tol relative to the first one
+            const double tol64 = tol * std::sqrt(rn0 / rn);
+            CGResult fr = mgp ? pcg_mg<double, double>(C, mg, Fk, Uk, true, v64, tol64, maxit - tot.iters, &cancel_flag)
+                              : pcg_jacobi<double, double>(C, mg, Fk, Uk, true, v64, tol64, maxit - tot.iters,
+                                                           &cancel_flag);
+            TOPO_CHECK(!fr.nonfinite, TOPO_E_BREAKDOWN, "non-finite CG scalar");
+            tot.iters += fr.iters; tot.vcycles += fr.vcycles; tot.applyA += fr.applyA;
+            fallback_solves++;
+            rn = fr.relres * fr.relres * rn;
         }
         *outer_out += outer;
+        tot.relres = std::sqrt(rn / (rn0 > 1e-300 ? rn0 : 1e-300));
         return tot;
     }
 
@@ -1996,7 +2040,7 @@ int topo_op_applyA(topo_solver* h, int32_t level, int32_t prec, const double* p,
     TOPO_CHECK(h && p && Ap, TOPO_E_ARG, "null argument");
     Solver& S = h->s;
     TOPO_CHECK(level >= 0 && level < (int)S.mg.L.size(), TOPO_E_ARG, "level %d out of range", level);
-    TOPO_CHECK(prec == TOPO_PREC_FP64 || (prec == TOPO_PREC_MIXED && S.cfg.precision == TOPO_PREC_MIXED), TOPO_E_ARG,
+    TOPO_CHECK(prec == TOPO_PREC_FP64 || (prec == TOPO_PREC_MIXED && S.cfg.precision != TOPO_PREC_FP64), TOPO_E_ARG,
                "FP32 operator needs a mixed-precision solver");
     MGLevel& l = S.mg.L[level];
     Ctx& C = *S.c;
@@ -2028,7 +2072,7 @@ int topo_op_vcycle(topo_solver* h, int32_t prec, const double* r, double* z) {
         double* zz = S.mg.vcycle<double>(C, 0, rd, nullptr);
         TOPO_HIP(hipMemcpyAsync(z, zz, l.ndof * sizeof(double), hipMemcpyDeviceToHost, C.s));
     } else {
-        TOPO_CHECK(S.cfg.precision == TOPO_PREC_MIXED, TOPO_E_ARG, "FP32 V-cycle needs a mixed-precision solver");
+        TOPO_CHECK(S.cfg.precision != TOPO_PREC_FP64, TOPO_E_ARG, "FP32 V-cycle needs a mixed-precision solver");
         float* r32 = (float*)S.v32.r;
         hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(l.ndof)), dim3(256), 0, C.s, r32, rd, l.ndof);
         float* zz = S.mg.vcycle<float>(C, 0, r32, nullptr);
@@ -2115,7 +2159,7 @@ int topo_time_op(topo_solver* h, int32_t op, int32_t prec, int32_t level, int32_
     TOPO_CHECK(level >= 0 && level < (int)S.mg.L.size(), TOPO_E_ARG, "level out of range");
     MGLevel& l = S.mg.L[level];
     const bool f32 = prec == TOPO_PREC_MIXED;
-    TOPO_CHECK(!f32 || S.cfg.precision == TOPO_PREC_MIXED, TOPO_E_ARG, "FP32 timing needs a mixed solver");
+    TOPO_CHECK(!f32 || S.cfg.precision != TOPO_PREC_FP64, TOPO_E_ARG, "FP32 timing needs a mixed solver");
     hipEvent_t e0, e1;
     TOPO_HIP(hipEventCreate(&e0));
     TOPO_HIP(hipEventCreate(&e1));
