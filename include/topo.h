@@ -98,7 +98,7 @@ typedef struct topo_config {
     int32_t mech_out_dof;       /* mechanism: output DOF                               */
     double  mech_k_in;          /* input spring stiffness                              */
     double  mech_k_out;         /* output spring stiffness                             */
-    int32_t rhs_tile;           /* block-PCG RHS tile (0 = autotune)                   */
+    int32_t rhs_tile;           /* load cases per batched MG-PCG (0 autotune, 1, 2, 4) */
     int32_t use_graphs;         /* capture MG-PCG iterations in hipGraphs (1 default)  */
     double  skip_factor;        /* elements with E <= skip_factor*Emin are left out of the
                                    operator (reference: 1000); 0 keeps every element so K
@@ -246,6 +246,20 @@ enum { TOPO_FIELD_U = 0,        /* double[nload*ndof]                    */
        TOPO_FIELD_STRESS = 5,   /* float[nel]  E_e * w_e                 */
        TOPO_FIELD_ADJOINT = 6 };/* double[ndof] adjoint (mechanism)      */
 int topo_solver_get_field(topo_solver* s, int32_t field, void* out);
+
+/* ---- batching and tuning (no reference counterpart) ------------------
+ * With nload > 1, MG-PCG and precision FP64 or MIXED_PC, the load cases are
+ * solved by one batched MG-PCG per group of rhs_tile cases (one launch per
+ * operation carries all of them, the coarsest-level inverse is read once per
+ * group, each case keeps its own CG scalars and stops on its own). rhs_tile
+ * 0 autotunes at the first solve: the group size R in {1, 2, 4} and the
+ * x-march element-tile shape (workgroup NT in {512, 1024}, x-planes CX in
+ * {auto, 4, 8, 16}) with the least time per right-hand side for the fine
+ * operator plus one V-cycle. This reports the choice (rhs_tile 1 when no
+ * batching applies) and the timings: log receives max_entries x {R, NT, CX,
+ * ms per right-hand side}. Returns the number of tuning entries. */
+int topo_solver_tuning(topo_solver* s, int32_t* rhs_tile, int32_t* xm_nt, int32_t* xm_cx, double* log,
+                       int32_t max_entries);
 
 /* ---- differentiation: forward/backward pair --------------------------
  * forward: at the current design x (set_design) and SIMP iteration counter,

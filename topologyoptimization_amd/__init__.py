@@ -91,7 +91,7 @@ SYMBOLS = [
     "topo_solver_set_design", "topo_solver_get_design", "topo_solver_set_iteration", "topo_solver_update",
     "topo_solver_step", "topo_solver_get_field", "topo_solver_fe", "topo_solver_set_density", "topo_op_applyA",
     "topo_op_vcycle", "topo_op_energy", "topo_op_filter", "topo_op_filter_adjoint", "topo_op_oc", "topo_time_op",
-    "topo_solver_cancel", "topo_solver_forward", "topo_solver_backward", "topo_op_helmholtz",
+    "topo_solver_cancel", "topo_solver_forward", "topo_solver_backward", "topo_op_helmholtz", "topo_solver_tuning",
 ]
 
 
@@ -139,6 +139,7 @@ def lib():
     L.topo_solver_forward.argtypes = [vp, vp, vp, vp]
     L.topo_solver_backward.argtypes = [vp, vp, vp]
     L.topo_op_helmholtz.argtypes = [vp, vp, vp, vp]
+    L.topo_solver_tuning.argtypes = [vp, vp, vp, vp, vp, i32]
     _lib = L
     return L
 
@@ -393,6 +394,13 @@ class Solver:
         dV = np.zeros(self.nel)
         _check(lib().topo_solver_backward(self.h, _p(dJ), _p(dV)))
         return dJ, dV
+
+    def tuning(self):
+        """(rhs_tile, xm_nt, xm_cx, log[n x (R, NT, CX, ms per RHS)]) of the batched-solve autotuner"""
+        r, nt, cx = C.c_int32(), C.c_int32(), C.c_int32()
+        log = np.zeros((64, 4))
+        n = _check(lib().topo_solver_tuning(self.h, C.byref(r), C.byref(nt), C.byref(cx), _p(log), 64))
+        return r.value, nt.value, cx.value, log[:n]
 
     def helmholtz(self, v):
         v = np.ascontiguousarray(v, np.float64)

@@ -38,6 +38,30 @@ __host__ __device__ __forceinline__ void local_offsets(int L, int& a, int& b, in
 __device__ __forceinline__ double js_max(double a, double b) { return (a != a || b != b) ? __builtin_nan("") : (a > b ? a : b); }
 __device__ __forceinline__ double js_min(double a, double b) { return (a != a || b != b) ? __builtin_nan("") : (a < b ? a : b); }
 
+// CG scalars living on the device (no host round trip per iteration)
+struct CGState {
+    double rz, rz_new, pAp, alpha, beta;
+    double rn;        // ||r||^2 at the last update
+    double r0;        // ||r0||^2
+    double tolSq;
+    double tol;
+    int done;         // converged: later update kernels are no-ops
+    int iters;        // iterations completed
+    int maxit;
+    int nonfinite;
+    double guard;     // denominator guard: 1e-12 in the reference's PCGs, 0 elsewhere
+};
+
+// Batched launches (several right-hand sides in one launch): blockIdx.y
+// selects the right-hand side q; its vectors start q * n entries after the
+// first (n = the level's DOF count) and its gate is the `done` flag of the
+// q-th CGState of an array. With gridDim.y = 1 nothing changes.
+constexpr int kGateStride = (int)(sizeof(CGState) / sizeof(int));
+__device__ __forceinline__ const int* batch_gate(const int* gate) {
+    return gate ? gate + (int64_t)blockIdx.y * kGateStride : nullptr;
+}
+__device__ __forceinline__ int64_t batch_off(int64_t n) { return (int64_t)blockIdx.y * n; }
+
 // ---------------------------------------------------------------------------
 // element order conversions
 template <typename T>
@@ -160,7 +184,13 @@ __global__ void __launch_bounds__(256) k_applyA_exact(const T* __restrict__ p, T
                                                       const double* __restrict__ omp, const int* gate,
                                                       const T* __restrict__ ke) {
 #pragma clang fp contract(off)
+    gate = batch_gate(gate);
     if (gate && *gate) return;
+    {
+        const int64_t bo = batch_off(g.ndof());
+        p += bo; out += bo;
+        if (b) b += bo;
+    }
     const T omega = omp ? (T)*omp : T(0);
     const int nny = g.nny(), nnz = g.nnz();
     const int64_t nyz = (int64_t)nny * nnz;
@@ -283,109 +313,164 @@ __device__ __forceinline__ bool grid_reduce(double (&v)[NV], double* partials, u
     return false;
 }
 
-// CG scalars living on the device (no host round trip per iteration)
-struct CGState {
-    double rz, rz_new, pAp, alpha, beta;
-    double rn;        // ||r||^2 at the last update
-    double r0;        // ||r0||^2
-    double tolSq;
-    double tol;
-    int done;         // converged: later update kernels are no-ops
-    int iters;        // iterations completed
-    int maxit;
-    int nonfinite;
-    double guard;     // denominator guard: 1e-12 in the reference's PCGs, 0 elsewhere
-};
 
-// rn = ||r||^2 and (optionally) rz = r.z; used at CG start
-template <typename T>
+// The CG kernels handle NR right-hand sides in one launch (NR = 1: a single
+// solve): vector q starts at q * n, its scalars live in st[q], and a
+// converged right-hand side (st[q].done) is skipped while the others go on.
+// Every block reads the same st[] values (written by the last block of the
+// previous launch), so all blocks take part in the reduction consistently.
+
+// rn = ||r||^2 and rz = r.z; used at CG start
+template <typename T, int NR = 1>
 __global__ void __launch_bounds__(kRedThreads) k_cg_start(const T* __restrict__ r, const T* __restrict__ z, int64_t n,
                                                          CGState* st, double* partials, unsigned* ticket) {
-    double v[2] = {0.0, 0.0};
-    for (int64_t i = gtid(); i < n; i += gstride()) {
-        const double ri = (double)r[i];
-        v[0] += ri * ri;
-        v[1] += ri * (double)z[i];
+    double v[2 * NR];
+#pragma unroll
+    for (int q = 0; q < NR; q++) {
+        double a = 0.0, b = 0.0;
+        const T* rq = r + q * n;
+        const T* zq = z + q * n;
+        for (int64_t i = gtid(); i < n; i += gstride()) {
+            const double ri = (double)rq[i];
+            a += ri * ri;
+            b += ri * (double)zq[i];
+        }
+        v[2 * q] = a;
+        v[2 * q + 1] = b;
     }
-    double tot[2];
-    if (grid_reduce<2>(v, partials, ticket, tot)) {
-        st->r0 = tot[0];
-        st->rn = tot[0];
-        st->rz = tot[1];
-        st->tolSq = st->tol * st->tol * (tot[0] > 1e-30 ? tot[0] : 1e-30);
-        st->done = (st->maxit <= 0) || (tot[0] < st->tolSq);
-        st->iters = 0;
-        st->nonfinite = 0;
+    double tot[2 * NR];
+    if (grid_reduce<2 * NR>(v, partials, ticket, tot)) {
+        for (int q = 0; q < NR; q++) {
+            CGState& s = st[q];
+            s.r0 = tot[2 * q];
+            s.rn = tot[2 * q];
+            s.rz = tot[2 * q + 1];
+            s.tolSq = s.tol * s.tol * (tot[2 * q] > 1e-30 ? tot[2 * q] : 1e-30);
+            s.done = (s.maxit <= 0) || (tot[2 * q] < s.tolSq);
+            s.iters = 0;
+            s.nonfinite = 0;
+        }
     }
 }
 
+__device__ __forceinline__ bool all_done(const CGState* st, int nr) {
+    for (int q = 0; q < nr; q++)
+        if (!st[q].done) return false;
+    return true;
+}
+
 // pAp = p.Ap -> alpha = rz/(pAp + 1e-12) (ebePCG :695-699)
-template <typename T>
+template <typename T, int NR = 1>
 __global__ void __launch_bounds__(kRedThreads) k_cg_alpha(const T* __restrict__ p, const T* __restrict__ Ap, int64_t n,
                                                          CGState* st, double* partials, unsigned* ticket) {
-    if (st->done) return;
-    double v[1] = {0.0};
-    for (int64_t i = gtid(); i < n; i += gstride()) v[0] += (double)p[i] * (double)Ap[i];
-    double tot[1];
-    if (grid_reduce<1>(v, partials, ticket, tot)) {
-        st->pAp = tot[0];
-        st->alpha = st->rz / (tot[0] + st->guard);
-        if (!isfinite(st->alpha)) st->nonfinite = 1;
+    if (all_done(st, NR)) return;
+    double v[NR];
+#pragma unroll
+    for (int q = 0; q < NR; q++) {
+        double a = 0.0;
+        if (!st[q].done) {
+            const T* pq = p + q * n;
+            const T* aq = Ap + q * n;
+            for (int64_t i = gtid(); i < n; i += gstride()) a += (double)pq[i] * (double)aq[i];
+        }
+        v[q] = a;
+    }
+    double tot[NR];
+    if (grid_reduce<NR>(v, partials, ticket, tot)) {
+        for (int q = 0; q < NR; q++) {
+            CGState& s = st[q];
+            if (s.done) continue;
+            s.pAp = tot[q];
+            s.alpha = s.rz / (tot[q] + s.guard);
+            if (!isfinite(s.alpha)) s.nonfinite = 1;
+        }
     }
 }
 
 // Jacobi variant: U += alpha p; r -= alpha Ap; z = invD r; rz_new, ||r||^2;
 // then beta = rz_new/(rz + 1e-12) (ebePCG :701-719). The convergence test
 // of the next iteration (:645-650) is folded in: done = ||r||^2 < tolSq.
-template <typename T, typename TU>
+template <typename T, typename TU, int NR = 1>
 __global__ void __launch_bounds__(kRedThreads) k_cg_update_jacobi(TU* __restrict__ U, T* __restrict__ r, T* __restrict__ z,
                                                                  const T* __restrict__ p, const T* __restrict__ Ap,
                                                                  const T* __restrict__ invD, int64_t n, CGState* st,
                                                                  double* partials, unsigned* ticket) {
-    if (st->done) return;
-    const double alpha = st->alpha;
-    double v[2] = {0.0, 0.0};
-    for (int64_t i = gtid(); i < n; i += gstride()) {
-        const T ri = (T)((double)r[i] - alpha * (double)Ap[i]);
-        const T zi = invD[i] * ri;
-        U[i] = (TU)((double)U[i] + alpha * (double)p[i]);
-        r[i] = ri;
-        z[i] = zi;
-        v[0] += (double)ri * (double)zi;
-        v[1] += (double)ri * (double)ri;
+    if (all_done(st, NR)) return;
+    double v[2 * NR];
+#pragma unroll
+    for (int q = 0; q < NR; q++) {
+        double a = 0.0, b = 0.0;
+        if (!st[q].done) {
+            const double alpha = st[q].alpha;
+            TU* Uq = U + q * n;
+            T* rq = r + q * n;
+            T* zq = z + q * n;
+            const T* pq = p + q * n;
+            const T* aq = Ap + q * n;
+            for (int64_t i = gtid(); i < n; i += gstride()) {
+                const T ri = (T)((double)rq[i] - alpha * (double)aq[i]);
+                const T zi = invD[i] * ri;
+                Uq[i] = (TU)((double)Uq[i] + alpha * (double)pq[i]);
+                rq[i] = ri;
+                zq[i] = zi;
+                a += (double)ri * (double)zi;
+                b += (double)ri * (double)ri;
+            }
+        }
+        v[2 * q] = a;
+        v[2 * q + 1] = b;
     }
-    double tot[2];
-    if (grid_reduce<2>(v, partials, ticket, tot)) {
-        st->rz_new = tot[0];
-        st->rn = tot[1];
-        st->beta = tot[0] / (st->rz + st->guard);
-        st->rz = tot[0];
-        st->iters += 1;
-        if (!isfinite(tot[0]) || !isfinite(tot[1])) st->nonfinite = 1;
-        st->done = (tot[1] < st->tolSq) || (st->iters >= st->maxit) || st->nonfinite;
+    double tot[2 * NR];
+    if (grid_reduce<2 * NR>(v, partials, ticket, tot)) {
+        for (int q = 0; q < NR; q++) {
+            CGState& s = st[q];
+            if (s.done) continue;
+            s.rz_new = tot[2 * q];
+            s.rn = tot[2 * q + 1];
+            s.beta = tot[2 * q] / (s.rz + s.guard);
+            s.rz = tot[2 * q];
+            s.iters += 1;
+            if (!isfinite(tot[2 * q]) || !isfinite(tot[2 * q + 1])) s.nonfinite = 1;
+            s.done = (tot[2 * q + 1] < s.tolSq) || (s.iters >= s.maxit) || s.nonfinite;
+        }
     }
 }
 
 // MG variant part 1: U += alpha p; r -= alpha Ap (ebeMGPCG :1158-1166) and
 // ||r||^2 for the next convergence test.
-template <typename T, typename TU>
+template <typename T, typename TU, int NR = 1>
 __global__ void __launch_bounds__(kRedThreads) k_cg_update_ur(TU* __restrict__ U, T* __restrict__ r, const T* __restrict__ p,
                                                              const T* __restrict__ Ap, int64_t n, CGState* st,
                                                              double* partials, unsigned* ticket) {
-    if (st->done) return;
-    const double alpha = st->alpha;
-    double v[1] = {0.0};
-    for (int64_t i = gtid(); i < n; i += gstride()) {
-        U[i] = (TU)((double)U[i] + alpha * (double)p[i]);
-        const T ri = (T)((double)r[i] - alpha * (double)Ap[i]);
-        r[i] = ri;
-        v[0] += (double)ri * (double)ri;
+    if (all_done(st, NR)) return;
+    double v[NR];
+#pragma unroll
+    for (int q = 0; q < NR; q++) {
+        double a = 0.0;
+        if (!st[q].done) {
+            const double alpha = st[q].alpha;
+            TU* Uq = U + q * n;
+            T* rq = r + q * n;
+            const T* pq = p + q * n;
+            const T* aq = Ap + q * n;
+            for (int64_t i = gtid(); i < n; i += gstride()) {
+                Uq[i] = (TU)((double)Uq[i] + alpha * (double)pq[i]);
+                const T ri = (T)((double)rq[i] - alpha * (double)aq[i]);
+                rq[i] = ri;
+                a += (double)ri * (double)ri;
+            }
+        }
+        v[q] = a;
     }
-    double tot[1];
-    if (grid_reduce<1>(v, partials, ticket, tot)) {
-        st->rn = tot[0];
-        st->iters += 1;
-        if (!isfinite(tot[0])) st->nonfinite = 1;
+    double tot[NR];
+    if (grid_reduce<NR>(v, partials, ticket, tot)) {
+        for (int q = 0; q < NR; q++) {
+            CGState& s = st[q];
+            if (s.done) continue;
+            s.rn = tot[q];
+            s.iters += 1;
+            if (!isfinite(tot[q])) s.nonfinite = 1;
+        }
     }
 }
 
@@ -393,51 +478,79 @@ __global__ void __launch_bounds__(kRedThreads) k_cg_update_ur(TU* __restrict__ U
 // rz_new = r.z, beta = rz_new/(rz + 1e-12) (:1174-1187); the convergence
 // flag is set here so the p-update of a converged iteration still runs,
 // exactly like the reference (which tests at the top of the next iteration).
-template <typename T>
+template <typename T, int NR = 1>
 __global__ void __launch_bounds__(kRedThreads) k_cg_rz(const T* __restrict__ r, const T* __restrict__ z, int64_t n,
                                                       CGState* st, double* partials, unsigned* ticket) {
-    if (st->done) return;
-    double v[1] = {0.0};
-    for (int64_t i = gtid(); i < n; i += gstride()) v[0] += (double)r[i] * (double)z[i];
-    double tot[1];
-    if (grid_reduce<1>(v, partials, ticket, tot)) {
-        st->rz_new = tot[0];
-        st->beta = tot[0] / (st->rz + st->guard);
-        st->rz = tot[0];
-        if (!isfinite(tot[0])) st->nonfinite = 1;
+    if (all_done(st, NR)) return;
+    double v[NR];
+#pragma unroll
+    for (int q = 0; q < NR; q++) {
+        double a = 0.0;
+        if (!st[q].done) {
+            const T* rq = r + q * n;
+            const T* zq = z + q * n;
+            for (int64_t i = gtid(); i < n; i += gstride()) a += (double)rq[i] * (double)zq[i];
+        }
+        v[q] = a;
+    }
+    double tot[NR];
+    if (grid_reduce<NR>(v, partials, ticket, tot)) {
+        for (int q = 0; q < NR; q++) {
+            CGState& s = st[q];
+            if (s.done) continue;
+            s.rz_new = tot[q];
+            s.beta = tot[q] / (s.rz + s.guard);
+            s.rz = tot[q];
+            if (!isfinite(tot[q])) s.nonfinite = 1;
+        }
     }
 }
 
-// p = z + beta p; afterwards evaluate the convergence flag
+// p = z + beta p (blockIdx.y = right-hand side); afterwards evaluate the
+// convergence flag
 template <typename T>
 __global__ void k_cg_pupdate(T* __restrict__ p, const T* __restrict__ z, int64_t n, const CGState* st) {
+    st += blockIdx.y;
     if (st->done) return;
+    p += batch_off(n);
+    z += batch_off(n);
     const T beta = (T)st->beta;
     for (int64_t i = gtid(); i < n; i += gstride()) p[i] = z[i] + beta * p[i];
 }
-__global__ void k_cg_check(CGState* st) {
-    if (st->done) return;
-    st->done = (st->rn < st->tolSq) || (st->iters >= st->maxit) || st->nonfinite;
+__global__ void k_cg_check(CGState* st, int nr) {
+    for (int q = 0; q < nr; q++) {
+        CGState& s = st[q];
+        if (s.done) continue;
+        s.done = (s.rn < s.tolSq) || (s.iters >= s.maxit) || s.nonfinite;
+    }
 }
 
 // ---------------------------------------------------------------------------
 // vector helpers
 template <typename T>
 __global__ void k_fill(T* __restrict__ v, T val, int64_t n, const int* gate) {
+    gate = batch_gate(gate);
     if (gate && *gate) return;
+    v += batch_off(n);
     for (int64_t i = gtid(); i < n; i += gstride()) v[i] = val;
 }
 template <typename TD, typename TS>
 __global__ void k_convert(TD* __restrict__ d, const TS* __restrict__ s, int64_t n) {
+    d += batch_off(n);
+    s += batch_off(n);
     for (int64_t i = gtid(); i < n; i += gstride()) d[i] = (TD)s[i];
 }
 template <typename TD, typename TS>
 __global__ void k_convert_gated(TD* __restrict__ d, const TS* __restrict__ s, int64_t n, const int* gate) {
+    gate = batch_gate(gate);
     if (gate && *gate) return;
+    d += batch_off(n);
+    s += batch_off(n);
     for (int64_t i = gtid(); i < n; i += gstride()) d[i] = (TD)s[i];
 }
 template <typename T>
 __global__ void k_mask(T* __restrict__ v, const uint8_t* __restrict__ fixed, int64_t n) {
+    v += batch_off(n);
     for (int64_t i = gtid(); i < n; i += gstride())
         if (fixed[i]) v[i] = T(0);
 }
@@ -525,7 +638,10 @@ template <typename T>
 __global__ void k_restrict(const T* __restrict__ res, T* __restrict__ bC, Grid gf, Grid gc,
                            const uint8_t* __restrict__ fixedC, const int* gate) {
 #pragma clang fp contract(off)
+    gate = batch_gate(gate);
     if (gate && *gate) return;
+    res += batch_off(gf.ndof());
+    bC += batch_off(gc.ndof());
     const int nxF = gf.nnx(), nyF = gf.nny(), nzF = gf.nnz();
     const int nxC = gc.nnx(), nyC = gc.nny(), nzC = gc.nnz();
     for (int64_t nc = gtid(); nc < gc.nnodes(); nc += gstride()) {
@@ -573,7 +689,10 @@ template <typename T>
 __global__ void k_prolong_add(T* __restrict__ x, const T* __restrict__ xC, Grid gf, Grid gc,
                               const uint8_t* __restrict__ fixedF, const int* gate) {
 #pragma clang fp contract(off)
+    gate = batch_gate(gate);
     if (gate && *gate) return;
+    x += batch_off(gf.ndof());
+    xC += batch_off(gc.ndof());
     const int nyF = gf.nny(), nzF = gf.nnz();
     const int nyC = gc.nny(), nzC = gc.nnz(), nxC = gc.nnx();
     for (int64_t nf = gtid(); nf < gf.nnodes(); nf += gstride()) {
@@ -643,7 +762,13 @@ __global__ void __launch_bounds__(256) k_dense_apply(const T* __restrict__ K, co
                                                      int64_t n, const uint8_t* __restrict__ fixed, int mode,
                                                      const T* __restrict__ b, const T* __restrict__ invD,
                                                      const double* __restrict__ omp, const int* gate) {
+    gate = batch_gate(gate);
     if (gate && *gate) return;
+    {
+        const int64_t bo = batch_off(n);
+        xin += bo; out += bo;
+        if (b) b += bo;
+    }
     const T omega = omp ? (T)*omp : T(0);
     const int lane = threadIdx.x & 63;
     const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;

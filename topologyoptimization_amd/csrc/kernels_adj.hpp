@@ -201,8 +201,11 @@ struct Springs {
 template <typename T>
 __global__ void k_spring(const T* __restrict__ in, T* __restrict__ out, Springs sp, int mode,
                          const uint8_t* __restrict__ fixed, const T* __restrict__ invD, const double* __restrict__ omp,
-                         const int* gate) {
+                         const int* gate, int64_t n) {
+    gate = batch_gate(gate);
     if (gate && *gate) return;
+    in += batch_off(n);
+    out += batch_off(n);
     const int s = threadIdx.x;
     if (blockIdx.x != 0 || s >= sp.n) return;
     const int64_t d = sp.dof[s];

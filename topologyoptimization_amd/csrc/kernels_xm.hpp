@@ -125,7 +125,13 @@ __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __
                                                   const T* __restrict__ bvec, const T* __restrict__ invD,
                                                   const double* __restrict__ omp, const int* gate, HadCoef<T> co,
                                                   int CX, int gx, int gy) {
+    gate = batch_gate(gate);
     if (gate && *gate) return;
+    {
+        const int64_t bo = batch_off(g.ndof());
+        p += bo; out += bo;
+        if (bvec) bvec += bo;
+    }
     // XCD-aware work order: hardware places workgroup L on XCD L % 8; work
     // item w = (L % 8) * ceil(N/8) + L / 8 gives each XCD a contiguous range
     // of (column, x-chunk) items with the x-chunks of a column adjacent, so

@@ -32,6 +32,8 @@ static inline int nblk(int64_t n, int bs = 256) {
 
 
 // ---------------------------------------------------------------------------
+constexpr int kMaxRHS = 4;   // right-hand sides one batched CG launch carries
+
 struct Ctx {
     hipStream_t s = nullptr;
     int device = 0;
@@ -79,6 +81,11 @@ struct Ctx {
         return ex;
     }
 
+    void graph_clear() {
+        for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
+        graphs.clear();
+    }
+
     Ctx() {
         int n = 0;
         if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
@@ -89,14 +96,14 @@ struct Ctx {
         TOPO_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         partials = alloc<double>(32 * kRedBlocks);
         ticket = alloc<unsigned>(4);
-        st = alloc<CGState>(1);
+        st = alloc<CGState>(kMaxRHS);
         oc = alloc<OCState>(1);
         dsc = alloc<double>(16);
-        TOPO_HIP(hipHostMalloc((void**)&h_st, sizeof(CGState), hipHostMallocDefault));
+        TOPO_HIP(hipHostMalloc((void**)&h_st, kMaxRHS * sizeof(CGState), hipHostMallocDefault));
         TOPO_HIP(hipHostMalloc((void**)&h_oc, sizeof(OCState), hipHostMallocDefault));
         TOPO_HIP(hipHostMalloc((void**)&h_sc, 16 * sizeof(double), hipHostMallocDefault));
         TOPO_HIP(hipMemsetAsync(ticket, 0, 4 * sizeof(unsigned), s));
-        TOPO_HIP(hipMemsetAsync(st, 0, sizeof(CGState), s));
+        TOPO_HIP(hipMemsetAsync(st, 0, kMaxRHS * sizeof(CGState), s));
         TOPO_HIP(hipStreamSynchronize(s));
     }
     ~Ctx() {
@@ -128,8 +135,8 @@ struct Ctx {
     void sync() { TOPO_HIP(hipStreamSynchronize(s)); }
     void check_launch() { TOPO_HIP(hipGetLastError()); }
     // read the device CG state
-    const CGState& read_st() {
-        TOPO_HIP(hipMemcpyAsync(h_st, st, sizeof(CGState), hipMemcpyDeviceToHost, s));
+    const CGState& read_st(int nr = 1) {
+        TOPO_HIP(hipMemcpyAsync(h_st, st, nr * sizeof(CGState), hipMemcpyDeviceToHost, s));
         sync();
         return *h_st;
     }
@@ -297,7 +304,10 @@ template <> struct Sel<float> {
 template <typename T>
 __global__ void k_jacobi_first(T* __restrict__ x, const T* __restrict__ b, const T* __restrict__ invD,
                                const double* __restrict__ omp, int64_t n, const int* gate) {
+    gate = batch_gate(gate);
     if (gate && *gate) return;
+    x += batch_off(n);
+    b += batch_off(n);
     const T omega = (T)*omp;
     for (int64_t i = gtid(); i < n; i += gstride()) x[i] = T(0) + omega * invD[i] * b[i];
 }
@@ -307,46 +317,70 @@ __global__ void k_jacobi_first(T* __restrict__ x, const T* __restrict__ b, const
 // Kinv[i][j] bf[j]; x = 0 on fixed DOFs.
 template <typename T>
 __global__ void k_gather_free(const T* __restrict__ b, const int32_t* __restrict__ freeIdx, int64_t nf, int64_t nfp,
-                              double* __restrict__ bf, const int* gate) {
+                              double* __restrict__ bf, const int* gate, int64_t ndof) {
+    gate = batch_gate(gate);
     if (gate && *gate) return;
+    b += batch_off(ndof);
+    bf += batch_off(nfp);
     for (int64_t j = gtid(); j < nfp; j += gstride()) bf[j] = j < nf ? (double)b[freeIdx[j]] : 0.0;
 }
 // one wave per 4 consecutive rows; each lane streams 16-byte vectors of the
-// four rows against the same slice of bf (read once per wave from L2). The
-// inverse and the accumulation are FP64 in every precision; x receives the
-// result in the V-cycle's precision T.
-template <typename T>
+// four rows against the same slices of the NR right-hand sides bf (read from
+// L2), so the inverse is read from HBM once for all of them. The inverse and
+// the accumulation are FP64 in every precision; x receives the result in the
+// V-cycle's precision T. A converged right-hand side (gate) is not written.
+template <typename T, int NR>
 __global__ void __launch_bounds__(256) k_inv_apply(const double* __restrict__ Kinv, const int32_t* __restrict__ freeIdx,
                                                    int64_t nf, int64_t nfp, const double* __restrict__ bf,
-                                                   T* __restrict__ x, const int* gate) {
-    if (gate && *gate) return;
+                                                   T* __restrict__ x, const int* gate, int64_t ndof) {
+    bool live[NR];
+    bool any = false;
+#pragma unroll
+    for (int q = 0; q < NR; q++) {
+        live[q] = !(gate && gate[(int64_t)q * kGateStride]);
+        any = any || live[q];
+    }
+    if (!any) return;
     const int lane = threadIdx.x & 63;
     const int64_t row0 = (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 4;
     if (row0 >= nf) return;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    const double2* bv = (const double2*)bf;
+    double acc[4][NR];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int q = 0; q < NR; q++) acc[r][q] = 0.0;
     const int64_t nv = nfp / 2;
+    const double2* bv = (const double2*)bf;
     const double2* r0 = (const double2*)(Kinv + row0 * nfp);
     const int nr = (int)std::min<int64_t>(4, nf - row0);
     for (int64_t j = lane; j < nv; j += 64) {
-        const double2 bj = bv[j];
+        double2 bj[NR];
+#pragma unroll
+        for (int q = 0; q < NR; q++) bj[q] = bv[q * nv + j];
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             if (r < nr) {
                 const double2 a = r0[r * nv + j];
-                acc[r] += a.x * bj.x + a.y * bj.y;
+#pragma unroll
+                for (int q = 0; q < NR; q++) acc[r][q] += a.x * bj[q].x + a.y * bj[q].y;
             }
         }
     }
 #pragma unroll
     for (int r = 0; r < 4; r++)
-        for (int off = 32; off > 0; off >>= 1) acc[r] += __shfl_down(acc[r], off, 64);
+#pragma unroll
+        for (int q = 0; q < NR; q++)
+            for (int off = 32; off > 0; off >>= 1) acc[r][q] += __shfl_down(acc[r][q], off, 64);
     if (lane == 0)
-        for (int r = 0; r < nr; r++) x[freeIdx[row0 + r]] = (T)acc[r];
+        for (int q = 0; q < NR; q++)
+            if (live[q])
+                for (int r = 0; r < nr; r++) x[q * ndof + freeIdx[row0 + r]] = (T)acc[r][q];
 }
 template <typename T>
 __global__ void k_zero_fixed_or_all(T* __restrict__ x, int64_t n, const int* gate) {
+    gate = batch_gate(gate);
     if (gate && *gate) return;
+    x += batch_off(n);
     for (int64_t i = gtid(); i < n; i += gstride()) x[i] = T(0);
 }
 
@@ -608,6 +642,7 @@ struct MGH {
     int xm_cx = 0;        // x-march planes per workgroup (0 auto)
     DevKe* K0 = nullptr;
     Springs springs{};    // mechanism ports on the finest level
+    int nrhs = 1;         // right-hand sides a batched V-cycle can carry (level vectors are nrhs x ndof)
     template <typename T> const T* ke() const;
     template <typename T> HadCoef<T> had() const;
 
@@ -663,16 +698,17 @@ struct MGH {
                 if (fp32) l.E32 = c.alloc<float>(l.g.nel());
             }
             l.invD = c.alloc<double>(l.ndof);
-            l.b = c.alloc<double>(l.ndof);
-            l.x = c.alloc<double>(l.ndof);
-            l.x2 = c.alloc<double>(l.ndof);
-            l.res = c.alloc<double>(l.ndof);
+            const int64_t nv = l.ndof * nrhs;
+            l.b = c.alloc<double>(nv);
+            l.x = c.alloc<double>(nv);
+            l.x2 = c.alloc<double>(nv);
+            l.res = c.alloc<double>(nv);
             if (fp32) {
                 l.invD32 = c.alloc<float>(l.ndof);
-                l.b32 = c.alloc<float>(l.ndof);
-                l.x32 = c.alloc<float>(l.ndof);
-                l.x232 = c.alloc<float>(l.ndof);
-                l.res32 = c.alloc<float>(l.ndof);
+                l.b32 = c.alloc<float>(nv);
+                l.x32 = c.alloc<float>(nv);
+                l.x232 = c.alloc<float>(nv);
+                l.res32 = c.alloc<float>(nv);
             }
         }
     }
@@ -800,7 +836,7 @@ struct MGH {
             l.freeOf = c.alloc<int32_t>(l.ndof);
             TOPO_HIP(hipMemcpy(l.freeOf, of.data(), l.ndof * sizeof(int32_t), hipMemcpyHostToDevice));
             l.Kinv = c.alloc<double>(l.nfp * l.nfp);
-            l.bf = c.alloc<double>(l.nfp);
+            l.bf = c.alloc<double>(l.nfp * nrhs);
         }
         const int64_t n = l.nfp;
         coarse_block(c, l);
@@ -873,11 +909,11 @@ struct MGH {
 
     // out = A in (mode 0), masked (1), b - A in (2), Jacobi sweep (3)
     template <typename T>
-    void apply(Ctx& c, int li, const T* in, T* out, int mode, const T* b, const int* gate) {
+    void apply(Ctx& c, int li, const T* in, T* out, int mode, const T* b, const int* gate, int nr = 1) {
         MGLevel& l = L[li];
         const double* om = mode == 3 ? omp(li) : nullptr;
         if (l.dense) {
-            hipLaunchKernelGGL(k_dense_apply<T>, dim3(nblk(l.ndof * 64)), dim3(256), 0, c.s, Sel<T>::K(l), in, out, l.ndof,
+            hipLaunchKernelGGL(k_dense_apply<T>, dim3(nblk(l.ndof * 64), nr), dim3(256), 0, c.s, Sel<T>::K(l), in, out, l.ndof,
                                l.fixed, mode, b, Sel<T>::invD(l), om, gate);
         } else {
             typedef typename Sel<T>::TE TE;
@@ -891,63 +927,69 @@ struct MGH {
                 const int gy = (l.g.nnx() + CX - 1) / CX;
                 if (NT == 512) {
                     const size_t lds = xm_lds_bytes<512>(l.g.nnz(), sizeof(T));
-                    hipLaunchKernelGGL((k_applyA_xm<T, TE, 512>), dim3(8 * ((gx * gy + 7) / 8)), dim3(512), lds, c.s,
+                    hipLaunchKernelGGL((k_applyA_xm<T, TE, 512>), dim3(8 * ((gx * gy + 7) / 8), nr), dim3(512), lds, c.s,
                                        in, out, (const TE*)Sel<T>::E(l), thr, l.g, (const uint8_t*)l.fixed, mode, b,
                                        (const T*)Sel<T>::invD(l), om, gate, had<T>(), CX, gx, gy);
                 } else {
                     const size_t lds = xm_lds_bytes<1024>(l.g.nnz(), sizeof(T));
-                    hipLaunchKernelGGL((k_applyA_xm<T, TE, 1024>), dim3(8 * ((gx * gy + 7) / 8)), dim3(1024), lds, c.s,
+                    hipLaunchKernelGGL((k_applyA_xm<T, TE, 1024>), dim3(8 * ((gx * gy + 7) / 8), nr), dim3(1024), lds, c.s,
                                        in, out, (const TE*)Sel<T>::E(l), thr, l.g, (const uint8_t*)l.fixed, mode, b,
                                        (const T*)Sel<T>::invD(l), om, gate, had<T>(), CX, gx, gy);
                 }
             } else {
-                hipLaunchKernelGGL((k_applyA_exact<T, TE>), dim3(nblk(l.g.nnodes())), dim3(256), 0, c.s, in, out,
+                hipLaunchKernelGGL((k_applyA_exact<T, TE>), dim3(nblk(l.g.nnodes()), nr), dim3(256), 0, c.s, in, out,
                                    (const TE*)Sel<T>::E(l), thr, l.g, l.fixed, mode, b, Sel<T>::invD(l), om, gate,
                                    ke<T>());
             }
         }
         if (li == 0 && springs.n > 0)
-            hipLaunchKernelGGL(k_spring<T>, dim3(1), dim3(64), 0, c.s, in, out, springs, mode, (const uint8_t*)l.fixed,
-                               (const T*)Sel<T>::invD(l), om, gate);
+            hipLaunchKernelGGL(k_spring<T>, dim3(1, nr), dim3(64), 0, c.s, in, out, springs, mode, (const uint8_t*)l.fixed,
+                               (const T*)Sel<T>::invD(l), om, gate, l.ndof);
     }
 
     // one V-cycle (mgVcycle assembly/index.ts:837-1048) for A x = b from
     // x = 0 on level li; returns the buffer holding x.
     template <typename T>
-    T* vcycle(Ctx& c, int li, const T* b, const int* gate) {
+    T* vcycle(Ctx& c, int li, const T* b, const int* gate, int nr = 1) {
         MGLevel& l = L[li];
         T* x = Sel<T>::x(l);
         T* x2 = Sel<T>::x2(l);
         const bool coarsest = li == (int)L.size() - 1;
         if (coarsest && l.chol) {
-            hipLaunchKernelGGL(k_zero_fixed_or_all<T>, dim3(nblk(l.ndof)), dim3(256), 0, c.s, x, l.ndof, gate);
-            hipLaunchKernelGGL(k_gather_free<T>, dim3(nblk(l.nfp)), dim3(256), 0, c.s, b, l.freeIdx, l.nf, l.nfp, l.bf,
-                               gate);
-            hipLaunchKernelGGL(k_inv_apply<T>, dim3((unsigned)((l.nf + 15) / 16)), dim3(256), 0, c.s,
-                               (const double*)l.Kinv, l.freeIdx, l.nf, l.nfp, (const double*)l.bf, x, gate);
+            hipLaunchKernelGGL(k_zero_fixed_or_all<T>, dim3(nblk(l.ndof), nr), dim3(256), 0, c.s, x, l.ndof, gate);
+            hipLaunchKernelGGL(k_gather_free<T>, dim3(nblk(l.nfp), nr), dim3(256), 0, c.s, b, l.freeIdx, l.nf, l.nfp, l.bf,
+                               gate, l.ndof);
+            const dim3 gi((unsigned)((l.nf + 15) / 16));
+            const double* Ki = l.Kinv;
+            switch (nr) {
+            case 1: hipLaunchKernelGGL((k_inv_apply<T, 1>), gi, dim3(256), 0, c.s, Ki, l.freeIdx, l.nf, l.nfp, (const double*)l.bf, x, gate, l.ndof); break;
+            case 2: hipLaunchKernelGGL((k_inv_apply<T, 2>), gi, dim3(256), 0, c.s, Ki, l.freeIdx, l.nf, l.nfp, (const double*)l.bf, x, gate, l.ndof); break;
+            case 4: hipLaunchKernelGGL((k_inv_apply<T, 4>), gi, dim3(256), 0, c.s, Ki, l.freeIdx, l.nf, l.nfp, (const double*)l.bf, x, gate, l.ndof); break;
+            default: TOPO_CHECK(false, TOPO_E_ARG, "batched V-cycle: %d right-hand sides (1, 2 or 4)", nr);
+            }
             return x;
         }
         const int nsweep_pre = coarsest ? nu1 + coarse_sweeps : nu1;
         if (nsweep_pre > 0) {
-            hipLaunchKernelGGL(k_jacobi_first<T>, dim3(nblk(l.ndof)), dim3(256), 0, c.s, x, b, (const T*)Sel<T>::invD(l),
-                               omp(li), l.ndof, gate);
+            hipLaunchKernelGGL(k_jacobi_first<T>, dim3(nblk(l.ndof), nr), dim3(256), 0, c.s, x, b,
+                               (const T*)Sel<T>::invD(l), omp(li), l.ndof, gate);
         } else {
-            hipLaunchKernelGGL(k_fill<T>, dim3(nblk(l.ndof)), dim3(256), 0, c.s, x, T(0), l.ndof, gate);
+            hipLaunchKernelGGL(k_fill<T>, dim3(nblk(l.ndof), nr), dim3(256), 0, c.s, x, T(0), l.ndof, gate);
         }
         for (int s = 1; s < nsweep_pre; s++) {
-            apply<T>(c, li, x, x2, 3, b, gate);
+            apply<T>(c, li, x, x2, 3, b, gate, nr);
             std::swap(x, x2);
         }
         if (coarsest) return x;
-        apply<T>(c, li, x, Sel<T>::res(l), 2, b, gate);
+        apply<T>(c, li, x, Sel<T>::res(l), 2, b, gate, nr);
         MGLevel& lc = L[li + 1];
-        hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(lc.g.nnodes())), dim3(256), 0, c.s, (const T*)Sel<T>::res(l),
+        hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(lc.g.nnodes()), nr), dim3(256), 0, c.s, (const T*)Sel<T>::res(l),
                            Sel<T>::b(lc), l.g, lc.g, lc.fixed, gate);
-        T* xc = vcycle<T>(c, li + 1, Sel<T>::b(lc), gate);
-        hipLaunchKernelGGL(k_prolong_add<T>, dim3(nblk(l.g.nnodes())), dim3(256), 0, c.s, x, (const T*)xc, l.g, lc.g,
-                           l.fixed, gate);
+        T* xc = vcycle<T>(c, li + 1, Sel<T>::b(lc), gate, nr);
+        hipLaunchKernelGGL(k_prolong_add<T>, dim3(nblk(l.g.nnodes()), nr), dim3(256), 0, c.s, x, (const T*)xc, l.g,
+                           lc.g, l.fixed, gate);
         for (int s = 0; s < nu2; s++) {
-            apply<T>(c, li, x, x2, 3, b, gate);
+            apply<T>(c, li, x, x2, 3, b, gate, nr);
             std::swap(x, x2);
         }
         return x;
@@ -975,15 +1017,24 @@ struct CGResult {
     double relres = 0;   // largest final relative residual of the solves summed here
 };
 
-static void cg_set(Ctx& c, double tol, int maxit, double guard = 1e-12) {
+static void cg_set(Ctx& c, double tol, int maxit, double guard = 1e-12, int nr = 1) {
     CGState h;
     std::memset(&h, 0, sizeof h);
     h.tol = tol;
     h.maxit = maxit;
     h.guard = guard;
-    *c.h_st = h;
-    TOPO_HIP(hipMemcpyAsync(c.st, c.h_st, sizeof(CGState), hipMemcpyHostToDevice, c.s));
+    for (int q = 0; q < nr; q++) c.h_st[q] = h;
+    TOPO_HIP(hipMemcpyAsync(c.st, c.h_st, nr * sizeof(CGState), hipMemcpyHostToDevice, c.s));
 }
+
+// the CG reduction kernels for nr = 1, 2 or 4 right-hand sides
+#define TOPO_NR_SWITCH(nr, ...)                                                          \
+    switch (nr) {                                                                        \
+    case 1: { constexpr int NR = 1; __VA_ARGS__; } break;                                \
+    case 2: { constexpr int NR = 2; __VA_ARGS__; } break;                                \
+    case 4: { constexpr int NR = 4; __VA_ARGS__; } break;                                \
+    default: TOPO_CHECK(false, TOPO_E_ARG, "%d right-hand sides per batch (1, 2 or 4)", nr); \
+    }
 
 // Jacobi-PCG on the fine level (ebePCG assembly/index.ts:504-733 in full
 // space with fixed DOFs masked): U is the warm start (TU precision), rhs in T.
@@ -1053,82 +1104,93 @@ static CGResult pcg_jacobi(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVe
 // inside an FP64 CG when PT = float: r is rounded to FP32, the V-cycle runs on
 // the FP32 operators, z comes back in FP64). Returns the buffer holding z.
 template <typename T, typename PT>
-static T* precondition(Ctx& c, MGH& mg, const T* r, T* z, CGVecs& v32, const int* gate) {
+static T* precondition(Ctx& c, MGH& mg, const T* r, T* z, CGVecs& v32, const int* gate, int nr) {
+    const int64_t n = mg.L[0].ndof;
     if constexpr (std::is_same<T, PT>::value) {
-        T* zz = mg.vcycle<T>(c, 0, r, gate);
-        hipLaunchKernelGGL((k_convert<T, T>), dim3(nblk(mg.L[0].ndof)), dim3(256), 0, c.s, z, (const T*)zz,
-                           mg.L[0].ndof);
+        T* zz = mg.vcycle<T>(c, 0, r, gate, nr);
+        hipLaunchKernelGGL((k_convert<T, T>), dim3(nblk(n), nr), dim3(256), 0, c.s, z, (const T*)zz, n);
         return z;
     } else {
-        const int64_t n = mg.L[0].ndof;
         PT* r32 = (PT*)v32.r;
-        hipLaunchKernelGGL((k_convert_gated<PT, T>), dim3(nblk(n)), dim3(256), 0, c.s, r32, r, n, gate);
-        PT* zz = mg.vcycle<PT>(c, 0, r32, gate);
-        hipLaunchKernelGGL((k_convert_gated<T, PT>), dim3(nblk(n)), dim3(256), 0, c.s, z, (const PT*)zz, n, gate);
+        hipLaunchKernelGGL((k_convert_gated<PT, T>), dim3(nblk(n), nr), dim3(256), 0, c.s, r32, r, n, gate);
+        PT* zz = mg.vcycle<PT>(c, 0, r32, gate, nr);
+        hipLaunchKernelGGL((k_convert_gated<T, PT>), dim3(nblk(n), nr), dim3(256), 0, c.s, z, (const PT*)zz, n, gate);
         return z;
     }
 }
 
-// MG-PCG on the fine level (ebeMGPCG assembly/index.ts:1076-1193); PT is the
+// MG-PCG on the fine level (ebeMGPCG assembly/index.ts:1076-1193) for nr
+// right-hand sides at once (rhs, U and the CG vectors hold nr vectors of n
+// entries; each has its own CG scalars and stops on its own); PT is the
 // precision of the V-cycle preconditioner.
 template <typename T, typename TU, typename PT = T>
 static CGResult pcg_mg(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVecs& v, double tol, int maxit,
-                       volatile int* cancel, CGVecs* v32 = nullptr) {
+                       volatile int* cancel, CGVecs* v32 = nullptr, int nr = 1) {
     MGLevel& l = mg.L[0];
     const int64_t n = l.ndof;
+    TOPO_CHECK(nr >= 1 && nr <= kMaxRHS && nr <= mg.nrhs, TOPO_E_ARG, "batch of %d right-hand sides too large", nr);
     T* r = (T*)v.r;
     T* z = (T*)v.z;
     T* p = (T*)v.p;
     T* Ap = (T*)v.Ap;
+    CGVecs& w32 = *(v32 ? v32 : &v);
+    const dim3 gv(nblk(n), nr);
     CGResult res;
     if (warm) {
         if (std::is_same<T, TU>::value) {
-            mg.apply<T>(c, 0, (const T*)U, r, 2, rhs, nullptr);
+            mg.apply<T>(c, 0, (const T*)U, r, 2, rhs, nullptr, nr);
         } else {
-            hipLaunchKernelGGL((k_convert<T, TU>), dim3(nblk(n)), dim3(256), 0, c.s, z, (const TU*)U, n);
-            mg.apply<T>(c, 0, z, r, 2, rhs, nullptr);
+            hipLaunchKernelGGL((k_convert<T, TU>), gv, dim3(256), 0, c.s, z, (const TU*)U, n);
+            mg.apply<T>(c, 0, z, r, 2, rhs, nullptr, nr);
         }
-        hipLaunchKernelGGL(k_mask<TU>, dim3(nblk(n)), dim3(256), 0, c.s, U, l.fixed, n);
-        res.applyA++;
+        hipLaunchKernelGGL(k_mask<TU>, gv, dim3(256), 0, c.s, U, l.fixed, n);
+        res.applyA += nr;
     } else {
-        hipLaunchKernelGGL(k_fill<TU>, dim3(nblk(n)), dim3(256), 0, c.s, U, TU(0), n, (const int*)nullptr);
-        hipLaunchKernelGGL((k_convert<T, T>), dim3(nblk(n)), dim3(256), 0, c.s, r, rhs, n);
-        hipLaunchKernelGGL(k_mask<T>, dim3(nblk(n)), dim3(256), 0, c.s, r, l.fixed, n);
+        hipLaunchKernelGGL(k_fill<TU>, gv, dim3(256), 0, c.s, U, TU(0), n, (const int*)nullptr);
+        hipLaunchKernelGGL((k_convert<T, T>), gv, dim3(256), 0, c.s, r, rhs, n);
+        hipLaunchKernelGGL(k_mask<T>, gv, dim3(256), 0, c.s, r, l.fixed, n);
     }
-    precondition<T, PT>(c, mg, r, z, *(v32 ? v32 : &v), nullptr);
-    res.vcycles++;
-    hipLaunchKernelGGL(k_mask<T>, dim3(nblk(n)), dim3(256), 0, c.s, z, l.fixed, n);
-    hipLaunchKernelGGL((k_convert<T, T>), dim3(nblk(n)), dim3(256), 0, c.s, p, (const T*)z, n);
-    cg_set(c, tol, maxit);
-    hipLaunchKernelGGL(k_cg_start<T>, dim3(kRedBlocks), dim3(kRedThreads), 0, c.s, (const T*)r, (const T*)z, n, c.st,
-                       c.partials, c.ticket);
+    precondition<T, PT>(c, mg, r, z, w32, nullptr, nr);
+    res.vcycles += nr;
+    hipLaunchKernelGGL(k_mask<T>, gv, dim3(256), 0, c.s, z, l.fixed, n);
+    hipLaunchKernelGGL((k_convert<T, T>), gv, dim3(256), 0, c.s, p, (const T*)z, n);
+    cg_set(c, tol, maxit, 1e-12, nr);
+    TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_cg_start<T, NR>), dim3(kRedBlocks), dim3(kRedThreads), 0, c.s,
+                                           (const T*)r, (const T*)z, n, c.st, c.partials, c.ticket));
     c.check_launch();
     const int* gate = &c.st->done;
     const int check = 4;
     auto body = [&]() {
         for (int k = 0; k < check; k++) {
-            mg.apply<T>(c, 0, p, Ap, 1, nullptr, gate);
-            hipLaunchKernelGGL(k_cg_alpha<T>, dim3(kRedBlocks), dim3(kRedThreads), 0, c.s, (const T*)p, (const T*)Ap, n,
-                               c.st, c.partials, c.ticket);
-            hipLaunchKernelGGL((k_cg_update_ur<T, TU>), dim3(kRedBlocks), dim3(kRedThreads), 0, c.s, U, r, (const T*)p,
-                               (const T*)Ap, n, c.st, c.partials, c.ticket);
-            precondition<T, PT>(c, mg, r, z, *(v32 ? v32 : &v), gate);
-            hipLaunchKernelGGL(k_mask<T>, dim3(nblk(n)), dim3(256), 0, c.s, z, l.fixed, n);
-            hipLaunchKernelGGL(k_cg_rz<T>, dim3(kRedBlocks), dim3(kRedThreads), 0, c.s, (const T*)r, (const T*)z, n,
-                               c.st, c.partials, c.ticket);
-            hipLaunchKernelGGL(k_cg_pupdate<T>, dim3(nblk(n)), dim3(256), 0, c.s, p, (const T*)z, n, c.st);
-            hipLaunchKernelGGL(k_cg_check, dim3(1), dim3(1), 0, c.s, c.st);
+            mg.apply<T>(c, 0, p, Ap, 1, nullptr, gate, nr);
+            TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_cg_alpha<T, NR>), dim3(kRedBlocks), dim3(kRedThreads), 0, c.s,
+                                                   (const T*)p, (const T*)Ap, n, c.st, c.partials, c.ticket));
+            TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_cg_update_ur<T, TU, NR>), dim3(kRedBlocks), dim3(kRedThreads), 0,
+                                                   c.s, U, r, (const T*)p, (const T*)Ap, n, c.st, c.partials,
+                                                   c.ticket));
+            precondition<T, PT>(c, mg, r, z, w32, gate, nr);
+            hipLaunchKernelGGL(k_mask<T>, gv, dim3(256), 0, c.s, z, l.fixed, n);
+            TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_cg_rz<T, NR>), dim3(kRedBlocks), dim3(kRedThreads), 0, c.s,
+                                                   (const T*)r, (const T*)z, n, c.st, c.partials, c.ticket));
+            hipLaunchKernelGGL(k_cg_pupdate<T>, gv, dim3(256), 0, c.s, p, (const T*)z, n, c.st);
+            hipLaunchKernelGGL(k_cg_check, dim3(1), dim3(1), 0, c.s, c.st, nr);
         }
     };
     hipGraphExec_t ex = nullptr;
     if (c.use_graphs)
-        ex = c.graph_capture({(const void*)&pcg_mg<T, TU, PT>, U, r, z, p, Ap, rhs, &mg}, body);
+        ex = c.graph_capture({(const void*)&pcg_mg<T, TU, PT>, U, r, z, p, Ap, rhs, &mg, (const void*)(intptr_t)nr},
+                             body);
     for (;;) {
-        const CGState& h = c.read_st();
-        if (h.done) {
-            res.iters = h.iters;
-            res.relres = std::sqrt(h.rn / (h.r0 > 1e-300 ? h.r0 : 1e-300));
-            res.nonfinite = h.nonfinite;
+        c.read_st(nr);
+        bool done = true;
+        for (int q = 0; q < nr; q++) done = done && c.h_st[q].done;
+        if (done) {
+            for (int q = 0; q < nr; q++) {
+                const CGState& h = c.h_st[q];
+                res.iters += h.iters;
+                res.relres = std::max(res.relres, std::sqrt(h.rn / (h.r0 > 1e-300 ? h.r0 : 1e-300)));
+                res.nonfinite |= h.nonfinite;
+            }
             break;
         }
         if (cancel && *cancel) throw TopoError{TOPO_E_CANCELLED};
@@ -1182,6 +1244,9 @@ struct Solver {
     int helm_iters = 0;
     double mixed_eta = 0;         // inner tolerance of the mixed solve (0: clamp(sqrt(tol), 1e-4, 1e-1))
     int fallback_solves = 0;      // mixed solves finished in FP64 after the FP32 inner solve stalled
+    int batch_cap = 1;            // right-hand sides the batched buffers hold
+    int rhs_tile = 0;             // right-hand sides per batched solve (0: not tuned yet)
+    bool tuned = false;
 
     void init(const topo_config* cfgin, const uint8_t* hfixed, const float* hF, const uint8_t* hpres,
               const uint8_t* hvoid) {
@@ -1271,10 +1336,21 @@ struct Solver {
         U = C.alloc<double>(ndof * cfg.nload);
         hipLaunchKernelGGL(k_fill<double>, dim3(nblk(ndof * cfg.nload)), dim3(256), 0, C.s, U, 0.0, ndof * cfg.nload,
                            (const int*)nullptr);
-        v64.r = C.alloc<double>(ndof); v64.z = C.alloc<double>(ndof);
-        v64.p = C.alloc<double>(ndof); v64.Ap = C.alloc<double>(ndof);
+        // load cases solved together by one batched MG-PCG (FP64, or FP64 with
+        // the FP32 V-cycle): at most kMaxRHS per batch
+        batch_cap = 1;
+        if (cfg.nload > 1 && cfg.linear_solver == TOPO_SOLVER_MGPCG && cfg.precision != TOPO_PREC_MIXED &&
+            cfg.rhs_tile != 1) {
+            TOPO_CHECK(cfg.rhs_tile == 0 || cfg.rhs_tile == 2 || cfg.rhs_tile == 4, TOPO_E_ARG,
+                       "rhs_tile must be 0 (autotune), 1, 2 or 4");
+            batch_cap = cfg.rhs_tile > 0 ? cfg.rhs_tile : kMaxRHS;
+            while (batch_cap > cfg.nload) batch_cap /= 2;
+        }
+        const int64_t nvb = ndof * batch_cap;
+        v64.r = C.alloc<double>(nvb); v64.z = C.alloc<double>(nvb);
+        v64.p = C.alloc<double>(nvb); v64.Ap = C.alloc<double>(nvb);
         if (mixed) {
-            v32.r = C.alloc<float>(ndof); v32.z = C.alloc<float>(ndof);
+            v32.r = C.alloc<float>(nvb); v32.z = C.alloc<float>(ndof);
             v32.p = C.alloc<float>(ndof); v32.Ap = C.alloc<float>(ndof);
             r64 = C.alloc<double>(ndof);
             d32 = C.alloc<float>(ndof);
@@ -1298,6 +1374,7 @@ struct Solver {
         if (const char* e = getenv("TOPO_XM_CX")) mg.xm_cx = atoi(e);
         if (const char* e = getenv("TOPO_MIXED_ETA")) mixed_eta = atof(e);
         const int nlev = cfg.linear_solver == TOPO_SOLVER_MGPCG ? cfg.mg_levels : 1;
+        mg.nrhs = batch_cap;
         mg.build(C, g, fixed, cfg.linear_solver == TOPO_SOLVER_MGPCG ? nlev : 1, mixed);
         mg.L[0].E = E;
         mg.L[0].E32 = E32;
@@ -1429,8 +1506,21 @@ struct Solver {
         const int maxit = (int)std::min<int64_t>(nfree_cap, nfree);
         CGResult tot;
         int outer = 0;
-        for (int k = 0; k < cfg.nload; k++)
-            solve_one(F + (int64_t)k * ndof, U + (int64_t)k * ndof, warm, tol, maxit, tot, outer);
+        if (batch_cap > 1) {
+            if (rhs_tile == 0) {
+                if (cfg.rhs_tile > 0) rhs_tile = std::min(cfg.rhs_tile, batch_cap);
+                else autotune();
+            }
+            for (int k = 0; k < cfg.nload;) {
+                int r = rhs_tile;
+                while (r > cfg.nload - k) r /= 2;
+                solve_batch(F + (int64_t)k * ndof, U + (int64_t)k * ndof, r, warm, tol, maxit, tot);
+                k += r;
+            }
+        } else {
+            for (int k = 0; k < cfg.nload; k++)
+                solve_one(F + (int64_t)k * ndof, U + (int64_t)k * ndof, warm, tol, maxit, tot, outer);
+        }
         // the mechanism's adjoint: K lam = L (L = unit vector at the output DOF)
         if (cfg.objective == TOPO_OBJ_MECHANISM) solve_one(Lvec, adj, warm, tol, maxit, tot, outer);
         have_U = true;
@@ -1443,6 +1533,81 @@ struct Solver {
             info->residual = tot.relres;
         }
     }
+
+    // nr load cases at once: batched MG-PCG (FP64, or FP64 with the FP32
+    // V-cycle preconditioner)
+    void solve_batch(const double* rhs, double* sol, int nr, bool warm, double tol, int maxit, CGResult& tot) {
+        Ctx& C = *c;
+        CGResult r = cfg.precision == TOPO_PREC_MIXED_PC
+                         ? pcg_mg<double, double, float>(C, mg, rhs, sol, warm, v64, tol, maxit, &cancel_flag, &v32, nr)
+                         : pcg_mg<double, double>(C, mg, rhs, sol, warm, v64, tol, maxit, &cancel_flag, nullptr, nr);
+        TOPO_CHECK(!r.nonfinite, TOPO_E_BREAKDOWN, "non-finite CG scalar");
+        tot.iters += r.iters; tot.vcycles += r.vcycles; tot.applyA += r.applyA;
+        tot.relres = std::max(tot.relres, r.relres);
+    }
+
+    // Autotuner of the batched solve: for every candidate (right-hand sides
+    // per batch R in {1, 2, 4} up to batch_cap; x-march workgroup NT in {512,
+    // 1024} and x-planes per workgroup CX in {auto, 4, 8, 16}) time the
+    // fine-level operator and one preconditioner application of R
+    // right-hand sides with HIP events on the solver's stream, and keep the
+    // shape with the least time per right-hand side (operator + V-cycle, the
+    // work of one CG iteration). Runs once, on the operators of the first
+    // solve; the cached graphs are dropped afterwards.
+    void autotune() {
+        Ctx& C = *c;
+        const bool pc = cfg.precision == TOPO_PREC_MIXED_PC;
+        // right-hand sides: copies of the first load case
+        for (int q = 0; q < batch_cap; q++)
+            TOPO_HIP(hipMemcpyAsync((double*)v64.r + q * ndof, F + (int64_t)(q % cfg.nload) * ndof,
+                                    ndof * sizeof(double), hipMemcpyDeviceToDevice, C.s));
+        hipEvent_t e0, e1;
+        TOPO_HIP(hipEventCreate(&e0));
+        TOPO_HIP(hipEventCreate(&e1));
+        auto time_it = [&](auto&& fn, int reps) {
+            fn();   // warm-up (code object load, caches)
+            TOPO_HIP(hipEventRecord(e0, C.s));
+            for (int k = 0; k < reps; k++) fn();
+            TOPO_HIP(hipEventRecord(e1, C.s));
+            TOPO_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            TOPO_HIP(hipEventElapsedTime(&ms, e0, e1));
+            return (double)ms / reps;
+        };
+        const int nt_c[2] = {512, 1024};
+        const int cx_c[4] = {0, 4, 8, 16};
+        double best = 1e300;
+        int bR = 1, bNT = mg.xm_nt, bCX = mg.xm_cx;
+        tune_log.clear();
+        for (int R = 1; R <= batch_cap; R *= 2)
+            for (int a = 0; a < 2; a++)
+                for (int b = 0; b < 4; b++) {
+                    mg.xm_nt = nt_c[a];
+                    mg.xm_cx = cx_c[b];
+                    const double ms = time_it([&]() {
+                        mg.apply<double>(C, 0, (const double*)v64.r, (double*)v64.Ap, 1, nullptr, nullptr, R);
+                        if (pc)
+                            precondition<double, float>(C, mg, (const double*)v64.r, (double*)v64.z, v32, nullptr, R);
+                        else
+                            precondition<double, double>(C, mg, (const double*)v64.r, (double*)v64.z, v64, nullptr, R);
+                    }, 3);
+                    const double per = ms / R;
+                    tune_log.push_back({R, nt_c[a], cx_c[b], per});
+                    if (per < best) { best = per; bR = R; bNT = nt_c[a]; bCX = cx_c[b]; }
+                }
+        TOPO_HIP(hipEventDestroy(e0));
+        TOPO_HIP(hipEventDestroy(e1));
+        rhs_tile = bR;
+        mg.xm_nt = bNT;
+        mg.xm_cx = bCX;
+        C.graph_clear();
+        tuned = true;
+    }
+    struct TuneEntry {
+        int R, nt, cx;
+        double ms_per_rhs;
+    };
+    std::vector<TuneEntry> tune_log;
 
     // K sol = rhs with the configured solver and precision
     void solve_one(const double* rhs, double* sol, bool warm, double tol, int maxit, CGResult& tot, int& outer) {
@@ -1943,6 +2108,26 @@ int topo_solver_get_field(topo_solver* h, int32_t field, void* out) {
     default: TOPO_CHECK(false, TOPO_E_ARG, "unknown field %d", field);
     }
     return TOPO_OK;
+    TOPO_API_END
+}
+
+int topo_solver_tuning(topo_solver* h, int32_t* rhs_tile, int32_t* xm_nt, int32_t* xm_cx, double* log,
+                       int32_t max_entries) {
+    TOPO_API_BEGIN
+    TOPO_CHECK(h, TOPO_E_ARG, "null solver");
+    Solver& S = h->s;
+    if (rhs_tile) *rhs_tile = S.batch_cap > 1 ? S.rhs_tile : 1;
+    if (xm_nt) *xm_nt = S.mg.xm_nt;
+    if (xm_cx) *xm_cx = S.mg.xm_cx;
+    const int n = (int)S.tune_log.size();
+    if (log)
+        for (int i = 0; i < n && i < max_entries; i++) {
+            log[4 * i] = S.tune_log[i].R;
+            log[4 * i + 1] = S.tune_log[i].nt;
+            log[4 * i + 2] = S.tune_log[i].cx;
+            log[4 * i + 3] = S.tune_log[i].ms_per_rhs;
+        }
+    return n;
     TOPO_API_END
 }
 

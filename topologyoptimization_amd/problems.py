@@ -7,7 +7,7 @@ the seed of the random initial densities.
 """
 import numpy as np
 
-from . import (DOF_ALL, FIX_BOTTOM_CORNERS, FIX_LEFT, LOAD_TOP_CENTER, PREC_FP64, PREC_MIXED, SOLVER_MGPCG,
+from . import (DOF_ALL, FIX_BOTTOM_CORNERS, FIX_LEFT, LOAD_TOP_CENTER, PREC_FP64, PREC_MIXED, PREC_MIXED_PC, SOLVER_MGPCG,
                SOLVER_PCG, COARSE_CHOLESKY, FILTER_HELMHOLTZ, FILTER_SENSITIVITY, OBJ_MECHANISM, default_config, dims,
                fixed_dofs, load_vector, preserved_elements)
 
@@ -96,12 +96,12 @@ def cantilever_c3(precision=PREC_FP64, nlevels=4, **kw):
     return cfg, fixed, F, pres, void, splitmix64_uniform(nel)
 
 
-def bridge_c4(**kw):
-    """128x64x64 bridge: the two bottom edges along y at x = 0 and x = nelx
+def bridge(nelx, nely, nelz, **kw):
+    """Multi-load bridge: the two bottom edges along y at x = 0 and x = nelx
     (z = 0) pinned in xyz; 4 load cases, each a -z load of total 1.0 on an
-    8x8-node patch of the top face (z = nelz) centred in y at x = 0.2, 0.4,
-    0.6, 0.8 of the span."""
-    nelx, nely, nelz = 128, 64, 64
+    8x8-node patch (float32 -1/64 per node) of the top face (z = nelz) centred
+    in y at x = 0.2, 0.4, 0.6, 0.8 of the span. volfrac 0.12, p = 3,
+    sensitivity filter rmin 1.5, MG-PCG, tol 1e-6, every element kept."""
     cfg = default_config(nelx, nely, nelz, nload=4, volfrac=0.12, penal=3.0, penal_start=3.0, rmin=1.5,
                          use_projection=0, max_iterations=10, cg_tol=1e-6, linear_solver=SOLVER_MGPCG,
                          skip_factor=0.0, mg_omega_cap=1.6)
@@ -123,6 +123,16 @@ def bridge_c4(**kw):
                 F[k, 3 * node_index(nely, nelz, ix, iy, nelz) + 2] = per
     x0 = np.full(nel, np.float32(cfg.volfrac), np.float32)
     return cfg, fixed, F.ravel(), None, None, x0
+
+
+def bridge_c4(**kw):
+    """The 128x64x64 bridge (1,635,075 DOF, 4 load cases), batched MG-PCG
+    (4 levels, FP64 Cholesky coarsest, omega 0.6 capped), FP32 V-cycle inside
+    FP64 CG, autotuned batch and element-tile shapes."""
+    args = dict(mg_levels=4, mg_omega=0.6, coarse_solver=COARSE_CHOLESKY, galerkin_max_ndof=0,
+                precision=PREC_MIXED_PC, rhs_tile=0)
+    args.update(kw)
+    return bridge(128, 64, 64, **args)
 
 
 def inverter(nelx=160, nely=80, nelz=80, **kw):
