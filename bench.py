@@ -28,11 +28,13 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_solver(T, P, grid, prec, steps, warmup, levels, plain=False):
-    """plain: the reference's plain method on the same problem -- Jacobi-PCG
-    in FP64, with the 400-iteration cap lifted so it reaches the same
-    tolerance (the compliance is then comparable)."""
+def make_problem(T, P, workload, grid, prec, levels, **kw):
     nx, ny, nz = grid
+    if workload == "c4":
+        return P.bridge(nx, ny, nz, mg_levels=levels, mg_omega=0.6, coarse_solver=T.COARSE_CHOLESKY,
+                        galerkin_max_ndof=0, precision=prec, **kw)
+    if workload == "c5":
+        return P.inverter(nx, ny, nz, precision=prec, mg_levels=levels, **kw)
     if (nx, ny, nz) == (192, 96, 96):
         cfg, fixed, F, pres, void, x0 = P.cantilever_c3(precision=prec, nlevels=levels)
     else:
@@ -41,7 +43,18 @@ def run_solver(T, P, grid, prec, steps, warmup, levels, plain=False):
                                                      coarse_solver=T.COARSE_CHOLESKY, galerkin_max_ndof=0)
         nel, _ = T.dims(nx, ny, nz)
         x0 = P.splitmix64_uniform(nel)
-    cfg.mg_omega_cap = 1.6
+    for k, v in kw.items():
+        setattr(cfg, k, v)
+    return cfg, fixed, F, pres, void, x0
+
+
+def run_solver(T, P, grid, prec, steps, warmup, levels, plain=False, workload="c3", **kw):
+    """plain: the reference's plain method on the same problem -- Jacobi-PCG
+    in FP64, with the 400-iteration cap lifted so it reaches the same
+    tolerance (the compliance is then comparable)."""
+    cfg, fixed, F, pres, void, x0 = make_problem(T, P, workload, grid, prec, levels, **kw)
+    if workload == "c3":
+        cfg.mg_omega_cap = 1.6
     cfg.max_iterations = 1000
     if plain:
         cfg.linear_solver = T.SOLVER_PCG
@@ -76,6 +89,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--workload", choices=("c3", "c4", "c5"), default="c3",
+                    help="c3: 192x96x96 cantilever (headline); c4: 128x64x64 bridge, 4 load cases, batched; "
+                         "c5: 160x80x80 force inverter, forward + adjoint solves")
     ap.add_argument("--mixed", choices=("pc", "ir"), default="pc",
                     help="pc: FP64 MG-PCG with an FP32 V-cycle preconditioner; ir: FP64 iterative refinement "
                          "around an FP32 inner MG-PCG")
@@ -89,6 +105,8 @@ def main():
         sys.exit(2)
     import topologyoptimization_amd as T
     import topologyoptimization_amd.problems as P
+    if args.workload != "c3":
+        return other_workload(T, P, args)
     grid = tuple(int(v) for v in args.grid.split(","))
     n, name = T.device_info()
     assert n > 0, "no HIP device"
@@ -155,6 +173,60 @@ def main():
         "cpu_baseline": cpu,
         "device": name,
     }
+    print(json.dumps(out))
+
+
+def other_workload(T, P, args):
+    """c4: the 4-load bridge solved by the batched MG-PCG with autotuned batch
+    and tile shapes, against the same solver one load case at a time; c5: the
+    force inverter (forward and adjoint solves per step). One JSON line."""
+    n, name = T.device_info()
+    assert n > 0, "no HIP device"
+    log("device:", name)
+    mixed_prec = T.PREC_MIXED_PC if args.mixed == "pc" else T.PREC_MIXED
+    if args.workload == "c4":
+        grid = (128, 64, 64)
+        S, infos, t_b = run_solver(T, P, grid, mixed_prec, args.steps, args.warmup, args.levels, workload="c4",
+                                   rhs_tile=0)
+        R, nt, cx, tlog = S.tuning()
+        ms32, by32 = S.time_op(T.TIME_APPLYA, T.PREC_MIXED, 0, 50)
+        S.close()
+        S1, infos1, t_s = run_solver(T, P, grid, mixed_prec, args.steps, args.warmup, args.levels, workload="c4",
+                                     rhs_tile=1)
+        S1.close()
+        extra = {"batched_vs_one_at_a_time": t_s / t_b, "one_at_a_time_iters_per_s": 1.0 / t_s,
+                 "compliance_rel_vs_one_at_a_time": _rel(infos, infos1),
+                 "autotuned": {"rhs_tile": R, "xm_nt": nt, "xm_cx": cx,
+                               "candidates": [[int(a), int(b), int(c), float(d)] for a, b, c, d in tlog]}}
+        workload = ("bridge 128x64x64 hex8, 1635075 DOF, 4 load cases (8x8-node patches, top face), batched "
+                    "%d-level MG-PCG, FP32 V-cycle in FP64 CG, FP64 direct coarsest, tol 1e-6" % args.levels)
+        metric = "SIMP iters/s, 4-load bridge, batched MG-PCG"
+    else:
+        grid = (160, 80, 80)
+        S, infos, t_b = run_solver(T, P, grid, mixed_prec, args.steps, args.warmup, args.levels, workload="c5",
+                                   cg_max_iter=2000)
+        ms32, by32 = S.time_op(T.TIME_APPLYA, T.PREC_MIXED, 0, 50)
+        S.close()
+        extra = {}
+        workload = ("force inverter 160x80x80 hex8, 3168963 DOF, output displacement (adjoint solve), Helmholtz "
+                    "filter rmin 2 + Heaviside beta 8, %d-level MG-PCG, FP32 V-cycle in FP64 CG, tol 1e-6" % args.levels)
+        metric = "SIMP iters/s, compliant-mechanism (forward + adjoint) MG-PCG"
+    nel, ndof = T.dims(*grid)
+    ach = by32 / (ms32 * 1e-3) / 1e9
+    cpu = None if args.no_cpu else cpu_baseline(grid, infos, args.levels)
+    out = {
+        "metric": metric, "value": 1.0 / t_b, "unit": "SIMP iters/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": t_b * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32+fp64", "data": "synthetic",
+        "config": {"workload": workload, "nel": nel, "ndof": ndof, "parallelism": "single GPU"},
+        "cg_iters_per_step": float(np.mean([i["cg_iters"] for i in infos])),
+        "compliance": infos[-1]["compliance"],
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": 8000.0, "unit": "GB/s", "frac": ach / 8000.0,
+                     "traffic": measured_traffic(grid), "kernel": "k_applyA_xm<float> (fine level)",
+                     "algorithmic_bytes": by32, "avg_ms": ms32},
+        "cpu_baseline": cpu, "device": name,
+    }
+    out.update(extra)
     print(json.dumps(out))
 
 
