@@ -48,7 +48,7 @@ def make_problem(T, P, workload, grid, prec, levels, **kw):
     return cfg, fixed, F, pres, void, x0
 
 
-def run_solver(T, P, grid, prec, steps, warmup, levels, plain=False, workload="c3", **kw):
+def run_solver(T, P, grid, prec, steps, warmup, levels, plain=False, workload="c3", coarse_fp32=1, **kw):
     """plain: the reference's plain method on the same problem -- Jacobi-PCG
     in FP64, with the 400-iteration cap lifted so it reaches the same
     tolerance (the compliance is then comparable)."""
@@ -56,6 +56,7 @@ def run_solver(T, P, grid, prec, steps, warmup, levels, plain=False, workload="c
     if workload == "c3":
         cfg.mg_omega_cap = 1.6
     cfg.max_iterations = 1000
+    cfg.coarse_fp32 = coarse_fp32
     if plain:
         cfg.linear_solver = T.SOLVER_PCG
         cfg.precision = T.PREC_FP64
@@ -92,6 +93,8 @@ def main():
     ap.add_argument("--workload", choices=("c3", "c4", "c5"), default="c3",
                     help="c3: 192x96x96 cantilever (headline); c4: 128x64x64 bridge, 4 load cases, batched; "
                          "c5: 160x80x80 force inverter, forward + adjoint solves")
+    ap.add_argument("--coarse-fp32", type=int, default=1,
+                    help="1: the FP32 V-cycle applies an FP32 copy of the FP64 coarse inverse")
     ap.add_argument("--mixed", choices=("pc", "ir"), default="pc",
                     help="pc: FP64 MG-PCG with an FP32 V-cycle preconditioner; ir: FP64 iterative refinement "
                          "around an FP32 inner MG-PCG")
@@ -115,7 +118,8 @@ def main():
 
     # mixed precision (the headline)
     mixed_prec = T.PREC_MIXED_PC if args.mixed == "pc" else T.PREC_MIXED
-    S, infos, t_mixed = run_solver(T, P, grid, mixed_prec, args.steps, args.warmup, args.levels)
+    S, infos, t_mixed = run_solver(T, P, grid, mixed_prec, args.steps, args.warmup, args.levels,
+                                   coarse_fp32=args.coarse_fp32)
     # compliance of the mixed path must match the FP64 path (checked below)
     # dominant kernel: fine-level FP32 applyA, timed with HIP events on the solver's stream
     ms32, by32 = S.time_op(T.TIME_APPLYA, T.PREC_MIXED, 0, 50)

@@ -109,7 +109,9 @@ typedef struct topo_config {
     double  mg_omega_cap;       /* > 0: per level omega_eff = min(mg_omega, cap / rho),
                                    rho ~ lambda_max(D^-1 A) by 20 power iterations from
                                    a hashed vector on free DOFs; 0 = off (reference)  */
-    int32_t reserved[8];
+    int32_t coarse_fp32;        /* 1: FP32 V-cycles apply an FP32 copy of the coarsest
+                                   level's FP64 inverse (half the bytes); 0: FP64  */
+    int32_t reserved[7];
 } topo_config;
 
 /* Fill *cfg with the reference defaults (js/optimizer-worker-3d.js:20-64,

@@ -52,7 +52,8 @@ class Config(C.Structure):
         ("mech_k_in", C.c_double), ("mech_k_out", C.c_double),
         ("rhs_tile", C.c_int32), ("use_graphs", C.c_int32), ("skip_factor", C.c_double), ("exact_apply", C.c_int32),
         ("mg_omega_cap", C.c_double),
-        ("reserved", C.c_int32 * 8),
+        ("coarse_fp32", C.c_int32),
+        ("reserved", C.c_int32 * 7),
     ]
 
     def copy(self):

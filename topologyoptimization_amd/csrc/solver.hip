@@ -32,6 +32,15 @@ static inline int nblk(int64_t n, int bs = 256) {
 
 
 // ---------------------------------------------------------------------------
+// dispatch of the batched kernels for nr = 1, 2 or 4 right-hand sides
+#define TOPO_NR_SWITCH(nr, ...)                                                          \
+    switch (nr) {                                                                        \
+    case 1: { constexpr int NR = 1; __VA_ARGS__; } break;                                \
+    case 2: { constexpr int NR = 2; __VA_ARGS__; } break;                                \
+    case 4: { constexpr int NR = 4; __VA_ARGS__; } break;                                \
+    default: TOPO_CHECK(false, TOPO_E_ARG, "%d right-hand sides per batch (1, 2 or 4)", nr); \
+    }
+
 constexpr int kMaxRHS = 4;   // right-hand sides one batched CG launch carries
 
 struct Ctx {
@@ -273,6 +282,7 @@ struct MGLevel {
     int32_t* freeOf = nullptr;     // DOF -> free index or -1
     double* bf = nullptr;          // gathered right-hand side (padded)
     double* Kinv = nullptr;
+    float* Kinv32 = nullptr;       // FP32 rounding of Kinv (coarse_fp32)
     double* work = nullptr;
 };
 
@@ -329,8 +339,8 @@ __global__ void k_gather_free(const T* __restrict__ b, const int32_t* __restrict
 // L2), so the inverse is read from HBM once for all of them. The inverse and
 // the accumulation are FP64 in every precision; x receives the result in the
 // V-cycle's precision T. A converged right-hand side (gate) is not written.
-template <typename T, int NR>
-__global__ void __launch_bounds__(256) k_inv_apply(const double* __restrict__ Kinv, const int32_t* __restrict__ freeIdx,
+template <typename T, int NR, typename TK>
+__global__ void __launch_bounds__(256) k_inv_apply(const TK* __restrict__ Kinv, const int32_t* __restrict__ freeIdx,
                                                    int64_t nf, int64_t nfp, const double* __restrict__ bf,
                                                    T* __restrict__ x, const int* gate, int64_t ndof) {
     bool live[NR];
@@ -349,20 +359,45 @@ __global__ void __launch_bounds__(256) k_inv_apply(const double* __restrict__ Ki
     for (int r = 0; r < 4; r++)
 #pragma unroll
         for (int q = 0; q < NR; q++) acc[r][q] = 0.0;
-    const int64_t nv = nfp / 2;
-    const double2* bv = (const double2*)bf;
-    const double2* r0 = (const double2*)(Kinv + row0 * nfp);
     const int nr = (int)std::min<int64_t>(4, nf - row0);
-    for (int64_t j = lane; j < nv; j += 64) {
-        double2 bj[NR];
+    if constexpr (std::is_same<TK, double>::value) {
+        const int64_t nv = nfp / 2;
+        const double2* bv = (const double2*)bf;
+        const double2* r0 = (const double2*)(Kinv + row0 * nfp);
+        for (int64_t j = lane; j < nv; j += 64) {
+            double2 bj[NR];
 #pragma unroll
-        for (int q = 0; q < NR; q++) bj[q] = bv[q * nv + j];
+            for (int q = 0; q < NR; q++) bj[q] = bv[q * nv + j];
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            if (r < nr) {
-                const double2 a = r0[r * nv + j];
+            for (int r = 0; r < 4; r++) {
+                if (r < nr) {
+                    const double2 a = r0[r * nv + j];
 #pragma unroll
-                for (int q = 0; q < NR; q++) acc[r][q] += a.x * bj[q].x + a.y * bj[q].y;
+                    for (int q = 0; q < NR; q++) acc[r][q] += a.x * bj[q].x + a.y * bj[q].y;
+                }
+            }
+        }
+    } else {
+        // FP32 copy of the FP64 inverse: 4 entries per 16-byte load, FP64 sums
+        const int64_t nv = nfp / 4;
+        const double2* bv = (const double2*)bf;
+        const float4* r0 = (const float4*)(Kinv + row0 * nfp);
+        for (int64_t j = lane; j < nv; j += 64) {
+            double2 b0[NR], b1[NR];
+#pragma unroll
+            for (int q = 0; q < NR; q++) {
+                b0[q] = bv[q * (nfp / 2) + 2 * j];
+                b1[q] = bv[q * (nfp / 2) + 2 * j + 1];
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                if (r < nr) {
+                    const float4 a = r0[r * nv + j];
+#pragma unroll
+                    for (int q = 0; q < NR; q++)
+                        acc[r][q] += (double)a.x * b0[q].x + (double)a.y * b0[q].y + (double)a.z * b1[q].x +
+                                     (double)a.w * b1[q].y;
+                }
             }
         }
     }
@@ -643,6 +678,7 @@ struct MGH {
     DevKe* K0 = nullptr;
     Springs springs{};    // mechanism ports on the finest level
     int nrhs = 1;         // right-hand sides a batched V-cycle can carry (level vectors are nrhs x ndof)
+    bool coarse_fp32 = false;   // FP32 V-cycles apply an FP32 copy of the coarsest inverse
     template <typename T> const T* ke() const;
     template <typename T> HadCoef<T> had() const;
 
@@ -837,6 +873,7 @@ struct MGH {
             TOPO_HIP(hipMemcpy(l.freeOf, of.data(), l.ndof * sizeof(int32_t), hipMemcpyHostToDevice));
             l.Kinv = c.alloc<double>(l.nfp * l.nfp);
             l.bf = c.alloc<double>(l.nfp * nrhs);
+            if (want32 && coarse_fp32) l.Kinv32 = c.alloc<float>(l.nfp * l.nfp);
         }
         const int64_t n = l.nfp;
         coarse_block(c, l);
@@ -865,6 +902,9 @@ struct MGH {
             if (!done) coarse_block(c, l);   // the factorization overwrote the block
         }
         if (!done) gauss_jordan(c, l.Kinv, n);
+        if (l.Kinv32)
+            hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(n * n)), dim3(256), 0, c.s, l.Kinv32,
+                               (const double*)l.Kinv, n * n);
         c.check_launch();
         l.chol = true;
     }
@@ -960,12 +1000,16 @@ struct MGH {
             hipLaunchKernelGGL(k_gather_free<T>, dim3(nblk(l.nfp), nr), dim3(256), 0, c.s, b, l.freeIdx, l.nf, l.nfp, l.bf,
                                gate, l.ndof);
             const dim3 gi((unsigned)((l.nf + 15) / 16));
-            const double* Ki = l.Kinv;
-            switch (nr) {
-            case 1: hipLaunchKernelGGL((k_inv_apply<T, 1>), gi, dim3(256), 0, c.s, Ki, l.freeIdx, l.nf, l.nfp, (const double*)l.bf, x, gate, l.ndof); break;
-            case 2: hipLaunchKernelGGL((k_inv_apply<T, 2>), gi, dim3(256), 0, c.s, Ki, l.freeIdx, l.nf, l.nfp, (const double*)l.bf, x, gate, l.ndof); break;
-            case 4: hipLaunchKernelGGL((k_inv_apply<T, 4>), gi, dim3(256), 0, c.s, Ki, l.freeIdx, l.nf, l.nfp, (const double*)l.bf, x, gate, l.ndof); break;
-            default: TOPO_CHECK(false, TOPO_E_ARG, "batched V-cycle: %d right-hand sides (1, 2 or 4)", nr);
+            // the FP32 V-cycle reads the FP32 rounding of the FP64 inverse
+            // (half the bytes) when coarse_fp32 is set
+            if (std::is_same<T, float>::value && l.Kinv32) {
+                const float* Ki = l.Kinv32;
+                TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_inv_apply<T, NR, float>), gi, dim3(256), 0, c.s, Ki, l.freeIdx,
+                                                      l.nf, l.nfp, (const double*)l.bf, x, gate, l.ndof));
+            } else {
+                const double* Ki = l.Kinv;
+                TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_inv_apply<T, NR, double>), gi, dim3(256), 0, c.s, Ki, l.freeIdx,
+                                                      l.nf, l.nfp, (const double*)l.bf, x, gate, l.ndof));
             }
             return x;
         }
@@ -1027,14 +1071,7 @@ static void cg_set(Ctx& c, double tol, int maxit, double guard = 1e-12, int nr =
     TOPO_HIP(hipMemcpyAsync(c.st, c.h_st, nr * sizeof(CGState), hipMemcpyHostToDevice, c.s));
 }
 
-// the CG reduction kernels for nr = 1, 2 or 4 right-hand sides
-#define TOPO_NR_SWITCH(nr, ...)                                                          \
-    switch (nr) {                                                                        \
-    case 1: { constexpr int NR = 1; __VA_ARGS__; } break;                                \
-    case 2: { constexpr int NR = 2; __VA_ARGS__; } break;                                \
-    case 4: { constexpr int NR = 4; __VA_ARGS__; } break;                                \
-    default: TOPO_CHECK(false, TOPO_E_ARG, "%d right-hand sides per batch (1, 2 or 4)", nr); \
-    }
+
 
 // Jacobi-PCG on the fine level (ebePCG assembly/index.ts:504-733 in full
 // space with fixed DOFs masked): U is the warm start (TU precision), rhs in T.
@@ -1375,6 +1412,7 @@ struct Solver {
         if (const char* e = getenv("TOPO_MIXED_ETA")) mixed_eta = atof(e);
         const int nlev = cfg.linear_solver == TOPO_SOLVER_MGPCG ? cfg.mg_levels : 1;
         mg.nrhs = batch_cap;
+        mg.coarse_fp32 = cfg.coarse_fp32 != 0;
         mg.build(C, g, fixed, cfg.linear_solver == TOPO_SOLVER_MGPCG ? nlev : 1, mixed);
         mg.L[0].E = E;
         mg.L[0].E32 = E32;
