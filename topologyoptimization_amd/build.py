@@ -38,7 +38,7 @@ def _run(cmd):
 def build_hip():
     srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
     tmp = LIB + ".tmp"
-    _run([HIPCC] + HIP_FLAGS + ["-o", tmp] + srcs + ["-lrocblas", "-lrocsolver"])
+    _run([HIPCC] + HIP_FLAGS + ["-o", tmp] + srcs + ["-lrocblas"])
     os.replace(tmp, LIB)
     return LIB
 

@@ -1,0 +1,263 @@
+// kernels_chol.hpp -- FP64 banded Cholesky of the coarsest MG level and the
+// explicit inverse of its factor (build-only: the reference smooths its
+// coarsest level instead of solving it).
+//
+// The free-DOF block A (n x n, n a multiple of 64, row-major, symmetric) of a
+// structured-grid operator is banded: with the reference's x-major node order
+// every nonzero satisfies |i - j| <= w, w ~ 3 (nny nnz + nnz + 1). Cholesky
+// keeps the band, so the blocked right-looking factorization only visits the
+// 64 x 64 blocks within wb = ceil(w / 64) block-diagonals:
+//   for K: L_KK = chol(A_KK) and L_KK^-1     (k_chol_diag, one workgroup)
+//          A_IK <- A_IK L_KK^-T, I in (K, K+wb]         (k_chol_panel)
+//          A_IJ -= A_IK A_JK^T,  K < J <= I <= K+wb      (k_chol_update)
+// Z = L^-1 is dense lower triangular; its row blocks follow from
+//   Z_II = L_II^-1,  Z_IJ = -L_II^-1 sum_{K=max(J,I-wb)}^{I-1} L_IK Z_KJ  (k_chol_zrow),
+// all J of one I in parallel. The coarse solve is then x = Z^T (Z b): two
+// triangular matrix-vector products (k_tri_apply) over Z and its transpose.
+// All of it is O(n w^2 + n^2 w) work instead of the O(n^3) of a dense inverse.
+#pragma once
+#include "kernels.hpp"
+
+namespace topo {
+
+constexpr int kCB = 64;           // block size
+constexpr int kCS = kCB + 1;      // LDS row stride (bank spread)
+
+// load a 64 x 64 block (row-major, ld) into LDS with stride kCS
+__device__ __forceinline__ void blk_load(double* s, const double* A, int64_t ld) {
+    for (int e = threadIdx.x; e < kCB * kCB; e += blockDim.x) {
+        const int r = e / kCB, c = e % kCB;
+        s[r * kCS + c] = A[(int64_t)r * ld + c];
+    }
+}
+
+// Cholesky of the diagonal block K in LDS, then the inverse of its factor.
+// Writes L_KK (upper part of the block zeroed) back to A and L_KK^-1 to
+// Linv[K]. A pivot <= tiny * (original diagonal of that DOF) sets *flag:
+// the block is not (numerically) positive definite.
+__global__ void __launch_bounds__(256) k_chol_diag(double* __restrict__ A, int64_t ld, int K,
+                                                   double* __restrict__ Linv, const double* __restrict__ d0,
+                                                   double tiny, int* flag) {
+    __shared__ double a[kCB * kCS];
+    __shared__ double z[kCB * kCS];
+    double* Ab = A + (int64_t)K * kCB * ld + (int64_t)K * kCB;
+    blk_load(a, Ab, ld);
+    __syncthreads();
+    const int t = threadIdx.x;
+    for (int j = 0; j < kCB; j++) {
+        if (t == 0) {
+            const double d = a[j * kCS + j];
+            if (!(d > tiny * d0[(int64_t)K * kCB + j])) *flag = 1;
+            a[j * kCS + j] = d > 0 ? sqrt(d) : 1.0;
+        }
+        __syncthreads();
+        const double ljj = a[j * kCS + j];
+        for (int i = j + 1 + t; i < kCB; i += blockDim.x) a[i * kCS + j] /= ljj;
+        __syncthreads();
+        // trailing update of the lower triangle: (i, k), j < k <= i
+        const int m = kCB - 1 - j;
+        for (int e = t; e < m * m; e += blockDim.x) {
+            const int i = j + 1 + e / m, k = j + 1 + e % m;
+            if (k <= i) a[i * kCS + k] -= a[i * kCS + j] * a[k * kCS + j];
+        }
+        __syncthreads();
+    }
+    // inverse of the lower-triangular factor, one column per thread
+    if (t < kCB) {
+        const int c = t;
+        for (int i = 0; i < c; i++) z[i * kCS + c] = 0.0;
+        z[c * kCS + c] = 1.0 / a[c * kCS + c];
+        for (int i = c + 1; i < kCB; i++) {
+            double s = 0.0;
+            for (int k = c; k < i; k++) s += a[i * kCS + k] * z[k * kCS + c];
+            z[i * kCS + c] = -s / a[i * kCS + i];
+        }
+    }
+    __syncthreads();
+    double* Li = Linv + (int64_t)K * kCB * kCB;
+    for (int e = t; e < kCB * kCB; e += blockDim.x) {
+        const int r = e / kCB, c = e % kCB;
+        Ab[(int64_t)r * ld + c] = c <= r ? a[r * kCS + c] : 0.0;
+        Li[e] = z[r * kCS + c];
+    }
+}
+
+// C (64 x 64, 4 x 4 per thread) = sum_k X[r][k] Y[c][k] over LDS blocks
+__device__ __forceinline__ void blk_mul_nt(const double* X, const double* Y, double (&acc)[4][4]) {
+    const int r0 = (threadIdx.x / 16) * 4, c0 = (threadIdx.x % 16) * 4;
+    for (int k = 0; k < kCB; k++) {
+        double xv[4], yv[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) { xv[i] = X[(r0 + i) * kCS + k]; yv[i] = Y[(c0 + i) * kCS + k]; }
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int jj = 0; jj < 4; jj++) acc[i][jj] += xv[i] * yv[jj];
+    }
+}
+
+// A_IK <- A_IK L_KK^-T for the wb blocks below the diagonal block K
+__global__ void __launch_bounds__(256) k_chol_panel(double* __restrict__ A, int64_t ld, int K, int NB, int wb,
+                                                    const double* __restrict__ Linv) {
+    const int I = K + 1 + (int)blockIdx.x;
+    if (I >= NB || I > K + wb) return;
+    __shared__ double x[kCB * kCS];
+    __shared__ double y[kCB * kCS];
+    double* Ab = A + (int64_t)I * kCB * ld + (int64_t)K * kCB;
+    blk_load(x, Ab, ld);
+    const double* Li = Linv + (int64_t)K * kCB * kCB;
+    for (int e = threadIdx.x; e < kCB * kCB; e += blockDim.x) y[(e / kCB) * kCS + e % kCB] = Li[e];
+    __syncthreads();
+    double acc[4][4] = {};
+    blk_mul_nt(x, y, acc);   // (A_IK L^-T)[r][c] = sum_k A_IK[r][k] Linv[c][k]
+    const int r0 = (threadIdx.x / 16) * 4, c0 = (threadIdx.x % 16) * 4;
+    for (int i = 0; i < 4; i++)
+        for (int jj = 0; jj < 4; jj++) Ab[(int64_t)(r0 + i) * ld + c0 + jj] = acc[i][jj];
+}
+
+// A_IJ -= A_IK A_JK^T for K < J <= I <= min(K + wb, NB - 1); blockIdx.x runs
+// over the lower-triangular pairs (I, J) of the window
+__global__ void __launch_bounds__(256) k_chol_update(double* __restrict__ A, int64_t ld, int K, int NB, int wb) {
+    int p = (int)blockIdx.x, di = 0;
+    while (p > di) { p -= di + 1; di++; }
+    const int I = K + 1 + di, J = K + 1 + p;
+    if (I >= NB) return;
+    __shared__ double x[kCB * kCS];
+    __shared__ double y[kCB * kCS];
+    blk_load(x, A + (int64_t)I * kCB * ld + (int64_t)K * kCB, ld);
+    blk_load(y, A + (int64_t)J * kCB * ld + (int64_t)K * kCB, ld);
+    __syncthreads();
+    double acc[4][4] = {};
+    blk_mul_nt(x, y, acc);
+    double* Cb = A + (int64_t)I * kCB * ld + (int64_t)J * kCB;
+    const int r0 = (threadIdx.x / 16) * 4, c0 = (threadIdx.x % 16) * 4;
+    for (int i = 0; i < 4; i++)
+        for (int jj = 0; jj < 4; jj++) Cb[(int64_t)(r0 + i) * ld + c0 + jj] -= acc[i][jj];
+}
+
+// row block I of Z = L^-1: Z_IJ for J = blockIdx.x (J < I), Z_II = L_II^-1
+__global__ void __launch_bounds__(256) k_chol_zrow(const double* __restrict__ L, double* __restrict__ Z, int64_t ld,
+                                                   int I, int wb, const double* __restrict__ Linv) {
+    const int J = (int)blockIdx.x;
+    if (J > I) return;
+    double* Zb = Z + (int64_t)I * kCB * ld + (int64_t)J * kCB;
+    const double* Li = Linv + (int64_t)I * kCB * kCB;
+    if (J == I) {
+        for (int e = threadIdx.x; e < kCB * kCB; e += blockDim.x) Zb[(int64_t)(e / kCB) * ld + e % kCB] = Li[e];
+        return;
+    }
+    __shared__ double x[kCB * kCS];
+    __shared__ double y[kCB * kCS];
+    const int r0 = (threadIdx.x / 16) * 4, c0 = (threadIdx.x % 16) * 4;
+    double acc[4][4] = {};
+    // T = sum_K L_IK Z_KJ: y holds Z_KJ transposed so blk_mul_nt applies
+    for (int K = max(J, I - wb); K < I; K++) {
+        __syncthreads();
+        blk_load(x, L + (int64_t)I * kCB * ld + (int64_t)K * kCB, ld);
+        const double* Zk = Z + (int64_t)K * kCB * ld + (int64_t)J * kCB;
+        for (int e = threadIdx.x; e < kCB * kCB; e += blockDim.x) {
+            const int r = e / kCB, c = e % kCB;
+            y[c * kCS + r] = Zk[(int64_t)r * ld + c];
+        }
+        __syncthreads();
+        blk_mul_nt(x, y, acc);
+    }
+    // Z_IJ = -L_II^-1 T
+    __syncthreads();
+    for (int i = 0; i < 4; i++)
+        for (int jj = 0; jj < 4; jj++) y[(c0 + jj) * kCS + r0 + i] = acc[i][jj];   // y = T^T
+    for (int e = threadIdx.x; e < kCB * kCB; e += blockDim.x) x[(e / kCB) * kCS + e % kCB] = Li[e];
+    __syncthreads();
+    double out[4][4] = {};
+    blk_mul_nt(x, y, out);
+    for (int i = 0; i < 4; i++)
+        for (int jj = 0; jj < 4; jj++) Zb[(int64_t)(r0 + i) * ld + c0 + jj] = -out[i][jj];
+}
+
+// out = in^T for an n x n row-major matrix (64 x 64 tiles through LDS)
+template <typename TI, typename TO>
+__global__ void __launch_bounds__(256) k_transpose(const TI* __restrict__ in, TO* __restrict__ out, int64_t n) {
+    __shared__ double s[kCB * kCS];
+    const int64_t bi = blockIdx.y, bj = blockIdx.x;
+    for (int e = threadIdx.x; e < kCB * kCB; e += blockDim.x) {
+        const int r = e / kCB, c = e % kCB;
+        s[r * kCS + c] = (double)in[(bi * kCB + r) * n + bj * kCB + c];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kCB * kCB; e += blockDim.x) {
+        const int r = e / kCB, c = e % kCB;
+        out[(bj * kCB + r) * n + bi * kCB + c] = (TO)s[c * kCS + r];
+    }
+}
+
+// Triangular matrix-vector product for NR right-hand sides: rows of M (n x n,
+// row-major, TK = double or float) against bf (NR x n, FP64); lower: row i
+// uses columns [0, i], upper: [i, n) (the other part of M is zero, so whole
+// 16-byte vectors around the diagonal can be read). One wave per 4 rows.
+// scatter = 0: y[q n + i] = result (FP64); scatter = 1: x[q ndof + free[i]]
+// = result in T, for i < nf (converged right-hand sides are not written).
+template <typename T, int NR, typename TK>
+__global__ void __launch_bounds__(256) k_tri_apply(const TK* __restrict__ M, int64_t n, int upper,
+                                                   const double* __restrict__ bf, double* __restrict__ y,
+                                                   T* __restrict__ x, const int32_t* __restrict__ freeIdx, int64_t nf,
+                                                   int64_t ndof, const int* gate) {
+    bool live[NR];
+    bool any = false;
+#pragma unroll
+    for (int q = 0; q < NR; q++) {
+        live[q] = !(gate && gate[(int64_t)q * kGateStride]);
+        any = any || live[q];
+    }
+    if (!any) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t row0 = (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 4;
+    if (row0 >= n) return;
+    double acc[4][NR];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int q = 0; q < NR; q++) acc[r][q] = 0.0;
+    constexpr int VN = std::is_same<TK, double>::value ? 2 : 4;
+    // column range in vectors of VN entries
+    const int64_t v0 = upper ? row0 / VN : 0;
+    const int64_t v1 = upper ? n / VN : (row0 + 4 + VN - 1) / VN;
+    const int64_t nvr = n / VN;
+    for (int64_t j = v0 + lane; j < v1; j += 64) {
+        double b[NR][VN];
+#pragma unroll
+        for (int q = 0; q < NR; q++)
+#pragma unroll
+            for (int u = 0; u < VN; u++) b[q][u] = bf[q * n + j * VN + u];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            if constexpr (VN == 2) {
+                const double2 a = ((const double2*)(M + (row0 + r) * n))[j];
+#pragma unroll
+                for (int q = 0; q < NR; q++) acc[r][q] += a.x * b[q][0] + a.y * b[q][1];
+            } else {
+                const float4 a = ((const float4*)(M + (row0 + r) * n))[j];
+#pragma unroll
+                for (int q = 0; q < NR; q++)
+                    acc[r][q] += (double)a.x * b[q][0] + (double)a.y * b[q][1] + (double)a.z * b[q][2] +
+                                 (double)a.w * b[q][3];
+            }
+        }
+        (void)nvr;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int q = 0; q < NR; q++)
+            for (int off = 32; off > 0; off >>= 1) acc[r][q] += __shfl_down(acc[r][q], off, 64);
+    if (lane == 0)
+        for (int q = 0; q < NR; q++) {
+            if (!live[q]) continue;
+            for (int r = 0; r < 4; r++) {
+                const int64_t i = row0 + r;
+                if (!x) y[q * n + i] = acc[r][q];
+                else if (i < nf) x[q * ndof + freeIdx[i]] = (T)acc[r][q];
+            }
+        }
+}
+
+}  // namespace topo

@@ -18,8 +18,8 @@
 #include "kernels.hpp"
 #include "kernels_xm.hpp"
 #include "kernels_adj.hpp"
+#include "kernels_chol.hpp"
 #include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
 
 namespace topo {
 
@@ -281,8 +281,15 @@ struct MGLevel {
     int32_t* freeIdx = nullptr;
     int32_t* freeOf = nullptr;     // DOF -> free index or -1
     double* bf = nullptr;          // gathered right-hand side (padded)
-    double* Kinv = nullptr;
+    double* Kinv = nullptr;        // banded factor L, then Z^T (triangular mode); or the inverse
     float* Kinv32 = nullptr;       // FP32 rounding of Kinv (coarse_fp32)
+    bool tri = false;              // coarse solve by x = Z^T (Z b), Z = L^-1 (else Kinv b)
+    int bandw = 0;                 // max |i - j| of the free block's nonzeros
+    double* Z = nullptr;           // L^-1 (lower triangular)
+    float* Z32 = nullptr;
+    double* Linv = nullptr;        // inverses of the diagonal blocks of L
+    double* ytmp = nullptr;        // Z b (nrhs x nfp)
+    double* d0c = nullptr;         // original diagonal (pivot test)
     double* work = nullptr;
 };
 
@@ -653,14 +660,6 @@ __global__ void k_regularize(double* A, int64_t nf) {
             A[i * nf + i] = 1.0;
         }
 }
-// copy the triangle a LAPACK-style inverse left filled (column-major lower =
-// row-major upper) to the other one
-__global__ void k_symmetrize_upper(double* A, int64_t n) {
-    for (int64_t w = gtid(); w < n * n; w += gstride()) {
-        const int64_t r = w / n, c = w % n;
-        if (r > c) A[w] = A[c * n + r];
-    }
-}
 
 struct MGH {
     std::vector<MGLevel> L;
@@ -866,45 +865,97 @@ struct MGH {
             l.nf = (int64_t)fr.size();
             TOPO_CHECK(l.nf > 0 && l.nf <= 20000, TOPO_E_UNSUPPORTED,
                        "coarsest level has %lld free DOFs; the direct solve supports 1..20000", (long long)l.nf);
-            l.nfp = (l.nf + 3) / 4 * 4;
+            // bandwidth of the free block: nonzeros couple the DOFs of nodes
+            // at most one element apart
+            const int nny = l.g.nny(), nnz = l.g.nnz();
+            int w = 0;
+            for (int64_t fa = 0; fa < l.nf; fa++) {
+                const int64_t na = fr[fa] / 3;
+                const int iz = (int)(na % nnz), iy = (int)((na / nnz) % nny), ix = (int)(na / ((int64_t)nnz * nny));
+                for (int dx = -1; dx <= 1; dx++)
+                    for (int dy = -1; dy <= 1; dy++)
+                        for (int dz = -1; dz <= 1; dz++) {
+                            const int jx = ix + dx, jy = iy + dy, jz = iz + dz;
+                            if (jx < 0 || jx > l.g.nelx || jy < 0 || jy > l.g.nely || jz < 0 || jz > l.g.nelz) continue;
+                            for (int d = 0; d < 3; d++) {
+                                const int32_t fb = of[3 * (((int64_t)jx * nny + jy) * nnz + jz) + d];
+                                if (fb >= 0) w = std::max<int>(w, (int)std::abs(fb - fa));
+                            }
+                        }
+            }
+            l.bandw = w;
+            l.nfp = (l.nf + kCB - 1) / kCB * kCB;
             l.freeIdx = c.alloc<int32_t>(l.nf);
             TOPO_HIP(hipMemcpy(l.freeIdx, fr.data(), l.nf * sizeof(int32_t), hipMemcpyHostToDevice));
             l.freeOf = c.alloc<int32_t>(l.ndof);
             TOPO_HIP(hipMemcpy(l.freeOf, of.data(), l.ndof * sizeof(int32_t), hipMemcpyHostToDevice));
             l.Kinv = c.alloc<double>(l.nfp * l.nfp);
             l.bf = c.alloc<double>(l.nfp * nrhs);
+            l.ytmp = c.alloc<double>(l.nfp * nrhs);
+            l.d0c = c.alloc<double>(l.nfp);
             if (want32 && coarse_fp32) l.Kinv32 = c.alloc<float>(l.nfp * l.nfp);
         }
         const int64_t n = l.nfp;
+        const int NB = (int)(n / kCB);
+        const int wb = std::max(1, (l.bandw + kCB - 1) / kCB);
         coarse_block(c, l);
+        // banded Cholesky when the band is narrow enough to pay off
         bool done = false;
-        if (n > 256) {
-            int* dinfo = (int*)(c.dsc + 14);   // device scratch word
-            rocblas_handle h = c.blas_handle();
-            TOPO_CHECK(rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)n, l.Kinv, (rocblas_int)n, dinfo) ==
-                           rocblas_status_success, TOPO_E_HIP, "rocsolver_dpotrf failed");
-            int info = 0;
-            TOPO_HIP(hipMemcpyAsync(&c.h_sc[0], dinfo, sizeof(int), hipMemcpyDeviceToHost, c.s));
-            c.sync();
-            std::memcpy(&info, &c.h_sc[0], sizeof(int));
-            if (info == 0) {
-                TOPO_CHECK(rocsolver_dpotri(h, rocblas_fill_lower, (rocblas_int)n, l.Kinv, (rocblas_int)n, dinfo) ==
-                               rocblas_status_success, TOPO_E_HIP, "rocsolver_dpotri failed");
-                TOPO_HIP(hipMemcpyAsync(&c.h_sc[0], dinfo, sizeof(int), hipMemcpyDeviceToHost, c.s));
-                c.sync();
-                std::memcpy(&info, &c.h_sc[0], sizeof(int));
-                if (info == 0) {
-                    hipLaunchKernelGGL(k_symmetrize_upper, dim3(nblk(n * n)), dim3(256), 0, c.s, l.Kinv, n);
-                    done = true;
+        if (NB > 2 * wb) {
+            if (!l.Z) {
+                l.Z = c.alloc<double>(n * n);
+                l.Linv = c.alloc<double>((int64_t)NB * kCB * kCB);
+                if (want32 && coarse_fp32) l.Z32 = c.alloc<float>(n * n);
+            }
+            int* dflag = (int*)(c.dsc + 14);   // device scratch word
+            hipLaunchKernelGGL(k_diag_copy, dim3(nblk(n)), dim3(256), 0, c.s, (const double*)l.Kinv, n, l.d0c);
+            hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, c.s, dflag, 0, (int64_t)1, (const int*)nullptr);
+            for (int K = 0; K < NB; K++) {
+                hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(256), 0, c.s, l.Kinv, n, K, l.Linv,
+                                   (const double*)l.d0c, 1e-11, dflag);
+                const int nwin = std::min(wb, NB - 1 - K);
+                if (nwin > 0) {
+                    hipLaunchKernelGGL(k_chol_panel, dim3(nwin), dim3(256), 0, c.s, l.Kinv, n, K, NB, wb,
+                                       (const double*)l.Linv);
+                    hipLaunchKernelGGL(k_chol_update, dim3(nwin * (nwin + 1) / 2), dim3(256), 0, c.s, l.Kinv, n, K,
+                                       NB, wb);
                 }
             }
+            c.check_launch();
+            TOPO_HIP(hipMemcpyAsync(&c.h_sc[0], dflag, sizeof(int), hipMemcpyDeviceToHost, c.s));
             c.sync();
-            if (!done) coarse_block(c, l);   // the factorization overwrote the block
+            int flag = 0;
+            std::memcpy(&flag, &c.h_sc[0], sizeof(int));
+            if (!flag) {
+                // Z = L^-1 row block by row block, then Z^T into the Kinv buffer
+                hipLaunchKernelGGL(k_fill<double>, dim3(nblk(n * n)), dim3(256), 0, c.s, l.Z, 0.0, n * n,
+                                   (const int*)nullptr);
+                for (int I = 0; I < NB; I++)
+                    hipLaunchKernelGGL(k_chol_zrow, dim3(I + 1), dim3(256), 0, c.s, (const double*)l.Kinv, l.Z, n, I,
+                                       wb, (const double*)l.Linv);
+                hipLaunchKernelGGL((k_transpose<double, double>), dim3(NB, NB), dim3(256), 0, c.s,
+                                   (const double*)l.Z, l.Kinv, n);
+                if (l.Z32) {
+                    hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(n * n)), dim3(256), 0, c.s, l.Z32,
+                                       (const double*)l.Z, n * n);
+                    hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(n * n)), dim3(256), 0, c.s, l.Kinv32,
+                                       (const double*)l.Kinv, n * n);
+                }
+                done = true;
+                l.tri = true;
+            } else {
+                coarse_block(c, l);   // the factorization overwrote the block
+            }
         }
-        if (!done) gauss_jordan(c, l.Kinv, n);
-        if (l.Kinv32)
-            hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(n * n)), dim3(256), 0, c.s, l.Kinv32,
-                               (const double*)l.Kinv, n * n);
+        if (!done) {
+            // not positive definite (or too small a block to band): dense
+            // Gauss-Jordan inverse with pivot dropping
+            gauss_jordan(c, l.Kinv, n);
+            if (l.Kinv32)
+                hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(n * n)), dim3(256), 0, c.s, l.Kinv32,
+                                   (const double*)l.Kinv, n * n);
+            l.tri = false;
+        }
         c.check_launch();
         l.chol = true;
     }
@@ -1000,9 +1051,32 @@ struct MGH {
             hipLaunchKernelGGL(k_gather_free<T>, dim3(nblk(l.nfp), nr), dim3(256), 0, c.s, b, l.freeIdx, l.nf, l.nfp, l.bf,
                                gate, l.ndof);
             const dim3 gi((unsigned)((l.nf + 15) / 16));
+            const bool f32 = std::is_same<T, float>::value && l.Kinv32;
+            if (l.tri) {
+                // x = Z^T (Z b): lower-triangular Z into ytmp, then the upper
+                // triangular Z^T scattered to the free DOFs; the FP32 V-cycle
+                // reads the FP32 roundings (coarse_fp32)
+                const dim3 gt((unsigned)(l.nfp / 16));
+                if (f32) {
+                    TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_apply<T, NR, float>), gt, dim3(256), 0, c.s,
+                                                          (const float*)l.Z32, l.nfp, 0, (const double*)l.bf, l.ytmp,
+                                                          (T*)nullptr, l.freeIdx, l.nf, l.ndof, gate));
+                    TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_apply<T, NR, float>), gt, dim3(256), 0, c.s,
+                                                          (const float*)l.Kinv32, l.nfp, 1, (const double*)l.ytmp,
+                                                          (double*)nullptr, x, l.freeIdx, l.nf, l.ndof, gate));
+                } else {
+                    TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_apply<T, NR, double>), gt, dim3(256), 0, c.s,
+                                                          (const double*)l.Z, l.nfp, 0, (const double*)l.bf, l.ytmp,
+                                                          (T*)nullptr, l.freeIdx, l.nf, l.ndof, gate));
+                    TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_apply<T, NR, double>), gt, dim3(256), 0, c.s,
+                                                          (const double*)l.Kinv, l.nfp, 1, (const double*)l.ytmp,
+                                                          (double*)nullptr, x, l.freeIdx, l.nf, l.ndof, gate));
+                }
+                return x;
+            }
             // the FP32 V-cycle reads the FP32 rounding of the FP64 inverse
             // (half the bytes) when coarse_fp32 is set
-            if (std::is_same<T, float>::value && l.Kinv32) {
+            if (f32) {
                 const float* Ki = l.Kinv32;
                 TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_inv_apply<T, NR, float>), gi, dim3(256), 0, c.s, Ki, l.freeIdx,
                                                       l.nf, l.nfp, (const double*)l.bf, x, gate, l.ndof));
