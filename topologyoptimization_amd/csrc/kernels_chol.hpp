@@ -7,13 +7,14 @@
 // every nonzero satisfies |i - j| <= w, w ~ 3 (nny nnz + nnz + 1). Cholesky
 // keeps the band, so the blocked right-looking factorization only visits the
 // 64 x 64 blocks within wb = ceil(w / 64) block-diagonals:
-//   for K: L_KK = chol(A_KK) and L_KK^-1     (k_chol_diag, one workgroup)
+//   for K: L_KK = chol(A_KK) and L_KK^-1     (k_chol_diag, one wavefront)
 //          A_IK <- A_IK L_KK^-T, I in (K, K+wb]         (k_chol_panel)
 //          A_IJ -= A_IK A_JK^T,  K < J <= I <= K+wb      (k_chol_update)
 // Z = L^-1 is dense lower triangular; its row blocks follow from
-//   Z_II = L_II^-1,  Z_IJ = -L_II^-1 sum_{K=max(J,I-wb)}^{I-1} L_IK Z_KJ  (k_chol_zrow),
-// all J of one I in parallel. The coarse solve is then x = Z^T (Z b): two
-// triangular matrix-vector products (k_tri_apply) over Z and its transpose.
+//   Z_II = L_II^-1,  Z_I,0:I = -L_II^-1 sum_{K=max(0,I-wb)}^{I-1} L_IK Z_K,0:I
+// (two DGEMMs per row block on the host side). The coarse solve is then
+// x = Z^T (Z b): two triangular matrix-vector products (k_tri_apply) over Z
+// and its transpose.
 // All of it is O(n w^2 + n^2 w) work instead of the O(n^3) of a dense inverse.
 #pragma once
 #include "kernels.hpp"
@@ -31,51 +32,49 @@ __device__ __forceinline__ void blk_load(double* s, const double* A, int64_t ld)
     }
 }
 
-// Cholesky of the diagonal block K in LDS, then the inverse of its factor.
-// Writes L_KK (upper part of the block zeroed) back to A and L_KK^-1 to
-// Linv[K]. A pivot <= tiny * (original diagonal of that DOF) sets *flag:
-// the block is not (numerically) positive definite.
-__global__ void __launch_bounds__(256) k_chol_diag(double* __restrict__ A, int64_t ld, int K,
-                                                   double* __restrict__ Linv, const double* __restrict__ d0,
-                                                   double tiny, int* flag) {
+// Cholesky of the diagonal block K in LDS, then the inverse of its factor,
+// by one wavefront (blockDim 64: lane i owns row i, so the barriers are
+// single-wave and cheap). Writes L_KK (upper part of the block zeroed) back
+// to A and L_KK^-1 to Linv[K]. A pivot <= tiny * (original diagonal of that
+// DOF) sets *flag: the block is not (numerically) positive definite.
+__global__ void __launch_bounds__(64) k_chol_diag(double* __restrict__ A, int64_t ld, int K,
+                                                  double* __restrict__ Linv, const double* __restrict__ d0,
+                                                  double tiny, int* flag) {
     __shared__ double a[kCB * kCS];
     __shared__ double z[kCB * kCS];
     double* Ab = A + (int64_t)K * kCB * ld + (int64_t)K * kCB;
     blk_load(a, Ab, ld);
     __syncthreads();
-    const int t = threadIdx.x;
+    const int i = threadIdx.x;
     for (int j = 0; j < kCB; j++) {
-        if (t == 0) {
+        if (i == j) {
             const double d = a[j * kCS + j];
             if (!(d > tiny * d0[(int64_t)K * kCB + j])) *flag = 1;
             a[j * kCS + j] = d > 0 ? sqrt(d) : 1.0;
         }
         __syncthreads();
-        const double ljj = a[j * kCS + j];
-        for (int i = j + 1 + t; i < kCB; i += blockDim.x) a[i * kCS + j] /= ljj;
+        if (i > j) a[i * kCS + j] /= a[j * kCS + j];
         __syncthreads();
-        // trailing update of the lower triangle: (i, k), j < k <= i
-        const int m = kCB - 1 - j;
-        for (int e = t; e < m * m; e += blockDim.x) {
-            const int i = j + 1 + e / m, k = j + 1 + e % m;
-            if (k <= i) a[i * kCS + k] -= a[i * kCS + j] * a[k * kCS + j];
+        if (i > j) {
+            const double lij = a[i * kCS + j];
+            for (int k = j + 1; k <= i; k++) a[i * kCS + k] -= lij * a[k * kCS + j];
         }
-        __syncthreads();
     }
-    // inverse of the lower-triangular factor, one column per thread
-    if (t < kCB) {
-        const int c = t;
-        for (int i = 0; i < c; i++) z[i * kCS + c] = 0.0;
+    __syncthreads();
+    // inverse of the lower-triangular factor, lane c computes column c
+    {
+        const int c = i;
+        for (int r = 0; r < c; r++) z[r * kCS + c] = 0.0;
         z[c * kCS + c] = 1.0 / a[c * kCS + c];
-        for (int i = c + 1; i < kCB; i++) {
+        for (int r = c + 1; r < kCB; r++) {
             double s = 0.0;
-            for (int k = c; k < i; k++) s += a[i * kCS + k] * z[k * kCS + c];
-            z[i * kCS + c] = -s / a[i * kCS + i];
+            for (int k = c; k < r; k++) s += a[r * kCS + k] * z[k * kCS + c];
+            z[r * kCS + c] = -s / a[r * kCS + r];
         }
     }
     __syncthreads();
     double* Li = Linv + (int64_t)K * kCB * kCB;
-    for (int e = t; e < kCB * kCB; e += blockDim.x) {
+    for (int e = i; e < kCB * kCB; e += blockDim.x) {
         const int r = e / kCB, c = e % kCB;
         Ab[(int64_t)r * ld + c] = c <= r ? a[r * kCS + c] : 0.0;
         Li[e] = z[r * kCS + c];
@@ -135,43 +134,12 @@ __global__ void __launch_bounds__(256) k_chol_update(double* __restrict__ A, int
         for (int jj = 0; jj < 4; jj++) Cb[(int64_t)(r0 + i) * ld + c0 + jj] -= acc[i][jj];
 }
 
-// row block I of Z = L^-1: Z_IJ for J = blockIdx.x (J < I), Z_II = L_II^-1
-__global__ void __launch_bounds__(256) k_chol_zrow(const double* __restrict__ L, double* __restrict__ Z, int64_t ld,
-                                                   int I, int wb, const double* __restrict__ Linv) {
-    const int J = (int)blockIdx.x;
-    if (J > I) return;
-    double* Zb = Z + (int64_t)I * kCB * ld + (int64_t)J * kCB;
-    const double* Li = Linv + (int64_t)I * kCB * kCB;
-    if (J == I) {
-        for (int e = threadIdx.x; e < kCB * kCB; e += blockDim.x) Zb[(int64_t)(e / kCB) * ld + e % kCB] = Li[e];
-        return;
+// copy a 64 x 64 block (row-major, leading dimensions lds / ldd)
+__global__ void k_copy_block(const double* __restrict__ src, int64_t lds, double* __restrict__ dst, int64_t ldd) {
+    for (int e = threadIdx.x; e < kCB * kCB; e += blockDim.x) {
+        const int r = e / kCB, c = e % kCB;
+        dst[(int64_t)r * ldd + c] = src[(int64_t)r * lds + c];
     }
-    __shared__ double x[kCB * kCS];
-    __shared__ double y[kCB * kCS];
-    const int r0 = (threadIdx.x / 16) * 4, c0 = (threadIdx.x % 16) * 4;
-    double acc[4][4] = {};
-    // T = sum_K L_IK Z_KJ: y holds Z_KJ transposed so blk_mul_nt applies
-    for (int K = max(J, I - wb); K < I; K++) {
-        __syncthreads();
-        blk_load(x, L + (int64_t)I * kCB * ld + (int64_t)K * kCB, ld);
-        const double* Zk = Z + (int64_t)K * kCB * ld + (int64_t)J * kCB;
-        for (int e = threadIdx.x; e < kCB * kCB; e += blockDim.x) {
-            const int r = e / kCB, c = e % kCB;
-            y[c * kCS + r] = Zk[(int64_t)r * ld + c];
-        }
-        __syncthreads();
-        blk_mul_nt(x, y, acc);
-    }
-    // Z_IJ = -L_II^-1 T
-    __syncthreads();
-    for (int i = 0; i < 4; i++)
-        for (int jj = 0; jj < 4; jj++) y[(c0 + jj) * kCS + r0 + i] = acc[i][jj];   // y = T^T
-    for (int e = threadIdx.x; e < kCB * kCB; e += blockDim.x) x[(e / kCB) * kCS + e % kCB] = Li[e];
-    __syncthreads();
-    double out[4][4] = {};
-    blk_mul_nt(x, y, out);
-    for (int i = 0; i < 4; i++)
-        for (int jj = 0; jj < 4; jj++) Zb[(int64_t)(r0 + i) * ld + c0 + jj] = -out[i][jj];
 }
 
 // out = in^T for an n x n row-major matrix (64 x 64 tiles through LDS)

@@ -290,6 +290,7 @@ struct MGLevel {
     double* Linv = nullptr;        // inverses of the diagonal blocks of L
     double* ytmp = nullptr;        // Z b (nrhs x nfp)
     double* d0c = nullptr;         // original diagonal (pivot test)
+    double* ztmp = nullptr;        // 64 x nfp scratch of the Z row blocks
     double* work = nullptr;
 };
 
@@ -911,7 +912,7 @@ struct MGH {
             hipLaunchKernelGGL(k_diag_copy, dim3(nblk(n)), dim3(256), 0, c.s, (const double*)l.Kinv, n, l.d0c);
             hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, c.s, dflag, 0, (int64_t)1, (const int*)nullptr);
             for (int K = 0; K < NB; K++) {
-                hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(256), 0, c.s, l.Kinv, n, K, l.Linv,
+                hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(64), 0, c.s, l.Kinv, n, K, l.Linv,
                                    (const double*)l.d0c, 1e-11, dflag);
                 const int nwin = std::min(wb, NB - 1 - K);
                 if (nwin > 0) {
@@ -930,9 +931,7 @@ struct MGH {
                 // Z = L^-1 row block by row block, then Z^T into the Kinv buffer
                 hipLaunchKernelGGL(k_fill<double>, dim3(nblk(n * n)), dim3(256), 0, c.s, l.Z, 0.0, n * n,
                                    (const int*)nullptr);
-                for (int I = 0; I < NB; I++)
-                    hipLaunchKernelGGL(k_chol_zrow, dim3(I + 1), dim3(256), 0, c.s, (const double*)l.Kinv, l.Z, n, I,
-                                       wb, (const double*)l.Linv);
+                zrows(c, l, NB, wb);
                 hipLaunchKernelGGL((k_transpose<double, double>), dim3(NB, NB), dim3(256), 0, c.s,
                                    (const double*)l.Z, l.Kinv, n);
                 if (l.Z32) {
@@ -959,6 +958,38 @@ struct MGH {
         c.check_launch();
         l.chol = true;
     }
+    // Z = L^-1 by row blocks: Z_II = L_II^-1 and, for the columns left of
+    // the diagonal block, Z_I,0:I = -L_II^-1 (L_I,band Z_band,0:I), band = the
+    // wb row blocks above I. Both products are plain rocBLAS DGEMMs on
+    // row-major sub-matrices (a row-major C = A B is the column-major
+    // C^T = B^T A^T). T is scratch of 64 x n.
+    void zrows(Ctx& c, MGLevel& l, int NB, int wb) {
+        const int64_t n = l.nfp;
+        if (!l.ztmp) l.ztmp = c.alloc<double>(kCB * n);
+        rocblas_handle h = c.blas_handle();
+        for (int I = 0; I < NB; I++) {
+            // Z_II
+            hipLaunchKernelGGL(k_copy_block, dim3(1), dim3(256), 0, c.s, (const double*)l.Linv + (int64_t)I * kCB * kCB,
+                               (int64_t)kCB, l.Z + (int64_t)I * kCB * n + (int64_t)I * kCB, n);
+            if (I == 0) continue;
+            const int K0 = std::max(0, I - wb);
+            const int kk = (I - K0) * kCB;     // inner dimension
+            const int nc = I * kCB;             // columns 0 .. I*64
+            // T (64 x nc, ld n) = L[I, K0:I] (64 x kk, ld n) * Z[K0:I, 0:nc] (kk x nc, ld n)
+            const double one = 1.0, zero = 0.0, mone = -1.0;
+            const double* Lrow = l.Kinv + (int64_t)I * kCB * n + (int64_t)K0 * kCB;
+            const double* Zb = l.Z + (int64_t)K0 * kCB * n;
+            TOPO_CHECK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nc, kCB, kk, &one, Zb,
+                                     (rocblas_int)n, Lrow, (rocblas_int)n, &zero, l.ztmp, (rocblas_int)n) ==
+                           rocblas_status_success, TOPO_E_HIP, "rocblas_dgemm failed");
+            // Z[I, 0:nc] = -Linv_I (64 x 64) * T (64 x nc)
+            TOPO_CHECK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nc, kCB, kCB, &mone,
+                                     l.ztmp, (rocblas_int)n, l.Linv + (int64_t)I * kCB * kCB, kCB, &zero,
+                                     l.Z + (int64_t)I * kCB * n, (rocblas_int)n) == rocblas_status_success,
+                       TOPO_E_HIP, "rocblas_dgemm failed");
+        }
+    }
+
     // in-place inverse of the n x n block A (row-major, ld n) by blocked
     // Gauss-Jordan with pivot dropping
     void gauss_jordan(Ctx& c, double* A, int64_t nf) {
