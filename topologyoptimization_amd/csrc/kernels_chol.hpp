@@ -32,52 +32,65 @@ __device__ __forceinline__ void blk_load(double* s, const double* A, int64_t ld)
     }
 }
 
-// Cholesky of the diagonal block K in LDS, then the inverse of its factor,
-// by one wavefront (blockDim 64: lane i owns row i, so the barriers are
-// single-wave and cheap). Writes L_KK (upper part of the block zeroed) back
-// to A and L_KK^-1 to Linv[K]. A pivot <= tiny * (original diagonal of that
-// DOF) sets *flag: the block is not (numerically) positive definite.
+// double from lane `lane` (a compile-time or wave-uniform index)
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
+// Cholesky of the diagonal block K and the inverse of its factor, by one
+// wavefront holding the block in registers: lane i owns row i (r[0..63]);
+// at step j the pivot and the column l_kj come from the owning lanes by
+// readlane (uniform operands, no LDS round trips, no dependent chains
+// across steps other than the pivot). The factor goes through LDS to the
+// inverse, which lane c computes for column c by forward substitution.
+// Writes L_KK (upper part zeroed) back to A and L_KK^-1 to Linv[K]. A pivot
+// <= tiny * (original diagonal of that DOF) sets *flag.
 __global__ void __launch_bounds__(64) k_chol_diag(double* __restrict__ A, int64_t ld, int K,
                                                   double* __restrict__ Linv, const double* __restrict__ d0,
                                                   double tiny, int* flag) {
     __shared__ double a[kCB * kCS];
-    __shared__ double z[kCB * kCS];
     double* Ab = A + (int64_t)K * kCB * ld + (int64_t)K * kCB;
     blk_load(a, Ab, ld);
     __syncthreads();
     const int i = threadIdx.x;
+    double r[kCB];
+#pragma unroll
+    for (int k = 0; k < kCB; k++) r[k] = a[i * kCS + k];
+#pragma unroll
     for (int j = 0; j < kCB; j++) {
-        if (i == j) {
-            const double d = a[j * kCS + j];
-            if (!(d > tiny * d0[(int64_t)K * kCB + j])) *flag = 1;
-            a[j * kCS + j] = d > 0 ? sqrt(d) : 1.0;
-        }
-        __syncthreads();
-        if (i > j) a[i * kCS + j] /= a[j * kCS + j];
-        __syncthreads();
-        if (i > j) {
-            const double lij = a[i * kCS + j];
-            for (int k = j + 1; k <= i; k++) a[i * kCS + k] -= lij * a[k * kCS + j];
-        }
-    }
-    __syncthreads();
-    // inverse of the lower-triangular factor, lane c computes column c
-    {
-        const int c = i;
-        for (int r = 0; r < c; r++) z[r * kCS + c] = 0.0;
-        z[c * kCS + c] = 1.0 / a[c * kCS + c];
-        for (int r = c + 1; r < kCB; r++) {
-            double s = 0.0;
-            for (int k = c; k < r; k++) s += a[r * kCS + k] * z[k * kCS + c];
-            z[r * kCS + c] = -s / a[r * kCS + r];
+        const double d = readlane_d(r[j], j);
+        if (i == 0 && !(d > tiny * d0[(int64_t)K * kCB + j])) *flag = 1;
+        const double ljj = d > 0 ? sqrt(d) : 1.0;
+        r[j] = i == j ? ljj : (i > j ? r[j] / ljj : 0.0);
+#pragma unroll
+        for (int k = j + 1; k < kCB; k++) {
+            r[k] -= r[j] * readlane_d(r[j], k);
+            // keep the scheduler from hoisting all readlanes of a step
+            // ahead of their uses (SGPR pressure)
+            if ((k & 7) == 7) __builtin_amdgcn_sched_barrier(0);
         }
     }
     __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kCB; k++) a[i * kCS + k] = k <= i ? r[k] : 0.0;
+    __syncthreads();
+    // inverse: lane c solves L x = e_c (x_r = 0 for r < c falls out)
+    const int c = i;
+#pragma unroll
+    for (int rr = 0; rr < kCB; rr++) {
+        double s = rr == c ? 1.0 : 0.0;
+#pragma unroll
+        for (int k = 0; k < rr; k++) s -= a[rr * kCS + k] * r[k];
+        r[rr] = s / a[rr * kCS + rr];
+    }
     double* Li = Linv + (int64_t)K * kCB * kCB;
+#pragma unroll
+    for (int rr = 0; rr < kCB; rr++) Li[rr * kCB + c] = r[rr];
     for (int e = i; e < kCB * kCB; e += blockDim.x) {
-        const int r = e / kCB, c = e % kCB;
-        Ab[(int64_t)r * ld + c] = c <= r ? a[r * kCS + c] : 0.0;
-        Li[e] = z[r * kCS + c];
+        const int rw = e / kCB, cl = e % kCB;
+        Ab[(int64_t)rw * ld + cl] = a[rw * kCS + cl];
     }
 }
 
