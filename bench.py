@@ -122,6 +122,7 @@ def main():
                                    coarse_fp32=args.coarse_fp32)
     # compliance of the mixed path must match the FP64 path (checked below)
     # dominant kernel: fine-level FP32 applyA, timed with HIP events on the solver's stream
+    tR, tnt, tcx, _ = S.tuning()
     ms32, by32 = S.time_op(T.TIME_APPLYA, T.PREC_MIXED, 0, 50)
     ms64, by64 = S.time_op(T.TIME_APPLYA, T.PREC_FP64, 0, 30)
     msc, byc = S.time_op(T.TIME_COPY, T.PREC_FP64, 0, 30)
@@ -175,6 +176,7 @@ def main():
         "compliance": infos[-1]["compliance"],
         "roofline": roof,
         "cpu_baseline": cpu,
+        "autotuned_fine_tile": {"xm_nt": tnt, "xm_cx": tcx},
         "device": name,
     }
     print(json.dumps(out))

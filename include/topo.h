@@ -254,12 +254,15 @@ int topo_solver_get_field(topo_solver* s, int32_t field, void* out);
  * solved by one batched MG-PCG per group of rhs_tile cases (one launch per
  * operation carries all of them, the coarsest-level inverse is read once per
  * group, each case keeps its own CG scalars and stops on its own). rhs_tile
- * 0 autotunes at the first solve: the group size R in {1, 2, 4} and the
- * x-march element-tile shape (workgroup NT in {512, 1024}, x-planes CX in
- * {auto, 4, 8, 16}) with the least time per right-hand side for the fine
- * operator plus one V-cycle. This reports the choice (rhs_tile 1 when no
- * batching applies) and the timings: log receives max_entries x {R, NT, CX,
- * ms per right-hand side}. Returns the number of tuning entries. */
+ * 0 autotunes at the first solve: the x-march element-tile shape of the two
+ * finest levels (x-planes per workgroup CX in {auto, 2, 3, 4, 5, 6, 8, 10,
+ * 12, 16}; results are bit-identical for every CX) and, with several load
+ * cases, the group size R in {1, 2, 4} with the least time per right-hand
+ * side for the fine operator plus one V-cycle. This reports the choice
+ * (rhs_tile 1 when no batching applies, NT/CX of the finest level) and the
+ * timings: log receives max_entries x {what, NT, CX, ms}, what = the MG level
+ * of a tile timing (ms per launch) or -R for a batch timing (ms per
+ * right-hand side). Returns the number of tuning entries. */
 int topo_solver_tuning(topo_solver* s, int32_t* rhs_tile, int32_t* xm_nt, int32_t* xm_cx, double* log,
                        int32_t max_entries);
 

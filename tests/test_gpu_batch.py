@@ -64,13 +64,35 @@ def test_autotuner_and_simp_steps(gpu):
     with T.Solver(cfg, fixed, F) as S:
         infos = [S.step() for _ in range(3)]
         R, nt, cx, log = S.tuning()
-    assert R in (1, 2, 4) and nt in (512, 1024) and cx in (0, 4, 8, 16)
-    assert len(log) == 3 * 8 and np.all(log[:, 3] > 0)
-    best = log[np.argmin(log[:, 3])]
-    assert (best[0], best[1], best[2]) == (R, nt, cx)
-    print("autotuned: R=%d NT=%d CX=%d, %.3f ms per right-hand side" % (R, nt, cx, best[3]))
+    tiles, batches = log[log[:, 0] >= 0], log[log[:, 0] < 0]
+    assert len(batches) == 3 and np.all(batches[:, 3] > 0)
+    assert R == -int(batches[np.argmin(batches[:, 3]), 0])
+    fine = tiles[tiles[:, 0] == 0]
+    assert len(fine) == 10 and np.all(fine[:, 3] > 0)
+    best = fine[np.argmin(fine[:, 3])]
+    assert (nt, cx) == (int(best[1]), int(best[2]))
+    print("autotuned: R=%d NT=%d CX=%d, fine operator %.4f ms" % (R, nt, cx, best[3]))
     cfg1, fixed, F, _, _, _ = _bridge(T, precision=2, rhs_tile=1, cg_tol=1e-6, cg_max_iter=400)
     with T.Solver(cfg1, fixed, F) as S:
         ref = [S.step() for _ in range(3)]
     for a, b in zip(infos, ref):
         assert abs(a.compliance - b.compliance) <= 1e-6 * abs(b.compliance)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_tile_shape_does_not_change_results(gpu, prec):
+    """The autotuned element tiles give the same bits as the default ones:
+    every node's sums are formed in the same order whatever the tiling.
+    (E-average coarse levels: the dense Galerkin assembly of the reference
+    mode adds with atomics and is reproducible to rounding only.)"""
+    T = gpu
+    nel, ndof = T.dims(40, 20, 20)
+    out = []
+    for tile in (0, 1):
+        cfg, fixed, F, _, _, x0 = P.cantilever(40, 20, 20, linear_solver=T.SOLVER_MGPCG, precision=prec,
+                                               rhs_tile=tile, mg_levels=3, cg_tol=1e-8, galerkin_max_ndof=0)
+        with T.Solver(cfg, fixed, F) as S:
+            U, c, _ = S.fe(P.splitmix64_uniform(nel, lo=0.2), 3.0)
+            out.append((U, c, S.tuning()))
+    assert np.array_equal(out[0][0], out[1][0]) and out[0][1] == out[1][1]
+    assert len(out[0][2][3]) > 0 and len(out[1][2][3]) == 0

@@ -673,8 +673,9 @@ struct MGH {
     double thr = 1e-6;   // skip threshold 1000 * Emin
     bool want32 = false;
     bool exact = false;   // reference-order kernel everywhere
-    int xm_nt = 0;        // x-march workgroup size (0 auto, 512, 1024)
-    int xm_cx = 0;        // x-march planes per workgroup (0 auto)
+    int xm_nt = 0;        // x-march workgroup size for every level (0 auto, 512, 1024)
+    int xm_cx = 0;        // x-march planes per workgroup for every level (0 auto)
+    std::vector<int> lv_nt, lv_cx;   // per-level choices of the autotuner (0 auto)
     DevKe* K0 = nullptr;
     Springs springs{};    // mechanism ports on the finest level
     int nrhs = 1;         // right-hand sides a batched V-cycle can carry (level vectors are nrhs x ndof)
@@ -1040,11 +1041,13 @@ struct MGH {
         } else {
             typedef typename Sel<T>::TE TE;
             const int H = l.g.nnz() + 1;
-            const int NT = (xm_nt == 1024 || (xm_nt == 0 && 4 * H > 512)) ? 1024 : 512;
+            const int nt_sel = xm_nt ? xm_nt : (li < (int)lv_nt.size() ? lv_nt[li] : 0);
+            const int cx_sel = xm_cx ? xm_cx : (li < (int)lv_cx.size() ? lv_cx[li] : 0);
+            const int NT = (nt_sel == 1024 || (nt_sel == 0 && 4 * H > 512)) ? 1024 : 512;
             if (!exact && K0->had_ok && 2 * H <= NT) {
                 const int nyz = l.g.nny() * l.g.nnz();
                 const int gx = (nyz + (NT - H) - 1) / (NT - H);
-                int CX = xm_cx > 0 ? xm_cx : (int)((int64_t)l.g.nnx() * gx * (NT / 512) / 1024);
+                int CX = cx_sel > 0 ? cx_sel : (int)((int64_t)l.g.nnx() * gx * (NT / 512) / 1024);
                 CX = std::max(2, std::min(32, CX));
                 const int gy = (l.g.nnx() + CX - 1) / CX;
                 if (NT == 512) {
@@ -1649,11 +1652,12 @@ struct Solver {
         const int maxit = (int)std::min<int64_t>(nfree_cap, nfree);
         CGResult tot;
         int outer = 0;
+        if (!tuned) {
+            if (cfg.rhs_tile == 0) autotune();
+            else rhs_tile = std::min(std::max(cfg.rhs_tile, 1), batch_cap);
+            tuned = true;
+        }
         if (batch_cap > 1) {
-            if (rhs_tile == 0) {
-                if (cfg.rhs_tile > 0) rhs_tile = std::min(cfg.rhs_tile, batch_cap);
-                else autotune();
-            }
             for (int k = 0; k < cfg.nload;) {
                 int r = rhs_tile;
                 while (r > cfg.nload - k) r /= 2;
@@ -1689,18 +1693,23 @@ struct Solver {
         tot.relres = std::max(tot.relres, r.relres);
     }
 
-    // Autotuner of the batched solve: for every candidate (right-hand sides
-    // per batch R in {1, 2, 4} up to batch_cap; x-march workgroup NT in {512,
-    // 1024} and x-planes per workgroup CX in {auto, 4, 8, 16}) time the
-    // fine-level operator and one preconditioner application of R
-    // right-hand sides with HIP events on the solver's stream, and keep the
-    // shape with the least time per right-hand side (operator + V-cycle, the
-    // work of one CG iteration). Runs once, on the operators of the first
-    // solve; the cached graphs are dropped afterwards.
+    // Autotuner (rhs_tile 0), run once on the operators of the first solve
+    // with HIP events on the solver's stream:
+    //  1. element-tile shape of the x-march operator on the two finest
+    //     levels: x-planes per workgroup CX in {auto, 2, 3, 4, 5, 6, 8, 10,
+    //     12, 16} (the workgroup size keeps its rule: another one is another
+    //     compiled kernel whose contractions may round differently), timed
+    //     on the operator in the V-cycle's precision; CX only moves work
+    //     between workgroups, every node's sums keep their order, so the
+    //     results are bit-identical whatever the choice;
+    //  2. with several load cases, the batch size R in {1, 2, 4} up to
+    //     batch_cap, timed on one fine operator plus one V-cycle of R
+    //     right-hand sides, normalised per right-hand side.
+    // The cached graphs are dropped afterwards.
     void autotune() {
         Ctx& C = *c;
         const bool pc = cfg.precision == TOPO_PREC_MIXED_PC;
-        // right-hand sides: copies of the first load case
+        const bool f32 = cfg.precision != TOPO_PREC_FP64;
         for (int q = 0; q < batch_cap; q++)
             TOPO_HIP(hipMemcpyAsync((double*)v64.r + q * ndof, F + (int64_t)(q % cfg.nload) * ndof,
                                     ndof * sizeof(double), hipMemcpyDeviceToDevice, C.s));
@@ -1717,38 +1726,63 @@ struct Solver {
             TOPO_HIP(hipEventElapsedTime(&ms, e0, e1));
             return (double)ms / reps;
         };
-        const int nt_c[2] = {512, 1024};
-        const int cx_c[4] = {0, 4, 8, 16};
-        double best = 1e300;
-        int bR = 1, bNT = mg.xm_nt, bCX = mg.xm_cx;
         tune_log.clear();
-        for (int R = 1; R <= batch_cap; R *= 2)
-            for (int a = 0; a < 2; a++)
-                for (int b = 0; b < 4; b++) {
-                    mg.xm_nt = nt_c[a];
-                    mg.xm_cx = cx_c[b];
+        const int cx_c[10] = {0, 2, 3, 4, 5, 6, 8, 10, 12, 16};
+        const int nl = (int)mg.L.size();
+        mg.lv_nt.assign(nl, 0);
+        mg.lv_cx.assign(nl, 0);
+        if (!mg.xm_nt && !mg.xm_cx && !mg.exact) {
+            for (int li = 0; li < std::min(nl, 2); li++) {
+                MGLevel& l = mg.L[li];
+                if (l.dense) continue;
+                // operands: the level's own vectors, filled from the rhs copy
+                hipLaunchKernelGGL((k_convert<double, double>), dim3(nblk(l.ndof)), dim3(256), 0, C.s, l.x,
+                                   (const double*)v64.r, l.ndof);
+                if (f32)
+                    hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(l.ndof)), dim3(256), 0, C.s, l.x32,
+                                       (const double*)v64.r, l.ndof);
+                double best = 1e300;
+                const int H = l.g.nnz() + 1;
+                const int nt_rule = 4 * H > 512 ? 1024 : 512;
+                for (int b = 0; b < 10; b++) {
+                    mg.lv_nt[li] = 0;
+                    mg.lv_cx[li] = cx_c[b];
                     const double ms = time_it([&]() {
-                        mg.apply<double>(C, 0, (const double*)v64.r, (double*)v64.Ap, 1, nullptr, nullptr, R);
-                        if (pc)
-                            precondition<double, float>(C, mg, (const double*)v64.r, (double*)v64.z, v32, nullptr, R);
-                        else
-                            precondition<double, double>(C, mg, (const double*)v64.r, (double*)v64.z, v64, nullptr, R);
-                    }, 3);
-                    const double per = ms / R;
-                    tune_log.push_back({R, nt_c[a], cx_c[b], per});
-                    if (per < best) { best = per; bR = R; bNT = nt_c[a]; bCX = cx_c[b]; }
+                        if (f32) mg.apply<float>(C, li, l.x32, l.x232, 1, nullptr, nullptr);
+                        else mg.apply<double>(C, li, l.x, l.x2, 1, nullptr, nullptr);
+                    }, 5);
+                    tune_log.push_back({li, nt_rule, cx_c[b], ms});
+                    if (ms < best) { best = ms; tuned_nt[li] = nt_rule; tuned_cx[li] = cx_c[b]; }
                 }
+                mg.lv_nt[li] = 0;
+                mg.lv_cx[li] = tuned_cx[li];
+            }
+        }
+        rhs_tile = 1;
+        if (batch_cap > 1) {
+            double best = 1e300;
+            for (int R = 1; R <= batch_cap; R *= 2) {
+                const double ms = time_it([&]() {
+                    mg.apply<double>(C, 0, (const double*)v64.r, (double*)v64.Ap, 1, nullptr, nullptr, R);
+                    if (pc)
+                        precondition<double, float>(C, mg, (const double*)v64.r, (double*)v64.z, v32, nullptr, R);
+                    else
+                        precondition<double, double>(C, mg, (const double*)v64.r, (double*)v64.z, v64, nullptr, R);
+                }, 3);
+                tune_log.push_back({-R, 0, 0, ms / R});
+                if (ms / R < best) { best = ms / R; rhs_tile = R; }
+            }
+        }
         TOPO_HIP(hipEventDestroy(e0));
         TOPO_HIP(hipEventDestroy(e1));
-        rhs_tile = bR;
-        mg.xm_nt = bNT;
-        mg.xm_cx = bCX;
         C.graph_clear();
         tuned = true;
     }
+    int tuned_nt[2] = {0, 0}, tuned_cx[2] = {0, 0};
     struct TuneEntry {
-        int R, nt, cx;
-        double ms_per_rhs;
+        int what;          // >= 0: tile timing of that MG level; < 0: -R, batch of R right-hand sides
+        int nt, cx;
+        double ms;         // per launch (tiles) or per right-hand side (batches)
     };
     std::vector<TuneEntry> tune_log;
 
@@ -2260,15 +2294,15 @@ int topo_solver_tuning(topo_solver* h, int32_t* rhs_tile, int32_t* xm_nt, int32_
     TOPO_CHECK(h, TOPO_E_ARG, "null solver");
     Solver& S = h->s;
     if (rhs_tile) *rhs_tile = S.batch_cap > 1 ? S.rhs_tile : 1;
-    if (xm_nt) *xm_nt = S.mg.xm_nt;
-    if (xm_cx) *xm_cx = S.mg.xm_cx;
+    if (xm_nt) *xm_nt = S.mg.xm_nt ? S.mg.xm_nt : S.tuned_nt[0];
+    if (xm_cx) *xm_cx = S.mg.xm_cx ? S.mg.xm_cx : S.tuned_cx[0];
     const int n = (int)S.tune_log.size();
     if (log)
         for (int i = 0; i < n && i < max_entries; i++) {
-            log[4 * i] = S.tune_log[i].R;
+            log[4 * i] = S.tune_log[i].what;
             log[4 * i + 1] = S.tune_log[i].nt;
             log[4 * i + 2] = S.tune_log[i].cx;
-            log[4 * i + 3] = S.tune_log[i].ms_per_rhs;
+            log[4 * i + 3] = S.tune_log[i].ms;
         }
     return n;
     TOPO_API_END
