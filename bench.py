@@ -48,7 +48,7 @@ def make_problem(T, P, workload, grid, prec, levels, **kw):
     return cfg, fixed, F, pres, void, x0
 
 
-def run_solver(T, P, grid, prec, steps, warmup, levels, plain=False, workload="c3", coarse_fp32=1, **kw):
+def run_solver(T, P, grid, prec, steps, warmup, levels, plain=False, workload="c3", coarse_fp32=2, **kw):
     """plain: the reference's plain method on the same problem -- Jacobi-PCG
     in FP64, with the 400-iteration cap lifted so it reaches the same
     tolerance (the compliance is then comparable)."""
@@ -93,8 +93,9 @@ def main():
     ap.add_argument("--workload", choices=("c3", "c4", "c5"), default="c3",
                     help="c3: 192x96x96 cantilever (headline); c4: 128x64x64 bridge, 4 load cases, batched; "
                          "c5: 160x80x80 force inverter, forward + adjoint solves")
-    ap.add_argument("--coarse-fp32", type=int, default=1,
-                    help="1: the FP32 V-cycle applies an FP32 copy of the FP64 coarse inverse")
+    ap.add_argument("--coarse-fp32", type=int, default=2, choices=(0, 1, 2),
+                    help="copy of the FP64 coarse factor inverse read by the FP32 V-cycle: 0 none (FP64), "
+                         "1 FP32, 2 FP16 with one scale per row")
     ap.add_argument("--mixed", choices=("pc", "ir"), default="pc",
                     help="pc: FP64 MG-PCG with an FP32 V-cycle preconditioner; ir: FP64 iterative refinement "
                          "around an FP32 inner MG-PCG")

@@ -109,8 +109,10 @@ typedef struct topo_config {
     double  mg_omega_cap;       /* > 0: per level omega_eff = min(mg_omega, cap / rho),
                                    rho ~ lambda_max(D^-1 A) by 20 power iterations from
                                    a hashed vector on free DOFs; 0 = off (reference)  */
-    int32_t coarse_fp32;        /* 1: FP32 V-cycles apply an FP32 copy of the coarsest
-                                   level's FP64 inverse (half the bytes); 0: FP64  */
+    int32_t coarse_fp32;        /* rounded copy of the coarsest level's FP64 factor
+                                   inverse read by FP32 V-cycles: 0 none (FP64),
+                                   1 FP32 (half the bytes), 2 FP16 with one scale per
+                                   row (a quarter; the coarse operator stays SPD)  */
     int32_t reserved[7];
 } topo_config;
 

@@ -526,6 +526,201 @@ __global__ void k_cg_check(CGState* st, int nr) {
 }
 
 // ---------------------------------------------------------------------------
+// Fence-free CG iteration. A producer kernel leaves one partial sum per
+// block (no atomics, no fences); each kernel that needs the grid sum adds
+// the partials itself, every block in the same fixed order, so all blocks
+// compute the same bits, and k_cgp_finish keeps the books. The kernel
+// boundary makes the partials visible. This replaces the last-block
+// pattern in the iteration loops: there each block's device-scope release
+// (an L2 write-back per block) serialises, ~20 ns per block.
+// Quantities: 0 = p.Ap, 1 = ||r||^2, 2 = r.z; row (k, q) of the partials
+// holds quantity k of right-hand side q for each of the kCgBlocks blocks.
+constexpr int kCgBlocks = 1024;
+constexpr int kCgRows = 4;   // right-hand sides per quantity
+__device__ __forceinline__ int64_t cg_row(int k, int q) { return (int64_t)(k * kCgRows + q) * kCgBlocks; }
+
+// v[i] -> row (k0 + i / nr, i % nr), this block's column
+template <int NV>
+__device__ __forceinline__ void put_partials(double (&v)[NV], double* part, int k0, int nr) {
+    __shared__ double sh[NV * (kRedThreads / 64)];
+    block_reduce<NV>(v, sh);
+    if (threadIdx.x == 0)
+        for (int i = 0; i < NV; i++) part[cg_row(k0 + i / nr, i % nr) + blockIdx.x] = v[i];
+}
+// tot[i] = sum of row (k0 + i / nr, i % nr) over kCgBlocks columns, in a
+// fixed order (thread t takes columns t, t + 256, ...; then the block tree);
+// every thread of the block gets the result
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double* part, int k0, int nr, double (&tot)[NV]) {
+    __shared__ double sh[NV * (kRedThreads / 64)];
+    __shared__ double res[NV];
+    double w[NV];
+#pragma unroll
+    for (int i = 0; i < NV; i++) {
+        const double* row = part + cg_row(k0 + i / nr, i % nr);
+        double s = 0;
+        for (int b = threadIdx.x; b < kCgBlocks; b += kRedThreads) s += row[b];
+        w[i] = s;
+    }
+    block_reduce<NV>(w, sh);
+    if (threadIdx.x == 0)
+        for (int i = 0; i < NV; i++) res[i] = w[i];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NV; i++) tot[i] = res[i];
+}
+
+// p.Ap partials (quantity 0)
+template <typename T, int NR>
+__global__ void __launch_bounds__(kRedThreads) k_cgp_pAp(const T* __restrict__ p, const T* __restrict__ Ap, int64_t n,
+                                                        const CGState* st, double* part) {
+    if (all_done(st, NR)) return;
+    double v[NR];
+#pragma unroll
+    for (int q = 0; q < NR; q++) {
+        double a = 0.0;
+        if (!st[q].done) {
+            const T* pq = p + q * n;
+            const T* aq = Ap + q * n;
+            for (int64_t i = gtid(); i < n; i += gstride()) a += (double)pq[i] * (double)aq[i];
+        }
+        v[q] = a;
+    }
+    put_partials<NR>(v, part, 0, NR);
+}
+
+// alpha = rz / (p.Ap + guard); U += alpha p; r -= alpha Ap (ebeMGPCG
+// :1158-1166); r_lo (may be null) receives r rounded to the V-cycle's
+// precision; ||r||^2 partials (quantity 1)
+template <typename T, typename TU, int NR, typename TL>
+__global__ void __launch_bounds__(kRedThreads) k_cgp_update_ur(TU* __restrict__ U, T* __restrict__ r,
+                                                              const T* __restrict__ p, const T* __restrict__ Ap,
+                                                              int64_t n, const CGState* st, double* part,
+                                                              TL* __restrict__ r_lo) {
+    if (all_done(st, NR)) return;
+    double pAp[NR];
+    sum_partials<NR>(part, 0, NR, pAp);
+    double v[NR];
+#pragma unroll
+    for (int q = 0; q < NR; q++) {
+        double a = 0.0;
+        if (!st[q].done) {
+            const double alpha = st[q].rz / (pAp[q] + st[q].guard);
+            TU* Uq = U + q * n;
+            T* rq = r + q * n;
+            const T* pq = p + q * n;
+            const T* aq = Ap + q * n;
+            TL* lq = r_lo ? r_lo + q * n : nullptr;
+            for (int64_t i = gtid(); i < n; i += gstride()) {
+                Uq[i] = (TU)((double)Uq[i] + alpha * (double)pq[i]);
+                const T ri = (T)((double)rq[i] - alpha * (double)aq[i]);
+                rq[i] = ri;
+                if (lq) lq[i] = (TL)ri;
+                a += (double)ri * (double)ri;
+            }
+        }
+        v[q] = a;
+    }
+    put_partials<NR>(v, part, 1, NR);
+}
+
+// Jacobi variant (ebePCG :701-719, one right-hand side): alpha from the
+// p.Ap partials; U += alpha p; r -= alpha Ap; z = invD r; partials of
+// ||r||^2 (quantity 1) and r.z (quantity 2)
+template <typename T, typename TU>
+__global__ void __launch_bounds__(kRedThreads) k_cgp_update_jacobi(TU* __restrict__ U, T* __restrict__ r,
+                                                                  T* __restrict__ z, const T* __restrict__ p,
+                                                                  const T* __restrict__ Ap,
+                                                                  const T* __restrict__ invD, int64_t n,
+                                                                  const CGState* st, double* part) {
+    if (st->done) return;
+    double pAp[1];
+    sum_partials<1>(part, 0, 1, pAp);
+    const double alpha = st->rz / (pAp[0] + st->guard);
+    double v[2] = {0.0, 0.0};
+    for (int64_t i = gtid(); i < n; i += gstride()) {
+        const T ri = (T)((double)r[i] - alpha * (double)Ap[i]);
+        const T zi = invD[i] * ri;
+        U[i] = (TU)((double)U[i] + alpha * (double)p[i]);
+        r[i] = ri;
+        z[i] = zi;
+        v[0] += (double)ri * (double)ri;
+        v[1] += (double)ri * (double)zi;
+    }
+    put_partials<2>(v, part, 1, 1);
+}
+
+// r.z partials (quantity 2) with z = (MASK && fixed ? 0 : zz), zz the
+// preconditioner's output (precision TZ). MASK = false when zz is zero at
+// the fixed DOFs by construction (the V-cycle: invD = 0 there, sweeps keep x,
+// prolongation adds to free DOFs only; Jacobi z = invD r).
+template <typename T, typename TZ, int NR, bool MASK>
+__global__ void __launch_bounds__(kRedThreads) k_cgp_rz(const T* __restrict__ r, const TZ* __restrict__ zz,
+                                                       const uint8_t* __restrict__ fixed, int64_t n,
+                                                       const CGState* st, double* part) {
+    if (all_done(st, NR)) return;
+    double v[NR];
+#pragma unroll
+    for (int q = 0; q < NR; q++) {
+        double a = 0.0;
+        if (!st[q].done) {
+            const T* rq = r + q * n;
+            const TZ* zq = zz + q * n;
+            for (int64_t i = gtid(); i < n; i += gstride()) {
+                const double zi = (MASK && fixed[i]) ? 0.0 : (double)zq[i];
+                a += (double)rq[i] * zi;
+            }
+        }
+        v[q] = a;
+    }
+    put_partials<NR>(v, part, 2, NR);
+}
+
+// beta = rz_new / (rz + guard) from the r.z partials; p = z + beta p with
+// z = (MASK && fixed ? 0 : zz)
+template <typename T, typename TZ, int NR, bool MASK>
+__global__ void __launch_bounds__(kRedThreads) k_cgp_pupdate(T* __restrict__ p, const TZ* __restrict__ zz,
+                                                            const uint8_t* __restrict__ fixed, int64_t n,
+                                                            const CGState* st, const double* part) {
+    if (all_done(st, NR)) return;
+    double rz[NR];
+    sum_partials<NR>(part, 2, NR, rz);
+#pragma unroll
+    for (int q = 0; q < NR; q++) {
+        if (st[q].done) continue;
+        const T beta = (T)(rz[q] / (st[q].rz + st[q].guard));
+        T* pq = p + q * n;
+        const TZ* zq = zz + q * n;
+        for (int64_t i = gtid(); i < n; i += gstride())
+            pq[i] = ((MASK && fixed[i]) ? T(0) : (T)zq[i]) + beta * pq[i];
+    }
+}
+
+// the books of one iteration (one block of kRedThreads): alpha, ||r||^2,
+// rz, beta, iteration count, non-finite flag, convergence
+template <int NR>
+__global__ void __launch_bounds__(kRedThreads) k_cgp_finish(CGState* st, const double* part) {
+    if (all_done(st, NR)) return;
+    double tot[3 * NR];
+    sum_partials<3 * NR>(part, 0, NR, tot);
+    if (threadIdx.x != 0) return;
+    for (int q = 0; q < NR; q++) {
+        CGState& s = st[q];
+        if (s.done) continue;
+        const double pAp = tot[q], rn = tot[NR + q], rz = tot[2 * NR + q];
+        s.pAp = pAp;
+        s.alpha = s.rz / (pAp + s.guard);
+        s.rn = rn;
+        s.rz_new = rz;
+        s.beta = rz / (s.rz + s.guard);
+        s.rz = rz;
+        s.iters += 1;
+        if (!isfinite(s.alpha) || !isfinite(rn) || !isfinite(rz)) s.nonfinite = 1;
+        s.done = (rn < s.tolSq) || (s.iters >= s.maxit) || s.nonfinite;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // vector helpers
 template <typename T>
 __global__ void k_fill(T* __restrict__ v, T val, int64_t n, const int* gate) {
@@ -631,9 +826,26 @@ __device__ __forceinline__ void pw1d(int f, int nC, int& c0, int& c1, double& w0
     w1 = t;
 }
 
+// Contributors of coarse index c along one axis under pw1d: fine 2c-1
+// (weight 1/2, when c > 0), 2c (1) and 2c+1 (1/2, or 1 at the last coarse
+// index; absent past the end of the fine axis). Absent slots get weight 0
+// and an in-range index, so the gather below has a fixed trip count.
+template <typename T>
+__device__ __forceinline__ void rw1d(int c, int nF, int nC, int (&f)[3], T (&w)[3]) {
+    f[0] = c > 0 ? 2 * c - 1 : 0;
+    w[0] = c > 0 ? T(0.5) : T(0);
+    f[1] = 2 * c;
+    w[1] = T(1);
+    const bool has = 2 * c + 1 < nF;
+    f[2] = has ? 2 * c + 1 : 2 * c;
+    w[2] = has ? (c < nC - 1 ? T(0.5) : T(1)) : T(0);
+}
+
 // bC = P^T res, gathered per coarse node; contributions are added in the
 // reference's visiting order (fine fz, fy, fx ascending; one corner each,
 // mgVcycle assembly/index.ts:868-961), then fixed coarse DOFs zeroed (:964-967).
+// The weights are products of 1/2 and 1, exact in T; a zero-weight slot adds
+// an exact zero, so the sums are those of the contributors alone.
 template <typename T>
 __global__ void k_restrict(const T* __restrict__ res, T* __restrict__ bC, Grid gf, Grid gc,
                            const uint8_t* __restrict__ fixedC, const int* gate) {
@@ -645,39 +857,25 @@ __global__ void k_restrict(const T* __restrict__ res, T* __restrict__ bC, Grid g
     const int nxF = gf.nnx(), nyF = gf.nny(), nzF = gf.nnz();
     const int nxC = gc.nnx(), nyC = gc.nny(), nzC = gc.nnz();
     for (int64_t nc = gtid(); nc < gc.nnodes(); nc += gstride()) {
-        const int cz = (int)(nc % nzC);
-        const int64_t t = nc / nzC;
-        const int cy = (int)(t % nyC), cx = (int)(t / nyC);
-        // per-axis contributing fine indices (ascending) and weights
-        int fl[3][6];
-        double wl[3][6];
-        int cnt[3] = {0, 0, 0};
-        const int cc[3] = {cx, cy, cz};
-        const int nF[3] = {nxF, nyF, nzF};
-        const int nC[3] = {nxC, nyC, nzC};
-        for (int ax = 0; ax < 3; ax++) {
-            const int lo = max(0, 2 * cc[ax] - 1), hi = min(nF[ax] - 1, 2 * cc[ax] + 4);
-            for (int f = lo; f <= hi; f++) {
-                int c0, c1;
-                double w0, w1;
-                pw1d(f, nC[ax], c0, c1, w0, w1);
-                double w = 0;
-                if (c0 == cc[ax] && w0 > 0) w = w0;
-                else if (c1 == cc[ax] && w1 > 0) w = w1;
-                if (w > 0 && cnt[ax] < 6) {
-                    fl[ax][cnt[ax]] = f;
-                    wl[ax][cnt[ax]] = w;
-                    cnt[ax]++;
-                }
-            }
-        }
+        // 32-bit index arithmetic (node counts below 2^32)
+        const uint32_t u = (uint32_t)nc, t = u / (uint32_t)nzC, cx = t / (uint32_t)nyC;
+        const int cz = (int)(u - t * (uint32_t)nzC), cy = (int)(t - cx * (uint32_t)nyC);
+        int fx[3], fy[3], fz[3];
+        T wx[3], wy[3], wz[3];
+        rw1d<T>((int)cx, nxF, nxC, fx, wx);
+        rw1d<T>(cy, nyF, nyC, fy, wy);
+        rw1d<T>(cz, nzF, nzC, fz, wz);
         T acc[3] = {T(0), T(0), T(0)};
-        for (int iz = 0; iz < cnt[2]; iz++)
-            for (int iy = 0; iy < cnt[1]; iy++)
-                for (int ix = 0; ix < cnt[0]; ix++) {
-                    const double w = (wl[0][ix] * wl[1][iy]) * wl[2][iz];
-                    const int64_t nf = ((int64_t)fl[0][ix] * nyF + fl[1][iy]) * nzF + fl[2][iz];
-                    for (int d = 0; d < 3; d++) acc[d] += (T)w * res[3 * nf + d];
+#pragma unroll
+        for (int iz = 0; iz < 3; iz++)
+#pragma unroll
+            for (int iy = 0; iy < 3; iy++)
+#pragma unroll
+                for (int ix = 0; ix < 3; ix++) {
+                    const T w = (wx[ix] * wy[iy]) * wz[iz];
+                    const int64_t nf = ((int64_t)fx[ix] * nyF + fy[iy]) * nzF + fz[iz];
+#pragma unroll
+                    for (int d = 0; d < 3; d++) acc[d] += w * res[3 * nf + d];
                 }
         for (int d = 0; d < 3; d++) bC[3 * nc + d] = fixedC[3 * nc + d] ? T(0) : acc[d];
     }
@@ -696,28 +894,30 @@ __global__ void k_prolong_add(T* __restrict__ x, const T* __restrict__ xC, Grid 
     const int nyF = gf.nny(), nzF = gf.nnz();
     const int nyC = gc.nny(), nzC = gc.nnz(), nxC = gc.nnx();
     for (int64_t nf = gtid(); nf < gf.nnodes(); nf += gstride()) {
-        const int fz = (int)(nf % nzF);
-        const int64_t t = nf / nzF;
-        const int fy = (int)(t % nyF), fx = (int)(t / nyF);
+        // 32-bit index arithmetic (node counts below 2^32)
+        const uint32_t u = (uint32_t)nf, t = u / (uint32_t)nzF, ux = t / (uint32_t)nyF;
+        const int fz = (int)(u - t * (uint32_t)nzF), fy = (int)(t - ux * (uint32_t)nyF), fx = (int)ux;
         int cx[2], cy[2], cz[2];
         double wx[2], wy[2], wz[2];
         pw1d(fx, nxC, cx[0], cx[1], wx[0], wx[1]);
         pw1d(fy, nyC, cy[0], cy[1], wy[0], wy[1]);
         pw1d(fz, nzC, cz[0], cz[1], wz[0], wz[1]);
+        // corner weights (products of 1/2 and 1: exact in T) and node indices
+        T w[8];
+        int64_t nc[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int dx = k & 1, dy = (k >> 1) & 1, dz = k >> 2;
+            w[k] = (T)((wx[dx] * wy[dy]) * wz[dz]);
+            nc[k] = 3 * (((int64_t)cx[dx] * nyC + cy[dy]) * nzC + cz[dz]);
+        }
+#pragma unroll
         for (int d = 0; d < 3; d++) {
             const int64_t i = 3 * nf + d;
             if (fixedF[i]) continue;
-            T v = T(0);
-            bool first = true;
-            for (int dz = 0; dz < 2; dz++)
-                for (int dy = 0; dy < 2; dy++)
-                    for (int dx = 0; dx < 2; dx++) {
-                        const double w = (wx[dx] * wy[dy]) * wz[dz];
-                        const int64_t nc = ((int64_t)cx[dx] * nyC + cy[dy]) * nzC + cz[dz];
-                        const T term = xC[3 * nc + d] * (T)w;
-                        v = first ? term : v + term;
-                        first = false;
-                    }
+            T v = xC[nc[0] + d] * w[0];
+#pragma unroll
+            for (int k = 1; k < 8; k++) v = v + xC[nc[k] + d] * w[k];
             x[i] = x[i] + v;
         }
     }

@@ -171,17 +171,39 @@ __global__ void __launch_bounds__(256) k_transpose(const TI* __restrict__ in, TO
     }
 }
 
+// FP16 copy of a lower-triangular Z with one scale per row: Zq[i][j] =
+// Z[i][j] / s_i, s_i = max_j |Z[i][j]| (1 for a zero row), s2[i] = s_i^2.
+// Z^T (Z b) = Zq^T (S^2 (Zq b)) with the transposed copy of the same Zq: the
+// operator applied stays symmetric positive definite whatever the rounding.
+// One wavefront per row.
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+__global__ void __launch_bounds__(256) k_rowscale_f16(const double* __restrict__ Z, int64_t n,
+                                                      _Float16* __restrict__ Zq, double* __restrict__ s2) {
+    const int lane = threadIdx.x & 63;
+    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (i >= n) return;
+    const double* row = Z + i * n;
+    double m = 0.0;
+    for (int64_t j = lane; j <= i; j += 64) m = fmax(m, fabs(row[j]));
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    const double sc = m > 0.0 ? m : 1.0;
+    const double inv = 1.0 / sc;
+    for (int64_t j = lane; j < n; j += 64) Zq[i * n + j] = (_Float16)(float)(j <= i ? row[j] * inv : 0.0);
+    if (lane == 0) s2[i] = sc * sc;
+}
+
 // Triangular matrix-vector product for NR right-hand sides: rows of M (n x n,
-// row-major, TK = double or float) against bf (NR x n, FP64); lower: row i
-// uses columns [0, i], upper: [i, n) (the other part of M is zero, so whole
-// 16-byte vectors around the diagonal can be read). One wave per 4 rows.
-// scatter = 0: y[q n + i] = result (FP64); scatter = 1: x[q ndof + free[i]]
-// = result in T, for i < nf (converged right-hand sides are not written).
+// row-major, TK = double, float or _Float16) against bf (NR x n, FP64);
+// lower: row i uses columns [0, i], upper: [i, n) (the other part of M is
+// zero, so whole 16-byte vectors around the diagonal can be read). One wave
+// per 4 rows. scatter = 0: y[q n + i] = result (FP64), times ys[i] when ys
+// is given; scatter = 1: x[q ndof + free[i]] = result in T, for i < nf
+// (converged right-hand sides are not written).
 template <typename T, int NR, typename TK>
 __global__ void __launch_bounds__(256) k_tri_apply(const TK* __restrict__ M, int64_t n, int upper,
                                                    const double* __restrict__ bf, double* __restrict__ y,
                                                    T* __restrict__ x, const int32_t* __restrict__ freeIdx, int64_t nf,
-                                                   int64_t ndof, const int* gate) {
+                                                   int64_t ndof, const int* gate, const double* __restrict__ ys = nullptr) {
     bool live[NR];
     bool any = false;
 #pragma unroll
@@ -198,7 +220,7 @@ __global__ void __launch_bounds__(256) k_tri_apply(const TK* __restrict__ M, int
     for (int r = 0; r < 4; r++)
 #pragma unroll
         for (int q = 0; q < NR; q++) acc[r][q] = 0.0;
-    constexpr int VN = std::is_same<TK, double>::value ? 2 : 4;
+    constexpr int VN = 16 / (int)sizeof(TK);
     // column range in vectors of VN entries
     const int64_t v0 = upper ? row0 / VN : 0;
     const int64_t v1 = upper ? n / VN : (row0 + 4 + VN - 1) / VN;
@@ -215,6 +237,15 @@ __global__ void __launch_bounds__(256) k_tri_apply(const TK* __restrict__ M, int
                 const double2 a = ((const double2*)(M + (row0 + r) * n))[j];
 #pragma unroll
                 for (int q = 0; q < NR; q++) acc[r][q] += a.x * b[q][0] + a.y * b[q][1];
+            } else if constexpr (VN == 8) {
+                const half8_t a = ((const half8_t*)(M + (row0 + r) * n))[j];
+#pragma unroll
+                for (int q = 0; q < NR; q++) {
+                    double t = 0.0;
+#pragma unroll
+                    for (int u = 0; u < 8; u++) t += (double)(float)a[u] * b[q][u];
+                    acc[r][q] += t;
+                }
             } else {
                 const float4 a = ((const float4*)(M + (row0 + r) * n))[j];
 #pragma unroll
@@ -235,7 +266,7 @@ __global__ void __launch_bounds__(256) k_tri_apply(const TK* __restrict__ M, int
             if (!live[q]) continue;
             for (int r = 0; r < 4; r++) {
                 const int64_t i = row0 + r;
-                if (!x) y[q * n + i] = acc[r][q];
+                if (!x) y[q * n + i] = ys ? ys[i] * acc[r][q] : acc[r][q];
                 else if (i < nf) x[q * ndof + freeIdx[i]] = (T)acc[r][q];
             }
         }

@@ -116,12 +116,13 @@ __host__ __device__ inline size_t xm_lds_bytes(int nnz, size_t elt) {
     return ((size_t)(NT + nnz + 1) * 3 + (size_t)NT * 9) * elt;
 }
 
-// mode as k_applyA_exact: 0 Ap, 1 masked, 2 b - Ap (free) / 0 (fixed),
-// 3 Jacobi sweep x + omega invD (b - Ap) (free) / x (fixed)
-template <typename T, typename TE, int NT>
+// MODE as k_applyA_exact: 0 Ap, 1 masked, 2 b - Ap (free) / 0 (fixed),
+// 3 Jacobi sweep x + omega invD (b - Ap) (free) / x (fixed); a template
+// parameter so each variant holds only the registers its epilogue needs
+template <typename T, typename TE, int NT, int MODE>
 __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __restrict__ out,
                                                   const TE* __restrict__ E, double thr, Grid g,
-                                                  const uint8_t* __restrict__ fixed, int mode,
+                                                  const uint8_t* __restrict__ fixed,
                                                   const T* __restrict__ bvec, const T* __restrict__ invD,
                                                   const double* __restrict__ omp, const int* gate, HadCoef<T> co,
                                                   int CX, int gx, int gy) {
@@ -183,10 +184,15 @@ __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __
             }
         }
     };
+    // own node values of the last staged plane (MODE 3 reads x at the
+    // output node from here instead of from memory)
+    T stg[3] = {T(0), T(0), T(0)};
     // stage the prefetched plane in LDS, prefetch plane ixn, and return this
     // thread's face transform of the staged plane
     auto stage_face = [&](int ixn, T (&f)[12]) {
         T own[3] = {pf[0], pf[1], pf[2]};
+#pragma unroll
+        for (int d = 0; d < 3; d++) stg[d] = own[d];
 #pragma unroll
         for (int d = 0; d < 3; d++) s_in[3 * t + d] = own[d];
         if (t < H) {
@@ -211,6 +217,21 @@ __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __
     fetch(X0 - 1);
     stage_face(X0, fprev);
     for (int ex = X0 - 1; ex < X1; ex++) {
+        // operands of this step's output epilogue (node plane ex), loaded
+        // before the element work so their latency overlaps it
+        const bool outstep = is_out && ex >= X0;
+        const int64_t obase = (int64_t)ex * plane + 3 * (int64_t)j;
+        T eb[3] = {T(0), T(0), T(0)}, ed[3] = {T(0), T(0), T(0)};
+        const T ep[3] = {stg[0], stg[1], stg[2]};
+        uint8_t ef[3] = {0, 0, 0};
+        if (outstep && MODE >= 1) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                ef[d] = fixed[obase + d];
+                if (MODE >= 2) eb[d] = bvec[obase + d];
+                if (MODE == 3) ed[d] = invD[obase + d];
+            }
+        }
         T A[12], Bq[12];
         {
             T fnew[12];
@@ -257,22 +278,20 @@ __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __
         __syncthreads();
         // node (iy, iz) = face(iy,iz).q00 + face(iy,iz-1).q01 + face(iy-1,iz).q10
         //               + face(iy-1,iz-1).q11 (faces beyond the grid carry zeros)
-        if (is_out && ex >= X0) {
-            const int64_t base = (int64_t)ex * plane + 3 * (int64_t)j;
+        if (outstep) {
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 T v = q00[d] + s_q[9 * (t - 1) + d];
                 v = v + (s_q[9 * (t - nnz) + 3 + d] + s_q[9 * (t - nnz - 1) + 6 + d]);
-                const int64_t i = base + d;
-                if (mode == 1) {
-                    if (fixed[i]) v = T(0);
-                } else if (mode == 2) {
-                    v = fixed[i] ? T(0) : bvec[i] - v;
-                } else if (mode == 3) {
-                    const T r = bvec[i] - v;
-                    v = fixed[i] ? p[i] : p[i] + omega * invD[i] * r;
+                if (MODE == 1) {
+                    if (ef[d]) v = T(0);
+                } else if (MODE == 2) {
+                    v = ef[d] ? T(0) : eb[d] - v;
+                } else if (MODE == 3) {
+                    const T r = eb[d] - v;
+                    v = ef[d] ? ep[d] : ep[d] + omega * ed[d] * r;
                 }
-                out[i] = v;
+                out[obase + d] = v;
             }
         }
     }

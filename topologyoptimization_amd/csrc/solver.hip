@@ -46,7 +46,7 @@ constexpr int kMaxRHS = 4;   // right-hand sides one batched CG launch carries
 struct Ctx {
     hipStream_t s = nullptr;
     int device = 0;
-    double* partials = nullptr;   // 32 * kRedBlocks
+    double* partials = nullptr;   // reduction partials (last-block and fence-free kernels)
     unsigned* ticket = nullptr;
     CGState* st = nullptr;        // device CG state
     CGState* h_st = nullptr;      // pinned mirror
@@ -103,7 +103,7 @@ struct Ctx {
         }
         TOPO_HIP(hipGetDevice(&device));
         TOPO_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-        partials = alloc<double>(32 * kRedBlocks);
+        partials = alloc<double>(std::max(32 * kRedBlocks, 3 * kCgRows * kCgBlocks));
         ticket = alloc<unsigned>(4);
         st = alloc<CGState>(kMaxRHS);
         oc = alloc<OCState>(1);
@@ -287,6 +287,9 @@ struct MGLevel {
     int bandw = 0;                 // max |i - j| of the free block's nonzeros
     double* Z = nullptr;           // L^-1 (lower triangular)
     float* Z32 = nullptr;
+    _Float16* Z16 = nullptr;       // Z / s_i per row (coarse_fp32 = 2)
+    _Float16* Zt16 = nullptr;      // its transpose
+    double* zs2 = nullptr;         // s_i^2
     double* Linv = nullptr;        // inverses of the diagonal blocks of L
     double* ytmp = nullptr;        // Z b (nrhs x nfp)
     double* d0c = nullptr;         // original diagonal (pivot test)
@@ -679,7 +682,9 @@ struct MGH {
     DevKe* K0 = nullptr;
     Springs springs{};    // mechanism ports on the finest level
     int nrhs = 1;         // right-hand sides a batched V-cycle can carry (level vectors are nrhs x ndof)
-    bool coarse_fp32 = false;   // FP32 V-cycles apply an FP32 copy of the coarsest inverse
+    // FP32 V-cycles apply a rounded copy of the coarsest level's factor
+    // inverse: 0 none (FP64), 1 FP32, 2 FP16 with one scale per row
+    int coarse_fp32 = 0;
     template <typename T> const T* ke() const;
     template <typename T> HadCoef<T> had() const;
 
@@ -687,6 +692,9 @@ struct MGH {
     // (js/optimizer-worker-3d.js:408-442); nlevels <= 0 = reference rule
     void build(Ctx& c, Grid g0, const uint8_t* dfixed0, int nlevels, bool fp32) {
         want32 = fp32;
+        // the transfer kernels index nodes in 32 bits
+        TOPO_CHECK(g0.nnodes() < ((int64_t)1 << 32), TOPO_E_ARG, "grid of %lld nodes too large",
+                   (long long)g0.nnodes());
         L.clear();
         MGLevel l0;
         l0.g = g0;
@@ -907,7 +915,12 @@ struct MGH {
             if (!l.Z) {
                 l.Z = c.alloc<double>(n * n);
                 l.Linv = c.alloc<double>((int64_t)NB * kCB * kCB);
-                if (want32 && coarse_fp32) l.Z32 = c.alloc<float>(n * n);
+                if (want32 && coarse_fp32 == 1) l.Z32 = c.alloc<float>(n * n);
+                if (want32 && coarse_fp32 == 2) {
+                    l.Z16 = c.alloc<_Float16>(n * n);
+                    l.Zt16 = c.alloc<_Float16>(n * n);
+                    l.zs2 = c.alloc<double>(n);
+                }
             }
             int* dflag = (int*)(c.dsc + 14);   // device scratch word
             hipLaunchKernelGGL(k_diag_copy, dim3(nblk(n)), dim3(256), 0, c.s, (const double*)l.Kinv, n, l.d0c);
@@ -935,6 +948,12 @@ struct MGH {
                 zrows(c, l, NB, wb);
                 hipLaunchKernelGGL((k_transpose<double, double>), dim3(NB, NB), dim3(256), 0, c.s,
                                    (const double*)l.Z, l.Kinv, n);
+                if (l.Z16) {
+                    hipLaunchKernelGGL(k_rowscale_f16, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, c.s,
+                                       (const double*)l.Z, n, l.Z16, l.zs2);
+                    hipLaunchKernelGGL((k_transpose<_Float16, _Float16>), dim3(NB, NB), dim3(256), 0, c.s,
+                                       (const _Float16*)l.Z16, l.Zt16, n);
+                }
                 if (l.Z32) {
                     hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(n * n)), dim3(256), 0, c.s, l.Z32,
                                        (const double*)l.Z, n * n);
@@ -1050,17 +1069,27 @@ struct MGH {
                 int CX = cx_sel > 0 ? cx_sel : (int)((int64_t)l.g.nnx() * gx * (NT / 512) / 1024);
                 CX = std::max(2, std::min(32, CX));
                 const int gy = (l.g.nnx() + CX - 1) / CX;
+                const dim3 grid(8 * ((gx * gy + 7) / 8), nr);
+                const size_t lds = NT == 512 ? xm_lds_bytes<512>(l.g.nnz(), sizeof(T))
+                                             : xm_lds_bytes<1024>(l.g.nnz(), sizeof(T));
+#define TOPO_XM_LAUNCH(NTV, MV)                                                                                  \
+    hipLaunchKernelGGL((k_applyA_xm<T, TE, NTV, MV>), grid, dim3(NTV), lds, c.s, in, out,                      \
+                       (const TE*)Sel<T>::E(l), thr, l.g, (const uint8_t*)l.fixed, b, (const T*)Sel<T>::invD(l), \
+                       om, gate, had<T>(), CX, gx, gy)
+#define TOPO_XM_MODES(NTV)                                 \
+    switch (mode) {                                        \
+    case 0: TOPO_XM_LAUNCH(NTV, 0); break;                 \
+    case 1: TOPO_XM_LAUNCH(NTV, 1); break;                 \
+    case 2: TOPO_XM_LAUNCH(NTV, 2); break;                 \
+    default: TOPO_XM_LAUNCH(NTV, 3); break;                \
+    }
                 if (NT == 512) {
-                    const size_t lds = xm_lds_bytes<512>(l.g.nnz(), sizeof(T));
-                    hipLaunchKernelGGL((k_applyA_xm<T, TE, 512>), dim3(8 * ((gx * gy + 7) / 8), nr), dim3(512), lds, c.s,
-                                       in, out, (const TE*)Sel<T>::E(l), thr, l.g, (const uint8_t*)l.fixed, mode, b,
-                                       (const T*)Sel<T>::invD(l), om, gate, had<T>(), CX, gx, gy);
+                    TOPO_XM_MODES(512)
                 } else {
-                    const size_t lds = xm_lds_bytes<1024>(l.g.nnz(), sizeof(T));
-                    hipLaunchKernelGGL((k_applyA_xm<T, TE, 1024>), dim3(8 * ((gx * gy + 7) / 8), nr), dim3(1024), lds, c.s,
-                                       in, out, (const TE*)Sel<T>::E(l), thr, l.g, (const uint8_t*)l.fixed, mode, b,
-                                       (const T*)Sel<T>::invD(l), om, gate, had<T>(), CX, gx, gy);
+                    TOPO_XM_MODES(1024)
                 }
+#undef TOPO_XM_MODES
+#undef TOPO_XM_LAUNCH
             } else {
                 hipLaunchKernelGGL((k_applyA_exact<T, TE>), dim3(nblk(l.g.nnodes()), nr), dim3(256), 0, c.s, in, out,
                                    (const TE*)Sel<T>::E(l), thr, l.g, l.fixed, mode, b, Sel<T>::invD(l), om, gate,
@@ -1089,9 +1118,19 @@ struct MGH {
             if (l.tri) {
                 // x = Z^T (Z b): lower-triangular Z into ytmp, then the upper
                 // triangular Z^T scattered to the free DOFs; the FP32 V-cycle
-                // reads the FP32 roundings (coarse_fp32)
+                // reads the rounded copies (coarse_fp32: FP32, or FP16 with
+                // the row scales applied to ytmp)
                 const dim3 gt((unsigned)(l.nfp / 16));
-                if (f32) {
+                if (std::is_same<T, float>::value && l.Z16) {
+                    TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_apply<T, NR, _Float16>), gt, dim3(256), 0, c.s,
+                                                          (const _Float16*)l.Z16, l.nfp, 0, (const double*)l.bf,
+                                                          l.ytmp, (T*)nullptr, l.freeIdx, l.nf, l.ndof, gate,
+                                                          (const double*)l.zs2));
+                    TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_apply<T, NR, _Float16>), gt, dim3(256), 0, c.s,
+                                                          (const _Float16*)l.Zt16, l.nfp, 1, (const double*)l.ytmp,
+                                                          (double*)nullptr, x, l.freeIdx, l.nf, l.ndof, gate,
+                                                          (const double*)nullptr));
+                } else if (f32 && l.Z32) {
                     TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_apply<T, NR, float>), gt, dim3(256), 0, c.s,
                                                           (const float*)l.Z32, l.nfp, 0, (const double*)l.bf, l.ytmp,
                                                           (T*)nullptr, l.freeIdx, l.nf, l.ndof, gate));
@@ -1218,11 +1257,13 @@ static CGResult pcg_jacobi(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVe
     auto body = [&]() {
         for (int k = 0; k < check; k++) {
             mg.apply<T>(c, 0, p, Ap, 1, nullptr, gate);
-            hipLaunchKernelGGL(k_cg_alpha<T>, dim3(kRedBlocks), dim3(kRedThreads), 0, c.s, (const T*)p, (const T*)Ap, n,
-                               c.st, c.partials, c.ticket);
-            hipLaunchKernelGGL((k_cg_update_jacobi<T, TU>), dim3(kRedBlocks), dim3(kRedThreads), 0, c.s, U, r, z,
-                               (const T*)p, (const T*)Ap, (const T*)Sel<T>::invD(l), n, c.st, c.partials, c.ticket);
-            hipLaunchKernelGGL(k_cg_pupdate<T>, dim3(nblk(n)), dim3(256), 0, c.s, p, (const T*)z, n, c.st);
+            hipLaunchKernelGGL((k_cgp_pAp<T, 1>), dim3(kCgBlocks), dim3(kRedThreads), 0, c.s, (const T*)p,
+                               (const T*)Ap, n, c.st, c.partials);
+            hipLaunchKernelGGL((k_cgp_update_jacobi<T, TU>), dim3(kCgBlocks), dim3(kRedThreads), 0, c.s, U, r, z,
+                               (const T*)p, (const T*)Ap, (const T*)Sel<T>::invD(l), n, c.st, c.partials);
+            hipLaunchKernelGGL((k_cgp_pupdate<T, T, 1, false>), dim3(kCgBlocks), dim3(kRedThreads), 0, c.s, p, (const T*)z,
+                               (const uint8_t*)nullptr, n, c.st, (const double*)c.partials);
+            hipLaunchKernelGGL(k_cgp_finish<1>, dim3(1), dim3(kRedThreads), 0, c.s, c.st, (const double*)c.partials);
         }
     };
     hipGraphExec_t ex = nullptr;
@@ -1261,6 +1302,24 @@ static T* precondition(Ctx& c, MGH& mg, const T* r, T* z, CGVecs& v32, const int
         PT* zz = mg.vcycle<PT>(c, 0, r32, gate, nr);
         hipLaunchKernelGGL((k_convert_gated<T, PT>), dim3(nblk(n), nr), dim3(256), 0, c.s, z, (const PT*)zz, n, gate);
         return z;
+    }
+}
+
+// The V-cycle part of precondition() without the widening copy: returns the
+// V-cycle's own output buffer (precision PT, fixed DOFs not yet zeroed). With
+// r_lo_ready the FP32 residual has already been written (k_cg_update_ur).
+template <typename T, typename PT>
+static const PT* precondition_raw(Ctx& c, MGH& mg, const T* r, CGVecs& v32, const int* gate, int nr,
+                                  bool r_lo_ready) {
+    const int64_t n = mg.L[0].ndof;
+    if constexpr (std::is_same<T, PT>::value) {
+        (void)n; (void)v32; (void)r_lo_ready;
+        return mg.vcycle<T>(c, 0, r, gate, nr);
+    } else {
+        PT* r32 = (PT*)v32.r;
+        if (!r_lo_ready)
+            hipLaunchKernelGGL((k_convert_gated<PT, T>), dim3(nblk(n), nr), dim3(256), 0, c.s, r32, r, n, gate);
+        return mg.vcycle<PT>(c, 0, r32, gate, nr);
     }
 }
 
@@ -1307,18 +1366,28 @@ static CGResult pcg_mg(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVecs& 
     const int check = 4;
     auto body = [&]() {
         for (int k = 0; k < check; k++) {
+            // fence-free iteration: each reduction's partials are summed by
+            // the kernel that needs them, k_cgp_finish keeps the books
             mg.apply<T>(c, 0, p, Ap, 1, nullptr, gate, nr);
-            TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_cg_alpha<T, NR>), dim3(kRedBlocks), dim3(kRedThreads), 0, c.s,
-                                                   (const T*)p, (const T*)Ap, n, c.st, c.partials, c.ticket));
-            TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_cg_update_ur<T, TU, NR>), dim3(kRedBlocks), dim3(kRedThreads), 0,
-                                                   c.s, U, r, (const T*)p, (const T*)Ap, n, c.st, c.partials,
-                                                   c.ticket));
-            precondition<T, PT>(c, mg, r, z, w32, gate, nr);
-            hipLaunchKernelGGL(k_mask<T>, gv, dim3(256), 0, c.s, z, l.fixed, n);
-            TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_cg_rz<T, NR>), dim3(kRedBlocks), dim3(kRedThreads), 0, c.s,
-                                                   (const T*)r, (const T*)z, n, c.st, c.partials, c.ticket));
-            hipLaunchKernelGGL(k_cg_pupdate<T>, gv, dim3(256), 0, c.s, p, (const T*)z, n, c.st);
-            hipLaunchKernelGGL(k_cg_check, dim3(1), dim3(1), 0, c.s, c.st, nr);
+            TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_cgp_pAp<T, NR>), dim3(kCgBlocks), dim3(kRedThreads), 0, c.s,
+                                                   (const T*)p, (const T*)Ap, n, c.st, c.partials));
+            // r_lo: the V-cycle's FP32 residual, written by the update itself
+            PT* r_lo = std::is_same<T, PT>::value ? nullptr : (PT*)w32.r;
+            TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_cgp_update_ur<T, TU, NR, PT>), dim3(kCgBlocks),
+                                                   dim3(kRedThreads), 0, c.s, U, r, (const T*)p, (const T*)Ap, n,
+                                                   c.st, c.partials, r_lo));
+            // z is never formed: r.z and p = z + beta p read the V-cycle's
+            // output, which is zero at the fixed DOFs (the mask of ebeMGPCG
+            // :1170-1172 is a no-op on it)
+            const PT* zz = precondition_raw<T, PT>(c, mg, r, w32, gate, nr, true);
+            TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_cgp_rz<T, PT, NR, false>), dim3(kCgBlocks), dim3(kRedThreads), 0,
+                                                   c.s, (const T*)r, zz, (const uint8_t*)l.fixed, n, c.st,
+                                                   c.partials));
+            TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_cgp_pupdate<T, PT, NR, false>), dim3(kCgBlocks), dim3(kRedThreads), 0,
+                                                   c.s, p, zz, (const uint8_t*)l.fixed, n, c.st,
+                                                   (const double*)c.partials));
+            TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_cgp_finish<NR>), dim3(1), dim3(kRedThreads), 0, c.s, c.st,
+                                                   (const double*)c.partials));
         }
     };
     hipGraphExec_t ex = nullptr;
@@ -1520,7 +1589,7 @@ struct Solver {
         if (const char* e = getenv("TOPO_MIXED_ETA")) mixed_eta = atof(e);
         const int nlev = cfg.linear_solver == TOPO_SOLVER_MGPCG ? cfg.mg_levels : 1;
         mg.nrhs = batch_cap;
-        mg.coarse_fp32 = cfg.coarse_fp32 != 0;
+        mg.coarse_fp32 = cfg.coarse_fp32;
         mg.build(C, g, fixed, cfg.linear_solver == TOPO_SOLVER_MGPCG ? nlev : 1, mixed);
         mg.L[0].E = E;
         mg.L[0].E32 = E32;

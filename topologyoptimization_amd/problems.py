@@ -128,9 +128,10 @@ def bridge(nelx, nely, nelz, **kw):
 def bridge_c4(**kw):
     """The 128x64x64 bridge (1,635,075 DOF, 4 load cases), batched MG-PCG
     (4 levels, FP64 Cholesky coarsest, omega 0.6 capped), FP32 V-cycle inside
-    FP64 CG, autotuned batch and element-tile shapes."""
+    FP64 CG reading an FP16 row-scaled copy of the coarse factor inverse,
+    autotuned batch and element-tile shapes."""
     args = dict(mg_levels=4, mg_omega=0.6, coarse_solver=COARSE_CHOLESKY, galerkin_max_ndof=0,
-                precision=PREC_MIXED_PC, rhs_tile=0)
+                precision=PREC_MIXED_PC, rhs_tile=0, coarse_fp32=2)
     args.update(kw)
     return bridge(128, 64, 64, **args)
 
