@@ -51,7 +51,9 @@ enum { TOPO_LOAD_DOWN = 0, TOPO_LOAD_UP = 1, TOPO_LOAD_LEFT = 2,
        TOPO_LOAD_RIGHT = 3, TOPO_LOAD_TOP_CENTER = 4, TOPO_LOAD_NONE = 5 };
 
 enum { TOPO_SOLVER_PCG = 0,      /* Jacobi-PCG, ebePCG semantics              */
-       TOPO_SOLVER_MGPCG = 1 };  /* geometric MG-PCG, ebeMGPCG semantics      */
+       TOPO_SOLVER_MGPCG = 1,    /* geometric MG-PCG, ebeMGPCG semantics      */
+       TOPO_SOLVER_KSP_BDDC = 2 };/* 'petsc': PCG with the domain-decomposition
+                                    preconditioner, ebeKSP_BDDC semantics (FP64) */
 enum { TOPO_PREC_FP64 = 0,       /* everything in FP64                        */
        TOPO_PREC_MIXED = 1,      /* FP32 operator/V-cycle inside FP64 refinement */
        TOPO_PREC_MIXED_PC = 2 }; /* FP64 MG-PCG, FP32 V-cycle preconditioner (MG-PCG only;
@@ -201,6 +203,26 @@ typedef struct topo_mg_level {
 int topo_ebeMGPCG(const topo_mg_level* levels, int32_t numLevels, const double* ke,
                   const double* F, double* U, const uint8_t* fixedMask0, int32_t ndof0,
                   int32_t maxIter, double tolerance, double* cgWork);
+
+/* Subdomain descriptor of ebeKSP_BDDC (BDDC_SUB_STRIDE record,
+ * assembly/index.ts:1204-1222): the 32-byte record with its offsets widened
+ * to host pointers. localInvDiag has ndof entries. */
+typedef struct topo_bddc_sub {
+    int32_t numElements, numDofs;
+    const int32_t* elements;   const int32_t* dofs;   const double* localInvDiag;
+} topo_bddc_sub;
+
+/* replaces ebeKSP_BDDC assembly/index.ts:1410-1517 (the 'petsc' solver's
+ * PCG with the domain-decomposition preconditioner bddcApply :1272-1371:
+ * averaged subdomain-local Jacobi, smootherIters - 1 global Jacobi sweeps,
+ * Jacobi correction on the coarse (corner) DOFs); cold start. Returns the CG
+ * iteration count (maxIter if not converged). A DOF may belong to at most 8
+ * subdomains. `work` is ignored. */
+int topo_ebeKSP_BDDC(const topo_bddc_sub* subs, int32_t numSubs, const int32_t* coarse,
+                     int32_t numCoarse, const double* ke, const int32_t* edofs, const double* evals,
+                     const int32_t* active, int32_t activeCount, const uint8_t* fixedMask,
+                     const double* invDiag, const double* F, double* U, int32_t ndof,
+                     int32_t maxIter, double tolerance, int32_t smootherIters, double* work);
 
 /* ---- device-resident SIMP solver (the fast path) --------------------- */
 typedef struct topo_solver topo_solver;

@@ -56,6 +56,17 @@ def lib():
         _lib.or_simp_run.argtypes = [C.c_void_p] * 5 + [C.c_int] + [C.c_void_p] * 5
         _lib.or_helmholtz.argtypes = [C.c_int] * 3 + [C.c_double, C.c_void_p, C.c_void_p]
         _lib.or_fb.argtypes = [C.c_void_p] * 4 + [C.c_int] + [C.c_void_p] * 6
+        _lib.or_bddc_create.restype = C.c_void_p
+        _lib.or_bddc_create.argtypes = [C.c_int] * 3
+        _lib.or_bddc_destroy.argtypes = [C.c_void_p]
+        _lib.or_bddc_info.argtypes = [C.c_void_p, C.c_void_p]
+        _lib.or_bddc_update.argtypes = [C.c_void_p] * 4
+        _lib.or_bddc_sub.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 4
+        _lib.or_bddc_coarse.argtypes = [C.c_void_p, C.c_void_p]
+        _lib.or_bddc_precond.argtypes = [C.c_void_p] * 4 + [C.c_int] + [C.c_void_p] * 4 + [C.c_int]
+        _lib.or_bddc_solve.argtypes = [C.c_void_p] * 4 + [C.c_int] + [C.c_void_p] * 4 + [C.c_int, C.c_double,
+                                                                                            C.c_int]
+        _lib.or_bddc_fe.argtypes = [C.c_void_p] * 3 + [C.c_double] * 4 + [C.c_void_p] * 3 + [C.c_int, C.c_double]
     return _lib
 
 
@@ -202,6 +213,70 @@ class MG:
         F = np.ascontiguousarray(F, np.float64)
         U = np.zeros(self.ndof) if U is None else np.ascontiguousarray(U, np.float64).copy()
         it = lib().or_mgpcg(self.h, _p(F), _p(U), maxIter, tol, int(warm))
+        return U, it
+
+
+class Bddc:
+    """KSPSolver partition + bddcApply / ebeKSP_BDDC restatement (see
+    topo_oracle.cpp)."""
+
+    def __init__(self, nelx, nely, nelz):
+        self.dims = (nelx, nely, nelz)
+        self.nel, self.ndof = dims(nelx, nely, nelz)
+        self.h = lib().or_bddc_create(nelx, nely, nelz)
+        info = np.zeros(5, np.int32)
+        lib().or_bddc_info(self.h, _p(info))
+        self.nsubs, self.ns, self.ncoarse = int(info[0]), tuple(int(v) for v in info[1:4]), int(info[4])
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_bddc_destroy(self.h)
+            self.h = None
+
+    def coarse(self):
+        c = np.zeros(self.ncoarse, np.int32)
+        lib().or_bddc_coarse(self.h, _p(c))
+        return c
+
+    def update(self, ke, evals, fixed):
+        self._ke = np.ascontiguousarray(ke, np.float64).ravel()
+        self._E = np.ascontiguousarray(evals, np.float64)
+        self._fixed = np.ascontiguousarray(fixed, np.uint8)
+        lib().or_bddc_update(self.h, _p(self._ke), _p(self._E), _p(self._fixed))
+
+    def sub(self, si):
+        """(elements, dofs, local inverse diagonal[ndof]) of subdomain si."""
+        nn = np.zeros(2, np.int32)
+        lib().or_bddc_sub(self.h, si, _p(nn), None, None, None)
+        el = np.zeros(nn[0], np.int32)
+        ds = np.zeros(nn[1], np.int32)
+        li = np.zeros(self.ndof)
+        lib().or_bddc_sub(self.h, si, _p(nn), _p(el), _p(ds), _p(li))
+        return el, ds, li
+
+    def precond(self, active, invD, r, sweeps=3):
+        active = np.ascontiguousarray(active, np.int32)
+        z = np.zeros(self.ndof)
+        lib().or_bddc_precond(self.h, _p(self._ke), _p(self._E), _p(active), len(active), _p(self._fixed),
+                              _p(np.ascontiguousarray(invD, np.float64)), _p(np.ascontiguousarray(r, np.float64)),
+                              _p(z), sweeps)
+        return z
+
+    def solve(self, active, invD, F, maxIter, tol, sweeps=3):
+        active = np.ascontiguousarray(active, np.int32)
+        U = np.zeros(self.ndof)
+        it = lib().or_bddc_solve(self.h, _p(self._ke), _p(self._E), _p(active), len(active), _p(self._fixed),
+                                 _p(np.ascontiguousarray(invD, np.float64)), _p(np.ascontiguousarray(F, np.float64)),
+                                 _p(U), maxIter, tol, sweeps)
+        return U, it
+
+    def fe(self, ke, xphys, penal, fixed, F, maxIter, tol, E0=1.0, Emin=1e-9, skip=1000.0):
+        """_solveWithWasmKSP_BDDC: moduli by Math.pow, active above skip*Emin."""
+        self._ke = np.ascontiguousarray(ke, np.float64).ravel()
+        self._fixed = np.ascontiguousarray(fixed, np.uint8)
+        U = np.zeros(self.ndof)
+        it = lib().or_bddc_fe(self.h, _p(self._ke), _p(np.ascontiguousarray(xphys, np.float32)), penal, E0, Emin,
+                              skip, _p(self._fixed), _p(np.ascontiguousarray(F, np.float64)), _p(U), maxIter, tol)
         return U, it
 
 

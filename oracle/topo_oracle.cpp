@@ -23,6 +23,9 @@
 //   or_density_filter brute force + central finite differences of compliance
 //   or_oc            volume constraint, brute-force lambda scan
 //   or_heaviside     closed form and its derivative
+//   or_bddc_*        partition counts and corner sets (hand-computed known
+//                    answers), local step closed form on a uniform grid,
+//                    KSP solve vs dense solve
 // Functions of the build-only features (Cholesky coarse solve, Helmholtz
 // filter, mechanism objective, multi-load) have no reference counterpart;
 // they are pinned by the invariants named at each.
@@ -970,12 +973,20 @@ int or_oc(int64_t nel, const float* x, double volfrac, const float* dc, const ui
     return passes;
 }
 
+// KSP-CG with the domain-decomposition preconditioner (defined below)
+void* or_bddc_create(int nelx, int nely, int nelz);
+void or_bddc_destroy(void* h);
+static int bddc_fe(void* h, const double* ke, const float* xPhys, double penal, double E0, double Emin, double skip,
+                   const uint8_t* fixed, const double* F, double* U, int maxIter, double tol);
+
+
 // ---------------------------------------------------------------------------
 // SIMP loop: optimize() js/optimizer-worker-3d.js:1923-2352 restricted to the
 // hot path (no AMR/manufacturing/UI). Runs `niter` iterations (or stops at
 // change <= 0.01 like :2074). Outputs per-iteration compliance and the final
 // design. Solver per cfg->linear_solver: PCG = ebePCG with warm start
-// (:3572-3629); MGPCG = ebeMGPCG cold start (:3642-3807) with the MG options.
+// (:3572-3629); MGPCG = ebeMGPCG cold start (:3642-3807) with the MG options;
+// KSP_BDDC = ebeKSP_BDDC cold start (:3814-3999).
 int or_simp_run(const topo_config* cfg, const uint8_t* fixed, const float* F32, const uint8_t* pres,
                 const uint8_t* voidm, int niter, float* x_io, double* comp_out, double* change_out,
                 int32_t* cg_out, double* U_out) {
@@ -991,6 +1002,7 @@ int or_simp_run(const topo_config* cfg, const uint8_t* fixed, const float* F32, 
     std::vector<float> x(x_io, x_io + nel), xnew(nel), xold(nel), xPhys(nel), w(nel), dc(nel), dcn(nel), xf(nel);
     dvec U(ndof, 0.0);
     void* mg = nullptr;
+    void* bddc = cfg->linear_solver == TOPO_SOLVER_KSP_BDDC ? or_bddc_create(nelx, nely, nelz) : nullptr;
     if (cfg->linear_solver == TOPO_SOLVER_MGPCG)
     {
         mg = or_mg_create(nelx, nely, nelz, fixed, ke.data(), cfg->mg_levels, cfg->mg_omega, cfg->mg_nu1, cfg->mg_nu2,
@@ -1032,6 +1044,10 @@ int or_simp_run(const topo_config* cfg, const uint8_t* fixed, const float* F32, 
             int maxI = std::min((int)freed.size(), cfg->cg_max_iter);
             its = or_ebePCG(dens.data(), ke.data(), nelx, nely, nelz, F.data(), U.data(), freed.data(), (int)freed.size(),
                             cfg->Emin, cfg->E0, penal, maxI, tol);
+        } else if (bddc) {
+            int maxI = std::min((int)freed.size(), cfg->cg_max_iter);
+            its = bddc_fe(bddc, ke.data(), xPhys.data(), penal, cfg->E0, cfg->Emin, cfg->skip_factor, fixed, F.data(),
+                          U.data(), maxI, tol);
         } else {
             or_mg_update(mg, xPhys.data(), penal, cfg->E0, cfg->Emin);
             int maxI = std::min((int)freed.size(), cfg->cg_max_iter);
@@ -1068,7 +1084,238 @@ int or_simp_run(const topo_config* cfg, const uint8_t* fixed, const float* F32, 
     std::copy(x.begin(), x.end(), x_io);
     if (U_out) std::copy(U.begin(), U.end(), U_out);
     if (mg) or_mg_destroy(mg);
+    if (bddc) or_bddc_destroy(bddc);
     return loop;
+}
+
+// ---------------------------------------------------------------------------
+// KSP-CG with the domain-decomposition preconditioner of the 'petsc' solver
+// (additive-Schwarz local Jacobi, two global Jacobi sweeps, a Jacobi
+// correction on the subdomain-corner DOFs). Partition and local diagonals:
+// KSPSolver._buildSubdomains / _updateBDDC js/optimizer-worker-3d.js:1155-1262
+// (with the E_vals of _solveWithWasmKSP_BDDC :3824-3845); preconditioner
+// bddcApply assembly/index.ts:1272-1371; solve ebeKSP_BDDC :1410-1517.
+struct OrBddc {
+    int nelx, nely, nelz, ns[3];
+    int64_t ndof;
+    std::vector<std::vector<int32_t>> elements, dofs;
+    std::vector<dvec> linv;          // per subdomain, full ndof (0 off its DOFs)
+    std::vector<int32_t> coarse;     // sorted corner DOFs
+};
+
+// subdomains per axis (:1162-1166)
+static void bddc_counts(int nelx, int nely, int nelz, int ns[3]) {
+    const double tot = (double)nelx * nely * nelz;
+    const double target = std::max(8.0, std::min(64.0, std::ceil(std::cbrt(tot / 100))));
+    auto one = [&](int n, int a, int b) {
+        const double v = std::round(std::cbrt(target * ((double)n * n) / ((double)a * b)));
+        return (int)std::max(2.0, std::min((double)n, v));
+    };
+    ns[0] = one(nelx, nely, nelz);
+    ns[1] = one(nely, nelx, nelz);
+    ns[2] = one(nelz, nelx, nely);
+}
+
+void* or_bddc_create(int nelx, int nely, int nelz) {
+    OrBddc* b = new OrBddc();
+    b->nelx = nelx; b->nely = nely; b->nelz = nelz;
+    b->ndof = 3LL * (nelx + 1) * (nely + 1) * (nelz + 1);
+    bddc_counts(nelx, nely, nelz, b->ns);
+    const int nsx = b->ns[0], nsy = b->ns[1], nsz = b->ns[2];
+    const int nny = nely + 1, nnz = nelz + 1;
+    std::set<int32_t> cset;
+    for (int sz = 0; sz < nsz; sz++) {
+        const int z0 = (int)std::floor((double)sz * nelz / nsz), z1 = (int)std::floor((double)(sz + 1) * nelz / nsz);
+        for (int sy = 0; sy < nsy; sy++) {
+            const int y0 = (int)std::floor((double)sy * nely / nsy), y1 = (int)std::floor((double)(sy + 1) * nely / nsy);
+            for (int sx = 0; sx < nsx; sx++) {
+                const int x0 = (int)std::floor((double)sx * nelx / nsx), x1 = (int)std::floor((double)(sx + 1) * nelx / nsx);
+                std::vector<int32_t> el;
+                for (int ez = z0; ez < z1; ez++)
+                    for (int ey = y0; ey < y1; ey++)
+                        for (int ex = x0; ex < x1; ex++) el.push_back(ex + ey * nelx + ez * nelx * nely);
+                if (el.empty()) continue;
+                b->elements.push_back(el);
+                // corner nodes -> primal DOFs (:1236-1254)
+                const int cx[2] = {x0, x1}, cy[2] = {y0, y1}, cz[2] = {z0, z1};
+                for (int k = 0; k < 8; k++) {
+                    const int X = std::min(cx[k & 1], nelx), Y = std::min(cy[(k >> 1) & 1], nely),
+                              Z = std::min(cz[k >> 2], nelz);
+                    const int64_t n = node_id(X, Y, Z, nny, nnz);
+                    for (int d = 0; d < 3; d++) cset.insert((int32_t)(3 * n + d));
+                }
+            }
+        }
+    }
+    b->coarse.assign(cset.begin(), cset.end());
+    // DOFs touched by each subdomain, ascending (:1240-1248)
+    std::vector<int32_t> edof((int64_t)nelx * nely * nelz * 24);
+    or_edofs(nelx, nely, nelz, edof.data());
+    for (auto& el : b->elements) {
+        std::set<int32_t> ds;
+        for (int32_t e : el)
+            for (int i = 0; i < 24; i++) ds.insert(edof[(int64_t)e * 24 + i]);
+        b->dofs.emplace_back(ds.begin(), ds.end());
+    }
+    b->linv.assign(b->elements.size(), dvec());
+    return b;
+}
+void or_bddc_destroy(void* h) { delete (OrBddc*)h; }
+
+// nsubs, ns[3], ncoarse
+int or_bddc_info(void* h, int32_t* out5) {
+    OrBddc* b = (OrBddc*)h;
+    out5[0] = (int32_t)b->elements.size();
+    out5[1] = b->ns[0]; out5[2] = b->ns[1]; out5[3] = b->ns[2];
+    out5[4] = (int32_t)b->coarse.size();
+    return 0;
+}
+
+// local inverse diagonals from the element moduli (_updateBDDC :1256-1278):
+// per subdomain, its elements ascending, E >= 1e-30, K0 diagonal; 1/diag
+// on free DOFs of the subdomain with diag > 1e-30
+int or_bddc_update(void* h, const double* ke, const double* evals, const uint8_t* fixed) {
+    OrBddc* b = (OrBddc*)h;
+    std::vector<int32_t> edof((int64_t)b->nelx * b->nely * b->nelz * 24);
+    or_edofs(b->nelx, b->nely, b->nelz, edof.data());
+    for (size_t s = 0; s < b->elements.size(); s++) {
+        dvec ld(b->ndof, 0.0);
+        for (int32_t e : b->elements[s]) {
+            const double E = evals[e];
+            if (E < 1e-30) continue;
+            for (int i = 0; i < 24; i++) ld[edof[(int64_t)e * 24 + i]] += E * ke[i * 24 + i];
+        }
+        dvec& li = b->linv[s];
+        li.assign(b->ndof, 0.0);
+        for (int32_t d : b->dofs[s])
+            if (!fixed[d] && ld[d] > 1e-30) li[d] = 1.0 / ld[d];
+    }
+    return 0;
+}
+
+// subdomain si: element and DOF counts, and (when the pointers are given)
+// its elements, DOFs and local inverse diagonal (ndof)
+int or_bddc_sub(void* h, int si, int32_t* ne_nd, int32_t* elements, int32_t* dofs, double* linv) {
+    OrBddc* b = (OrBddc*)h;
+    ne_nd[0] = (int32_t)b->elements[si].size();
+    ne_nd[1] = (int32_t)b->dofs[si].size();
+    if (elements) std::copy(b->elements[si].begin(), b->elements[si].end(), elements);
+    if (dofs) std::copy(b->dofs[si].begin(), b->dofs[si].end(), dofs);
+    if (linv) std::copy(b->linv[si].begin(), b->linv[si].end(), linv);
+    return 0;
+}
+int or_bddc_coarse(void* h, int32_t* coarse) {
+    OrBddc* b = (OrBddc*)h;
+    std::copy(b->coarse.begin(), b->coarse.end(), coarse);
+    return 0;
+}
+
+// bddcApply assembly/index.ts:1272-1371 (z = M^-1 r); applyA over the active
+// elements (bddcApplyA :1227-1262 = applyAEbe3D)
+static void bddc_apply(const OrBddc* b, const double* ke, const double* evals, const int32_t* active, int nactive,
+                       const uint8_t* fixed, const double* invD, const double* r, double* z, int sweeps) {
+    const int64_t n = b->ndof;
+    dvec w(n, 0.0), Az(n), cc(n, 0.0);
+    for (int64_t i = 0; i < n; i++) z[i] = 0.0;
+    for (size_t s = 0; s < b->elements.size(); s++)
+        for (int32_t d : b->dofs[s]) {
+            z[d] += b->linv[s][d] * r[d];
+            w[d] += 1.0;
+        }
+    for (int64_t i = 0; i < n; i++)
+        if (w[i] > 1e-12) z[i] /= w[i];
+    for (int it = 1; it < sweeps; it++) {
+        or_applyA(b->nelx, b->nely, b->nelz, ke, evals, active, nactive, z, Az.data());
+        for (int64_t i = 0; i < n; i++) z[i] += invD[i] * (r[i] - Az[i]);
+    }
+    if (!b->coarse.empty()) {
+        or_applyA(b->nelx, b->nely, b->nelz, ke, evals, active, nactive, z, Az.data());
+        for (int32_t d : b->coarse) cc[d] = invD[d] * (r[d] - Az[d]);
+    }
+    for (int64_t i = 0; i < n; i++) z[i] += cc[i];
+    for (int64_t i = 0; i < n; i++)
+        if (fixed[i]) z[i] = 0.0;
+}
+
+int or_bddc_precond(void* h, const double* ke, const double* evals, const int32_t* active, int nactive,
+                    const uint8_t* fixed, const double* invD, const double* r, double* z, int sweeps) {
+    bddc_apply((OrBddc*)h, ke, evals, active, nactive, fixed, invD, r, z, sweeps);
+    return 0;
+}
+
+// ebeKSP_BDDC assembly/index.ts:1410-1517: cold start, r = masked F, PCG
+// with the +1e-12 guards; returns the iteration count (maxIter if not
+// converged)
+int or_bddc_solve(void* h, const double* ke, const double* evals, const int32_t* active, int nactive,
+                  const uint8_t* fixed, const double* invD, const double* F, double* U, int maxIter, double tol,
+                  int sweeps) {
+    const OrBddc* b = (const OrBddc*)h;
+    const int64_t n = b->ndof;
+    const double EPS = 1e-12;
+    dvec r(n), z(n), p(n), Ap(n);
+    for (int64_t i = 0; i < n; i++) {
+        U[i] = 0.0;
+        r[i] = fixed[i] ? 0.0 : F[i];
+    }
+    bddc_apply(b, ke, evals, active, nactive, fixed, invD, r.data(), z.data(), sweeps);
+    p = z;
+    double rz = 0, r0 = 0;
+    for (int64_t i = 0; i < n; i++) {
+        rz += r[i] * z[i];
+        r0 += r[i] * r[i];
+    }
+    const double tolSq = tol * tol * (r0 > 1e-30 ? r0 : 1e-30);
+    for (int it = 0; it < maxIter; it++) {
+        double rn = 0;
+        for (int64_t i = 0; i < n; i++) rn += r[i] * r[i];
+        if (rn < tolSq) return it;
+        or_applyA(b->nelx, b->nely, b->nelz, ke, evals, active, nactive, p.data(), Ap.data());
+        for (int64_t i = 0; i < n; i++)
+            if (fixed[i]) Ap[i] = 0.0;
+        double pAp = 0;
+        for (int64_t i = 0; i < n; i++) pAp += p[i] * Ap[i];
+        const double alpha = rz / (pAp + EPS);
+        for (int64_t i = 0; i < n; i++) {
+            U[i] += alpha * p[i];
+            r[i] -= alpha * Ap[i];
+        }
+        bddc_apply(b, ke, evals, active, nactive, fixed, invD, r.data(), z.data(), sweeps);
+        double rzn = 0;
+        for (int64_t i = 0; i < n; i++) rzn += r[i] * z[i];
+        const double beta = rzn / (rz + EPS);
+        for (int64_t i = 0; i < n; i++) p[i] = z[i] + beta * p[i];
+        rz = rzn;
+    }
+    return maxIter;
+}
+
+// _solveWithWasmKSP_BDDC js/optimizer-worker-3d.js:3814-3999 around the
+// above: E by Math.pow, active above skip * Emin, global inverse diagonal of
+// the active elements; 3 smoother sweeps (KSP_BDDC_SMOOTHER_ITERS :1031)
+static int bddc_fe(void* h, const double* ke, const float* xPhys, double penal, double E0, double Emin, double skip,
+                   const uint8_t* fixed, const double* F, double* U, int maxIter, double tol) {
+    OrBddc* b = (OrBddc*)h;
+    const int64_t nel = (int64_t)b->nelx * b->nely * b->nelz;
+    dvec E(nel);
+    std::vector<int32_t> act;
+    for (int64_t e = 0; e < nel; e++) {
+        E[e] = Emin + std::pow((double)xPhys[e], penal) * (E0 - Emin);
+        if (E[e] > skip * Emin) act.push_back((int32_t)e);
+    }
+    std::vector<int32_t> edof(nel * 24);
+    or_edofs(b->nelx, b->nely, b->nelz, edof.data());
+    dvec diag(b->ndof, 0.0), invD(b->ndof, 0.0);
+    for (int32_t e : act)
+        for (int i = 0; i < 24; i++) diag[edof[(int64_t)e * 24 + i]] += E[e] * ke[i * 24 + i];
+    for (int64_t i = 0; i < b->ndof; i++)
+        if (!fixed[i] && diag[i] > 1e-30) invD[i] = 1.0 / diag[i];
+    or_bddc_update(h, ke, E.data(), fixed);
+    return or_bddc_solve(h, ke, E.data(), act.data(), (int)act.size(), fixed, invD.data(), F, U, maxIter, tol, 3);
+}
+
+int or_bddc_fe(void* h, const double* ke, const float* xPhys, double penal, double E0, double Emin, double skip,
+               const uint8_t* fixed, const double* F, double* U, int maxIter, double tol) {
+    return bddc_fe(h, ke, xPhys, penal, E0, Emin, skip, fixed, F, U, maxIter, tol);
 }
 
 // ---------------------------------------------------------------------------

@@ -1,8 +1,9 @@
 'use strict';
 // WASM-shaped facade: lays out ke / edofs / evals / active / p in the
 // facade's WebAssembly.Memory the way the reference's worker does before
-// calling the matrix-ops exports, calls applyAEbe3D and ebePCG by byte
-// offset, and prints the results (argv[2] = {n, evals, p, dens, F, free}).
+// calling the matrix-ops exports, calls applyAEbe3D, ebePCG and (when
+// argv[2].bddc is given) ebeKSP_BDDC by byte offset, and prints the results
+// (argv[2] = {n, evals, p, dens, F, free, bddc?}).
 const path = require('path');
 const NODE = path.join(__dirname, '..', '..', 'topologyoptimization_amd', 'node');
 const { createWasmFacade, addon } = require(NODE);
@@ -33,4 +34,35 @@ new Int32Array(wasm.memory.buffer, frPtr, inp.free.length).set(inp.free);
 const iters = wasm.ebePCG(dPtr, kePtr, edPtr, fPtr, uPtr, frPtr, nel, 24, ndof, inp.free.length,
     1e-9, 1.0, 3.0, 400, 1e-8, 0);
 const U = Array.from(new Float64Array(wasm.memory.buffer, uPtr, ndof));
-process.stdout.write(JSON.stringify({ Ap, iters, U, activeCount: cnt }) + '\n');
+
+// ebeKSP_BDDC with the subdomain descriptors laid out like
+// _solveWithWasmKSP_BDDC (js/optimizer-worker-3d.js:3939-3962)
+let bddc = null;
+if (inp.bddc) {
+    const B = inp.bddc;
+    const ns = B.subs.length;
+    const subsPtr = alloc(32 * ns);
+    const dv = new DataView(wasm.memory.buffer);
+    for (let si = 0; si < ns; si++) {
+        const [el, ds, li] = B.subs[si];
+        const eo = alloc(el.length * 4), d0 = alloc(ds.length * 4), lo = alloc(ndof * 8);
+        new Int32Array(wasm.memory.buffer, eo, el.length).set(el);
+        new Int32Array(wasm.memory.buffer, d0, ds.length).set(ds);
+        new Float64Array(wasm.memory.buffer, lo, ndof).set(li);
+        const o = subsPtr + 32 * si;
+        dv.setInt32(o, el.length, true); dv.setInt32(o + 4, ds.length, true);
+        dv.setUint32(o + 8, eo, true); dv.setUint32(o + 12, d0, true); dv.setUint32(o + 16, lo, true);
+    }
+    const cPtr = alloc(B.coarse.length * 4), e2 = alloc(nel * 8), a2 = alloc(B.active.length * 4);
+    const fxPtr = alloc(ndof), ivPtr = alloc(ndof * 8), f2 = alloc(ndof * 8), u2 = alloc(ndof * 8);
+    new Int32Array(wasm.memory.buffer, cPtr, B.coarse.length).set(B.coarse);
+    new Float64Array(wasm.memory.buffer, e2, nel).set(B.evals);
+    new Int32Array(wasm.memory.buffer, a2, B.active.length).set(B.active);
+    new Uint8Array(wasm.memory.buffer, fxPtr, ndof).set(B.fixed);
+    new Float64Array(wasm.memory.buffer, ivPtr, ndof).set(B.invD);
+    new Float64Array(wasm.memory.buffer, f2, ndof).set(B.F);
+    const it2 = wasm.ebeKSP_BDDC(subsPtr, ns, cPtr, B.coarse.length, kePtr, edPtr, e2, a2, B.active.length,
+        fxPtr, ivPtr, f2, u2, ndof, B.maxIter, B.tol, 3, 0);
+    bddc = { iters: it2, U: Array.from(new Float64Array(wasm.memory.buffer, u2, ndof)) };
+}
+process.stdout.write(JSON.stringify({ Ap, iters, U, activeCount: cnt, bddc }) + '\n');

@@ -232,3 +232,82 @@ def test_wasm_facade_matches_oracle(gpu, oracle):
     it = oracle.ebePCG(dens, oracle.k0(0.3), n, n, n, F, U, free, 1e-9, 1.0, 3.0, 400, 1e-8)
     assert abs(out["iters"] - it) <= 1
     assert np.abs(np.array(out["U"]) - U).max() <= 1e-6 * np.abs(U).max()
+
+
+@pytest.mark.gpu
+def test_wasm_facade_ksp_bddc(gpu, oracle):
+    """ebeKSP_BDDC called by byte offset on the facade's memory with the
+    32-byte subdomain descriptors, against the oracle (30 iterations, random
+    densities: the range over which the iterates are comparable)."""
+    n = 6
+    nel, ndof = T.dims(n, n, n)
+    rng = np.random.default_rng(22)
+    x = rng.uniform(0.0, 1.0, nel)
+    K0 = oracle.k0(0.3)
+    E = 1e-9 + x ** 3 * (1 - 1e-9)
+    act = np.flatnonzero(E > 1e-6).astype(np.int32)
+    fixed = oracle.fixed_dofs(n, n, n, 0)
+    F = oracle.load_vector(n, n, n, 0, -1.0).astype(np.float64)
+    ed = oracle.edofs(n, n, n)
+    diag = np.zeros(ndof)
+    for e in act:
+        np.add.at(diag, ed[e], E[e] * np.diag(K0))
+    invD = np.where((fixed == 0) & (diag > 1e-30), 1.0 / np.where(diag > 0, diag, 1.0), 0.0)
+    b = oracle.Bddc(n, n, n)
+    b.update(K0, E, fixed)
+    subs = [tuple(a.tolist() for a in b.sub(si)) for si in range(b.nsubs)]
+    evals = np.ones(nel)
+    out = run_js("facade_apply.js", {"n": n, "evals": evals.tolist(), "p": [0.0] * ndof, "dens": [1.0] * nel,
+                                     "F": F.tolist(), "free": np.flatnonzero(fixed == 0).tolist(),
+                                     "bddc": {"subs": subs, "coarse": b.coarse().tolist(), "evals": E.tolist(),
+                                              "active": act.tolist(), "fixed": fixed.tolist(), "invD": invD.tolist(),
+                                              "F": F.tolist(), "maxIter": 30, "tol": 1e-8}})
+    Uo, ito = b.solve(act, invD, F, 30, 1e-8, 3)
+    assert out["bddc"]["iters"] == ito
+    assert np.abs(np.array(out["bddc"]["U"]) - Uo).max() <= 1e-10 * np.abs(Uo).max()
+
+
+@pytest.mark.gpu
+def test_topology_solver_petsc_matches_oracle(gpu, oracle):
+    """solver 'petsc' (petscPC 'bddc'): the KSP/BDDC solve behind
+    TopologySolver on the 5^3 cube, 30 CG iterations per solve, against the
+    oracle's SIMP loop for the first two iterations (with unconverged solves
+    later iterations are sensitive to rounding)."""
+    n = 5
+    arg = _c1_js_config(n)
+    arg["config"].update({"solver": "petsc", "maxIterations": 2, "cgMaxIter": 30})
+    out = run_js("run_optimize.js", arg)
+    assert out["ok"], out
+    cfg, fixed, F, pres, void, x0 = P.cube_reference(n, linear_solver=T.SOLVER_KSP_BDDC, cg_max_iter=30,
+                                                     max_iterations=2)
+    ref = oracle.simp_run(cfg, fixed, F, pres, void, 2, x0)
+    comp = np.array(out["compliance"])
+    assert len(comp) == 2
+    rel = np.abs(comp - ref["compliance"]) / np.abs(ref["compliance"])
+    assert rel.max() <= 1e-8, rel
+
+
+@pytest.mark.gpu
+def test_topology_solver_volumetric_snapshots(gpu):
+    """volumetricOutputMode (js/optimizer-worker-3d.js:1573-1603, :2280-2321):
+    'every-iteration' sends the stress field E_e w_e with every progress
+    message and the final result; requestVolumetricSnapshot() answers with a
+    'volumetric' message; 'off' sends none."""
+    n = 5
+    arg = _c1_js_config(n)
+    arg["config"].update({"maxIterations": 3, "volumetricOutputMode": "every-iteration"})
+    arg["actions"] = {"1": "volumetric"}
+    out = run_js("run_optimize.js", arg)
+    assert out["ok"], out
+    vol = out["volumetric"]
+    prog = [v for v in vol if v[0] == "progress"]
+    assert [v[1] for v in prog] == [1, 2, 3]
+    for v in prog:
+        assert v[4] == n ** 3 and v[2] > 0 and abs(v[3] - v[2]) <= 1e-6 * v[2]
+    assert any(v[0] == "volumetric" and v[1] in (1, 2) for v in vol), vol
+    comp = [v for v in vol if v[0] == "complete"]
+    assert len(comp) == 1 and comp[0][1] == 3
+    arg["config"]["volumetricOutputMode"] = "off"
+    arg["actions"] = {}
+    out = run_js("run_optimize.js", arg)
+    assert out["ok"] and out["volumetric"] == [["complete", None]]

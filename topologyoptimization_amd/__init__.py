@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(_HERE, "libtopo_hip.so")
 FIX_LEFT, FIX_RIGHT, FIX_BOTTOM, FIX_TOP, FIX_BOTTOM_CORNERS, FIX_NONE = range(6)
 DOF_X, DOF_Y, DOF_Z, DOF_ALL = 1, 2, 4, 7
 LOAD_DOWN, LOAD_UP, LOAD_LEFT, LOAD_RIGHT, LOAD_TOP_CENTER, LOAD_NONE = range(6)
-SOLVER_PCG, SOLVER_MGPCG = 0, 1
+SOLVER_PCG, SOLVER_MGPCG, SOLVER_KSP_BDDC = 0, 1, 2
 PREC_FP64, PREC_MIXED, PREC_MIXED_PC = 0, 1, 2
 COARSE_SWEEPS, COARSE_CHOLESKY = 0, 1
 FILTER_SENSITIVITY, FILTER_DENSITY, FILTER_HELMHOLTZ = 0, 1, 2
@@ -82,13 +82,18 @@ class MGLevelDesc(C.Structure):
                                   "res", "b", "x", "scratch")]
 
 
+class BddcSubDesc(C.Structure):
+    _fields_ = [("numElements", C.c_int32), ("numDofs", C.c_int32), ("elements", C.c_void_p), ("dofs", C.c_void_p),
+                ("localInvDiag", C.c_void_p)]
+
+
 _lib = None
 
 # exported C symbols declared in include/topo.h
 SYMBOLS = [
     "topo_config_default", "topo_version", "topo_last_error", "topo_device_info", "topo_k0", "topo_edofs",
     "topo_fixed_dofs", "topo_load_vector", "topo_preserved_elements", "topo_load_paint", "topo_fixed_paint",
-    "topo_applyAEbe3D", "topo_ebePCG", "topo_ebeMGPCG", "topo_solver_create", "topo_solver_destroy",
+    "topo_applyAEbe3D", "topo_ebePCG", "topo_ebeMGPCG", "topo_ebeKSP_BDDC", "topo_solver_create", "topo_solver_destroy",
     "topo_solver_set_design", "topo_solver_get_design", "topo_solver_set_iteration", "topo_solver_update",
     "topo_solver_step", "topo_solver_get_field", "topo_solver_fe", "topo_solver_set_density", "topo_op_applyA",
     "topo_op_vcycle", "topo_op_energy", "topo_op_filter", "topo_op_filter_adjoint", "topo_op_oc", "topo_time_op",
@@ -119,6 +124,7 @@ def lib():
     L.topo_applyAEbe3D.argtypes = [vp, vp, vp, vp, i32, vp, vp, i32, vp]
     L.topo_ebePCG.argtypes = [vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, dbl, dbl, dbl, i32, dbl, vp]
     L.topo_ebeMGPCG.argtypes = [vp, i32, vp, vp, vp, vp, i32, i32, dbl, vp]
+    L.topo_ebeKSP_BDDC.argtypes = [vp, i32, vp, i32, vp, vp, vp, vp, i32, vp, vp, vp, vp, i32, i32, dbl, i32, vp]
     L.topo_solver_create.argtypes = [vp, vp, vp, vp, vp, vp]
     L.topo_solver_destroy.argtypes = [vp]
     L.topo_solver_set_design.argtypes = [vp, vp]
@@ -265,6 +271,33 @@ def ebeMGPCG(levels, ke, F, U, fixed0, maxIter, tol):
     F = np.ascontiguousarray(F, np.float64)
     fixed0 = np.ascontiguousarray(fixed0, np.uint8)
     return _check(lib().topo_ebeMGPCG(arr, len(levels), _p(ke), _p(F), _p(U), _p(fixed0), F.size, maxIter, tol, None))
+
+
+def ebeKSP_BDDC(subs, coarse, ke, edofs_arr, evals, active, fixed, invDiag, F, maxIter, tol, smoother_iters=3):
+    """subs: list of (elements, dofs, localInvDiag[ndof]) per subdomain;
+    returns (U, CG iterations)."""
+    keep = []
+    arr = (BddcSubDesc * len(subs))()
+    for i, (el, ds, li) in enumerate(subs):
+        el = np.ascontiguousarray(el, np.int32)
+        ds = np.ascontiguousarray(ds, np.int32)
+        li = np.ascontiguousarray(li, np.float64)
+        keep += [el, ds, li]
+        arr[i].numElements, arr[i].numDofs = el.size, ds.size
+        arr[i].elements, arr[i].dofs, arr[i].localInvDiag = el.ctypes.data, ds.ctypes.data, li.ctypes.data
+    coarse = np.ascontiguousarray(coarse, np.int32)
+    ke = np.ascontiguousarray(ke, np.float64).ravel()
+    ed = np.ascontiguousarray(edofs_arr, np.int32).ravel()
+    evals = np.ascontiguousarray(evals, np.float64)
+    active = np.ascontiguousarray(active, np.int32)
+    fixed = np.ascontiguousarray(fixed, np.uint8)
+    invDiag = np.ascontiguousarray(invDiag, np.float64)
+    F = np.ascontiguousarray(F, np.float64)
+    U = np.zeros(F.size)
+    it = _check(lib().topo_ebeKSP_BDDC(arr, len(subs), _p(coarse), coarse.size, _p(ke), _p(ed), _p(evals), _p(active),
+                                       active.size, _p(fixed), _p(invDiag), _p(F), _p(U), F.size, maxIter, tol,
+                                       smoother_iters, None))
+    return U, it
 
 
 # ---- device-resident solver ------------------------------------------------

@@ -19,6 +19,7 @@
 #include "kernels_xm.hpp"
 #include "kernels_adj.hpp"
 #include "kernels_chol.hpp"
+#include "kernels_bddc.hpp"
 #include <rocblas/rocblas.h>
 
 namespace topo {
@@ -685,6 +686,15 @@ struct MGH {
     // FP32 V-cycles apply a rounded copy of the coarsest level's factor
     // inverse: 0 none (FP64), 1 FP32, 2 FP16 with one scale per row
     int coarse_fp32 = 0;
+    // 'petsc' solver: the domain-decomposition preconditioner replaces the
+    // V-cycle (one level; omega 1 for its global Jacobi sweeps)
+    bool bddc = false;
+    int bddc_sweeps = 3;                 // KSP_BDDC_SMOOTHER_ITERS (js :1031)
+    double* bddc_val = nullptr;          // kBddcSlots x ndof local inverse diagonals
+    uint8_t* bddc_cnt = nullptr;         // subdomains containing each DOF
+    uint8_t* bddc_corner = nullptr;      // primal (corner) DOF flags
+    int32_t *bddc_sx = nullptr, *bddc_sy = nullptr, *bddc_sz = nullptr;   // element -> subdomain, per axis
+    int bddc_ns[3] = {0, 0, 0};
     template <typename T> const T* ke() const;
     template <typename T> HadCoef<T> had() const;
 
@@ -813,6 +823,7 @@ struct MGH {
             }
         upload_omega(c);
         if (coarse_solver == TOPO_COARSE_CHOLESKY) factor_coarsest(c);
+        if (bddc) bddc_build(c);
     }
 
     void upload_omega(Ctx& c) {
@@ -1047,6 +1058,76 @@ struct MGH {
             c.free_one(Rp);
         }
         c.free_one(d0);
+    }
+
+    // Box partition of the structured grid (KSPSolver._buildSubdomains
+    // js/optimizer-worker-3d.js:1155-1225): subdomain counts per axis, the
+    // element -> subdomain maps and the corner DOF flags (:1230-1254).
+    void bddc_setup(Ctx& c) {
+        MGLevel& l = L[0];
+        const Grid& g = l.g;
+        const double tot = (double)g.nelx * g.nely * g.nelz;
+        const double target = std::max(8.0, std::min(64.0, std::ceil(std::cbrt(tot / 100))));
+        auto count = [&](int n, int a, int b) {
+            const double v = std::round(std::cbrt(target * ((double)n * n) / ((double)a * b)));
+            return (int)std::max(2.0, std::min((double)n, v));
+        };
+        bddc_ns[0] = count(g.nelx, g.nely, g.nelz);
+        bddc_ns[1] = count(g.nely, g.nelx, g.nelz);
+        bddc_ns[2] = count(g.nelz, g.nelx, g.nely);
+        const int ne[3] = {g.nelx, g.nely, g.nelz};
+        int32_t** sub[3] = {&bddc_sx, &bddc_sy, &bddc_sz};
+        uint8_t* bnd[3];
+        for (int a = 0; a < 3; a++) {
+            const int n = ne[a], ns = bddc_ns[a];
+            std::vector<int32_t> hs(n);
+            std::vector<uint8_t> hb(n + 1, 0);
+            for (int k = 0; k < ns; k++) {
+                // floor(k n / ns) .. floor((k+1) n / ns) (:1172-1181)
+                const int a0 = (int)((int64_t)k * n / ns), a1 = (int)((int64_t)(k + 1) * n / ns);
+                for (int e = a0; e < a1; e++) hs[e] = k;
+                hb[a0] = 1;
+                hb[a1] = 1;
+            }
+            *sub[a] = c.alloc<int32_t>(n);
+            TOPO_HIP(hipMemcpy(*sub[a], hs.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+            bnd[a] = c.alloc<uint8_t>(n + 1);
+            TOPO_HIP(hipMemcpy(bnd[a], hb.data(), n + 1, hipMemcpyHostToDevice));
+        }
+        bddc_val = c.alloc<double>((int64_t)kBddcSlots * l.ndof);
+        bddc_cnt = c.alloc<uint8_t>(l.ndof);
+        bddc_corner = c.alloc<uint8_t>(l.ndof);
+        hipLaunchKernelGGL(k_bddc_corner_flags, dim3(nblk(g.nnodes())), dim3(256), 0, c.s, g, (const uint8_t*)bnd[0],
+                           (const uint8_t*)bnd[1], (const uint8_t*)bnd[2], bddc_corner);
+        c.sync();
+        for (int a = 0; a < 3; a++) c.free_one(bnd[a]);
+        bddc = true;
+    }
+    // local inverse diagonals of the current moduli (_updateBDDC :1231-1278)
+    void bddc_build(Ctx& c) {
+        MGLevel& l = L[0];
+        hipLaunchKernelGGL(k_bddc_build, dim3(nblk(l.g.nnodes())), dim3(256), 0, c.s, (const double*)l.E,
+                           (const uint8_t*)l.fixed, l.g, (const int32_t*)bddc_sx, (const int32_t*)bddc_sy,
+                           (const int32_t*)bddc_sz, bddc_ns[0], bddc_ns[1], (const double*)K0->ke, bddc_val,
+                           bddc_cnt);
+        c.check_launch();
+    }
+    // z = M^-1 r (bddcApply assembly/index.ts:1272-1371), FP64 on level 0;
+    // returns the buffer holding z (zero at the fixed DOFs)
+    double* bddc_apply(Ctx& c, const double* r, const int* gate) {
+        MGLevel& l = L[0];
+        double* z = l.x;
+        double* z2 = l.x2;
+        hipLaunchKernelGGL(k_bddc_local, dim3(nblk(l.ndof)), dim3(256), 0, c.s, r, z, (const double*)bddc_val,
+                           (const uint8_t*)bddc_cnt, l.ndof, gate);
+        for (int s = 1; s < bddc_sweeps; s++) {
+            apply<double>(c, 0, z, z2, 3, r, gate);
+            std::swap(z, z2);
+        }
+        apply<double>(c, 0, z, l.res, 0, nullptr, gate);
+        hipLaunchKernelGGL(k_bddc_corner, dim3(nblk(l.ndof)), dim3(256), 0, c.s, z, r, (const double*)l.res,
+                           (const double*)l.invD, (const uint8_t*)bddc_corner, (const uint8_t*)l.fixed, l.ndof, gate);
+        return z;
     }
 
     // out = A in (mode 0), masked (1), b - A in (2), Jacobi sweep (3)
@@ -1286,25 +1367,6 @@ static CGResult pcg_jacobi(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVe
     return res;
 }
 
-// z = M^-1 r by one V-cycle in precision PT (T when equal; an FP32 V-cycle
-// inside an FP64 CG when PT = float: r is rounded to FP32, the V-cycle runs on
-// the FP32 operators, z comes back in FP64). Returns the buffer holding z.
-template <typename T, typename PT>
-static T* precondition(Ctx& c, MGH& mg, const T* r, T* z, CGVecs& v32, const int* gate, int nr) {
-    const int64_t n = mg.L[0].ndof;
-    if constexpr (std::is_same<T, PT>::value) {
-        T* zz = mg.vcycle<T>(c, 0, r, gate, nr);
-        hipLaunchKernelGGL((k_convert<T, T>), dim3(nblk(n), nr), dim3(256), 0, c.s, z, (const T*)zz, n);
-        return z;
-    } else {
-        PT* r32 = (PT*)v32.r;
-        hipLaunchKernelGGL((k_convert_gated<PT, T>), dim3(nblk(n), nr), dim3(256), 0, c.s, r32, r, n, gate);
-        PT* zz = mg.vcycle<PT>(c, 0, r32, gate, nr);
-        hipLaunchKernelGGL((k_convert_gated<T, PT>), dim3(nblk(n), nr), dim3(256), 0, c.s, z, (const PT*)zz, n, gate);
-        return z;
-    }
-}
-
 // The V-cycle part of precondition() without the widening copy: returns the
 // V-cycle's own output buffer (precision PT, fixed DOFs not yet zeroed). With
 // r_lo_ready the FP32 residual has already been written (k_cg_update_ur).
@@ -1314,12 +1376,34 @@ static const PT* precondition_raw(Ctx& c, MGH& mg, const T* r, CGVecs& v32, cons
     const int64_t n = mg.L[0].ndof;
     if constexpr (std::is_same<T, PT>::value) {
         (void)n; (void)v32; (void)r_lo_ready;
+        if constexpr (std::is_same<T, double>::value)
+            if (mg.bddc) return mg.bddc_apply(c, r, gate);
+        TOPO_CHECK(!mg.bddc, TOPO_E_UNSUPPORTED, "the domain-decomposition preconditioner is FP64 only");
         return mg.vcycle<T>(c, 0, r, gate, nr);
     } else {
         PT* r32 = (PT*)v32.r;
         if (!r_lo_ready)
             hipLaunchKernelGGL((k_convert_gated<PT, T>), dim3(nblk(n), nr), dim3(256), 0, c.s, r32, r, n, gate);
         return mg.vcycle<PT>(c, 0, r32, gate, nr);
+    }
+}
+
+// z = M^-1 r by one V-cycle in precision PT (T when equal; an FP32 V-cycle
+// inside an FP64 CG when PT = float: r is rounded to FP32, the V-cycle runs on
+// the FP32 operators, z comes back in FP64). Returns the buffer holding z.
+template <typename T, typename PT>
+static T* precondition(Ctx& c, MGH& mg, const T* r, T* z, CGVecs& v32, const int* gate, int nr) {
+    const int64_t n = mg.L[0].ndof;
+    if constexpr (std::is_same<T, PT>::value) {
+        const T* zz = precondition_raw<T, PT>(c, mg, r, v32, gate, nr, false);
+        hipLaunchKernelGGL((k_convert<T, T>), dim3(nblk(n), nr), dim3(256), 0, c.s, z, zz, n);
+        return z;
+    } else {
+        PT* r32 = (PT*)v32.r;
+        hipLaunchKernelGGL((k_convert_gated<PT, T>), dim3(nblk(n), nr), dim3(256), 0, c.s, r32, r, n, gate);
+        PT* zz = mg.vcycle<PT>(c, 0, r32, gate, nr);
+        hipLaunchKernelGGL((k_convert_gated<T, PT>), dim3(nblk(n), nr), dim3(256), 0, c.s, z, (const PT*)zz, n, gate);
+        return z;
     }
 }
 
@@ -1467,8 +1551,11 @@ struct Solver {
         cfg = *cfgin;
         TOPO_CHECK(cfg.nelx >= 1 && cfg.nely >= 1 && cfg.nelz >= 1, TOPO_E_ARG, "grid dims must be >= 1");
         TOPO_CHECK(cfg.nload >= 1 && cfg.nload <= 64, TOPO_E_ARG, "nload must be in 1..64");
-        TOPO_CHECK(cfg.linear_solver == TOPO_SOLVER_PCG || cfg.linear_solver == TOPO_SOLVER_MGPCG, TOPO_E_ARG,
-                   "unknown linear_solver %d", cfg.linear_solver);
+        TOPO_CHECK(cfg.linear_solver == TOPO_SOLVER_PCG || cfg.linear_solver == TOPO_SOLVER_MGPCG ||
+                       cfg.linear_solver == TOPO_SOLVER_KSP_BDDC, TOPO_E_ARG, "unknown linear_solver %d",
+                   cfg.linear_solver);
+        TOPO_CHECK(cfg.linear_solver != TOPO_SOLVER_KSP_BDDC || cfg.precision == TOPO_PREC_FP64, TOPO_E_ARG,
+                   "the KSP/BDDC solver runs in FP64 (precision 0)");
         TOPO_CHECK(cfg.precision == TOPO_PREC_FP64 || cfg.precision == TOPO_PREC_MIXED ||
                        cfg.precision == TOPO_PREC_MIXED_PC, TOPO_E_ARG, "unknown precision %d", cfg.precision);
         TOPO_CHECK(hfixed && hF, TOPO_E_ARG, "fixedMask and F are required");
@@ -1590,9 +1677,15 @@ struct Solver {
         const int nlev = cfg.linear_solver == TOPO_SOLVER_MGPCG ? cfg.mg_levels : 1;
         mg.nrhs = batch_cap;
         mg.coarse_fp32 = cfg.coarse_fp32;
+        if (cfg.linear_solver == TOPO_SOLVER_KSP_BDDC) {
+            // plain global Jacobi sweeps inside the preconditioner
+            mg.omega = 1.0;
+            mg.omega_cap = 0.0;
+        }
         mg.build(C, g, fixed, cfg.linear_solver == TOPO_SOLVER_MGPCG ? nlev : 1, mixed);
         mg.L[0].E = E;
         mg.L[0].E32 = E32;
+        if (cfg.linear_solver == TOPO_SOLVER_KSP_BDDC) mg.bddc_setup(C);
         // filter stencil + normalisers
         setup_filter(cfg.rmin);
         // initial design: volfrac, preserved -> 1, void -> 0 (:1928-2022)
@@ -1693,8 +1786,9 @@ struct Solver {
     // E and the operators of all levels from the physical densities (zyx)
     void set_density_dev(const float* dxphys, double penal) {
         Ctx& C = *c;
-        // ebePCG uses Math.pow (assembly/index.ts:544); the MG path _powDensity
-        const int powmode = cfg.linear_solver == TOPO_SOLVER_PCG ? 1 : 0;
+        // ebePCG uses Math.pow (assembly/index.ts:544), so does the KSP/BDDC
+        // wrapper (js/optimizer-worker-3d.js:3824-3830); the MG path _powDensity
+        const int powmode = cfg.linear_solver == TOPO_SOLVER_MGPCG ? 0 : 1;
         hipLaunchKernelGGL(k_moduli<float>, dim3(nblk(nel)), dim3(256), 0, C.s, dxphys, penal, cfg.E0, cfg.Emin, powmode, E,
                            E32, nel);
         C.check_launch();
@@ -1713,10 +1807,11 @@ struct Solver {
         const double tol = tolerance_for(iteration, maxIt);
         const int64_t nfree_cap = cfg.cg_max_iter;
         const bool mgp = cfg.linear_solver == TOPO_SOLVER_MGPCG;
-        // warm start: PCG warm (:3589-3595), WASM MGPCG cold (:1101)
+        // warm start: PCG warm (:3589-3595), WASM MGPCG and KSP/BDDC cold
+        // (assembly/index.ts:1101, :1465)
         bool warm = iteration > 1 && have_U;
         if (cfg.warm_start == 0) warm = false;
-        if (cfg.warm_start == -1 && mgp) warm = false;
+        if (cfg.warm_start == -1 && (mgp || mg.bddc)) warm = false;
         // maxIter = min(nfree, MAX_CG_ITERATIONS) (:3609, :3779)
         const int maxit = (int)std::min<int64_t>(nfree_cap, nfree);
         CGResult tot;
@@ -1865,7 +1960,8 @@ struct Solver {
             tot.iters += r.iters; tot.vcycles += r.vcycles; tot.applyA += r.applyA;
             tot.relres = std::max(tot.relres, r.relres);
         } else if (cfg.precision == TOPO_PREC_FP64 || cfg.precision == TOPO_PREC_MIXED_PC) {
-            CGResult r = mgp ? pcg_mg<double, double>(C, mg, rhs, sol, warm, v64, tol, maxit, &cancel_flag)
+            // KSP/BDDC: the same PCG with the domain-decomposition preconditioner
+            CGResult r = (mgp || mg.bddc) ? pcg_mg<double, double>(C, mg, rhs, sol, warm, v64, tol, maxit, &cancel_flag)
                              : pcg_jacobi<double, double>(C, mg, rhs, sol, warm, v64, tol, maxit, &cancel_flag);
             TOPO_CHECK(!r.nonfinite, TOPO_E_BREAKDOWN, "non-finite CG scalar");
             tot.iters += r.iters; tot.vcycles += r.vcycles; tot.applyA += r.applyA;
@@ -2892,6 +2988,94 @@ int topo_ebeMGPCG(const topo_mg_level* levels, int32_t numLevels, const double* 
     TOPO_HIP(hipMemcpyAsync(dF, Fh, ndof0 * sizeof(double), hipMemcpyHostToDevice, C.s));
     CGResult r = pcg_mg<double, double>(C, mg, dF, dU, false, v, tolerance, maxIter, nullptr);
     TOPO_HIP(hipMemcpyAsync(Uh, dU, ndof0 * sizeof(double), hipMemcpyDeviceToHost, C.s));
+    C.sync();
+    for (void* q : tmp) C.free_one(q);
+    TOPO_CHECK(!r.nonfinite, TOPO_E_BREAKDOWN, "non-finite CG scalar");
+    return r.iters;
+    TOPO_API_END
+}
+
+int topo_ebeKSP_BDDC(const topo_bddc_sub* subs, int32_t numSubs, const int32_t* coarse, int32_t numCoarse,
+                     const double* ke, const int32_t* edofs, const double* evals, const int32_t* active,
+                     int32_t activeCount, const uint8_t* fixedMask, const double* invDiag, const double* Fh,
+                     double* Uh, int32_t ndof, int32_t maxIter, double tolerance, int32_t smootherIters,
+                     double* work) {
+    (void)work;
+    TOPO_API_BEGIN
+    TOPO_CHECK(subs && numSubs >= 1 && (numCoarse == 0 || coarse) && ke && edofs && evals && fixedMask && invDiag &&
+                   Fh && Uh && ndof > 0 && smootherIters >= 1 && (activeCount == 0 || active),
+               TOPO_E_ARG, "topo_ebeKSP_BDDC: bad argument");
+    std::lock_guard<std::mutex> lk(g_twin_mu);
+    TwinCtx& T = g_twin;
+    T.ensure();
+    Ctx& C = *T.c;
+    T.set_ke(ke);
+    const Grid gg = T.grid_of(edofs, ndof);
+    // per-DOF slots of the subdomains' local inverse diagonals, in ascending
+    // subdomain order (the accumulation order of bddcApply :1294-1310)
+    std::vector<double> hval((size_t)kBddcSlots * ndof, 0.0);
+    std::vector<uint8_t> hcnt(ndof, 0), hcor(ndof, 0);
+    for (int si = 0; si < numSubs; si++) {
+        const topo_bddc_sub& d = subs[si];
+        TOPO_CHECK(d.numDofs >= 0 && (d.numDofs == 0 || (d.dofs && d.localInvDiag)), TOPO_E_ARG,
+                   "subdomain %d: dofs/localInvDiag missing", si);
+        for (int k = 0; k < d.numDofs; k++) {
+            const int32_t g = d.dofs[k];
+            TOPO_CHECK(g >= 0 && g < ndof, TOPO_E_ARG, "subdomain %d: DOF %d out of range", si, g);
+            TOPO_CHECK(hcnt[g] < kBddcSlots, TOPO_E_UNSUPPORTED, "DOF %d belongs to more than %d subdomains", g,
+                       kBddcSlots);
+            hval[(size_t)hcnt[g] * ndof + g] = d.localInvDiag[g];
+            hcnt[g]++;
+        }
+    }
+    for (int k = 0; k < numCoarse; k++) {
+        TOPO_CHECK(coarse[k] >= 0 && coarse[k] < ndof, TOPO_E_ARG, "coarse DOF %d out of range", coarse[k]);
+        hcor[coarse[k]] = 1;
+    }
+    std::vector<void*> tmp;
+    auto A = [&](size_t bytes) {
+        void* q = C.alloc<uint8_t>((int64_t)bytes);
+        tmp.push_back(q);
+        return q;
+    };
+    MGH mg;
+    mg.K0 = &T.K0;
+    mg.thr = 0.0;   // inactive elements carry E = 0
+    mg.exact = true;
+    mg.omega = 1.0;
+    MGLevel l;
+    l.g = gg;
+    l.ndof = ndof;
+    l.fixed = (uint8_t*)A(ndof);
+    TOPO_HIP(hipMemcpyAsync(l.fixed, fixedMask, ndof, hipMemcpyHostToDevice, C.s));
+    l.invD = (double*)A(ndof * sizeof(double));
+    TOPO_HIP(hipMemcpyAsync(l.invD, invDiag, ndof * sizeof(double), hipMemcpyHostToDevice, C.s));
+    std::vector<double> Ez = masked_E_zyx(gg, evals, active, activeCount);
+    l.E = (double*)A(Ez.size() * sizeof(double));
+    TOPO_HIP(hipMemcpyAsync(l.E, Ez.data(), Ez.size() * sizeof(double), hipMemcpyHostToDevice, C.s));
+    l.b = (double*)A(ndof * sizeof(double));
+    l.x = (double*)A(ndof * sizeof(double));
+    l.x2 = (double*)A(ndof * sizeof(double));
+    l.res = (double*)A(ndof * sizeof(double));
+    mg.L.push_back(l);
+    mg.bddc = true;
+    mg.bddc_sweeps = smootherIters;
+    mg.bddc_val = (double*)A(hval.size() * sizeof(double));
+    TOPO_HIP(hipMemcpyAsync(mg.bddc_val, hval.data(), hval.size() * sizeof(double), hipMemcpyHostToDevice, C.s));
+    mg.bddc_cnt = (uint8_t*)A(ndof);
+    TOPO_HIP(hipMemcpyAsync(mg.bddc_cnt, hcnt.data(), ndof, hipMemcpyHostToDevice, C.s));
+    mg.bddc_corner = (uint8_t*)A(ndof);
+    TOPO_HIP(hipMemcpyAsync(mg.bddc_corner, hcor.data(), ndof, hipMemcpyHostToDevice, C.s));
+    C.sync();
+    mg.upload_omega(C);
+    double* dF = (double*)A(ndof * sizeof(double));
+    double* dU = (double*)A(ndof * sizeof(double));
+    CGVecs v;
+    v.r = A(ndof * sizeof(double)); v.z = A(ndof * sizeof(double));
+    v.p = A(ndof * sizeof(double)); v.Ap = A(ndof * sizeof(double));
+    TOPO_HIP(hipMemcpyAsync(dF, Fh, ndof * sizeof(double), hipMemcpyHostToDevice, C.s));
+    CGResult r = pcg_mg<double, double>(C, mg, dF, dU, false, v, tolerance, maxIter, nullptr);
+    TOPO_HIP(hipMemcpyAsync(Uh, dU, ndof * sizeof(double), hipMemcpyDeviceToHost, C.s));
     C.sync();
     for (void* q : tmp) C.free_one(q);
     TOPO_CHECK(!r.nonfinite, TOPO_E_BREAKDOWN, "non-finite CG scalar");

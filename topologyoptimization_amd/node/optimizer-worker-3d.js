@@ -18,6 +18,31 @@ let state = { cancelled: false, paused: false, resume: null, pending: null };
 
 function post(msg, transfer) { parentPort.postMessage(msg, transfer); }
 
+const FIELD_STRESS = 5;
+
+/* Volumetric snapshots (_buildVolumetricSnapshot :1579-1592 and the
+ * volumetricOutputMode rules :1573-1577, :2116-2120, :2280-2289, :2319-2321):
+ * the element stress field E_e w_e of the last iteration, read from the
+ * device when a snapshot is due. */
+const vol = { mode: 'on-stop', h: null, nel: 0, dims: null, iteration: 0, maxStress: 0 };
+
+function normalizeVolumetricMode(mode) {
+    if (mode === 'every-iteration') return 'every-iteration';
+    if (mode === 'off') return 'off';
+    return 'on-stop';
+}
+
+function volumetricSnapshot() {
+    if (!vol.h || !vol.iteration) return null;
+    const stress = new Float32Array(vol.nel);
+    addon.solverGetField(vol.h, FIELD_STRESS, stress);
+    return {
+        payload: { nx: vol.dims.nx, ny: vol.dims.ny, nz: vol.dims.nz, iteration: vol.iteration,
+            maxStress: vol.maxStress, stress },
+        transfer: [stress.buffer],
+    };
+}
+
 const yieldToLoop = () => new Promise((r) => setImmediate(r));
 
 async function optimize(model, config) {
@@ -27,6 +52,8 @@ async function optimize(model, config) {
     const nel = nx * ny * nz;
     const h = addon.createSolver(cfg, problem.fixedMask, problem.F, problem.preserved, problem.voidMask);
     if (problem.x0) addon.solverSetDesign(h, problem.x0);
+    vol.mode = normalizeVolumetricMode(config.volumetricOutputMode);
+    vol.h = h; vol.nel = nel; vol.dims = { nx, ny, nz }; vol.iteration = 0; vol.maxStress = 0;
 
     if (config.feaOnly) {
         // single FE analysis at the given densities (:1652-1883); tol -> 1e-8
@@ -75,13 +102,16 @@ async function optimize(model, config) {
             const p = state.pending;
             state.pending = null;
             if (p.maxIterations !== undefined) maxIterations = p.maxIterations;
+            if (p.volumetricOutputMode !== undefined) vol.mode = normalizeVolumetricMode(p.volumetricOutputMode);
             addon.solverUpdate(h, p.penaltyFactor !== undefined ? p.penaltyFactor : -1,
                 p.filterRadius !== undefined ? p.filterRadius : -1,
                 p.volumeFraction !== undefined ? p.volumeFraction : -1,
                 p.maxIterations !== undefined ? p.maxIterations : -1);
         }
         if (state.paused) {
-            post({ type: 'paused', iteration: loop, volumetricData: null });
+            const snap = vol.mode !== 'off' ? volumetricSnapshot() : null;
+            if (snap) post({ type: 'paused', iteration: loop, volumetricData: snap.payload }, snap.transfer);
+            else post({ type: 'paused', iteration: loop, volumetricData: null });
             await new Promise((r) => { state.resume = r; });
             if (state.cancelled) {
                 post({ type: 'cancelled', iteration: loop });
@@ -94,24 +124,29 @@ async function optimize(model, config) {
         c = info.compliance;
         maxStress = info.maxStress;
         iterationTimes.push(info.totalMs);
+        vol.iteration = loop;
+        vol.maxStress = maxStress;
         const avg = iterationTimes.reduce((a, b) => a + b, 0) / iterationTimes.length;
+        const snap = vol.mode === 'every-iteration' ? volumetricSnapshot() : null;
         post({
-            type: 'progress', iteration: loop, compliance: c, meshData: null, maxStress, volumetricData: null,
+            type: 'progress', iteration: loop, compliance: c, meshData: null, maxStress,
+            volumetricData: snap ? snap.payload : null,
             timing: { iterationTime: info.totalMs, avgIterationTime: avg, elapsedTime: Date.now() - start,
                 usingWasm: false, usingGPU: true },
             solverStats: { cgIterations: info.cgIterations, tolerance: info.tolerance, backend: 'hip',
                 outerIterations: info.outerIterations, change: info.change, volume: info.volume },
-        });
+        }, snap ? snap.transfer : undefined);
     }
     const densities = new Float32Array(nel);
     addon.solverGetDesign(h, densities);
     const avg = iterationTimes.length ? iterationTimes.reduce((a, b) => a + b, 0) / iterationTimes.length : 0;
+    const snap = vol.mode !== 'off' ? volumetricSnapshot() : null;
     post({ type: 'complete', result: {
         densities, finalCompliance: c, iterations: loop, volumeFraction: cfg.volfrac, nx, ny, nz,
-        meshData: null, maxStress, volumetricData: null, amrStats: null,
+        meshData: null, maxStress, volumetricData: snap ? snap.payload : null, amrStats: null,
         timing: { elapsedTime: Date.now() - start, avgIterationTime: avg, iterationTimes, usingWasm: false,
             usingGPU: true },
-    } }, [densities.buffer]);
+    } }, snap ? [densities.buffer].concat(snap.transfer) : [densities.buffer]);
 }
 
 parentPort.on('message', (data) => {
@@ -134,9 +169,13 @@ parentPort.on('message', (data) => {
     case 'updateConfig':
         state.pending = Object.assign(state.pending || {}, data.config || {});
         break;
-    case 'requestVolumetric':
-        post({ type: 'volumetric', reason: 'unavailable', volumetricData: null });
+    case 'requestVolumetric': {
+        // postLatestVolumetricSnapshot('on-demand') (:1595-1603)
+        const snap = volumetricSnapshot();
+        if (snap) post({ type: 'volumetric', reason: 'on-demand', volumetricData: snap.payload }, snap.transfer);
+        else post({ type: 'volumetric', reason: 'on-demand', volumetricData: null });
         break;
+    }
     default:
         break;
     }

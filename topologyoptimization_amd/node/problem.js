@@ -108,7 +108,7 @@ function buildProblem(addon, model, config) {
  * topo_config from the reference's config keys (volumeFraction,
  * penaltyFactor, filterRadius, maxIterations, youngsModulus, poissonsRatio,
  * penalStart, continuationIters, useProjection, betaMax, betaInterval,
- * linearSolver) plus this library's own (precision 'fp64'|'mixed',
+ * linearSolver, petscPC) plus this library's own (precision 'fp64'|'mixed',
  * filterType, mgLevels, mgOmega, mgOmegaCap, mgPreSmooth, mgPostSmooth,
  * coarseSolver 'sweeps'|'cholesky', cgTol, cgMaxIter, skipFactor,
  * exactApply, useGraphs, move).
@@ -130,8 +130,17 @@ function toTopoConfig(addon, model, config) {
     c.use_projection = config.useProjection !== false ? 1 : 0;
     if (config.betaMax != null) c.beta_max = config.betaMax;
     if (config.betaInterval != null) c.beta_interval = config.betaInterval;
-    if (config.linearSolver) c.linear_solver = config.linearSolver === 'mgpcg' ? 1 : 0;
-    if (config.precision) c.precision = config.precision === 'mixed' ? 1 : 0;
+    if (config.linearSolver === 'petsc') {
+        // 'petsc' (FE() :4176-4181): the WASM KSP/BDDC solve for petscPC
+        // 'bddc' (the default); the other preconditioners run the reference's
+        // JS KSP loop, here the MG-PCG ('mg') or Jacobi-PCG ('jacobi') solves
+        const pc = config.petscPC || 'bddc';
+        c.linear_solver = pc === 'bddc' ? 2 : pc === 'mg' ? 1 : 0;
+        if (pc === 'bddc') c.precision = 0;
+    } else if (config.linearSolver) {
+        c.linear_solver = config.linearSolver === 'mgpcg' ? 1 : 0;
+    }
+    if (config.precision && c.linear_solver !== 2) c.precision = config.precision === 'mixed' ? 1 : 0;
     if (config.filterType && Object.prototype.hasOwnProperty.call(FILTER, config.filterType)) c.filter_type = FILTER[config.filterType];
     if (num(config.mgLevels)) c.mg_levels = config.mgLevels;
     if (num(config.mgOmega)) c.mg_omega = config.mgOmega;

@@ -290,6 +290,38 @@ napi_value EbeMGPCG(napi_env env, napi_callback_info info) {
     return i32v(env, rc);
 }
 
+// ebeKSP_BDDC(subs[{numElements, numDofs, elements, dofs, localInvDiag}],
+//             coarse, ke, edofs, evals, active, fixedMask, invDiag, F, U,
+//             ndof, maxIter, tolerance, smootherIters) -> iterations
+napi_value EbeKSPBDDC(napi_env env, napi_callback_info info) {
+    napi_value a[14];
+    if (!get_args(env, info, 14, a)) return nullptr;
+    uint32_t ns = 0;
+    napi_get_array_length(env, a[0], &ns);
+    std::vector<topo_bddc_sub> S(ns);
+    for (uint32_t i = 0; i < ns; i++) {
+        napi_value o;
+        napi_get_element(env, a[0], i, &o);
+        topo_bddc_sub& d = S[i];
+        std::memset(&d, 0, sizeof d);
+        d.numElements = to_i32(env, get(env, o, "numElements"));
+        d.numDofs = to_i32(env, get(env, o, "numDofs"));
+        d.elements = (const int32_t*)ta_ptr(env, get(env, o, "elements"));
+        d.dofs = (const int32_t*)ta_ptr(env, get(env, o, "dofs"));
+        d.localInvDiag = (const double*)ta_ptr(env, get(env, o, "localInvDiag"));
+    }
+    size_t ncoarse = 0, nactive = 0;
+    const int32_t* coarse = (const int32_t*)ta_ptr(env, a[1], &ncoarse);
+    const int32_t* active = (const int32_t*)ta_ptr(env, a[5], &nactive);
+    int rc = topo_ebeKSP_BDDC(S.data(), (int32_t)ns, coarse, (int32_t)ncoarse, (const double*)ta_ptr(env, a[2]),
+                              (const int32_t*)ta_ptr(env, a[3]), (const double*)ta_ptr(env, a[4]), active,
+                              (int32_t)nactive, (const uint8_t*)ta_ptr(env, a[6]), (const double*)ta_ptr(env, a[7]),
+                              (const double*)ta_ptr(env, a[8]), (double*)ta_ptr(env, a[9]), to_i32(env, a[10]),
+                              to_i32(env, a[11]), to_f64(env, a[12]), to_i32(env, a[13]), nullptr);
+    if (rc < 0) return throw_topo(env, rc);
+    return i32v(env, rc);
+}
+
 // ---- device-resident solver --------------------------------------------------
 void solver_finalize(napi_env, void* data, void*) { topo_solver_destroy((topo_solver*)data); }
 
@@ -435,7 +467,7 @@ NAPI_MODULE_INIT() {
         {"version", Version}, {"deviceInfo", DeviceInfo}, {"configDefault", ConfigDefault}, {"k0", K0},
         {"edofs", Edofs}, {"fixedDofs", FixedDofs}, {"loadVector", LoadVector}, {"preservedElements", Preserved},
         {"loadPaint", LoadPaint}, {"fixedPaint", FixedPaint}, {"applyAEbe3D", ApplyAEbe3D}, {"ebePCG", EbePCG},
-        {"ebeMGPCG", EbeMGPCG}, {"createSolver", CreateSolver}, {"solverStep", SolverStep},
+        {"ebeMGPCG", EbeMGPCG}, {"ebeKSP_BDDC", EbeKSPBDDC}, {"createSolver", CreateSolver}, {"solverStep", SolverStep},
         {"solverSetDesign", SolverSetDesign}, {"solverGetDesign", SolverGetDesign},
         {"solverGetField", SolverGetField}, {"solverUpdate", SolverUpdate}, {"solverFe", SolverFe},
         {"solverSetIteration", SolverSetIteration}, {"solverCancel", SolverCancel}, {"solverEnergy", SolverEnergy},

@@ -16,7 +16,7 @@ const path = require('path');
 
 const IS_NODE = typeof process !== 'undefined' && !!(process.versions && process.versions.node);
 
-const UNSUPPORTED = ['2d', 'petsc', 'genetic', 'nonlinear', 'fracture', 'dynamics', 'fatigue'];
+const UNSUPPORTED = ['2d', 'genetic', 'nonlinear', 'fracture', 'dynamics', 'fatigue'];
 
 class TopologySolver {
     constructor() {
@@ -53,6 +53,7 @@ class TopologySolver {
             let feaOnly = false;
             if (solverKey === 'mgpcg') linearSolver = 'mgpcg';
             else if (solverKey === 'cg') linearSolver = 'cg';
+            else if (solverKey === 'petsc') linearSolver = 'petsc';
             else if (solverKey === 'fea') feaOnly = true;
             const use3D = solverKey !== 'auto' || model.type === 'cube' || (model.nz != null && model.nz > 1);
             if (!use3D) {

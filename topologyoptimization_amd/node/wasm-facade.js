@@ -2,12 +2,12 @@
 /**
  * WASM-shaped facade of the HIP library: the same object shape as the
  * reference's instantiated matrix-ops module exports -- `memory` (a real
- * WebAssembly.Memory) plus `applyAEbe3D`, `ebePCG`, `ebeMGPCG` taking byte
- * offsets into that memory in the argument order of assembly/index.ts
- * (:403-413, :504-521, :1076-1087). Call sites such as
- * js/optimizer-worker-3d.js:822-825, :3605-3609 and :3783-3787 work
- * unchanged against it; the arrays they wrote into `memory` are handed to
- * the C ABI without copies.
+ * WebAssembly.Memory) plus `applyAEbe3D`, `ebePCG`, `ebeMGPCG`,
+ * `ebeKSP_BDDC` taking byte offsets into that memory in the argument order
+ * of assembly/index.ts (:403-413, :504-521, :1076-1087, :1410-1429). Call
+ * sites such as js/optimizer-worker-3d.js:822-825, :3605-3609, :3783-3787
+ * and :3972-3979 work unchanged against it; the arrays they wrote into
+ * `memory` are handed to the C ABI without copies.
  */
 const path = require('path');
 const addon = require(path.join(__dirname, 'topo_addon.node'));
@@ -64,7 +64,26 @@ function createWasmFacade(initialPages) {
             ndof0, maxIter, tolerance);
     }
 
-    return { memory, applyAEbe3D, ebePCG, ebeMGPCG };
+    // subdomain descriptor: BDDC_SUB_STRIDE = 32 bytes (assembly/index.ts:1204-1222)
+    function ebeKSP_BDDC(subsPtr, numSubs, coarsePtr, numCoarse, kePtr, edofsPtr, evalsPtr, activePtr, activeCount,
+        fixedMask, invDiagPtr, fPtr, uPtr, ndof, maxIter, tolerance, smootherIters, workPtr) {
+        const g = gridOf(i32(edofsPtr, 24), ndof);
+        const subs = [];
+        for (let si = 0; si < numSubs; si++) {
+            const d = i32(subsPtr + 32 * si, 8);
+            const u32 = new Uint32Array(memory.buffer, subsPtr + 32 * si, 8);
+            subs.push({
+                numElements: d[0], numDofs: d[1], elements: i32(u32[2], Math.max(d[0], 0)),
+                dofs: i32(u32[3], Math.max(d[1], 0)), localInvDiag: f64(u32[4], ndof),
+            });
+        }
+        return addon.ebeKSP_BDDC(subs, i32(coarsePtr, Math.max(numCoarse, 0)), f64(kePtr, 576),
+            i32(edofsPtr, g.nel * 24), f64(evalsPtr, g.nel), i32(activePtr, Math.max(activeCount, 0)),
+            u8(fixedMask, ndof), f64(invDiagPtr, ndof), f64(fPtr, ndof), f64(uPtr, ndof), ndof, maxIter, tolerance,
+            smootherIters);
+    }
+
+    return { memory, applyAEbe3D, ebePCG, ebeMGPCG, ebeKSP_BDDC };
 }
 
 module.exports = { createWasmFacade, gridOf };
