@@ -108,7 +108,8 @@ function buildProblem(addon, model, config) {
  * topo_config from the reference's config keys (volumeFraction,
  * penaltyFactor, filterRadius, maxIterations, youngsModulus, poissonsRatio,
  * penalStart, continuationIters, useProjection, betaMax, betaInterval,
- * linearSolver, petscPC) plus this library's own (precision 'fp64'|'mixed',
+ * linearSolver, petscPC) plus this library's own (precision
+ * 'fp64'|'mixed'|'mixed-pc', coarseFp32, rhsTile,
  * filterType, mgLevels, mgOmega, mgOmegaCap, mgPreSmooth, mgPostSmooth,
  * coarseSolver 'sweeps'|'cholesky', cgTol, cgMaxIter, skipFactor,
  * exactApply, useGraphs, move).
@@ -140,7 +141,8 @@ function toTopoConfig(addon, model, config) {
     } else if (config.linearSolver) {
         c.linear_solver = config.linearSolver === 'mgpcg' ? 1 : 0;
     }
-    if (config.precision && c.linear_solver !== 2) c.precision = config.precision === 'mixed' ? 1 : 0;
+    if (config.precision && c.linear_solver !== 2)
+        c.precision = config.precision === 'mixed' ? 1 : config.precision === 'mixed-pc' ? 2 : 0;
     if (config.filterType && Object.prototype.hasOwnProperty.call(FILTER, config.filterType)) c.filter_type = FILTER[config.filterType];
     if (num(config.mgLevels)) c.mg_levels = config.mgLevels;
     if (num(config.mgOmega)) c.mg_omega = config.mgOmega;
@@ -154,6 +156,8 @@ function toTopoConfig(addon, model, config) {
     if (config.exactApply != null) c.exact_apply = config.exactApply ? 1 : 0;
     if (config.useGraphs != null) c.use_graphs = config.useGraphs ? 1 : 0;
     if (num(config.move)) c.move = config.move;
+    if (num(config.coarseFp32)) c.coarse_fp32 = config.coarseFp32;
+    if (num(config.rhsTile)) c.rhs_tile = config.rhsTile;
     return c;
 }
 
