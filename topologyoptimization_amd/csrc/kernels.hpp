@@ -883,6 +883,10 @@ __global__ void k_restrict(const T* __restrict__ res, T* __restrict__ bC, Grid g
 
 // x += P xC on free fine DOFs (mgVcycle :977-1042): the 8 corner terms are
 // summed left to right in the order 000,100,010,110,001,101,011,111.
+// Thread (q, C), q = (dx, dy) in {0,1}^2, C a coarse node: the fine z-pair
+// f = (2Cx + dx, 2Cy + dy, 2Cz + {0,1}) (inside the grid) has the corners C
+// and min(C+1, nC-1) per axis (pw1d of f gives c0 = C), so 8 coarse values
+// serve two fine nodes, and consecutive threads update consecutive z-pairs.
 template <typename T>
 __global__ void k_prolong_add(T* __restrict__ x, const T* __restrict__ xC, Grid gf, Grid gc,
                               const uint8_t* __restrict__ fixedF, const int* gate) {
@@ -891,34 +895,48 @@ __global__ void k_prolong_add(T* __restrict__ x, const T* __restrict__ xC, Grid 
     if (gate && *gate) return;
     x += batch_off(gf.ndof());
     xC += batch_off(gc.ndof());
-    const int nyF = gf.nny(), nzF = gf.nnz();
-    const int nyC = gc.nny(), nzC = gc.nnz(), nxC = gc.nnx();
-    for (int64_t nf = gtid(); nf < gf.nnodes(); nf += gstride()) {
+    const int nxF = gf.nnx(), nyF = gf.nny(), nzF = gf.nnz();
+    const int nxC = gc.nnx(), nyC = gc.nny(), nzC = gc.nnz();
+    const int64_t ncn = gc.nnodes();
+    for (int64_t tq = gtid(); tq < 4 * ncn; tq += gstride()) {
+        const int q = (int)(tq / ncn);
+        const int64_t nc = tq - (int64_t)q * ncn;
         // 32-bit index arithmetic (node counts below 2^32)
-        const uint32_t u = (uint32_t)nf, t = u / (uint32_t)nzF, ux = t / (uint32_t)nyF;
-        const int fz = (int)(u - t * (uint32_t)nzF), fy = (int)(t - ux * (uint32_t)nyF), fx = (int)ux;
-        int cx[2], cy[2], cz[2];
-        double wx[2], wy[2], wz[2];
-        pw1d(fx, nxC, cx[0], cx[1], wx[0], wx[1]);
-        pw1d(fy, nyC, cy[0], cy[1], wy[0], wy[1]);
-        pw1d(fz, nzC, cz[0], cz[1], wz[0], wz[1]);
-        // corner weights (products of 1/2 and 1: exact in T) and node indices
-        T w[8];
-        int64_t nc[8];
+        const uint32_t u = (uint32_t)nc, t = u / (uint32_t)nzC, ux = t / (uint32_t)nyC;
+        const int cz = (int)(u - t * (uint32_t)nzC), cy = (int)(t - ux * (uint32_t)nyC), cx = (int)ux;
+        const int dxf = q & 1, dyf = q >> 1;
+        const int fx = 2 * cx + dxf, fy = 2 * cy + dyf;
+        if (fx >= nxF || fy >= nyF) continue;
+        const int c1[3] = {min(cx + 1, nxC - 1), min(cy + 1, nyC - 1), min(cz + 1, nzC - 1)};
+        const int c0[3] = {cx, cy, cz};
+        T v8[8][3];
 #pragma unroll
         for (int k = 0; k < 8; k++) {
-            const int dx = k & 1, dy = (k >> 1) & 1, dz = k >> 2;
-            w[k] = (T)((wx[dx] * wy[dy]) * wz[dz]);
-            nc[k] = 3 * (((int64_t)cx[dx] * nyC + cy[dy]) * nzC + cz[dz]);
+            const int64_t n = ((int64_t)(k & 1 ? c1[0] : c0[0]) * nyC + (k & 2 ? c1[1] : c0[1])) * nzC +
+                              (k & 4 ? c1[2] : c0[2]);
+#pragma unroll
+            for (int d = 0; d < 3; d++) v8[k][d] = xC[3 * n + d];
         }
+        const double tx = (c0[0] != c1[0] && dxf) ? 0.5 : 0.0;
+        const double ty = (c0[1] != c1[1] && dyf) ? 0.5 : 0.0;
+        const double wx[2] = {1 - tx, tx}, wy[2] = {1 - ty, ty};
 #pragma unroll
-        for (int d = 0; d < 3; d++) {
-            const int64_t i = 3 * nf + d;
-            if (fixedF[i]) continue;
-            T v = xC[nc[0] + d] * w[0];
+        for (int dzf = 0; dzf < 2; dzf++) {
+            const int fz = 2 * cz + dzf;
+            if (fz >= nzF) continue;
+            const double tz = (c0[2] != c1[2] && dzf) ? 0.5 : 0.0;
+            const double wz[2] = {1 - tz, tz};
+            const int64_t nf = ((int64_t)fx * nyF + fy) * nzF + fz;
+            // no branch on the mask: a fixed DOF adds an exact zero
 #pragma unroll
-            for (int k = 1; k < 8; k++) v = v + xC[nc[k] + d] * w[k];
-            x[i] = x[i] + v;
+            for (int d = 0; d < 3; d++) {
+                const int64_t i = 3 * nf + d;
+                T v = v8[0][d] * (T)((wx[0] * wy[0]) * wz[0]);
+#pragma unroll
+                for (int k = 1; k < 8; k++) v = v + v8[k][d] * (T)((wx[k & 1] * wy[(k >> 1) & 1]) * wz[k >> 2]);
+                const T xi = x[i];
+                x[i] = fixedF[i] ? xi : xi + v;
+            }
         }
     }
 }

@@ -193,15 +193,16 @@ __global__ void __launch_bounds__(256) k_rowscale_f16(const double* __restrict__
 }
 
 // Triangular matrix-vector product for NR right-hand sides: rows of M (n x n,
-// row-major, TK = double, float or _Float16) against bf (NR x n, FP64);
+// row-major, TK = double, float or _Float16) against bf (NR x n, TB = FP64,
+// or FP32 with the FP16 matrix); the sums are FP64;
 // lower: row i uses columns [0, i], upper: [i, n) (the other part of M is
 // zero, so whole 16-byte vectors around the diagonal can be read). One wave
-// per 4 rows. scatter = 0: y[q n + i] = result (FP64), times ys[i] when ys
+// per 4 rows. scatter = 0: y[q n + i] = result (TY), times ys[i] when ys
 // is given; scatter = 1: x[q ndof + free[i]] = result in T, for i < nf
 // (converged right-hand sides are not written).
-template <typename T, int NR, typename TK>
+template <typename T, int NR, typename TK, typename TB = double, typename TY = double>
 __global__ void __launch_bounds__(256) k_tri_apply(const TK* __restrict__ M, int64_t n, int upper,
-                                                   const double* __restrict__ bf, double* __restrict__ y,
+                                                   const TB* __restrict__ bf, TY* __restrict__ y,
                                                    T* __restrict__ x, const int32_t* __restrict__ freeIdx, int64_t nf,
                                                    int64_t ndof, const int* gate, const double* __restrict__ ys = nullptr) {
     bool live[NR];
@@ -230,7 +231,7 @@ __global__ void __launch_bounds__(256) k_tri_apply(const TK* __restrict__ M, int
 #pragma unroll
         for (int q = 0; q < NR; q++)
 #pragma unroll
-            for (int u = 0; u < VN; u++) b[q][u] = bf[q * n + j * VN + u];
+            for (int u = 0; u < VN; u++) b[q][u] = (double)bf[q * n + j * VN + u];
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             if constexpr (VN == 2) {
@@ -266,7 +267,7 @@ __global__ void __launch_bounds__(256) k_tri_apply(const TK* __restrict__ M, int
             if (!live[q]) continue;
             for (int r = 0; r < 4; r++) {
                 const int64_t i = row0 + r;
-                if (!x) y[q * n + i] = ys ? ys[i] * acc[r][q] : acc[r][q];
+                if (!x) y[q * n + i] = (TY)(ys ? ys[i] * acc[r][q] : acc[r][q]);
                 else if (i < nf) x[q * ndof + freeIdx[i]] = (T)acc[r][q];
             }
         }

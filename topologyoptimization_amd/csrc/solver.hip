@@ -70,6 +70,29 @@ struct Ctx {
         }
         return blas;
     }
+    // second stream (default priority) for work that overlaps a dependency
+    // chain on s, with its own rocBLAS handle and a pool of ordering events
+    hipStream_t s2 = nullptr;
+    rocblas_handle blas2 = nullptr;
+    std::vector<hipEvent_t> evs;
+    void side_ready(int nev) {
+        if (!s2) TOPO_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+        if (!blas2) {
+            TOPO_CHECK(rocblas_create_handle(&blas2) == rocblas_status_success, TOPO_E_HIP,
+                       "rocblas_create_handle failed");
+            TOPO_CHECK(rocblas_set_stream(blas2, s2) == rocblas_status_success, TOPO_E_HIP, "rocblas_set_stream failed");
+        }
+        while ((int)evs.size() < nev) {
+            hipEvent_t e;
+            TOPO_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            evs.push_back(e);
+        }
+    }
+    // `to` waits for the work issued so far on `from` (event k of the pool)
+    void order(hipStream_t from, hipStream_t to, int k) {
+        TOPO_HIP(hipEventRecord(evs[k], from));
+        TOPO_HIP(hipStreamWaitEvent(to, evs[k], 0));
+    }
 
     hipGraphExec_t graph_find(const std::vector<const void*>& key) {
         for (auto& g : graphs)
@@ -118,8 +141,12 @@ struct Ctx {
     }
     ~Ctx() {
         if (s) (void)hipStreamSynchronize(s);
+        if (s2) (void)hipStreamSynchronize(s2);
         for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
         if (blas) (void)rocblas_destroy_handle(blas);
+        if (blas2) (void)rocblas_destroy_handle(blas2);
+        for (hipEvent_t e : evs) (void)hipEventDestroy(e);
+        if (s2) (void)hipStreamDestroy(s2);
         for (void* p : allocs) (void)hipFree(p);
         if (h_st) (void)hipHostFree(h_st);
         if (h_oc) (void)hipHostFree(h_oc);
@@ -337,14 +364,14 @@ __global__ void k_jacobi_first(T* __restrict__ x, const T* __restrict__ b, const
 // DOFs, padded to ld = nfp, a multiple of 4, so rows are 16-byte aligned):
 // bf[j] = b[free[j]] (zero for the padding), then x[free[i]] = sum_j
 // Kinv[i][j] bf[j]; x = 0 on fixed DOFs.
-template <typename T>
+template <typename T, typename TB = double>
 __global__ void k_gather_free(const T* __restrict__ b, const int32_t* __restrict__ freeIdx, int64_t nf, int64_t nfp,
-                              double* __restrict__ bf, const int* gate, int64_t ndof) {
+                              TB* __restrict__ bf, const int* gate, int64_t ndof) {
     gate = batch_gate(gate);
     if (gate && *gate) return;
     b += batch_off(ndof);
     bf += batch_off(nfp);
-    for (int64_t j = gtid(); j < nfp; j += gstride()) bf[j] = j < nf ? (double)b[freeIdx[j]] : 0.0;
+    for (int64_t j = gtid(); j < nfp; j += gstride()) bf[j] = j < nf ? (TB)b[freeIdx[j]] : TB(0);
 }
 // one wave per 4 consecutive rows; each lane streams 16-byte vectors of the
 // four rows against the same slices of the NR right-hand sides bf (read from
@@ -936,9 +963,19 @@ struct MGH {
             int* dflag = (int*)(c.dsc + 14);   // device scratch word
             hipLaunchKernelGGL(k_diag_copy, dim3(nblk(n)), dim3(256), 0, c.s, (const double*)l.Kinv, n, l.d0c);
             hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, c.s, dflag, 0, (int64_t)1, (const int*)nullptr);
+            // Z = L^-1 by row blocks on the second stream: row block I of L
+            // is final once the diagonal step I is done (the panels of the
+            // earlier steps wrote its band), and the factorization goes on
+            // below it, so the DGEMMs of row I overlap steps I+1, ...
+            c.side_ready(NB + 2);
+            if (!l.ztmp) l.ztmp = c.alloc<double>(kCB * n);
+            hipLaunchKernelGGL(k_fill<double>, dim3(nblk(n * n)), dim3(256), 0, c.s, l.Z, 0.0, n * n,
+                               (const int*)nullptr);
             for (int K = 0; K < NB; K++) {
                 hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(64), 0, c.s, l.Kinv, n, K, l.Linv,
                                    (const double*)l.d0c, 1e-11, dflag);
+                c.order(c.s, c.s2, K);
+                zrow(c, l, K, wb, c.s2, c.blas2);
                 const int nwin = std::min(wb, NB - 1 - K);
                 if (nwin > 0) {
                     hipLaunchKernelGGL(k_chol_panel, dim3(nwin), dim3(256), 0, c.s, l.Kinv, n, K, NB, wb,
@@ -947,16 +984,14 @@ struct MGH {
                                        NB, wb);
                 }
             }
+            c.order(c.s2, c.s, NB);
             c.check_launch();
             TOPO_HIP(hipMemcpyAsync(&c.h_sc[0], dflag, sizeof(int), hipMemcpyDeviceToHost, c.s));
             c.sync();
             int flag = 0;
             std::memcpy(&flag, &c.h_sc[0], sizeof(int));
             if (!flag) {
-                // Z = L^-1 row block by row block, then Z^T into the Kinv buffer
-                hipLaunchKernelGGL(k_fill<double>, dim3(nblk(n * n)), dim3(256), 0, c.s, l.Z, 0.0, n * n,
-                                   (const int*)nullptr);
-                zrows(c, l, NB, wb);
+                // Z^T into the Kinv buffer
                 hipLaunchKernelGGL((k_transpose<double, double>), dim3(NB, NB), dim3(256), 0, c.s,
                                    (const double*)l.Z, l.Kinv, n);
                 if (l.Z16) {
@@ -989,20 +1024,19 @@ struct MGH {
         c.check_launch();
         l.chol = true;
     }
-    // Z = L^-1 by row blocks: Z_II = L_II^-1 and, for the columns left of
+    // Row block I of Z = L^-1: Z_II = L_II^-1 and, for the columns left of
     // the diagonal block, Z_I,0:I = -L_II^-1 (L_I,band Z_band,0:I), band = the
     // wb row blocks above I. Both products are plain rocBLAS DGEMMs on
     // row-major sub-matrices (a row-major C = A B is the column-major
-    // C^T = B^T A^T). T is scratch of 64 x n.
-    void zrows(Ctx& c, MGLevel& l, int NB, int wb) {
+    // C^T = B^T A^T). ztmp is scratch of 64 x n; the rows above I must be
+    // done on stream `st` before.
+    void zrow(Ctx& c, MGLevel& l, int I, int wb, hipStream_t st, rocblas_handle h) {
         const int64_t n = l.nfp;
-        if (!l.ztmp) l.ztmp = c.alloc<double>(kCB * n);
-        rocblas_handle h = c.blas_handle();
-        for (int I = 0; I < NB; I++) {
+        {
             // Z_II
-            hipLaunchKernelGGL(k_copy_block, dim3(1), dim3(256), 0, c.s, (const double*)l.Linv + (int64_t)I * kCB * kCB,
+            hipLaunchKernelGGL(k_copy_block, dim3(1), dim3(256), 0, st, (const double*)l.Linv + (int64_t)I * kCB * kCB,
                                (int64_t)kCB, l.Z + (int64_t)I * kCB * n + (int64_t)I * kCB, n);
-            if (I == 0) continue;
+            if (I == 0) return;
             const int K0 = std::max(0, I - wb);
             const int kk = (I - K0) * kCB;     // inner dimension
             const int nc = I * kCB;             // columns 0 .. I*64
@@ -1192,8 +1226,14 @@ struct MGH {
         const bool coarsest = li == (int)L.size() - 1;
         if (coarsest && l.chol) {
             hipLaunchKernelGGL(k_zero_fixed_or_all<T>, dim3(nblk(l.ndof), nr), dim3(256), 0, c.s, x, l.ndof, gate);
-            hipLaunchKernelGGL(k_gather_free<T>, dim3(nblk(l.nfp), nr), dim3(256), 0, c.s, b, l.freeIdx, l.nf, l.nfp, l.bf,
-                               gate, l.ndof);
+            // the FP16 coarse copy works on FP32 vectors (float views of bf, ytmp)
+            const bool f16 = std::is_same<T, float>::value && l.Z16 && l.tri;
+            if (f16)
+                hipLaunchKernelGGL((k_gather_free<T, float>), dim3(nblk(l.nfp), nr), dim3(256), 0, c.s, b, l.freeIdx,
+                                   l.nf, l.nfp, (float*)l.bf, gate, l.ndof);
+            else
+                hipLaunchKernelGGL((k_gather_free<T, double>), dim3(nblk(l.nfp), nr), dim3(256), 0, c.s, b, l.freeIdx,
+                                   l.nf, l.nfp, l.bf, gate, l.ndof);
             const dim3 gi((unsigned)((l.nf + 15) / 16));
             const bool f32 = std::is_same<T, float>::value && l.Kinv32;
             if (l.tri) {
@@ -1202,15 +1242,15 @@ struct MGH {
                 // reads the rounded copies (coarse_fp32: FP32, or FP16 with
                 // the row scales applied to ytmp)
                 const dim3 gt((unsigned)(l.nfp / 16));
-                if (std::is_same<T, float>::value && l.Z16) {
-                    TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_apply<T, NR, _Float16>), gt, dim3(256), 0, c.s,
-                                                          (const _Float16*)l.Z16, l.nfp, 0, (const double*)l.bf,
-                                                          l.ytmp, (T*)nullptr, l.freeIdx, l.nf, l.ndof, gate,
-                                                          (const double*)l.zs2));
-                    TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_apply<T, NR, _Float16>), gt, dim3(256), 0, c.s,
-                                                          (const _Float16*)l.Zt16, l.nfp, 1, (const double*)l.ytmp,
-                                                          (double*)nullptr, x, l.freeIdx, l.nf, l.ndof, gate,
-                                                          (const double*)nullptr));
+                if (f16) {
+                    TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_apply<T, NR, _Float16, float, float>), gt, dim3(256),
+                                                          0, c.s, (const _Float16*)l.Z16, l.nfp, 0,
+                                                          (const float*)l.bf, (float*)l.ytmp, (T*)nullptr, l.freeIdx,
+                                                          l.nf, l.ndof, gate, (const double*)l.zs2));
+                    TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_apply<T, NR, _Float16, float, float>), gt, dim3(256),
+                                                          0, c.s, (const _Float16*)l.Zt16, l.nfp, 1,
+                                                          (const float*)l.ytmp, (float*)nullptr, x, l.freeIdx, l.nf,
+                                                          l.ndof, gate, (const double*)nullptr));
                 } else if (f32 && l.Z32) {
                     TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_apply<T, NR, float>), gt, dim3(256), 0, c.s,
                                                           (const float*)l.Z32, l.nfp, 0, (const double*)l.bf, l.ytmp,
@@ -1258,7 +1298,7 @@ struct MGH {
         hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(lc.g.nnodes()), nr), dim3(256), 0, c.s, (const T*)Sel<T>::res(l),
                            Sel<T>::b(lc), l.g, lc.g, lc.fixed, gate);
         T* xc = vcycle<T>(c, li + 1, Sel<T>::b(lc), gate, nr);
-        hipLaunchKernelGGL(k_prolong_add<T>, dim3(nblk(l.g.nnodes()), nr), dim3(256), 0, c.s, x, (const T*)xc, l.g,
+        hipLaunchKernelGGL(k_prolong_add<T>, dim3(nblk(4 * lc.g.nnodes()), nr), dim3(256), 0, c.s, x, (const T*)xc, l.g,
                            lc.g, l.fixed, gate);
         for (int s = 0; s < nu2; s++) {
             apply<T>(c, li, x, x2, 3, b, gate, nr);
