@@ -843,16 +843,36 @@ struct MGH {
             c.check_launch();
         }
         omegaL.assign(L.size(), omega);
-        if (omega_cap > 0)
-            for (size_t li = 0; li < L.size(); li++) {
-                const double rho = power_rho(c, (int)li);
+        const int nl = (int)std::min<size_t>(L.size(), 16);
+        if (omega_cap > 0) {
+            if (!d_psc) d_psc = c.alloc<double>(4 * 16);
+            // the power iterations of every level run on the second stream,
+            // overlapping the coarse factorization (which does not depend on
+            // the smoother weights); their results are read after it
+            c.side_ready(NB_EV_MIN);
+            c.order(c.s, c.s2, 0);
+            std::swap(c.s, c.s2);
+            for (int li = 0; li < nl; li++) power_launch(c, li, d_psc + 4 * li);
+            std::swap(c.s, c.s2);
+            c.check_launch();
+        }
+        if (coarse_solver == TOPO_COARSE_CHOLESKY) factor_coarsest(c);
+        if (omega_cap > 0) {
+            TOPO_HIP(hipStreamSynchronize(c.s2));
+            double h[64];
+            TOPO_HIP(hipMemcpy(h, d_psc, sizeof(double) * 4 * nl, hipMemcpyDeviceToHost));
+            for (int li = 0; li < nl; li++) {
+                // v is unit-norm after the first step, so rho = ||D^-1 A v||
+                const double rho = std::sqrt(h[4 * li + 1]);
                 if (rho > 0) omegaL[li] = std::min(omega, omega_cap / rho);
             }
+        }
         upload_omega(c);
-        if (coarse_solver == TOPO_COARSE_CHOLESKY) factor_coarsest(c);
         if (bddc) bddc_build(c);
     }
 
+    double* d_psc = nullptr;   // power-iteration scratch, 4 doubles per level
+    static constexpr int NB_EV_MIN = 2;
     void upload_omega(Ctx& c) {
         if (!d_omega) d_omega = c.alloc<double>(16);
         double h[16] = {0};
@@ -865,28 +885,26 @@ struct MGH {
     // smoother safeguard: rho ~ lambda_max(D^-1 A) of level li by 20 power
     // iterations from the hashed vector v_i = frac(i*2654435761/2^32) - 1/2 on
     // free DOFs (the oracle's power_rho); FP64.
-    double power_rho(Ctx& c, int li) {
+    // (launches only; sc[1] receives rho^2 for the level, sc 4 doubles)
+    void power_launch(Ctx& c, int li, double* sc) {
         MGLevel& l = L[li];
         hipLaunchKernelGGL(k_power_init, dim3(nblk(l.ndof)), dim3(256), 0, c.s, l.x, (const uint8_t*)l.fixed, l.ndof,
-                           c.dsc + 8);
+                           sc);
         for (int it = 0; it < 20; it++) {
             apply<double>(c, li, l.x, l.x2, 0, nullptr, nullptr);
             hipLaunchKernelGGL(k_power_scale, dim3(kRedBlocks), dim3(kRedThreads), 0, c.s, l.x2,
-                               (const double*)l.invD, l.ndof, c.dsc + 8, c.partials, c.ticket);
+                               (const double*)l.invD, l.ndof, sc, c.partials, c.ticket);
             hipLaunchKernelGGL(k_power_norm, dim3(nblk(l.ndof)), dim3(256), 0, c.s, l.x, (const double*)l.x2, l.ndof,
-                               (const double*)(c.dsc + 8));
+                               (const double*)sc);
         }
-        c.check_launch();
-        // v is unit-norm after the first step, so rho = ||D^-1 A v||
-        return std::sqrt(c.read_d(c.dsc + 9));
     }
 
     // FP64 direct solve of the coarsest level: the free-DOF block is
-    // assembled (deterministically, no atomics), factored by Cholesky
-    // (rocSOLVER dpotrf) and inverted (dpotri) once per density update; the
-    // V-cycle then applies the inverse with one GEMV. A block that is not
-    // positive definite (voided elements left out of the operator make
-    // mechanisms) falls back to Gauss-Jordan with pivot dropping.
+    // assembled (deterministically, no atomics), factored by the banded
+    // blocked Cholesky and Z = L^-1 formed once per density update; the
+    // V-cycle applies x = Z^T (Z b). A block that is not positive definite
+    // (voided elements left out of the operator make mechanisms) falls back
+    // to Gauss-Jordan with pivot dropping and one GEMV with the inverse.
     void coarse_block(Ctx& c, MGLevel& l) {
         const int64_t nf = l.nf, ld = l.nfp;
         hipLaunchKernelGGL(k_fill<double>, dim3(nblk(ld * ld)), dim3(256), 0, c.s, l.Kinv, 0.0, ld * ld,
