@@ -24,11 +24,21 @@ namespace topo {
 constexpr int kCB = 64;           // block size
 constexpr int kCS = kCB + 1;      // LDS row stride (bank spread)
 
-// load a 64 x 64 block (row-major, ld) into LDS with stride kCS
+// load a 64 x 64 block (row-major, ld) into LDS with stride kCS by NT
+// threads; all of a thread's global reads are issued before the LDS writes
+template <int NT>
 __device__ __forceinline__ void blk_load(double* s, const double* A, int64_t ld) {
-    for (int e = threadIdx.x; e < kCB * kCB; e += blockDim.x) {
-        const int r = e / kCB, c = e % kCB;
-        s[r * kCS + c] = A[(int64_t)r * ld + c];
+    constexpr int IT = kCB * kCB / NT;
+    double v[IT];
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+        const int e = (int)threadIdx.x + it * NT;
+        v[it] = A[(int64_t)(e / kCB) * ld + e % kCB];
+    }
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+        const int e = (int)threadIdx.x + it * NT;
+        s[(e / kCB) * kCS + e % kCB] = v[it];
     }
 }
 
@@ -40,41 +50,71 @@ __device__ __forceinline__ double readlane_d(double v, int lane) {
 }
 
 // Cholesky of the diagonal block K and the inverse of its factor, by one
-// wavefront holding the block in registers: lane i owns row i (r[0..63]);
-// at step j the pivot and the column l_kj come from the owning lanes by
-// readlane (uniform operands, no LDS round trips, no dependent chains
-// across steps other than the pivot). The factor goes through LDS to the
-// inverse, which lane c computes for column c by forward substitution.
-// Writes L_KK (upper part zeroed) back to A and L_KK^-1 to Linv[K]. A pivot
-// <= tiny * (original diagonal of that DOF) sets *flag.
+// wavefront on the block in LDS (lane i owns row i). Blocked by 16 columns:
+// per column j of a sub-block every lane reads the pivot, scales its own
+// entry l_ij and updates the rest of the sub-block's columns; then each lane
+// updates its trailing columns k with the sub-block's panel,
+// r[k] -= l_ij l_kj for j ascending (the same operations in the same order
+// as the unblocked elimination, with the panel read as LDS broadcasts). The
+// block load issues all of a lane's reads at once and the pivot thresholds
+// are read once, so no global-memory latency sits inside the elimination.
+// The factor goes through LDS to the inverse, which lane c computes for
+// column c by forward substitution. Writes L_KK (upper part zeroed) back to
+// A and L_KK^-1 to Linv[K]. A pivot <= tiny * (original diagonal of that DOF)
+// sets *flag.
 __global__ void __launch_bounds__(64) k_chol_diag(double* __restrict__ A, int64_t ld, int K,
                                                   double* __restrict__ Linv, const double* __restrict__ d0,
                                                   double tiny, int* flag) {
+    constexpr int SB = 16;           // sub-block
     __shared__ double a[kCB * kCS];
     double* Ab = A + (int64_t)K * kCB * ld + (int64_t)K * kCB;
-    blk_load(a, Ab, ld);
-    __syncthreads();
     const int i = threadIdx.x;
+    // the block: all 64 loads of a lane in flight together
+#pragma unroll 16
+    for (int rw = 0; rw < kCB; rw++) a[rw * kCS + i] = Ab[(int64_t)rw * ld + i];
+    // this lane's pivot threshold, read once
+    const double thr_i = tiny * d0[(int64_t)K * kCB + i];
+    __syncthreads();
+    double* ri = a + i * kCS;        // this lane's row
+#pragma unroll
+    for (int c0 = 0; c0 < kCB; c0 += SB) {
+#pragma unroll
+        for (int j = c0; j < c0 + SB; j++) {
+            const double d = a[j * kCS + j];
+            if (i == j && !(d > thr_i)) *flag = 1;
+            const double ljj = d > 0 ? sqrt(d) : 1.0;
+            const double lij = i == j ? ljj : (i > j ? ri[j] / ljj : 0.0);
+            __syncthreads();                 // every lane has read the pivot
+            ri[j] = lij;
+            __syncthreads();                 // column j scaled
+#pragma unroll
+            for (int k = j + 1; k < c0 + SB; k++) ri[k] -= lij * a[k * kCS + j];
+            __syncthreads();
+        }
+        if (c0 + SB < kCB) {
+            double l[SB];
+#pragma unroll
+            for (int j = 0; j < SB; j++) l[j] = ri[c0 + j];
+#pragma unroll 4
+            for (int k = c0 + SB; k < kCB; k++) {
+                const double* rk = a + k * kCS + c0;
+                double v = ri[k];
+#pragma unroll
+                for (int j = 0; j < SB; j++) v -= l[j] * rk[j];
+                ri[k] = v;
+            }
+            __syncthreads();
+        }
+    }
     double r[kCB];
 #pragma unroll
-    for (int k = 0; k < kCB; k++) r[k] = a[i * kCS + k];
-#pragma unroll
-    for (int j = 0; j < kCB; j++) {
-        const double d = readlane_d(r[j], j);
-        if (i == 0 && !(d > tiny * d0[(int64_t)K * kCB + j])) *flag = 1;
-        const double ljj = d > 0 ? sqrt(d) : 1.0;
-        r[j] = i == j ? ljj : (i > j ? r[j] / ljj : 0.0);
-#pragma unroll
-        for (int k = j + 1; k < kCB; k++) {
-            r[k] -= r[j] * readlane_d(r[j], k);
-            // keep the scheduler from hoisting all readlanes of a step
-            // ahead of their uses (SGPR pressure)
-            if ((k & 7) == 7) __builtin_amdgcn_sched_barrier(0);
-        }
+    for (int k = 0; k < kCB; k++) {
+        const double v = ri[k];
+        r[k] = k <= i ? v : 0.0;
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < kCB; k++) a[i * kCS + k] = k <= i ? r[k] : 0.0;
+    for (int k = 0; k < kCB; k++) ri[k] = r[k];
     __syncthreads();
     // inverse: lane c solves L x = e_c (x_r = 0 for r < c falls out)
     const int c = i;
@@ -116,9 +156,8 @@ __global__ void __launch_bounds__(256) k_chol_panel(double* __restrict__ A, int6
     __shared__ double x[kCB * kCS];
     __shared__ double y[kCB * kCS];
     double* Ab = A + (int64_t)I * kCB * ld + (int64_t)K * kCB;
-    blk_load(x, Ab, ld);
-    const double* Li = Linv + (int64_t)K * kCB * kCB;
-    for (int e = threadIdx.x; e < kCB * kCB; e += blockDim.x) y[(e / kCB) * kCS + e % kCB] = Li[e];
+    blk_load<256>(x, Ab, ld);
+    blk_load<256>(y, Linv + (int64_t)K * kCB * kCB, kCB);
     __syncthreads();
     double acc[4][4] = {};
     blk_mul_nt(x, y, acc);   // (A_IK L^-T)[r][c] = sum_k A_IK[r][k] Linv[c][k]
@@ -136,8 +175,8 @@ __global__ void __launch_bounds__(256) k_chol_update(double* __restrict__ A, int
     if (I >= NB) return;
     __shared__ double x[kCB * kCS];
     __shared__ double y[kCB * kCS];
-    blk_load(x, A + (int64_t)I * kCB * ld + (int64_t)K * kCB, ld);
-    blk_load(y, A + (int64_t)J * kCB * ld + (int64_t)K * kCB, ld);
+    blk_load<256>(x, A + (int64_t)I * kCB * ld + (int64_t)K * kCB, ld);
+    blk_load<256>(y, A + (int64_t)J * kCB * ld + (int64_t)K * kCB, ld);
     __syncthreads();
     double acc[4][4] = {};
     blk_mul_nt(x, y, acc);
