@@ -117,8 +117,12 @@ __host__ __device__ inline size_t xm_lds_bytes(int nnz, size_t elt) {
 }
 
 // MODE as k_applyA_exact: 0 Ap, 1 masked, 2 b - Ap (free) / 0 (fixed),
-// 3 Jacobi sweep x + omega invD (b - Ap) (free) / x (fixed); a template
-// parameter so each variant holds only the registers its epilogue needs
+// 3 Jacobi sweep x + omega invD (b - Ap) (free) / x (fixed); 4 the first two
+// damped-Jacobi sweeps from x = 0 in one pass: the operand x0 = omega invD b
+// is formed from b and invD as the planes are loaded (p is not read), and the
+// output is the mode-3 sweep from x0 (the staged own b, invD and x0 serve the
+// epilogue). A template parameter so each variant holds only the registers
+// its epilogue needs.
 template <typename T, typename TE, int NT, int MODE>
 __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __restrict__ out,
                                                   const TE* __restrict__ E, double thr, Grid g,
@@ -169,30 +173,54 @@ __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __
 
     // register prefetch of one node plane (own node + halo node)
     T pf[3], ph[3];
+    T pfb[3], pfd[3];   // MODE 4: own b and invD of the prefetched plane
     auto fetch = [&](int ix) {
 #pragma unroll
-        for (int d = 0; d < 3; d++) { pf[d] = T(0); ph[d] = T(0); }
+        for (int d = 0; d < 3; d++) { pf[d] = T(0); ph[d] = T(0); pfb[d] = T(0); pfd[d] = T(0); }
         if (ix >= 0 && ix < nnx) {
-            const T* src = p + (int64_t)ix * plane;
-            if (jv) {
+            if constexpr (MODE == 4) {
+                // x0 = 0 + omega invD b, as the separate first sweep forms it
+                const int64_t o = (int64_t)ix * plane;
+                if (jv) {
 #pragma unroll
-                for (int d = 0; d < 3; d++) pf[d] = src[3 * j + d];
-            }
-            if (jhv) {
+                    for (int d = 0; d < 3; d++) {
+                        pfb[d] = bvec[o + 3 * j + d];
+                        pfd[d] = invD[o + 3 * j + d];
+                        pf[d] = T(0) + omega * pfd[d] * pfb[d];
+                    }
+                }
+                if (jhv) {
 #pragma unroll
-                for (int d = 0; d < 3; d++) ph[d] = src[3 * jh + d];
+                    for (int d = 0; d < 3; d++)
+                        ph[d] = T(0) + omega * invD[o + 3 * jh + d] * bvec[o + 3 * jh + d];
+                }
+            } else {
+                const T* src = p + (int64_t)ix * plane;
+                if (jv) {
+#pragma unroll
+                    for (int d = 0; d < 3; d++) pf[d] = src[3 * j + d];
+                }
+                if (jhv) {
+#pragma unroll
+                    for (int d = 0; d < 3; d++) ph[d] = src[3 * jh + d];
+                }
             }
         }
     };
     // own node values of the last staged plane (MODE 3 reads x at the
     // output node from here instead of from memory)
     T stg[3] = {T(0), T(0), T(0)};
+    T stgb[3] = {T(0), T(0), T(0)}, stgd[3] = {T(0), T(0), T(0)};   // MODE 4
     // stage the prefetched plane in LDS, prefetch plane ixn, and return this
     // thread's face transform of the staged plane
     auto stage_face = [&](int ixn, T (&f)[12]) {
         T own[3] = {pf[0], pf[1], pf[2]};
 #pragma unroll
         for (int d = 0; d < 3; d++) stg[d] = own[d];
+        if constexpr (MODE == 4) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) { stgb[d] = pfb[d]; stgd[d] = pfd[d]; }
+        }
 #pragma unroll
         for (int d = 0; d < 3; d++) s_in[3 * t + d] = own[d];
         if (t < H) {
@@ -223,12 +251,16 @@ __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __
         const int64_t obase = (int64_t)ex * plane + 3 * (int64_t)j;
         T eb[3] = {T(0), T(0), T(0)}, ed[3] = {T(0), T(0), T(0)};
         const T ep[3] = {stg[0], stg[1], stg[2]};
+        if constexpr (MODE == 4) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) { eb[d] = stgb[d]; ed[d] = stgd[d]; }
+        }
         uint8_t ef[3] = {0, 0, 0};
         if (outstep && MODE >= 1) {
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 ef[d] = fixed[obase + d];
-                if (MODE >= 2) eb[d] = bvec[obase + d];
+                if (MODE == 2 || MODE == 3) eb[d] = bvec[obase + d];
                 if (MODE == 3) ed[d] = invD[obase + d];
             }
         }
@@ -287,7 +319,7 @@ __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __
                     if (ef[d]) v = T(0);
                 } else if (MODE == 2) {
                     v = ef[d] ? T(0) : eb[d] - v;
-                } else if (MODE == 3) {
+                } else if (MODE == 3 || MODE == 4) {
                     const T r = eb[d] - v;
                     v = ef[d] ? ep[d] : ep[d] + omega * ed[d] * r;
                 }

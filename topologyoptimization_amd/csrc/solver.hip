@@ -1182,11 +1182,24 @@ struct MGH {
         return z;
     }
 
-    // out = A in (mode 0), masked (1), b - A in (2), Jacobi sweep (3)
+    // whether level li's operator is the x-march kernel (mode 4 needs it)
+    bool xm_level(int li) const {
+        const MGLevel& l = L[li];
+        if (l.dense || exact || !K0->had_ok) return false;
+        const int H = l.g.nnz() + 1;
+        const int nt_sel = xm_nt ? xm_nt : (li < (int)lv_nt.size() ? lv_nt[li] : 0);
+        const int NT = (nt_sel == 1024 || (nt_sel == 0 && 4 * H > 512)) ? 1024 : 512;
+        return 2 * H <= NT;
+    }
+
+    // out = A in (mode 0), masked (1), b - A in (2), Jacobi sweep (3), the
+    // first two sweeps from x = 0 (4: `in` unused; x-march levels only)
     template <typename T>
     void apply(Ctx& c, int li, const T* in, T* out, int mode, const T* b, const int* gate, int nr = 1) {
         MGLevel& l = L[li];
-        const double* om = mode == 3 ? omp(li) : nullptr;
+        const double* om = mode >= 3 ? omp(li) : nullptr;
+        TOPO_CHECK(mode != 4 || (xm_level(li) && !(li == 0 && springs.n > 0)), TOPO_E_UNSUPPORTED,
+                   "fused first sweep on a level without the x-march operator");
         if (l.dense) {
             hipLaunchKernelGGL(k_dense_apply<T>, dim3(nblk(l.ndof * 64), nr), dim3(256), 0, c.s, Sel<T>::K(l), in, out, l.ndof,
                                l.fixed, mode, b, Sel<T>::invD(l), om, gate);
@@ -1214,7 +1227,8 @@ struct MGH {
     case 0: TOPO_XM_LAUNCH(NTV, 0); break;                 \
     case 1: TOPO_XM_LAUNCH(NTV, 1); break;                 \
     case 2: TOPO_XM_LAUNCH(NTV, 2); break;                 \
-    default: TOPO_XM_LAUNCH(NTV, 3); break;                \
+    case 3: TOPO_XM_LAUNCH(NTV, 3); break;                 \
+    default: TOPO_XM_LAUNCH(NTV, 4); break;                \
     }
                 if (NT == 512) {
                     TOPO_XM_MODES(512)
@@ -1300,13 +1314,19 @@ struct MGH {
             return x;
         }
         const int nsweep_pre = coarsest ? nu1 + coarse_sweeps : nu1;
-        if (nsweep_pre > 0) {
+        int s0 = 1;
+        if (nsweep_pre >= 2 && xm_level(li) && !(li == 0 && springs.n > 0)) {
+            // the first two sweeps in one pass (x0 = omega invD b formed on the fly)
+            apply<T>(c, li, (const T*)b, x2, 4, b, gate, nr);
+            std::swap(x, x2);
+            s0 = 2;
+        } else if (nsweep_pre > 0) {
             hipLaunchKernelGGL(k_jacobi_first<T>, dim3(nblk(l.ndof), nr), dim3(256), 0, c.s, x, b,
                                (const T*)Sel<T>::invD(l), omp(li), l.ndof, gate);
         } else {
             hipLaunchKernelGGL(k_fill<T>, dim3(nblk(l.ndof), nr), dim3(256), 0, c.s, x, T(0), l.ndof, gate);
         }
-        for (int s = 1; s < nsweep_pre; s++) {
+        for (int s = s0; s < nsweep_pre; s++) {
             apply<T>(c, li, x, x2, 3, b, gate, nr);
             std::swap(x, x2);
         }
