@@ -534,10 +534,13 @@ __global__ void k_cg_check(CGState* st, int nr) {
 // pattern in the iteration loops: there each block's device-scope release
 // (an L2 write-back per block) serialises, ~20 ns per block.
 // Quantities: 0 = p.Ap, 1 = ||r||^2, 2 = r.z; row (k, q) of the partials
-// holds quantity k of right-hand side q for each of the kCgBlocks blocks.
+// holds quantity k of right-hand side q for each block of its producer
+// (kCgBlocks for the vector kernels; p.Ap may also come from the x-march
+// applyA, one partial per workgroup, up to kCgStride).
 constexpr int kCgBlocks = 1024;
+constexpr int kCgStride = 8192;
 constexpr int kCgRows = 4;   // right-hand sides per quantity
-__device__ __forceinline__ int64_t cg_row(int k, int q) { return (int64_t)(k * kCgRows + q) * kCgBlocks; }
+__device__ __forceinline__ int64_t cg_row(int k, int q) { return (int64_t)(k * kCgRows + q) * kCgStride; }
 
 // v[i] -> row (k0 + i / nr, i % nr), this block's column
 template <int NV>
@@ -547,11 +550,12 @@ __device__ __forceinline__ void put_partials(double (&v)[NV], double* part, int 
     if (threadIdx.x == 0)
         for (int i = 0; i < NV; i++) part[cg_row(k0 + i / nr, i % nr) + blockIdx.x] = v[i];
 }
-// tot[i] = sum of row (k0 + i / nr, i % nr) over kCgBlocks columns, in a
+// tot[i] = sum of row (k0 + i / nr, i % nr) over its nb columns, in a
 // fixed order (thread t takes columns t, t + 256, ...; then the block tree);
 // every thread of the block gets the result
 template <int NV>
-__device__ __forceinline__ void sum_partials(const double* part, int k0, int nr, double (&tot)[NV]) {
+__device__ __forceinline__ void sum_partials(const double* part, int k0, int nr, double (&tot)[NV],
+                                             int nb = kCgBlocks) {
     __shared__ double sh[NV * (kRedThreads / 64)];
     __shared__ double res[NV];
     double w[NV];
@@ -559,7 +563,7 @@ __device__ __forceinline__ void sum_partials(const double* part, int k0, int nr,
     for (int i = 0; i < NV; i++) {
         const double* row = part + cg_row(k0 + i / nr, i % nr);
         double s = 0;
-        for (int b = threadIdx.x; b < kCgBlocks; b += kRedThreads) s += row[b];
+        for (int b = threadIdx.x; b < nb; b += kRedThreads) s += row[b];
         w[i] = s;
     }
     block_reduce<NV>(w, sh);
@@ -596,10 +600,10 @@ template <typename T, typename TU, int NR, typename TL>
 __global__ void __launch_bounds__(kRedThreads) k_cgp_update_ur(TU* __restrict__ U, T* __restrict__ r,
                                                               const T* __restrict__ p, const T* __restrict__ Ap,
                                                               int64_t n, const CGState* st, double* part,
-                                                              TL* __restrict__ r_lo) {
+                                                              TL* __restrict__ r_lo, int nb0) {
     if (all_done(st, NR)) return;
     double pAp[NR];
-    sum_partials<NR>(part, 0, NR, pAp);
+    sum_partials<NR>(part, 0, NR, pAp, nb0);
     double v[NR];
 #pragma unroll
     for (int q = 0; q < NR; q++) {
@@ -699,10 +703,18 @@ __global__ void __launch_bounds__(kRedThreads) k_cgp_pupdate(T* __restrict__ p, 
 // the books of one iteration (one block of kRedThreads): alpha, ||r||^2,
 // rz, beta, iteration count, non-finite flag, convergence
 template <int NR>
-__global__ void __launch_bounds__(kRedThreads) k_cgp_finish(CGState* st, const double* part) {
+__global__ void __launch_bounds__(kRedThreads) k_cgp_finish(CGState* st, const double* part, int nb0) {
     if (all_done(st, NR)) return;
-    double tot[3 * NR];
-    sum_partials<3 * NR>(part, 0, NR, tot);
+    double pAp[NR], tot[3 * NR];
+    sum_partials<NR>(part, 0, NR, pAp, nb0);
+    {
+        double rest[2 * NR];
+        sum_partials<2 * NR>(part, 1, NR, rest);
+#pragma unroll
+        for (int q = 0; q < NR; q++) tot[q] = pAp[q];
+#pragma unroll
+        for (int q = 0; q < 2 * NR; q++) tot[NR + q] = rest[q];
+    }
     if (threadIdx.x != 0) return;
     for (int q = 0; q < NR; q++) {
         CGState& s = st[q];

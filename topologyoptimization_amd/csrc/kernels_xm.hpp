@@ -122,14 +122,16 @@ __host__ __device__ inline size_t xm_lds_bytes(int nnz, size_t elt) {
 // is formed from b and invD as the planes are loaded (p is not read), and the
 // output is the mode-3 sweep from x0 (the staged own b, invD and x0 serve the
 // epilogue). A template parameter so each variant holds only the registers
-// its epilogue needs.
+// its epilogue needs. With `part` (MODE 1), each workgroup also leaves its
+// share of p.Ap (own p of each output node times its output, FP64, fixed
+// order) at part[blockIdx.y * kCgStride + blockIdx.x] for the CG update.
 template <typename T, typename TE, int NT, int MODE>
 __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __restrict__ out,
                                                   const TE* __restrict__ E, double thr, Grid g,
                                                   const uint8_t* __restrict__ fixed,
                                                   const T* __restrict__ bvec, const T* __restrict__ invD,
                                                   const double* __restrict__ omp, const int* gate, HadCoef<T> co,
-                                                  int CX, int gx, int gy) {
+                                                  int CX, int gx, int gy, double* __restrict__ part) {
     gate = batch_gate(gate);
     if (gate && *gate) return;
     {
@@ -144,7 +146,12 @@ __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __
     const int nwork = gx * gy;
     const int per = (nwork + 7) / 8;
     const int w = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
-    if (w >= nwork) return;
+    if (w >= nwork) {
+        // padding workgroup: an empty share
+        if (MODE == 1 && part && threadIdx.x == 0) part[(int64_t)blockIdx.y * kCgStride + blockIdx.x] = 0.0;
+        return;
+    }
+    double pap = 0.0;   // this thread's share of p.Ap (MODE 1 with part)
     const int bx = w / gy, by = w - bx * gy;
     const T omega = omp ? (T)*omp : T(0);
     extern __shared__ __align__(16) unsigned char xm_smem[];
@@ -323,8 +330,20 @@ __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __
                     const T r = eb[d] - v;
                     v = ef[d] ? ep[d] : ep[d] + omega * ed[d] * r;
                 }
+                if (MODE == 1) pap += (double)ep[d] * (double)v;
                 out[obase + d] = v;
             }
+        }
+    }
+    if (MODE == 1 && part) {
+        __shared__ double red[NT / 64];
+        for (int off = 32; off > 0; off >>= 1) pap += __shfl_down(pap, off, 64);
+        if ((t & 63) == 0) red[t >> 6] = pap;
+        __syncthreads();
+        if (t == 0) {
+            double sum = 0.0;
+            for (int k = 0; k < NT / 64; k++) sum += red[k];
+            part[(int64_t)blockIdx.y * kCgStride + blockIdx.x] = sum;
         }
     }
 }

@@ -127,7 +127,7 @@ struct Ctx {
         }
         TOPO_HIP(hipGetDevice(&device));
         TOPO_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-        partials = alloc<double>(std::max(32 * kRedBlocks, 3 * kCgRows * kCgBlocks));
+        partials = alloc<double>(std::max(32 * kRedBlocks, 3 * kCgRows * kCgStride));
         ticket = alloc<unsigned>(4);
         st = alloc<CGState>(kMaxRHS);
         oc = alloc<OCState>(1);
@@ -1194,8 +1194,12 @@ struct MGH {
 
     // out = A in (mode 0), masked (1), b - A in (2), Jacobi sweep (3), the
     // first two sweeps from x = 0 (4: `in` unused; x-march levels only)
+    // With pap (mode 1, x-march level, no springs) the launch also leaves its
+    // p.Ap partials there and returns their count (0 when it cannot).
     template <typename T>
-    void apply(Ctx& c, int li, const T* in, T* out, int mode, const T* b, const int* gate, int nr = 1) {
+    int apply(Ctx& c, int li, const T* in, T* out, int mode, const T* b, const int* gate, int nr = 1,
+              double* pap = nullptr) {
+        int npart = 0;
         MGLevel& l = L[li];
         const double* om = mode >= 3 ? omp(li) : nullptr;
         TOPO_CHECK(mode != 4 || (xm_level(li) && !(li == 0 && springs.n > 0)), TOPO_E_UNSUPPORTED,
@@ -1216,12 +1220,15 @@ struct MGH {
                 CX = std::max(2, std::min(32, CX));
                 const int gy = (l.g.nnx() + CX - 1) / CX;
                 const dim3 grid(8 * ((gx * gy + 7) / 8), nr);
+                double* part = (mode == 1 && pap && (int)grid.x <= kCgStride && !(li == 0 && springs.n > 0)) ? pap
+                                                                                                             : nullptr;
+                if (part) npart = (int)grid.x;
                 const size_t lds = NT == 512 ? xm_lds_bytes<512>(l.g.nnz(), sizeof(T))
                                              : xm_lds_bytes<1024>(l.g.nnz(), sizeof(T));
 #define TOPO_XM_LAUNCH(NTV, MV)                                                                                  \
     hipLaunchKernelGGL((k_applyA_xm<T, TE, NTV, MV>), grid, dim3(NTV), lds, c.s, in, out,                      \
                        (const TE*)Sel<T>::E(l), thr, l.g, (const uint8_t*)l.fixed, b, (const T*)Sel<T>::invD(l), \
-                       om, gate, had<T>(), CX, gx, gy)
+                       om, gate, had<T>(), CX, gx, gy, part)
 #define TOPO_XM_MODES(NTV)                                 \
     switch (mode) {                                        \
     case 0: TOPO_XM_LAUNCH(NTV, 0); break;                 \
@@ -1246,6 +1253,7 @@ struct MGH {
         if (li == 0 && springs.n > 0)
             hipLaunchKernelGGL(k_spring<T>, dim3(1, nr), dim3(64), 0, c.s, in, out, springs, mode, (const uint8_t*)l.fixed,
                                (const T*)Sel<T>::invD(l), om, gate, l.ndof);
+        return npart;
     }
 
     // one V-cycle (mgVcycle assembly/index.ts:837-1048) for A x = b from
@@ -1422,7 +1430,8 @@ static CGResult pcg_jacobi(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVe
                                (const T*)p, (const T*)Ap, (const T*)Sel<T>::invD(l), n, c.st, c.partials);
             hipLaunchKernelGGL((k_cgp_pupdate<T, T, 1, false>), dim3(kCgBlocks), dim3(kRedThreads), 0, c.s, p, (const T*)z,
                                (const uint8_t*)nullptr, n, c.st, (const double*)c.partials);
-            hipLaunchKernelGGL(k_cgp_finish<1>, dim3(1), dim3(kRedThreads), 0, c.s, c.st, (const double*)c.partials);
+            hipLaunchKernelGGL(k_cgp_finish<1>, dim3(1), dim3(kRedThreads), 0, c.s, c.st, (const double*)c.partials,
+                               kCgBlocks);
         }
     };
     hipGraphExec_t ex = nullptr;
@@ -1530,14 +1539,18 @@ static CGResult pcg_mg(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVecs& 
         for (int k = 0; k < check; k++) {
             // fence-free iteration: each reduction's partials are summed by
             // the kernel that needs them, k_cgp_finish keeps the books
-            mg.apply<T>(c, 0, p, Ap, 1, nullptr, gate, nr);
-            TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_cgp_pAp<T, NR>), dim3(kCgBlocks), dim3(kRedThreads), 0, c.s,
-                                                   (const T*)p, (const T*)Ap, n, c.st, c.partials));
+            // the applyA leaves the p.Ap partials itself when it can
+            int nb0 = mg.apply<T>(c, 0, p, Ap, 1, nullptr, gate, nr, c.partials);
+            if (nb0 == 0) {
+                TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_cgp_pAp<T, NR>), dim3(kCgBlocks), dim3(kRedThreads), 0, c.s,
+                                                       (const T*)p, (const T*)Ap, n, c.st, c.partials));
+                nb0 = kCgBlocks;
+            }
             // r_lo: the V-cycle's FP32 residual, written by the update itself
             PT* r_lo = std::is_same<T, PT>::value ? nullptr : (PT*)w32.r;
             TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_cgp_update_ur<T, TU, NR, PT>), dim3(kCgBlocks),
                                                    dim3(kRedThreads), 0, c.s, U, r, (const T*)p, (const T*)Ap, n,
-                                                   c.st, c.partials, r_lo));
+                                                   c.st, c.partials, r_lo, nb0));
             // z is never formed: r.z and p = z + beta p read the V-cycle's
             // output, which is zero at the fixed DOFs (the mask of ebeMGPCG
             // :1170-1172 is a no-op on it)
@@ -1549,7 +1562,7 @@ static CGResult pcg_mg(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVecs& 
                                                    c.s, p, zz, (const uint8_t*)l.fixed, n, c.st,
                                                    (const double*)c.partials));
             TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_cgp_finish<NR>), dim3(1), dim3(kRedThreads), 0, c.s, c.st,
-                                                   (const double*)c.partials));
+                                                   (const double*)c.partials, nb0));
         }
     };
     hipGraphExec_t ex = nullptr;
