@@ -34,7 +34,7 @@ extern "C" {
 #define TOPO_E_HIP          -2   /* HIP runtime or launch failure            */
 #define TOPO_E_OOM          -3   /* device allocation failed                 */
 #define TOPO_E_BREAKDOWN    -4   /* non-finite CG scalar (pAp, rz)           */
-#define TOPO_E_UNSTRUCTURED -5   /* edofs is not a structured hex8 grid      */
+#define TOPO_E_UNSTRUCTURED -5   /* edofs is not a structured hex8/quad4 grid */
 #define TOPO_E_NODEVICE     -6   /* no HIP device                            */
 #define TOPO_E_STATE        -7   /* call out of order                        */
 #define TOPO_E_UNSUPPORTED  -8   /* valid request this build does not do     */
@@ -180,7 +180,9 @@ int topo_applyAEbe3D(const double* ke, const int32_t* edofs, const double* evals
 
 /* replaces ebePCG assembly/index.ts:504-733. Returns the CG iteration count
  * (cgMax if not converged), like the export; U is the warm start on input.
- * edofSize must be 24 (the 2-D quad4 path is out of scope). `work` is
+ * edofSize 24: the 3-D hex8 grid above; edofSize 8: the 2-D quad4 grid of
+ * _precomputeEdofs2D (js/optimizer-worker.js:240-268, the use in
+ * tests/test-wasm-ebePCG.js), checked in full like the 3-D one. `work` is
  * ignored (device workspace is owned by the library). */
 int topo_ebePCG(const double* dens, const double* ke, const int32_t* edofs,
                 const double* F, double* U, const int32_t* freedofs, int32_t nel,
@@ -348,6 +350,90 @@ int topo_time_op(topo_solver* s, int32_t op, int32_t prec, int32_t level, int32_
 
 /* cancel flag polled between CG iterations and SIMP steps (thread-safe) */
 int topo_solver_cancel(topo_solver* s);
+
+/* ---- 2-D SIMP path (js/optimizer-worker.js) --------------------------
+ * Bilinear quad4 elements, plane stress, on an nelx x nely grid in the
+ * 2-D worker's numbering: node n = (nely+1)*ix + iy, DOFs 2n (x), 2n+1 (y),
+ * element e = ey + ex*nely (column-major), element DOF order
+ * [n1x n1y n2x n2y (n2+1)x (n2+1)y (n1+1)x (n1+1)y] with n1 = (nely+1)*ex +
+ * ey, n2 = n1 + nely + 1 (_precomputeEdofs2D :240-268). Positions and
+ * directions use the TOPO_FIX_* / TOPO_LOAD_* values above (left, right,
+ * bottom, top; down, up, left, right), DOF masks TOPO_DOF_X / TOPO_DOF_Y. */
+
+/* replaces lk() js/optimizer-worker.js:1732-1767 (+ _flattenKE :240-247):
+ * 8x8 row-major quad4 stiffness, E = 1, unit thickness. */
+int topo_k0_2d(double nu, double* ke64);
+
+/* replaces _precomputeEdofs2D js/optimizer-worker.js:240-268 */
+int topo_edofs_2d(int32_t nelx, int32_t nely, int32_t* edofs /* nel*8 */);
+
+/* replaces getFixedDOFs js/optimizer-worker.js:1885-1921 (left, right,
+ * bottom, top). Sets fixedMask[dof]=1 (does not clear). Returns #DOFs. */
+int topo_fixed_dofs_2d(int32_t nelx, int32_t nely, int32_t position, int32_t dof_mask,
+                       uint8_t* fixedMask /* ndof */);
+
+/* replaces getLoadVector js/optimizer-worker.js:1923-1964. forceVector may be
+ * NULL (otherwise 2 doubles: direction at node (nelx, nely)). Writes F
+ * (float32, zeroed first). */
+int topo_load_vector_2d(int32_t nelx, int32_t nely, int32_t direction, double magnitude,
+                        const double* forceVector, float* F /* ndof */);
+
+/* replaces getConstraintElements (:1966-1986) and getForceElements
+ * (:1988-2014): sets preserved[e]=1 (position < 0 / direction < 0: none). */
+int topo_preserved_elements_2d(int32_t nelx, int32_t nely, int32_t position, int32_t direction,
+                               uint8_t* preserved /* nel */);
+
+/* replaces getFixedDOFsFromPaint (:2016-2041): cells is n x 2 int32 (vx, vy)
+ * of painted voxels; their 4 corner nodes are fixed in the masked DOFs. */
+int topo_fixed_paint_2d(int32_t nelx, int32_t nely, const int32_t* cells, int32_t ncells,
+                        int32_t dof_mask, uint8_t* fixedMask);
+
+/* replaces getLoadVectorFromPaint (:2043-2098): magnitude spread evenly over
+ * the unique corner nodes of the painted cells along dir2 (unit). F is
+ * accumulated. Returns the node count. */
+int topo_load_paint_2d(int32_t nelx, int32_t nely, const int32_t* cells, int32_t ncells,
+                       const double* dir2, double magnitude, float* F);
+
+/* 2-D defaults (the 2-D worker's constants :15-18 and config reads
+ * :361-378, :570-578): E0 1, Emin 1e-9, nu 0.3, CG tolerance schedule
+ * max(1e-8, change*1e-2) on ||r||, at most 2000 CG iterations; cfg->nelz = 1. */
+int topo_config_default_2d(topo_config* cfg, int32_t nelx, int32_t nely);
+
+/* Device-resident 2-D SIMP solver: the loop of optimize() :556-700 (Heaviside
+ * projection with beta continuation, penalty continuation, Jacobi-PCG from a
+ * cold start with the absolute tolerance schedule, energies, sensitivities,
+ * sensitivity filter, OC, change) with every vector in HBM. Uses nelx, nely,
+ * E0, Emin, nu, volfrac, penal, penal_start, continuation_iters,
+ * max_iterations, rmin, use_projection, beta_max, beta_interval, move,
+ * cg_max_iter, cg_tol (> 0: fixed absolute tolerance), skip_factor,
+ * use_graphs of cfg; filter_type must be TOPO_FILTER_SENSITIVITY and nload 1.
+ * fixedMask u8[ndof], F float32[ndof], preserved / voidm u8[nel] or NULL;
+ * x starts at volfrac, preserved -> 1, void -> 0 (:517-553). The AMR
+ * grouping, void filling and manufacturing constraints of the 2-D worker are
+ * not part of this path. */
+typedef struct topo_solver2d topo_solver2d;
+int topo_solver2d_create(const topo_config* cfg, const uint8_t* fixedMask, const float* F,
+                         const uint8_t* preserved, const uint8_t* voidm, topo_solver2d** out);
+int topo_solver2d_destroy(topo_solver2d* s);
+int topo_solver2d_set_design(topo_solver2d* s, const float* x);
+int topo_solver2d_get_design(topo_solver2d* s, float* x);
+/* live config patch (updateConfig :587-595): volfrac and max_iterations act
+ * on the next step; penal changes only the stress metric (the loop keeps the
+ * penalty schedule fixed at its start), rmin is recorded but the filter
+ * keeps the weights prepared at the start, as in the 2-D worker. */
+int topo_solver2d_update(topo_solver2d* s, double penal, double rmin, double volfrac,
+                         int32_t max_iterations);
+/* one SIMP iteration (:602-700); info->max_stress is the mesh builder's
+ * max_e (Emin + x_e^penal (E0 - Emin)) w_e on the new design (:1166-1173) */
+int topo_solver2d_step(topo_solver2d* s, topo_step_info* info);
+/* FEA-only analysis (:384-485): FE at the densities x (float32[nel]) with
+ * penalty penal and tolerance tol (absolute on ||r||; the worker uses 1e-8),
+ * then energies and stress_e = (Emin + x_e^p (E0 - Emin)) w_e. compliance
+ * and max_stress land in info; the fields U, ENERGY, STRESS are kept. */
+int topo_solver2d_fea(topo_solver2d* s, const float* x, double penal, double tol, topo_step_info* info);
+/* TOPO_FIELD_U (double[ndof]), XPHYS, ENERGY, DC, DCN, STRESS (float[nel]) */
+int topo_solver2d_get_field(topo_solver2d* s, int32_t field, void* out);
+int topo_solver2d_cancel(topo_solver2d* s);
 
 #ifdef __cplusplus
 }

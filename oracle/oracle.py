@@ -67,6 +67,15 @@ def lib():
         _lib.or_bddc_solve.argtypes = [C.c_void_p] * 4 + [C.c_int] + [C.c_void_p] * 4 + [C.c_int, C.c_double,
                                                                                             C.c_int]
         _lib.or_bddc_fe.argtypes = [C.c_void_p] * 3 + [C.c_double] * 4 + [C.c_void_p] * 3 + [C.c_int, C.c_double]
+        _lib.or2_k0.argtypes = [C.c_double, C.c_void_p]
+        _lib.or2_edofs.argtypes = [C.c_int, C.c_int, C.c_void_p]
+        _lib.or2_applyA.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 3 + [C.c_int] + [C.c_void_p] * 2
+        _lib.or_ebePCG_ed.argtypes = [C.c_void_p] * 3 + [C.c_int] * 3 + [C.c_void_p] * 3 + [C.c_int] + \
+            [C.c_double] * 3 + [C.c_int, C.c_double]
+        _lib.or2_fe.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p] + [C.c_double] * 4 + [C.c_void_p] * 2 + \
+            [C.c_double, C.c_int, C.c_void_p, C.c_void_p]
+        _lib.or2_sens_filter.argtypes = [C.c_int, C.c_int, C.c_double] + [C.c_void_p] * 3
+        _lib.or2_simp_run.argtypes = [C.c_void_p] * 5 + [C.c_int] + [C.c_void_p] * 5
     return _lib
 
 
@@ -435,3 +444,199 @@ def fb(cfg, fixed, F, x, it=1):
     lam = np.zeros(ndof)
     lib().or_fb(C.byref(cfg), _p(fixed), _p(F), _p(x), it, C.byref(J), C.byref(vol), _p(dJ), _p(dV), _p(U), _p(lam))
     return dict(J=J.value, vol=vol.value, dJ=dJ, dV=dV, U=U, lam=lam)
+
+
+# ---- 2-D path (js/optimizer-worker.js) ----------------------------------------
+# node n = (nely+1) ix + iy, DOFs 2n, 2n+1; element e = ey + ex nely.
+def dims2(nelx, nely):
+    return nelx * nely, 2 * (nelx + 1) * (nely + 1)
+
+
+def k0_2d(nu=0.3):
+    ke = np.zeros(64)
+    lib().or2_k0(C.c_double(nu), _p(ke))
+    return ke.reshape(8, 8)
+
+
+def edofs2(nelx, nely):
+    nel, _ = dims2(nelx, nely)
+    ed = np.zeros(nel * 8, np.int32)
+    lib().or2_edofs(nelx, nely, _p(ed))
+    return ed.reshape(nel, 8)
+
+
+def applyA2(nelx, nely, ke, evals, active, p):
+    ke = np.ascontiguousarray(ke, np.float64).ravel()
+    evals = np.ascontiguousarray(evals, np.float64)
+    active = np.ascontiguousarray(active, np.int32)
+    p = np.ascontiguousarray(p, np.float64)
+    Ap = np.zeros_like(p)
+    lib().or2_applyA(nelx, nely, _p(ke), _p(evals), _p(active), active.size, _p(p), _p(Ap))
+    return Ap
+
+
+def assemble_dense2(nelx, nely, ke, evals):
+    """Dense global K of the 2-D grid (test helper, brute force)."""
+    nel, ndof = dims2(nelx, nely)
+    ed = edofs2(nelx, nely)
+    K = np.zeros((ndof, ndof))
+    for e in range(nel):
+        K[np.ix_(ed[e], ed[e])] += evals[e] * ke
+    return K
+
+
+def ebePCG_ed(dens, ke, ed, F, U, freedofs, Emin, E0, penal, maxIter, tol):
+    """ebePCG assembly/index.ts:504-733 over any connectivity (edofSize =
+    columns of ed); U is the warm start and is overwritten."""
+    dens = np.ascontiguousarray(dens, np.float64)
+    ke = np.ascontiguousarray(ke, np.float64).ravel()
+    ed = np.ascontiguousarray(ed, np.int32)
+    F = np.ascontiguousarray(F, np.float64)
+    freedofs = np.ascontiguousarray(freedofs, np.int32)
+    assert U.dtype == np.float64 and U.flags.c_contiguous
+    return lib().or_ebePCG_ed(_p(dens), _p(ke), _p(ed), ed.shape[0], ed.shape[1], F.size, _p(F), _p(U), _p(freedofs),
+                              freedofs.size, Emin, E0, penal, maxIter, tol)
+
+
+def fe2(nelx, nely, ke, x, penal, fixed, F, tol=1e-8, maxit=2000, E0=1.0, Emin=1e-9, skip=1000.0):
+    """FE + solveCG of the 2-D worker (:1780-1873): returns U, c, iterations."""
+    _, ndof = dims2(nelx, nely)
+    ke = np.ascontiguousarray(ke, np.float64).ravel()
+    x = np.ascontiguousarray(x, np.float32)
+    fixed = np.ascontiguousarray(fixed, np.uint8)
+    F = np.ascontiguousarray(F, np.float32)
+    U = np.zeros(ndof)
+    c = C.c_double()
+    it = lib().or2_fe(nelx, nely, _p(ke), _p(x), penal, E0, Emin, skip, _p(fixed), _p(F), tol, maxit, _p(U),
+                      C.byref(c))
+    return U, c.value, it
+
+
+def sens_filter2(nelx, nely, rmin, x, dc):
+    x = np.ascontiguousarray(x, np.float32)
+    dc = np.ascontiguousarray(dc, np.float32)
+    out = np.zeros(x.size, np.float32)
+    lib().or2_sens_filter(nelx, nely, rmin, _p(x), _p(dc), _p(out))
+    return out
+
+
+def simp_run2(cfg, fixed, F, pres, void, niter, x0):
+    nel, _ = dims2(cfg.nelx, cfg.nely)
+    x = np.ascontiguousarray(x0, np.float32).copy()
+    comp, change, smax = np.zeros(niter), np.zeros(niter), np.zeros(niter)
+    cg = np.zeros(niter, np.int32)
+    fixed = np.ascontiguousarray(fixed, np.uint8)
+    F = np.ascontiguousarray(F, np.float32)
+    pres = None if pres is None else np.ascontiguousarray(pres, np.uint8)
+    void = None if void is None else np.ascontiguousarray(void, np.uint8)
+    n = lib().or2_simp_run(C.byref(cfg), _p(fixed), _p(F), _p(pres), _p(void), niter, _p(x), _p(comp), _p(change),
+                           _p(cg), _p(smax))
+    return dict(iterations=n, x=x, compliance=comp[:n], change=change[:n], cg=cg[:n], max_stress=smax[:n])
+
+
+# boundary conditions of the 2-D worker, restated in Python (small integer
+# work): getFixedDOFs :1885-1921, getLoadVector :1923-1964,
+# getConstraintElements :1966-1986, getForceElements :1988-2014,
+# getFixedDOFsFromPaint :2016-2041, getLoadVectorFromPaint :2043-2098
+def fixed_dofs2(nelx, nely, position, mode="all"):
+    _, ndof = dims2(nelx, nely)
+    mask = np.zeros(ndof, np.uint8)
+
+    def add(n):
+        if mode == "all" or "x" in mode:
+            mask[2 * n] = 1
+        if mode == "all" or "y" in mode:
+            mask[2 * n + 1] = 1
+    if position == "left":
+        for j in range(nely + 1):
+            add(j)
+    elif position == "right":
+        for j in range(nely + 1):
+            add((nely + 1) * nelx + j)
+    elif position == "bottom":
+        for i in range(nelx + 1):
+            add((nely + 1) * i)
+    elif position == "top":
+        for i in range(nelx + 1):
+            add((nely + 1) * i + nely)
+    return mask
+
+
+def load_vector2(nelx, nely, direction, magnitude, force_vector=None):
+    _, ndof = dims2(nelx, nely)
+    F = np.zeros(ndof, np.float32)
+    if force_vector is not None and len(force_vector) >= 2:
+        ln = float(np.sqrt(force_vector[0] ** 2 + force_vector[1] ** 2))
+        if ln > 0:
+            n = (nely + 1) * nelx + nely
+            F[2 * n] = (force_vector[0] / ln) * magnitude
+            F[2 * n + 1] = (force_vector[1] / ln) * magnitude
+        return F
+    if direction == "down":
+        F[2 * ((nely + 1) * nelx + nely) + 1] = -magnitude
+    elif direction == "up":
+        F[2 * ((nely + 1) * nelx) + 1] = magnitude
+    elif direction == "left":
+        F[2 * ((nely + 1) * nelx + nely // 2)] = -magnitude
+    elif direction == "right":
+        F[2 * (nely // 2)] = magnitude
+    return F
+
+
+def preserved2(nelx, nely, position, direction):
+    pres = np.zeros(nelx * nely, np.uint8)
+    if position == "left":
+        pres[[ey for ey in range(nely)]] = 1
+    elif position == "right":
+        pres[[ey + (nelx - 1) * nely for ey in range(nely)]] = 1
+    elif position == "bottom":
+        pres[[ex * nely for ex in range(nelx)]] = 1
+    elif position == "top":
+        pres[[(nely - 1) + ex * nely for ex in range(nelx)]] = 1
+    e = {"down": (nely - 1) + (nelx - 1) * nely, "up": (nelx - 1) * nely, "left": nely // 2 + (nelx - 1) * nely,
+         "right": nely // 2}.get(direction)
+    if e is not None:
+        pres[e] = 1
+    return pres
+
+
+def _paint_nodes2(nelx, nely, cells):
+    order, seen = [], set()
+    for vx, vy in cells:
+        n1 = (nely + 1) * vx + vy
+        n2 = (nely + 1) * (vx + 1) + vy
+        for n in (n1, n2, n2 + 1, n1 + 1):
+            if 0 <= n < (nelx + 1) * (nely + 1) and n not in seen:
+                seen.add(n)
+                order.append(n)
+    return order
+
+
+def fixed_paint2(nelx, nely, cells, mode="all"):
+    _, ndof = dims2(nelx, nely)
+    mask = np.zeros(ndof, np.uint8)
+    for n in _paint_nodes2(nelx, nely, cells):
+        if mode == "all" or "x" in mode:
+            mask[2 * n] = 1
+        if mode == "all" or "y" in mode:
+            mask[2 * n + 1] = 1
+    return mask
+
+
+def load_paint2(nelx, nely, cells, direction="down", magnitude=1.0, force_vector=None):
+    _, ndof = dims2(nelx, nely)
+    F = np.zeros(ndof, np.float32)
+    fx = fy = 0.0
+    if force_vector is not None and len(force_vector) >= 2:
+        ln = float(np.sqrt(force_vector[0] ** 2 + force_vector[1] ** 2))
+        if ln > 0:
+            fx, fy = force_vector[0] / ln, force_vector[1] / ln
+    else:
+        fx, fy = {"up": (0, 1), "left": (-1, 0), "right": (1, 0)}.get(direction, (0, -1))
+    nodes = _paint_nodes2(nelx, nely, cells)
+    if nodes:
+        per = magnitude / len(nodes)
+        for n in nodes:
+            F[2 * n] = np.float32(float(F[2 * n]) + fx * per)
+            F[2 * n + 1] = np.float32(float(F[2 * n + 1]) + fy * per)
+    return F

@@ -1563,4 +1563,303 @@ int or_fb(const topo_config* cfg, const uint8_t* fixed, const float* F32, const 
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// 2-D SIMP path (js/optimizer-worker.js): quad4, node n = (nely+1) ix + iy,
+// DOFs 2n, 2n+1, element e = ey + ex nely.
+//
+// Pinning (tests/test_oracle.py):
+//   or2_k0        symmetry, the 3 rigid-body null modes, positive diagonal,
+//                 equality with a 2x2 Gauss-quadrature plane-stress quad
+//                 (an independent derivation in the test)
+//   or2_edofs     closed-form index formulas
+//   or2_applyA    brute-force assembled dense K, rigid modes, symmetry
+//   or_ebePCG_ed  dense solve on small grids (the reference test's meshes),
+//                 zero force -> zero U, U^T K U = F^T U
+//   or2_fe        dense solve, patch test (uniform tension: u linear),
+//                 cantilever compliance convergence and Timoshenko bound,
+//                 Poisson contraction (tests/test-fea-benchmarks.js)
+//   or2_sens_filter brute-force neighbour sums
+//   or2_simp_run  volume constraint each iteration, compliance decrease,
+//                 the shared OC / Heaviside restatements pinned above
+
+// lk() js/optimizer-worker.js:1732-1767: the eight coefficients k_0..k_7 and
+// the row pattern of the plane-stress bilinear quad, divided by (1 - nu^2)
+int or2_k0(double nu, double* KE) {
+    const double k[8] = {1.0 / 2 - nu / 6,  1.0 / 8 + nu / 8, -1.0 / 4 - nu / 12, -1.0 / 8 + 3 * nu / 8,
+                         -1.0 / 4 + nu / 12, -1.0 / 8 - nu / 8, nu / 6,            1.0 / 8 - 3 * nu / 8};
+    static const int P[8][8] = {{0, 1, 2, 3, 4, 5, 6, 7}, {1, 0, 7, 6, 5, 4, 3, 2}, {2, 7, 0, 5, 6, 3, 4, 1},
+                                {3, 6, 5, 0, 7, 2, 1, 4}, {4, 5, 6, 7, 0, 1, 2, 3}, {5, 4, 3, 2, 1, 0, 7, 6},
+                                {6, 3, 4, 1, 2, 7, 0, 5}, {7, 2, 1, 4, 3, 6, 5, 0}};
+    for (int i = 0; i < 8; i++)
+        for (int j = 0; j < 8; j++) KE[i * 8 + j] = k[P[i][j]] * 1 / (1 - nu * nu);
+    return 0;
+}
+
+// _precomputeEdofs2D :240-268
+int or2_edofs(int nelx, int nely, int32_t* ed) {
+    for (int ely = 0; ely < nely; ely++)
+        for (int elx = 0; elx < nelx; elx++) {
+            const int64_t o = ((int64_t)ely + (int64_t)elx * nely) * 8;
+            const int32_t n1 = (nely + 1) * elx + ely, n2 = (nely + 1) * (elx + 1) + ely;
+            ed[o] = 2 * n1; ed[o + 1] = 2 * n1 + 1; ed[o + 2] = 2 * n2; ed[o + 3] = 2 * n2 + 1;
+            ed[o + 4] = 2 * n2 + 2; ed[o + 5] = 2 * n2 + 3; ed[o + 6] = 2 * n1 + 2; ed[o + 7] = 2 * n1 + 3;
+        }
+    return 0;
+}
+
+// _ebeMatVec :287-315 in full space: Ap = sum over active e (ascending) of
+// E_e K u_e, scattered row by row
+int or2_applyA(int nelx, int nely, const double* ke, const double* evals, const int32_t* active, int cnt,
+               const double* p, double* Ap) {
+    const int64_t nel = (int64_t)nelx * nely, ndof = 2LL * (nelx + 1) * (nely + 1);
+    std::vector<int32_t> ed(nel * 8);
+    or2_edofs(nelx, nely, ed.data());
+    for (int64_t i = 0; i < ndof; i++) Ap[i] = 0.0;
+    for (int a = 0; a < cnt; a++) {
+        const int64_t e = active[a];
+        const double E = evals[e];
+        const int32_t* d = &ed[e * 8];
+        for (int i = 0; i < 8; i++) {
+            double sum = 0;
+            for (int j = 0; j < 8; j++) sum += ke[i * 8 + j] * p[d[j]];
+            Ap[d[i]] += E * sum;
+        }
+    }
+    return 0;
+}
+
+// ebePCG assembly/index.ts:504-733 over an arbitrary connectivity (edofs,
+// edofSize), the way the export runs it: E by pow, active above 1000 Emin,
+// Jacobi diagonal, warm start from U at free DOFs, relative tolerance,
+// cgMax = min(maxIter, nfree). Returns the iteration count.
+int or_ebePCG_ed(const double* dens, const double* ke, const int32_t* edofs, int nel, int edofSize, int ndof,
+                 const double* F, double* U, const int32_t* freedofs, int nfree, double Emin, double E0,
+                 double penal, int maxIter, double tolerance) {
+    const double EPS = 1e-12, dE = E0 - Emin, skip = Emin * 1000.0;
+    dvec ev(nel);
+    std::vector<int32_t> act;
+    for (int e = 0; e < nel; e++) {
+        ev[e] = Emin + std::pow(dens[e], penal) * dE;
+        if (ev[e] > skip) act.push_back(e);
+    }
+    auto apply = [&](const dvec& pf, dvec& af) {
+        std::fill(af.begin(), af.end(), 0.0);
+        for (int32_t e : act) {
+            const int32_t* d = &edofs[(int64_t)e * edofSize];
+            for (int i = 0; i < edofSize; i++) {
+                double s = 0;
+                for (int j = 0; j < edofSize; j++) s += ke[i * edofSize + j] * pf[d[j]];
+                af[d[i]] += ev[e] * s;
+            }
+        }
+    };
+    dvec diag(ndof, 0.0);
+    for (int32_t e : act)
+        for (int i = 0; i < edofSize; i++) diag[edofs[(int64_t)e * edofSize + i]] += ev[e] * ke[i * edofSize + i];
+    dvec invD(nfree), Uf(nfree), r(nfree), z(nfree), pp(nfree), Ap(nfree), pfull(ndof, 0.0), apfull(ndof);
+    for (int i = 0; i < nfree; i++) {
+        const double d = diag[freedofs[i]];
+        invD[i] = d > 1e-30 ? 1.0 / d : 0.0;
+        Uf[i] = U[freedofs[i]];
+        pfull[freedofs[i]] = Uf[i];
+    }
+    apply(pfull, apfull);
+    for (int i = 0; i < nfree; i++) r[i] = F[freedofs[i]] - apfull[freedofs[i]];
+    double rz = 0, r0 = 0;
+    for (int i = 0; i < nfree; i++) {
+        z[i] = invD[i] * r[i];
+        pp[i] = z[i];
+        rz += r[i] * z[i];
+        r0 += r[i] * r[i];
+    }
+    const double tolSq = tolerance * tolerance * (r0 > 1e-30 ? r0 : 1e-30);
+    const int cgMax = maxIter < nfree ? maxIter : nfree;
+    int ret = cgMax;
+    for (int it = 0; it < cgMax; it++) {
+        double rn = 0;
+        for (int i = 0; i < nfree; i++) rn += r[i] * r[i];
+        if (rn < tolSq) { ret = it; break; }
+        std::fill(pfull.begin(), pfull.end(), 0.0);
+        for (int i = 0; i < nfree; i++) pfull[freedofs[i]] = pp[i];
+        apply(pfull, apfull);
+        for (int i = 0; i < nfree; i++) Ap[i] = apfull[freedofs[i]];
+        double pAp = 0;
+        for (int i = 0; i < nfree; i++) pAp += pp[i] * Ap[i];
+        const double alpha = rz / (pAp + EPS);
+        double rzn = 0;
+        for (int i = 0; i < nfree; i++) {
+            Uf[i] += alpha * pp[i];
+            r[i] -= alpha * Ap[i];
+            z[i] = invD[i] * r[i];
+            rzn += r[i] * z[i];
+        }
+        const double beta = rzn / (rz + EPS);
+        for (int i = 0; i < nfree; i++) pp[i] = z[i] + beta * pp[i];
+        rz = rzn;
+    }
+    for (int i = 0; i < ndof; i++) U[i] = 0.0;
+    for (int i = 0; i < nfree; i++) U[freedofs[i]] = Uf[i];
+    return ret;
+}
+
+// FE :1780-1797 with solveCG :1799-1873: E by pow at xPhys, active above
+// skip Emin, Jacobi diagonal on the free DOFs, PCG from U = 0 in the reduced
+// space with the absolute test ||r||^2 < tol^2 at the top of each iteration,
+// maxIter = min(nfree, maxit); c = sum F_i U_i (Float32 F). Returns the
+// iteration count.
+int or2_fe(int nelx, int nely, const double* ke, const float* x, double penal, double E0, double Emin,
+           double skip, const uint8_t* fixed, const float* F, double tol, int maxit, double* U, double* c) {
+    const int64_t nel = (int64_t)nelx * nely, ndof = 2LL * (nelx + 1) * (nely + 1);
+    std::vector<int32_t> ed(nel * 8);
+    or2_edofs(nelx, nely, ed.data());
+    dvec ev(nel);
+    std::vector<int32_t> act;
+    const double thr = skip > 0 ? Emin * skip : -1.0;
+    for (int64_t e = 0; e < nel; e++) {
+        ev[e] = Emin + std::pow((double)x[e], penal) * (E0 - Emin);
+        if (ev[e] > thr) act.push_back((int32_t)e);
+    }
+    std::vector<int32_t> fr;
+    for (int64_t i = 0; i < ndof; i++) if (!fixed[i]) fr.push_back((int32_t)i);
+    const int n = (int)fr.size();
+    dvec diag(ndof, 0.0);
+    for (int32_t e : act)
+        for (int i = 0; i < 8; i++) diag[ed[(int64_t)e * 8 + i]] += ev[e] * ke[i * 8 + i];
+    dvec invD(n), Uf(n, 0.0), r(n), z(n), p(n), Ap(n), pf(ndof), af(ndof);
+    for (int i = 0; i < n; i++) {
+        const double d = diag[fr[i]];
+        invD[i] = d > 1e-30 ? 1.0 / d : 0.0;
+        r[i] = F[fr[i]];
+    }
+    double rz = 0;
+    for (int i = 0; i < n; i++) {
+        z[i] = invD[i] * r[i];
+        p[i] = z[i];
+        rz += r[i] * z[i];
+    }
+    const int maxIter = std::min(n, maxit);
+    const double tolSq = tol * tol;
+    int it = 0;
+    for (; it < maxIter; it++) {
+        double rn = 0;
+        for (int i = 0; i < n; i++) rn += r[i] * r[i];
+        if (rn < tolSq) break;
+        std::fill(pf.begin(), pf.end(), 0.0);
+        for (int i = 0; i < n; i++) pf[fr[i]] = p[i];
+        or2_applyA(nelx, nely, ke, ev.data(), act.data(), (int)act.size(), pf.data(), af.data());
+        for (int i = 0; i < n; i++) Ap[i] = af[fr[i]];
+        double pAp = 0;
+        for (int i = 0; i < n; i++) pAp += p[i] * Ap[i];
+        const double alpha = rz / (pAp + 1e-12);
+        double rzn = 0;
+        for (int i = 0; i < n; i++) {
+            Uf[i] += alpha * p[i];
+            r[i] -= alpha * Ap[i];
+            z[i] = invD[i] * r[i];
+            rzn += r[i] * z[i];
+        }
+        const double beta = rzn / (rz + 1e-12);
+        for (int i = 0; i < n; i++) p[i] = z[i] + beta * p[i];
+        rz = rzn;
+    }
+    for (int64_t i = 0; i < ndof; i++) U[i] = 0.0;
+    for (int i = 0; i < n; i++) U[fr[i]] = Uf[i];
+    double cc = 0;
+    for (int64_t i = 0; i < ndof; i++) cc += F[i] * U[i];
+    if (c) *c = cc;
+    return it;
+}
+
+// prepareFilter :1391-1429 as the reference builds it (explicit iH, jH, sH
+// lists, e1 x-major) and filterSensitivities :1431-1443 (Float32 dcn and Hs)
+int or2_sens_filter(int nelx, int nely, double rmin, const float* x, const float* dc, float* dcn) {
+    std::vector<int64_t> iH, jH;
+    dvec sH;
+    const int r = (int)std::floor(rmin);
+    for (int i = 0; i < nelx; i++)
+        for (int j = 0; j < nely; j++) {
+            const int64_t e1 = (int64_t)i * nely + j;
+            for (int k = std::max(i - r, 0); k <= std::min(i + r, nelx - 1); k++)
+                for (int l = std::max(j - r, 0); l <= std::min(j + r, nely - 1); l++) {
+                    const double dist = std::sqrt((double)((i - k) * (i - k) + (j - l) * (j - l)));
+                    if (dist <= rmin) {
+                        iH.push_back(e1);
+                        jH.push_back((int64_t)k * nely + l);
+                        sH.push_back(std::max(0.0, rmin - dist));
+                    }
+                }
+        }
+    const int64_t nel = (int64_t)nelx * nely;
+    std::vector<float> Hs(nel, 0.f);
+    for (size_t q = 0; q < iH.size(); q++) Hs[iH[q]] = (float)((double)Hs[iH[q]] + sH[q]);
+    for (int64_t e = 0; e < nel; e++) dcn[e] = 0.f;
+    for (size_t q = 0; q < iH.size(); q++)
+        dcn[iH[q]] = (float)((double)dcn[iH[q]] + sH[q] * (double)x[jH[q]] * (double)dc[jH[q]]);
+    for (int64_t e = 0; e < nel; e++) dcn[e] = (float)((double)dcn[e] / ((double)Hs[e] * std::max(1e-3, (double)x[e])));
+    return 0;
+}
+
+// The loop of optimize() :583-700 without AMR, void filling and
+// manufacturing constraints: niter iterations from x_io (or fewer, stopping
+// at change <= 0.01 like :583). Per iteration: compliance, change, CG
+// iterations and the mesh builder's max stress (:1166-1173, with the target
+// penalty and the new design). Returns the iterations run.
+int or2_simp_run(const topo_config* cfg, const uint8_t* fixed, const float* F, const uint8_t* pres,
+                 const uint8_t* voidm, int niter, float* x_io, double* comp_out, double* change_out,
+                 int32_t* cg_out, double* smax_out) {
+    const int nelx = cfg->nelx, nely = cfg->nely;
+    const int64_t nel = (int64_t)nelx * nely, ndof = 2LL * (nelx + 1) * (nely + 1);
+    double ke[64];
+    or2_k0(cfg->nu, ke);
+    std::vector<int32_t> ed(nel * 8);
+    or2_edofs(nelx, nely, ed.data());
+    std::vector<float> x(x_io, x_io + nel), xold(nel), xp(nel), dc(nel), dcn(nel), xnew(nel), w(nel);
+    dvec U(ndof);
+    double change = 1.0;
+    int loop = 0;
+    const double pt = cfg->penal, ps = cfg->penal_start;
+    while (change > 0.01 && loop < niter) {
+        loop++;
+        xold = x;
+        const double tol = cfg->cg_tol > 0 ? cfg->cg_tol : std::max(1e-8, change * 1e-2);
+        const double penal = ps + (pt - ps) * std::min(1.0, (double)(loop - 1) / cfg->continuation_iters);
+        const int proj = cfg->use_projection != 0;
+        const double beta = proj ? std::min(cfg->beta_max, std::pow(2.0, std::floor((double)(loop - 1) / cfg->beta_interval))) : 1.0;
+        if (proj) or_heaviside(x.data(), nel, beta, 0.5, xp.data());
+        else xp = x;
+        double c = 0;
+        const int it = or2_fe(nelx, nely, ke, xp.data(), penal, cfg->E0, cfg->Emin, cfg->skip_factor, fixed, F, tol,
+                              cfg->cg_max_iter, U.data(), &c);
+        const double denom = 2 * std::tanh(beta * 0.5);
+        for (int64_t e = 0; e < nel; e++) {
+            double Ue[8];
+            for (int i = 0; i < 8; i++) Ue[i] = U[ed[e * 8 + i]];
+            double en = 0;
+            for (int i = 0; i < 8; i++)
+                for (int j = 0; j < 8; j++) en += Ue[i] * ke[i * 8 + j] * Ue[j];
+            w[e] = (float)en;
+            const double dphys = -penal * std::pow((double)xp[e], penal - 1) * cfg->E0 * en;
+            const double th = std::tanh(beta * ((double)x[e] - 0.5));
+            const double dd = proj ? beta * (1 - th * th) / denom : 1.0;
+            dc[e] = (float)(dphys * dd);
+        }
+        or2_sens_filter(nelx, nely, cfg->rmin, x.data(), dc.data(), dcn.data());
+        or_oc(nel, x.data(), cfg->volfrac, dcn.data(), pres, voidm, cfg->move, xnew.data());
+        change = 0;
+        for (int64_t e = 0; e < nel; e++) change = std::max(change, (double)std::fabs((double)xnew[e] - (double)xold[e]));
+        x = xnew;
+        double smax = 0;
+        for (int64_t e = 0; e < nel; e++) {
+            const float s = (float)((cfg->Emin + std::pow((double)x[e], cfg->penal) * (cfg->E0 - cfg->Emin)) * (double)w[e]);
+            if (s > smax) smax = s;
+        }
+        if (comp_out) comp_out[loop - 1] = c;
+        if (change_out) change_out[loop - 1] = change;
+        if (cg_out) cg_out[loop - 1] = it;
+        if (smax_out) smax_out[loop - 1] = smax;
+    }
+    std::copy(x.begin(), x.end(), x_io);
+    return loop;
+}
+
 }  // extern "C"

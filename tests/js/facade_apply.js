@@ -65,4 +65,20 @@ if (inp.bddc) {
         fxPtr, ivPtr, f2, u2, ndof, B.maxIter, B.tol, 3, 0);
     bddc = { iters: it2, U: Array.from(new Float64Array(wasm.memory.buffer, u2, ndof)) };
 }
-process.stdout.write(JSON.stringify({ Ap, iters, U, activeCount: cnt, bddc }) + '\n');
+// ebePCG on the 2-D quad4 grid (edofSize 8, the use of
+// tests/test-wasm-ebePCG.js) when argv[2].twod = {nelx, nely, dens, F, free}
+let twod = null;
+if (inp.twod) {
+    const D = inp.twod;
+    const nel2 = D.nelx * D.nely, ndof2 = 2 * (D.nelx + 1) * (D.nely + 1);
+    const k2 = alloc(64 * 8), ed2 = alloc(nel2 * 8 * 4), d2 = alloc(nel2 * 8), f2 = alloc(ndof2 * 8);
+    const u2 = alloc(ndof2 * 8), fr2 = alloc(D.free.length * 4);
+    addon.k0_2d(0.3, new Float64Array(wasm.memory.buffer, k2, 64));
+    addon.edofs2d(D.nelx, D.nely, new Int32Array(wasm.memory.buffer, ed2, nel2 * 8));
+    new Float64Array(wasm.memory.buffer, d2, nel2).set(D.dens);
+    new Float64Array(wasm.memory.buffer, f2, ndof2).set(D.F);
+    new Int32Array(wasm.memory.buffer, fr2, D.free.length).set(D.free);
+    const it = wasm.ebePCG(d2, k2, ed2, f2, u2, fr2, nel2, 8, ndof2, D.free.length, 1e-9, 1.0, 3.0, 2000, 1e-8, 0);
+    twod = { iters: it, U: Array.from(new Float64Array(wasm.memory.buffer, u2, ndof2)) };
+}
+process.stdout.write(JSON.stringify({ Ap, iters, U, activeCount: cnt, bddc, twod }) + '\n');

@@ -311,3 +311,120 @@ def test_topology_solver_volumetric_snapshots(gpu):
     arg["actions"] = {}
     out = run_js("run_optimize.js", arg)
     assert out["ok"] and out["volumetric"] == [["complete", None]]
+
+
+# ---- 2-D worker (js/optimizer-worker.js) -------------------------------------
+@pytest.mark.parametrize("pos,dofs,direction,mag", [("left", "all", "down", 1.0), ("right", "x", "up", 2.0),
+                                                     ("bottom", "y", "left", -1.5), ("top", "all", "right", 0.5),
+                                                     ("bottom-corners", "all", "top-center", 1.0)])
+def test_2d_preset_problem_matches_oracle(oracle, pos, dofs, direction, mag):
+    nx, ny = 7, 4
+    nel, ndof = T.dims2(nx, ny)
+    out = run_js("problem_dump.js", {"twod": True, "model": {"nx": nx, "ny": ny, "nz": 1},
+                                     "config": {"constraintPosition": pos, "constraintDOFs": dofs,
+                                                "forceDirection": direction, "forceMagnitude": mag,
+                                                "volumeFraction": 0.3, "penaltyFactor": 3, "filterRadius": 1.5,
+                                                "maxIterations": 90}})
+    assert np.array_equal(_mask(out["fixed"], ndof), oracle.fixed_dofs2(nx, ny, pos, dofs))
+    assert np.array_equal(_dense_F(out["F"], ndof), oracle.load_vector2(nx, ny, direction, mag))
+    assert np.array_equal(_mask(out["preserved"], nel), oracle.preserved2(nx, ny, pos, direction))
+    c = out["config"]
+    assert (c["nelx"], c["nely"], c["volfrac"], c["penal"], c["rmin"], c["max_iterations"]) == (nx, ny, 0.3, 3, 1.5, 90)
+    # penalStart / continuationIters defaults of js/optimizer-worker.js:570-572
+    assert c["penal_start"] == 1.5 and c["continuation_iters"] == 30
+    assert c["cg_max_iter"] == 2000 and c["cg_tol"] == 0
+
+
+def test_2d_painted_problem_matches_oracle(oracle):
+    nx, ny = 6, 5
+    nel, ndof = T.dims2(nx, ny)
+    pc = ["0,0,0,4", "0,1", "0,4,2", "bad", "9,9"]
+    pf = ["5,4,0,1", "5,3"]
+    keep = ["2,2", "-1,2"]
+    for fv in (None, [1.0, -2.0]):
+        cfg = {"constraintPosition": "left", "constraintDOFs": "x", "forceDirection": "right", "forceMagnitude": 2.5,
+               "paintedConstraints": pc, "paintedForces": pf, "paintedKeep": keep, "volumeFraction": 0.4}
+        if fv:
+            cfg["forceVector"] = fv
+        out = run_js("problem_dump.js", {"twod": True, "model": {"nx": nx, "ny": ny, "nz": 1}, "config": cfg})
+        cells = lambda keys: [tuple(int(v) for v in k.split(",")[:2]) for k in keys
+                              if len(k.split(",")) >= 2 and all(p.lstrip("-").isdigit() for p in k.split(",")[:2])]
+        assert np.array_equal(_mask(out["fixed"], ndof), oracle.fixed_paint2(nx, ny, cells(pc), "x"))
+        assert np.array_equal(_dense_F(out["F"], ndof), oracle.load_paint2(nx, ny, cells(pf), "right", 2.5, fv))
+        pres = np.zeros(nel, np.uint8)
+        for vx, vy in cells(pc + pf + keep):
+            if 0 <= vx < nx and 0 <= vy < ny:
+                pres[vy + vx * ny] = 1
+        assert np.array_equal(_mask(out["preserved"], nel), pres)
+
+
+def _2d_js_config(nx, ny, it):
+    return {"model": {"nx": nx, "ny": ny, "nz": 1},
+            "config": {"solver": "auto", "volumeFraction": 0.4, "penaltyFactor": 3, "filterRadius": 1.5,
+                       "maxIterations": it, "constraintPosition": "left", "forceDirection": "down",
+                       "forceMagnitude": 1.0}}
+
+
+@pytest.mark.gpu
+def test_topology_solver_2d_matches_oracle(gpu, oracle):
+    """TopologySolver.optimize on a 2-D model ('auto' with nz = 1 runs the
+    2-D worker) against the oracle's 2-D loop: per-iteration compliance and
+    the final densities (row-major, as the reference returns them)."""
+    nx, ny, it = 30, 10, 8
+    out = run_js("run_optimize.js", _2d_js_config(nx, ny, it))
+    assert out["ok"], out
+    cfg = T.default_config_2d(nx, ny, volfrac=0.4, max_iterations=it, penal_start=1.5, continuation_iters=20)
+    fixed = oracle.fixed_dofs2(nx, ny, "left")
+    F = oracle.load_vector2(nx, ny, "down", 1.0)
+    pres = oracle.preserved2(nx, ny, "left", "down")
+    x0 = np.full(nx * ny, 0.4, np.float32)
+    x0[pres > 0] = 1
+    ref = oracle.simp_run2(cfg, fixed, F, pres, None, it, x0)
+    assert out["iterations"] == ref["iterations"]
+    comp = np.array(out["compliance"])
+    assert np.abs(comp - ref["compliance"]).max() <= 1e-5 * np.abs(ref["compliance"]).max()
+    d = np.array(out["densities"], np.float32)
+    xr = ref["x"].reshape(nx, ny).T.ravel()      # e = ey + ex*nely -> row-major ex + ey*nx
+    assert d.size == nx * ny and np.abs(d - xr).max() <= 1e-3
+
+
+@pytest.mark.gpu
+def test_topology_solver_2d_fea_only(gpu, oracle):
+    nx, ny = 20, 8
+    arg = _2d_js_config(nx, ny, 1)
+    arg["config"]["solver"] = "fea"
+    el = (np.random.default_rng(2).random(nx * ny) > 0.3).astype(float)
+    arg["model"]["elements"] = el.tolist()
+    out = run_js("run_optimize.js", arg)
+    assert out["ok"] and out["feaOnly"], out
+    x = np.zeros(nx * ny, np.float32)
+    for ex in range(nx):
+        for ey in range(ny):
+            x[ey + ex * ny] = 1.0 if el[ex + ey * nx] > 0.5 else 1e-9
+    U, c, _ = oracle.fe2(nx, ny, oracle.k0_2d(0.3), x, 3.0, oracle.fixed_dofs2(nx, ny, "left"),
+                         oracle.load_vector2(nx, ny, "down", 1.0), tol=1e-8)
+    assert abs(out["finalCompliance"] - c) <= 1e-6 * abs(c)
+    assert len(out["elementStress"]) == nx * ny and abs(out["maxStress"] - max(out["elementStress"])) == 0
+
+
+@pytest.mark.gpu
+def test_wasm_facade_ebePCG_2d(gpu, oracle):
+    """ebePCG with edofSize 8 by byte offset on the facade's memory (the
+    2-D use of tests/test-wasm-ebePCG.js, graded densities of its Test 4)."""
+    nelx, nely = 10, 10
+    nel, ndof = T.dims2(nelx, nely)
+    fixed = oracle.fixed_dofs2(nelx, nely, "left")
+    free = np.flatnonzero(fixed == 0).astype(np.int32)
+    F = np.zeros(ndof)
+    F[2 * ((nely + 1) * nelx + nely) + 1] = -1.0
+    dens = 0.2 + 0.8 * (np.arange(nel) / nel)
+    n = 2
+    n3, d3 = T.dims(n, n, n)
+    out = run_js("facade_apply.js", {"n": n, "evals": [1.0] * n3, "p": [0.0] * d3, "dens": [1.0] * n3,
+                                     "F": [0.0] * d3, "free": list(range(d3)),
+                                     "twod": {"nelx": nelx, "nely": nely, "dens": dens.tolist(), "F": F.tolist(),
+                                              "free": free.tolist()}})
+    U = np.zeros(ndof)
+    it = oracle.ebePCG_ed(dens, oracle.k0_2d(0.3), oracle.edofs2(nelx, nely), F, U, free, 1e-9, 1.0, 3.0, 2000, 1e-8)
+    assert out["twod"]["iters"] == it
+    assert np.abs(np.array(out["twod"]["U"]) - U).max() <= 1e-8

@@ -98,6 +98,10 @@ SYMBOLS = [
     "topo_solver_step", "topo_solver_get_field", "topo_solver_fe", "topo_solver_set_density", "topo_op_applyA",
     "topo_op_vcycle", "topo_op_energy", "topo_op_filter", "topo_op_filter_adjoint", "topo_op_oc", "topo_time_op",
     "topo_solver_cancel", "topo_solver_forward", "topo_solver_backward", "topo_op_helmholtz", "topo_solver_tuning",
+    "topo_k0_2d", "topo_edofs_2d", "topo_fixed_dofs_2d", "topo_load_vector_2d", "topo_preserved_elements_2d",
+    "topo_fixed_paint_2d", "topo_load_paint_2d", "topo_config_default_2d", "topo_solver2d_create",
+    "topo_solver2d_destroy", "topo_solver2d_set_design", "topo_solver2d_get_design", "topo_solver2d_update",
+    "topo_solver2d_step", "topo_solver2d_fea", "topo_solver2d_get_field", "topo_solver2d_cancel",
 ]
 
 
@@ -147,6 +151,23 @@ def lib():
     L.topo_solver_backward.argtypes = [vp, vp, vp]
     L.topo_op_helmholtz.argtypes = [vp, vp, vp, vp]
     L.topo_solver_tuning.argtypes = [vp, vp, vp, vp, vp, i32]
+    L.topo_k0_2d.argtypes = [dbl, vp]
+    L.topo_edofs_2d.argtypes = [i32, i32, vp]
+    L.topo_fixed_dofs_2d.argtypes = [i32, i32, i32, i32, vp]
+    L.topo_load_vector_2d.argtypes = [i32, i32, i32, dbl, vp, vp]
+    L.topo_preserved_elements_2d.argtypes = [i32, i32, i32, i32, vp]
+    L.topo_fixed_paint_2d.argtypes = [i32, i32, vp, i32, i32, vp]
+    L.topo_load_paint_2d.argtypes = [i32, i32, vp, i32, vp, dbl, vp]
+    L.topo_config_default_2d.argtypes = [vp, i32, i32]
+    L.topo_solver2d_create.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.topo_solver2d_destroy.argtypes = [vp]
+    L.topo_solver2d_set_design.argtypes = [vp, vp]
+    L.topo_solver2d_get_design.argtypes = [vp, vp]
+    L.topo_solver2d_update.argtypes = [vp, dbl, dbl, dbl, i32]
+    L.topo_solver2d_step.argtypes = [vp, vp]
+    L.topo_solver2d_fea.argtypes = [vp, vp, dbl, dbl, vp]
+    L.topo_solver2d_get_field.argtypes = [vp, i32, vp]
+    L.topo_solver2d_cancel.argtypes = [vp]
     _lib = L
     return L
 
@@ -239,13 +260,16 @@ def applyAEbe3D(ke, edofs_arr, evals, active, p):
 
 
 def ebePCG(dens, ke, edofs_arr, F, U, freedofs, Emin, E0, penal, maxIter, tol):
+    """ebePCG twin; the element DOF count (24 hex8, 8 quad4) is taken from
+    ke (24x24 or 8x8)."""
     dens = np.ascontiguousarray(dens, np.float64)
     ke = np.ascontiguousarray(ke, np.float64).ravel()
+    esz = int(round(np.sqrt(ke.size)))
     ed = np.ascontiguousarray(edofs_arr, np.int32).ravel()
     F = np.ascontiguousarray(F, np.float64)
     freedofs = np.ascontiguousarray(freedofs, np.int32)
     assert U.dtype == np.float64 and U.flags.c_contiguous
-    return _check(lib().topo_ebePCG(_p(dens), _p(ke), _p(ed), _p(F), _p(U), _p(freedofs), dens.size, 24, F.size,
+    return _check(lib().topo_ebePCG(_p(dens), _p(ke), _p(ed), _p(F), _p(U), _p(freedofs), dens.size, esz, F.size,
                                     freedofs.size, Emin, E0, penal, maxIter, tol, None))
 
 
@@ -442,3 +466,127 @@ class Solver:
         it = C.c_int32()
         _check(lib().topo_op_helmholtz(self.h, _p(v), _p(out), C.byref(it)))
         return out, it.value
+
+
+# ---- 2-D path (js/optimizer-worker.js) ---------------------------------------
+def dims2(nelx, nely):
+    return nelx * nely, 2 * (nelx + 1) * (nely + 1)
+
+
+def default_config_2d(nelx, nely, **kw):
+    c = Config()
+    _check(lib().topo_config_default_2d(C.byref(c), nelx, nely))
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+def k0_2d(nu=0.3):
+    ke = np.zeros(64)
+    _check(lib().topo_k0_2d(nu, _p(ke)))
+    return ke.reshape(8, 8)
+
+
+def edofs_2d(nelx, nely):
+    ed = np.zeros(nelx * nely * 8, np.int32)
+    _check(lib().topo_edofs_2d(nelx, nely, _p(ed)))
+    return ed.reshape(-1, 8)
+
+
+def fixed_dofs_2d(nelx, nely, position, dof_mask=DOF_ALL, mask=None):
+    _, ndof = dims2(nelx, nely)
+    mask = np.zeros(ndof, np.uint8) if mask is None else mask
+    _check(lib().topo_fixed_dofs_2d(nelx, nely, position, dof_mask, _p(mask)))
+    return mask
+
+
+def load_vector_2d(nelx, nely, direction, magnitude=1.0, force_vector=None):
+    _, ndof = dims2(nelx, nely)
+    F = np.zeros(ndof, np.float32)
+    fv = None if force_vector is None else np.ascontiguousarray(force_vector, np.float64)
+    _check(lib().topo_load_vector_2d(nelx, nely, direction, magnitude, _p(fv), _p(F)))
+    return F
+
+
+def preserved_elements_2d(nelx, nely, position, direction):
+    pres = np.zeros(nelx * nely, np.uint8)
+    _check(lib().topo_preserved_elements_2d(nelx, nely, position, direction, _p(pres)))
+    return pres
+
+
+def fixed_paint_2d(nelx, nely, cells, dof_mask=DOF_ALL, mask=None):
+    _, ndof = dims2(nelx, nely)
+    mask = np.zeros(ndof, np.uint8) if mask is None else mask
+    cells = np.ascontiguousarray(cells, np.int32).reshape(-1, 2)
+    _check(lib().topo_fixed_paint_2d(nelx, nely, _p(cells), cells.shape[0], dof_mask, _p(mask)))
+    return mask
+
+
+def load_paint_2d(nelx, nely, cells, direction2, magnitude, F=None):
+    _, ndof = dims2(nelx, nely)
+    F = np.zeros(ndof, np.float32) if F is None else F
+    cells = np.ascontiguousarray(cells, np.int32).reshape(-1, 2)
+    d = np.ascontiguousarray(direction2, np.float64)
+    _check(lib().topo_load_paint_2d(nelx, nely, _p(cells), cells.shape[0], _p(d), magnitude, _p(F)))
+    return F
+
+
+class Solver2D:
+    """Handle of topo_solver2d (device-resident 2-D SIMP state)."""
+
+    def __init__(self, cfg, fixed, F, preserved=None, void=None):
+        self.cfg = cfg.copy()
+        self.nel, self.ndof = dims2(cfg.nelx, cfg.nely)
+        fixed = np.ascontiguousarray(fixed, np.uint8)
+        F = np.ascontiguousarray(F, np.float32)
+        assert fixed.size == self.ndof and F.size == self.ndof
+        pres = None if preserved is None else np.ascontiguousarray(preserved, np.uint8)
+        vd = None if void is None else np.ascontiguousarray(void, np.uint8)
+        h = C.c_void_p()
+        _check(lib().topo_solver2d_create(C.byref(self.cfg), _p(fixed), _p(F), _p(pres), _p(vd), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().topo_solver2d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_design(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        _check(lib().topo_solver2d_set_design(self.h, _p(x)))
+
+    def design(self):
+        x = np.zeros(self.nel, np.float32)
+        _check(lib().topo_solver2d_get_design(self.h, _p(x)))
+        return x
+
+    def update(self, penal=0.0, rmin=0.0, volfrac=0.0, max_iterations=0):
+        _check(lib().topo_solver2d_update(self.h, penal, rmin, volfrac, max_iterations))
+
+    def step(self):
+        info = StepInfo()
+        _check(lib().topo_solver2d_step(self.h, C.byref(info)))
+        return info
+
+    def fea(self, x, penal=3.0, tol=1e-8):
+        x = np.ascontiguousarray(x, np.float32)
+        info = StepInfo()
+        _check(lib().topo_solver2d_fea(self.h, _p(x), penal, tol, C.byref(info)))
+        return info
+
+    def field(self, which):
+        out = np.zeros(self.ndof) if which == FIELD_U else np.zeros(self.nel, np.float32)
+        _check(lib().topo_solver2d_get_field(self.h, which, _p(out)))
+        return out
+
+    def cancel(self):
+        lib().topo_solver2d_cancel(self.h)

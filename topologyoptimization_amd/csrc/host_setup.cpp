@@ -380,4 +380,181 @@ int topo_fixed_paint(int32_t nelx, int32_t nely, int32_t nelz, const int32_t* fa
     return cnt;
 }
 
+// ---------------------------------------------------------------------------
+// 2-D path (js/optimizer-worker.js): node n = (nely+1) ix + iy, element
+// e = ey + ex nely.
+
+// lk() :1732-1767: the plane-stress bilinear quad stiffness of the classic
+// 88-line code, K = k[P(i,j)] / (1 - nu^2) with the eight entries below.
+// Rows 4..7 repeat rows 0..3 with the two node pairs exchanged
+// (P(i+4, j) = P(i, (j+4) mod 8)).
+int topo_k0_2d(double nu, double* ke) {
+    if (!ke || !(nu > -1.0 && nu < 0.5)) {
+        set_error("topo_k0_2d: bad arguments");
+        return TOPO_E_ARG;
+    }
+    const double k[8] = {1.0 / 2 - nu / 6,  1.0 / 8 + nu / 8, -1.0 / 4 - nu / 12, -1.0 / 8 + 3 * nu / 8,
+                         -1.0 / 4 + nu / 12, -1.0 / 8 - nu / 8, nu / 6,            1.0 / 8 - 3 * nu / 8};
+    static const int P[4][8] = {{0, 1, 2, 3, 4, 5, 6, 7}, {1, 0, 7, 6, 5, 4, 3, 2},
+                                {2, 7, 0, 5, 6, 3, 4, 1}, {3, 6, 5, 0, 7, 2, 1, 4}};
+    const double den = 1 - nu * nu;
+    for (int i = 0; i < 8; i++)
+        for (int j = 0; j < 8; j++) {
+            const int pij = i < 4 ? P[i][j] : P[i - 4][(j + 4) % 8];
+            ke[i * 8 + j] = k[pij] * 1 / den;
+        }
+    return TOPO_OK;
+}
+
+int topo_edofs_2d(int32_t nelx, int32_t nely, int32_t* ed) {
+    if (!ed || nelx < 1 || nely < 1) {
+        set_error("topo_edofs_2d: bad arguments");
+        return TOPO_E_ARG;
+    }
+    for (int ex = 0; ex < nelx; ex++)
+        for (int ey = 0; ey < nely; ey++) {
+            const int64_t e = (int64_t)ey + (int64_t)ex * nely;
+            const int32_t n1 = (nely + 1) * ex + ey, n2 = (nely + 1) * (ex + 1) + ey;
+            const int32_t v[8] = {2 * n1, 2 * n1 + 1, 2 * n2, 2 * n2 + 1, 2 * n2 + 2, 2 * n2 + 3, 2 * n1 + 2, 2 * n1 + 3};
+            for (int i = 0; i < 8; i++) ed[e * 8 + i] = v[i];
+        }
+    return TOPO_OK;
+}
+
+int topo_fixed_dofs_2d(int32_t nelx, int32_t nely, int32_t position, int32_t dof_mask, uint8_t* mask) {
+    if (!mask || nelx < 1 || nely < 1) {
+        set_error("topo_fixed_dofs_2d: bad arguments");
+        return TOPO_E_ARG;
+    }
+    int cnt = 0;
+    auto add = [&](int64_t n) {
+        for (int d = 0; d < 2; d++)
+            if (dof_mask & (1 << d)) {
+                mask[2 * n + d] = 1;
+                cnt++;
+            }
+    };
+    const int64_t nny = nely + 1;
+    switch (position) {
+    case TOPO_FIX_LEFT: for (int j = 0; j <= nely; j++) add(j); break;
+    case TOPO_FIX_RIGHT: for (int j = 0; j <= nely; j++) add(nny * nelx + j); break;
+    case TOPO_FIX_BOTTOM: for (int i = 0; i <= nelx; i++) add(nny * i); break;
+    case TOPO_FIX_TOP: for (int i = 0; i <= nelx; i++) add(nny * i + nely); break;
+    case TOPO_FIX_NONE: break;
+    default:
+        set_error("topo_fixed_dofs_2d: position %d is not a 2-D edge", position);
+        return TOPO_E_ARG;
+    }
+    return cnt;
+}
+
+int topo_load_vector_2d(int32_t nelx, int32_t nely, int32_t direction, double mag, const double* fv, float* F) {
+    if (!F || nelx < 1 || nely < 1) {
+        set_error("topo_load_vector_2d: bad arguments");
+        return TOPO_E_ARG;
+    }
+    const int64_t nny = nely + 1;
+    std::memset(F, 0, 2 * nny * (nelx + 1) * sizeof(float));
+    if (fv) {
+        const double len = std::sqrt(fv[0] * fv[0] + fv[1] * fv[1]);
+        if (len > 0) {
+            const int64_t n = nny * nelx + nely;
+            F[2 * n] = (float)((fv[0] / len) * mag);
+            F[2 * n + 1] = (float)((fv[1] / len) * mag);
+        }
+        return TOPO_OK;
+    }
+    switch (direction) {
+    case TOPO_LOAD_DOWN: F[2 * (nny * nelx + nely) + 1] = (float)-mag; break;
+    case TOPO_LOAD_UP: F[2 * (nny * nelx) + 1] = (float)mag; break;
+    case TOPO_LOAD_LEFT: F[2 * (nny * nelx + nely / 2)] = (float)-mag; break;
+    case TOPO_LOAD_RIGHT: F[2 * (nely / 2)] = (float)mag; break;
+    case TOPO_LOAD_NONE: break;
+    default:
+        set_error("topo_load_vector_2d: direction %d is not a 2-D direction", direction);
+        return TOPO_E_ARG;
+    }
+    return TOPO_OK;
+}
+
+int topo_preserved_elements_2d(int32_t nelx, int32_t nely, int32_t position, int32_t direction, uint8_t* pres) {
+    if (!pres || nelx < 1 || nely < 1) {
+        set_error("topo_preserved_elements_2d: bad arguments");
+        return TOPO_E_ARG;
+    }
+    auto E = [&](int ex, int ey) { return (int64_t)ey + (int64_t)ex * nely; };
+    switch (position) {
+    case TOPO_FIX_LEFT: for (int ey = 0; ey < nely; ey++) pres[E(0, ey)] = 1; break;
+    case TOPO_FIX_RIGHT: for (int ey = 0; ey < nely; ey++) pres[E(nelx - 1, ey)] = 1; break;
+    case TOPO_FIX_BOTTOM: for (int ex = 0; ex < nelx; ex++) pres[E(ex, 0)] = 1; break;
+    case TOPO_FIX_TOP: for (int ex = 0; ex < nelx; ex++) pres[E(ex, nely - 1)] = 1; break;
+    default: break;
+    }
+    switch (direction) {
+    case TOPO_LOAD_DOWN: pres[E(nelx - 1, nely - 1)] = 1; break;
+    case TOPO_LOAD_UP: pres[E(nelx - 1, 0)] = 1; break;
+    case TOPO_LOAD_LEFT: pres[E(nelx - 1, nely / 2)] = 1; break;
+    case TOPO_LOAD_RIGHT: pres[E(0, nely / 2)] = 1; break;
+    default: break;
+    }
+    return TOPO_OK;
+}
+
+// corner nodes n1, n2, n2 + 1, n1 + 1 of painted cells, first occurrence order
+static void painted_nodes_2d(int nelx, int nely, const int32_t* cells, int32_t n, std::vector<int64_t>& order) {
+    std::set<int64_t> seen;
+    const int64_t nny = nely + 1, maxNode = (int64_t)(nelx + 1) * nny;
+    for (int32_t c = 0; c < n; c++) {
+        const int64_t vx = cells[2 * c], vy = cells[2 * c + 1];
+        const int64_t n1 = nny * vx + vy, n2 = nny * (vx + 1) + vy;
+        const int64_t nodes[4] = {n1, n2, n2 + 1, n1 + 1};
+        for (int64_t q : nodes)
+            if (q >= 0 && q < maxNode && seen.insert(q).second) order.push_back(q);
+    }
+}
+
+int topo_fixed_paint_2d(int32_t nelx, int32_t nely, const int32_t* cells, int32_t ncells, int32_t dof_mask,
+                        uint8_t* mask) {
+    if (!mask || (ncells > 0 && !cells) || nelx < 1 || nely < 1) {
+        set_error("topo_fixed_paint_2d: bad arguments");
+        return TOPO_E_ARG;
+    }
+    std::vector<int64_t> nodes;
+    painted_nodes_2d(nelx, nely, cells, ncells, nodes);
+    int cnt = 0;
+    for (int64_t q : nodes)
+        for (int d = 0; d < 2; d++)
+            if (dof_mask & (1 << d)) {
+                mask[2 * q + d] = 1;
+                cnt++;
+            }
+    return cnt;
+}
+
+int topo_load_paint_2d(int32_t nelx, int32_t nely, const int32_t* cells, int32_t ncells, const double* dir2,
+                       double magnitude, float* F) {
+    if (!F || !dir2 || (ncells > 0 && !cells) || nelx < 1 || nely < 1) {
+        set_error("topo_load_paint_2d: bad arguments");
+        return TOPO_E_ARG;
+    }
+    std::vector<int64_t> nodes;
+    painted_nodes_2d(nelx, nely, cells, ncells, nodes);
+    if (nodes.empty()) return 0;
+    const double per = magnitude / (double)nodes.size();
+    for (int64_t q : nodes)
+        for (int d = 0; d < 2; d++) F[2 * q + d] = (float)((double)F[2 * q + d] + dir2[d] * per);
+    return (int)nodes.size();
+}
+
+// js/optimizer-worker.js:15-18 (CG constants), :361-378 and :570-578
+// (config reads with their defaults)
+int topo_config_default_2d(topo_config* c, int32_t nelx, int32_t nely) {
+    const int rc = topo_config_default(c, nelx, nely, 1);
+    if (rc < 0) return rc;
+    c->linear_solver = TOPO_SOLVER_PCG;
+    c->cg_max_iter = 2000;            // MAX_CG_ITERATIONS
+    c->cg_tol = 0.0;                  // schedule max(CG_TOLERANCE, change * CG_TOLERANCE_COARSE)
+    return TOPO_OK;
+}
+
 }  // extern "C"

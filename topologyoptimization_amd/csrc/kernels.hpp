@@ -50,6 +50,7 @@ struct CGState {
     int maxit;
     int nonfinite;
     double guard;     // denominator guard: 1e-12 in the reference's PCGs, 0 elsewhere
+    int abs_tol;      // 1: stop at ||r|| < tol (the 2-D worker's solveCG), 0: ||r|| < tol ||r0||
 };
 
 // Batched launches (several right-hand sides in one launch): blockIdx.y
@@ -345,7 +346,7 @@ __global__ void __launch_bounds__(kRedThreads) k_cg_start(const T* __restrict__ 
             s.r0 = tot[2 * q];
             s.rn = tot[2 * q];
             s.rz = tot[2 * q + 1];
-            s.tolSq = s.tol * s.tol * (tot[2 * q] > 1e-30 ? tot[2 * q] : 1e-30);
+            s.tolSq = s.abs_tol ? s.tol * s.tol : s.tol * s.tol * (tot[2 * q] > 1e-30 ? tot[2 * q] : 1e-30);
             s.done = (s.maxit <= 0) || (tot[2 * q] < s.tolSq);
             s.iters = 0;
             s.nonfinite = 0;

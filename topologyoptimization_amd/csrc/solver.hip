@@ -1373,27 +1373,31 @@ struct CGResult {
     int vcycles = 0;
     int applyA = 0;
     double relres = 0;   // largest final relative residual of the solves summed here
+    double rn = 0;       // final ||r||^2 (Jacobi-PCG)
 };
 
-static void cg_set(Ctx& c, double tol, int maxit, double guard = 1e-12, int nr = 1) {
+static void cg_set(Ctx& c, double tol, int maxit, double guard = 1e-12, int nr = 1, int abs_tol = 0) {
     CGState h;
     std::memset(&h, 0, sizeof h);
     h.tol = tol;
     h.maxit = maxit;
     h.guard = guard;
+    h.abs_tol = abs_tol;
     for (int q = 0; q < nr; q++) c.h_st[q] = h;
     TOPO_HIP(hipMemcpyAsync(c.st, c.h_st, nr * sizeof(CGState), hipMemcpyHostToDevice, c.s));
 }
 
 
 
-// Jacobi-PCG on the fine level (ebePCG assembly/index.ts:504-733 in full
-// space with fixed DOFs masked): U is the warm start (TU precision), rhs in T.
-template <typename T, typename TU>
-static CGResult pcg_jacobi(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVecs& v, double tol, int maxit,
-                           volatile int* cancel) {
-    MGLevel& l = mg.L[0];
-    const int64_t n = l.ndof;
+// Jacobi-PCG (ebePCG assembly/index.ts:504-733 in full space with fixed DOFs
+// masked) over an operator `op(p, out, mode, b, gate)`: mode 1 out = A p with
+// fixed DOFs zeroed, mode 2 out = b - A p at free DOFs (0 at fixed). U is the
+// warm start (TU precision), rhs in T. abs_tol: stop at ||r|| < tol instead
+// of ||r|| < tol ||r0||. key identifies the operator for the graph cache.
+template <typename T, typename TU, typename Op>
+static CGResult pcg_jacobi_op(Ctx& c, Op&& op, const uint8_t* fixed, const T* invD, int64_t n, const T* rhs, TU* U,
+                              bool warm, CGVecs& v, double tol, int maxit, volatile int* cancel, const void* key,
+                              int abs_tol = 0) {
     T* r = (T*)v.r;
     T* z = (T*)v.z;
     T* p = (T*)v.p;
@@ -1402,20 +1406,20 @@ static CGResult pcg_jacobi(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVe
     if (warm) {
         // r = rhs - A U on free DOFs (U in T precision for the operator)
         if (std::is_same<T, TU>::value) {
-            mg.apply<T>(c, 0, (const T*)U, r, 2, rhs, nullptr);
+            op((const T*)U, r, 2, rhs, (const int*)nullptr);
         } else {
             hipLaunchKernelGGL((k_convert<T, TU>), dim3(nblk(n)), dim3(256), 0, c.s, z, (const TU*)U, n);
-            mg.apply<T>(c, 0, z, r, 2, rhs, nullptr);
+            op((const T*)z, r, 2, rhs, (const int*)nullptr);
         }
         res.applyA++;
     } else {
         hipLaunchKernelGGL(k_fill<TU>, dim3(nblk(n)), dim3(256), 0, c.s, U, TU(0), n, (const int*)nullptr);
         hipLaunchKernelGGL((k_convert<T, T>), dim3(nblk(n)), dim3(256), 0, c.s, r, rhs, n);
-        hipLaunchKernelGGL(k_mask<T>, dim3(nblk(n)), dim3(256), 0, c.s, r, l.fixed, n);
+        hipLaunchKernelGGL(k_mask<T>, dim3(nblk(n)), dim3(256), 0, c.s, r, fixed, n);
     }
-    hipLaunchKernelGGL(k_jacobi_apply<T>, dim3(nblk(n)), dim3(256), 0, c.s, z, (const T*)r, (const T*)Sel<T>::invD(l), n);
+    hipLaunchKernelGGL(k_jacobi_apply<T>, dim3(nblk(n)), dim3(256), 0, c.s, z, (const T*)r, invD, n);
     hipLaunchKernelGGL((k_convert<T, T>), dim3(nblk(n)), dim3(256), 0, c.s, p, (const T*)z, n);
-    cg_set(c, tol, maxit);
+    cg_set(c, tol, maxit, 1e-12, 1, abs_tol);
     hipLaunchKernelGGL(k_cg_start<T>, dim3(kRedBlocks), dim3(kRedThreads), 0, c.s, (const T*)r, (const T*)z, n, c.st,
                        c.partials, c.ticket);
     c.check_launch();
@@ -1423,11 +1427,11 @@ static CGResult pcg_jacobi(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVe
     const int check = 8;
     auto body = [&]() {
         for (int k = 0; k < check; k++) {
-            mg.apply<T>(c, 0, p, Ap, 1, nullptr, gate);
+            op((const T*)p, Ap, 1, (const T*)nullptr, gate);
             hipLaunchKernelGGL((k_cgp_pAp<T, 1>), dim3(kCgBlocks), dim3(kRedThreads), 0, c.s, (const T*)p,
                                (const T*)Ap, n, c.st, c.partials);
             hipLaunchKernelGGL((k_cgp_update_jacobi<T, TU>), dim3(kCgBlocks), dim3(kRedThreads), 0, c.s, U, r, z,
-                               (const T*)p, (const T*)Ap, (const T*)Sel<T>::invD(l), n, c.st, c.partials);
+                               (const T*)p, (const T*)Ap, invD, n, c.st, c.partials);
             hipLaunchKernelGGL((k_cgp_pupdate<T, T, 1, false>), dim3(kCgBlocks), dim3(kRedThreads), 0, c.s, p, (const T*)z,
                                (const uint8_t*)nullptr, n, c.st, (const double*)c.partials);
             hipLaunchKernelGGL(k_cgp_finish<1>, dim3(1), dim3(kRedThreads), 0, c.s, c.st, (const double*)c.partials,
@@ -1436,12 +1440,13 @@ static CGResult pcg_jacobi(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVe
     };
     hipGraphExec_t ex = nullptr;
     if (c.use_graphs)
-        ex = c.graph_capture({(const void*)&pcg_jacobi<T, TU>, U, r, z, p, Ap, rhs, &mg}, body);
+        ex = c.graph_capture({(const void*)&pcg_jacobi_op<T, TU, Op>, U, r, z, p, Ap, rhs, key}, body);
     for (;;) {
         const CGState& h = c.read_st();
         if (h.done) {
             res.iters = h.iters;
             res.relres = std::sqrt(h.rn / (h.r0 > 1e-300 ? h.r0 : 1e-300));
+            res.rn = h.rn;
             res.nonfinite = h.nonfinite;
             break;
         }
@@ -1452,6 +1457,18 @@ static CGResult pcg_jacobi(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVe
     }
     res.applyA += res.iters;
     return res;
+}
+
+// Jacobi-PCG on the fine level of a hierarchy (its level-0 operator)
+template <typename T, typename TU>
+static CGResult pcg_jacobi(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVecs& v, double tol, int maxit,
+                           volatile int* cancel) {
+    MGLevel& l = mg.L[0];
+    auto op = [&](const T* p, T* out, int mode, const T* b, const int* gate) {
+        mg.apply<T>(c, 0, p, out, mode, b, gate);
+    };
+    return pcg_jacobi_op<T, TU>(c, op, l.fixed, (const T*)Sel<T>::invD(l), l.ndof, rhs, U, warm, v, tol, maxit, cancel,
+                                &mg);
 }
 
 // The V-cycle part of precondition() without the widening copy: returns the
@@ -2838,6 +2855,9 @@ int topo_solver_cancel(topo_solver* h) {
 
 }  // extern "C"
 
+// the 2-D SIMP path (quad4 operator, 2-D solver handle)
+#include "solver2d.hpp"
+
 // ---------------------------------------------------------------------------
 // WASM-op twins. One process-wide context, serialised by a mutex (the
 // AssemblyScript exports are single-threaded and not re-entrant either).
@@ -2906,6 +2926,67 @@ static std::vector<double> masked_E_zyx(const Grid& gg, const double* evals, con
     return Ez;
 }
 
+// ebePCG with edofSize 8 (assembly/index.ts:504-733 over the quad4 grid of
+// _precomputeEdofs2D): E by Math.pow, active above 1000 Emin, Jacobi
+// diagonal, warm start from U at free DOFs, relative tolerance, cgMax =
+// min(maxIter, nfree); U returned with fixed DOFs 0.
+static int ebepcg_2d(const double* dens, const double* ke, const int32_t* edofs, const double* Fh, double* Uh,
+                     const int32_t* freedofs, int32_t nel, int32_t ndof, int32_t nfree, double Emin, double E0,
+                     double penal, int32_t maxIter, double tolerance) {
+    TOPO_CHECK(edofs, TOPO_E_ARG, "edofs is null");
+    Grid2 g;
+    TOPO_CHECK(edofs2d_structured(edofs, nel, ndof, &g), TOPO_E_UNSTRUCTURED,
+               "edofs is not a structured quad4 grid in the 2-D worker's numbering");
+    std::lock_guard<std::mutex> lk(g_twin_mu);
+    TwinCtx& T = g_twin;
+    T.ensure();
+    Ctx& C = *T.c;
+    std::vector<uint8_t> fixedh(ndof, 1);
+    for (int32_t i = 0; i < nfree; i++) {
+        TOPO_CHECK(freedofs[i] >= 0 && freedofs[i] < ndof, TOPO_E_ARG, "freedofs entry out of range");
+        fixedh[freedofs[i]] = 0;
+    }
+    std::vector<void*> tmp;
+    auto A = [&](size_t bytes) {
+        void* q = C.alloc<uint8_t>((int64_t)bytes);
+        tmp.push_back(q);
+        return q;
+    };
+    uint8_t* dfixed = (uint8_t*)A(ndof);
+    double* dke = (double*)A(64 * sizeof(double));
+    double* ddens = (double*)A(nel * sizeof(double));
+    double* dE = (double*)A(nel * sizeof(double));
+    double* dinvD = (double*)A(ndof * sizeof(double));
+    double* dF = (double*)A(ndof * sizeof(double));
+    double* dU = (double*)A(ndof * sizeof(double));
+    CGVecs v;
+    v.r = A(ndof * sizeof(double)); v.z = A(ndof * sizeof(double));
+    v.p = A(ndof * sizeof(double)); v.Ap = A(ndof * sizeof(double));
+    TOPO_HIP(hipMemcpyAsync(dfixed, fixedh.data(), ndof, hipMemcpyHostToDevice, C.s));
+    TOPO_HIP(hipMemcpyAsync(dke, ke, 64 * sizeof(double), hipMemcpyHostToDevice, C.s));
+    TOPO_HIP(hipMemcpyAsync(ddens, dens, nel * sizeof(double), hipMemcpyHostToDevice, C.s));
+    TOPO_HIP(hipMemcpyAsync(dF, Fh, ndof * sizeof(double), hipMemcpyHostToDevice, C.s));
+    TOPO_HIP(hipMemcpyAsync(dU, Uh, ndof * sizeof(double), hipMemcpyHostToDevice, C.s));
+    hipLaunchKernelGGL(k_q4_moduli<double>, dim3(nblk(nel)), dim3(256), 0, C.s, (const double*)ddens, penal, E0, Emin,
+                       Emin * 1000.0, dE, (int64_t)nel);
+    hipLaunchKernelGGL(k_q4_invdiag, dim3(nblk(g.nnodes())), dim3(256), 0, C.s, (const double*)dE,
+                       (const uint8_t*)dfixed, dinvD, g, (const double*)dke);
+    // the warm start reads only free DOFs (Uf = U[free], :583-587)
+    hipLaunchKernelGGL(k_mask<double>, dim3(nblk(ndof)), dim3(256), 0, C.s, dU, (const uint8_t*)dfixed, (int64_t)ndof);
+    C.check_launch();
+    auto op = [&](const double* p, double* out, int mode, const double* b, const int* gate) {
+        q4_apply(C.s, mode, p, out, dE, g, dfixed, b, gate, dke);
+    };
+    const int cgMax = maxIter < nfree ? maxIter : nfree;
+    CGResult r = pcg_jacobi_op<double, double>(C, op, dfixed, (const double*)dinvD, ndof, (const double*)dF, dU, true,
+                                               v, tolerance, cgMax, nullptr, &T);
+    TOPO_HIP(hipMemcpyAsync(Uh, dU, ndof * sizeof(double), hipMemcpyDeviceToHost, C.s));
+    C.sync();
+    for (void* q : tmp) C.free_one(q);
+    TOPO_CHECK(!r.nonfinite, TOPO_E_BREAKDOWN, "non-finite CG scalar");
+    return r.iters;
+}
+
 }  // namespace topo
 
 extern "C" {
@@ -2948,7 +3029,10 @@ int topo_ebePCG(const double* dens, const double* ke, const int32_t* edofs, cons
     TOPO_API_BEGIN
     TOPO_CHECK(dens && ke && Fh && Uh && (nfree == 0 || freedofs) && ndof > 0 && nel > 0 && nfree >= 0 && nfree <= ndof,
                TOPO_E_ARG, "topo_ebePCG: bad argument");
-    TOPO_CHECK(edofSize == 24, TOPO_E_UNSUPPORTED, "topo_ebePCG: edofSize %d (only the 3-D hex8 path, 24)", edofSize);
+    TOPO_CHECK(edofSize == 24 || edofSize == 8, TOPO_E_UNSUPPORTED,
+               "topo_ebePCG: edofSize %d (24: 3-D hex8, 8: 2-D quad4)", edofSize);
+    if (edofSize == 8)
+        return ebepcg_2d(dens, ke, edofs, Fh, Uh, freedofs, nel, ndof, nfree, Emin, E0, penal, maxIter, tolerance);
     std::lock_guard<std::mutex> lk(g_twin_mu);
     TwinCtx& T = g_twin;
     T.ensure();

@@ -456,6 +456,176 @@ napi_value SolverCancel(napi_env env, napi_callback_info info) {
     return nullptr;
 }
 
+// ---- 2-D path -----------------------------------------------------------------
+napi_value ConfigDefault2d(napi_env env, napi_callback_info info) {
+    napi_value a[2];
+    if (!get_args(env, info, 2, a)) return nullptr;
+    topo_config c;
+    int rc = topo_config_default_2d(&c, to_i32(env, a[0]), to_i32(env, a[1]));
+    if (rc < 0) return throw_topo(env, rc);
+    return cfg_to_js(env, c);
+}
+
+// k0_2d(nu, Float64Array(64))
+napi_value K0_2d(napi_env env, napi_callback_info info) {
+    napi_value a[2];
+    if (!get_args(env, info, 2, a)) return nullptr;
+    int rc = topo_k0_2d(to_f64(env, a[0]), (double*)ta_ptr(env, a[1]));
+    if (rc < 0) return throw_topo(env, rc);
+    return a[1];
+}
+
+// edofs2d(nelx, nely, Int32Array)
+napi_value Edofs2d(napi_env env, napi_callback_info info) {
+    napi_value a[3];
+    if (!get_args(env, info, 3, a)) return nullptr;
+    int rc = topo_edofs_2d(to_i32(env, a[0]), to_i32(env, a[1]), (int32_t*)ta_ptr(env, a[2]));
+    if (rc < 0) return throw_topo(env, rc);
+    return a[2];
+}
+
+// fixedDofs2d(nelx, nely, position, dofMask, Uint8Array) -> count
+napi_value FixedDofs2d(napi_env env, napi_callback_info info) {
+    napi_value a[5];
+    if (!get_args(env, info, 5, a)) return nullptr;
+    int rc = topo_fixed_dofs_2d(to_i32(env, a[0]), to_i32(env, a[1]), to_i32(env, a[2]), to_i32(env, a[3]),
+                                (uint8_t*)ta_ptr(env, a[4]));
+    if (rc < 0) return throw_topo(env, rc);
+    return i32v(env, rc);
+}
+
+// loadVector2d(nelx, nely, direction, magnitude, forceVector|null, Float32Array)
+napi_value LoadVector2d(napi_env env, napi_callback_info info) {
+    napi_value a[6];
+    if (!get_args(env, info, 6, a)) return nullptr;
+    int rc = topo_load_vector_2d(to_i32(env, a[0]), to_i32(env, a[1]), to_i32(env, a[2]), to_f64(env, a[3]),
+                                 (const double*)ta_ptr(env, a[4]), (float*)ta_ptr(env, a[5]));
+    if (rc < 0) return throw_topo(env, rc);
+    return a[5];
+}
+
+// preservedElements2d(nelx, nely, position, direction, Uint8Array)
+napi_value Preserved2d(napi_env env, napi_callback_info info) {
+    napi_value a[5];
+    if (!get_args(env, info, 5, a)) return nullptr;
+    int rc = topo_preserved_elements_2d(to_i32(env, a[0]), to_i32(env, a[1]), to_i32(env, a[2]), to_i32(env, a[3]),
+                                        (uint8_t*)ta_ptr(env, a[4]));
+    if (rc < 0) return throw_topo(env, rc);
+    return a[4];
+}
+
+// fixedPaint2d(nelx, nely, Int32Array cells (vx,vy pairs), dofMask, Uint8Array) -> count
+napi_value FixedPaint2d(napi_env env, napi_callback_info info) {
+    napi_value a[5];
+    if (!get_args(env, info, 5, a)) return nullptr;
+    size_t n = 0;
+    const int32_t* cells = (const int32_t*)ta_ptr(env, a[2], &n);
+    int rc = topo_fixed_paint_2d(to_i32(env, a[0]), to_i32(env, a[1]), cells, (int32_t)(n / 2), to_i32(env, a[3]),
+                                 (uint8_t*)ta_ptr(env, a[4]));
+    if (rc < 0) return throw_topo(env, rc);
+    return i32v(env, rc);
+}
+
+// loadPaint2d(nelx, nely, Int32Array cells, Float64Array dir2, magnitude, Float32Array F) -> count
+napi_value LoadPaint2d(napi_env env, napi_callback_info info) {
+    napi_value a[6];
+    if (!get_args(env, info, 6, a)) return nullptr;
+    size_t n = 0;
+    const int32_t* cells = (const int32_t*)ta_ptr(env, a[2], &n);
+    int rc = topo_load_paint_2d(to_i32(env, a[0]), to_i32(env, a[1]), cells, (int32_t)(n / 2),
+                                (const double*)ta_ptr(env, a[3]), to_f64(env, a[4]), (float*)ta_ptr(env, a[5]));
+    if (rc < 0) return throw_topo(env, rc);
+    return i32v(env, rc);
+}
+
+void solver2d_finalize(napi_env, void* data, void*) { topo_solver2d_destroy((topo_solver2d*)data); }
+
+topo_solver2d* solver2d_of(napi_env env, napi_value v) {
+    void* p = nullptr;
+    napi_get_value_external(env, v, &p);
+    if (!p) napi_throw_error(env, nullptr, "solver handle already destroyed");
+    return (topo_solver2d*)p;
+}
+
+// createSolver2d(config, fixedMask, F, preserved|null, void|null) -> handle
+napi_value CreateSolver2d(napi_env env, napi_callback_info info) {
+    napi_value a[5];
+    if (!get_args(env, info, 5, a)) return nullptr;
+    topo_config c;
+    int rc = topo_config_default_2d(&c, to_i32(env, get(env, a[0], "nelx")), to_i32(env, get(env, a[0], "nely")));
+    if (rc < 0) return throw_topo(env, rc);
+    cfg_from_js(env, a[0], &c);
+    topo_solver2d* s = nullptr;
+    rc = topo_solver2d_create(&c, (const uint8_t*)ta_ptr(env, a[1]), (const float*)ta_ptr(env, a[2]),
+                              (const uint8_t*)ta_ptr(env, a[3]), (const uint8_t*)ta_ptr(env, a[4]), &s);
+    if (rc < 0) return throw_topo(env, rc);
+    napi_value h;
+    NAPI_CALL(env, napi_create_external(env, s, solver2d_finalize, nullptr, &h));
+    return h;
+}
+
+napi_value Solver2dStep(napi_env env, napi_callback_info info) {
+    napi_value a[1];
+    if (!get_args(env, info, 1, a)) return nullptr;
+    topo_step_info si;
+    std::memset(&si, 0, sizeof si);
+    int rc = topo_solver2d_step(solver2d_of(env, a[0]), &si);
+    if (rc < 0) return throw_topo(env, rc);
+    return info_to_js(env, si);
+}
+
+// fea(h, x Float32Array, penal, tol) -> info
+napi_value Solver2dFea(napi_env env, napi_callback_info info) {
+    napi_value a[4];
+    if (!get_args(env, info, 4, a)) return nullptr;
+    topo_step_info si;
+    std::memset(&si, 0, sizeof si);
+    int rc = topo_solver2d_fea(solver2d_of(env, a[0]), (const float*)ta_ptr(env, a[1]), to_f64(env, a[2]),
+                               to_f64(env, a[3]), &si);
+    if (rc < 0) return throw_topo(env, rc);
+    return info_to_js(env, si);
+}
+
+napi_value Solver2dSetDesign(napi_env env, napi_callback_info info) {
+    napi_value a[2];
+    if (!get_args(env, info, 2, a)) return nullptr;
+    int rc = topo_solver2d_set_design(solver2d_of(env, a[0]), (const float*)ta_ptr(env, a[1]));
+    if (rc < 0) return throw_topo(env, rc);
+    return a[1];
+}
+
+napi_value Solver2dGetDesign(napi_env env, napi_callback_info info) {
+    napi_value a[2];
+    if (!get_args(env, info, 2, a)) return nullptr;
+    int rc = topo_solver2d_get_design(solver2d_of(env, a[0]), (float*)ta_ptr(env, a[1]));
+    if (rc < 0) return throw_topo(env, rc);
+    return a[1];
+}
+
+napi_value Solver2dGetField(napi_env env, napi_callback_info info) {
+    napi_value a[3];
+    if (!get_args(env, info, 3, a)) return nullptr;
+    int rc = topo_solver2d_get_field(solver2d_of(env, a[0]), to_i32(env, a[1]), ta_ptr(env, a[2]));
+    if (rc < 0) return throw_topo(env, rc);
+    return a[2];
+}
+
+napi_value Solver2dUpdate(napi_env env, napi_callback_info info) {
+    napi_value a[5];
+    if (!get_args(env, info, 5, a)) return nullptr;
+    int rc = topo_solver2d_update(solver2d_of(env, a[0]), to_f64(env, a[1]), to_f64(env, a[2]), to_f64(env, a[3]),
+                                  to_i32(env, a[4]));
+    if (rc < 0) return throw_topo(env, rc);
+    return nullptr;
+}
+
+napi_value Solver2dCancel(napi_env env, napi_callback_info info) {
+    napi_value a[1];
+    if (!get_args(env, info, 1, a)) return nullptr;
+    topo_solver2d_cancel(solver2d_of(env, a[0]));
+    return nullptr;
+}
+
 }  // namespace
 
 NAPI_MODULE_INIT() {
@@ -471,6 +641,12 @@ NAPI_MODULE_INIT() {
         {"solverSetDesign", SolverSetDesign}, {"solverGetDesign", SolverGetDesign},
         {"solverGetField", SolverGetField}, {"solverUpdate", SolverUpdate}, {"solverFe", SolverFe},
         {"solverSetIteration", SolverSetIteration}, {"solverCancel", SolverCancel}, {"solverEnergy", SolverEnergy},
+        {"configDefault2d", ConfigDefault2d}, {"k0_2d", K0_2d}, {"edofs2d", Edofs2d}, {"fixedDofs2d", FixedDofs2d},
+        {"loadVector2d", LoadVector2d}, {"preservedElements2d", Preserved2d}, {"fixedPaint2d", FixedPaint2d},
+        {"loadPaint2d", LoadPaint2d}, {"createSolver2d", CreateSolver2d}, {"solver2dStep", Solver2dStep},
+        {"solver2dFea", Solver2dFea}, {"solver2dSetDesign", Solver2dSetDesign},
+        {"solver2dGetDesign", Solver2dGetDesign}, {"solver2dGetField", Solver2dGetField},
+        {"solver2dUpdate", Solver2dUpdate}, {"solver2dCancel", Solver2dCancel},
     };
     for (const Fn& f : fns) {
         napi_value v;

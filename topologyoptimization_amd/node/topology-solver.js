@@ -7,16 +7,19 @@
  * that drives the device-resident solver of libtopo_hip through the N-API
  * addon. CommonJS, so it also runs on Node 12.
  *
- * config.solver ('auto' | '3d' | 'mgpcg' | 'cg' | 'fea') selects the linear
- * solver as in the reference (:213-253); the 2-D, genetic, nonlinear,
- * fracture and dynamics solvers of the reference are not part of this path
- * and reject with an Error naming the solver.
+ * config.solver ('auto' | '2d' | '3d' | 'mgpcg' | 'cg' | 'petsc' | 'fea')
+ * selects the worker and the linear solver as in the reference (:213-253):
+ * '2d', and 'auto' / 'fea' on a model with nz <= 1 that is not a cube, run
+ * the 2-D worker (optimizer-worker-2d.js); the others the 3-D worker. The
+ * genetic, nonlinear, fracture, dynamics and fatigue solvers of the
+ * reference are not part of this path and reject with an Error naming the
+ * solver.
  */
 const path = require('path');
 
 const IS_NODE = typeof process !== 'undefined' && !!(process.versions && process.versions.node);
 
-const UNSUPPORTED = ['2d', 'genetic', 'nonlinear', 'fracture', 'dynamics', 'fatigue'];
+const UNSUPPORTED = ['genetic', 'nonlinear', 'fracture', 'dynamics', 'fatigue'];
 
 class TopologySolver {
     constructor() {
@@ -55,15 +58,12 @@ class TopologySolver {
             else if (solverKey === 'cg') linearSolver = 'cg';
             else if (solverKey === 'petsc') linearSolver = 'petsc';
             else if (solverKey === 'fea') feaOnly = true;
-            const use3D = solverKey !== 'auto' || model.type === 'cube' || (model.nz != null && model.nz > 1);
-            if (!use3D) {
-                reject(new Error('TopologySolver: 2-D models are not part of the MI355X SIMP path'));
-                return;
-            }
+            const byGeometry = model.type === 'cube' || (model.nz != null && model.nz > 1);
+            const use3D = solverKey === '2d' ? false : (solverKey === 'auto' || solverKey === 'fea') ? byGeometry : true;
             let worker;
             try {
                 const { Worker } = require('worker_threads');
-                worker = new Worker(path.join(__dirname, 'optimizer-worker-3d.js'));
+                worker = new Worker(path.join(__dirname, use3D ? 'optimizer-worker-3d.js' : 'optimizer-worker-2d.js'));
             } catch (err) {
                 reject(new Error(`TopologySolver: failed to create worker — ${err.message}`));
                 return;
