@@ -90,9 +90,10 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--workload", choices=("c3", "c4", "c5"), default="c3",
+    ap.add_argument("--workload", choices=("c3", "c4", "c5", "2d"), default="c3",
                     help="c3: 192x96x96 cantilever (headline); c4: 128x64x64 bridge, 4 load cases, batched; "
-                         "c5: 160x80x80 force inverter, forward + adjoint solves")
+                         "c5: 160x80x80 force inverter, forward + adjoint solves; 2d: the 2-D SIMP leg alone "
+                         "(1024x512 quad4 cantilever)")
     ap.add_argument("--coarse-fp32", type=int, default=2, choices=(0, 1, 2),
                     help="copy of the FP64 coarse factor inverse read by the FP32 V-cycle: 0 none (FP64), "
                          "1 FP32, 2 FP16 with one scale per row")
@@ -100,6 +101,7 @@ def main():
                     help="pc: FP64 MG-PCG with an FP32 V-cycle preconditioner; ir: FP64 iterative refinement "
                          "around an FP32 inner MG-PCG")
     ap.add_argument("--profile-kernel", action="store_true")
+    ap.add_argument("--no-2d", action="store_true", help="skip the 2-D SIMP leg")
     args = ap.parse_args()
     if args.profile_kernel:
         return profile_kernels(args)
@@ -109,6 +111,9 @@ def main():
         sys.exit(2)
     import topologyoptimization_amd as T
     import topologyoptimization_amd.problems as P
+    if args.workload == "2d":
+        print(json.dumps(simp2d_leg(T, steps=args.steps, warmup=args.warmup, cpu=not args.no_cpu)))
+        return
     if args.workload != "c3":
         return other_workload(T, P, args)
     grid = tuple(int(v) for v in args.grid.split(","))
@@ -147,6 +152,7 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(grid, infos, args.levels)
     fine_calls = np.mean([i["applyA_fine"] for i in infos])
+    simp2d = None if args.no_2d else simp2d_leg(T, cpu=not args.no_cpu)
     out = {
         "metric": "SIMP iters/s + GDOF-matvec/s at 5.4M DOF, mixed-FP32 vs FP64 MGPCG",
         "value": value,
@@ -178,6 +184,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "autotuned_fine_tile": {"xm_nt": tnt, "xm_cx": tcx},
+        "simp2d": simp2d,
         "device": name,
     }
     print(json.dumps(out))
@@ -297,6 +304,50 @@ def cpu_baseline(grid, infos, levels):
                 "host": platform.processor() or platform.machine(), "nproc": os.cpu_count()}
     except Exception as e:  # reported, never fatal
         return {"value": None, "error": repr(e)}
+
+
+def simp2d_leg(T, grid=(1024, 512), steps=5, warmup=1, cpu=True):
+    """The 2-D path (js/optimizer-worker.js loop) on a quad4 cantilever:
+    left edge clamped, unit load down at node (nelx, nely), volfrac 0.4,
+    rmin 1.5, penalty 1.5 -> 3 over 20 iterations, Heaviside projection,
+    Jacobi-PCG (FP64) to max(1e-8, change/100) on ||r||, at most 2000
+    iterations. SIMP iters/s over `steps` steps after `warmup`; the CPU
+    baseline is the oracle's solveCG on 20 CG iterations at the same grid,
+    scaled by the GPU steps' CG iteration counts."""
+    nelx, nely = grid
+    nel, ndof = T.dims2(nelx, nely)
+    cfg = T.default_config_2d(nelx, nely, volfrac=0.4, max_iterations=100, penal_start=1.5, continuation_iters=33)
+    fixed = T.fixed_dofs_2d(nelx, nely, T.FIX_LEFT)
+    F = T.load_vector_2d(nelx, nely, T.LOAD_DOWN, 1.0)
+    pres = T.preserved_elements_2d(nelx, nely, T.FIX_LEFT, T.LOAD_DOWN)
+    with T.Solver2D(cfg, fixed, F, pres) as S:
+        for _ in range(warmup):
+            S.step()
+        t0 = time.perf_counter()
+        infos = [S.step() for _ in range(steps)]
+        dt = time.perf_counter() - t0
+    cg = [i.cg_iters for i in infos]
+    solve_ms = sum(i.t_solve_ms for i in infos)
+    out = {"workload": "2-D cantilever %dx%d quad4, %d DOF, Jacobi-PCG FP64, steps %d..%d" %
+                       (nelx, nely, ndof, warmup + 1, warmup + steps),
+           "value": steps / dt, "unit": "SIMP iters/s", "ms_per_step": dt / steps * 1e3, "cg_iters": cg,
+           "us_per_cg_iteration": solve_ms * 1e3 / max(1, sum(cg)),
+           "compliance": infos[-1].compliance, "cpu_baseline": None}
+    if cpu:
+        try:
+            from oracle import oracle as O
+            x = np.full(nel, 0.4, np.float32)
+            t0 = time.perf_counter()
+            _, _, it = O.fe2(nelx, nely, O.k0_2d(0.3), x, 3.0, fixed, F, tol=1e-30, maxit=20)
+            t_it = (time.perf_counter() - t0) / it
+            n_cg = float(np.mean(cg))
+            out["cpu_baseline"] = {"value": 1.0 / (n_cg * t_it), "unit": "SIMP iters/s", "cores": 1, "kind": "port",
+                                   "sample": "oracle solveCG, %d CG iterations at %dx%d (%.1f ms each), scaled by %.0f "
+                                             "CG iterations per step (energies, filter and OC not counted)" %
+                                             (it, nelx, nely, t_it * 1e3, n_cg)}
+        except Exception as e:  # reported, never fatal
+            out["cpu_baseline"] = {"value": None, "error": repr(e)}
+    return out
 
 
 def profile_kernels(args):

@@ -596,7 +596,8 @@ __global__ void __launch_bounds__(kRedThreads) k_cgp_pAp(const T* __restrict__ p
 
 // alpha = rz / (p.Ap + guard); U += alpha p; r -= alpha Ap (ebeMGPCG
 // :1158-1166); r_lo (may be null) receives r rounded to the V-cycle's
-// precision; ||r||^2 partials (quantity 1)
+// precision; ||r||^2 partials (quantity 1). With U null the U update is
+// left to k_cgp_pupdate (which reads p anyway).
 template <typename T, typename TU, int NR, typename TL>
 __global__ void __launch_bounds__(kRedThreads) k_cgp_update_ur(TU* __restrict__ U, T* __restrict__ r,
                                                               const T* __restrict__ p, const T* __restrict__ Ap,
@@ -617,7 +618,7 @@ __global__ void __launch_bounds__(kRedThreads) k_cgp_update_ur(TU* __restrict__ 
             const T* aq = Ap + q * n;
             TL* lq = r_lo ? r_lo + q * n : nullptr;
             for (int64_t i = gtid(); i < n; i += gstride()) {
-                Uq[i] = (TU)((double)Uq[i] + alpha * (double)pq[i]);
+                if (U) Uq[i] = (TU)((double)Uq[i] + alpha * (double)pq[i]);
                 const T ri = (T)((double)rq[i] - alpha * (double)aq[i]);
                 rq[i] = ri;
                 if (lq) lq[i] = (TL)ri;
@@ -682,22 +683,37 @@ __global__ void __launch_bounds__(kRedThreads) k_cgp_rz(const T* __restrict__ r,
 }
 
 // beta = rz_new / (rz + guard) from the r.z partials; p = z + beta p with
-// z = (MASK && fixed ? 0 : zz)
-template <typename T, typename TZ, int NR, bool MASK>
+// z = (MASK && fixed ? 0 : zz). With U given, first U += alpha p with the
+// iteration's alpha = rz / (p.Ap + guard) (the p.Ap partials, nb0 of them):
+// the update k_cgp_update_ur leaves out, on the p this kernel reads anyway.
+template <typename T, typename TZ, int NR, bool MASK, typename TU = T>
 __global__ void __launch_bounds__(kRedThreads) k_cgp_pupdate(T* __restrict__ p, const TZ* __restrict__ zz,
                                                             const uint8_t* __restrict__ fixed, int64_t n,
-                                                            const CGState* st, const double* part) {
+                                                            const CGState* st, const double* part,
+                                                            TU* __restrict__ U = nullptr, int nb0 = kCgBlocks) {
     if (all_done(st, NR)) return;
     double rz[NR];
     sum_partials<NR>(part, 2, NR, rz);
+    double pAp[NR];
+    if (U) sum_partials<NR>(part, 0, NR, pAp, nb0);
 #pragma unroll
     for (int q = 0; q < NR; q++) {
         if (st[q].done) continue;
         const T beta = (T)(rz[q] / (st[q].rz + st[q].guard));
         T* pq = p + q * n;
         const TZ* zq = zz + q * n;
-        for (int64_t i = gtid(); i < n; i += gstride())
-            pq[i] = ((MASK && fixed[i]) ? T(0) : (T)zq[i]) + beta * pq[i];
+        if (U) {
+            const double alpha = st[q].rz / (pAp[q] + st[q].guard);
+            TU* Uq = U + q * n;
+            for (int64_t i = gtid(); i < n; i += gstride()) {
+                const T pi = pq[i];
+                Uq[i] = (TU)((double)Uq[i] + alpha * (double)pi);
+                pq[i] = ((MASK && fixed[i]) ? T(0) : (T)zq[i]) + beta * pi;
+            }
+        } else {
+            for (int64_t i = gtid(); i < n; i += gstride())
+                pq[i] = ((MASK && fixed[i]) ? T(0) : (T)zq[i]) + beta * pq[i];
+        }
     }
 }
 

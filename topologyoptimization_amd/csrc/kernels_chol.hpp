@@ -13,8 +13,9 @@
 // Z = L^-1 is dense lower triangular; its row blocks follow from
 //   Z_II = L_II^-1,  Z_I,0:I = -L_II^-1 sum_{K=max(0,I-wb)}^{I-1} L_IK Z_K,0:I
 // (two DGEMMs per row block on the host side). The coarse solve is then
-// x = Z^T (Z b): two triangular matrix-vector products (k_tri_apply) over Z
-// and its transpose.
+// x = Z^T (Z b): two triangular matrix-vector products over Z (row sums by
+// k_tri_part / k_tri_apply, column sums from the same rows by k_tri_colpart
+// for the FP16 copy, or over a stored transpose for the FP32/FP64 copies).
 // All of it is O(n w^2 + n^2 w) work instead of the O(n^3) of a dense inverse.
 #pragma once
 #include "kernels.hpp"
@@ -42,31 +43,34 @@ __device__ __forceinline__ void blk_load(double* s, const double* A, int64_t ld)
     }
 }
 
-// double from lane `lane` (a compile-time or wave-uniform index)
+
+// Cholesky of the diagonal block K and the inverse of its factor, by one
+// wavefront (lane i owns row i), blocked by 16 columns. Within a sub-block
+// the lane's 16 entries stay in registers and the column steps need no LDS
+// or barrier: the pivot and the column below it are read from the owning
+// lanes (v_readlane), 1/l_jj comes from the hardware reciprocal square root
+// refined by two Newton steps (l_jj = d / sqrt(d) to ~1 ulp), and each lane
+// updates its own entries. After a sub-block the rows go to LDS and every
+// lane updates its trailing columns k with the sub-block's panel,
+// r[k] -= l_ij l_kj (the panel read as LDS broadcasts). The inverse, lane c
+// solving L x = e_c by forward substitution, multiplies by the kept
+// reciprocals and splits each dot product over four partial sums. The block
+// load issues all of a lane's reads at once and the pivot thresholds are
+// read once. Writes L_KK (upper part zeroed) back to A and L_KK^-1 to
+// Linv[K]. A pivot <= tiny * (original diagonal of that DOF) sets *flag.
 __device__ __forceinline__ double readlane_d(double v, int lane) {
     const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
     const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
     return __hiloint2double(hi, lo);
 }
 
-// Cholesky of the diagonal block K and the inverse of its factor, by one
-// wavefront on the block in LDS (lane i owns row i). Blocked by 16 columns:
-// per column j of a sub-block every lane reads the pivot, scales its own
-// entry l_ij and updates the rest of the sub-block's columns; then each lane
-// updates its trailing columns k with the sub-block's panel,
-// r[k] -= l_ij l_kj for j ascending (the same operations in the same order
-// as the unblocked elimination, with the panel read as LDS broadcasts). The
-// block load issues all of a lane's reads at once and the pivot thresholds
-// are read once, so no global-memory latency sits inside the elimination.
-// The factor goes through LDS to the inverse, which lane c computes for
-// column c by forward substitution. Writes L_KK (upper part zeroed) back to
-// A and L_KK^-1 to Linv[K]. A pivot <= tiny * (original diagonal of that DOF)
-// sets *flag.
 __global__ void __launch_bounds__(64) k_chol_diag(double* __restrict__ A, int64_t ld, int K,
                                                   double* __restrict__ Linv, const double* __restrict__ d0,
                                                   double tiny, int* flag) {
     constexpr int SB = 16;           // sub-block
+    constexpr int ILP = 4;           // partial sums of the inverse's dot products
     __shared__ double a[kCB * kCS];
+    __shared__ double rdiag[kCB];
     double* Ab = A + (int64_t)K * kCB * ld + (int64_t)K * kCB;
     const int i = threadIdx.x;
     // the block: all 64 loads of a lane in flight together
@@ -78,29 +82,38 @@ __global__ void __launch_bounds__(64) k_chol_diag(double* __restrict__ A, int64_
     double* ri = a + i * kCS;        // this lane's row
 #pragma unroll
     for (int c0 = 0; c0 < kCB; c0 += SB) {
+        double seg[SB];
 #pragma unroll
-        for (int j = c0; j < c0 + SB; j++) {
-            const double d = a[j * kCS + j];
+        for (int m = 0; m < SB; m++) seg[m] = ri[c0 + m];
+#pragma unroll
+        for (int m = 0; m < SB; m++) {
+            const int j = c0 + m;
+            double colv[SB];         // a_kj of the lanes k below the pivot
+#pragma unroll
+            for (int m2 = m + 1; m2 < SB; m2++) colv[m2] = readlane_d(seg[m], c0 + m2);
+            const double d = readlane_d(seg[m], j);
             if (i == j && !(d > thr_i)) *flag = 1;
-            const double ljj = d > 0 ? sqrt(d) : 1.0;
-            const double lij = i == j ? ljj : (i > j ? ri[j] / ljj : 0.0);
-            __syncthreads();                 // every lane has read the pivot
-            ri[j] = lij;
-            __syncthreads();                 // column j scaled
+            const double dd = d > 0 ? d : 1.0;
+            double y = __builtin_amdgcn_rsq(dd);
+            y = y * (1.5 - 0.5 * dd * y * y);
+            y = y * (1.5 - 0.5 * dd * y * y);
+            const double ljj = dd * y;
+            const double lij = i == j ? ljj : (i > j ? seg[m] * y : 0.0);
+            seg[m] = lij;
+            if (i == 0) rdiag[j] = y;
 #pragma unroll
-            for (int k = j + 1; k < c0 + SB; k++) ri[k] -= lij * a[k * kCS + j];
-            __syncthreads();
+            for (int m2 = m + 1; m2 < SB; m2++) seg[m2] -= lij * (colv[m2] * y);
         }
-        if (c0 + SB < kCB) {
-            double l[SB];
 #pragma unroll
-            for (int j = 0; j < SB; j++) l[j] = ri[c0 + j];
+        for (int m = 0; m < SB; m++) ri[c0 + m] = seg[m];
+        __syncthreads();
+        if (c0 + SB < kCB) {
 #pragma unroll 4
             for (int k = c0 + SB; k < kCB; k++) {
                 const double* rk = a + k * kCS + c0;
                 double v = ri[k];
 #pragma unroll
-                for (int j = 0; j < SB; j++) v -= l[j] * rk[j];
+                for (int m = 0; m < SB; m++) v -= seg[m] * rk[m];
                 ri[k] = v;
             }
             __syncthreads();
@@ -120,10 +133,15 @@ __global__ void __launch_bounds__(64) k_chol_diag(double* __restrict__ A, int64_
     const int c = i;
 #pragma unroll
     for (int rr = 0; rr < kCB; rr++) {
-        double s = rr == c ? 1.0 : 0.0;
+        double s[ILP];
 #pragma unroll
-        for (int k = 0; k < rr; k++) s -= a[rr * kCS + k] * r[k];
-        r[rr] = s / a[rr * kCS + rr];
+        for (int q = 0; q < ILP; q++) s[q] = 0.0;
+#pragma unroll
+        for (int k = 0; k < rr; k++) s[k % ILP] += a[rr * kCS + k] * r[k];
+        double t = rr == c ? 1.0 : 0.0;
+#pragma unroll
+        for (int q = 0; q < ILP; q++) t -= s[q];
+        r[rr] = t * rdiag[rr];
     }
     double* Li = Linv + (int64_t)K * kCB * kCB;
 #pragma unroll
@@ -166,12 +184,24 @@ __global__ void __launch_bounds__(256) k_chol_panel(double* __restrict__ A, int6
         for (int jj = 0; jj < 4; jj++) Ab[(int64_t)(r0 + i) * ld + c0 + jj] = acc[i][jj];
 }
 
-// A_IJ -= A_IK A_JK^T for K < J <= I <= min(K + wb, NB - 1); blockIdx.x runs
-// over the lower-triangular pairs (I, J) of the window
-__global__ void __launch_bounds__(256) k_chol_update(double* __restrict__ A, int64_t ld, int K, int NB, int wb) {
-    int p = (int)blockIdx.x, di = 0;
-    while (p > di) { p -= di + 1; di++; }
-    const int I = K + 1 + di, J = K + 1 + p;
+// A_IJ -= A_IK A_JK^T for K < J <= I <= min(K + wb, NB - 1). part 0: every
+// pair of the window (blockIdx.x runs over its lower-triangular pairs);
+// part 1: the next block column only (J = K + 1, blockIdx.x = I - K - 1),
+// which is all the next diagonal and panel steps need; part 2: the other
+// pairs (J >= K + 2), left to run beside those steps.
+__global__ void __launch_bounds__(256) k_chol_update(double* __restrict__ A, int64_t ld, int K, int NB, int wb,
+                                                     int part = 0) {
+    int I, J;
+    if (part == 1) {
+        I = K + 1 + (int)blockIdx.x;
+        J = K + 1;
+    } else {
+        int p = (int)blockIdx.x, di = 0;
+        while (p > di) { p -= di + 1; di++; }
+        const int o = part == 2 ? 2 : 1;
+        I = K + o + di;
+        J = K + o + p;
+    }
     if (I >= NB) return;
     __shared__ double x[kCB * kCS];
     __shared__ double y[kCB * kCS];
@@ -212,7 +242,7 @@ __global__ void __launch_bounds__(256) k_transpose(const TI* __restrict__ in, TO
 
 // FP16 copy of a lower-triangular Z with one scale per row: Zq[i][j] =
 // Z[i][j] / s_i, s_i = max_j |Z[i][j]| (1 for a zero row), s2[i] = s_i^2.
-// Z^T (Z b) = Zq^T (S^2 (Zq b)) with the transposed copy of the same Zq: the
+// Z^T (Z b) = Zq^T (S^2 (Zq b)) with the same Zq in both products: the
 // operator applied stays symmetric positive definite whatever the rounding.
 // One wavefront per row.
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
@@ -267,10 +297,14 @@ __global__ void __launch_bounds__(256) k_tri_apply(const TK* __restrict__ M, int
     const int64_t nvr = n / VN;
     for (int64_t j = v0 + lane; j < v1; j += 64) {
         double b[NR][VN];
+        float b32[NR][VN];
 #pragma unroll
         for (int q = 0; q < NR; q++)
 #pragma unroll
-            for (int u = 0; u < VN; u++) b[q][u] = (double)bf[q * n + j * VN + u];
+            for (int u = 0; u < VN; u++) {
+                if constexpr (VN == 8) b32[q][u] = (float)bf[q * n + j * VN + u];
+                else b[q][u] = (double)bf[q * n + j * VN + u];
+            }
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             if constexpr (VN == 2) {
@@ -278,13 +312,16 @@ __global__ void __launch_bounds__(256) k_tri_apply(const TK* __restrict__ M, int
 #pragma unroll
                 for (int q = 0; q < NR; q++) acc[r][q] += a.x * b[q][0] + a.y * b[q][1];
             } else if constexpr (VN == 8) {
+                // FP16 entries: the 8 products of a vector are summed in
+                // FP32 (errors ~1e-7, far below the 2^-11 of the entries),
+                // the vector sums in FP64
                 const half8_t a = ((const half8_t*)(M + (row0 + r) * n))[j];
 #pragma unroll
                 for (int q = 0; q < NR; q++) {
-                    double t = 0.0;
+                    float t = 0.0f;
 #pragma unroll
-                    for (int u = 0; u < 8; u++) t += (double)(float)a[u] * b[q][u];
-                    acc[r][q] += t;
+                    for (int u = 0; u < 8; u++) t = fmaf((float)a[u], b32[q][u], t);
+                    acc[r][q] += (double)t;
                 }
             } else {
                 const float4 a = ((const float4*)(M + (row0 + r) * n))[j];
@@ -310,6 +347,185 @@ __global__ void __launch_bounds__(256) k_tri_apply(const TK* __restrict__ M, int
                 else if (i < nf) x[q * ndof + freeIdx[i]] = (T)acc[r][q];
             }
         }
+}
+
+// Column-chunked form of k_tri_apply for long rows: block (ch, g) covers RW
+// rows per wave (4 waves, 4 RW rows) over the columns [ch cw, (ch+1) cw)
+// of their triangular range and leaves one FP64 partial per row and chunk,
+// part[(q nch + ch) n + i] (0 where the chunk misses the range). No wave
+// walks more than cw columns, so the kernel is not held by the latency chain
+// of the longest rows; k_tri_sum adds the chunks in ascending order.
+template <int NR, typename TK, typename TB, int RW = 4>
+__global__ void __launch_bounds__(256) k_tri_part(const TK* __restrict__ M, int64_t n, int upper,
+                                                  const TB* __restrict__ bf, double* __restrict__ part, int cw, int nch,
+                                                  const int* gate) {
+    bool live[NR];
+    bool any = false;
+#pragma unroll
+    for (int q = 0; q < NR; q++) {
+        live[q] = !(gate && gate[(int64_t)q * kGateStride]);
+        any = any || live[q];
+    }
+    if (!any) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t row0 = (((int64_t)blockIdx.y * blockDim.x + threadIdx.x) >> 6) * RW;
+    if (row0 >= n) return;
+    const int ch = blockIdx.x;
+    constexpr int VN = 16 / (int)sizeof(TK);
+    const int64_t c0 = (int64_t)ch * cw;
+    const int64_t c1 = c0 + cw < n ? c0 + cw : n;
+    // triangular range in whole VN-vectors (the other triangle of M is 0)
+    const int64_t lo = upper ? (c0 > row0 / VN * VN ? c0 : row0 / VN * VN) : c0;
+    const int64_t hv = (row0 + RW + VN - 1) / VN * VN;
+    const int64_t hi = upper ? c1 : (c1 < hv ? c1 : hv);
+    double acc[RW][NR];
+#pragma unroll
+    for (int r = 0; r < RW; r++)
+#pragma unroll
+        for (int q = 0; q < NR; q++) acc[r][q] = 0.0;
+    for (int64_t j = lo / VN + lane; j < hi / VN; j += 64) {
+        double b[NR][VN];
+        float b32[NR][VN];
+#pragma unroll
+        for (int q = 0; q < NR; q++)
+#pragma unroll
+            for (int u = 0; u < VN; u++) {
+                if constexpr (VN == 8) b32[q][u] = (float)bf[q * n + j * VN + u];
+                else b[q][u] = (double)bf[q * n + j * VN + u];
+            }
+#pragma unroll
+        for (int r = 0; r < RW; r++) {
+            if constexpr (VN == 2) {
+                const double2 a = ((const double2*)(M + (row0 + r) * n))[j];
+#pragma unroll
+                for (int q = 0; q < NR; q++) acc[r][q] += a.x * b[q][0] + a.y * b[q][1];
+            } else if constexpr (VN == 8) {
+                // FP16 entries: the 8 products of a vector are summed in
+                // FP32 (errors ~1e-7, far below the 2^-11 of the entries),
+                // the vector sums in FP64
+                const half8_t a = ((const half8_t*)(M + (row0 + r) * n))[j];
+#pragma unroll
+                for (int q = 0; q < NR; q++) {
+                    float t = 0.0f;
+#pragma unroll
+                    for (int u = 0; u < 8; u++) t = fmaf((float)a[u], b32[q][u], t);
+                    acc[r][q] += (double)t;
+                }
+            } else {
+                const float4 a = ((const float4*)(M + (row0 + r) * n))[j];
+#pragma unroll
+                for (int q = 0; q < NR; q++)
+                    acc[r][q] += (double)a.x * b[q][0] + (double)a.y * b[q][1] + (double)a.z * b[q][2] +
+                                 (double)a.w * b[q][3];
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < RW; r++)
+#pragma unroll
+        for (int q = 0; q < NR; q++)
+            for (int off = 32; off > 0; off >>= 1) acc[r][q] += __shfl_down(acc[r][q], off, 64);
+    if (lane == 0)
+        for (int q = 0; q < NR; q++)
+            for (int r = 0; r < RW; r++) part[((int64_t)q * nch + ch) * n + row0 + r] = acc[r][q];
+}
+
+// Column sums x_j = sum_i M[i][j] y_i of a triangular FP16 M (row-major,
+// zeros in the other triangle; upper = 1 for an upper-triangular M): each
+// lane streams 16-byte pieces of whole rows, the right-hand side is one
+// uniform value per row, so these tiles read the matrix at close to the
+// copy bandwidth (the row-oriented k_tri_part re-reads the right-hand side
+// with every row piece). Z b is the column sums of Z^T, Z^T y those of Z.
+// Block
+// (cb, rb) covers the rows [rb 256, rb 256 + 256) and the columns
+// [cb 512, cb 512 + 512): lane l owns 8 columns, wave w 64 of the rows
+// (y_i read once per wave as a uniform value); the 4 waves' FP32 column sums
+// meet in LDS and leave one FP64 partial per column and row block,
+// part[(q nrb + rb) n + j] (0 for blocks above the diagonal), summed by
+// k_tri_sum.
+constexpr int kTcRows = 256;      // rows per block
+constexpr int kTcCols = 512;      // columns per block (64 lanes x 8)
+template <int NR, typename TB>
+__global__ void __launch_bounds__(256) k_tri_colpart(const _Float16* __restrict__ M, int64_t n,
+                                                     const TB* __restrict__ y, double* __restrict__ part, int nrb,
+                                                     const int* gate, int upper = 0) {
+    __shared__ float sh[4][NR][kTcCols];
+    const int cb = blockIdx.x, rb = blockIdx.y;
+    const int64_t J0 = (int64_t)cb * kTcCols, I0 = (int64_t)rb * kTcRows;
+    bool live[NR];
+#pragma unroll
+    for (int q = 0; q < NR; q++) live[q] = !(gate && gate[(int64_t)q * kGateStride]);
+    if (upper ? (I0 > J0 + kTcCols - 1) : (J0 > I0 + kTcRows - 1)) {
+        // outside the triangle: zero partials
+        for (int q = 0; q < NR; q++)
+            for (int t = threadIdx.x; t < kTcCols; t += blockDim.x)
+                if (J0 + t < n) part[((int64_t)q * nrb + rb) * n + J0 + t] = 0.0;
+        return;
+    }
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int64_t j = J0 + 8 * lane;
+    const bool colok = j < n;
+    float acc[NR][8];
+#pragma unroll
+    for (int q = 0; q < NR; q++)
+#pragma unroll
+        for (int u = 0; u < 8; u++) acc[q][u] = 0.0f;
+    const int64_t r0 = I0 + (int64_t)w * (kTcRows / 4);
+    const int64_t r1 = r0 + kTcRows / 4 < n ? r0 + kTcRows / 4 : n;
+    if (colok) {
+#pragma unroll 8
+        for (int64_t i = r0; i < r1; i++) {
+            const half8_t a = *(const half8_t*)(M + i * n + j);
+#pragma unroll
+            for (int q = 0; q < NR; q++) {
+                const float yv = (float)y[q * n + i];
+#pragma unroll
+                for (int u = 0; u < 8; u++) acc[q][u] = fmaf((float)a[u], yv, acc[q][u]);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NR; q++)
+#pragma unroll
+        for (int u = 0; u < 8; u++) sh[w][q][8 * lane + u] = acc[q][u];
+    __syncthreads();
+    for (int q = 0; q < NR; q++) {
+        if (!live[q]) continue;
+        for (int t = threadIdx.x; t < kTcCols; t += blockDim.x) {
+            if (J0 + t >= n) continue;
+            const double v = ((double)sh[0][q][t] + (double)sh[1][q][t]) + ((double)sh[2][q][t] + (double)sh[3][q][t]);
+            part[((int64_t)q * nrb + rb) * n + J0 + t] = v;
+        }
+    }
+}
+
+// row sums of the chunk partials, written like k_tri_apply's epilogue:
+// y[q n + i] = (ys ? ys[i] : 1) * sum (x == null), or x[q ndof + free[i]] =
+// sum for i < nf. Four lanes per row, lane s adding the chunks s, s+4, ...;
+// the four sums meet by two exchanges (the same bits in all four lanes).
+template <typename T, int NR, typename TY>
+__global__ void k_tri_sum(const double* __restrict__ part, int nch, int64_t n, TY* __restrict__ y, T* __restrict__ x,
+                          const int32_t* __restrict__ freeIdx, int64_t nf, int64_t ndof, const int* gate,
+                          const double* __restrict__ ys) {
+    const int sub = threadIdx.x & 3;
+    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 2;
+    const bool ok = i < n;
+#pragma unroll
+    for (int q = 0; q < NR; q++) {
+        if (gate && gate[(int64_t)q * kGateStride]) continue;
+        double s = 0.0;
+        if (ok) {
+            const double* pq = part + (int64_t)q * nch * n + i;
+            for (int c = sub; c < nch; c += 4) s += pq[(int64_t)c * n];
+        }
+        s += __shfl_xor(s, 1, 64);
+        s += __shfl_xor(s, 2, 64);
+        if (ok && sub == 0) {
+            if (!x) y[q * n + i] = (TY)(ys ? ys[i] * s : s);
+            else if (i < nf) x[q * ndof + freeIdx[i]] = (T)s;
+        }
+    }
 }
 
 }  // namespace topo

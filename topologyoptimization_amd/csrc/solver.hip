@@ -71,12 +71,15 @@ struct Ctx {
         return blas;
     }
     // second stream (default priority) for work that overlaps a dependency
-    // chain on s, with its own rocBLAS handle and a pool of ordering events
+    // chain on s, with its own rocBLAS handle and a pool of ordering events;
+    // a third one (s3, default priority) for the factorization's look-ahead
     hipStream_t s2 = nullptr;
+    hipStream_t s3 = nullptr;
     rocblas_handle blas2 = nullptr;
     std::vector<hipEvent_t> evs;
     void side_ready(int nev) {
         if (!s2) TOPO_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+        if (!s3) TOPO_HIP(hipStreamCreateWithFlags(&s3, hipStreamNonBlocking));
         if (!blas2) {
             TOPO_CHECK(rocblas_create_handle(&blas2) == rocblas_status_success, TOPO_E_HIP,
                        "rocblas_create_handle failed");
@@ -142,11 +145,13 @@ struct Ctx {
     ~Ctx() {
         if (s) (void)hipStreamSynchronize(s);
         if (s2) (void)hipStreamSynchronize(s2);
+        if (s3) (void)hipStreamSynchronize(s3);
         for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
         if (blas) (void)rocblas_destroy_handle(blas);
         if (blas2) (void)rocblas_destroy_handle(blas2);
         for (hipEvent_t e : evs) (void)hipEventDestroy(e);
         if (s2) (void)hipStreamDestroy(s2);
+        if (s3) (void)hipStreamDestroy(s3);
         for (void* p : allocs) (void)hipFree(p);
         if (h_st) (void)hipHostFree(h_st);
         if (h_oc) (void)hipHostFree(h_oc);
@@ -316,14 +321,37 @@ struct MGLevel {
     double* Z = nullptr;           // L^-1 (lower triangular)
     float* Z32 = nullptr;
     _Float16* Z16 = nullptr;       // Z / s_i per row (coarse_fp32 = 2)
-    _Float16* Zt16 = nullptr;      // its transpose
     double* zs2 = nullptr;         // s_i^2
     double* Linv = nullptr;        // inverses of the diagonal blocks of L
     double* ytmp = nullptr;        // Z b (nrhs x nfp)
+    double* tpart = nullptr;       // chunk partials of the triangular products (nrhs x max(tnch, tnrb) x nfp)
+    int tcw = 0, tnch = 0;         // column chunk width (0: one wave per row quad) and count
+    int tnrb = 0;                  // row blocks of the transposed FP16 product (column tiles of Z16)
     double* d0c = nullptr;         // original diagonal (pivot test)
     double* ztmp = nullptr;        // 64 x nfp scratch of the Z row blocks
     double* work = nullptr;
 };
+
+// One triangular product of the coarse solve (k_tri_apply semantics): by
+// column chunks (k_tri_part + k_tri_sum) when the level has a chunk width,
+// else one wave per row quad.
+template <typename T, typename TK, typename TB, typename TY>
+static void tri_product(Ctx& c, const MGLevel& l, const TK* M, int upper, const TB* bf, TY* y, T* x, const int* gate,
+                        const double* ys, int nr) {
+    if (l.tcw) {
+        constexpr int RW = 4;
+        const dim3 gp((unsigned)l.tnch, (unsigned)(l.nfp / (4 * RW)));
+        TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_part<NR, TK, TB, RW>), gp, dim3(256), 0, c.s, M, l.nfp, upper,
+                                              bf, l.tpart, l.tcw, l.tnch, gate));
+        TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_sum<T, NR, TY>), dim3(nblk(4 * l.nfp)), dim3(256), 0, c.s,
+                                              (const double*)l.tpart, l.tnch, l.nfp, y, x, (const int32_t*)l.freeIdx,
+                                              l.nf, l.ndof, gate, ys));
+    } else {
+        const dim3 gt((unsigned)(l.nfp / 16));
+        TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_apply<T, NR, TK, TB, TY>), gt, dim3(256), 0, c.s, M, l.nfp, upper,
+                                              bf, y, x, (const int32_t*)l.freeIdx, l.nf, l.ndof, gate, ys));
+    }
+}
 
 template <typename T> struct Sel;
 template <> struct Sel<double> {
@@ -958,6 +986,14 @@ struct MGH {
             l.Kinv = c.alloc<double>(l.nfp * l.nfp);
             l.bf = c.alloc<double>(l.nfp * nrhs);
             l.ytmp = c.alloc<double>(l.nfp * nrhs);
+            {
+                const char* e = std::getenv("TOPO_TRI_CW");
+                l.tcw = e ? std::atoi(e) : 2048;
+                if (l.tcw < 0 || l.tcw % 64) l.tcw = 2048;
+                l.tnch = l.tcw ? (int)((l.nfp + l.tcw - 1) / l.tcw) : 0;
+                l.tnrb = (int)((l.nfp + kTcRows - 1) / kTcRows);
+                l.tpart = c.alloc<double>((int64_t)nrhs * std::max(l.tnch, l.tnrb) * l.nfp);
+            }
             l.d0c = c.alloc<double>(l.nfp);
             if (want32 && coarse_fp32) l.Kinv32 = c.alloc<float>(l.nfp * l.nfp);
         }
@@ -974,7 +1010,6 @@ struct MGH {
                 if (want32 && coarse_fp32 == 1) l.Z32 = c.alloc<float>(n * n);
                 if (want32 && coarse_fp32 == 2) {
                     l.Z16 = c.alloc<_Float16>(n * n);
-                    l.Zt16 = c.alloc<_Float16>(n * n);
                     l.zs2 = c.alloc<double>(n);
                 }
             }
@@ -985,10 +1020,18 @@ struct MGH {
             // is final once the diagonal step I is done (the panels of the
             // earlier steps wrote its band), and the factorization goes on
             // below it, so the DGEMMs of row I overlap steps I+1, ...
-            c.side_ready(NB + 2);
+            // events: [0, NB) diagonal steps -> Z rows, [NB, 2 NB) panels ->
+            // the rest of their windows, [2 NB, 3 NB) those rests done,
+            // 3 NB and 3 NB + 1 the joins
+            c.side_ready(3 * NB + 2);
             if (!l.ztmp) l.ztmp = c.alloc<double>(kCB * n);
             hipLaunchKernelGGL(k_fill<double>, dim3(nblk(n * n)), dim3(256), 0, c.s, l.Z, 0.0, n * n,
                                (const int*)nullptr);
+            // look-ahead: the update of step K is split into the next block
+            // column (on s: all that the diagonal and panel steps K+1 read)
+            // and the rest of the window (on s3, beside those steps). The
+            // rest of step K writes blocks of columns >= K+2, so s waits for
+            // it before the column update of step K+1, the next write there.
             for (int K = 0; K < NB; K++) {
                 hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(64), 0, c.s, l.Kinv, n, K, l.Linv,
                                    (const double*)l.d0c, 1e-11, dflag);
@@ -998,11 +1041,20 @@ struct MGH {
                 if (nwin > 0) {
                     hipLaunchKernelGGL(k_chol_panel, dim3(nwin), dim3(256), 0, c.s, l.Kinv, n, K, NB, wb,
                                        (const double*)l.Linv);
-                    hipLaunchKernelGGL(k_chol_update, dim3(nwin * (nwin + 1) / 2), dim3(256), 0, c.s, l.Kinv, n, K,
-                                       NB, wb);
+                    if (nwin > 1) {
+                        c.order(c.s, c.s3, NB + K);
+                        hipLaunchKernelGGL(k_chol_update, dim3(nwin * (nwin - 1) / 2), dim3(256), 0, c.s3, l.Kinv, n,
+                                           K, NB, wb, 2);
+                    }
+                }
+                TOPO_HIP(hipEventRecord(c.evs[2 * NB + K], c.s3));
+                if (nwin > 0) {
+                    if (K > 0) TOPO_HIP(hipStreamWaitEvent(c.s, c.evs[2 * NB + K - 1], 0));
+                    hipLaunchKernelGGL(k_chol_update, dim3(nwin), dim3(256), 0, c.s, l.Kinv, n, K, NB, wb, 1);
                 }
             }
-            c.order(c.s2, c.s, NB);
+            c.order(c.s3, c.s, 3 * NB);
+            c.order(c.s2, c.s, 3 * NB + 1);
             c.check_launch();
             TOPO_HIP(hipMemcpyAsync(&c.h_sc[0], dflag, sizeof(int), hipMemcpyDeviceToHost, c.s));
             c.sync();
@@ -1015,8 +1067,6 @@ struct MGH {
                 if (l.Z16) {
                     hipLaunchKernelGGL(k_rowscale_f16, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, c.s,
                                        (const double*)l.Z, n, l.Z16, l.zs2);
-                    hipLaunchKernelGGL((k_transpose<_Float16, _Float16>), dim3(NB, NB), dim3(256), 0, c.s,
-                                       (const _Float16*)l.Z16, l.Zt16, n);
                 }
                 if (l.Z32) {
                     hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(n * n)), dim3(256), 0, c.s, l.Z32,
@@ -1281,30 +1331,30 @@ struct MGH {
                 // triangular Z^T scattered to the free DOFs; the FP32 V-cycle
                 // reads the rounded copies (coarse_fp32: FP32, or FP16 with
                 // the row scales applied to ytmp)
-                const dim3 gt((unsigned)(l.nfp / 16));
                 if (f16) {
-                    TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_apply<T, NR, _Float16, float, float>), gt, dim3(256),
-                                                          0, c.s, (const _Float16*)l.Z16, l.nfp, 0,
-                                                          (const float*)l.bf, (float*)l.ytmp, (T*)nullptr, l.freeIdx,
-                                                          l.nf, l.ndof, gate, (const double*)l.zs2));
-                    TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_apply<T, NR, _Float16, float, float>), gt, dim3(256),
-                                                          0, c.s, (const _Float16*)l.Zt16, l.nfp, 1,
-                                                          (const float*)l.ytmp, (float*)nullptr, x, l.freeIdx, l.nf,
-                                                          l.ndof, gate, (const double*)nullptr));
+                    // S^2 Zq b by row chunks, then Zq^T y by column tiles of
+                    // the same matrix (just read: the second pass finds much
+                    // of it in the memory-side cache)
+                    tri_product<T, _Float16, float, float>(c, l, l.Z16, 0, (const float*)l.bf, (float*)l.ytmp,
+                                                           (T*)nullptr, gate, l.zs2, nr);
+                    const dim3 gc((unsigned)((l.nfp + kTcCols - 1) / kTcCols), (unsigned)l.tnrb);
+                    TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_colpart<NR, float>), gc, dim3(256), 0, c.s,
+                                                          (const _Float16*)l.Z16, l.nfp, (const float*)l.ytmp, l.tpart,
+                                                          l.tnrb, gate, 0));
+                    TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_sum<T, NR, float>), dim3(nblk(4 * l.nfp)), dim3(256),
+                                                          0, c.s, (const double*)l.tpart, l.tnrb, l.nfp,
+                                                          (float*)nullptr, x, (const int32_t*)l.freeIdx, l.nf, l.ndof,
+                                                          gate, (const double*)nullptr));
                 } else if (f32 && l.Z32) {
-                    TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_apply<T, NR, float>), gt, dim3(256), 0, c.s,
-                                                          (const float*)l.Z32, l.nfp, 0, (const double*)l.bf, l.ytmp,
-                                                          (T*)nullptr, l.freeIdx, l.nf, l.ndof, gate));
-                    TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_apply<T, NR, float>), gt, dim3(256), 0, c.s,
-                                                          (const float*)l.Kinv32, l.nfp, 1, (const double*)l.ytmp,
-                                                          (double*)nullptr, x, l.freeIdx, l.nf, l.ndof, gate));
+                    tri_product<T, float, double, double>(c, l, l.Z32, 0, (const double*)l.bf, l.ytmp, (T*)nullptr,
+                                                          gate, nullptr, nr);
+                    tri_product<T, float, double, double>(c, l, l.Kinv32, 1, (const double*)l.ytmp, (double*)nullptr,
+                                                          x, gate, nullptr, nr);
                 } else {
-                    TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_apply<T, NR, double>), gt, dim3(256), 0, c.s,
-                                                          (const double*)l.Z, l.nfp, 0, (const double*)l.bf, l.ytmp,
-                                                          (T*)nullptr, l.freeIdx, l.nf, l.ndof, gate));
-                    TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_apply<T, NR, double>), gt, dim3(256), 0, c.s,
-                                                          (const double*)l.Kinv, l.nfp, 1, (const double*)l.ytmp,
-                                                          (double*)nullptr, x, l.freeIdx, l.nf, l.ndof, gate));
+                    tri_product<T, double, double, double>(c, l, l.Z, 0, (const double*)l.bf, l.ytmp, (T*)nullptr,
+                                                           gate, nullptr, nr);
+                    tri_product<T, double, double, double>(c, l, l.Kinv, 1, (const double*)l.ytmp, (double*)nullptr,
+                                                           x, gate, nullptr, nr);
                 }
                 return x;
             }
@@ -1565,9 +1615,10 @@ static CGResult pcg_mg(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVecs& 
             }
             // r_lo: the V-cycle's FP32 residual, written by the update itself
             PT* r_lo = std::is_same<T, PT>::value ? nullptr : (PT*)w32.r;
+            // (U += alpha p is done by the p update below)
             TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_cgp_update_ur<T, TU, NR, PT>), dim3(kCgBlocks),
-                                                   dim3(kRedThreads), 0, c.s, U, r, (const T*)p, (const T*)Ap, n,
-                                                   c.st, c.partials, r_lo, nb0));
+                                                   dim3(kRedThreads), 0, c.s, (TU*)nullptr, r, (const T*)p,
+                                                   (const T*)Ap, n, c.st, c.partials, r_lo, nb0));
             // z is never formed: r.z and p = z + beta p read the V-cycle's
             // output, which is zero at the fixed DOFs (the mask of ebeMGPCG
             // :1170-1172 is a no-op on it)
@@ -1575,9 +1626,9 @@ static CGResult pcg_mg(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVecs& 
             TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_cgp_rz<T, PT, NR, false>), dim3(kCgBlocks), dim3(kRedThreads), 0,
                                                    c.s, (const T*)r, zz, (const uint8_t*)l.fixed, n, c.st,
                                                    c.partials));
-            TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_cgp_pupdate<T, PT, NR, false>), dim3(kCgBlocks), dim3(kRedThreads), 0,
-                                                   c.s, p, zz, (const uint8_t*)l.fixed, n, c.st,
-                                                   (const double*)c.partials));
+            TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_cgp_pupdate<T, PT, NR, false, TU>), dim3(kCgBlocks),
+                                                   dim3(kRedThreads), 0, c.s, p, zz, (const uint8_t*)l.fixed, n,
+                                                   c.st, (const double*)c.partials, U, nb0));
             TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_cgp_finish<NR>), dim3(1), dim3(kRedThreads), 0, c.s, c.st,
                                                    (const double*)c.partials, nb0));
         }
