@@ -68,7 +68,8 @@ def test_autotuner_and_simp_steps(gpu):
     assert len(batches) == 3 and np.all(batches[:, 3] > 0)
     assert R == -int(batches[np.argmin(batches[:, 3]), 0])
     fine = tiles[tiles[:, 0] == 0]
-    assert len(fine) == 10 and np.all(fine[:, 3] > 0)
+    assert len(fine) == 8 and np.all(fine[:, 3] > 0)
+    assert np.all(fine[:, 2] % 2 == 0)   # even x-chunks (0 = the size rule, made even)
     best = fine[np.argmin(fine[:, 3])]
     assert (nt, cx) == (int(best[1]), int(best[2]))
     print("autotuned: R=%d NT=%d CX=%d, fine operator %.4f ms" % (R, nt, cx, best[3]))

@@ -541,7 +541,7 @@ __global__ void k_cg_check(CGState* st, int nr) {
 constexpr int kCgBlocks = 1024;
 constexpr int kCgStride = 8192;
 constexpr int kCgRows = 4;   // right-hand sides per quantity
-__device__ __forceinline__ int64_t cg_row(int k, int q) { return (int64_t)(k * kCgRows + q) * kCgStride; }
+__host__ __device__ __forceinline__ int64_t cg_row(int k, int q) { return (int64_t)(k * kCgRows + q) * kCgStride; }
 
 // v[i] -> row (k0 + i / nr, i % nr), this block's column
 template <int NV>
@@ -552,8 +552,10 @@ __device__ __forceinline__ void put_partials(double (&v)[NV], double* part, int 
         for (int i = 0; i < NV; i++) part[cg_row(k0 + i / nr, i % nr) + blockIdx.x] = v[i];
 }
 // tot[i] = sum of row (k0 + i / nr, i % nr) over its nb columns, in a
-// fixed order (thread t takes columns t, t + 256, ...; then the block tree);
-// every thread of the block gets the result
+// fixed order (thread t takes columns t + 256 m into 8 accumulators by
+// m mod 8, which meet in a fixed tree; then the block tree), so the loads
+// of a thread are in flight together; every thread of the block gets the
+// result
 template <int NV>
 __device__ __forceinline__ void sum_partials(const double* part, int k0, int nr, double (&tot)[NV],
                                              int nb = kCgBlocks) {
@@ -563,9 +565,16 @@ __device__ __forceinline__ void sum_partials(const double* part, int k0, int nr,
 #pragma unroll
     for (int i = 0; i < NV; i++) {
         const double* row = part + cg_row(k0 + i / nr, i % nr);
-        double s = 0;
-        for (int b = threadIdx.x; b < nb; b += kRedThreads) s += row[b];
-        w[i] = s;
+        double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        int b = threadIdx.x;
+        for (; b + 7 * kRedThreads < nb; b += 8 * kRedThreads) {
+#pragma unroll
+            for (int u = 0; u < 8; u++) a[u] += row[b + u * kRedThreads];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            if (b + u * kRedThreads < nb) a[u] += row[b + u * kRedThreads];
+        w[i] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
     }
     block_reduce<NV>(w, sh);
     if (threadIdx.x == 0)
@@ -690,10 +699,11 @@ template <typename T, typename TZ, int NR, bool MASK, typename TU = T>
 __global__ void __launch_bounds__(kRedThreads) k_cgp_pupdate(T* __restrict__ p, const TZ* __restrict__ zz,
                                                             const uint8_t* __restrict__ fixed, int64_t n,
                                                             const CGState* st, const double* part,
-                                                            TU* __restrict__ U = nullptr, int nb0 = kCgBlocks) {
+                                                            TU* __restrict__ U = nullptr, int nb0 = kCgBlocks,
+                                                            int nb2 = kCgBlocks) {
     if (all_done(st, NR)) return;
     double rz[NR];
-    sum_partials<NR>(part, 2, NR, rz);
+    sum_partials<NR>(part, 2, NR, rz, nb2);
     double pAp[NR];
     if (U) sum_partials<NR>(part, 0, NR, pAp, nb0);
 #pragma unroll
@@ -720,17 +730,21 @@ __global__ void __launch_bounds__(kRedThreads) k_cgp_pupdate(T* __restrict__ p, 
 // the books of one iteration (one block of kRedThreads): alpha, ||r||^2,
 // rz, beta, iteration count, non-finite flag, convergence
 template <int NR>
-__global__ void __launch_bounds__(kRedThreads) k_cgp_finish(CGState* st, const double* part, int nb0) {
+__global__ void __launch_bounds__(kRedThreads) k_cgp_finish(CGState* st, const double* part, int nb0,
+                                                           int nb2 = kCgBlocks) {
     if (all_done(st, NR)) return;
     double pAp[NR], tot[3 * NR];
     sum_partials<NR>(part, 0, NR, pAp, nb0);
     {
-        double rest[2 * NR];
-        sum_partials<2 * NR>(part, 1, NR, rest);
+        double rn[NR], rz[NR];
+        sum_partials<NR>(part, 1, NR, rn);
+        sum_partials<NR>(part, 2, NR, rz, nb2);
 #pragma unroll
-        for (int q = 0; q < NR; q++) tot[q] = pAp[q];
-#pragma unroll
-        for (int q = 0; q < 2 * NR; q++) tot[NR + q] = rest[q];
+        for (int q = 0; q < NR; q++) {
+            tot[q] = pAp[q];
+            tot[NR + q] = rn[q];
+            tot[2 * NR + q] = rz[q];
+        }
     }
     if (threadIdx.x != 0) return;
     for (int q = 0; q < NR; q++) {

@@ -122,22 +122,29 @@ __host__ __device__ inline size_t xm_lds_bytes(int nnz, size_t elt) {
 // is formed from b and invD as the planes are loaded (p is not read), and the
 // output is the mode-3 sweep from x0 (the staged own b, invD and x0 serve the
 // epilogue). A template parameter so each variant holds only the registers
-// its epilogue needs. With `part` (MODE 1), each workgroup also leaves its
-// share of p.Ap (own p of each output node times its output, FP64, fixed
-// order) at part[blockIdx.y * kCgStride + blockIdx.x] for the CG update.
+// its epilogue needs. With `part` (MODE 1), each workgroup also leaves, per
+// pair k of output node planes (2k, 2k+1) of its yz tile bx, the pair's share
+// of p.Ap (own p of each output node times its output, FP64, fixed order) at
+// part[blockIdx.y * kCgStride + bx * ceil(nnx/2) + k] for the CG update
+// (gx * ceil(nnx/2) partials whatever the even CX, so the CG scalars do not
+// depend on the autotuned x-chunking); MODE 5 is the mode-3 sweep that
+// leaves the shares of r.z the same way (r = rdot, the CG's FP64 residual;
+// z = the sweep's output, the V-cycle's last). CX must be even.
 template <typename T, typename TE, int NT, int MODE>
 __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __restrict__ out,
                                                   const TE* __restrict__ E, double thr, Grid g,
                                                   const uint8_t* __restrict__ fixed,
                                                   const T* __restrict__ bvec, const T* __restrict__ invD,
                                                   const double* __restrict__ omp, const int* gate, HadCoef<T> co,
-                                                  int CX, int gx, int gy, double* __restrict__ part) {
+                                                  int CX, int gx, int gy, double* __restrict__ part,
+                                                  const double* __restrict__ rdot = nullptr) {
     gate = batch_gate(gate);
     if (gate && *gate) return;
     {
         const int64_t bo = batch_off(g.ndof());
         p += bo; out += bo;
         if (bvec) bvec += bo;
+        if (rdot) rdot += bo;
     }
     // XCD-aware work order: hardware places workgroup L on XCD L % 8; work
     // item w = (L % 8) * ceil(N/8) + L / 8 gives each XCD a contiguous range
@@ -146,13 +153,9 @@ __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __
     const int nwork = gx * gy;
     const int per = (nwork + 7) / 8;
     const int w = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
-    if (w >= nwork) {
-        // padding workgroup: an empty share
-        if (MODE == 1 && part && threadIdx.x == 0) part[(int64_t)blockIdx.y * kCgStride + blockIdx.x] = 0.0;
-        return;
-    }
-    double pap = 0.0;   // this thread's share of p.Ap (MODE 1 with part)
+    if (w >= nwork) return;   // padding workgroup
     const int bx = w / gy, by = w - bx * gy;
+    __shared__ double s_red[16 * (NT / 64)];   // per-plane-pair wave sums (MODE 1, 5 with part; CX <= 32)
     const T omega = omp ? (T)*omp : T(0);
     extern __shared__ __align__(16) unsigned char xm_smem[];
     const int nnz = g.nnz(), nny = g.nny(), nnx = g.nnx();
@@ -263,12 +266,14 @@ __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __
             for (int d = 0; d < 3; d++) { eb[d] = stgb[d]; ed[d] = stgd[d]; }
         }
         uint8_t ef[3] = {0, 0, 0};
+        double er[3] = {0.0, 0.0, 0.0};
         if (outstep && MODE >= 1) {
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 ef[d] = fixed[obase + d];
-                if (MODE == 2 || MODE == 3) eb[d] = bvec[obase + d];
-                if (MODE == 3) ed[d] = invD[obase + d];
+                if (MODE == 2 || MODE == 3 || MODE == 5) eb[d] = bvec[obase + d];
+                if (MODE == 3 || MODE == 5) ed[d] = invD[obase + d];
+                if (MODE == 5) er[d] = rdot[obase + d];
             }
         }
         T A[12], Bq[12];
@@ -317,6 +322,7 @@ __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __
         __syncthreads();
         // node (iy, iz) = face(iy,iz).q00 + face(iy,iz-1).q01 + face(iy-1,iz).q10
         //               + face(iy-1,iz-1).q11 (faces beyond the grid carry zeros)
+        double cpl = 0.0;   // this thread's share of the plane's p.Ap (MODE 1) or r.z (MODE 5)
         if (outstep) {
 #pragma unroll
             for (int d = 0; d < 3; d++) {
@@ -326,24 +332,34 @@ __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __
                     if (ef[d]) v = T(0);
                 } else if (MODE == 2) {
                     v = ef[d] ? T(0) : eb[d] - v;
-                } else if (MODE == 3 || MODE == 4) {
+                } else if (MODE == 3 || MODE == 4 || MODE == 5) {
                     const T r = eb[d] - v;
                     v = ef[d] ? ep[d] : ep[d] + omega * ed[d] * r;
                 }
-                if (MODE == 1) pap += (double)ep[d] * (double)v;
+                if (MODE == 1) cpl += (double)ep[d] * (double)v;
+                if (MODE == 5) cpl += er[d] * (double)v;
                 out[obase + d] = v;
             }
         }
+        // one partial per (yz tile, node-plane pair 2k, 2k+1): the same
+        // sums whatever the (even) x-chunking, so the CG scalars do not
+        // depend on the element tiles. Each wave adds its sum of the plane to
+        // its own LDS slot of the pair (no barrier here); the slots are
+        // added after the march.
+        if ((MODE == 1 || MODE == 5) && part && ex >= X0) {
+            for (int off = 32; off > 0; off >>= 1) cpl += __shfl_down(cpl, off, 64);
+            if ((t & 63) == 0) {
+                double& sl = s_red[((ex - X0) >> 1) * (NT / 64) + (t >> 6)];
+                sl = ((ex - X0) & 1) ? sl + cpl : cpl;
+            }
+        }
     }
-    if (MODE == 1 && part) {
-        __shared__ double red[NT / 64];
-        for (int off = 32; off > 0; off >>= 1) pap += __shfl_down(pap, off, 64);
-        if ((t & 63) == 0) red[t >> 6] = pap;
+    if ((MODE == 1 || MODE == 5) && part) {
         __syncthreads();
-        if (t == 0) {
+        for (int k = t; 2 * k < X1 - X0; k += NT) {
             double sum = 0.0;
-            for (int k = 0; k < NT / 64; k++) sum += red[k];
-            part[(int64_t)blockIdx.y * kCgStride + blockIdx.x] = sum;
+            for (int wv = 0; wv < NT / 64; wv++) sum += s_red[k * (NT / 64) + wv];
+            part[(int64_t)blockIdx.y * kCgStride + (int64_t)bx * ((nnx + 1) / 2) + (X0 >> 1) + k] = sum;
         }
     }
 }

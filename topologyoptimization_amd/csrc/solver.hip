@@ -734,6 +734,7 @@ struct MGH {
     bool exact = false;   // reference-order kernel everywhere
     int xm_nt = 0;        // x-march workgroup size for every level (0 auto, 512, 1024)
     int xm_cx = 0;        // x-march planes per workgroup for every level (0 auto)
+    bool rz_fuse = true;  // r.z partials from the V-cycle's last sweep (else a separate kernel)
     std::vector<int> lv_nt, lv_cx;   // per-level choices of the autotuner (0 auto)
     DevKe* K0 = nullptr;
     Springs springs{};    // mechanism ports on the finest level
@@ -1243,14 +1244,17 @@ struct MGH {
     }
 
     // out = A in (mode 0), masked (1), b - A in (2), Jacobi sweep (3), the
-    // first two sweeps from x = 0 (4: `in` unused; x-march levels only)
-    // With pap (mode 1, x-march level, no springs) the launch also leaves its
-    // p.Ap partials there and returns their count (0 when it cannot).
+    // first two sweeps from x = 0 (4: `in` unused; x-march levels only), a
+    // Jacobi sweep that also leaves the r.z partials of its output with r =
+    // rdot (5). With pap (mode 1 or 5, x-march level, no springs) the launch
+    // leaves its partials there and returns their count (0 when it cannot:
+    // then mode 5 runs as mode 3).
     template <typename T>
     int apply(Ctx& c, int li, const T* in, T* out, int mode, const T* b, const int* gate, int nr = 1,
-              double* pap = nullptr) {
+              double* pap = nullptr, const double* rdot = nullptr) {
         int npart = 0;
         MGLevel& l = L[li];
+        if (mode == 5 && !(rdot && pap && xm_level(li) && !(li == 0 && springs.n > 0))) mode = 3;
         const double* om = mode >= 3 ? omp(li) : nullptr;
         TOPO_CHECK(mode != 4 || (xm_level(li) && !(li == 0 && springs.n > 0)), TOPO_E_UNSUPPORTED,
                    "fused first sweep on a level without the x-march operator");
@@ -1268,23 +1272,28 @@ struct MGH {
                 const int gx = (nyz + (NT - H) - 1) / (NT - H);
                 int CX = cx_sel > 0 ? cx_sel : (int)((int64_t)l.g.nnx() * gx * (NT / 512) / 1024);
                 CX = std::max(2, std::min(32, CX));
+                CX += CX & 1;   // even: the CG partials cover whole node-plane pairs
                 const int gy = (l.g.nnx() + CX - 1) / CX;
                 const dim3 grid(8 * ((gx * gy + 7) / 8), nr);
-                double* part = (mode == 1 && pap && (int)grid.x <= kCgStride && !(li == 0 && springs.n > 0)) ? pap
-                                                                                                             : nullptr;
-                if (part) npart = (int)grid.x;
+                // CG partials: one per (yz tile, node-plane pair), whatever CX
+                const int64_t nplane = (int64_t)gx * ((l.g.nnx() + 1) / 2);
+                double* part = ((mode == 1 || mode == 5) && pap && nplane <= kCgStride &&
+                                !(li == 0 && springs.n > 0)) ? pap : nullptr;
+                if (mode == 5 && !part) mode = 3;
+                if (part) npart = (int)nplane;
                 const size_t lds = NT == 512 ? xm_lds_bytes<512>(l.g.nnz(), sizeof(T))
                                              : xm_lds_bytes<1024>(l.g.nnz(), sizeof(T));
 #define TOPO_XM_LAUNCH(NTV, MV)                                                                                  \
     hipLaunchKernelGGL((k_applyA_xm<T, TE, NTV, MV>), grid, dim3(NTV), lds, c.s, in, out,                      \
                        (const TE*)Sel<T>::E(l), thr, l.g, (const uint8_t*)l.fixed, b, (const T*)Sel<T>::invD(l), \
-                       om, gate, had<T>(), CX, gx, gy, part)
+                       om, gate, had<T>(), CX, gx, gy, part, rdot)
 #define TOPO_XM_MODES(NTV)                                 \
     switch (mode) {                                        \
     case 0: TOPO_XM_LAUNCH(NTV, 0); break;                 \
     case 1: TOPO_XM_LAUNCH(NTV, 1); break;                 \
     case 2: TOPO_XM_LAUNCH(NTV, 2); break;                 \
     case 3: TOPO_XM_LAUNCH(NTV, 3); break;                 \
+    case 5: TOPO_XM_LAUNCH(NTV, 5); break;                 \
     default: TOPO_XM_LAUNCH(NTV, 4); break;                \
     }
                 if (NT == 512) {
@@ -1307,9 +1316,13 @@ struct MGH {
     }
 
     // one V-cycle (mgVcycle assembly/index.ts:837-1048) for A x = b from
-    // x = 0 on level li; returns the buffer holding x.
+    // x = 0 on level li; returns the buffer holding x. With rdot (level 0)
+    // the last post-smoothing sweep also leaves the partials of r.z, r =
+    // rdot and z the V-cycle's output, in rzpart and their count in *nrz (0
+    // when it could not).
     template <typename T>
-    T* vcycle(Ctx& c, int li, const T* b, const int* gate, int nr = 1) {
+    T* vcycle(Ctx& c, int li, const T* b, const int* gate, int nr = 1, const double* rdot = nullptr,
+              double* rzpart = nullptr, int* nrz = nullptr) {
         MGLevel& l = L[li];
         T* x = Sel<T>::x(l);
         T* x2 = Sel<T>::x2(l);
@@ -1397,7 +1410,10 @@ struct MGH {
         hipLaunchKernelGGL(k_prolong_add<T>, dim3(nblk(4 * lc.g.nnodes()), nr), dim3(256), 0, c.s, x, (const T*)xc, l.g,
                            lc.g, l.fixed, gate);
         for (int s = 0; s < nu2; s++) {
-            apply<T>(c, li, x, x2, 3, b, gate, nr);
+            const bool last = rdot && li == 0 && s == nu2 - 1;
+            const int np = apply<T>(c, li, x, x2, last ? 5 : 3, b, gate, nr, last ? rzpart : nullptr,
+                                    last ? rdot : nullptr);
+            if (last && nrz) *nrz = np;
             std::swap(x, x2);
         }
         return x;
@@ -1526,19 +1542,22 @@ static CGResult pcg_jacobi(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVe
 // r_lo_ready the FP32 residual has already been written (k_cg_update_ur).
 template <typename T, typename PT>
 static const PT* precondition_raw(Ctx& c, MGH& mg, const T* r, CGVecs& v32, const int* gate, int nr,
-                                  bool r_lo_ready) {
+                                  bool r_lo_ready, double* rzpart = nullptr, int* nrz = nullptr) {
     const int64_t n = mg.L[0].ndof;
+    // r.z partials from the V-cycle's last sweep need the FP64 residual
+    const double* rdot = nullptr;
+    if constexpr (std::is_same<T, double>::value) rdot = rzpart ? r : nullptr;
     if constexpr (std::is_same<T, PT>::value) {
         (void)n; (void)v32; (void)r_lo_ready;
         if constexpr (std::is_same<T, double>::value)
             if (mg.bddc) return mg.bddc_apply(c, r, gate);
         TOPO_CHECK(!mg.bddc, TOPO_E_UNSUPPORTED, "the domain-decomposition preconditioner is FP64 only");
-        return mg.vcycle<T>(c, 0, r, gate, nr);
+        return mg.vcycle<T>(c, 0, r, gate, nr, rdot, rzpart, nrz);
     } else {
         PT* r32 = (PT*)v32.r;
         if (!r_lo_ready)
             hipLaunchKernelGGL((k_convert_gated<PT, T>), dim3(nblk(n), nr), dim3(256), 0, c.s, r32, r, n, gate);
-        return mg.vcycle<PT>(c, 0, r32, gate, nr);
+        return mg.vcycle<PT>(c, 0, r32, gate, nr, rdot, rzpart, nrz);
     }
 }
 
@@ -1622,15 +1641,21 @@ static CGResult pcg_mg(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVecs& 
             // z is never formed: r.z and p = z + beta p read the V-cycle's
             // output, which is zero at the fixed DOFs (the mask of ebeMGPCG
             // :1170-1172 is a no-op on it)
-            const PT* zz = precondition_raw<T, PT>(c, mg, r, w32, gate, nr, true);
-            TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_cgp_rz<T, PT, NR, false>), dim3(kCgBlocks), dim3(kRedThreads), 0,
-                                                   c.s, (const T*)r, zz, (const uint8_t*)l.fixed, n, c.st,
-                                                   c.partials));
+            // the V-cycle's last sweep leaves the r.z partials when it can
+            int nb2 = 0;
+            const PT* zz = precondition_raw<T, PT>(c, mg, r, w32, gate, nr, true,
+                                                   mg.rz_fuse ? c.partials + cg_row(2, 0) : nullptr, &nb2);
+            if (nb2 == 0) {
+                TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_cgp_rz<T, PT, NR, false>), dim3(kCgBlocks), dim3(kRedThreads),
+                                                       0, c.s, (const T*)r, zz, (const uint8_t*)l.fixed, n, c.st,
+                                                       c.partials));
+                nb2 = kCgBlocks;
+            }
             TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_cgp_pupdate<T, PT, NR, false, TU>), dim3(kCgBlocks),
                                                    dim3(kRedThreads), 0, c.s, p, zz, (const uint8_t*)l.fixed, n,
-                                                   c.st, (const double*)c.partials, U, nb0));
+                                                   c.st, (const double*)c.partials, U, nb0, nb2));
             TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_cgp_finish<NR>), dim3(1), dim3(kRedThreads), 0, c.s, c.st,
-                                                   (const double*)c.partials, nb0));
+                                                   (const double*)c.partials, nb0, nb2));
         }
     };
     hipGraphExec_t ex = nullptr;
@@ -1832,6 +1857,7 @@ struct Solver {
         mg.exact = cfg.exact_apply != 0;
         if (const char* e = getenv("TOPO_XM_NT")) mg.xm_nt = atoi(e);
         if (const char* e = getenv("TOPO_XM_CX")) mg.xm_cx = atoi(e);
+        if (const char* e = getenv("TOPO_RZ_FUSE")) mg.rz_fuse = atoi(e) != 0;
         if (const char* e = getenv("TOPO_MIXED_ETA")) mixed_eta = atof(e);
         const int nlev = cfg.linear_solver == TOPO_SOLVER_MGPCG ? cfg.mg_levels : 1;
         mg.nrhs = batch_cap;
@@ -2050,7 +2076,7 @@ struct Solver {
             return (double)ms / reps;
         };
         tune_log.clear();
-        const int cx_c[10] = {0, 2, 3, 4, 5, 6, 8, 10, 12, 16};
+        const int cx_c[8] = {0, 2, 4, 6, 8, 10, 12, 16};   // even (the CG partials cover node-plane pairs)
         const int nl = (int)mg.L.size();
         mg.lv_nt.assign(nl, 0);
         mg.lv_cx.assign(nl, 0);
@@ -2067,7 +2093,7 @@ struct Solver {
                 double best = 1e300;
                 const int H = l.g.nnz() + 1;
                 const int nt_rule = 4 * H > 512 ? 1024 : 512;
-                for (int b = 0; b < 10; b++) {
+                for (int b = 0; b < 8; b++) {
                     mg.lv_nt[li] = 0;
                     mg.lv_cx[li] = cx_c[b];
                     const double ms = time_it([&]() {
