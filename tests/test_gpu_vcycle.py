@@ -70,3 +70,47 @@ def test_coarse_copies_keep_the_solution(gpu):
         assert info.residual <= 1e-6
         assert abs(c - c_ref) <= 1e-7 * abs(c_ref), (cf, c, c_ref)
     assert out[2] <= out[1] + 3, out
+
+
+@pytest.mark.parametrize("depth,leaf", [(1, 100000), (3, 200)])
+def test_nested_dissection_coarse_solve_matches_banded(gpu, monkeypatch, depth, leaf):
+    """The coarsest level factored in nested-dissection order (multifrontal
+    Cholesky over a separator tree, Z = L^-1 zero left of each row's
+    subtree, padded own ranges) solves the same system as the banded
+    factorization in the natural order: FP64 V-cycles agree to rounding and
+    stay symmetric and zero at the fixed DOFs, and FE solves with the FP16
+    coarse copy reach the banded solve's compliance."""
+    T = gpu
+    nel, ndof = T.dims(*DIMS)
+    x = P.splitmix64_uniform(nel)
+    rng = np.random.default_rng(5)
+    a = rng.normal(size=ndof)
+    b = rng.normal(size=ndof)
+    out = {}
+    for mode in ("banded", "nd"):
+        if mode == "banded":
+            monkeypatch.setenv("TOPO_ND_DEPTH", "0")
+        else:
+            monkeypatch.setenv("TOPO_ND_DEPTH", str(depth))
+            monkeypatch.setenv("TOPO_ND_MIN", "0")
+            monkeypatch.setenv("TOPO_ND_LEAF", str(leaf))
+        print("coarse factorization:", mode, depth, leaf, flush=True)
+        cfg, fixed, F, _, _, _ = _cfg(T, 3, 1, 2)
+        with T.Solver(cfg, fixed, F) as S:
+            S.set_density(x, 3.0)
+            Ma, Mb = S.vcycle(a, T.PREC_FP64), S.vcycle(b, T.PREC_FP64)
+        cfg.cg_tol = 1e-6
+        with T.Solver(cfg, fixed, F) as S:
+            _, c, info = S.fe(x, 3.0)
+        out[mode] = (Ma, Mb, c, info)
+    Ma0, Mb0, c0, i0 = out["banded"]
+    Ma, Mb, c, info = out["nd"]
+    assert np.abs(Ma - Ma0).max() <= 1e-10 * np.abs(Ma0).max(), np.abs(Ma - Ma0).max()
+    assert np.abs(Mb - Mb0).max() <= 1e-10 * np.abs(Mb0).max()
+    assert not Ma[fixed > 0].any()
+    free = fixed == 0
+    assert abs(Ma[free] @ b[free] - a[free] @ Mb[free]) <= 1e-12 * np.sqrt((Ma[free] @ a[free]) * (Mb[free] @ b[free]))
+    print("nd: c %.12e (banded %.12e), %d CG iterations (banded %d)" % (c, c0, info.cg_iters, i0.cg_iters))
+    assert info.residual <= 1e-6
+    assert abs(c - c0) <= 1e-7 * abs(c0)
+    assert abs(info.cg_iters - i0.cg_iters) <= 3

@@ -25,6 +25,20 @@ namespace topo {
 constexpr int kCB = 64;           // block size
 constexpr int kCS = kCB + 1;      // LDS row stride (bank spread)
 
+// One front of the nested-dissection factorization: a dense row-major matrix
+// whose rows and columns are the node's own DOFs (a multiple of 64) followed
+// by those of its ancestors. The chain kernels below, given an array of
+// these (blockIdx.y = front), factor the own blocks of several independent
+// fronts in the same launches: NB = own size / 64 block columns, window =
+// all of them (the own block is dense).
+struct FrontDev {
+    double* A;           // the front
+    int64_t ld;          // its size
+    double* Linv;        // NB inverses of the diagonal blocks
+    const double* d0;    // original diagonal of the own DOFs (pivot test)
+    int NB;
+};
+
 // load a 64 x 64 block (row-major, ld) into LDS with stride kCS by NT
 // threads; all of a thread's global reads are issued before the LDS writes
 template <int NT>
@@ -66,9 +80,17 @@ __device__ __forceinline__ double readlane_d(double v, int lane) {
 
 __global__ void __launch_bounds__(64) k_chol_diag(double* __restrict__ A, int64_t ld, int K,
                                                   double* __restrict__ Linv, const double* __restrict__ d0,
-                                                  double tiny, int* flag) {
+                                                  double tiny, int* flag, const FrontDev* fr = nullptr) {
     constexpr int SB = 16;           // sub-block
     constexpr int ILP = 4;           // partial sums of the inverse's dot products
+    if (fr) {
+        const FrontDev d = fr[blockIdx.y];
+        if (K >= d.NB) return;
+        A = d.A;
+        ld = d.ld;
+        Linv = d.Linv;
+        d0 = d.d0;
+    }
     __shared__ double a[kCB * kCS];
     __shared__ double rdiag[kCB];
     double* Ab = A + (int64_t)K * kCB * ld + (int64_t)K * kCB;
@@ -168,7 +190,15 @@ __device__ __forceinline__ void blk_mul_nt(const double* X, const double* Y, dou
 
 // A_IK <- A_IK L_KK^-T for the wb blocks below the diagonal block K
 __global__ void __launch_bounds__(256) k_chol_panel(double* __restrict__ A, int64_t ld, int K, int NB, int wb,
-                                                    const double* __restrict__ Linv) {
+                                                    const double* __restrict__ Linv, const FrontDev* fr = nullptr) {
+    if (fr) {
+        const FrontDev d = fr[blockIdx.y];
+        A = d.A;
+        ld = d.ld;
+        Linv = d.Linv;
+        NB = d.NB;
+        wb = d.NB;
+    }
     const int I = K + 1 + (int)blockIdx.x;
     if (I >= NB || I > K + wb) return;
     __shared__ double x[kCB * kCS];
@@ -190,7 +220,13 @@ __global__ void __launch_bounds__(256) k_chol_panel(double* __restrict__ A, int6
 // which is all the next diagonal and panel steps need; part 2: the other
 // pairs (J >= K + 2), left to run beside those steps.
 __global__ void __launch_bounds__(256) k_chol_update(double* __restrict__ A, int64_t ld, int K, int NB, int wb,
-                                                     int part = 0) {
+                                                     int part = 0, const FrontDev* fr = nullptr) {
+    if (fr) {
+        const FrontDev d = fr[blockIdx.y];
+        A = d.A;
+        ld = d.ld;
+        NB = d.NB;
+    }
     int I, J;
     if (part == 1) {
         I = K + 1 + (int)blockIdx.x;
@@ -224,6 +260,24 @@ __global__ void k_copy_block(const double* __restrict__ src, int64_t lds, double
     }
 }
 
+// Front of a nested-dissection node (f x f, row-major): F[a][b] =
+// A[g[a]][g[b]] where a or b is one of the node's own m DOFs, 0 between two
+// ancestor DOFs, plus the update blocks of its (at most two) children, whose
+// rows and columns are exactly this front's, added in child order. A: the
+// coarse free block in the nested-dissection order (ld n); g: front index ->
+// position; C0 / C1: a child's update block (ld0 / ld1) or null.
+__global__ void k_front_assemble(double* __restrict__ F, int64_t f, int64_t m, const int32_t* __restrict__ g,
+                                 const double* __restrict__ A, int64_t n, const double* __restrict__ C0, int64_t ld0,
+                                 const double* __restrict__ C1, int64_t ld1) {
+    for (int64_t w = gtid(); w < f * f; w += gstride()) {
+        const int64_t a = w / f, b = w - a * f;
+        double v = (a < m || b < m) ? A[(int64_t)g[a] * n + g[b]] : 0.0;
+        if (C0) v += C0[a * ld0 + b];
+        if (C1) v += C1[a * ld1 + b];
+        F[w] = v;
+    }
+}
+
 // out = in^T for an n x n row-major matrix (64 x 64 tiles through LDS)
 template <typename TI, typename TO>
 __global__ void __launch_bounds__(256) k_transpose(const TI* __restrict__ in, TO* __restrict__ out, int64_t n) {
@@ -246,18 +300,21 @@ __global__ void __launch_bounds__(256) k_transpose(const TI* __restrict__ in, TO
 // operator applied stays symmetric positive definite whatever the rounding.
 // One wavefront per row.
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+// (rfirst, optional: row i is zero before column rfirst[i], not read there.)
 __global__ void __launch_bounds__(256) k_rowscale_f16(const double* __restrict__ Z, int64_t n,
-                                                      _Float16* __restrict__ Zq, double* __restrict__ s2) {
+                                                      _Float16* __restrict__ Zq, double* __restrict__ s2,
+                                                      const int32_t* __restrict__ rfirst = nullptr) {
     const int lane = threadIdx.x & 63;
     const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     if (i >= n) return;
     const double* row = Z + i * n;
+    const int64_t rf = rfirst ? (int64_t)rfirst[i] : 0;
     double m = 0.0;
-    for (int64_t j = lane; j <= i; j += 64) m = fmax(m, fabs(row[j]));
+    for (int64_t j = rf + lane; j <= i; j += 64) m = fmax(m, fabs(row[j]));
     for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
     const double sc = m > 0.0 ? m : 1.0;
     const double inv = 1.0 / sc;
-    for (int64_t j = lane; j < n; j += 64) Zq[i * n + j] = (_Float16)(float)(j <= i ? row[j] * inv : 0.0);
+    for (int64_t j = lane; j < n; j += 64) Zq[i * n + j] = (_Float16)(float)(j >= rf && j <= i ? row[j] * inv : 0.0);
     if (lane == 0) s2[i] = sc * sc;
 }
 
@@ -344,7 +401,7 @@ __global__ void __launch_bounds__(256) k_tri_apply(const TK* __restrict__ M, int
             for (int r = 0; r < 4; r++) {
                 const int64_t i = row0 + r;
                 if (!x) y[q * n + i] = (TY)(ys ? ys[i] * acc[r][q] : acc[r][q]);
-                else if (i < nf) x[q * ndof + freeIdx[i]] = (T)acc[r][q];
+                else if (i < nf && freeIdx[i] >= 0) x[q * ndof + freeIdx[i]] = (T)acc[r][q];
             }
         }
 }
@@ -355,10 +412,13 @@ __global__ void __launch_bounds__(256) k_tri_apply(const TK* __restrict__ M, int
 // part[(q nch + ch) n + i] (0 where the chunk misses the range). No wave
 // walks more than cw columns, so the kernel is not held by the latency chain
 // of the longest rows; k_tri_sum adds the chunks in ascending order.
+// rfirst (lower only, optional): first column of each row's nonzeros (the
+// same for the 4 RW rows of a wave, a multiple of 16); the columns before
+// it are zero and not read.
 template <int NR, typename TK, typename TB, int RW = 4>
 __global__ void __launch_bounds__(256) k_tri_part(const TK* __restrict__ M, int64_t n, int upper,
                                                   const TB* __restrict__ bf, double* __restrict__ part, int cw, int nch,
-                                                  const int* gate) {
+                                                  const int* gate, const int32_t* __restrict__ rfirst = nullptr) {
     bool live[NR];
     bool any = false;
 #pragma unroll
@@ -375,7 +435,8 @@ __global__ void __launch_bounds__(256) k_tri_part(const TK* __restrict__ M, int6
     const int64_t c0 = (int64_t)ch * cw;
     const int64_t c1 = c0 + cw < n ? c0 + cw : n;
     // triangular range in whole VN-vectors (the other triangle of M is 0)
-    const int64_t lo = upper ? (c0 > row0 / VN * VN ? c0 : row0 / VN * VN) : c0;
+    const int64_t rf = (!upper && rfirst) ? (int64_t)rfirst[row0] : 0;
+    const int64_t lo = upper ? (c0 > row0 / VN * VN ? c0 : row0 / VN * VN) : (c0 > rf ? c0 : rf);
     const int64_t hv = (row0 + RW + VN - 1) / VN * VN;
     const int64_t hi = upper ? c1 : (c1 < hv ? c1 : hv);
     double acc[RW][NR];
@@ -442,13 +503,16 @@ __global__ void __launch_bounds__(256) k_tri_part(const TK* __restrict__ M, int6
 // (y_i read once per wave as a uniform value); the 4 waves' FP32 column sums
 // meet in LDS and leave one FP64 partial per column and row block,
 // part[(q nrb + rb) n + j] (0 for blocks above the diagonal), summed by
-// k_tri_sum.
+// k_tri_sum. rfirst (optional): first nonzero column of each row, the same
+// for the 64 rows of a wave; a wave whose rows are zero over the block's
+// columns does not read them.
 constexpr int kTcRows = 256;      // rows per block
 constexpr int kTcCols = 512;      // columns per block (64 lanes x 8)
 template <int NR, typename TB>
 __global__ void __launch_bounds__(256) k_tri_colpart(const _Float16* __restrict__ M, int64_t n,
                                                      const TB* __restrict__ y, double* __restrict__ part, int nrb,
-                                                     const int* gate, int upper = 0) {
+                                                     const int* gate, int upper = 0,
+                                                     const int32_t* __restrict__ rfirst = nullptr) {
     __shared__ float sh[4][NR][kTcCols];
     const int cb = blockIdx.x, rb = blockIdx.y;
     const int64_t J0 = (int64_t)cb * kTcCols, I0 = (int64_t)rb * kTcRows;
@@ -473,7 +537,8 @@ __global__ void __launch_bounds__(256) k_tri_colpart(const _Float16* __restrict_
         for (int u = 0; u < 8; u++) acc[q][u] = 0.0f;
     const int64_t r0 = I0 + (int64_t)w * (kTcRows / 4);
     const int64_t r1 = r0 + kTcRows / 4 < n ? r0 + kTcRows / 4 : n;
-    if (colok) {
+    const bool wz = rfirst && r0 < n && (int64_t)rfirst[r0] > J0 + kTcCols - 1;   // wave-uniform
+    if (colok && !wz) {
 #pragma unroll 8
         for (int64_t i = r0; i < r1; i++) {
             const half8_t a = *(const half8_t*)(M + i * n + j);
@@ -523,7 +588,7 @@ __global__ void k_tri_sum(const double* __restrict__ part, int nch, int64_t n, T
         s += __shfl_xor(s, 2, 64);
         if (ok && sub == 0) {
             if (!x) y[q * n + i] = (TY)(ys ? ys[i] * s : s);
-            else if (i < nf) x[q * ndof + freeIdx[i]] = (T)s;
+            else if (i < nf && freeIdx[i] >= 0) x[q * ndof + freeIdx[i]] = (T)s;
         }
     }
 }

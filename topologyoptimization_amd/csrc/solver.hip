@@ -12,6 +12,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <type_traits>
 #include <vector>
@@ -330,7 +331,107 @@ struct MGLevel {
     double* d0c = nullptr;         // original diagonal (pivot test)
     double* ztmp = nullptr;        // 64 x nfp scratch of the Z row blocks
     double* work = nullptr;
+    // nested-dissection order (nd): a separator tree in postorder over the
+    // free DOFs, one dense front per node; Z = L^-1 is zero left of the
+    // start of each row's subtree (rfirst)
+    struct NDNode {
+        int64_t o0 = 0, m = 0;    // own positions [o0, o0 + m), m a multiple of 64
+        int64_t s0 = 0, f = 0;    // first position of the subtree; front size (own + ancestors)
+        int parent = -1, kid[2] = {-1, -1}, height = 0;
+        double* F = nullptr;      // front (f x f, row-major)
+        double* Lb = nullptr;     // L of the ancestor rows ((f - m) x m)
+        int32_t* g = nullptr;     // front index -> position
+        double* Linv = nullptr;   // inverses of the own diagonal blocks
+    };
+    bool nd = false;
+    std::vector<NDNode> ndn;
+    std::vector<std::vector<int>> nd_h;   // nodes by height (leaves 0)
+    std::vector<FrontDev*> nd_fd;         // their front descriptors (device)
+    int32_t* rfirst = nullptr;            // per position: first nonzero column of its row of Z
+    double* ndW = nullptr;                // scratch of the Z blocks (max (f - m) x m)
 };
+
+// row-major C (r x c) = alpha op(A) op(B) + beta C as the column-major
+// C^T = alpha op(B)^T op(A)^T of rocBLAS
+static void gemm_rm(rocblas_handle h, bool ta, bool tb, int64_t r, int64_t c, int64_t k, double alpha,
+                    const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc) {
+    TOPO_CHECK(rocblas_dgemm(h, tb ? rocblas_operation_transpose : rocblas_operation_none,
+                             ta ? rocblas_operation_transpose : rocblas_operation_none, (rocblas_int)c, (rocblas_int)r,
+                             (rocblas_int)k, &alpha, B, (rocblas_int)ldb, A, (rocblas_int)lda, &beta, C,
+                             (rocblas_int)ldc) == rocblas_status_success,
+               TOPO_E_HIP, "rocblas_dgemm failed");
+}
+
+// Nested-dissection order of the free DOFs of a structured node grid: the
+// node box is cut through its middle node plane across its longest axis,
+// the plane's DOFs (the separator) are numbered after both halves, and the
+// halves are cut the same way, to `depth` levels or until a part has at
+// most `leafmax` free DOFs or fewer than 3 node planes on every axis. A part
+// keeps the natural DOF order. Nodes come out in postorder (children before
+// their parent), each own range padded to a multiple of 64 with positions
+// that map to no DOF (-1). Returns the position -> DOF map.
+static std::vector<int32_t> nd_order(const Grid& g, const std::vector<uint8_t>& fixed, int depth, int64_t leafmax,
+                                     std::vector<MGLevel::NDNode>& nodes) {
+    const int nnx = g.nnx(), nny = g.nny(), nnz = g.nnz();
+    struct Box { int lo[3], hi[3]; };
+    auto dofs = [&](const Box& b) {
+        std::vector<int32_t> v;
+        for (int ix = b.lo[0]; ix < b.hi[0]; ix++)
+            for (int iy = b.lo[1]; iy < b.hi[1]; iy++)
+                for (int iz = b.lo[2]; iz < b.hi[2]; iz++)
+                    for (int d = 0; d < 3; d++) {
+                        const int64_t dof = 3 * (((int64_t)ix * nny + iy) * nnz + iz) + d;
+                        if (!fixed[dof]) v.push_back((int32_t)dof);
+                    }
+        return v;
+    };
+    std::vector<std::vector<int32_t>> own;
+    nodes.clear();
+    std::function<int(const Box&, int)> rec = [&](const Box& b, int d) -> int {
+        int a = 0;
+        for (int q = 1; q < 3; q++)
+            if (b.hi[q] - b.lo[q] > b.hi[a] - b.lo[a]) a = q;
+        std::vector<int32_t> all = dofs(b);
+        if (d == 0 || (int64_t)all.size() <= leafmax || b.hi[a] - b.lo[a] < 3) {
+            nodes.emplace_back();
+            own.push_back(std::move(all));
+            return (int)nodes.size() - 1;
+        }
+        const int mid = (b.lo[a] + b.hi[a]) / 2;
+        Box b0 = b, b1 = b, bs = b;
+        b0.hi[a] = mid;
+        b1.lo[a] = mid + 1;
+        bs.lo[a] = mid;
+        bs.hi[a] = mid + 1;
+        const int k0 = rec(b0, d - 1);
+        const int k1 = rec(b1, d - 1);
+        nodes.emplace_back();
+        own.push_back(dofs(bs));
+        const int me = (int)nodes.size() - 1;
+        nodes[me].kid[0] = k0;
+        nodes[me].kid[1] = k1;
+        nodes[k0].parent = me;
+        nodes[k1].parent = me;
+        nodes[me].height = 1 + std::max(nodes[k0].height, nodes[k1].height);
+        return me;
+    };
+    rec(Box{{0, 0, 0}, {nnx, nny, nnz}}, depth);
+    std::vector<int32_t> fr;
+    for (size_t i = 0; i < nodes.size(); i++) {
+        MGLevel::NDNode& t = nodes[i];
+        const int64_t m = (int64_t)own[i].size();
+        t.o0 = (int64_t)fr.size();
+        t.m = std::max<int64_t>(kCB, (m + kCB - 1) / kCB * kCB);
+        fr.insert(fr.end(), own[i].begin(), own[i].end());
+        fr.insert(fr.end(), (size_t)(t.m - m), -1);
+        t.s0 = t.kid[0] >= 0 ? nodes[t.kid[0]].s0 : t.o0;
+    }
+    for (MGLevel::NDNode& t : nodes) {
+        t.f = t.m;
+        for (int p = t.parent; p >= 0; p = nodes[p].parent) t.f += nodes[p].m;
+    }
+    return fr;
+}
 
 // One triangular product of the coarse solve (k_tri_apply semantics): by
 // column chunks (k_tri_part + k_tri_sum) when the level has a chunk width,
@@ -342,7 +443,8 @@ static void tri_product(Ctx& c, const MGLevel& l, const TK* M, int upper, const 
         constexpr int RW = 4;
         const dim3 gp((unsigned)l.tnch, (unsigned)(l.nfp / (4 * RW)));
         TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_part<NR, TK, TB, RW>), gp, dim3(256), 0, c.s, M, l.nfp, upper,
-                                              bf, l.tpart, l.tcw, l.tnch, gate));
+                                              bf, l.tpart, l.tcw, l.tnch, gate,
+                                              upper ? (const int32_t*)nullptr : (const int32_t*)l.rfirst));
         TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_sum<T, NR, TY>), dim3(nblk(4 * l.nfp)), dim3(256), 0, c.s,
                                               (const double*)l.tpart, l.tnch, l.nfp, y, x, (const int32_t*)l.freeIdx,
                                               l.nf, l.ndof, gate, ys));
@@ -399,7 +501,7 @@ __global__ void k_gather_free(const T* __restrict__ b, const int32_t* __restrict
     if (gate && *gate) return;
     b += batch_off(ndof);
     bf += batch_off(nfp);
-    for (int64_t j = gtid(); j < nfp; j += gstride()) bf[j] = j < nf ? (TB)b[freeIdx[j]] : TB(0);
+    for (int64_t j = gtid(); j < nfp; j += gstride()) bf[j] = (j < nf && freeIdx[j] >= 0) ? (TB)b[freeIdx[j]] : TB(0);
 }
 // one wave per 4 consecutive rows; each lane streams 16-byte vectors of the
 // four rows against the same slices of the NR right-hand sides bf (read from
@@ -476,7 +578,8 @@ __global__ void __launch_bounds__(256) k_inv_apply(const TK* __restrict__ Kinv, 
     if (lane == 0)
         for (int q = 0; q < NR; q++)
             if (live[q])
-                for (int r = 0; r < nr; r++) x[q * ndof + freeIdx[row0 + r]] = (T)acc[r][q];
+                for (int r = 0; r < nr; r++)
+                    if (freeIdx[row0 + r] >= 0) x[q * ndof + freeIdx[row0 + r]] = (T)acc[r][q];
 }
 template <typename T>
 __global__ void k_zero_fixed_or_all(T* __restrict__ x, int64_t n, const int* gate) {
@@ -668,7 +771,7 @@ __global__ void k_extract_free(const double* __restrict__ K, int64_t n, const in
                                int64_t ld, double* __restrict__ A) {
     for (int64_t w = gtid(); w < nf * nf; w += gstride()) {
         const int64_t i = w / nf, j = w % nf;
-        A[i * ld + j] = K[(int64_t)freeIdx[i] * n + freeIdx[j]];
+        A[i * ld + j] = (freeIdx[i] >= 0 && freeIdx[j] >= 0) ? K[(int64_t)freeIdx[i] * n + freeIdx[j]] : 0.0;
     }
 }
 // Free-DOF block of the assembled operator sum_e E_e P_e^T K0 P_e, written
@@ -685,6 +788,7 @@ __global__ void k_assemble_free(const double* __restrict__ E, double thr, Grid g
         const int q = (int)(w % 81), nb = q / 3, d2 = q % 3;
         const int dx = nb % 3 - 1, dy = (nb / 3) % 3 - 1, dz = nb / 9 - 1;
         const int64_t a = freeIdx[fa];
+        if (a < 0) continue;   // padding position of the nested-dissection order
         const int64_t na = a / 3;
         const int d1 = (int)(a % 3);
         const int iz = (int)(na % nnz);
@@ -742,6 +846,10 @@ struct MGH {
     // FP32 V-cycles apply a rounded copy of the coarsest level's factor
     // inverse: 0 none (FP64), 1 FP32, 2 FP16 with one scale per row
     int coarse_fp32 = 0;
+    // nested-dissection coarse factorization: tree depth (0: banded, natural
+    // order), smallest block it is used for, largest part left uncut
+    int nd_depth = 3;
+    int64_t nd_min = 2048, nd_leaf = 768;
     // 'petsc' solver: the domain-decomposition preconditioner replaces the
     // V-cycle (one level; omega 1 for its global Jacobi sweeps)
     bool bddc = false;
@@ -979,6 +1087,21 @@ struct MGH {
                         }
             }
             l.bandw = w;
+            // nested-dissection order for larger blocks (positions include
+            // the padding of each tree node's own range)
+            if (nd_depth > 0 && l.nf >= nd_min) {
+                std::vector<int32_t> frn = nd_order(l.g, hf, nd_depth, nd_leaf, l.ndn);
+                if (l.ndn.size() > 1) {
+                    fr.swap(frn);
+                    std::fill(of.begin(), of.end(), -1);
+                    for (size_t p = 0; p < fr.size(); p++)
+                        if (fr[p] >= 0) of[fr[p]] = (int32_t)p;
+                    l.nf = (int64_t)fr.size();
+                    l.nd = true;
+                } else {
+                    l.ndn.clear();
+                }
+            }
             l.nfp = (l.nf + kCB - 1) / kCB * kCB;
             l.freeIdx = c.alloc<int32_t>(l.nf);
             TOPO_HIP(hipMemcpy(l.freeIdx, fr.data(), l.nf * sizeof(int32_t), hipMemcpyHostToDevice));
@@ -997,14 +1120,42 @@ struct MGH {
             }
             l.d0c = c.alloc<double>(l.nfp);
             if (want32 && coarse_fp32) l.Kinv32 = c.alloc<float>(l.nfp * l.nfp);
+            if (l.nd) nd_alloc(c, l);
         }
         const int64_t n = l.nfp;
         const int NB = (int)(n / kCB);
         const int wb = std::max(1, (l.bandw + kCB - 1) / kCB);
         coarse_block(c, l);
-        // banded Cholesky when the band is narrow enough to pay off
+        // nested-dissection or banded Cholesky when the band is narrow
+        // enough to pay off
         bool done = false;
-        if (NB > 2 * wb) {
+        if (l.nd) {
+            if (!l.Z) {
+                l.Z = c.alloc<double>(n * n);
+                // written only inside the factor's structure: the rest stays 0
+                hipLaunchKernelGGL(k_fill<double>, dim3(nblk(n * n)), dim3(256), 0, c.s, l.Z, 0.0, n * n,
+                                   (const int*)nullptr);
+                if (want32 && coarse_fp32 == 1) l.Z32 = c.alloc<float>(n * n);
+                if (want32 && coarse_fp32 == 2) {
+                    l.Z16 = c.alloc<_Float16>(n * n);
+                    l.zs2 = c.alloc<double>(n);
+                }
+            }
+            int* dflag = (int*)(c.dsc + 14);   // device scratch word
+            hipLaunchKernelGGL(k_diag_copy, dim3(nblk(n)), dim3(256), 0, c.s, (const double*)l.Kinv, n, l.d0c);
+            hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, c.s, dflag, 0, (int64_t)1, (const int*)nullptr);
+            nd_factor(c, l, dflag);
+            c.check_launch();
+            TOPO_HIP(hipMemcpyAsync(&c.h_sc[0], dflag, sizeof(int), hipMemcpyDeviceToHost, c.s));
+            c.sync();
+            int flag = 0;
+            std::memcpy(&flag, &c.h_sc[0], sizeof(int));
+            if (!flag) {
+                z_copies(c, l);
+                done = true;
+                l.tri = true;
+            }
+        } else if (NB > 2 * wb) {
             if (!l.Z) {
                 l.Z = c.alloc<double>(n * n);
                 l.Linv = c.alloc<double>((int64_t)NB * kCB * kCB);
@@ -1062,19 +1213,7 @@ struct MGH {
             int flag = 0;
             std::memcpy(&flag, &c.h_sc[0], sizeof(int));
             if (!flag) {
-                // Z^T into the Kinv buffer
-                hipLaunchKernelGGL((k_transpose<double, double>), dim3(NB, NB), dim3(256), 0, c.s,
-                                   (const double*)l.Z, l.Kinv, n);
-                if (l.Z16) {
-                    hipLaunchKernelGGL(k_rowscale_f16, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, c.s,
-                                       (const double*)l.Z, n, l.Z16, l.zs2);
-                }
-                if (l.Z32) {
-                    hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(n * n)), dim3(256), 0, c.s, l.Z32,
-                                       (const double*)l.Z, n * n);
-                    hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(n * n)), dim3(256), 0, c.s, l.Kinv32,
-                                       (const double*)l.Kinv, n * n);
-                }
+                z_copies(c, l);
                 done = true;
                 l.tri = true;
             } else {
@@ -1093,6 +1232,144 @@ struct MGH {
         c.check_launch();
         l.chol = true;
     }
+    // what the coarse solves read besides Z: Z^T in the Kinv buffer (FP64
+    // products), the FP16 copy with its row scales or the FP32 copies
+    void z_copies(Ctx& c, MGLevel& l) {
+        const int64_t n = l.nfp;
+        const int NB = (int)(n / kCB);
+        hipLaunchKernelGGL((k_transpose<double, double>), dim3(NB, NB), dim3(256), 0, c.s, (const double*)l.Z, l.Kinv,
+                           n);
+        if (l.Z16)
+            hipLaunchKernelGGL(k_rowscale_f16, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, c.s, (const double*)l.Z, n,
+                               l.Z16, l.zs2, (const int32_t*)l.rfirst);
+        if (l.Z32) {
+            hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(n * n)), dim3(256), 0, c.s, l.Z32,
+                               (const double*)l.Z, n * n);
+            hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(n * n)), dim3(256), 0, c.s, l.Kinv32,
+                               (const double*)l.Kinv, n * n);
+        }
+    }
+
+    // fronts, gather maps, descriptors and row starts of the
+    // nested-dissection factorization (once per hierarchy)
+    void nd_alloc(Ctx& c, MGLevel& l) {
+        const int64_t n = l.nfp;
+        int H = 0;
+        for (const MGLevel::NDNode& t : l.ndn) H = std::max(H, t.height + 1);
+        l.nd_h.assign(H, {});
+        int64_t wmax = 1;
+        std::vector<int32_t> rf(n, 0);
+        for (size_t i = 0; i < l.ndn.size(); i++) {
+            MGLevel::NDNode& t = l.ndn[i];
+            l.nd_h[t.height].push_back((int)i);
+            t.F = c.alloc<double>(t.f * t.f);
+            const int64_t u = t.f - t.m;
+            if (u > 0) t.Lb = c.alloc<double>(u * t.m);
+            wmax = std::max(wmax, u * t.m);
+            std::vector<int32_t> gi;
+            for (int64_t k = 0; k < t.m; k++) gi.push_back((int32_t)(t.o0 + k));
+            for (int p = t.parent; p >= 0; p = l.ndn[p].parent)
+                for (int64_t k = 0; k < l.ndn[p].m; k++) gi.push_back((int32_t)(l.ndn[p].o0 + k));
+            TOPO_CHECK((int64_t)gi.size() == t.f, TOPO_E_ARG, "front map size");
+            t.g = c.alloc<int32_t>(t.f);
+            TOPO_HIP(hipMemcpy(t.g, gi.data(), t.f * sizeof(int32_t), hipMemcpyHostToDevice));
+            t.Linv = c.alloc<double>(t.m * kCB);
+            for (int64_t k = 0; k < t.m; k++) rf[t.o0 + k] = (int32_t)t.s0;
+        }
+        l.ndW = c.alloc<double>(wmax);
+        l.rfirst = c.alloc<int32_t>(n);
+        TOPO_HIP(hipMemcpy(l.rfirst, rf.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+        l.nd_fd.assign(H, nullptr);
+        for (int h = 0; h < H; h++) {
+            std::vector<FrontDev> fd;
+            for (int id : l.nd_h[h]) {
+                const MGLevel::NDNode& t = l.ndn[id];
+                fd.push_back(FrontDev{t.F, t.f, t.Linv, l.d0c + t.o0, (int)(t.m / kCB)});
+            }
+            l.nd_fd[h] = (FrontDev*)c.alloc<char>(fd.size() * sizeof(FrontDev));
+            TOPO_HIP(hipMemcpy(l.nd_fd[h], fd.data(), fd.size() * sizeof(FrontDev), hipMemcpyHostToDevice));
+        }
+    }
+
+    // Multifrontal Cholesky of the block A (l.Kinv, nested-dissection order)
+    // and Z = L^-1. Bottom-up by height, independent nodes together: the
+    // front F = [own, ancestors] is A's rows/columns of the own DOFs plus the
+    // children's update blocks; its own block is factored by the chain
+    // kernels (L_DD), Z_DD = L_DD^-1 (rocBLAS trtri, into Z), the ancestor
+    // rows L_aD = F_aD Z_DD^T and the update block F_aa -= L_aD L_aD^T (for
+    // the parent's front). Then top-down (ancestors first): Z's blocks below
+    // the diagonal, Z_PD = -sum_{Q on the path from P down to the parent of
+    // D} Z_PQ (L_QD Z_DD), the rows of L on D's ancestor path being L_aD.
+    // Every other block of Z is zero (Z_ij != 0 only for i an ancestor of
+    // j), so each row of Z starts at its node's subtree.
+    void nd_factor(Ctx& c, MGLevel& l, int* dflag) {
+        const int64_t n = l.nfp;
+        rocblas_handle h = c.blas_handle();
+        for (size_t ht = 0; ht < l.nd_h.size(); ht++) {
+            int maxNB = 0;
+            for (int id : l.nd_h[ht]) {
+                MGLevel::NDNode& t = l.ndn[id];
+                const double* Cp[2] = {nullptr, nullptr};
+                int64_t Cl[2] = {0, 0};
+                for (int k = 0; k < 2; k++)
+                    if (t.kid[k] >= 0) {
+                        const MGLevel::NDNode& ch = l.ndn[t.kid[k]];
+                        Cp[k] = ch.F + ch.m * ch.f + ch.m;
+                        Cl[k] = ch.f;
+                    }
+                hipLaunchKernelGGL(k_front_assemble, dim3(nblk(t.f * t.f)), dim3(256), 0, c.s, t.F, t.f, t.m,
+                                   (const int32_t*)t.g, (const double*)l.Kinv, n, Cp[0], Cl[0], Cp[1], Cl[1]);
+                maxNB = std::max(maxNB, (int)(t.m / kCB));
+            }
+            const FrontDev* fd = l.nd_fd[ht];
+            const unsigned nfr = (unsigned)l.nd_h[ht].size();
+            for (int K = 0; K < maxNB; K++) {
+                hipLaunchKernelGGL(k_chol_diag, dim3(1, nfr), dim3(64), 0, c.s, (double*)nullptr, (int64_t)0, K,
+                                   (double*)nullptr, (const double*)nullptr, 1e-11, dflag, fd);
+                const int nwin = maxNB - 1 - K;
+                if (nwin > 0) {
+                    hipLaunchKernelGGL(k_chol_panel, dim3(nwin, nfr), dim3(256), 0, c.s, (double*)nullptr,
+                                       (int64_t)0, K, 0, 0, (const double*)nullptr, fd);
+                    hipLaunchKernelGGL(k_chol_update, dim3(nwin * (nwin + 1) / 2, nfr), dim3(256), 0, c.s,
+                                       (double*)nullptr, (int64_t)0, K, 0, 0, 0, fd);
+                }
+            }
+            for (int id : l.nd_h[ht]) {
+                MGLevel::NDNode& t = l.ndn[id];
+                double* Zd = l.Z + t.o0 * n + t.o0;
+                // row-major lower L_DD = column-major upper
+                TOPO_CHECK(rocblas_dtrtri(h, rocblas_fill_upper, rocblas_diagonal_non_unit, (rocblas_int)t.m, t.F,
+                                          (rocblas_int)t.f, Zd, (rocblas_int)n) == rocblas_status_success,
+                           TOPO_E_HIP, "rocblas_dtrtri failed");
+                const int64_t u = t.f - t.m;
+                if (u > 0) {
+                    gemm_rm(h, false, true, u, t.m, t.m, 1.0, t.F + t.m * t.f, t.f, Zd, n, 0.0, t.Lb, t.m);
+                    gemm_rm(h, false, true, u, u, t.m, -1.0, t.Lb, t.m, t.Lb, t.m, 1.0, t.F + t.m * t.f + t.m, t.f);
+                }
+            }
+        }
+        for (int id = (int)l.ndn.size() - 1; id >= 0; id--) {
+            const MGLevel::NDNode& t = l.ndn[id];
+            const int64_t u = t.f - t.m;
+            if (u == 0) continue;
+            const double* Zd = l.Z + t.o0 * n + t.o0;
+            // W = L_aD Z_DD (rows in the order of the ancestor path)
+            gemm_rm(h, false, false, u, t.m, t.m, 1.0, t.Lb, t.m, Zd, n, 0.0, l.ndW, t.m);
+            std::vector<int> path;
+            for (int p = t.parent; p >= 0; p = l.ndn[p].parent) path.push_back(p);
+            for (size_t i = 0; i < path.size(); i++) {
+                const MGLevel::NDNode& P = l.ndn[path[i]];
+                int64_t off = 0;
+                for (size_t j = 0; j <= i; j++) {
+                    const MGLevel::NDNode& Q = l.ndn[path[j]];
+                    gemm_rm(h, false, false, P.m, t.m, Q.m, -1.0, l.Z + P.o0 * n + Q.o0, n, l.ndW + off * t.m, t.m,
+                            j ? 1.0 : 0.0, l.Z + P.o0 * n + t.o0, n);
+                    off += Q.m;
+                }
+            }
+        }
+    }
+
     // Row block I of Z = L^-1: Z_II = L_II^-1 and, for the columns left of
     // the diagonal block, Z_I,0:I = -L_II^-1 (L_I,band Z_band,0:I), band = the
     // wb row blocks above I. Both products are plain rocBLAS DGEMMs on
@@ -1353,7 +1630,7 @@ struct MGH {
                     const dim3 gc((unsigned)((l.nfp + kTcCols - 1) / kTcCols), (unsigned)l.tnrb);
                     TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_colpart<NR, float>), gc, dim3(256), 0, c.s,
                                                           (const _Float16*)l.Z16, l.nfp, (const float*)l.ytmp, l.tpart,
-                                                          l.tnrb, gate, 0));
+                                                          l.tnrb, gate, 0, (const int32_t*)l.rfirst));
                     TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_sum<T, NR, float>), dim3(nblk(4 * l.nfp)), dim3(256),
                                                           0, c.s, (const double*)l.tpart, l.tnrb, l.nfp,
                                                           (float*)nullptr, x, (const int32_t*)l.freeIdx, l.nf, l.ndof,
@@ -1858,6 +2135,9 @@ struct Solver {
         if (const char* e = getenv("TOPO_XM_NT")) mg.xm_nt = atoi(e);
         if (const char* e = getenv("TOPO_XM_CX")) mg.xm_cx = atoi(e);
         if (const char* e = getenv("TOPO_RZ_FUSE")) mg.rz_fuse = atoi(e) != 0;
+        if (const char* e = getenv("TOPO_ND_DEPTH")) mg.nd_depth = atoi(e);
+        if (const char* e = getenv("TOPO_ND_MIN")) mg.nd_min = atoll(e);
+        if (const char* e = getenv("TOPO_ND_LEAF")) mg.nd_leaf = atoll(e);
         if (const char* e = getenv("TOPO_MIXED_ETA")) mixed_eta = atof(e);
         const int nlev = cfg.linear_solver == TOPO_SOLVER_MGPCG ? cfg.mg_levels : 1;
         mg.nrhs = batch_cap;
