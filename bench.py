@@ -48,7 +48,7 @@ def make_problem(T, P, workload, grid, prec, levels, **kw):
     return cfg, fixed, F, pres, void, x0
 
 
-def run_solver(T, P, grid, prec, steps, warmup, levels, plain=False, workload="c3", coarse_fp32=2, **kw):
+def run_solver(T, P, grid, prec, steps, warmup, levels, plain=False, workload="c3", coarse_fp32=None, **kw):
     """plain: the reference's plain method on the same problem -- Jacobi-PCG
     in FP64, with the 400-iteration cap lifted so it reaches the same
     tolerance (the compliance is then comparable)."""
@@ -56,7 +56,9 @@ def run_solver(T, P, grid, prec, steps, warmup, levels, plain=False, workload="c
     if workload == "c3":
         cfg.mg_omega_cap = 1.6
     cfg.max_iterations = 1000
-    cfg.coarse_fp32 = coarse_fp32
+    # FP16 copy of the coarse factor inverse, FP32 for the inverter (whose
+    # ill-conditioned coarse operator the FP16 copy does not represent well)
+    cfg.coarse_fp32 = coarse_fp32 if coarse_fp32 is not None else (1 if workload == "c5" else 2)
     if plain:
         cfg.linear_solver = T.SOLVER_PCG
         cfg.precision = T.PREC_FP64
@@ -94,9 +96,10 @@ def main():
                     help="c3: 192x96x96 cantilever (headline); c4: 128x64x64 bridge, 4 load cases, batched; "
                          "c5: 160x80x80 force inverter, forward + adjoint solves; 2d: the 2-D SIMP leg alone "
                          "(1024x512 quad4 cantilever)")
-    ap.add_argument("--coarse-fp32", type=int, default=2, choices=(0, 1, 2),
+    ap.add_argument("--coarse-fp32", type=int, default=None, choices=(0, 1, 2),
                     help="copy of the FP64 coarse factor inverse read by the FP32 V-cycle: 0 none (FP64), "
-                         "1 FP32, 2 FP16 with one scale per row")
+                         "1 FP32, 2 FP16 with one scale per row (default: the problem's own, FP16 for the "
+                         "cantilever and the bridge, FP32 for the inverter)")
     ap.add_argument("--mixed", choices=("pc", "ir"), default="pc",
                     help="pc: FP64 MG-PCG with an FP32 V-cycle preconditioner; ir: FP64 iterative refinement "
                          "around an FP32 inner MG-PCG")
@@ -201,12 +204,12 @@ def other_workload(T, P, args):
     if args.workload == "c4":
         grid = (128, 64, 64)
         S, infos, t_b = run_solver(T, P, grid, mixed_prec, args.steps, args.warmup, args.levels, workload="c4",
-                                   rhs_tile=0)
+                                   rhs_tile=0, coarse_fp32=args.coarse_fp32)
         R, nt, cx, tlog = S.tuning()
         ms32, by32 = S.time_op(T.TIME_APPLYA, T.PREC_MIXED, 0, 50)
         S.close()
         S1, infos1, t_s = run_solver(T, P, grid, mixed_prec, args.steps, args.warmup, args.levels, workload="c4",
-                                     rhs_tile=1)
+                                     rhs_tile=1, coarse_fp32=args.coarse_fp32)
         S1.close()
         extra = {"batched_vs_one_at_a_time": t_s / t_b, "one_at_a_time_iters_per_s": 1.0 / t_s,
                  "compliance_rel_vs_one_at_a_time": _rel(infos, infos1),
@@ -218,7 +221,7 @@ def other_workload(T, P, args):
     else:
         grid = (160, 80, 80)
         S, infos, t_b = run_solver(T, P, grid, mixed_prec, args.steps, args.warmup, args.levels, workload="c5",
-                                   cg_max_iter=2000)
+                                   cg_max_iter=2000, coarse_fp32=args.coarse_fp32)
         ms32, by32 = S.time_op(T.TIME_APPLYA, T.PREC_MIXED, 0, 50)
         S.close()
         extra = {}

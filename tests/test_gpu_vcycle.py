@@ -102,9 +102,16 @@ def test_nested_dissection_coarse_solve_matches_banded(gpu, monkeypatch, depth, 
         cfg.cg_tol = 1e-6
         with T.Solver(cfg, fixed, F) as S:
             _, c, info = S.fe(x, 3.0)
-        out[mode] = (Ma, Mb, c, info)
-    Ma0, Mb0, c0, i0 = out["banded"]
-    Ma, Mb, c, info = out["nd"]
+        cfg.coarse_fp32 = 1
+        with T.Solver(cfg, fixed, F) as S:
+            _, c32, info32 = S.fe(x, 3.0)
+        out[mode] = (Ma, Mb, c, info, c32, info32)
+    Ma0, Mb0, c0, i0, c320, i320 = out["banded"]
+    Ma, Mb, c, info, c32, info32 = out["nd"]
+    # the FP32 copy (both products over the same rows, the second by column tiles)
+    assert info32.residual <= 1e-6 and i320.residual <= 1e-6
+    assert abs(c32 - c320) <= 1e-7 * abs(c320) and abs(c32 - c0) <= 1e-7 * abs(c0)
+    assert abs(info32.cg_iters - i320.cg_iters) <= 3
     assert np.abs(Ma - Ma0).max() <= 1e-10 * np.abs(Ma0).max(), np.abs(Ma - Ma0).max()
     assert np.abs(Mb - Mb0).max() <= 1e-10 * np.abs(Mb0).max()
     assert not Ma[fixed > 0].any()

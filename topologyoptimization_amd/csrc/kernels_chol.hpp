@@ -265,15 +265,17 @@ __global__ void k_copy_block(const double* __restrict__ src, int64_t lds, double
 // ancestor DOFs, plus the update blocks of its (at most two) children, whose
 // rows and columns are exactly this front's, added in child order. A: the
 // coarse free block in the nested-dissection order (ld n); g: front index ->
-// position; C0 / C1: a child's update block (ld0 / ld1) or null.
+// position; C0 / C1: a child's update block (ld0 / ld1; its lower triangle,
+// read symmetrically) or null.
 __global__ void k_front_assemble(double* __restrict__ F, int64_t f, int64_t m, const int32_t* __restrict__ g,
                                  const double* __restrict__ A, int64_t n, const double* __restrict__ C0, int64_t ld0,
                                  const double* __restrict__ C1, int64_t ld1) {
     for (int64_t w = gtid(); w < f * f; w += gstride()) {
         const int64_t a = w / f, b = w - a * f;
         double v = (a < m || b < m) ? A[(int64_t)g[a] * n + g[b]] : 0.0;
-        if (C0) v += C0[a * ld0 + b];
-        if (C1) v += C1[a * ld1 + b];
+        const int64_t hi = a > b ? a : b, lo = a > b ? b : a;
+        if (C0) v += C0[hi * ld0 + lo];
+        if (C1) v += C1[hi * ld1 + lo];
         F[w] = v;
     }
 }
@@ -491,15 +493,16 @@ __global__ void __launch_bounds__(256) k_tri_part(const TK* __restrict__ M, int6
             for (int r = 0; r < RW; r++) part[((int64_t)q * nch + ch) * n + row0 + r] = acc[r][q];
 }
 
-// Column sums x_j = sum_i M[i][j] y_i of a triangular FP16 M (row-major,
-// zeros in the other triangle; upper = 1 for an upper-triangular M): each
+// Column sums x_j = sum_i M[i][j] y_i of a triangular FP16 or FP32 M
+// (row-major, zeros in the other triangle; upper = 1 for an upper-triangular
+// M): each
 // lane streams 16-byte pieces of whole rows, the right-hand side is one
 // uniform value per row, so these tiles read the matrix at close to the
 // copy bandwidth (the row-oriented k_tri_part re-reads the right-hand side
 // with every row piece). Z b is the column sums of Z^T, Z^T y those of Z.
 // Block
 // (cb, rb) covers the rows [rb 256, rb 256 + 256) and the columns
-// [cb 512, cb 512 + 512): lane l owns 8 columns, wave w 64 of the rows
+// [cb 512, cb 512 + 512) (FP32: 256 columns): lane l owns 8 (4) columns, wave w 64 of the rows
 // (y_i read once per wave as a uniform value); the 4 waves' FP32 column sums
 // meet in LDS and leave one FP64 partial per column and row block,
 // part[(q nrb + rb) n + j] (0 for blocks above the diagonal), summed by
@@ -507,12 +510,15 @@ __global__ void __launch_bounds__(256) k_tri_part(const TK* __restrict__ M, int6
 // for the 64 rows of a wave; a wave whose rows are zero over the block's
 // columns does not read them.
 constexpr int kTcRows = 256;      // rows per block
-constexpr int kTcCols = 512;      // columns per block (64 lanes x 8)
-template <int NR, typename TB>
-__global__ void __launch_bounds__(256) k_tri_colpart(const _Float16* __restrict__ M, int64_t n,
+constexpr int kTcCols = 512;      // FP16 columns per block (64 lanes x 8)
+template <typename TK> constexpr int tc_cols() { return 64 * (16 / (int)sizeof(TK)); }
+template <int NR, typename TB, typename TK = _Float16>
+__global__ void __launch_bounds__(256) k_tri_colpart(const TK* __restrict__ M, int64_t n,
                                                      const TB* __restrict__ y, double* __restrict__ part, int nrb,
                                                      const int* gate, int upper = 0,
                                                      const int32_t* __restrict__ rfirst = nullptr) {
+    constexpr int VN = 16 / (int)sizeof(TK);   // entries per 16-byte piece
+    constexpr int kTcCols = 64 * VN;
     __shared__ float sh[4][NR][kTcCols];
     const int cb = blockIdx.x, rb = blockIdx.y;
     const int64_t J0 = (int64_t)cb * kTcCols, I0 = (int64_t)rb * kTcRows;
@@ -528,32 +534,40 @@ __global__ void __launch_bounds__(256) k_tri_colpart(const _Float16* __restrict_
     }
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int64_t j = J0 + 8 * lane;
+    const int64_t j = J0 + VN * lane;
     const bool colok = j < n;
-    float acc[NR][8];
+    float acc[NR][VN];
 #pragma unroll
     for (int q = 0; q < NR; q++)
 #pragma unroll
-        for (int u = 0; u < 8; u++) acc[q][u] = 0.0f;
+        for (int u = 0; u < VN; u++) acc[q][u] = 0.0f;
     const int64_t r0 = I0 + (int64_t)w * (kTcRows / 4);
     const int64_t r1 = r0 + kTcRows / 4 < n ? r0 + kTcRows / 4 : n;
     const bool wz = rfirst && r0 < n && (int64_t)rfirst[r0] > J0 + kTcCols - 1;   // wave-uniform
     if (colok && !wz) {
 #pragma unroll 8
         for (int64_t i = r0; i < r1; i++) {
-            const half8_t a = *(const half8_t*)(M + i * n + j);
+            float av[VN];
+            if constexpr (VN == 8) {
+                const half8_t a = *(const half8_t*)(M + i * n + j);
+#pragma unroll
+                for (int u = 0; u < 8; u++) av[u] = (float)a[u];
+            } else {
+                const float4 a = *(const float4*)(M + i * n + j);
+                av[0] = a.x; av[1] = a.y; av[2] = a.z; av[3] = a.w;
+            }
 #pragma unroll
             for (int q = 0; q < NR; q++) {
                 const float yv = (float)y[q * n + i];
 #pragma unroll
-                for (int u = 0; u < 8; u++) acc[q][u] = fmaf((float)a[u], yv, acc[q][u]);
+                for (int u = 0; u < VN; u++) acc[q][u] = fmaf(av[u], yv, acc[q][u]);
             }
         }
     }
 #pragma unroll
     for (int q = 0; q < NR; q++)
 #pragma unroll
-        for (int u = 0; u < 8; u++) sh[w][q][8 * lane + u] = acc[q][u];
+        for (int u = 0; u < VN; u++) sh[w][q][VN * lane + u] = acc[q][u];
     __syncthreads();
     for (int q = 0; q < NR; q++) {
         if (!live[q]) continue;

@@ -316,8 +316,9 @@ struct MGLevel {
     int32_t* freeOf = nullptr;     // DOF -> free index or -1
     double* bf = nullptr;          // gathered right-hand side (padded)
     double* Kinv = nullptr;        // banded factor L, then Z^T (triangular mode); or the inverse
-    float* Kinv32 = nullptr;       // FP32 rounding of Kinv (coarse_fp32)
+    float* Kinv32 = nullptr;       // FP32 rounding of the Gauss-Jordan inverse (coarse_fp32)
     bool tri = false;              // coarse solve by x = Z^T (Z b), Z = L^-1 (else Kinv b)
+    bool zt = false;               // Kinv holds Z^T (the FP64 products' second operand)
     int bandw = 0;                 // max |i - j| of the free block's nonzeros
     double* Z = nullptr;           // L^-1 (lower triangular)
     float* Z32 = nullptr;
@@ -494,14 +495,21 @@ __global__ void k_jacobi_first(T* __restrict__ x, const T* __restrict__ b, const
 // DOFs, padded to ld = nfp, a multiple of 4, so rows are 16-byte aligned):
 // bf[j] = b[free[j]] (zero for the padding), then x[free[i]] = sum_j
 // Kinv[i][j] bf[j]; x = 0 on fixed DOFs.
+// With xz (a buffer other than b), also xz = 0 over the level's ndof DOFs
+// (the fixed DOFs of the solution keep that zero).
 template <typename T, typename TB = double>
 __global__ void k_gather_free(const T* __restrict__ b, const int32_t* __restrict__ freeIdx, int64_t nf, int64_t nfp,
-                              TB* __restrict__ bf, const int* gate, int64_t ndof) {
+                              TB* __restrict__ bf, const int* gate, int64_t ndof, T* __restrict__ xz = nullptr) {
     gate = batch_gate(gate);
     if (gate && *gate) return;
     b += batch_off(ndof);
     bf += batch_off(nfp);
-    for (int64_t j = gtid(); j < nfp; j += gstride()) bf[j] = (j < nf && freeIdx[j] >= 0) ? (TB)b[freeIdx[j]] : TB(0);
+    if (xz) xz += batch_off(ndof);
+    const int64_t m = xz && ndof > nfp ? ndof : nfp;
+    for (int64_t j = gtid(); j < m; j += gstride()) {
+        if (j < nfp) bf[j] = (j < nf && freeIdx[j] >= 0) ? (TB)b[freeIdx[j]] : TB(0);
+        if (xz && j < ndof) xz[j] = T(0);
+    }
 }
 // one wave per 4 consecutive rows; each lane streams 16-byte vectors of the
 // four rows against the same slices of the NR right-hand sides bf (read from
@@ -580,13 +588,6 @@ __global__ void __launch_bounds__(256) k_inv_apply(const TK* __restrict__ Kinv, 
             if (live[q])
                 for (int r = 0; r < nr; r++)
                     if (freeIdx[row0 + r] >= 0) x[q * ndof + freeIdx[row0 + r]] = (T)acc[r][q];
-}
-template <typename T>
-__global__ void k_zero_fixed_or_all(T* __restrict__ x, int64_t n, const int* gate) {
-    gate = batch_gate(gate);
-    if (gate && *gate) return;
-    x += batch_off(n);
-    for (int64_t i = gtid(); i < n; i += gstride()) x[i] = T(0);
 }
 
 // Dense symmetric positive (semi-)definite inverse by Gauss-Jordan without
@@ -816,13 +817,17 @@ __global__ void k_assemble_free(const double* __restrict__ E, double thr, Grid g
     }
 }
 // rows/cols of free DOFs without stiffness (and the padding rows) get a unit
-// diagonal
+// diagonal: one workgroup per row (grid nf)
 __global__ void k_regularize(double* A, int64_t nf) {
-    for (int64_t i = gtid(); i < nf; i += gstride())
-        if (!(A[i * nf + i] > 1e-30)) {
-            for (int64_t j = 0; j < nf; j++) { A[i * nf + j] = 0; A[j * nf + i] = 0; }
-            A[i * nf + i] = 1.0;
+    const int64_t i = blockIdx.x;
+    if (i < nf && !(A[i * nf + i] > 1e-30)) {
+        for (int64_t j = threadIdx.x; j < nf; j += blockDim.x) {
+            A[i * nf + j] = 0;
+            A[j * nf + i] = 0;
         }
+        __syncthreads();
+        if (threadIdx.x == 0) A[i * nf + i] = 1.0;
+    }
 }
 
 struct MGH {
@@ -1052,7 +1057,7 @@ struct MGH {
         else
             hipLaunchKernelGGL(k_assemble_free, dim3(nblk(nf * 81)), dim3(256), 0, c.s, (const double*)l.E, thr, l.g,
                                l.freeIdx, l.freeOf, nf, ld, (const double*)K0->ke, l.Kinv);
-        hipLaunchKernelGGL(k_regularize, dim3(nblk(ld)), dim3(256), 0, c.s, l.Kinv, ld);
+        hipLaunchKernelGGL(k_regularize, dim3((unsigned)ld), dim3(256), 0, c.s, l.Kinv, ld);
     }
     void factor_coarsest(Ctx& c) {
         MGLevel& l = L.back();
@@ -1231,23 +1236,33 @@ struct MGH {
         }
         c.check_launch();
         l.chol = true;
+        if (std::getenv("TOPO_VERBOSE"))
+            std::fprintf(stderr, "[topo] coarsest level: %lld positions, %s\n", (long long)l.nfp,
+                         l.tri ? (l.nd ? "nested-dissection Cholesky" : "banded Cholesky")
+                               : "Gauss-Jordan inverse (factorization met a non-positive pivot)");
     }
     // what the coarse solves read besides Z: Z^T in the Kinv buffer (FP64
     // products), the FP16 copy with its row scales or the FP32 copies
     void z_copies(Ctx& c, MGLevel& l) {
         const int64_t n = l.nfp;
-        const int NB = (int)(n / kCB);
-        hipLaunchKernelGGL((k_transpose<double, double>), dim3(NB, NB), dim3(256), 0, c.s, (const double*)l.Z, l.Kinv,
-                           n);
+        // Z^T now when the V-cycles of this solver read it (FP64, or the FP32
+        // copy made from it), else on the first FP64 coarse solve
+        l.zt = false;
+        if (!want32 || (!l.Z16 && !l.Z32)) transpose_z(c, l);
         if (l.Z16)
             hipLaunchKernelGGL(k_rowscale_f16, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, c.s, (const double*)l.Z, n,
                                l.Z16, l.zs2, (const int32_t*)l.rfirst);
-        if (l.Z32) {
+        if (l.Z32)
             hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(n * n)), dim3(256), 0, c.s, l.Z32,
                                (const double*)l.Z, n * n);
-            hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(n * n)), dim3(256), 0, c.s, l.Kinv32,
-                               (const double*)l.Kinv, n * n);
-        }
+    }
+
+    void transpose_z(Ctx& c, MGLevel& l) {
+        const int64_t n = l.nfp;
+        const int NB = (int)(n / kCB);
+        hipLaunchKernelGGL((k_transpose<double, double>), dim3(NB, NB), dim3(256), 0, c.s, (const double*)l.Z, l.Kinv,
+                           n);
+        l.zt = true;
     }
 
     // fronts, gather maps, descriptors and row starts of the
@@ -1342,6 +1357,8 @@ struct MGH {
                                           (rocblas_int)t.f, Zd, (rocblas_int)n) == rocblas_status_success,
                            TOPO_E_HIP, "rocblas_dtrtri failed");
                 const int64_t u = t.f - t.m;
+                // plain GEMMs: faster here than rocBLAS's triangular (trmm) and
+                // symmetric (syrk) forms at half the flops
                 if (u > 0) {
                     gemm_rm(h, false, true, u, t.m, t.m, 1.0, t.F + t.m * t.f, t.f, Zd, n, 0.0, t.Lb, t.m);
                     gemm_rm(h, false, true, u, u, t.m, -1.0, t.Lb, t.m, t.Lb, t.m, 1.0, t.F + t.m * t.f + t.m, t.f);
@@ -1356,15 +1373,23 @@ struct MGH {
             // W = L_aD Z_DD (rows in the order of the ancestor path)
             gemm_rm(h, false, false, u, t.m, t.m, 1.0, t.Lb, t.m, Zd, n, 0.0, l.ndW, t.m);
             std::vector<int> path;
-            for (int p = t.parent; p >= 0; p = l.ndn[p].parent) path.push_back(p);
+            std::vector<int64_t> offs;
+            int64_t off = 0;
+            for (int p = t.parent; p >= 0; p = l.ndn[p].parent) {
+                path.push_back(p);
+                offs.push_back(off);
+                off += l.ndn[p].m;
+            }
             for (size_t i = 0; i < path.size(); i++) {
                 const MGLevel::NDNode& P = l.ndn[path[i]];
-                int64_t off = 0;
-                for (size_t j = 0; j <= i; j++) {
+                double* C = l.Z + P.o0 * n + t.o0;
+                // diagonal term first (Z_PP lower triangular, zero above)
+                gemm_rm(h, false, false, P.m, t.m, P.m, -1.0, l.Z + P.o0 * n + P.o0, n, l.ndW + offs[i] * t.m, t.m,
+                        0.0, C, n);
+                for (size_t j = 0; j < i; j++) {
                     const MGLevel::NDNode& Q = l.ndn[path[j]];
-                    gemm_rm(h, false, false, P.m, t.m, Q.m, -1.0, l.Z + P.o0 * n + Q.o0, n, l.ndW + off * t.m, t.m,
-                            j ? 1.0 : 0.0, l.Z + P.o0 * n + t.o0, n);
-                    off += Q.m;
+                    gemm_rm(h, false, false, P.m, t.m, Q.m, -1.0, l.Z + P.o0 * n + Q.o0, n, l.ndW + offs[j] * t.m,
+                            t.m, 1.0, C, n);
                 }
             }
         }
@@ -1605,15 +1630,16 @@ struct MGH {
         T* x2 = Sel<T>::x2(l);
         const bool coarsest = li == (int)L.size() - 1;
         if (coarsest && l.chol) {
-            hipLaunchKernelGGL(k_zero_fixed_or_all<T>, dim3(nblk(l.ndof), nr), dim3(256), 0, c.s, x, l.ndof, gate);
-            // the FP16 coarse copy works on FP32 vectors (float views of bf, ytmp)
+            // the FP16 coarse copy works on FP32 vectors (float views of bf,
+            // ytmp); the gather also zeroes x (b is another buffer)
             const bool f16 = std::is_same<T, float>::value && l.Z16 && l.tri;
+            const int gn = nblk(std::max(l.nfp, l.ndof));
             if (f16)
-                hipLaunchKernelGGL((k_gather_free<T, float>), dim3(nblk(l.nfp), nr), dim3(256), 0, c.s, b, l.freeIdx,
-                                   l.nf, l.nfp, (float*)l.bf, gate, l.ndof);
+                hipLaunchKernelGGL((k_gather_free<T, float>), dim3(gn, nr), dim3(256), 0, c.s, b, l.freeIdx, l.nf,
+                                   l.nfp, (float*)l.bf, gate, l.ndof, x);
             else
-                hipLaunchKernelGGL((k_gather_free<T, double>), dim3(nblk(l.nfp), nr), dim3(256), 0, c.s, b, l.freeIdx,
-                                   l.nf, l.nfp, l.bf, gate, l.ndof);
+                hipLaunchKernelGGL((k_gather_free<T, double>), dim3(gn, nr), dim3(256), 0, c.s, b, l.freeIdx, l.nf,
+                                   l.nfp, l.bf, gate, l.ndof, x);
             const dim3 gi((unsigned)((l.nf + 15) / 16));
             const bool f32 = std::is_same<T, float>::value && l.Kinv32;
             if (l.tri) {
@@ -1627,7 +1653,7 @@ struct MGH {
                     // of it in the memory-side cache)
                     tri_product<T, _Float16, float, float>(c, l, l.Z16, 0, (const float*)l.bf, (float*)l.ytmp,
                                                            (T*)nullptr, gate, l.zs2, nr);
-                    const dim3 gc((unsigned)((l.nfp + kTcCols - 1) / kTcCols), (unsigned)l.tnrb);
+                    const dim3 gc((unsigned)((l.nfp + tc_cols<_Float16>() - 1) / tc_cols<_Float16>()), (unsigned)l.tnrb);
                     TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_colpart<NR, float>), gc, dim3(256), 0, c.s,
                                                           (const _Float16*)l.Z16, l.nfp, (const float*)l.ytmp, l.tpart,
                                                           l.tnrb, gate, 0, (const int32_t*)l.rfirst));
@@ -1636,11 +1662,27 @@ struct MGH {
                                                           (float*)nullptr, x, (const int32_t*)l.freeIdx, l.nf, l.ndof,
                                                           gate, (const double*)nullptr));
                 } else if (f32 && l.Z32) {
+                    // Z b by rows, Z^T y by column tiles of the same FP32 copy
                     tri_product<T, float, double, double>(c, l, l.Z32, 0, (const double*)l.bf, l.ytmp, (T*)nullptr,
                                                           gate, nullptr, nr);
-                    tri_product<T, float, double, double>(c, l, l.Kinv32, 1, (const double*)l.ytmp, (double*)nullptr,
-                                                          x, gate, nullptr, nr);
+                    const dim3 gc((unsigned)((l.nfp + tc_cols<float>() - 1) / tc_cols<float>()), (unsigned)l.tnrb);
+                    TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_colpart<NR, double, float>), gc, dim3(256), 0, c.s,
+                                                          (const float*)l.Z32, l.nfp, (const double*)l.ytmp, l.tpart,
+                                                          l.tnrb, gate, 0, (const int32_t*)l.rfirst));
+                    TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_sum<T, NR, double>), dim3(nblk(4 * l.nfp)), dim3(256),
+                                                          0, c.s, (const double*)l.tpart, l.tnrb, l.nfp,
+                                                          (double*)nullptr, x, (const int32_t*)l.freeIdx, l.nf, l.ndof,
+                                                          gate, (const double*)nullptr));
                 } else {
+                    if (!l.zt) {
+                        // an FP64 coarse solve in a solver whose V-cycles are
+                        // FP16: Z^T made once, never inside a captured graph
+                        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+                        TOPO_HIP(hipStreamIsCapturing(c.s, &cs));
+                        TOPO_CHECK(cs == hipStreamCaptureStatusNone, TOPO_E_STATE,
+                                   "FP64 coarse solve first reached under graph capture");
+                        transpose_z(c, l);
+                    }
                     tri_product<T, double, double, double>(c, l, l.Z, 0, (const double*)l.bf, l.ytmp, (T*)nullptr,
                                                            gate, nullptr, nr);
                     tri_product<T, double, double, double>(c, l, l.Kinv, 1, (const double*)l.ytmp, (double*)nullptr,
@@ -2138,6 +2180,7 @@ struct Solver {
         if (const char* e = getenv("TOPO_ND_DEPTH")) mg.nd_depth = atoi(e);
         if (const char* e = getenv("TOPO_ND_MIN")) mg.nd_min = atoll(e);
         if (const char* e = getenv("TOPO_ND_LEAF")) mg.nd_leaf = atoll(e);
+
         if (const char* e = getenv("TOPO_MIXED_ETA")) mixed_eta = atof(e);
         const int nlev = cfg.linear_solver == TOPO_SOLVER_MGPCG ? cfg.mg_levels : 1;
         mg.nrhs = batch_cap;

@@ -145,12 +145,18 @@ def inverter(nelx=160, nely=80, nelz=80, **kw):
     chosen here): every DOF of the x = 0 face nodes in the four corner
     squares of side nely/8 x nelz/8. Helmholtz filter rmin 2 (PDE length
     2 / (2 sqrt 3)), Heaviside beta 8, eta 0.5, volfrac 0.3, p = 3, FP32
-    MG-PCG inside FP64 refinement (4 levels, Cholesky coarsest), tol 1e-6."""
+    MG-PCG inside FP64 refinement (4 levels, Cholesky coarsest), tol 1e-6.
+    The V-cycle reads the FP32 copy of the coarse factor inverse: with only
+    the corner supports the coarse operator is ill-conditioned enough that
+    the FP16 copy changes the solution (output displacement 12 % off the
+    FP64 coarse solve's at the same CG tolerance) and doubles the CG
+    iterations."""
     cfg = default_config(nelx, nely, nelz, volfrac=0.3, penal=3.0, penal_start=3.0, rmin=2.0,
                          filter_type=FILTER_HELMHOLTZ, use_projection=1, beta_fixed=8, eta=0.5, max_iterations=20,
                          cg_tol=1e-6, skip_factor=0.0, mg_omega_cap=1.6, objective=OBJ_MECHANISM,
                          linear_solver=SOLVER_MGPCG, precision=PREC_MIXED, mg_levels=4, mg_omega=0.6,
-                         coarse_solver=COARSE_CHOLESKY, galerkin_max_ndof=0, mech_k_in=1.0, mech_k_out=0.001)
+                         coarse_solver=COARSE_CHOLESKY, galerkin_max_ndof=0, mech_k_in=1.0, mech_k_out=0.001,
+                         coarse_fp32=1)
     for k, v in kw.items():
         setattr(cfg, k, v)
     nel, ndof = dims(nelx, nely, nelz)
