@@ -134,6 +134,8 @@ def main():
     tR, tnt, tcx, _ = S.tuning()
     ms32, by32 = S.time_op(T.TIME_APPLYA, T.PREC_MIXED, 0, 50)
     ms64, by64 = S.time_op(T.TIME_APPLYA, T.PREC_FP64, 0, 30)
+    # the CG's FP64 operator (fixed DOFs masked, p.Ap partials in the epilogue)
+    ms64m, _ = S.time_op(T.TIME_APPLYA_MASKED, T.PREC_FP64, 0, 30)
     msc, byc = S.time_op(T.TIME_COPY, T.PREC_FP64, 0, 30)
     msv, _ = S.time_op(T.TIME_VCYCLE, T.PREC_MIXED, 0, 5)
     S.close()
@@ -180,7 +182,7 @@ def main():
         "mixed_vs_plain_fp64_pcg": (t_plain / t_mixed) if t_plain else None,
         "compliance_rel_vs_plain": _rel(infos, infos_p) if t_plain else None,
         "gdof_matvec_per_s": {"fp32": ndof / (ms32 * 1e-3) / 1e9, "fp64": ndof / (ms64 * 1e-3) / 1e9},
-        "applyA_ms": {"fp32": ms32, "fp64": ms64}, "vcycle_ms_fp32": msv,
+        "applyA_ms": {"fp32": ms32, "fp64": ms64, "fp64_masked_pap": ms64m}, "vcycle_ms_fp32": msv,
         "copy_GBps": byc / (msc * 1e-3) / 1e9,
         "fine_applyA_per_step": fine_calls,
         "compliance": infos[-1]["compliance"],

@@ -342,7 +342,8 @@ int topo_op_oc(topo_solver* s, const float* x, const float* dcn, float* xnew,
 
 /* ---- timing (HIP events on the stream the kernels run on) ------------ */
 enum { TOPO_TIME_APPLYA = 0, TOPO_TIME_VCYCLE = 1, TOPO_TIME_ENERGY = 2,
-       TOPO_TIME_FILTER = 3, TOPO_TIME_OC = 4, TOPO_TIME_COPY = 5 };
+       TOPO_TIME_FILTER = 3, TOPO_TIME_OC = 4, TOPO_TIME_COPY = 5,
+       TOPO_TIME_APPLYA_MASKED = 6 /* the CG's operator: fixed DOFs masked, p.Ap partials */ };
 /* run `op` reps times back to back on device-resident data and return the
  * average duration of one launch in ms (and the algorithmic bytes of one). */
 int topo_time_op(topo_solver* s, int32_t op, int32_t prec, int32_t level, int32_t reps,

@@ -12,6 +12,27 @@
 
 namespace topo {
 
+// Sum of a double over the 64 lanes of a wavefront, left in lane 63 (the
+// other lanes hold partial sums): DPP row shifts by 1, 2, 4, 8 (inclusive
+// sums within each row of 16 lanes), then the row broadcasts of lane 15 and
+// lane 31 carry the row totals up. VALU only, no LDS round trips. The
+// order of additions is fixed by the lane positions.
+template <int CTRL, int RMASK>
+__device__ __forceinline__ double dpp_d(double x) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, RMASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, RMASK, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+    v += dpp_d<0x111, 0xf>(v);   // row_shr:1
+    v += dpp_d<0x112, 0xf>(v);   // row_shr:2
+    v += dpp_d<0x114, 0xf>(v);   // row_shr:4
+    v += dpp_d<0x118, 0xf>(v);   // row_shr:8
+    v += dpp_d<0x142, 0xa>(v);   // row_bcast:15 into rows 1 and 3
+    v += dpp_d<0x143, 0xc>(v);   // row_bcast:31 into rows 2 and 3
+    return v;
+}
+
 // K0 (24x24 row-major, FP64 and FP32 copies) and the filter stencil live in
 // per-solver device buffers passed by pointer; every lane reads the same
 // address, so the loads are scalar.

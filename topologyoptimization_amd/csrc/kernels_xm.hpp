@@ -250,6 +250,7 @@ __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __
     };
 
     T fprev[12], carry[12];
+    double cpair = 0.0;   // this thread's share of the current plane pair (MODE 1, 5)
 #pragma unroll
     for (int k = 0; k < 12; k++) carry[k] = T(0);
     fetch(X0 - 1);
@@ -343,14 +344,16 @@ __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __
         }
         // one partial per (yz tile, node-plane pair 2k, 2k+1): the same
         // sums whatever the (even) x-chunking, so the CG scalars do not
-        // depend on the element tiles. Each wave adds its sum of the plane to
-        // its own LDS slot of the pair (no barrier here); the slots are
-        // added after the march.
+        // depend on the element tiles. Each thread adds the pair's two
+        // planes, then each wave sums the pair (DPP, lane 63) into its own
+        // LDS slot of the pair (no barrier here); the slots are added after
+        // the march.
         if ((MODE == 1 || MODE == 5) && part && ex >= X0) {
-            for (int off = 32; off > 0; off >>= 1) cpl += __shfl_down(cpl, off, 64);
-            if ((t & 63) == 0) {
-                double& sl = s_red[((ex - X0) >> 1) * (NT / 64) + (t >> 6)];
-                sl = ((ex - X0) & 1) ? sl + cpl : cpl;
+            cpair += cpl;
+            if (((ex - X0) & 1) || ex == X1 - 1) {
+                const double ws = wave_sum_dpp(cpair);
+                if ((t & 63) == 63) s_red[((ex - X0) >> 1) * (NT / 64) + (t >> 6)] = ws;
+                cpair = 0.0;
             }
         }
     }

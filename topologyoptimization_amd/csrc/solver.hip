@@ -3205,6 +3205,11 @@ int topo_time_op(topo_solver* h, int32_t op, int32_t prec, int32_t level, int32_
             if (f32) S.mg.apply<float>(C, level, l.x32, l.x232, 0, nullptr, nullptr);
             else S.mg.apply<double>(C, level, l.x, l.x2, 0, nullptr, nullptr);
             break;
+        case TOPO_TIME_APPLYA_MASKED:
+            // the CG's operator: masked, with the p.Ap partials
+            if (f32) S.mg.apply<float>(C, level, l.x32, l.x232, 1, nullptr, nullptr, 1, C.partials);
+            else S.mg.apply<double>(C, level, l.x, l.x2, 1, nullptr, nullptr, 1, C.partials);
+            break;
         case TOPO_TIME_VCYCLE:
             if (f32) S.mg.vcycle<float>(C, 0, S.mg.L[0].b32, nullptr);
             else S.mg.vcycle<double>(C, 0, S.mg.L[0].b, nullptr);
@@ -3228,6 +3233,7 @@ int topo_time_op(topo_solver* h, int32_t op, int32_t prec, int32_t level, int32_
     // algorithmic bytes of one launch
     const double sv = f32 ? 4.0 : 8.0;
     if (op == TOPO_TIME_APPLYA) by = 2.0 * l.ndof * sv + (double)l.g.nel() * sv;
+    else if (op == TOPO_TIME_APPLYA_MASKED) by = 2.0 * l.ndof * sv + (double)l.g.nel() * sv + (double)l.ndof;
     else if (op == TOPO_TIME_COPY) by = 2.0 * l.ndof * 8.0;
     else if (op == TOPO_TIME_ENERGY) by = (double)S.ndof * 8 + (double)S.nel * 4 * 4;
     else if (op == TOPO_TIME_FILTER) by = (double)S.nel * 4 * 3;
