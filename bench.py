@@ -131,7 +131,7 @@ def main():
                                    coarse_fp32=args.coarse_fp32)
     # compliance of the mixed path must match the FP64 path (checked below)
     # dominant kernel: fine-level FP32 applyA, timed with HIP events on the solver's stream
-    tR, tnt, tcx, _ = S.tuning()
+    tR, tnt, tcx, tlog = S.tuning()
     ms32, by32 = S.time_op(T.TIME_APPLYA, T.PREC_MIXED, 0, 50)
     ms64, by64 = S.time_op(T.TIME_APPLYA, T.PREC_FP64, 0, 30)
     # the CG's FP64 operator (fixed DOFs masked, p.Ap partials in the epilogue)
@@ -189,6 +189,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "autotuned_fine_tile": {"xm_nt": tnt, "xm_cx": tcx},
+        "autotune_candidates": [[int(a), int(b), int(c), float(d)] for a, b, c, d in tlog],
         "simp2d": simp2d,
         "device": name,
     }

@@ -507,19 +507,21 @@ __global__ void __launch_bounds__(256) k_tri_part(const TK* __restrict__ M, int6
 // meet in LDS and leave one FP64 partial per column and row block,
 // part[(q nrb + rb) n + j] (0 for blocks above the diagonal), summed by
 // k_tri_sum. rfirst (optional): first nonzero column of each row, the same
-// for the 64 rows of a wave; a wave whose rows are zero over the block's
+// for the rows of a wave; a wave whose rows are zero over the block's
 // columns does not read them.
 constexpr int kTcRows = 256;      // rows per block
 constexpr int kTcCols = 512;      // FP16 columns per block (64 lanes x 8)
 template <typename TK> constexpr int tc_cols() { return 64 * (16 / (int)sizeof(TK)); }
-template <int NR, typename TB, typename TK = _Float16>
-__global__ void __launch_bounds__(256) k_tri_colpart(const TK* __restrict__ M, int64_t n,
-                                                     const TB* __restrict__ y, double* __restrict__ part, int nrb,
-                                                     const int* gate, int upper = 0,
-                                                     const int32_t* __restrict__ rfirst = nullptr) {
+// NW waves per block (4 or 8), kTcRows / NW rows each.
+template <int NR, typename TB, typename TK = _Float16, int NW = 4>
+__global__ void __launch_bounds__(64 * NW) k_tri_colpart(const TK* __restrict__ M, int64_t n,
+                                                         const TB* __restrict__ y, double* __restrict__ part, int nrb,
+                                                         const int* gate, int upper = 0,
+                                                         const int32_t* __restrict__ rfirst = nullptr) {
     constexpr int VN = 16 / (int)sizeof(TK);   // entries per 16-byte piece
     constexpr int kTcCols = 64 * VN;
-    __shared__ float sh[4][NR][kTcCols];
+    static_assert(NW == 4 || NW == 8, "4 or 8 waves per block");
+    __shared__ float sh[NW][NR][kTcCols];
     const int cb = blockIdx.x, rb = blockIdx.y;
     const int64_t J0 = (int64_t)cb * kTcCols, I0 = (int64_t)rb * kTcRows;
     bool live[NR];
@@ -541,8 +543,8 @@ __global__ void __launch_bounds__(256) k_tri_colpart(const TK* __restrict__ M, i
     for (int q = 0; q < NR; q++)
 #pragma unroll
         for (int u = 0; u < VN; u++) acc[q][u] = 0.0f;
-    const int64_t r0 = I0 + (int64_t)w * (kTcRows / 4);
-    const int64_t r1 = r0 + kTcRows / 4 < n ? r0 + kTcRows / 4 : n;
+    const int64_t r0 = I0 + (int64_t)w * (kTcRows / NW);
+    const int64_t r1 = r0 + kTcRows / NW < n ? r0 + kTcRows / NW : n;
     const bool wz = rfirst && r0 < n && (int64_t)rfirst[r0] > J0 + kTcCols - 1;   // wave-uniform
     if (colok && !wz) {
 #pragma unroll 8
@@ -573,7 +575,9 @@ __global__ void __launch_bounds__(256) k_tri_colpart(const TK* __restrict__ M, i
         if (!live[q]) continue;
         for (int t = threadIdx.x; t < kTcCols; t += blockDim.x) {
             if (J0 + t >= n) continue;
-            const double v = ((double)sh[0][q][t] + (double)sh[1][q][t]) + ((double)sh[2][q][t] + (double)sh[3][q][t]);
+            double v = ((double)sh[0][q][t] + (double)sh[1][q][t]) + ((double)sh[2][q][t] + (double)sh[3][q][t]);
+            if constexpr (NW == 8)
+                v += ((double)sh[4][q][t] + (double)sh[5][q][t]) + ((double)sh[6][q][t] + (double)sh[7][q][t]);
             part[((int64_t)q * nrb + rb) * n + J0 + t] = v;
         }
     }

@@ -328,6 +328,7 @@ struct MGLevel {
     double* ytmp = nullptr;        // Z b (nrhs x nfp)
     double* tpart = nullptr;       // chunk partials of the triangular products (nrhs x max(tnch, tnrb) x nfp)
     int tcw = 0, tnch = 0;         // column chunk width (0: one wave per row quad) and count
+    int trw = 8;                   // rows per wave of the chunked product (4 or 8)
     int tnrb = 0;                  // row blocks of the transposed FP16 product (column tiles of Z16)
     double* d0c = nullptr;         // original diagonal (pivot test)
     double* ztmp = nullptr;        // 64 x nfp scratch of the Z row blocks
@@ -441,11 +442,16 @@ template <typename T, typename TK, typename TB, typename TY>
 static void tri_product(Ctx& c, const MGLevel& l, const TK* M, int upper, const TB* bf, TY* y, T* x, const int* gate,
                         const double* ys, int nr) {
     if (l.tcw) {
-        constexpr int RW = 4;
-        const dim3 gp((unsigned)l.tnch, (unsigned)(l.nfp / (4 * RW)));
-        TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_part<NR, TK, TB, RW>), gp, dim3(256), 0, c.s, M, l.nfp, upper,
-                                              bf, l.tpart, l.tcw, l.tnch, gate,
-                                              upper ? (const int32_t*)nullptr : (const int32_t*)l.rfirst));
+        const int32_t* rf = upper ? (const int32_t*)nullptr : (const int32_t*)l.rfirst;
+        if (l.trw == 8) {
+            const dim3 gp((unsigned)l.tnch, (unsigned)(l.nfp / 32));
+            TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_part<NR, TK, TB, 8>), gp, dim3(256), 0, c.s, M, l.nfp, upper,
+                                                  bf, l.tpart, l.tcw, l.tnch, gate, rf));
+        } else {
+            const dim3 gp((unsigned)l.tnch, (unsigned)(l.nfp / 16));
+            TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_part<NR, TK, TB, 4>), gp, dim3(256), 0, c.s, M, l.nfp, upper,
+                                                  bf, l.tpart, l.tcw, l.tnch, gate, rf));
+        }
         TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_sum<T, NR, TY>), dim3(nblk(4 * l.nfp)), dim3(256), 0, c.s,
                                               (const double*)l.tpart, l.tnch, l.nfp, y, x, (const int32_t*)l.freeIdx,
                                               l.nf, l.ndof, gate, ys));
@@ -454,6 +460,22 @@ static void tri_product(Ctx& c, const MGLevel& l, const TK* M, int upper, const 
         TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_apply<T, NR, TK, TB, TY>), gt, dim3(256), 0, c.s, M, l.nfp, upper,
                                               bf, y, x, (const int32_t*)l.freeIdx, l.nf, l.ndof, gate, ys));
     }
+}
+
+// Column-tile product Z^T y of the coarse solve: 8 waves per tile (32 rows
+// each) for up to two right-hand sides, else 4 (the LDS of the wave sums)
+template <int NR, typename TB, typename TK>
+static void tri_colpart(hipStream_t s, const TK* M, int64_t n, const TB* y, double* part, int nrb, const int* gate,
+                        const int32_t* rfirst, int nw) {
+    const dim3 gc((unsigned)((n + tc_cols<TK>() - 1) / tc_cols<TK>()), (unsigned)nrb);
+    if constexpr (NR <= 2) {
+        if (nw == 8) {
+            hipLaunchKernelGGL((k_tri_colpart<NR, TB, TK, 8>), gc, dim3(512), 0, s, M, n, y, part, nrb, gate, 0,
+                               rfirst);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_tri_colpart<NR, TB, TK, 4>), gc, dim3(256), 0, s, M, n, y, part, nrb, gate, 0, rfirst);
 }
 
 template <typename T> struct Sel;
@@ -855,6 +877,7 @@ struct MGH {
     // order), smallest block it is used for, largest part left uncut
     int nd_depth = 3;
     int64_t nd_min = 2048, nd_leaf = 768;
+    int tc_nw = 8;   // waves per column tile of the coarse Z^T y product (4 or 8)
     // 'petsc' solver: the domain-decomposition preconditioner replaces the
     // V-cycle (one level; omega 1 for its global Jacobi sweeps)
     bool bddc = false;
@@ -1118,6 +1141,8 @@ struct MGH {
             {
                 const char* e = std::getenv("TOPO_TRI_CW");
                 l.tcw = e ? std::atoi(e) : 2048;
+                const char* er = std::getenv("TOPO_TRI_RW");
+                l.trw = er && std::atoi(er) == 4 ? 4 : 8;
                 if (l.tcw < 0 || l.tcw % 64) l.tcw = 2048;
                 l.tnch = l.tcw ? (int)((l.nfp + l.tcw - 1) / l.tcw) : 0;
                 l.tnrb = (int)((l.nfp + kTcRows - 1) / kTcRows);
@@ -1653,10 +1678,9 @@ struct MGH {
                     // of it in the memory-side cache)
                     tri_product<T, _Float16, float, float>(c, l, l.Z16, 0, (const float*)l.bf, (float*)l.ytmp,
                                                            (T*)nullptr, gate, l.zs2, nr);
-                    const dim3 gc((unsigned)((l.nfp + tc_cols<_Float16>() - 1) / tc_cols<_Float16>()), (unsigned)l.tnrb);
-                    TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_colpart<NR, float>), gc, dim3(256), 0, c.s,
-                                                          (const _Float16*)l.Z16, l.nfp, (const float*)l.ytmp, l.tpart,
-                                                          l.tnrb, gate, 0, (const int32_t*)l.rfirst));
+                    TOPO_NR_SWITCH(nr, (tri_colpart<NR, float, _Float16>(c.s, (const _Float16*)l.Z16, l.nfp,
+                                                                         (const float*)l.ytmp, l.tpart, l.tnrb, gate,
+                                                                         (const int32_t*)l.rfirst, tc_nw)));
                     TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_sum<T, NR, float>), dim3(nblk(4 * l.nfp)), dim3(256),
                                                           0, c.s, (const double*)l.tpart, l.tnrb, l.nfp,
                                                           (float*)nullptr, x, (const int32_t*)l.freeIdx, l.nf, l.ndof,
@@ -1665,10 +1689,9 @@ struct MGH {
                     // Z b by rows, Z^T y by column tiles of the same FP32 copy
                     tri_product<T, float, double, double>(c, l, l.Z32, 0, (const double*)l.bf, l.ytmp, (T*)nullptr,
                                                           gate, nullptr, nr);
-                    const dim3 gc((unsigned)((l.nfp + tc_cols<float>() - 1) / tc_cols<float>()), (unsigned)l.tnrb);
-                    TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_colpart<NR, double, float>), gc, dim3(256), 0, c.s,
-                                                          (const float*)l.Z32, l.nfp, (const double*)l.ytmp, l.tpart,
-                                                          l.tnrb, gate, 0, (const int32_t*)l.rfirst));
+                    TOPO_NR_SWITCH(nr, (tri_colpart<NR, double, float>(c.s, (const float*)l.Z32, l.nfp,
+                                                                        (const double*)l.ytmp, l.tpart, l.tnrb, gate,
+                                                                        (const int32_t*)l.rfirst, tc_nw)));
                     TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_sum<T, NR, double>), dim3(nblk(4 * l.nfp)), dim3(256),
                                                           0, c.s, (const double*)l.tpart, l.tnrb, l.nfp,
                                                           (double*)nullptr, x, (const int32_t*)l.freeIdx, l.nf, l.ndof,
@@ -2180,6 +2203,7 @@ struct Solver {
         if (const char* e = getenv("TOPO_ND_DEPTH")) mg.nd_depth = atoi(e);
         if (const char* e = getenv("TOPO_ND_MIN")) mg.nd_min = atoll(e);
         if (const char* e = getenv("TOPO_ND_LEAF")) mg.nd_leaf = atoll(e);
+        if (const char* e = getenv("TOPO_TC_NW")) mg.tc_nw = atoi(e) == 4 ? 4 : 8;
 
         if (const char* e = getenv("TOPO_MIXED_ETA")) mixed_eta = atof(e);
         const int nlev = cfg.linear_solver == TOPO_SOLVER_MGPCG ? cfg.mg_levels : 1;
