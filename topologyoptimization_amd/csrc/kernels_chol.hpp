@@ -521,7 +521,7 @@ __global__ void __launch_bounds__(64 * NW) k_tri_colpart(const TK* __restrict__ 
     constexpr int VN = 16 / (int)sizeof(TK);   // entries per 16-byte piece
     constexpr int kTcCols = 64 * VN;
     static_assert(NW == 4 || NW == 8, "4 or 8 waves per block");
-    __shared__ float sh[NW][NR][kTcCols];
+    __shared__ float sh[NW][kTcCols];   // the waves' column sums, one right-hand side at a time
     const int cb = blockIdx.x, rb = blockIdx.y;
     const int64_t J0 = (int64_t)cb * kTcCols, I0 = (int64_t)rb * kTcRows;
     bool live[NR];
@@ -567,19 +567,19 @@ __global__ void __launch_bounds__(64 * NW) k_tri_colpart(const TK* __restrict__ 
         }
     }
 #pragma unroll
-    for (int q = 0; q < NR; q++)
-#pragma unroll
-        for (int u = 0; u < VN; u++) sh[w][q][VN * lane + u] = acc[q][u];
-    __syncthreads();
     for (int q = 0; q < NR; q++) {
-        if (!live[q]) continue;
-        for (int t = threadIdx.x; t < kTcCols; t += blockDim.x) {
-            if (J0 + t >= n) continue;
-            double v = ((double)sh[0][q][t] + (double)sh[1][q][t]) + ((double)sh[2][q][t] + (double)sh[3][q][t]);
-            if constexpr (NW == 8)
-                v += ((double)sh[4][q][t] + (double)sh[5][q][t]) + ((double)sh[6][q][t] + (double)sh[7][q][t]);
-            part[((int64_t)q * nrb + rb) * n + J0 + t] = v;
+#pragma unroll
+        for (int u = 0; u < VN; u++) sh[w][VN * lane + u] = acc[q][u];
+        __syncthreads();
+        if (live[q]) {
+            for (int t = threadIdx.x; t < kTcCols; t += blockDim.x) {
+                if (J0 + t >= n) continue;
+                double v = ((double)sh[0][t] + (double)sh[1][t]) + ((double)sh[2][t] + (double)sh[3][t]);
+                if constexpr (NW == 8) v += ((double)sh[4][t] + (double)sh[5][t]) + ((double)sh[6][t] + (double)sh[7][t]);
+                part[((int64_t)q * nrb + rb) * n + J0 + t] = v;
+            }
         }
+        __syncthreads();
     }
 }
 

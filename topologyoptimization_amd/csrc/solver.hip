@@ -462,20 +462,17 @@ static void tri_product(Ctx& c, const MGLevel& l, const TK* M, int upper, const 
     }
 }
 
-// Column-tile product Z^T y of the coarse solve: 8 waves per tile (32 rows
-// each) for up to two right-hand sides, else 4 (the LDS of the wave sums)
+// Column-tile product Z^T y of the coarse solve: nw = 8 waves per tile (32
+// rows each) or 4 (64 rows each), the same for every batch width, so a
+// batched solve gives the bits of one right-hand side at a time
 template <int NR, typename TB, typename TK>
 static void tri_colpart(hipStream_t s, const TK* M, int64_t n, const TB* y, double* part, int nrb, const int* gate,
                         const int32_t* rfirst, int nw) {
     const dim3 gc((unsigned)((n + tc_cols<TK>() - 1) / tc_cols<TK>()), (unsigned)nrb);
-    if constexpr (NR <= 2) {
-        if (nw == 8) {
-            hipLaunchKernelGGL((k_tri_colpart<NR, TB, TK, 8>), gc, dim3(512), 0, s, M, n, y, part, nrb, gate, 0,
-                               rfirst);
-            return;
-        }
-    }
-    hipLaunchKernelGGL((k_tri_colpart<NR, TB, TK, 4>), gc, dim3(256), 0, s, M, n, y, part, nrb, gate, 0, rfirst);
+    if (nw == 8)
+        hipLaunchKernelGGL((k_tri_colpart<NR, TB, TK, 8>), gc, dim3(512), 0, s, M, n, y, part, nrb, gate, 0, rfirst);
+    else
+        hipLaunchKernelGGL((k_tri_colpart<NR, TB, TK, 4>), gc, dim3(256), 0, s, M, n, y, part, nrb, gate, 0, rfirst);
 }
 
 template <typename T> struct Sel;
