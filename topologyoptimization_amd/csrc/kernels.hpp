@@ -660,30 +660,49 @@ __global__ void __launch_bounds__(kRedThreads) k_cgp_update_ur(TU* __restrict__ 
     put_partials<NR>(v, part, 1, NR);
 }
 
-// Jacobi variant (ebePCG :701-719, one right-hand side): alpha from the
-// p.Ap partials; U += alpha p; r -= alpha Ap; z = invD r; partials of
-// ||r||^2 (quantity 1) and r.z (quantity 2)
-template <typename T, typename TU>
-__global__ void __launch_bounds__(kRedThreads) k_cgp_update_jacobi(TU* __restrict__ U, T* __restrict__ r,
-                                                                  T* __restrict__ z, const T* __restrict__ p,
-                                                                  const T* __restrict__ Ap,
-                                                                  const T* __restrict__ invD, int64_t n,
-                                                                  const CGState* st, double* part) {
+
+// Jacobi-PCG (ebePCG assembly/index.ts:701-719, one right-hand side):
+// alpha = rz / (p.Ap + guard) (p.Ap from nb0 partials), r -= alpha Ap, and
+// the ||r||^2 and r.z partials (quantities 1, 2) with z =
+// invD r formed here and not stored (the p update forms it again from the
+// same r and invD); U += alpha p is left to the p update, which reads p.
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads) k_cgp_update_rj(T* __restrict__ r, const T* __restrict__ Ap,
+                                                              const T* __restrict__ invD, int64_t n, const CGState* st,
+                                                              double* part, int nb0) {
     if (st->done) return;
     double pAp[1];
-    sum_partials<1>(part, 0, 1, pAp);
+    sum_partials<1>(part, 0, 1, pAp, nb0);
     const double alpha = st->rz / (pAp[0] + st->guard);
     double v[2] = {0.0, 0.0};
     for (int64_t i = gtid(); i < n; i += gstride()) {
         const T ri = (T)((double)r[i] - alpha * (double)Ap[i]);
         const T zi = invD[i] * ri;
-        U[i] = (TU)((double)U[i] + alpha * (double)p[i]);
         r[i] = ri;
-        z[i] = zi;
         v[0] += (double)ri * (double)ri;
         v[1] += (double)ri * (double)zi;
     }
     put_partials<2>(v, part, 1, 1);
+}
+// Jacobi-PCG p update: U += alpha p (this iteration's alpha), p = z + beta p
+// with z = invD r
+template <typename T, typename TU>
+__global__ void __launch_bounds__(kRedThreads) k_cgp_pupdate_rj(T* __restrict__ p, const T* __restrict__ r,
+                                                               const T* __restrict__ invD, TU* __restrict__ U,
+                                                               int64_t n, const CGState* st, const double* part,
+                                                               int nb0) {
+    if (st->done) return;
+    double rz[1], pAp[1];
+    sum_partials<1>(part, 2, 1, rz);
+    sum_partials<1>(part, 0, 1, pAp, nb0);
+    const T beta = (T)(rz[0] / (st->rz + st->guard));
+    const double alpha = st->rz / (pAp[0] + st->guard);
+    for (int64_t i = gtid(); i < n; i += gstride()) {
+        const T pi = p[i];
+        const T zi = invD[i] * r[i];
+        U[i] = (TU)((double)U[i] + alpha * (double)pi);
+        p[i] = zi + beta * pi;
+    }
 }
 
 // r.z partials (quantity 2) with z = (MASK && fixed ? 0 : zz), zz the

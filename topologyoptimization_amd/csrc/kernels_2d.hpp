@@ -40,13 +40,16 @@ __device__ __forceinline__ void q4_gather(const double* __restrict__ v, int64_t 
 // block of ebePCG assembly/index.ts:654-692). E holds E_e for active
 // elements and 0 for the others (skipped, like elements off the active list).
 // MODE 0: out = K p; 1: out = K p, 0 at fixed DOFs; 2: out = b - K p at free
-// DOFs, 0 at fixed (the warm-start residual :589-619).
+// DOFs, 0 at fixed (the warm-start residual :589-619). With part (MODE 1)
+// each block also leaves its share of p.Ap at part[blockIdx.x] (quantity 0
+// of the CG partials; each thread's nodes in order, then the block tree).
 template <int MODE>
 __global__ void k_q4_apply(const double* __restrict__ p, double* __restrict__ out, const double* __restrict__ E,
                            Grid2 g, const uint8_t* __restrict__ fixed, const double* __restrict__ b, const int* gate,
-                           const double* __restrict__ ke) {
+                           const double* __restrict__ ke, double* __restrict__ part = nullptr) {
 #pragma clang fp contract(off)
     if (gate && *gate) return;
+    double pap[1] = {0.0};
     const int nny = g.nny();
     for (int64_t nd = gtid(); nd < g.nnodes(); nd += gstride()) {
         const int ix = (int)(nd / nny), iy = (int)(nd % nny);
@@ -77,13 +80,16 @@ __global__ void k_q4_apply(const double* __restrict__ p, double* __restrict__ ou
             out[i0] = acc0;
             out[i0 + 1] = acc1;
         } else if (MODE == 1) {
-            out[i0] = fixed[i0] ? 0.0 : acc0;
-            out[i0 + 1] = fixed[i0 + 1] ? 0.0 : acc1;
+            const double o0 = fixed[i0] ? 0.0 : acc0, o1 = fixed[i0 + 1] ? 0.0 : acc1;
+            out[i0] = o0;
+            out[i0 + 1] = o1;
+            pap[0] += p[i0] * o0 + p[i0 + 1] * o1;
         } else {
             out[i0] = fixed[i0] ? 0.0 : b[i0] - acc0;
             out[i0 + 1] = fixed[i0 + 1] ? 0.0 : b[i0 + 1] - acc1;
         }
     }
+    if (MODE == 1 && part) put_partials<1>(pap, part, 0, 1);
 }
 
 // Jacobi diagonal: diag_i = sum over the active elements around the node,

@@ -1811,10 +1811,10 @@ static CGResult pcg_jacobi_op(Ctx& c, Op&& op, const uint8_t* fixed, const T* in
     if (warm) {
         // r = rhs - A U on free DOFs (U in T precision for the operator)
         if (std::is_same<T, TU>::value) {
-            op((const T*)U, r, 2, rhs, (const int*)nullptr);
+            op((const T*)U, r, 2, rhs, (const int*)nullptr, (double*)nullptr);
         } else {
             hipLaunchKernelGGL((k_convert<T, TU>), dim3(nblk(n)), dim3(256), 0, c.s, z, (const TU*)U, n);
-            op((const T*)z, r, 2, rhs, (const int*)nullptr);
+            op((const T*)z, r, 2, rhs, (const int*)nullptr, (double*)nullptr);
         }
         res.applyA++;
     } else {
@@ -1830,17 +1830,24 @@ static CGResult pcg_jacobi_op(Ctx& c, Op&& op, const uint8_t* fixed, const T* in
     c.check_launch();
     const int* gate = &c.st->done;
     const int check = 8;
+    // z = invD r is never stored: the residual update forms it for r.z,
+    // the p update again (same operands, same bits); U += alpha p rides on
+    // the p update, which reads p anyway; the operator may leave the p.Ap
+    // partials itself
     auto body = [&]() {
         for (int k = 0; k < check; k++) {
-            op((const T*)p, Ap, 1, (const T*)nullptr, gate);
-            hipLaunchKernelGGL((k_cgp_pAp<T, 1>), dim3(kCgBlocks), dim3(kRedThreads), 0, c.s, (const T*)p,
-                               (const T*)Ap, n, c.st, c.partials);
-            hipLaunchKernelGGL((k_cgp_update_jacobi<T, TU>), dim3(kCgBlocks), dim3(kRedThreads), 0, c.s, U, r, z,
-                               (const T*)p, (const T*)Ap, invD, n, c.st, c.partials);
-            hipLaunchKernelGGL((k_cgp_pupdate<T, T, 1, false>), dim3(kCgBlocks), dim3(kRedThreads), 0, c.s, p, (const T*)z,
-                               (const uint8_t*)nullptr, n, c.st, (const double*)c.partials);
+            int nb0 = op((const T*)p, Ap, 1, (const T*)nullptr, gate, c.partials);
+            if (nb0 == 0) {
+                hipLaunchKernelGGL((k_cgp_pAp<T, 1>), dim3(kCgBlocks), dim3(kRedThreads), 0, c.s, (const T*)p,
+                                   (const T*)Ap, n, c.st, c.partials);
+                nb0 = kCgBlocks;
+            }
+            hipLaunchKernelGGL((k_cgp_update_rj<T>), dim3(kCgBlocks), dim3(kRedThreads), 0, c.s, r, (const T*)Ap, invD,
+                               n, c.st, c.partials, nb0);
+            hipLaunchKernelGGL((k_cgp_pupdate_rj<T, TU>), dim3(kCgBlocks), dim3(kRedThreads), 0, c.s, p, (const T*)r,
+                               invD, U, n, c.st, (const double*)c.partials, nb0);
             hipLaunchKernelGGL(k_cgp_finish<1>, dim3(1), dim3(kRedThreads), 0, c.s, c.st, (const double*)c.partials,
-                               kCgBlocks);
+                               nb0);
         }
     };
     hipGraphExec_t ex = nullptr;
@@ -1869,8 +1876,10 @@ template <typename T, typename TU>
 static CGResult pcg_jacobi(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVecs& v, double tol, int maxit,
                            volatile int* cancel) {
     MGLevel& l = mg.L[0];
-    auto op = [&](const T* p, T* out, int mode, const T* b, const int* gate) {
+    // (the p.Ap partials of the 3-D operator are left to k_cgp_pAp: 0)
+    auto op = [&](const T* p, T* out, int mode, const T* b, const int* gate, double*) {
         mg.apply<T>(c, 0, p, out, mode, b, gate);
+        return 0;
     };
     return pcg_jacobi_op<T, TU>(c, op, l.fixed, (const T*)Sel<T>::invD(l), l.ndof, rhs, U, warm, v, tol, maxit, cancel,
                                 &mg);
@@ -3401,8 +3410,8 @@ static int ebepcg_2d(const double* dens, const double* ke, const int32_t* edofs,
     // the warm start reads only free DOFs (Uf = U[free], :583-587)
     hipLaunchKernelGGL(k_mask<double>, dim3(nblk(ndof)), dim3(256), 0, C.s, dU, (const uint8_t*)dfixed, (int64_t)ndof);
     C.check_launch();
-    auto op = [&](const double* p, double* out, int mode, const double* b, const int* gate) {
-        q4_apply(C.s, mode, p, out, dE, g, dfixed, b, gate, dke);
+    auto op = [&](const double* p, double* out, int mode, const double* b, const int* gate, double* pap) {
+        return q4_apply(C.s, mode, p, out, dE, g, dfixed, b, gate, dke, pap);
     };
     const int cgMax = maxIter < nfree ? maxIter : nfree;
     CGResult r = pcg_jacobi_op<double, double>(C, op, dfixed, (const double*)dinvD, ndof, (const double*)dF, dU, true,

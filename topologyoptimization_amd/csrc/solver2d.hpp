@@ -7,16 +7,19 @@
 
 namespace topo {
 
-// the <= 2-row apply of the quad4 operator on stream s (modes of k_q4_apply)
-static void q4_apply(hipStream_t s, int mode, const double* p, double* out, const double* E, const Grid2& g,
-                     const uint8_t* fixed, const double* b, const int* gate, const double* ke) {
-    const dim3 grid(nblk(g.nnodes())), blk(256);
+// the <= 2-row apply of the quad4 operator on stream s (modes of k_q4_apply);
+// with pap (mode 1) the launch leaves its p.Ap partials there and returns
+// their count (else 0)
+static int q4_apply(hipStream_t s, int mode, const double* p, double* out, const double* E, const Grid2& g,
+                    const uint8_t* fixed, const double* b, const int* gate, const double* ke, double* pap = nullptr) {
+    const dim3 grid(std::min(nblk(g.nnodes()), kCgStride)), blk(256);
     if (mode == 0)
-        hipLaunchKernelGGL(k_q4_apply<0>, grid, blk, 0, s, p, out, E, g, fixed, b, gate, ke);
+        hipLaunchKernelGGL(k_q4_apply<0>, grid, blk, 0, s, p, out, E, g, fixed, b, gate, ke, (double*)nullptr);
     else if (mode == 1)
-        hipLaunchKernelGGL(k_q4_apply<1>, grid, blk, 0, s, p, out, E, g, fixed, b, gate, ke);
+        hipLaunchKernelGGL(k_q4_apply<1>, grid, blk, 0, s, p, out, E, g, fixed, b, gate, ke, pap);
     else
-        hipLaunchKernelGGL(k_q4_apply<2>, grid, blk, 0, s, p, out, E, g, fixed, b, gate, ke);
+        hipLaunchKernelGGL(k_q4_apply<2>, grid, blk, 0, s, p, out, E, g, fixed, b, gate, ke, (double*)nullptr);
+    return (mode == 1 && pap) ? (int)grid.x : 0;
 }
 
 // Recover (nelx, nely) from a 2-D edofs array and check every entry against
@@ -159,8 +162,8 @@ struct Solver2D {
         hipLaunchKernelGGL(k_q4_invdiag, dim3(nblk(g.nnodes())), dim3(256), 0, C.s, (const double*)E,
                            (const uint8_t*)fixed, invD, g, (const double*)ke);
         C.check_launch();
-        auto op = [&](const double* p, double* out, int mode, const double* b, const int* gate) {
-            q4_apply(C.s, mode, p, out, E, g, fixed, b, gate, ke);
+        auto op = [&](const double* p, double* out, int mode, const double* b, const int* gate, double* pap) {
+            return q4_apply(C.s, mode, p, out, E, g, fixed, b, gate, ke, pap);
         };
         const int maxit = (int)std::min<int64_t>(nfree, cfg.cg_max_iter);
         CGResult r = pcg_jacobi_op<double, double>(C, op, fixed, (const double*)invD, ndof, (const double*)F, U, false,
