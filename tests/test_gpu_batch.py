@@ -35,6 +35,28 @@ def test_batched_equals_sequential(gpu, prec):
         assert out[tile][1] == out[1][1] and out[tile][2] == out[1][2]
 
 
+@pytest.mark.parametrize("prec,cf", [(0, 0), (2, 2)])
+def test_batched_equals_sequential_nested_dissection(gpu, monkeypatch, prec, cf):
+    """The same with the coarsest level in nested-dissection order (forced
+    at this size; the FP16 copy's row-envelope products with 1, 2 and 4
+    right-hand sides): bit-identical to one load case at a time."""
+    T = gpu
+    monkeypatch.setenv("TOPO_ND_MIN", "0")
+    monkeypatch.setenv("TOPO_ND_LEAF", "100")
+    nel, ndof = T.dims(*DIMS)
+    x = P.splitmix64_uniform(nel, lo=0.2)
+    out = {}
+    for tile in (1, 2, 4):
+        cfg, fixed, F, _, _, _ = _bridge(T, precision=prec, rhs_tile=tile, coarse_fp32=cf)
+        with T.Solver(cfg, fixed, F) as S:
+            U, c, info = S.fe(x, 3.0)
+            out[tile] = (U, c, info.cg_iters)
+    assert out[1][2] > 0
+    for tile in (2, 4):
+        assert np.array_equal(out[tile][0], out[1][0]), tile
+        assert out[tile][1] == out[1][1] and out[tile][2] == out[1][2]
+
+
 def test_batched_vs_oracle_and_linearity(gpu, oracle):
     T = gpu
     nel, ndof = T.dims(*DIMS)
