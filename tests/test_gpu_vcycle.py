@@ -121,3 +121,41 @@ def test_nested_dissection_coarse_solve_matches_banded(gpu, monkeypatch, depth, 
     assert info.residual <= 1e-6
     assert abs(c - c0) <= 1e-7 * abs(c0)
     assert abs(info.cg_iters - i0.cg_iters) <= 3
+
+
+@pytest.mark.parametrize("mode", ["banded", "nd"])
+def test_singular_coarse_block_falls_back_to_gauss_jordan(gpu, monkeypatch, capfd, mode):
+    """A void slab (x-elements 24..31 at Emin, left out of the operator by the
+    skip threshold) cuts the cantilever in two: the free part beyond it makes
+    the coarsest block singular, the Cholesky (banded, or in nested-dissection
+    order) meets a non-positive pivot and the setup falls back to the
+    Gauss-Jordan inverse with pivot dropping. The V-cycle stays finite,
+    symmetric and zero at the fixed DOFs."""
+    T = gpu
+    monkeypatch.setenv("TOPO_VERBOSE", "1")
+    if mode == "nd":
+        monkeypatch.setenv("TOPO_ND_MIN", "0")
+        monkeypatch.setenv("TOPO_ND_LEAF", "200")
+    else:
+        monkeypatch.setenv("TOPO_ND_DEPTH", "0")
+    nelx, nely, nelz = DIMS
+    nel, ndof = T.dims(*DIMS)
+    x = np.ones((nelz, nely, nelx), np.float32)
+    x[:, :, 24:32] = 0.0
+    x = x.ravel()
+    rng = np.random.default_rng(9)
+    a = rng.normal(size=ndof)
+    b = rng.normal(size=ndof)
+    cfg, fixed, F, _, _, _ = _cfg(T, 3, 1, 0, prec=T.PREC_FP64)
+    cfg.skip_factor = 1000.0
+    capfd.readouterr()
+    with T.Solver(cfg, fixed, F) as S:
+        S.set_density(x, 3.0)
+        Ma, Mb = S.vcycle(a, T.PREC_FP64), S.vcycle(b, T.PREC_FP64)
+    err = capfd.readouterr().err
+    assert "Gauss-Jordan" in err, err
+    assert np.isfinite(Ma).all() and np.isfinite(Mb).all()
+    assert not Ma[fixed > 0].any()
+    free = fixed == 0
+    lhs, rhs = Ma[free] @ b[free], a[free] @ Mb[free]
+    assert abs(lhs - rhs) <= 1e-9 * np.sqrt(abs(Ma[free] @ a[free]) * abs(Mb[free] @ b[free])), (lhs, rhs)
