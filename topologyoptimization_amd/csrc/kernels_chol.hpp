@@ -14,10 +14,11 @@
 //   Z_II = L_II^-1,  Z_I,0:I = -L_II^-1 sum_{K=max(0,I-wb)}^{I-1} L_IK Z_K,0:I
 // (two DGEMMs per row block on the host side). The coarse solve is then
 // x = Z^T (Z b): two triangular matrix-vector products over Z (row sums by
-// k_tri_part / k_tri_apply, column sums from the same rows by k_tri_colpart
-// for the FP16 copy, or over a stored transpose for the FP32/FP64 copies).
+// k_tri_part / k_tri_apply, column sums from the same rows by
+// k_tri_colpart, for Z and for its FP32 and FP16 copies alike).
 // All of it is O(n w^2 + n^2 w) work instead of the O(n^3) of a dense inverse.
 #pragma once
+#include <type_traits>
 #include "kernels.hpp"
 
 namespace topo {
@@ -280,22 +281,6 @@ __global__ void k_front_assemble(double* __restrict__ F, int64_t f, int64_t m, c
     }
 }
 
-// out = in^T for an n x n row-major matrix (64 x 64 tiles through LDS)
-template <typename TI, typename TO>
-__global__ void __launch_bounds__(256) k_transpose(const TI* __restrict__ in, TO* __restrict__ out, int64_t n) {
-    __shared__ double s[kCB * kCS];
-    const int64_t bi = blockIdx.y, bj = blockIdx.x;
-    for (int e = threadIdx.x; e < kCB * kCB; e += blockDim.x) {
-        const int r = e / kCB, c = e % kCB;
-        s[r * kCS + c] = (double)in[(bi * kCB + r) * n + bj * kCB + c];
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < kCB * kCB; e += blockDim.x) {
-        const int r = e / kCB, c = e % kCB;
-        out[(bj * kCB + r) * n + bi * kCB + c] = (TO)s[c * kCS + r];
-    }
-}
-
 // FP16 copy of a lower-triangular Z with one scale per row: Zq[i][j] =
 // Z[i][j] / s_i, s_i = max_j |Z[i][j]| (1 for a zero row), s2[i] = s_i^2.
 // Z^T (Z b) = Zq^T (S^2 (Zq b)) with the same Zq in both products: the
@@ -509,6 +494,8 @@ __global__ void __launch_bounds__(256) k_tri_part(const TK* __restrict__ M, int6
 // k_tri_sum. rfirst (optional): first nonzero column of each row, the same
 // for the rows of a wave; a wave whose rows are zero over the block's
 // columns does not read them.
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return fma(a, b, c); }
 constexpr int kTcRows = 256;      // rows per block
 constexpr int kTcCols = 512;      // FP16 columns per block (64 lanes x 8)
 template <typename TK> constexpr int tc_cols() { return 64 * (16 / (int)sizeof(TK)); }
@@ -521,7 +508,9 @@ __global__ void __launch_bounds__(64 * NW) k_tri_colpart(const TK* __restrict__ 
     constexpr int VN = 16 / (int)sizeof(TK);   // entries per 16-byte piece
     constexpr int kTcCols = 64 * VN;
     static_assert(NW == 4 || NW == 8, "4 or 8 waves per block");
-    __shared__ float sh[NW][kTcCols];   // the waves' column sums, one right-hand side at a time
+    // FP16 / FP32 entries are summed in FP32 within a tile, FP64 in FP64
+    typedef typename std::conditional<std::is_same<TK, double>::value, double, float>::type TA;
+    __shared__ TA sh[NW][kTcCols];   // the waves' column sums, one right-hand side at a time
     const int cb = blockIdx.x, rb = blockIdx.y;
     const int64_t J0 = (int64_t)cb * kTcCols, I0 = (int64_t)rb * kTcRows;
     bool live[NR];
@@ -538,31 +527,34 @@ __global__ void __launch_bounds__(64 * NW) k_tri_colpart(const TK* __restrict__ 
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int64_t j = J0 + VN * lane;
     const bool colok = j < n;
-    float acc[NR][VN];
+    TA acc[NR][VN];
 #pragma unroll
     for (int q = 0; q < NR; q++)
 #pragma unroll
-        for (int u = 0; u < VN; u++) acc[q][u] = 0.0f;
+        for (int u = 0; u < VN; u++) acc[q][u] = TA(0);
     const int64_t r0 = I0 + (int64_t)w * (kTcRows / NW);
     const int64_t r1 = r0 + kTcRows / NW < n ? r0 + kTcRows / NW : n;
     const bool wz = rfirst && r0 < n && (int64_t)rfirst[r0] > J0 + kTcCols - 1;   // wave-uniform
     if (colok && !wz) {
 #pragma unroll 8
         for (int64_t i = r0; i < r1; i++) {
-            float av[VN];
+            TA av[VN];
             if constexpr (VN == 8) {
                 const half8_t a = *(const half8_t*)(M + i * n + j);
 #pragma unroll
                 for (int u = 0; u < 8; u++) av[u] = (float)a[u];
-            } else {
+            } else if constexpr (VN == 4) {
                 const float4 a = *(const float4*)(M + i * n + j);
                 av[0] = a.x; av[1] = a.y; av[2] = a.z; av[3] = a.w;
+            } else {
+                const double2 a = *(const double2*)(M + i * n + j);
+                av[0] = a.x; av[1] = a.y;
             }
 #pragma unroll
             for (int q = 0; q < NR; q++) {
-                const float yv = (float)y[q * n + i];
+                const TA yv = (TA)y[q * n + i];
 #pragma unroll
-                for (int u = 0; u < VN; u++) acc[q][u] = fmaf(av[u], yv, acc[q][u]);
+                for (int u = 0; u < VN; u++) acc[q][u] = fma_t(av[u], yv, acc[q][u]);
             }
         }
     }

@@ -315,10 +315,9 @@ struct MGLevel {
     int32_t* freeIdx = nullptr;
     int32_t* freeOf = nullptr;     // DOF -> free index or -1
     double* bf = nullptr;          // gathered right-hand side (padded)
-    double* Kinv = nullptr;        // banded factor L, then Z^T (triangular mode); or the inverse
+    double* Kinv = nullptr;        // the free-DOF block (banded: its factor L in place), or the Gauss-Jordan inverse
     float* Kinv32 = nullptr;       // FP32 rounding of the Gauss-Jordan inverse (coarse_fp32)
     bool tri = false;              // coarse solve by x = Z^T (Z b), Z = L^-1 (else Kinv b)
-    bool zt = false;               // Kinv holds Z^T (the FP64 products' second operand)
     int bandw = 0;                 // max |i - j| of the free block's nonzeros
     double* Z = nullptr;           // L^-1 (lower triangular)
     float* Z32 = nullptr;
@@ -1266,28 +1265,17 @@ struct MGH {
                          l.tri ? (l.nd ? "nested-dissection Cholesky" : "banded Cholesky")
                                : "Gauss-Jordan inverse (factorization met a non-positive pivot)");
     }
-    // what the coarse solves read besides Z: Z^T in the Kinv buffer (FP64
-    // products), the FP16 copy with its row scales or the FP32 copies
+    // the rounded copies of Z the FP32 V-cycle reads (FP16 with its row
+    // scales, or FP32); both triangular products read the same matrix, the
+    // second by column tiles, so no transposed copy is kept
     void z_copies(Ctx& c, MGLevel& l) {
         const int64_t n = l.nfp;
-        // Z^T now when the V-cycles of this solver read it (FP64, or the FP32
-        // copy made from it), else on the first FP64 coarse solve
-        l.zt = false;
-        if (!want32 || (!l.Z16 && !l.Z32)) transpose_z(c, l);
         if (l.Z16)
             hipLaunchKernelGGL(k_rowscale_f16, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, c.s, (const double*)l.Z, n,
                                l.Z16, l.zs2, (const int32_t*)l.rfirst);
         if (l.Z32)
             hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(n * n)), dim3(256), 0, c.s, l.Z32,
                                (const double*)l.Z, n * n);
-    }
-
-    void transpose_z(Ctx& c, MGLevel& l) {
-        const int64_t n = l.nfp;
-        const int NB = (int)(n / kCB);
-        hipLaunchKernelGGL((k_transpose<double, double>), dim3(NB, NB), dim3(256), 0, c.s, (const double*)l.Z, l.Kinv,
-                           n);
-        l.zt = true;
     }
 
     // fronts, gather maps, descriptors and row starts of the
@@ -1697,19 +1685,16 @@ struct MGH {
                                                           (double*)nullptr, x, (const int32_t*)l.freeIdx, l.nf, l.ndof,
                                                           gate, (const double*)nullptr));
                 } else {
-                    if (!l.zt) {
-                        // an FP64 coarse solve in a solver whose V-cycles are
-                        // FP16: Z^T made once, never inside a captured graph
-                        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-                        TOPO_HIP(hipStreamIsCapturing(c.s, &cs));
-                        TOPO_CHECK(cs == hipStreamCaptureStatusNone, TOPO_E_STATE,
-                                   "FP64 coarse solve first reached under graph capture");
-                        transpose_z(c, l);
-                    }
+                    // FP64: Z b by rows, Z^T y by column tiles of Z (FP64 sums)
                     tri_product<T, double, double, double>(c, l, l.Z, 0, (const double*)l.bf, l.ytmp, (T*)nullptr,
                                                            gate, nullptr, nr);
-                    tri_product<T, double, double, double>(c, l, l.Kinv, 1, (const double*)l.ytmp, (double*)nullptr,
-                                                           x, gate, nullptr, nr);
+                    TOPO_NR_SWITCH(nr, (tri_colpart<NR, double, double>(c.s, (const double*)l.Z, l.nfp,
+                                                                         (const double*)l.ytmp, l.tpart, l.tnrb, gate,
+                                                                         (const int32_t*)l.rfirst, tc_nw)));
+                    TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_sum<T, NR, double>), dim3(nblk(4 * l.nfp)), dim3(256),
+                                                          0, c.s, (const double*)l.tpart, l.tnrb, l.nfp,
+                                                          (double*)nullptr, x, (const int32_t*)l.freeIdx, l.nf, l.ndof,
+                                                          gate, (const double*)nullptr));
                 }
                 return x;
             }
