@@ -48,8 +48,13 @@ def make_problem(T, P, workload, grid, prec, levels, **kw):
     return cfg, fixed, F, pres, void, x0
 
 
-def run_solver(T, P, grid, prec, steps, warmup, levels, plain=False, workload="c3", coarse_fp32=None, **kw):
-    """plain: the reference's plain method on the same problem -- Jacobi-PCG
+def run_solver(T, P, grid, prec, steps, warmup, levels, plain=False, workload="c3", coarse_fp32=None,
+               record=False, **kw):
+    """Warm-up steps, then `steps` timed SIMP steps. Each step is timed on
+    its own (it ends with a stream synchronisation: the host reads the
+    step's scalars); with `record` the design before each timed step is
+    copied out between the timed regions, for the matched-design legs.
+    plain: the reference's plain method on the same problem -- Jacobi-PCG
     in FP64, with the 400-iteration cap lifted so it reaches the same
     tolerance (the compliance is then comparable)."""
     cfg, fixed, F, pres, void, x0 = make_problem(T, P, workload, grid, prec, levels, **kw)
@@ -65,21 +70,64 @@ def run_solver(T, P, grid, prec, steps, warmup, levels, plain=False, workload="c
         cfg.cg_max_iter = 100000
     S = T.Solver(cfg, fixed, F)
     S.set_design(x0)
-    infos = []
+    infos, designs, times = [], [], []
     for i in range(warmup):
         t0 = time.perf_counter()
         info = S.step()
-        log("warmup %d prec=%d: %.1f ms cg=%d outer=%d c=%.6g" % (i, prec, (time.perf_counter() - t0) * 1e3,
-                                                                  info.cg_iters, info.outer_iters, info.compliance))
-    t0 = time.perf_counter()
+        log("warmup %d prec=%d: %.1f ms cg=%d outer=%d c=%.8g vol=%.6f change=%.4g ocp=%d" % (
+            i, prec, (time.perf_counter() - t0) * 1e3, info.cg_iters, info.outer_iters, info.compliance,
+            info.volume, info.change, info.oc_passes))
     for i in range(steps):
+        if record:
+            designs.append(S.design())
+        t0 = time.perf_counter()
         info = S.step()
+        times.append(time.perf_counter() - t0)
         infos.append(info.as_dict())
-        log("step %d prec=%d: %.1f ms (setup %.1f solve %.1f post %.1f) cg=%d outer=%d vcyc=%d c=%.8g" % (
-            i, prec, info.t_total_ms, info.t_setup_ms, info.t_solve_ms, info.t_post_ms, info.cg_iters,
-            info.outer_iters, info.vcycles, info.compliance))
-    t1 = time.perf_counter()
-    return S, infos, (t1 - t0) / steps
+        log_step("step %d prec=%d" % (i, prec), info, times[-1])
+    return S, infos, float(np.mean(times)) if times else 0.0, designs
+
+
+def log_step(tag, info, dt):
+    log("%s: %.1f ms (setup %.1f solve %.1f post %.1f) cg=%d outer=%d vcyc=%d c=%.8g vol=%.6f change=%.4g "
+        "ocp=%d" % (tag, dt * 1e3, info.t_setup_ms, info.t_solve_ms, info.t_post_ms, info.cg_iters,
+                    info.outer_iters, info.vcycles, info.compliance, info.volume, info.change, info.oc_passes))
+
+
+def matched_leg(T, P, grid, prec, designs, first_iter, levels, plain=False, nsteps=None, workload="c3", **kw):
+    """The same SIMP iterations as the timed mixed steps, from the same
+    designs: each recorded design is handed over (set_design, iteration
+    counter) and one step is timed on it. The FP64 MG-PCG is cold-started in
+    every solve (as the mixed one); the plain Jacobi-PCG warm-starts from the
+    previous handed design's displacements (the reference's PCG warm start),
+    the first from zero. An untimed step on the first design first builds the
+    hierarchy, tunes and captures the graphs."""
+    cfg, fixed, F, _, _, _ = make_problem(T, P, workload, grid, prec, levels, **kw)
+    if workload == "c3":
+        cfg.mg_omega_cap = 1.6
+    cfg.max_iterations = 1000
+    cfg.coarse_fp32 = 2
+    if plain:
+        cfg.linear_solver = T.SOLVER_PCG
+        cfg.precision = T.PREC_FP64
+        cfg.cg_max_iter = 100000
+    S = T.Solver(cfg, fixed, F)
+    n = len(designs) if nsteps is None else min(nsteps, len(designs))
+    if not plain:
+        S.set_design(designs[0])
+        S.set_iteration(first_iter)
+        S.step()
+    infos, times = [], []
+    for k in range(n):
+        S.set_design(designs[k])
+        S.set_iteration(first_iter + k)
+        t0 = time.perf_counter()
+        info = S.step()
+        times.append(time.perf_counter() - t0)
+        infos.append(info.as_dict())
+        log_step("matched %s step %d" % ("plain Jacobi-PCG" if plain else "prec=%d" % prec, k), info, times[-1])
+    S.close()
+    return infos, float(np.mean(times))
 
 
 def main():
@@ -92,10 +140,12 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--workload", choices=("c3", "c4", "c5", "2d"), default="c3",
-                    help="c3: 192x96x96 cantilever (headline); c4: 128x64x64 bridge, 4 load cases, batched; "
-                         "c5: 160x80x80 force inverter, forward + adjoint solves; 2d: the 2-D SIMP leg alone "
-                         "(1024x512 quad4 cantilever)")
+    ap.add_argument("--plain-steps", type=int, default=1,
+                    help="timed designs the plain FP64 Jacobi-PCG leg solves (each takes seconds at 5.4M DOF)")
+    ap.add_argument("--workload", choices=("c2", "c3", "c4", "c5", "2d"), default="c3",
+                    help="c3: 192x96x96 cantilever (headline); c2: 96x48x48 cantilever, Jacobi-PCG FP64 and mixed; "
+                         "c4: 128x64x64 bridge, 4 load cases, batched; c5: 160x80x80 force inverter, forward + "
+                         "adjoint solves; 2d: the 2-D SIMP leg alone (1024x512 quad4 cantilever)")
     ap.add_argument("--coarse-fp32", type=int, default=None, choices=(0, 1, 2),
                     help="copy of the FP64 coarse factor inverse read by the FP32 V-cycle: 0 none (FP64), "
                          "1 FP32, 2 FP16 with one scale per row (default: the problem's own, FP16 for the "
@@ -117,6 +167,8 @@ def main():
     if args.workload == "2d":
         print(json.dumps(simp2d_leg(T, steps=args.steps, warmup=args.warmup, cpu=not args.no_cpu)))
         return
+    if args.workload == "c2":
+        return c2_workload(T, P, args)
     if args.workload != "c3":
         return other_workload(T, P, args)
     grid = tuple(int(v) for v in args.grid.split(","))
@@ -127,32 +179,33 @@ def main():
 
     # mixed precision (the headline)
     mixed_prec = T.PREC_MIXED_PC if args.mixed == "pc" else T.PREC_MIXED
-    S, infos, t_mixed = run_solver(T, P, grid, mixed_prec, args.steps, args.warmup, args.levels,
-                                   coarse_fp32=args.coarse_fp32)
-    # compliance of the mixed path must match the FP64 path (checked below)
-    # dominant kernel: fine-level FP32 applyA, timed with HIP events on the solver's stream
+    S, infos, t_mixed, designs = run_solver(T, P, grid, mixed_prec, args.steps, args.warmup, args.levels,
+                                            coarse_fp32=args.coarse_fp32, record=True)
+    x_end = S.design()
+    nonfinite = int((~np.isfinite(x_end)).sum())
+    # dominant kernels, timed with HIP events on the solver's stream
     tR, tnt, tcx, tlog = S.tuning()
     ms32, by32 = S.time_op(T.TIME_APPLYA, T.PREC_MIXED, 0, 50)
     ms64, by64 = S.time_op(T.TIME_APPLYA, T.PREC_FP64, 0, 30)
     # the CG's FP64 operator (fixed DOFs masked, p.Ap partials in the epilogue)
-    ms64m, _ = S.time_op(T.TIME_APPLYA_MASKED, T.PREC_FP64, 0, 30)
+    ms64m, by64m = S.time_op(T.TIME_APPLYA_MASKED, T.PREC_FP64, 0, 30)
     msc, byc = S.time_op(T.TIME_COPY, T.PREC_FP64, 0, 30)
     msv, _ = S.time_op(T.TIME_VCYCLE, T.PREC_MIXED, 0, 5)
     S.close()
-    res = {"mixed_ms": t_mixed * 1e3, "mixed_infos": infos}
-    t_fp64 = None
+    # the same designs through FP64 MG-PCG and through the plain FP64 method
+    infos64, t_fp64 = (None, None)
     if not args.no_fp64:
-        S64, infos64, t_fp64 = run_solver(T, P, grid, T.PREC_FP64, args.steps, args.warmup, args.levels)
-        S64.close()
-    t_plain = None
-    if not args.no_plain:
-        Sp, infos_p, t_plain = run_solver(T, P, grid, T.PREC_FP64, 1, args.warmup, args.levels, plain=True)
-        Sp.close()
+        infos64, t_fp64 = matched_leg(T, P, grid, T.PREC_FP64, designs, args.warmup, args.levels)
+    infos_p, t_plain = (None, None)
+    if not args.no_plain and args.plain_steps > 0:
+        infos_p, t_plain = matched_leg(T, P, grid, T.PREC_FP64, designs, args.warmup, args.levels, plain=True,
+                                       nsteps=args.plain_steps)
     value = 1.0 / t_mixed
-    ach = by32 / (ms32 * 1e-3) / 1e9
+    ach = by64m / (ms64m * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": ach, "peak": 8000.0, "unit": "GB/s", "frac": ach / 8000.0,
-            "traffic": measured_traffic(grid), "kernel": "k_applyA_xm<float> (fine level)",
-            "algorithmic_bytes": by32, "avg_ms": ms32}
+            "traffic": measured_traffic(grid, "applyA_fp64_masked"),
+            "kernel": "k_applyA_xm<double,double,512,1> (fine level, masked, p.Ap partials: the CG's operator)",
+            "algorithmic_bytes": by64m, "avg_ms": ms64m}
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(grid, infos, args.levels)
@@ -174,15 +227,27 @@ def main():
         "config": {"workload": "cantilever %dx%dx%d hex8, %d DOF, %d-level MG-PCG V(2,2) omega 0.6 (capped), FP64 "
                                "direct coarsest, tol 1e-6, x0~U[0.05,1] seed %d" % (grid + (ndof, args.levels, P.SEED)),
                    "nel": nel, "ndof": ndof, "parallelism": "single GPU"},
-        "fp64_iters_per_s": (1.0 / t_fp64) if t_fp64 else None,
-        "mixed_vs_fp64": (t_fp64 / t_mixed) if t_fp64 else None,
-        "compliance_rel_vs_fp64": _rel(infos, infos64) if t_fp64 else None,
-        "plain_fp64_pcg_iters_per_s": (1.0 / t_plain) if t_plain else None,
-        "plain_fp64_pcg_cg_iters": infos_p[0]["cg_iters"] if t_plain else None,
-        "mixed_vs_plain_fp64_pcg": (t_plain / t_mixed) if t_plain else None,
-        "compliance_rel_vs_plain": _rel(infos, infos_p) if t_plain else None,
-        "gdof_matvec_per_s": {"fp32": ndof / (ms32 * 1e-3) / 1e9, "fp64": ndof / (ms64 * 1e-3) / 1e9},
-        "applyA_ms": {"fp32": ms32, "fp64": ms64, "fp64_masked_pap": ms64m}, "vcycle_ms_fp32": msv,
+        "trajectory": [{k: round(i[k], 9) if isinstance(i[k], float) else i[k]
+                        for k in ("iteration", "compliance", "volume", "change", "oc_passes", "cg_iters",
+                                  "t_setup_ms", "t_solve_ms", "t_post_ms")} for i in infos],
+        "nonfinite_design_entries": nonfinite,
+        "matched_designs": {
+            "fp64_mgpcg_iters_per_s": (1.0 / t_fp64) if t_fp64 else None,
+            "mixed_vs_fp64_mgpcg": (t_fp64 / t_mixed) if t_fp64 else None,
+            "compliance_rel_vs_fp64_mgpcg": _rel(infos, infos64) if infos64 else None,
+            "fp64_mgpcg_cg_iters": [i["cg_iters"] for i in infos64] if infos64 else None,
+            "plain_fp64_pcg_iters_per_s": (1.0 / t_plain) if t_plain else None,
+            "mixed_vs_plain_fp64_pcg": (t_plain / t_mixed) if t_plain else None,
+            "compliance_rel_vs_plain_fp64_pcg": _rel(infos, infos_p) if infos_p else None,
+            "plain_fp64_pcg_cg_iters": [i["cg_iters"] for i in infos_p] if infos_p else None,
+            "plain_fp64_pcg_residual": [i["residual"] for i in infos_p] if infos_p else None,
+        },
+        "gdof_matvec_per_s": {"fp32": ndof / (ms32 * 1e-3) / 1e9, "fp64": ndof / (ms64 * 1e-3) / 1e9,
+                              "fp64_masked_pap": ndof / (ms64m * 1e-3) / 1e9},
+        "applyA_ms": {"fp32": ms32, "fp64": ms64, "fp64_masked_pap": ms64m},
+        "applyA_hbm_frac": {"fp32": by32 / (ms32 * 1e-3) / 8e12, "fp64": by64 / (ms64 * 1e-3) / 8e12,
+                            "fp64_masked_pap": by64m / (ms64m * 1e-3) / 8e12},
+        "vcycle_ms_fp32": msv,
         "copy_GBps": byc / (msc * 1e-3) / 1e9,
         "fine_applyA_per_step": fine_calls,
         "compliance": infos[-1]["compliance"],
@@ -192,6 +257,72 @@ def main():
         "autotune_candidates": [[int(a), int(b), int(c), float(d)] for a, b, c, d in tlog],
         "simp2d": simp2d,
         "device": name,
+    }
+    print(json.dumps(out))
+
+
+def c2_workload(T, P, args):
+    """The 96x48x48 cantilever (698,691 DOF) with the reference's plain
+    solver, Jacobi-PCG, tol 1e-6, in FP64 and in mixed precision (FP64
+    iterative refinement around an FP32 Jacobi-PCG), from x = volfrac, 20
+    SIMP iterations by default (--steps). The FP64 leg is timed on the mixed
+    leg's designs (matched compliance)."""
+    n, name = T.device_info()
+    assert n > 0, "no HIP device"
+    log("device:", name)
+    grid = (96, 48, 48)
+    nel, ndof = T.dims(*grid)
+    steps = args.steps if args.steps != 3 else 20
+
+    def make(prec):
+        cfg, fixed, F, _, _, x0 = P.cantilever_c2(precision=prec)
+        cfg.max_iterations = 1000
+        cfg.cg_max_iter = 20000
+        return cfg, fixed, F, x0
+
+    cfg, fixed, F, x0 = make(T.PREC_MIXED)
+    S = T.Solver(cfg, fixed, F)
+    S.set_design(x0)
+    designs, infos, times = [], [], []
+    for i in range(args.warmup):
+        S.step()
+    for i in range(steps):
+        designs.append(S.design())
+        t0 = time.perf_counter()
+        info = S.step()
+        times.append(time.perf_counter() - t0)
+        infos.append(info.as_dict())
+        log_step("c2 mixed step %d" % i, info, times[-1])
+    ms32, by32 = S.time_op(T.TIME_APPLYA, T.PREC_MIXED, 0, 50)
+    S.close()
+    cfg, fixed, F, x0 = make(T.PREC_FP64)
+    S = T.Solver(cfg, fixed, F)
+    infos64, times64 = [], []
+    for k, x in enumerate(designs):
+        S.set_design(x)
+        S.set_iteration(args.warmup + k)
+        t0 = time.perf_counter()
+        info = S.step()
+        times64.append(time.perf_counter() - t0)
+        infos64.append(info.as_dict())
+        log_step("c2 fp64 step %d" % k, info, times64[-1])
+    S.close()
+    t_m, t_64 = float(np.mean(times)), float(np.mean(times64))
+    ach = by32 / (ms32 * 1e-3) / 1e9
+    out = {
+        "metric": "SIMP iters/s, 96x48x48 cantilever, Jacobi-PCG mixed vs FP64", "value": 1.0 / t_m,
+        "unit": "SIMP iters/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup, "ms_per_step": t_m * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp64", "data": "synthetic",
+        "config": {"workload": "cantilever 96x48x48 hex8, %d DOF, Jacobi-PCG tol 1e-6, x0 = volfrac 0.12" % ndof,
+                   "nel": nel, "ndof": ndof, "parallelism": "single GPU"},
+        "fp64_iters_per_s": 1.0 / t_64, "mixed_vs_fp64": t_64 / t_m,
+        "compliance_rel_vs_fp64": _rel(infos, infos64),
+        "cg_iters": {"mixed": [i["cg_iters"] for i in infos], "fp64": [i["cg_iters"] for i in infos64]},
+        "trajectory": [{k: i[k] for k in ("iteration", "compliance", "volume", "change", "oc_passes")} for i in infos],
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": 8000.0, "unit": "GB/s", "frac": ach / 8000.0,
+                     "traffic": None, "kernel": "k_applyA_xm<float> (fine level)", "algorithmic_bytes": by32,
+                     "avg_ms": ms32},
+        "cpu_baseline": None if args.no_cpu else cpu_baseline_c2(infos), "device": name,
     }
     print(json.dumps(out))
 
@@ -206,13 +337,13 @@ def other_workload(T, P, args):
     mixed_prec = T.PREC_MIXED_PC if args.mixed == "pc" else T.PREC_MIXED
     if args.workload == "c4":
         grid = (128, 64, 64)
-        S, infos, t_b = run_solver(T, P, grid, mixed_prec, args.steps, args.warmup, args.levels, workload="c4",
-                                   rhs_tile=0, coarse_fp32=args.coarse_fp32)
+        S, infos, t_b, _ = run_solver(T, P, grid, mixed_prec, args.steps, args.warmup, args.levels, workload="c4",
+                                      rhs_tile=0, coarse_fp32=args.coarse_fp32)
         R, nt, cx, tlog = S.tuning()
         ms32, by32 = S.time_op(T.TIME_APPLYA, T.PREC_MIXED, 0, 50)
         S.close()
-        S1, infos1, t_s = run_solver(T, P, grid, mixed_prec, args.steps, args.warmup, args.levels, workload="c4",
-                                     rhs_tile=1, coarse_fp32=args.coarse_fp32)
+        S1, infos1, t_s, _ = run_solver(T, P, grid, mixed_prec, args.steps, args.warmup, args.levels, workload="c4",
+                                        rhs_tile=1, coarse_fp32=args.coarse_fp32)
         S1.close()
         extra = {"batched_vs_one_at_a_time": t_s / t_b, "one_at_a_time_iters_per_s": 1.0 / t_s,
                  "compliance_rel_vs_one_at_a_time": _rel(infos, infos1),
@@ -223,8 +354,8 @@ def other_workload(T, P, args):
         metric = "SIMP iters/s, 4-load bridge, batched MG-PCG"
     else:
         grid = (160, 80, 80)
-        S, infos, t_b = run_solver(T, P, grid, mixed_prec, args.steps, args.warmup, args.levels, workload="c5",
-                                   cg_max_iter=2000, coarse_fp32=args.coarse_fp32)
+        S, infos, t_b, _ = run_solver(T, P, grid, mixed_prec, args.steps, args.warmup, args.levels, workload="c5",
+                                      cg_max_iter=2000, coarse_fp32=args.coarse_fp32)
         ms32, by32 = S.time_op(T.TIME_APPLYA, T.PREC_MIXED, 0, 50)
         S.close()
         extra = {}
@@ -257,14 +388,14 @@ def _rel(a, b):
     return max(abs(a[i]["compliance"] - b[i]["compliance"]) / abs(b[i]["compliance"]) for i in range(n))
 
 
-def measured_traffic(grid):
-    """HBM bytes per fine FP32 applyA launch from the committed rocprofv3
-    counter runs (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
-    calibration), or None when no summary exists for this grid."""
+def measured_traffic(grid, kernel="applyA_fp32"):
+    """HBM bytes per launch of a fine-level applyA variant from the committed
+    rocprofv3 counter runs (FETCH_SIZE + WRITE_SIZE, scaled by the
+    copy-kernel calibration), or None when no summary exists for it."""
     path = os.path.join(ROOT, "profiles", "applyA_pmc_%dx%dx%d.json" % grid)
     try:
         with open(path) as f:
-            return json.load(f)["applyA_fp32"]["hbm_bytes_per_launch"]
+            return json.load(f)[kernel]["hbm_bytes_per_launch"]
     except (OSError, KeyError, ValueError):
         return None
 
@@ -308,6 +439,30 @@ def cpu_baseline(grid, infos, levels):
                           (nx, ny, nz, cpu_levels, t_setup, t_vc, t_ap, ndof / t_ap / 1e9, n_cg),
                 "gdof_matvec_per_s": ndof / t_ap / 1e9,
                 "host": platform.processor() or platform.machine(), "nproc": os.cpu_count()}
+    except Exception as e:  # reported, never fatal
+        return {"value": None, "error": repr(e)}
+
+
+def cpu_baseline_c2(infos):
+    """CPU baseline of the C2 workload: the oracle's ebePCG (the reference's
+    Jacobi-PCG, FP64, one thread) at 96x48x48 for 20 CG iterations from the
+    first design, scaled by the GPU steps' mean CG iteration count."""
+    try:
+        from oracle import oracle as O
+        import topologyoptimization_amd.problems as P
+        cfg, fixed, F, _, _, x0 = P.cantilever_c2()
+        nx, ny, nz = 96, 48, 48
+        nel, ndof = O.dims(nx, ny, nz)
+        free = np.where(fixed == 0)[0].astype(np.int32)
+        U = np.zeros(ndof)
+        t0 = time.perf_counter()
+        it = O.ebePCG(x0.astype(np.float64), O.k0(0.3), nx, ny, nz, F.astype(np.float64), U, free, 1e-9, 1.0, 3.0, 20,
+                      1e-30)
+        t_it = (time.perf_counter() - t0) / max(1, it)
+        n_cg = float(np.mean([i["cg_iters"] for i in infos]))
+        return {"value": 1.0 / (n_cg * t_it), "unit": "SIMP iters/s", "cores": 1, "kind": "port",
+                "sample": "oracle ebePCG at 96x48x48, %d CG iterations (%.1f ms each), scaled by %.0f CG iterations "
+                          "per step (energies, filter and OC not counted)" % (it, t_it * 1e3, n_cg)}
     except Exception as e:  # reported, never fatal
         return {"value": None, "error": repr(e)}
 

@@ -115,7 +115,13 @@ typedef struct topo_config {
                                    inverse read by FP32 V-cycles: 0 none (FP64),
                                    1 FP32 (half the bytes), 2 FP16 with one scale per
                                    row (a quarter; the coarse operator stays SPD)  */
-    int32_t reserved[7];
+    int32_t energy_clamp;       /* 0 (reference): element energies u_e^T K0 u_e as
+                                   summed, so a rounding-negative value gives dc > 0 and
+                                   a NaN design entry in OC (sqrt of a negative), which
+                                   makes `change` NaN and ends the loop; 1: energies
+                                   below 0 are set to 0 (K0 is positive semidefinite, so
+                                   a negative sum is rounding)                        */
+    int32_t reserved[6];
 } topo_config;
 
 /* Fill *cfg with the reference defaults (js/optimizer-worker-3d.js:20-64,

@@ -833,6 +833,18 @@ int or_mgpcg(void* h, const double* F, double* U, int maxIter, double tol, int w
     return maxIter;
 }
 
+// JS Math.max(a, b): NaN if either operand is NaN (the change loop
+// js/optimizer-worker-3d.js:2244-2247 and js/optimizer-worker.js:704-706)
+static double js_math_max(double a, double b) {
+    if (std::isnan(a) || std::isnan(b)) return std::nan("");
+    return a > b ? a : b;
+}
+
+// energy clamp (topo_config.energy_clamp, no reference counterpart): a
+// negative sum of the positive semidefinite form is rounding
+static int g_energy_clamp = 0;
+void or_set_energy_clamp(int on) { g_energy_clamp = on; }
+
 // ---------------------------------------------------------------------------
 // element energies js/optimizer-worker-3d.js:2163-2173 and
 // _computeElementEnergyFlat :1561-1571 (f64 sum, Float32 store)
@@ -846,6 +858,7 @@ int or_energy(int nelx, int nely, int nelz, const double* ke, const double* U, f
         double en = 0;
         for (int i = 0; i < 24; i++)
             for (int j = 0; j < 24; j++) en += Ue[i] * ke[i * 24 + j] * Ue[j];
+        if (g_energy_clamp && en < 0.0) en = 0.0;
         w[e] = (float)en;
     }
     return 0;
@@ -1010,6 +1023,7 @@ int or_simp_run(const topo_config* cfg, const uint8_t* fixed, const float* F32, 
         or_mg_set_omega_cap(mg, cfg->mg_omega_cap);
         or_mg_set_skip(mg, cfg->skip_factor > 0 ? cfg->Emin * cfg->skip_factor : -1.0);
     }
+    or_set_energy_clamp(cfg->energy_clamp);
     const double penalTarget = cfg->penal, penalStart = cfg->penal_start;
     const int contIters = cfg->continuation_iters;
     const bool proj = cfg->use_projection != 0;
@@ -1074,13 +1088,14 @@ int or_simp_run(const topo_config* cfg, const uint8_t* fixed, const float* F32, 
         }
         or_oc(nel, x.data(), cfg->volfrac, dcn.data(), pres, voidm, cfg->move, xnew.data());
         change = 0;
-        for (int64_t i = 0; i < nel; i++) change = std::max(change, (double)std::fabs(xnew[i] - xold[i]));
+        for (int64_t i = 0; i < nel; i++) change = js_math_max(change, (double)std::fabs(xnew[i] - xold[i]));
         std::swap(x, xnew);
         if (comp_out) comp_out[loop - 1] = c;
         if (change_out) change_out[loop - 1] = change;
         if (cg_out) cg_out[loop - 1] = its;
     }
     or_set_pcg_skip_factor(1000.0);
+    or_set_energy_clamp(0);
     std::copy(x.begin(), x.end(), x_io);
     if (U_out) std::copy(U.begin(), U.end(), U_out);
     if (mg) or_mg_destroy(mg);
@@ -1846,7 +1861,7 @@ int or2_simp_run(const topo_config* cfg, const uint8_t* fixed, const float* F, c
         or2_sens_filter(nelx, nely, cfg->rmin, x.data(), dc.data(), dcn.data());
         or_oc(nel, x.data(), cfg->volfrac, dcn.data(), pres, voidm, cfg->move, xnew.data());
         change = 0;
-        for (int64_t e = 0; e < nel; e++) change = std::max(change, (double)std::fabs((double)xnew[e] - (double)xold[e]));
+        for (int64_t e = 0; e < nel; e++) change = js_math_max(change, (double)std::fabs((double)xnew[e] - (double)xold[e]));
         x = xnew;
         double smax = 0;
         for (int64_t e = 0; e < nel; e++) {

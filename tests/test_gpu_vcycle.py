@@ -159,3 +159,49 @@ def test_singular_coarse_block_falls_back_to_gauss_jordan(gpu, monkeypatch, capf
     free = fixed == 0
     lhs, rhs = Ma[free] @ b[free], a[free] @ Mb[free]
     assert abs(lhs - rhs) <= 1e-9 * np.sqrt(abs(Ma[free] @ a[free]) * abs(Mb[free] @ b[free])), (lhs, rhs)
+
+
+@pytest.mark.parametrize("mode", ["banded", "nd"])
+def test_graph_replay_follows_coarse_fallback(gpu, monkeypatch, capfd, mode):
+    """Designs that alternate between a solid block (the coarsest block is
+    factored by Cholesky: triangular products in the V-cycle) and the same
+    block cut by a void slab (singular coarsest block: Gauss-Jordan inverse)
+    on one solver. The captured CG graphs must follow the coarse-solve branch
+    of each design: with graphs the solves are bit-identical to the same
+    sequence launched kernel by kernel. The load sits on the clamped side of
+    the slab, so the part beyond it stays at zero and every solve converges."""
+    T = gpu
+    monkeypatch.setenv("TOPO_VERBOSE", "1")
+    if mode == "nd":
+        monkeypatch.setenv("TOPO_ND_MIN", "0")
+        monkeypatch.setenv("TOPO_ND_LEAF", "200")
+    else:
+        monkeypatch.setenv("TOPO_ND_DEPTH", "0")
+    nelx, nely, nelz = DIMS
+    nel, ndof = T.dims(*DIMS)
+    solid = np.ones(nel, np.float32)
+    slab = np.ones((nelz, nely, nelx), np.float32)
+    slab[:, :, 24:32] = 0.0
+    slab = slab.ravel()
+    F = np.zeros(ndof, np.float32)
+    F[3 * P.node_index(nely, nelz, 16, nely // 2, nelz) + 2] = -1.0
+    runs = {}
+    for graphs in (1, 0):
+        cfg, fixed, _, _, _, _ = _cfg(T, 3, 1, 0, prec=T.PREC_FP64)
+        cfg.skip_factor = 1000.0
+        cfg.cg_tol = 1e-6
+        cfg.use_graphs = graphs
+        out = []
+        capfd.readouterr()
+        with T.Solver(cfg, fixed, F) as S:
+            for x in (solid, slab, solid, slab):
+                U, c, info = S.fe(x, 3.0)
+                print("graphs %d: c %.15e, %d CG iterations" % (graphs, c, info.cg_iters), flush=True)
+                assert np.isfinite(U).all() and info.residual <= 1e-5
+                out.append((U, c, info.cg_iters))
+        err = capfd.readouterr().err
+        assert err.count("Gauss-Jordan") == 2 and err.count("Cholesky") == 2, err
+        runs[graphs] = out
+    for (Ug, cg, ig), (U0, c0, i0) in zip(runs[1], runs[0]):
+        assert ig == i0 and cg == c0
+        assert np.array_equal(Ug, U0)

@@ -1,0 +1,63 @@
+"""SIMP loop cases shared by the oracle fixture script
+(tests/golden/make_trajectories.py) and the GPU tests
+(tests/test_gpu_loops.py). build(name, side) returns (cfg, fixed, F,
+preserved, void, x0, iterations); side "oracle" gives the FP64 configuration
+the oracle runs, side "gpu" the configuration the GPU test runs (the same
+problem; precision and coarse-solve options may differ, as each case says).
+"""
+import numpy as np
+
+import topologyoptimization_amd as T
+import topologyoptimization_amd.problems as P
+
+CASES = ("c3_recipe_32", "c3_recipe_48_3lev", "reference_default_16", "c2_recipe_24")
+
+
+def build(name, side):
+    if name == "c3_recipe_32":
+        # the 192x96x96 benchmark recipe (U[0.05, 1] start, 4-level MG-PCG,
+        # omega 0.6 capped, FP64 Cholesky coarsest, every element kept,
+        # energies clamped) on a 32x16x16 grid, 10 iterations: volume
+        # reaches 0.12 in the third
+        nx, ny, nz = 32, 16, 16
+        cfg, fixed, F, pres, void, _ = P.cantilever(nx, ny, nz, linear_solver=T.SOLVER_MGPCG, mg_levels=4, mg_omega=0.6,
+                                                    coarse_solver=T.COARSE_CHOLESKY, galerkin_max_ndof=0,
+                                                    precision=T.PREC_FP64 if side == "oracle" else T.PREC_MIXED_PC,
+                                                    coarse_fp32=2, max_iterations=1000)
+        return cfg, fixed, F, pres, void, P.splitmix64_uniform(T.dims(nx, ny, nz)[0]), 10
+    if name == "c3_recipe_48_3lev":
+        # the same recipe at 48x24x24 with 3 levels: the coarsest block
+        # (12x6x6, 1,764 free DOFs) is where the GPU test forces the
+        # nested-dissection factorization and its FP16 / FP32 copies
+        nx, ny, nz = 48, 24, 24
+        cfg, fixed, F, pres, void, _ = P.cantilever(nx, ny, nz, linear_solver=T.SOLVER_MGPCG, mg_levels=3, mg_omega=0.6,
+                                                    coarse_solver=T.COARSE_CHOLESKY, galerkin_max_ndof=0,
+                                                    precision=T.PREC_FP64 if side == "oracle" else T.PREC_MIXED_PC,
+                                                    max_iterations=1000)
+        return cfg, fixed, F, pres, void, P.splitmix64_uniform(T.dims(nx, ny, nz)[0]), 6
+    if name == "reference_default_16":
+        # the reference's default 3-D loop (js/optimizer-worker-3d.js:2074-2252
+        # with the defaults of :20-64 and :2056-2072): Heaviside projection
+        # with beta doubling every 5 iterations, penalty 1.5 -> 3 over 20
+        # iterations, CG tolerance schedule 1e-3 -> 1e-8 over maxIterations
+        # 15, elements at or below 1000 Emin left out of the operator, MG-PCG
+        # with the reference level rule (16x8x8 -> 8x4x4 -> 4x2x2), omega
+        # 0.5, dense Galerkin levels (<= 3000 DOF), 30 coarsest sweeps, cold
+        # start; 'left' supports, 'down' load, volfrac 0.3, x0 = volfrac
+        nx, ny, nz = 16, 8, 8
+        cfg = T.default_config(nx, ny, nz, max_iterations=15, continuation_iters=20)
+        fixed = T.fixed_dofs(nx, ny, nz, T.FIX_LEFT, T.DOF_ALL)
+        F = T.load_vector(nx, ny, nz, T.LOAD_DOWN, 1.0)
+        pres = T.preserved_elements(nx, ny, nz, T.FIX_LEFT, T.LOAD_DOWN)
+        x0 = np.full(T.dims(nx, ny, nz)[0], np.float32(cfg.volfrac), np.float32)
+        x0[pres > 0] = 1.0
+        return cfg, fixed, F, pres, None, x0, 15
+    if name == "c2_recipe_24":
+        # the 96x48x48 Jacobi-PCG recipe (x0 = volfrac 0.12, tol 1e-6) on a
+        # 24x12x12 grid, 6 iterations; the GPU runs it in mixed precision
+        # (FP64 refinement around FP32 Jacobi-PCG)
+        cfg, fixed, F, pres, void, x0 = P.cantilever(24, 12, 12, linear_solver=T.SOLVER_PCG,
+                                                     precision=T.PREC_FP64 if side == "oracle" else T.PREC_MIXED,
+                                                     cg_max_iter=20000, max_iterations=1000)
+        return cfg, fixed, F, pres, void, x0, 6
+    raise KeyError(name)

@@ -53,7 +53,8 @@ class Config(C.Structure):
         ("rhs_tile", C.c_int32), ("use_graphs", C.c_int32), ("skip_factor", C.c_double), ("exact_apply", C.c_int32),
         ("mg_omega_cap", C.c_double),
         ("coarse_fp32", C.c_int32),
-        ("reserved", C.c_int32 * 7),
+        ("energy_clamp", C.c_int32),
+        ("reserved", C.c_int32 * 6),
     ]
 
     def copy(self):
@@ -367,6 +368,10 @@ class Solver:
         info = StepInfo()
         _check(lib().topo_solver_step(self.h, C.byref(info)))
         return info
+
+    def set_iteration(self, loop):
+        """iterations already done (the next step() is iteration loop + 1)"""
+        _check(lib().topo_solver_set_iteration(self.h, loop))
 
     def field(self, which):
         if which == FIELD_U:

@@ -1235,10 +1235,13 @@ __global__ void k_heaviside(const float* __restrict__ x, float* __restrict__ xp,
 // compliance sensitivity dc (:2175-2192):
 //   dc = -p xPhys^(p-1) E0 w * (proj ? beta (1 - tanh^2(beta(xd - eta))) / (2 tanh(beta/2)) : 1)
 // where xd is the projected field's argument (x, or the density-filtered x).
+// clamp = 1 sets energies below 0 to 0 (u^T K0 u >= 0 for the positive
+// semidefinite K0, so a negative sum is rounding of a near-rigid element
+// motion); clamp = 0 keeps the reference's value.
 __global__ void k_energy_sens(const double* __restrict__ U, const float* __restrict__ xphys, const float* __restrict__ xd,
                               double penal, double E0, double Emin, int use_proj, double beta, double eta, Grid g,
                               float* __restrict__ w, float* __restrict__ stress, float* __restrict__ dc,
-                              const double* __restrict__ ke) {
+                              const double* __restrict__ ke, int clamp) {
 #pragma clang fp contract(off)
     const int nny = g.nny(), nnz = g.nnz();
     const int64_t nyz = (int64_t)nny * nnz;
@@ -1260,6 +1263,7 @@ __global__ void k_energy_sens(const double* __restrict__ U, const float* __restr
         double en = 0;
         for (int i = 0; i < 24; i++)
             for (int j = 0; j < 24; j++) en += u[i] * ke[i * 24 + j] * u[j];
+        if (clamp && en < 0.0) en = 0.0;
         const float wf = (float)en;
         w[e] = wf;
         const double xp = (double)xphys[e];
@@ -1460,18 +1464,24 @@ __global__ void k_oc_final(const float* __restrict__ x, const float* __restrict_
         xnew[i] = oc_xnew_any(mech, x[i], dc[i], lm, move, pres ? pres[i] : 0, vd ? vd[i] : 0);
 }
 
-// change = max |xnew - xold|, volume = sum xnew
+// mode 0: change = max |xnew - xold| with the NaN rule of JS Math.max (a NaN
+// entry makes the result NaN, js/optimizer-worker-3d.js:2244-2247), volume =
+// sum xnew; mode 1 (b unused): max(0, max_e a_e) with the comparison
+// `a_e > m` of the max-stress loop (:2180), which passes NaN over.
+__device__ __forceinline__ double chg_max(int mode, double m, double v) {
+    if (mode == 0) return js_max(m, v);
+    return v > m ? v : m;
+}
 __global__ void __launch_bounds__(kRedThreads) k_change(const float* __restrict__ a, const float* __restrict__ b, int64_t n,
-                                                       double* out2, double* partials, unsigned* ticket) {
+                                                       double* out2, double* partials, unsigned* ticket, int mode) {
     __shared__ double sh[kRedThreads / 64];
-    __shared__ bool last;
     double m = 0.0, s = 0.0;
     for (int64_t i = gtid(); i < n; i += gstride()) {
-        const float bv = b ? b[i] : 0.0f;
-        m = fmax(m, (double)fabsf(a[i] - bv));
+        const double v = mode == 0 ? (double)fabsf(a[i] - b[i]) : (double)a[i];
+        m = chg_max(mode, m, v);
         s += (double)a[i];
     }
-    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+    for (int off = 32; off > 0; off >>= 1) m = chg_max(mode, m, __shfl_down(m, off, 64));
     double vs[1] = {s};
     double tot[1];
     // max via a second small reduction in shared memory
@@ -1480,15 +1490,14 @@ __global__ void __launch_bounds__(kRedThreads) k_change(const float* __restrict_
     __syncthreads();
     if (threadIdx.x == 0) {
         double mm = 0;
-        for (int w = 0; w < kRedThreads / 64; w++) mm = fmax(mm, sh[w]);
+        for (int w = 0; w < kRedThreads / 64; w++) mm = chg_max(mode, mm, sh[w]);
         partials[2 * kRedBlocks + blockIdx.x] = mm;
     }
     __syncthreads();
-    (void)last;
     if (grid_reduce<1>(vs, partials, ticket, tot)) {
         out2[1] = tot[0];
         double mm = 0;
-        for (unsigned bI = 0; bI < gridDim.x; bI++) mm = fmax(mm, partials[2 * kRedBlocks + bI]);
+        for (unsigned bI = 0; bI < gridDim.x; bI++) mm = chg_max(mode, mm, partials[2 * kRedBlocks + bI]);
         out2[0] = mm;
     }
 }

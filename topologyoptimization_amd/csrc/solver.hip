@@ -45,6 +45,13 @@ static inline int nblk(int64_t n, int bs = 256) {
 
 constexpr int kMaxRHS = 4;   // right-hand sides one batched CG launch carries
 
+// TOPO_USE_GRAPHS=0 turns graph capture off for every solver of the process
+// (the CG iterations are then launched kernel by kernel; same results)
+static bool graphs_disabled_by_env() {
+    const char* e = std::getenv("TOPO_USE_GRAPHS");
+    return e && std::atoi(e) == 0;
+}
+
 struct Ctx {
     hipStream_t s = nullptr;
     int device = 0;
@@ -189,6 +196,43 @@ struct Ctx {
         return h_sc[0];
     }
 };
+
+// OC bisection (OC js/optimizer-worker-3d.js:3066-3101) on the device: each
+// (k_oc_plan, k_oc_eval) pair takes up to kOcDepth bisection steps, and once
+// the reference's stop test holds every later pair is a no-op, so a batch of
+// pairs runs without host round trips; the host reads the state once per
+// batch. A typical update takes 40-55 steps (one batch of 12 pairs); a NaN in
+// the sensitivities (the volume test then never passes) walks lambda down to
+// underflow in ~1100 steps, as the reference loop does. xnew is the last
+// executed step's, and *vol the volume sum that step compared.
+static int oc_bisect(Ctx& C, const float* x, const float* dcn, const uint8_t* pres, const uint8_t* voidm, double move,
+                     int64_t nel, double volfrac, int mech, float* xnew, double* vol) {
+    OCState h;
+    std::memset(&h, 0, sizeof h);
+    h.l1 = 0;
+    h.l2 = 1e9;
+    h.target = volfrac * (double)nel;
+    *C.h_oc = h;
+    TOPO_HIP(hipMemcpyAsync(C.oc, C.h_oc, sizeof(OCState), hipMemcpyHostToDevice, C.s));
+    constexpr int kPairsPerBatch = 12;
+    for (int batch = 0; batch < 40; batch++) {
+        for (int k = 0; k < kPairsPerBatch; k++) {
+            hipLaunchKernelGGL(k_oc_plan, dim3(1), dim3(64), 0, C.s, C.oc);
+            hipLaunchKernelGGL(k_oc_eval, dim3(kRedBlocks), dim3(kRedThreads), 0, C.s, x, dcn, pres, voidm, move, nel,
+                               C.oc, C.partials, C.ticket, mech);
+        }
+        C.check_launch();
+        TOPO_HIP(hipMemcpyAsync(C.h_oc, C.oc, sizeof(OCState), hipMemcpyDeviceToHost, C.s));
+        C.sync();
+        if (C.h_oc->finished) break;
+    }
+    TOPO_CHECK(C.h_oc->finished, TOPO_E_BREAKDOWN, "OC bisection did not terminate");
+    hipLaunchKernelGGL(k_oc_final, dim3(nblk(nel)), dim3(256), 0, C.s, x, dcn, pres, voidm, move, nel,
+                       (const double*)&C.oc->lmid_last, xnew, mech);
+    C.check_launch();
+    if (vol) *vol = C.h_oc->vol_last;
+    return C.h_oc->passes;
+}
 
 // Block coefficients of K0 in the Hadamard basis (see kernels_xm.hpp):
 // B = H K0 H^T / 64 with modes q = (qx,qy,qz), index 3*(4qx + 2qy + qz) + d.
@@ -886,8 +930,37 @@ struct MGH {
     uint8_t* bddc_corner = nullptr;      // primal (corner) DOF flags
     int32_t *bddc_sx = nullptr, *bddc_sy = nullptr, *bddc_sz = nullptr;   // element -> subdomain, per axis
     int bddc_ns[3] = {0, 0, 0};
+    // Jacobi-PCG iterations per captured graph between host polls (32
+    // measured 5 % faster than 8 on the 2-D leg)
+    int jcg_check = 32;
     template <typename T> const T* ke() const;
     template <typename T> HadCoef<T> had() const;
+
+    // Every host-side choice a captured CG graph bakes in besides its buffer
+    // pointers: the coarse-solve branch per level (triangular products or the
+    // Gauss-Jordan inverse; FP16/FP32 copies present), the dense/E-average
+    // operator kind, the autotuned element tiles, the kernel family and the
+    // iterations per graph. Appended to the graph-cache key, so a design
+    // update that changes one of them (a coarsest block that is no longer
+    // positive definite falls back to Gauss-Jordan) captures a new graph
+    // instead of replaying the old one.
+    void graph_sig(std::vector<const void*>& key) const {
+        auto put = [&](int64_t v) { key.push_back((const void*)(intptr_t)v); };
+        put((int64_t)L.size());
+        for (size_t li = 0; li < L.size(); li++) {
+            const MGLevel& l = L[li];
+            put((l.tri ? 1 : 0) | (l.dense ? 2 : 0) | (l.chol ? 4 : 0) | (l.nd ? 8 : 0) | (l.Z16 ? 16 : 0) |
+                (l.Z32 ? 32 : 0) | (l.Kinv32 ? 64 : 0));
+            put(li < lv_nt.size() ? lv_nt[li] : -1);
+            put(li < lv_cx.size() ? lv_cx[li] : -1);
+        }
+        put((exact ? 1 : 0) | (rz_fuse ? 2 : 0) | (bddc ? 4 : 0));
+        put(xm_nt);
+        put(xm_cx);
+        put(cg_check);
+        put(jcg_check);
+        put(coarse_fp32);
+    }
 
     // build levels: factor-2 coarsening of MGPrecond3D.ensure
     // (js/optimizer-worker-3d.js:408-442); nlevels <= 0 = reference rule
@@ -1789,8 +1862,8 @@ static void cg_set(Ctx& c, double tol, int maxit, double guard = 1e-12, int nr =
 // of ||r|| < tol ||r0||. key identifies the operator for the graph cache.
 template <typename T, typename TU, typename Op>
 static CGResult pcg_jacobi_op(Ctx& c, Op&& op, const uint8_t* fixed, const T* invD, int64_t n, const T* rhs, TU* U,
-                              bool warm, CGVecs& v, double tol, int maxit, volatile int* cancel, const void* key,
-                              int abs_tol = 0) {
+                              bool warm, CGVecs& v, double tol, int maxit, volatile int* cancel,
+                              std::vector<const void*> key, int jcg_check, int abs_tol = 0) {
     T* r = (T*)v.r;
     T* z = (T*)v.z;
     T* p = (T*)v.p;
@@ -1818,11 +1891,8 @@ static CGResult pcg_jacobi_op(Ctx& c, Op&& op, const uint8_t* fixed, const T* in
     c.check_launch();
     const int* gate = &c.st->done;
     // iterations per captured graph between host polls (converged ones are
-    // gated no-ops): 32 measured 5 % faster than 8 on the 2-D leg
-    static const int check = [] {
-        const char* e = std::getenv("TOPO_JCG_CHECK");
-        return e ? std::max(1, std::min(64, std::atoi(e))) : 32;
-    }();
+    // gated no-ops)
+    const int check = std::max(1, std::min(64, jcg_check));
     // z = invD r is never stored: the residual update forms it for r.z,
     // the p update again (same operands, same bits); U += alpha p rides on
     // the p update, which reads p anyway; the operator may leave the p.Ap
@@ -1845,7 +1915,10 @@ static CGResult pcg_jacobi_op(Ctx& c, Op&& op, const uint8_t* fixed, const T* in
     };
     hipGraphExec_t ex = nullptr;
     if (c.use_graphs)
-        ex = c.graph_capture({(const void*)&pcg_jacobi_op<T, TU, Op>, U, r, z, p, Ap, rhs, key}, body);
+    {
+        key.insert(key.begin(), {(const void*)&pcg_jacobi_op<T, TU, Op>, U, r, z, p, Ap, rhs});
+        ex = c.graph_capture(key, body);
+    }
     for (;;) {
         const CGState& h = c.read_st();
         if (h.done) {
@@ -1874,8 +1947,10 @@ static CGResult pcg_jacobi(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVe
         mg.apply<T>(c, 0, p, out, mode, b, gate);
         return 0;
     };
+    std::vector<const void*> key{&mg};
+    mg.graph_sig(key);
     return pcg_jacobi_op<T, TU>(c, op, l.fixed, (const T*)Sel<T>::invD(l), l.ndof, rhs, U, warm, v, tol, maxit, cancel,
-                                &mg);
+                                key, mg.jcg_check);
 }
 
 // The V-cycle part of precondition() without the widening copy: returns the
@@ -2001,8 +2076,12 @@ static CGResult pcg_mg(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVecs& 
     };
     hipGraphExec_t ex = nullptr;
     if (c.use_graphs)
-        ex = c.graph_capture({(const void*)&pcg_mg<T, TU, PT>, U, r, z, p, Ap, rhs, &mg, (const void*)(intptr_t)nr},
-                             body);
+    {
+        std::vector<const void*> key{(const void*)&pcg_mg<T, TU, PT>, U, r, z, p, Ap, rhs, &mg,
+                                     (const void*)(intptr_t)nr, w32.r};
+        mg.graph_sig(key);
+        ex = c.graph_capture(key, body);
+    }
     for (;;) {
         c.read_st(nr);
         bool done = true;
@@ -2093,7 +2172,7 @@ struct Solver {
         ndof = g.ndof();
         c.reset(new Ctx());
         Ctx& C = *c;
-        C.use_graphs = cfg.use_graphs != 0;
+        C.use_graphs = cfg.use_graphs != 0 && !graphs_disabled_by_env();
         {
             double hke[576];
             k0_matrix(cfg.nu, hke);
@@ -2204,6 +2283,7 @@ struct Solver {
         if (const char* e = getenv("TOPO_ND_LEAF")) mg.nd_leaf = atoll(e);
         if (const char* e = getenv("TOPO_TC_NW")) mg.tc_nw = atoi(e) == 4 ? 4 : 8;
         if (const char* e = getenv("TOPO_CG_CHECK")) mg.cg_check = std::max(1, std::min(16, atoi(e)));
+        if (const char* e = getenv("TOPO_JCG_CHECK")) mg.jcg_check = std::max(1, std::min(64, atoi(e)));
 
         if (const char* e = getenv("TOPO_MIXED_ETA")) mixed_eta = atof(e);
         const int nlev = cfg.linear_solver == TOPO_SOLVER_MGPCG ? cfg.mg_levels : 1;
@@ -2712,31 +2792,7 @@ struct Solver {
     }
 
     int oc_run(const float* dx, const float* ddcn, float* dxnew, double volfrac, double* vol, int mech = 0) {
-        Ctx& C = *c;
-        OCState h;
-        std::memset(&h, 0, sizeof h);
-        h.l1 = 0;
-        h.l2 = 1e9;
-        h.target = volfrac * (double)nel;
-        *C.h_oc = h;
-        TOPO_HIP(hipMemcpyAsync(C.oc, C.h_oc, sizeof(OCState), hipMemcpyHostToDevice, C.s));
-        for (int guard = 0; guard < 100; guard++) {
-            for (int k = 0; k < 3; k++) {
-                hipLaunchKernelGGL(k_oc_plan, dim3(1), dim3(64), 0, C.s, C.oc);
-                hipLaunchKernelGGL(k_oc_eval, dim3(kRedBlocks), dim3(kRedThreads), 0, C.s, dx, ddcn, pres, voidm,
-                                   cfg.move, nel, C.oc, C.partials, C.ticket, mech);
-            }
-            C.check_launch();
-            TOPO_HIP(hipMemcpyAsync(C.h_oc, C.oc, sizeof(OCState), hipMemcpyDeviceToHost, C.s));
-            C.sync();
-            if (C.h_oc->finished) break;
-        }
-        TOPO_CHECK(C.h_oc->finished, TOPO_E_BREAKDOWN, "OC bisection did not terminate");
-        hipLaunchKernelGGL(k_oc_final, dim3(nblk(nel)), dim3(256), 0, C.s, dx, ddcn, pres, voidm, cfg.move, nel,
-                           &C.oc->lmid_last, dxnew, mech);
-        C.check_launch();
-        if (vol) *vol = C.h_oc->vol_last;
-        return C.h_oc->passes;
+        return oc_bisect(*c, dx, ddcn, pres, voidm, cfg.move, nel, volfrac, mech, dxnew, vol);
     }
 
     void filter_fwd(int type, const float* dxd, const float* din, float* dout) {
@@ -2778,7 +2834,7 @@ struct Solver {
         double vol = 0;
         loc.oc_passes = oc_run(x, dcn, xnew, cfg.volfrac, &vol, cfg.objective == TOPO_OBJ_MECHANISM ? 1 : 0);
         hipLaunchKernelGGL(k_change, dim3(kRedBlocks), dim3(kRedThreads), 0, C.s, xnew, xold, nel, C.dsc + 4, C.partials,
-                           C.ticket);
+                           C.ticket, 0);
         C.check_launch();
         TOPO_HIP(hipMemcpyAsync(C.h_sc, C.dsc, 8 * sizeof(double), hipMemcpyDeviceToHost, C.s));
         C.sync();
@@ -2844,7 +2900,7 @@ struct Solver {
         for (int k = 0; k < cfg.nload; k++) {
             hipLaunchKernelGGL(k_energy_sens, dim3(nblk(nel)), dim3(256), 0, C.s, U + (int64_t)k * ndof, xp, xd, penal,
                                cfg.E0, cfg.Emin, (int)proj, beta, cfg.eta, g, w, stress, k == 0 ? dc : tmpf,
-                               (const double*)K0.ke);
+                               (const double*)K0.ke, cfg.energy_clamp);
             if (k > 0) hipLaunchKernelGGL(k_add_f, dim3(nblk(nel)), dim3(256), 0, C.s, dc, (const float*)tmpf, nel);
         }
         C.check_launch();
@@ -2858,7 +2914,7 @@ struct Solver {
         double vol = 0;
         loc.oc_passes = oc_run(x, dcn, xnew, cfg.volfrac, &vol);
         hipLaunchKernelGGL(k_change, dim3(kRedBlocks), dim3(kRedThreads), 0, C.s, xnew, xold, nel, C.dsc + 4, C.partials,
-                           C.ticket);
+                           C.ticket, 0);
         C.check_launch();
         TOPO_HIP(hipMemcpyAsync(C.h_sc, C.dsc, 8 * sizeof(double), hipMemcpyDeviceToHost, C.s));
         C.sync();
@@ -2867,7 +2923,7 @@ struct Solver {
         std::swap(x, xnew);
         // max stress
         hipLaunchKernelGGL(k_change, dim3(kRedBlocks), dim3(kRedThreads), 0, C.s, stress, (const float*)nullptr, nel,
-                           C.dsc + 6, C.partials, C.ticket);
+                           C.dsc + 6, C.partials, C.ticket, 1);
         TOPO_HIP(hipMemcpyAsync(C.h_sc, C.dsc, 8 * sizeof(double), hipMemcpyDeviceToHost, C.s));
         C.sync();
         auto t1 = std::chrono::steady_clock::now();
@@ -3152,7 +3208,8 @@ int topo_op_energy(topo_solver* h, const double* Uh, float* wout) {
     double* Ud = (double*)S.v64.z;
     TOPO_HIP(hipMemcpyAsync(Ud, Uh, S.ndof * sizeof(double), hipMemcpyHostToDevice, C.s));
     hipLaunchKernelGGL(k_energy_sens, dim3(nblk(S.nel)), dim3(256), 0, C.s, Ud, S.xphys, S.xphys, 3.0, S.cfg.E0,
-                       S.cfg.Emin, 0, 1.0, 0.5, S.g, S.w, (float*)nullptr, S.dc, (const double*)S.K0.ke);
+                       S.cfg.Emin, 0, 1.0, 0.5, S.g, S.w, (float*)nullptr, S.dc, (const double*)S.K0.ke,
+                       S.cfg.energy_clamp);
     S.get_field_f(S.w, wout);
     return TOPO_OK;
     TOPO_API_END
@@ -3240,7 +3297,8 @@ int topo_time_op(topo_solver* h, int32_t op, int32_t prec, int32_t level, int32_
             break;
         case TOPO_TIME_ENERGY:
             hipLaunchKernelGGL(k_energy_sens, dim3(nblk(S.nel)), dim3(256), 0, C.s, S.U, S.xphys, S.xphys, 3.0, S.cfg.E0,
-                               S.cfg.Emin, 0, 1.0, 0.5, S.g, S.w, S.stress, S.dc, (const double*)S.K0.ke);
+                               S.cfg.Emin, 0, 1.0, 0.5, S.g, S.w, S.stress, S.dc, (const double*)S.K0.ke,
+                               S.cfg.energy_clamp);
             break;
         case TOPO_TIME_FILTER:
             TOPO_CHECK(S.cfg.filter_type != TOPO_FILTER_HELMHOLTZ, TOPO_E_ARG, "no timing op for the Helmholtz filter");
@@ -3409,7 +3467,9 @@ static int ebepcg_2d(const double* dens, const double* ke, const int32_t* edofs,
     };
     const int cgMax = maxIter < nfree ? maxIter : nfree;
     CGResult r = pcg_jacobi_op<double, double>(C, op, dfixed, (const double*)dinvD, ndof, (const double*)dF, dU, true,
-                                               v, tolerance, cgMax, nullptr, &T);
+                                               v, tolerance, cgMax, nullptr,
+                                               {&T, dE, dinvD, dfixed, dke, (const void*)(intptr_t)g.nelx,
+                                                (const void*)(intptr_t)g.nely}, 32);
     TOPO_HIP(hipMemcpyAsync(Uh, dU, ndof * sizeof(double), hipMemcpyDeviceToHost, C.s));
     C.sync();
     for (void* q : tmp) C.free_one(q);

@@ -58,6 +58,7 @@ struct Solver2D {
     double change = 1.0;
     double penal_mesh = 3.0;       // this.penal of the worker (stress metric)
     volatile int cancel_flag = 0;
+    int jcg_check = 32;            // Jacobi-PCG iterations per captured graph between host polls
 
     void init(const topo_config* cf, const uint8_t* fixedMask, const float* Fh, const uint8_t* preserved,
               const uint8_t* vm) {
@@ -77,7 +78,8 @@ struct Solver2D {
         TOPO_CHECK(ndof < INT32_MAX, TOPO_E_ARG, "2-D grid too large (%lld DOFs)", (long long)ndof);
         c.reset(new Ctx());
         Ctx& C = *c;
-        C.use_graphs = cfg.use_graphs != 0;
+        C.use_graphs = cfg.use_graphs != 0 && !graphs_disabled_by_env();
+        if (const char* e = std::getenv("TOPO_JCG_CHECK")) jcg_check = std::max(1, std::min(64, std::atoi(e)));
         double keh[64];
         TOPO_CHECK(topo_k0_2d(cfg.nu, keh) == TOPO_OK, TOPO_E_ARG, "Poisson ratio %g", cfg.nu);
         ke = C.alloc<double>(64);
@@ -167,7 +169,7 @@ struct Solver2D {
         };
         const int maxit = (int)std::min<int64_t>(nfree, cfg.cg_max_iter);
         CGResult r = pcg_jacobi_op<double, double>(C, op, fixed, (const double*)invD, ndof, (const double*)F, U, false,
-                                                   v, tol, maxit, &cancel_flag, this, 1);
+                                                   v, tol, maxit, &cancel_flag, {this, E, invD}, jcg_check, 1);
         TOPO_CHECK(!r.nonfinite, TOPO_E_BREAKDOWN, "non-finite CG scalar");
         hipLaunchKernelGGL((k_dot<float, double>), dim3(kRedBlocks), dim3(kRedThreads), 0, C.s, (const float*)F32,
                            (const double*)U, ndof, C.dsc, C.partials, C.ticket);
@@ -178,40 +180,15 @@ struct Solver2D {
 
     // OC :1445-1492 with move 0.2 (the shared bisection kernels)
     int oc(double volfrac, double* vol) {
-        Ctx& C = *c;
-        OCState h;
-        std::memset(&h, 0, sizeof h);
-        h.l1 = 0;
-        h.l2 = 1e9;
-        h.target = volfrac * (double)nel;
-        *C.h_oc = h;
-        TOPO_HIP(hipMemcpyAsync(C.oc, C.h_oc, sizeof(OCState), hipMemcpyHostToDevice, C.s));
-        for (int guard = 0; guard < 100; guard++) {
-            for (int k = 0; k < 3; k++) {
-                hipLaunchKernelGGL(k_oc_plan, dim3(1), dim3(64), 0, C.s, C.oc);
-                hipLaunchKernelGGL(k_oc_eval, dim3(kRedBlocks), dim3(kRedThreads), 0, C.s, (const float*)x,
-                                   (const float*)dcn, (const uint8_t*)pres, (const uint8_t*)voidm, cfg.move, nel, C.oc,
-                                   C.partials, C.ticket, 0);
-            }
-            C.check_launch();
-            TOPO_HIP(hipMemcpyAsync(C.h_oc, C.oc, sizeof(OCState), hipMemcpyDeviceToHost, C.s));
-            C.sync();
-            if (C.h_oc->finished) break;
-        }
-        TOPO_CHECK(C.h_oc->finished, TOPO_E_BREAKDOWN, "OC bisection did not terminate");
-        hipLaunchKernelGGL(k_oc_final, dim3(nblk(nel)), dim3(256), 0, C.s, (const float*)x, (const float*)dcn,
-                           (const uint8_t*)pres, (const uint8_t*)voidm, cfg.move, nel, (const double*)&C.oc->lmid_last,
-                           xnew, 0);
-        C.check_launch();
-        if (vol) *vol = C.h_oc->vol_last;
-        return C.h_oc->passes;
+        return oc_bisect(*c, (const float*)x, (const float*)dcn, (const uint8_t*)pres, (const uint8_t*)voidm, cfg.move,
+                         nel, volfrac, 0, xnew, vol);
     }
 
     // max over elements of a nonnegative field (k_change against no array)
     double field_max(const float* f) {
         Ctx& C = *c;
         hipLaunchKernelGGL(k_change, dim3(kRedBlocks), dim3(kRedThreads), 0, C.s, f, (const float*)nullptr, nel,
-                           C.dsc + 4, C.partials, C.ticket);
+                           C.dsc + 4, C.partials, C.ticket, 1);
         C.check_launch();
         return C.read_d(C.dsc + 4);
     }
@@ -248,7 +225,7 @@ struct Solver2D {
         double vol = 0;
         const int passes = oc(cfg.volfrac, &vol);
         hipLaunchKernelGGL(k_change, dim3(kRedBlocks), dim3(kRedThreads), 0, C.s, (const float*)xnew, (const float*)xold,
-                           nel, C.dsc + 4, C.partials, C.ticket);
+                           nel, C.dsc + 4, C.partials, C.ticket, 0);
         C.check_launch();
         TOPO_HIP(hipMemcpyAsync(C.h_sc, C.dsc, 8 * sizeof(double), hipMemcpyDeviceToHost, C.s));
         C.sync();

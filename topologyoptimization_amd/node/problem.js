@@ -158,6 +158,7 @@ function toTopoConfig(addon, model, config) {
     if (num(config.move)) c.move = config.move;
     if (num(config.coarseFp32)) c.coarse_fp32 = config.coarseFp32;
     if (num(config.rhsTile)) c.rhs_tile = config.rhsTile;
+    if (config.energyClamp != null) c.energy_clamp = config.energyClamp ? 1 : 0;
     return c;
 }
 

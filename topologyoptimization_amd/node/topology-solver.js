@@ -1,202 +1,240 @@
 'use strict';
 /**
- * TopologySolver -- the reference's library class (lib/topology-solver.js:132-518)
- * over the MI355X path: same constructor, methods, promise/progress/pause/
- * resume/updateConfig/cancel semantics and worker message protocol. Each
- * optimize() call runs in a worker_threads Worker (optimizer-worker-3d.js)
- * that drives the device-resident solver of libtopo_hip through the N-API
- * addon. CommonJS, so it also runs on Node 12.
+ * TopologySolver over the MI355X path. Public surface of the reference's
+ * library class (lib/topology-solver.js:132-518): optimize(model, config,
+ * onProgress) -> Promise, analyzeFEA, pause, resume, updateConfig,
+ * requestVolumetricSnapshot, cancel, the onPaused / onVolumetric hooks with
+ * lastPausedData / lastVolumetricData, and TopologySolver.debug. CommonJS,
+ * so it also loads on Node 12.
  *
- * config.solver ('auto' | '2d' | '3d' | 'mgpcg' | 'cg' | 'petsc' | 'fea')
- * selects the worker and the linear solver as in the reference (:213-253):
- * '2d', and 'auto' / 'fea' on a model with nz <= 1 that is not a cube, run
- * the 2-D worker (optimizer-worker-2d.js); the others the 3-D worker. The
- * genetic, nonlinear, fracture, dynamics and fatigue solvers of the
- * reference are not part of this path and reject with an Error naming the
- * solver.
+ * Design. Each optimize() call is one Session: a worker_threads Worker
+ * (optimizer-worker-3d.js or optimizer-worker-2d.js, which drive the
+ * device-resident solver of libtopo_hip through the N-API addon) plus the
+ * promise it settles. A session settles exactly once -- on the worker's
+ * 'complete', 'cancelled' or 'error' message, on a worker crash, or when a
+ * cancel is not acknowledged within CANCEL_GRACE_MS -- and settling always
+ * stops the worker. Incoming messages are dispatched through a table of
+ * handlers keyed by message type; outgoing commands (pause, resume,
+ * updateConfig, requestVolumetric, cancel) go through Session.command, which
+ * drops them once the session is over. The worker applies pause and config
+ * patches at SIMP-iteration boundaries.
+ *
+ * Routing (config.solver, as the reference selects its workers): '2d', and
+ * 'auto' / 'fea' on a model that is neither a cube nor more than one element
+ * deep, use the 2-D worker; everything else the 3-D worker, with 'mgpcg',
+ * 'cg' and 'petsc' choosing its linear solver and 'fea' a single analysis.
+ * The genetic, nonlinear, fracture, dynamics and fatigue analyses are not on
+ * this path: optimize() rejects them with an Error naming the solver.
  */
 const path = require('path');
 
-const IS_NODE = typeof process !== 'undefined' && !!(process.versions && process.versions.node);
+const RUNTIME = typeof process !== 'undefined' && process.versions && process.versions.node ? 'node' : 'unknown';
+const OFF_PATH_SOLVERS = new Set(['genetic', 'nonlinear', 'fracture', 'dynamics', 'fatigue']);
+const LINEAR_SOLVER_KEYS = new Set(['mgpcg', 'cg', 'petsc']);
+const CANCELLED_MESSAGE = 'Optimization cancelled';
+// a cancelled worker that has not answered by then is stopped by the host
+const CANCEL_GRACE_MS = 300;
+const WORKER_SCRIPT = { 3: 'optimizer-worker-3d.js', 2: 'optimizer-worker-2d.js' };
 
-const UNSUPPORTED = ['genetic', 'nonlinear', 'fracture', 'dynamics', 'fatigue'];
+function offPathError(solver) {
+    return new Error(`TopologySolver: solver '${solver}' is not part of the MI355X SIMP path`);
+}
+
+// Which worker a (model, config) pair runs on and the options it receives.
+function planSession(model, config) {
+    const solver = String(config.solver || 'auto').toLowerCase();
+    if (OFF_PATH_SOLVERS.has(solver)) return { error: offPathError(solver) };
+    const volumetric = model.type === 'cube' || (model.nz != null && model.nz > 1);
+    let dims = 3;
+    if (solver === '2d' || ((solver === 'auto' || solver === 'fea') && !volumetric)) dims = 2;
+    const options = {};
+    for (const key of Object.keys(config)) if (key !== 'solver') options[key] = config[key];
+    if (LINEAR_SOLVER_KEYS.has(solver)) options.linearSolver = solver;
+    if (solver === 'fea') options.feaOnly = true;
+    return { script: WORKER_SCRIPT[dims], options };
+}
+
+// The 'start' command: the model fields the workers read, and the options.
+function startCommand(model, options) {
+    return {
+        type: 'start',
+        config: options,
+        model: {
+            nx: model.nx,
+            ny: model.ny,
+            nz: model.nz,
+            type: model.type,
+            meshType: model.meshType || 'box',
+            elements: model.elements ? Array.from(model.elements) : undefined,
+        },
+    };
+}
+
+class Session {
+    constructor(worker, settle, onProgress) {
+        this.worker = worker;
+        this.settleWith = settle;         // (ok, value) -> resolves or rejects the promise
+        this.onProgress = onProgress;
+        this.over = false;
+        this.pauseAsked = false;
+        this.cancelTimer = null;
+    }
+
+    command(type, fields) {
+        if (this.over) return false;
+        this.worker.postMessage(Object.assign({ type }, fields || {}));
+        return true;
+    }
+
+    end(ok, value) {
+        if (this.over) return false;
+        this.over = true;
+        if (this.cancelTimer !== null) clearTimeout(this.cancelTimer);
+        this.cancelTimer = null;
+        this.worker.terminate();
+        this.settleWith(ok, value);
+        return true;
+    }
+}
+
+function callHook(fn, arg) {
+    if (typeof fn === 'function') fn(arg);
+}
+
+// worker message type -> what the host does with it
+const HANDLERS = {
+    progress(host, session, m) {
+        if (session.onProgress) {
+            session.onProgress(m.iteration, m.compliance, m.meshData, m.timing, m.maxStress, m.volumetricData || null);
+        }
+    },
+    paused(host, session, m) {
+        host.lastPausedData = m;
+        if (m.volumetricData) host.lastVolumetricData = m.volumetricData;
+        callHook(host.onPaused, m);
+    },
+    volumetric(host, session, m) {
+        host.lastVolumetricData = m.volumetricData || null;
+        callHook(host.onVolumetric, m);
+    },
+    complete(host, session, m) {
+        const result = m.result;
+        host.lastVolumetricData = result && result.volumetricData ? result.volumetricData : null;
+        host._finish(session, true, result);
+    },
+    cancelled(host, session) {
+        host._finish(session, false, new Error(CANCELLED_MESSAGE));
+    },
+    error(host, session, m) {
+        host._finish(session, false, new Error(m.message));
+    },
+};
 
 class TopologySolver {
     constructor() {
-        this._worker = null;
-        this._paused = false;
-        this._cancelReject = null;
+        this._session = null;
         this.lastPausedData = null;
         this.lastVolumetricData = null;
         this.onPaused = null;
         this.onVolumetric = null;
-        this.environment = { env: IS_NODE ? 'node' : 'unknown', gpu: 'unknown' };
+        this.environment = { env: RUNTIME, gpu: 'unknown' };
     }
 
     /**
      * Run topology optimization in a background worker.
-     * @param {object} model  { nx, ny, nz, elements?, type? }
-     * @param {object} config reference config keys plus the build's own
+     * @param {object} model  { nx, ny, nz, elements?, type?, meshType? }
+     * @param {object} config the reference's config keys plus this build's own
      *   (precision, mgLevels, mgOmega, coarseSolver, cgTol, filterType, ...)
      * @param {function} [progressCallback]
      *   (iteration, compliance, meshData, timing, maxStress, volumetricData)
-     * @returns {Promise<object>} result of the 'complete' message
+     * @returns {Promise<object>} the result of the worker's 'complete' message
      */
     optimize(model, config, progressCallback) {
-        config = config || {};
+        const plan = planSession(model, config || {});
+        if (plan.error) return Promise.reject(plan.error);
         return new Promise((resolve, reject) => {
-            this._cancelReject = reject;
-            this._paused = false;
-            const solverKey = String(config.solver || 'auto').toLowerCase();
-            if (UNSUPPORTED.indexOf(solverKey) >= 0) {
-                reject(new Error(`TopologySolver: solver '${solverKey}' is not part of the MI355X SIMP path`));
-                return;
-            }
-            let linearSolver;
-            let feaOnly = false;
-            if (solverKey === 'mgpcg') linearSolver = 'mgpcg';
-            else if (solverKey === 'cg') linearSolver = 'cg';
-            else if (solverKey === 'petsc') linearSolver = 'petsc';
-            else if (solverKey === 'fea') feaOnly = true;
-            const byGeometry = model.type === 'cube' || (model.nz != null && model.nz > 1);
-            const use3D = solverKey === '2d' ? false : (solverKey === 'auto' || solverKey === 'fea') ? byGeometry : true;
             let worker;
             try {
                 const { Worker } = require('worker_threads');
-                worker = new Worker(path.join(__dirname, use3D ? 'optimizer-worker-3d.js' : 'optimizer-worker-2d.js'));
+                worker = new Worker(path.join(__dirname, plan.script));
             } catch (err) {
-                reject(new Error(`TopologySolver: failed to create worker — ${err.message}`));
+                reject(new Error(`TopologySolver: cannot start the optimizer worker (${err.message})`));
                 return;
             }
-            this._worker = worker;
-            worker.on('message', (msg) => {
-                switch (msg.type) {
-                case 'progress':
-                    if (progressCallback) {
-                        progressCallback(msg.iteration, msg.compliance, msg.meshData, msg.timing, msg.maxStress,
-                            msg.volumetricData || null);
-                    }
-                    break;
-                case 'paused':
-                    this.lastPausedData = msg;
-                    this.lastVolumetricData = msg.volumetricData || this.lastVolumetricData;
-                    if (typeof this.onPaused === 'function') this.onPaused(msg);
-                    break;
-                case 'volumetric':
-                    this.lastVolumetricData = msg.volumetricData || null;
-                    if (typeof this.onVolumetric === 'function') this.onVolumetric(msg);
-                    break;
-                case 'complete':
-                    this.lastVolumetricData = (msg.result && msg.result.volumetricData) || null;
-                    worker.terminate();
-                    this._worker = null;
-                    this._cancelReject = null;
-                    resolve(msg.result);
-                    break;
-                case 'cancelled':
-                    worker.terminate();
-                    this._worker = null;
-                    this._cancelReject = null;
-                    reject(new Error('Optimization cancelled'));
-                    break;
-                case 'error':
-                    worker.terminate();
-                    this._worker = null;
-                    reject(new Error(msg.message));
-                    break;
-                default:
-                    break;
-                }
+            const session = new Session(worker, (ok, value) => (ok ? resolve(value) : reject(value)),
+                progressCallback || null);
+            this._session = session;
+            worker.on('message', (m) => {
+                const handler = m && HANDLERS[m.type];
+                if (handler && !session.over) handler(this, session, m);
             });
-            worker.on('error', (err) => {
-                worker.terminate();
-                this._worker = null;
-                reject(new Error(err.message || String(err)));
-            });
-            const workerConfig = Object.assign({}, config);
-            delete workerConfig.solver;
-            if (linearSolver) workerConfig.linearSolver = linearSolver;
-            if (feaOnly) workerConfig.feaOnly = true;
-            worker.postMessage({
-                type: 'start',
-                model: {
-                    nx: model.nx, ny: model.ny, nz: model.nz, type: model.type,
-                    meshType: model.meshType || 'box',
-                    elements: model.elements ? Array.from(model.elements) : undefined,
-                },
-                config: workerConfig,
-            });
+            worker.on('error', (err) => this._finish(session, false, new Error(err.message || String(err))));
+            session.command('start', startCommand(model, plan.options));
         });
     }
 
-    /** single FE analysis at the model's densities (solver 'fea') */
+    /** one FE analysis at the model's densities (solver 'fea') */
     analyzeFEA(model, config, progressCallback) {
         return this.optimize(model, Object.assign({}, config || {}, { solver: 'fea' }), progressCallback);
     }
 
-    analyzeNonlinear() { return Promise.reject(new Error("TopologySolver: solver 'nonlinear' is not part of the MI355X SIMP path")); }
+    analyzeNonlinear() { return Promise.reject(offPathError('nonlinear')); }
 
-    analyzeFracture() { return Promise.reject(new Error("TopologySolver: solver 'fracture' is not part of the MI355X SIMP path")); }
+    analyzeFracture() { return Promise.reject(offPathError('fracture')); }
 
-    analyzeDynamics() { return Promise.reject(new Error("TopologySolver: solver 'dynamics' is not part of the MI355X SIMP path")); }
+    analyzeDynamics() { return Promise.reject(offPathError('dynamics')); }
 
-    /** pause at the next iteration boundary (worker answers 'paused') */
+    /** pause at the next iteration boundary; the worker answers 'paused' */
     pause() {
-        if (this._worker) {
-            this._paused = true;
-            this._worker.postMessage({ type: 'pause' });
-        }
+        const s = this._session;
+        if (s && !s.pauseAsked && s.command('pause')) s.pauseAsked = true;
     }
 
     resume() {
-        if (this._worker && this._paused) {
-            this._paused = false;
-            this._worker.postMessage({ type: 'resume' });
-        }
+        const s = this._session;
+        if (s && s.pauseAsked && s.command('resume')) s.pauseAsked = false;
     }
 
-    /** patch penaltyFactor, filterRadius, volumeFraction, maxIterations at the next iteration */
+    /** patch penaltyFactor, filterRadius, volumeFraction, maxIterations from the next iteration on */
     updateConfig(patch) {
-        if (this._worker) this._worker.postMessage({ type: 'updateConfig', config: patch });
+        if (this._session) this._session.command('updateConfig', { config: patch });
     }
 
+    /** the worker sends a 'volumetric' message; with no run going, the last snapshot is handed to onVolumetric */
     requestVolumetricSnapshot() {
-        if (this._worker) {
-            this._worker.postMessage({ type: 'requestVolumetric' });
-        } else if (this.lastVolumetricData && typeof this.onVolumetric === 'function') {
-            this.onVolumetric({ type: 'volumetric', reason: 'cached', volumetricData: this.lastVolumetricData });
+        if (this._session && this._session.command('requestVolumetric')) return;
+        if (this.lastVolumetricData) {
+            callHook(this.onVolumetric, { type: 'volumetric', reason: 'cached', volumetricData: this.lastVolumetricData });
         }
     }
 
     /** cancel; the optimize() promise rejects with 'Optimization cancelled' */
     cancel() {
-        if (this._worker) {
-            this._worker.postMessage({ type: 'cancel' });
-            setTimeout(() => {
-                if (this._worker) {
-                    this._worker.terminate();
-                    this._worker = null;
-                    if (this._cancelReject) {
-                        this._cancelReject(new Error('Optimization cancelled'));
-                        this._cancelReject = null;
-                    }
-                }
-            }, 300);
-        }
+        const s = this._session;
+        if (!s || !s.command('cancel') || s.cancelTimer !== null) return;
+        s.cancelTimer = setTimeout(() => {
+            s.cancelTimer = null;
+            this._finish(s, false, new Error(CANCELLED_MESSAGE));
+        }, CANCEL_GRACE_MS);
+    }
+
+    _finish(session, ok, value) {
+        session.end(ok, value);
+        if (this._session === session) this._session = null;
     }
 
     static get debug() {
         return {
             probeGPU() {
                 const addon = require(path.join(__dirname, 'topo_addon.node'));
-                const d = addon.deviceInfo();
-                return Promise.resolve(d.count > 0 ? 'hip' : 'none');
+                return Promise.resolve(addon.deviceInfo().count > 0 ? 'hip' : 'none');
             },
             detectEnvironment() {
-                return { env: IS_NODE ? 'node' : 'unknown', workerType: 'worker_threads' };
+                return { env: RUNTIME, workerType: 'worker_threads' };
             },
-            workerUrl() {
-                return 'file://' + path.join(__dirname, 'optimizer-worker-3d.js');
+            workerUrl(solverType) {
+                const dims = String(solverType || '3d').toLowerCase() === '2d' ? 2 : 3;
+                return 'file://' + path.join(__dirname, WORKER_SCRIPT[dims]);
             },
         };
     }

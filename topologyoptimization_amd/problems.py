@@ -65,9 +65,14 @@ def cantilever(nelx, nely, nelz, **kw):
     voided elements removed, disconnected parts make K singular and the CG
     iterates diverge along their rigid modes. MG smoothing weights are capped
     at 1.6 / lambda_max(D^-1 A) (damped Jacobi at omega 0.6 is unstable for
-    these density contrasts)."""
+    these density contrasts). Element energies are clamped at 0
+    (energy_clamp 1): from the U[0.05, 1] start the move limit voids large
+    regions within three iterations, elements there move almost rigidly, and
+    at 192x96x96 some energies u_e^T K0 u_e round to about -4e-13 in the
+    fourth iteration; the reference's OC then takes the square root of a
+    negative number and the design gets a NaN entry."""
     cfg = default_config(nelx, nely, nelz, volfrac=0.12, penal=3.0, penal_start=3.0, rmin=1.5, use_projection=0,
-                         max_iterations=20, cg_tol=1e-6, skip_factor=0.0, mg_omega_cap=1.6)
+                         max_iterations=20, cg_tol=1e-6, skip_factor=0.0, mg_omega_cap=1.6, energy_clamp=1)
     for k, v in kw.items():
         setattr(cfg, k, v)
     nel, ndof = dims(nelx, nely, nelz)
@@ -101,10 +106,11 @@ def bridge(nelx, nely, nelz, **kw):
     (z = 0) pinned in xyz; 4 load cases, each a -z load of total 1.0 on an
     8x8-node patch (float32 -1/64 per node) of the top face (z = nelz) centred
     in y at x = 0.2, 0.4, 0.6, 0.8 of the span. volfrac 0.12, p = 3,
-    sensitivity filter rmin 1.5, MG-PCG, tol 1e-6, every element kept."""
+    sensitivity filter rmin 1.5, MG-PCG, tol 1e-6, every element kept,
+    energies clamped at 0 (as the cantilever)."""
     cfg = default_config(nelx, nely, nelz, nload=4, volfrac=0.12, penal=3.0, penal_start=3.0, rmin=1.5,
                          use_projection=0, max_iterations=10, cg_tol=1e-6, linear_solver=SOLVER_MGPCG,
-                         skip_factor=0.0, mg_omega_cap=1.6)
+                         skip_factor=0.0, mg_omega_cap=1.6, energy_clamp=1)
     for k, v in kw.items():
         setattr(cfg, k, v)
     nel, ndof = dims(nelx, nely, nelz)
