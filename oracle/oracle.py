@@ -56,6 +56,7 @@ def lib():
         _lib.or_simp_run.argtypes = [C.c_void_p] * 5 + [C.c_int] + [C.c_void_p] * 5
         _lib.or_helmholtz.argtypes = [C.c_int] * 3 + [C.c_double, C.c_void_p, C.c_void_p]
         _lib.or_fb.argtypes = [C.c_void_p] * 4 + [C.c_int] + [C.c_void_p] * 6
+        _lib.or_simp_sens.argtypes = [C.c_void_p] * 7
         _lib.or_bddc_create.restype = C.c_void_p
         _lib.or_bddc_create.argtypes = [C.c_int] * 3
         _lib.or_bddc_destroy.argtypes = [C.c_void_p]
@@ -344,6 +345,19 @@ def simp_run(cfg, fixed, F, pres, void, niter, x0):
     n = lib().or_simp_run(C.byref(cfg), _p(fixed), _p(F), _p(pres), _p(void), niter, _p(x), _p(comp), _p(change),
                           _p(cg), _p(U))
     return dict(iterations=n, x=x, compliance=comp[:n], change=change[:n], cg=cg[:n], U=U)
+
+
+def simp_sens(cfg, fixed, F, x):
+    """First iteration of simp_run from x: (compliance, xPhys, dc) with dc the
+    unfiltered sensitivities of the loop (dJ/dx through the projection)."""
+    nel, _ = dims(cfg.nelx, cfg.nely, cfg.nelz)
+    x = np.ascontiguousarray(x, np.float32)
+    c = C.c_double()
+    xp = np.zeros(nel, np.float32)
+    dc = np.zeros(nel, np.float32)
+    lib().or_simp_sens(C.byref(cfg), _p(np.ascontiguousarray(fixed, np.uint8)), _p(np.ascontiguousarray(F, np.float32)),
+                       _p(x), C.byref(c), _p(xp), _p(dc))
+    return c.value, xp, dc
 
 
 # ---- painted supports / loads (pure Python; small grids) --------------------

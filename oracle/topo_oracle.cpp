@@ -1000,6 +1000,13 @@ static int bddc_fe(void* h, const double* ke, const float* xPhys, double penal, 
 // design. Solver per cfg->linear_solver: PCG = ebePCG with warm start
 // (:3572-3629); MGPCG = ebeMGPCG cold start (:3642-3807) with the MG options;
 // KSP_BDDC = ebeKSP_BDDC cold start (:3814-3999).
+// When set, or_simp_run copies the first iteration's physical densities and
+// unfiltered sensitivities dc (dJ/dx through the projection, before the
+// sensitivity filter) here: or_simp_sens reads them to pin the loop's own
+// sensitivity code against finite differences of its compliance.
+static float* g_probe_xphys = nullptr;
+static float* g_probe_dc = nullptr;
+
 int or_simp_run(const topo_config* cfg, const uint8_t* fixed, const float* F32, const uint8_t* pres,
                 const uint8_t* voidm, int niter, float* x_io, double* comp_out, double* change_out,
                 int32_t* cg_out, double* U_out) {
@@ -1081,6 +1088,10 @@ int or_simp_run(const topo_config* cfg, const uint8_t* fixed, const float* F32, 
             }
             dc[e] = (float)(dphys * d);
         }
+        if (loop == 1 && g_probe_dc) {
+            std::copy(dc.begin(), dc.end(), g_probe_dc);
+            std::copy(xPhys.begin(), xPhys.end(), g_probe_xphys);
+        }
         if (cfg->filter_type == TOPO_FILTER_DENSITY) {
             or_density_filter(nelx, nely, nelz, cfg->rmin, dc.data(), dcn.data(), 1);
         } else {
@@ -1101,6 +1112,20 @@ int or_simp_run(const topo_config* cfg, const uint8_t* fixed, const float* F32, 
     if (mg) or_mg_destroy(mg);
     if (bddc) or_bddc_destroy(bddc);
     return loop;
+}
+
+// one iteration of or_simp_run from x: its compliance, physical densities
+// and unfiltered sensitivities
+int or_simp_sens(const topo_config* cfg, const uint8_t* fixed, const float* F32, const float* x, double* comp,
+                 float* xphys_out, float* dc_out) {
+    const int64_t nel = (int64_t)cfg->nelx * cfg->nely * cfg->nelz;
+    std::vector<float> xi(x, x + nel);
+    g_probe_xphys = xphys_out;
+    g_probe_dc = dc_out;
+    const int n = or_simp_run(cfg, fixed, F32, nullptr, nullptr, 1, xi.data(), comp, nullptr, nullptr, nullptr);
+    g_probe_xphys = nullptr;
+    g_probe_dc = nullptr;
+    return n;
 }
 
 // ---------------------------------------------------------------------------

@@ -588,3 +588,85 @@ def test_bddc_ksp_loop(oracle):
     assert 0 < it4 <= 500 and np.isfinite(np.linalg.norm(res))
     assert np.abs(U4[3 * n * (n + 1) ** 2:]).max() > 1e-10
     assert not U4[fx > 0].any()
+
+
+# ---- known answers of the 3-D presets (getFixedDOFs / getLoadVector /
+# getConstraintElements / getForceElements, js/optimizer-worker-3d.js:4429-4619)
+# on a 3x2x1 grid: node n = 6 ix + 2 iy + iz (nny = 3, nnz = 2), element
+# e = ex + 3 ey + 6 ez. The lists below were worked out by hand from the
+# reference's geometric description of each preset.
+KAT_GRID = (3, 2, 1)
+KAT_FIXED_NODES = {
+    0: [0, 1, 2, 3, 4, 5],               # 'left': the 6 nodes of x = 0
+    1: [18, 19, 20, 21, 22, 23],         # 'right': x = 3
+    2: [0, 1, 6, 7, 12, 13, 18, 19],     # 'bottom': y = 0
+    3: [4, 5, 10, 11, 16, 17, 22, 23],   # 'top': y = 2
+    4: [0, 4, 18, 22],                   # 'bottom-corners': (0,0,0) (0,2,0) (3,0,0) (3,2,0)
+}
+# direction -> (DOF index, value) for magnitude 2.5
+KAT_LOADS = {
+    0: (70, -2.5),   # 'down': node (3,2,1) = 23, -y
+    1: (70, 2.5),    # 'up'
+    2: (60, -2.5),   # 'left': node (3,1,0) = 20, -x
+    3: (6, 2.5),     # 'right': node (0,1,0) = 2, +x
+    4: (29, 2.5),    # 'top-center': node (1,1,1) = 9, +z
+}
+KAT_CONSTRAINT_ELEMS = {0: [0, 3], 1: [2, 5], 2: [0, 1, 2], 3: [3, 4, 5], 4: [0, 2, 3, 5]}
+KAT_FORCE_ELEMS = {0: [5], 1: [5], 2: [5], 3: [3], 4: [4]}
+
+
+@pytest.mark.parametrize("pos", sorted(KAT_FIXED_NODES))
+def test_fixed_dof_presets_known_answers(oracle, pos):
+    nodes = KAT_FIXED_NODES[pos]
+    for mask, comps in ((7, (0, 1, 2)), (1, (0,)), (2, (1,)), (4, (2,)), (6, (1, 2)), (5, (0, 2))):
+        want = sorted(3 * n + d for n in nodes for d in comps)
+        got = np.flatnonzero(oracle.fixed_dofs(*KAT_GRID, pos, mask)).tolist()
+        assert got == want, (pos, mask, got, want)
+
+
+@pytest.mark.parametrize("direction", sorted(KAT_LOADS))
+def test_load_presets_known_answers(oracle, direction):
+    dof, val = KAT_LOADS[direction]
+    F = oracle.load_vector(*KAT_GRID, direction, 2.5)
+    assert F.dtype == np.float32 and np.flatnonzero(F).tolist() == [dof] and F[dof] == np.float32(val)
+    # a custom force vector goes to the top-right-back corner node (3,2,1) = 23,
+    # normalised and scaled by the magnitude, whatever the direction
+    Fv = oracle.load_vector(*KAT_GRID, direction, 2.5, [1.0, 2.0, 2.0])
+    assert np.flatnonzero(Fv).tolist() == [69, 70, 71]
+    assert np.array_equal(Fv[69:72], np.array([2.5 / 3, 5.0 / 3, 5.0 / 3], np.float32))
+
+
+@pytest.mark.parametrize("pos", sorted(KAT_CONSTRAINT_ELEMS))
+@pytest.mark.parametrize("direction", sorted(KAT_FORCE_ELEMS))
+def test_preserved_element_presets_known_answers(oracle, pos, direction):
+    want = sorted(set(KAT_CONSTRAINT_ELEMS[pos]) | set(KAT_FORCE_ELEMS[direction]))
+    assert np.flatnonzero(oracle.preserved(*KAT_GRID, pos, direction)).tolist() == want
+
+
+def test_simp_projected_sensitivity_finite_differences(oracle):
+    """The loop's own unfiltered sensitivity (or_simp_run: dc = -p xPhys^(p-1)
+    E0 w times the Heaviside derivative, js/optimizer-worker-3d.js:2175-2192)
+    against central finite differences of the loop's compliance with respect
+    to the design, projection on (beta 4), on 8 seeded elements. xPhys is
+    stored in float32 as in the reference, so the differences carry ~1e-5
+    relative noise at h = 1e-3; a dropped factor or a wrong sign is O(1)."""
+    import topologyoptimization_amd as T
+    import topologyoptimization_amd.problems as P
+    cfg, fixed, F, _, _, _ = P.cantilever(6, 3, 3, use_projection=1, beta_fixed=4, linear_solver=T.SOLVER_PCG,
+                                          cg_tol=1e-13, cg_max_iter=5000, energy_clamp=0, skip_factor=0.0)
+    nel = 54
+    rng = np.random.default_rng(11)
+    x = rng.uniform(0.3, 0.7, nel).astype(np.float32)
+    c0, xp, dc = oracle.simp_sens(cfg, fixed, F, x)
+    # projection: xPhys = H(x) in closed form
+    b = 4.0
+    H = (np.tanh(b * 0.5) + np.tanh(b * (x.astype(np.float64) - 0.5))) / (2 * np.tanh(b * 0.5))
+    assert np.allclose(xp, H, rtol=1e-6, atol=0)
+    h = 1e-3
+    for e in rng.choice(nel, 8, replace=False):
+        xa, xb = x.copy(), x.copy()
+        xa[e] += h
+        xb[e] -= h
+        fd = (oracle.simp_sens(cfg, fixed, F, xa)[0] - oracle.simp_sens(cfg, fixed, F, xb)[0]) / (
+            float(xa[e]) - float(xb[e]))
+        assert dc[e] < 0 and abs(fd - dc[e]) <= 2e-3 * abs(dc[e]), (e, fd, dc[e])
