@@ -1238,11 +1238,17 @@ __global__ void k_heaviside(const float* __restrict__ x, float* __restrict__ xp,
 // clamp = 1 sets energies below 0 to 0 (u^T K0 u >= 0 for the positive
 // semidefinite K0, so a negative sum is rounding of a near-rigid element
 // motion); clamp = 0 keeps the reference's value.
-__global__ void k_energy_sens(const double* __restrict__ U, const float* __restrict__ xphys, const float* __restrict__ xd,
+__global__ void __launch_bounds__(256) k_energy_sens(const double* __restrict__ U, const float* __restrict__ xphys, const float* __restrict__ xd,
                               double penal, double E0, double Emin, int use_proj, double beta, double eta, Grid g,
                               float* __restrict__ w, float* __restrict__ stress, float* __restrict__ dc,
                               const double* __restrict__ ke, int clamp) {
 #pragma clang fp contract(off)
+    // K0 staged in LDS (rows read back as broadcasts), and each thread's
+    // element vector (u_i for the row loop, which is not unrolled)
+    __shared__ double s_ke[576];
+    __shared__ double s_u[24][256];
+    for (int k = threadIdx.x; k < 576; k += blockDim.x) s_ke[k] = ke[k];
+    __syncthreads();
     const int nny = g.nny(), nnz = g.nnz();
     const int64_t nyz = (int64_t)nny * nnz;
     const double denom = 2 * tanh(beta * 0.5);
@@ -1252,6 +1258,7 @@ __global__ void k_energy_sens(const double* __restrict__ U, const float* __restr
         const int ey = (int)(t % g.nely), ex = (int)(t / g.nely);
         const int64_t n0 = ((int64_t)ex * nny + ey) * nnz + ez;
         double u[24];
+#pragma unroll
         for (int m = 0; m < 8; m++) {
             int a, b, c;
             local_offsets(m, a, b, c);
@@ -1260,9 +1267,18 @@ __global__ void k_energy_sens(const double* __restrict__ U, const float* __restr
             u[3 * m + 1] = U[3 * nm + 1];
             u[3 * m + 2] = U[3 * nm + 2];
         }
+        // the reference's order, term by term: row i of K0 and u_i from LDS,
+        // the u_j in registers
+#pragma unroll
+        for (int i = 0; i < 24; i++) s_u[i][threadIdx.x] = u[i];
         double en = 0;
-        for (int i = 0; i < 24; i++)
-            for (int j = 0; j < 24; j++) en += u[i] * ke[i * 24 + j] * u[j];
+#pragma unroll 1
+        for (int i = 0; i < 24; i++) {
+            const double ui = s_u[i][threadIdx.x];
+            const double* kr = s_ke + i * 24;
+#pragma unroll
+            for (int j = 0; j < 24; j++) en += ui * kr[j] * u[j];
+        }
         if (clamp && en < 0.0) en = 0.0;
         const float wf = (float)en;
         w[e] = wf;
@@ -1367,12 +1383,13 @@ __global__ void k_density_filter(const float* __restrict__ in, float* __restrict
 // of the next `depth` steps are evaluated in one sweep over the elements,
 // then the path through the tree is walked on the device. The sequence of
 // lmid values is exactly the reference's.
-constexpr int kOcDepth = 5;
+constexpr int kOcDepth = 3;
 constexpr int kOcCand = (1 << kOcDepth) - 1;
 
 struct OCState {
     double l1, l2;
     double lam[kOcCand];   // candidate midpoints (tree in BFS order)
+    double rsl[kOcCand];   // 1 / sqrt(lam) (the fast path's scale)
     int nlam;
     int passes;            // bisection steps taken
     int finished;
@@ -1388,6 +1405,39 @@ __device__ __forceinline__ float oc_xnew(float xf, float dcf, double lmid, doubl
     const double Be = -(double)dcf / lmid;
     const double xi = (double)xf;
     return (float)js_max(0.0, js_max(xi - move, js_min(1.0, js_min(xi + move, xi * sqrt(Be)))));
+}
+
+// The same xnew for an element with finite x > 0 and finite dc < 0 (not
+// preserved, not void), from s = x sqrt(-dc) (per element) and
+// rsl = 1 / sqrt(lmid) (per candidate): t = s rsl equals x sqrt(-dc / lmid)
+// to within 4 FP64 ulps, so where t is clearly above min(1, x + move) the
+// reference's clamp chain returns that bound, clearly below x - move (> 0)
+// that bound, and in between t itself, whose float rounding is the exact
+// value's unless t lies within 8 FP64 ulps of a float rounding midpoint.
+// Returns false when it cannot decide (the caller runs oc_xnew).
+__device__ __forceinline__ bool oc_xnew_fast(double s, double xi, double move, double rsl, float* out) {
+#pragma clang fp contract(off)
+    constexpr double eps = 1e-12;
+    const double t = s * rsl;
+    const double a = xi + move;
+    const double hi = a < 1.0 ? a : 1.0;
+    if (t > hi * (1.0 + eps)) { *out = (float)hi; return true; }
+    if (t >= hi * (1.0 - eps)) return false;
+    const double b = xi - move;
+    if (b > 0.0) {
+        if (t < b * (1.0 - eps)) { *out = (float)b; return true; }
+        if (t <= b * (1.0 + eps)) return false;
+    }
+    // interior: the chain returns t (0 <= t < 1)
+    const float f = (float)t;
+    const double fd = (double)f;
+    const unsigned fb = __float_as_uint(f);
+    const double up = 0.5 * ((double)__uint_as_float(fb + 1u) - fd);          // f >= 0
+    const double dn = fb ? 0.5 * (fd - (double)__uint_as_float(fb - 1u)) : up;
+    const double d = t - fd;
+    const double tol = 8.0 * 2.220446049250313e-16 * t;
+    if (d >= 0.0 ? (d + tol < up) : (tol - d < dn)) { *out = f; return true; }
+    return false;
 }
 
 // OC update of the compliant mechanism (the damped OC of the classic
@@ -1416,6 +1466,7 @@ __global__ void k_oc_plan(OCState* st) {
     for (int k = 0; k < kOcCand; k++) {
         const double m = 0.5 * (hi[k] + lo[k]);
         st->lam[k] = m;
+        st->rsl[k] = 1.0 / sqrt(m);
         const int lc = 2 * k + 1, rc = 2 * k + 2;
         if (lc < kOcCand) { lo[lc] = m; hi[lc] = hi[k]; }      // sum > target: l1 = lmid
         if (rc < kOcCand) { lo[rc] = lo[k]; hi[rc] = m; }      // else l2 = lmid
@@ -1429,14 +1480,23 @@ __global__ void __launch_bounds__(kRedThreads) k_oc_eval(const float* __restrict
                                                         unsigned* ticket, int mech) {
     if (st->finished) return;
     double v[kOcCand];
-    double lam[kOcCand];
+    double lam[kOcCand], rsl[kOcCand];
 #pragma unroll
-    for (int k = 0; k < kOcCand; k++) { v[k] = 0.0; lam[k] = st->lam[k]; }
+    for (int k = 0; k < kOcCand; k++) { v[k] = 0.0; lam[k] = st->lam[k]; rsl[k] = st->rsl[k]; }
     for (int64_t i = gtid(); i < n; i += gstride()) {
         const float xf = x[i], dcf = dc[i];
         const uint8_t pr = pres ? pres[i] : 0, vv = vd ? vd[i] : 0;
+        const double xi = (double)xf, di = (double)dcf;
+        // the fast path: finite x > 0, finite dc < 0, compliance OC
+        const bool fast = !mech && !pr && !vv && xi > 0.0 && xi <= 1e30 && di < 0.0 && di > -1e300;
+        const double se = fast ? xi * sqrt(-di) : 0.0;
 #pragma unroll
-        for (int k = 0; k < kOcCand; k++) v[k] += (double)oc_xnew_any(mech, xf, dcf, lam[k], move, pr, vv);
+        for (int k = 0; k < kOcCand; k++) {
+            float val;
+            if (!(fast && oc_xnew_fast(se, xi, move, rsl[k], &val)))
+                val = oc_xnew_any(mech, xf, dcf, lam[k], move, pr, vv);
+            v[k] += (double)val;
+        }
     }
     double tot[kOcCand];
     if (grid_reduce<kOcCand>(v, partials, ticket, tot)) {
@@ -1494,11 +1554,26 @@ __global__ void __launch_bounds__(kRedThreads) k_change(const float* __restrict_
         partials[2 * kRedBlocks + blockIdx.x] = mm;
     }
     __syncthreads();
-    if (grid_reduce<1>(vs, partials, ticket, tot)) {
+    // the last block (grid_reduce returns true in its thread 0) also takes
+    // the max over the block maxima, with all its threads
+    __shared__ int is_last;
+    __shared__ double smax[kRedThreads / 64];
+    const bool fin = grid_reduce<1>(vs, partials, ticket, tot);
+    if (threadIdx.x == 0) is_last = fin ? 1 : 0;
+    __syncthreads();
+    if (!is_last) return;
+    double mm = 0.0;
+    for (unsigned bI = threadIdx.x; bI < gridDim.x; bI += blockDim.x)
+        mm = chg_max(mode, mm, __hip_atomic_load(&partials[2 * kRedBlocks + bI], __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT));
+    for (int off = 32; off > 0; off >>= 1) mm = chg_max(mode, mm, __shfl_down(mm, off, 64));
+    if (lane == 0) smax[wid] = mm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m2 = 0.0;
+        for (int w = 0; w < kRedThreads / 64; w++) m2 = chg_max(mode, m2, smax[w]);
+        out2[0] = m2;
         out2[1] = tot[0];
-        double mm = 0;
-        for (unsigned bI = 0; bI < gridDim.x; bI++) mm = chg_max(mode, mm, partials[2 * kRedBlocks + bI]);
-        out2[0] = mm;
     }
 }
 

@@ -201,7 +201,7 @@ struct Ctx {
 // (k_oc_plan, k_oc_eval) pair takes up to kOcDepth bisection steps, and once
 // the reference's stop test holds every later pair is a no-op, so a batch of
 // pairs runs without host round trips; the host reads the state once per
-// batch. A typical update takes 40-55 steps (one batch of 12 pairs); a NaN in
+// batch. A typical update takes 40-55 steps (one batch of 20 pairs); a NaN in
 // the sensitivities (the volume test then never passes) walks lambda down to
 // underflow in ~1100 steps, as the reference loop does. xnew is the last
 // executed step's, and *vol the volume sum that step compared.
@@ -214,8 +214,8 @@ static int oc_bisect(Ctx& C, const float* x, const float* dcn, const uint8_t* pr
     h.target = volfrac * (double)nel;
     *C.h_oc = h;
     TOPO_HIP(hipMemcpyAsync(C.oc, C.h_oc, sizeof(OCState), hipMemcpyHostToDevice, C.s));
-    constexpr int kPairsPerBatch = 12;
-    for (int batch = 0; batch < 40; batch++) {
+    constexpr int kPairsPerBatch = 20;
+    for (int batch = 0; batch < 60; batch++) {
         for (int k = 0; k < kPairsPerBatch; k++) {
             hipLaunchKernelGGL(k_oc_plan, dim3(1), dim3(64), 0, C.s, C.oc);
             hipLaunchKernelGGL(k_oc_eval, dim3(kRedBlocks), dim3(kRedThreads), 0, C.s, x, dcn, pres, voidm, move, nel,
