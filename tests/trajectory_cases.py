@@ -16,14 +16,14 @@ CASES = ("c3_recipe_32", "c3_recipe_48_3lev", "reference_default_16", "c2_recipe
 def build(name, side):
     if name == "c3_recipe_32":
         # the 192x96x96 benchmark recipe (U[0.05, 1] start, 4-level MG-PCG,
-        # omega 0.6 capped, FP64 Cholesky coarsest, every element kept,
-        # energies clamped) on a 32x16x16 grid, 10 iterations: volume
+        # omega 0.6 capped at 1.8 / rho, FP64 Cholesky coarsest, every element
+        # kept, energies clamped) on a 32x16x16 grid, 10 iterations: volume
         # reaches 0.12 in the third
         nx, ny, nz = 32, 16, 16
         cfg, fixed, F, pres, void, _ = P.cantilever(nx, ny, nz, linear_solver=T.SOLVER_MGPCG, mg_levels=4, mg_omega=0.6,
                                                     coarse_solver=T.COARSE_CHOLESKY, galerkin_max_ndof=0,
                                                     precision=T.PREC_FP64 if side == "oracle" else T.PREC_MIXED_PC,
-                                                    coarse_fp32=2, max_iterations=1000)
+                                                    coarse_fp32=2, max_iterations=1000, mg_omega_cap=1.8)
         return cfg, fixed, F, pres, void, P.splitmix64_uniform(T.dims(nx, ny, nz)[0]), 10
     if name == "c3_recipe_48_3lev":
         # the same recipe at 48x24x24 with 3 levels: the coarsest block
@@ -33,7 +33,7 @@ def build(name, side):
         cfg, fixed, F, pres, void, _ = P.cantilever(nx, ny, nz, linear_solver=T.SOLVER_MGPCG, mg_levels=3, mg_omega=0.6,
                                                     coarse_solver=T.COARSE_CHOLESKY, galerkin_max_ndof=0,
                                                     precision=T.PREC_FP64 if side == "oracle" else T.PREC_MIXED_PC,
-                                                    max_iterations=1000)
+                                                    max_iterations=1000, mg_omega_cap=1.8)
         return cfg, fixed, F, pres, void, P.splitmix64_uniform(T.dims(nx, ny, nz)[0]), 6
     if name == "reference_default_16":
         # the reference's default 3-D loop (js/optimizer-worker-3d.js:2074-2252

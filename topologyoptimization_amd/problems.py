@@ -92,11 +92,13 @@ def cantilever_c2(precision=PREC_FP64, **kw):
 
 def cantilever_c3(precision=PREC_FP64, nlevels=4, **kw):
     """192x96x96 cantilever (5,447,811 DOF), 4-level MG-PCG, V(2,2) damped
-    Jacobi omega 0.6, FP64 direct solve of the coarsest level, tol 1e-6,
-    initial densities U[0.05, 1] from splitmix64(SEED)."""
-    cfg, fixed, F, pres, void, _ = cantilever(192, 96, 96, linear_solver=SOLVER_MGPCG, precision=precision,
-                                             mg_levels=nlevels, mg_omega=0.6, coarse_solver=COARSE_CHOLESKY,
-                                             galerkin_max_ndof=0, **kw)
+    Jacobi omega 0.6 capped at 1.8 / lambda_max(D^-1 A), FP64 direct solve of
+    the coarsest level, tol 1e-6, initial densities U[0.05, 1] from
+    splitmix64(SEED)."""
+    args = dict(linear_solver=SOLVER_MGPCG, precision=precision, mg_levels=nlevels, mg_omega=0.6,
+                coarse_solver=COARSE_CHOLESKY, galerkin_max_ndof=0, mg_omega_cap=1.8)
+    args.update(kw)
+    cfg, fixed, F, pres, void, _ = cantilever(192, 96, 96, **args)
     nel, _ = dims(192, 96, 96)
     return cfg, fixed, F, pres, void, splitmix64_uniform(nel)
 
