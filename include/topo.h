@@ -195,6 +195,23 @@ int topo_ebePCG(const double* dens, const double* ke, const int32_t* edofs,
                 int32_t edofSize, int32_t ndof, int32_t nfree, double Emin, double E0,
                 double penal, int32_t maxIter, double tolerance, double* work);
 
+/* No counterpart in the reference: topo_ebePCG (edofSize 24) for callers
+ * that cannot allocate device memory -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc. Same arguments (less the
+ * ignored `work`), semantics, errors and results as topo_ebePCG with
+ * edofSize 24; every device buffer it needs (input copies, CG vectors,
+ * reduction partials, CG state) lives in one file-scope device array of the
+ * library (about 20 MB, held on each GPU it runs on until the process ends),
+ * so it calls no hipMalloc / hipFree and no allocating library routine.
+ * Limits: ndof <= 262144 and nel <= 131072 (TOPO_E_UNSUPPORTED beyond).
+ * Runs on the null stream of the current device and returns after the
+ * result is copied back. Calls are serialised within a process; calls from
+ * several processes on one GPU each use their own copy of the array. */
+int topo_ebePCG_noalloc(const double* dens, const double* ke, const int32_t* edofs,
+                        const double* F, double* U, const int32_t* freedofs, int32_t nel,
+                        int32_t edofSize, int32_t ndof, int32_t nfree, double Emin, double E0,
+                        double penal, int32_t maxIter, double tolerance);
+
 /* Level descriptor of ebeMGPCG (assembly/index.ts:743-757): the 20 x i32
  * record with its offsets widened to host pointers. */
 typedef struct topo_mg_level {

@@ -103,6 +103,7 @@ SYMBOLS = [
     "topo_fixed_paint_2d", "topo_load_paint_2d", "topo_config_default_2d", "topo_solver2d_create",
     "topo_solver2d_destroy", "topo_solver2d_set_design", "topo_solver2d_get_design", "topo_solver2d_update",
     "topo_solver2d_step", "topo_solver2d_fea", "topo_solver2d_get_field", "topo_solver2d_cancel",
+    "topo_ebePCG_noalloc",
 ]
 
 
@@ -128,6 +129,7 @@ def lib():
     L.topo_fixed_paint.argtypes = [i32, i32, i32, vp, i32, i32, vp]
     L.topo_applyAEbe3D.argtypes = [vp, vp, vp, vp, i32, vp, vp, i32, vp]
     L.topo_ebePCG.argtypes = [vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, dbl, dbl, dbl, i32, dbl, vp]
+    L.topo_ebePCG_noalloc.argtypes = [vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, dbl, dbl, dbl, i32, dbl]
     L.topo_ebeMGPCG.argtypes = [vp, i32, vp, vp, vp, vp, i32, i32, dbl, vp]
     L.topo_ebeKSP_BDDC.argtypes = [vp, i32, vp, i32, vp, vp, vp, vp, i32, vp, vp, vp, vp, i32, i32, dbl, i32, vp]
     L.topo_solver_create.argtypes = [vp, vp, vp, vp, vp, vp]
@@ -272,6 +274,19 @@ def ebePCG(dens, ke, edofs_arr, F, U, freedofs, Emin, E0, penal, maxIter, tol):
     assert U.dtype == np.float64 and U.flags.c_contiguous
     return _check(lib().topo_ebePCG(_p(dens), _p(ke), _p(ed), _p(F), _p(U), _p(freedofs), dens.size, esz, F.size,
                                     freedofs.size, Emin, E0, penal, maxIter, tol, None))
+
+
+def ebePCG_noalloc(dens, ke, edofs_arr, F, U, freedofs, Emin, E0, penal, maxIter, tol):
+    """topo_ebePCG_noalloc (3-D hex8): ebePCG with no device allocation; U
+    is the warm start and is overwritten."""
+    dens = np.ascontiguousarray(dens, np.float64)
+    ke = np.ascontiguousarray(ke, np.float64).ravel()
+    ed = np.ascontiguousarray(edofs_arr, np.int32).ravel()
+    F = np.ascontiguousarray(F, np.float64)
+    freedofs = np.ascontiguousarray(freedofs, np.int32)
+    assert U.dtype == np.float64 and U.flags.c_contiguous
+    return _check(lib().topo_ebePCG_noalloc(_p(dens), _p(ke), _p(ed), _p(F), _p(U), _p(freedofs), dens.size, 24,
+                                            F.size, freedofs.size, Emin, E0, penal, maxIter, tol))
 
 
 def ebeMGPCG(levels, ke, F, U, fixed0, maxIter, tol):

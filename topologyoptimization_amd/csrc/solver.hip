@@ -150,7 +150,21 @@ struct Ctx {
         TOPO_HIP(hipMemsetAsync(st, 0, kMaxRHS * sizeof(CGState), s));
         TOPO_HIP(hipStreamSynchronize(s));
     }
+    // A context over device and host memory the caller owns: no allocation,
+    // the null stream, no graphs (the allocation-free entry point's).
+    struct Borrowed {};
+    bool borrowed = false;
+    Ctx(Borrowed, double* partials_, unsigned* ticket_, CGState* st_, OCState* oc_, double* dsc_, CGState* h_st_,
+        OCState* h_oc_, double* h_sc_)
+        : partials(partials_), ticket(ticket_), st(st_), h_st(h_st_), oc(oc_), h_oc(h_oc_), dsc(dsc_), h_sc(h_sc_),
+          borrowed(true) {
+        TOPO_HIP(hipGetDevice(&device));
+    }
     ~Ctx() {
+        if (borrowed) {
+            (void)hipStreamSynchronize(s);
+            return;
+        }
         if (s) (void)hipStreamSynchronize(s);
         if (s2) (void)hipStreamSynchronize(s2);
         if (s3) (void)hipStreamSynchronize(s3);
@@ -3353,6 +3367,61 @@ int topo_solver_cancel(topo_solver* h) {
 
 namespace topo {
 
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free ebePCG (topo_ebePCG_noalloc): one
+// file-scope device array, byte offsets below. Each GPU holds its own copy
+// for the life of the process; host code reaches it only through the
+// address hipGetSymbolAddress returns on that device.
+constexpr int64_t kNaMaxDof = 262144;   // DOFs (3 x nodes), e.g. a 40 x 40 x 40 grid has 206,763
+constexpr int64_t kNaMaxNel = 131072;   // elements
+constexpr size_t kNaAlign = 256;
+constexpr size_t na_up(size_t b) { return (b + kNaAlign - 1) / kNaAlign * kNaAlign; }
+constexpr size_t kNaOffKe = 0;
+constexpr size_t kNaOffDens = na_up(kNaOffKe + 576 * sizeof(double));
+constexpr size_t kNaOffDensZ = na_up(kNaOffDens + kNaMaxNel * sizeof(double));
+constexpr size_t kNaOffE = na_up(kNaOffDensZ + kNaMaxNel * sizeof(double));
+constexpr size_t kNaOffVec = na_up(kNaOffE + kNaMaxNel * sizeof(double));       // F U invD r z p Ap
+constexpr size_t kNaOffPart = na_up(kNaOffVec + 7 * kNaMaxDof * sizeof(double));
+constexpr size_t kNaPartDoubles = 3 * (size_t)kCgRows * kCgStride > 32 * (size_t)kRedBlocks
+                                      ? 3 * (size_t)kCgRows * kCgStride : 32 * (size_t)kRedBlocks;
+constexpr size_t kNaOffDsc = na_up(kNaOffPart + kNaPartDoubles * sizeof(double));
+constexpr size_t kNaOffSt = na_up(kNaOffDsc + 16 * sizeof(double));
+constexpr size_t kNaOffOc = na_up(kNaOffSt + kMaxRHS * sizeof(CGState));
+constexpr size_t kNaOffTicket = na_up(kNaOffOc + sizeof(OCState));
+constexpr size_t kNaOffFixed = na_up(kNaOffTicket + 4 * sizeof(unsigned));
+constexpr size_t kNaOffFree = na_up(kNaOffFixed + kNaMaxDof);
+constexpr size_t kNaBytes = na_up(kNaOffFree + kNaMaxDof * sizeof(int32_t));
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+static std::mutex g_na_mu;
+constexpr int kNaMaxDevices = 64;
+static unsigned char* g_na_ptr[kNaMaxDevices] = {};
+
+// the device address of `scratch` on the current device (looked up once
+// per device; the array's name is not a device address in host code)
+static unsigned char* na_scratch_ptr() {
+    int dev = 0;
+    TOPO_HIP(hipGetDevice(&dev));
+    TOPO_CHECK(dev >= 0 && dev < kNaMaxDevices, TOPO_E_ARG, "device %d out of range", dev);
+    if (!g_na_ptr[dev]) {
+        void* p = nullptr;
+        TOPO_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        TOPO_CHECK(p != nullptr, TOPO_E_HIP, "hipGetSymbolAddress(scratch) returned no address");
+        g_na_ptr[dev] = (unsigned char*)p;
+    }
+    return g_na_ptr[dev];
+}
+
+// fixed[freedofs[i]] = 0 (entries outside [0, ndof) are skipped)
+__global__ void k_mark_free(const int32_t* __restrict__ freedofs, int64_t nfree, int64_t ndof,
+                            uint8_t* __restrict__ fixed) {
+    for (int64_t i = gtid(); i < nfree; i += gstride()) {
+        const int64_t d = freedofs[i];
+        if (d >= 0 && d < ndof) fixed[d] = 0;
+    }
+}
+
 struct TwinCtx {
     std::unique_ptr<Ctx> c;
     DevKe K0;
@@ -3580,6 +3649,90 @@ int topo_ebePCG(const double* dens, const double* ke, const int32_t* edofs, cons
         C.free_one(l.invD); C.free_one(l.b); C.free_one(l.x); C.free_one(l.x2); C.free_one(l.res);
     }
     for (void* q : tmp) C.free_one(q);
+    TOPO_CHECK(!r.nonfinite, TOPO_E_BREAKDOWN, "non-finite CG scalar");
+    return r.iters;
+    TOPO_API_END
+}
+
+// ebePCG without allocation: every device buffer (inputs, CG vectors,
+// reduction partials, CG state) lives in one file-scope device array; see
+// include/topo.h for the contract and the size limits.
+int topo_ebePCG_noalloc(const double* dens, const double* ke, const int32_t* edofs, const double* Fh, double* Uh,
+                        const int32_t* freedofs, int32_t nel, int32_t edofSize, int32_t ndof, int32_t nfree,
+                        double Emin, double E0, double penal, int32_t maxIter, double tolerance) {
+    TOPO_API_BEGIN
+    TOPO_CHECK(dens && ke && edofs && Fh && Uh && (nfree == 0 || freedofs) && ndof > 0 && nel > 0 && nfree >= 0 &&
+                   nfree <= ndof, TOPO_E_ARG, "topo_ebePCG_noalloc: bad argument");
+    TOPO_CHECK(edofSize == 24, TOPO_E_UNSUPPORTED, "topo_ebePCG_noalloc: edofSize %d (the 3-D hex8 path, 24)",
+               edofSize);
+    TOPO_CHECK(ndof <= kNaMaxDof && nel <= kNaMaxNel, TOPO_E_UNSUPPORTED,
+               "topo_ebePCG_noalloc: %d DOFs / %d elements exceed the scratch limits (%lld DOFs, %lld elements)", ndof,
+               nel, (long long)kNaMaxDof, (long long)kNaMaxNel);
+    Grid gg;
+    TOPO_CHECK(edofs_structured(edofs, ndof, &gg), TOPO_E_UNSTRUCTURED,
+               "edofs is not a structured hex8 grid in the reference numbering");
+    TOPO_CHECK(gg.nel() == nel, TOPO_E_ARG, "nel %d does not match edofs (%lld)", nel, (long long)gg.nel());
+    for (int32_t i = 0; i < nfree; i++)
+        TOPO_CHECK(freedofs[i] >= 0 && freedofs[i] < ndof, TOPO_E_ARG, "freedofs entry out of range");
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    unsigned char* base = na_scratch_ptr();
+    double* d_ke = (double*)(base + kNaOffKe);
+    double* d_dens = (double*)(base + kNaOffDens);
+    double* d_densz = (double*)(base + kNaOffDensZ);
+    double* d_E = (double*)(base + kNaOffE);
+    double* d_F = (double*)(base + kNaOffVec);
+    double* d_U = d_F + kNaMaxDof;
+    double* d_invD = d_U + kNaMaxDof;
+    CGVecs v;
+    v.r = d_invD + kNaMaxDof;
+    v.z = (double*)v.r + kNaMaxDof;
+    v.p = (double*)v.z + kNaMaxDof;
+    v.Ap = (double*)v.p + kNaMaxDof;
+    double* d_part = (double*)(base + kNaOffPart);
+    double* d_dsc = (double*)(base + kNaOffDsc);
+    CGState* d_st = (CGState*)(base + kNaOffSt);
+    OCState* d_oc = (OCState*)(base + kNaOffOc);
+    unsigned* d_ticket = (unsigned*)(base + kNaOffTicket);
+    uint8_t* d_fixed = (uint8_t*)(base + kNaOffFixed);
+    int32_t* d_free = (int32_t*)(base + kNaOffFree);
+    // inputs into the scratch (the reference's order), copies of the caller's arrays
+    TOPO_HIP(hipMemcpy(d_ke, ke, 576 * sizeof(double), hipMemcpyHostToDevice));
+    TOPO_HIP(hipMemcpy(d_dens, dens, (size_t)nel * sizeof(double), hipMemcpyHostToDevice));
+    TOPO_HIP(hipMemcpy(d_F, Fh, (size_t)ndof * sizeof(double), hipMemcpyHostToDevice));
+    TOPO_HIP(hipMemcpy(d_U, Uh, (size_t)ndof * sizeof(double), hipMemcpyHostToDevice));
+    if (nfree > 0) TOPO_HIP(hipMemcpy(d_free, freedofs, (size_t)nfree * sizeof(int32_t), hipMemcpyHostToDevice));
+    // what the previous call left: the reduction ticket must start at zero
+    hipLaunchKernelGGL(k_fill<unsigned>, dim3(1), dim3(64), 0, nullptr, d_ticket, 0u, (int64_t)4, (const int*)nullptr);
+    hipLaunchKernelGGL(k_fill<uint8_t>, dim3(nblk(ndof)), dim3(256), 0, nullptr, d_fixed, (uint8_t)1, (int64_t)ndof,
+                       (const int*)nullptr);
+    if (nfree > 0)
+        hipLaunchKernelGGL(k_mark_free, dim3(nblk(nfree)), dim3(256), 0, nullptr, (const int32_t*)d_free,
+                           (int64_t)nfree, (int64_t)ndof, d_fixed);
+    hipLaunchKernelGGL(k_ref_to_zyx_t<double>, dim3(nblk(nel)), dim3(256), 0, nullptr, (const double*)d_dens, d_densz,
+                       gg);
+    // E via Math.pow (assembly/index.ts:541-550), the same kernels as topo_ebePCG
+    hipLaunchKernelGGL(k_moduli<double>, dim3(nblk(nel)), dim3(256), 0, nullptr, (const double*)d_densz, penal, E0,
+                       Emin, 1, d_E, (float*)nullptr, (int64_t)nel);
+    hipLaunchKernelGGL(k_mask<double>, dim3(nblk(ndof)), dim3(256), 0, nullptr, d_U, (const uint8_t*)d_fixed,
+                       (int64_t)ndof);
+    const double thr = Emin * 1000.0;
+    hipLaunchKernelGGL(k_invdiag, dim3(nblk(gg.nnodes())), dim3(256), 0, nullptr, (const double*)d_E, thr,
+                       (const uint8_t*)d_fixed, d_invD, (float*)nullptr, gg, (const double*)d_ke);
+    TOPO_HIP(hipGetLastError());
+    static CGState h_st[kMaxRHS];
+    static OCState h_oc;
+    static double h_sc[16];
+    Ctx C(Ctx::Borrowed{}, d_part, d_ticket, d_st, d_oc, d_dsc, h_st, &h_oc, h_sc);
+    auto op = [&](const double* p, double* out, int mode, const double* b, const int* gate, double*) {
+        hipLaunchKernelGGL((k_applyA_exact<double, double>), dim3(nblk(gg.nnodes()), 1), dim3(256), 0, nullptr, p, out,
+                           (const double*)d_E, thr, gg, (const uint8_t*)d_fixed, mode, b, (const double*)d_invD,
+                           (const double*)nullptr, gate, (const double*)d_ke);
+        return 0;
+    };
+    const int cgMax = maxIter < nfree ? maxIter : nfree;
+    CGResult r = pcg_jacobi_op<double, double>(C, op, d_fixed, (const double*)d_invD, ndof, (const double*)d_F, d_U,
+                                               true, v, tolerance, cgMax, nullptr, {}, 32);
+    TOPO_HIP(hipMemcpy(Uh, d_U, (size_t)ndof * sizeof(double), hipMemcpyDeviceToHost));
     TOPO_CHECK(!r.nonfinite, TOPO_E_BREAKDOWN, "non-finite CG scalar");
     return r.iters;
     TOPO_API_END
