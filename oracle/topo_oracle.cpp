@@ -36,6 +36,19 @@
 #include <algorithm>
 #include <set>
 #include "../include/topo.h"
+#include <omp.h>
+
+// Threads of the CPU-baseline timing mode (or_set_threads). With 1 (the
+// default, and what every test uses) each loop runs in the order the
+// restatement states; with more, the element scatters run colour by colour
+// (elements of one colour share no node), the vector loops and reductions
+// in parallel, and the coarsest Cholesky by parallel rows: the same
+// algorithm with other summation orders, for the CPU baseline only.
+static int g_threads = 1;
+extern "C" int or_set_threads(int n) {
+    g_threads = n > 0 ? n : omp_get_max_threads();
+    return g_threads;
+}
 
 typedef std::vector<double> dvec;
 
@@ -215,6 +228,29 @@ static double pow_density(double x, double p) {
 }
 
 // applyAEbe3D assembly/index.ts:403-439: Ap = sum over active e of E_e K0 u_e
+// the element scatter Ap += E_e K0 p_e over 8 colours (ex, ey, ez parities):
+// elements of one colour share no node, so each colour runs in parallel
+static void ebe_colored(int nelx, int nely, const double* ke, const double* evals, const int32_t* active,
+                        int64_t activeCount, const int32_t* edof, const double* p, double* Ap) {
+    for (int col = 0; col < 8; col++) {
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+        for (int64_t ai = 0; ai < activeCount; ai++) {
+            const int64_t e = active[ai];
+            const int64_t ex = e % nelx, t = e / nelx, ey = t % nely, ez = t / nely;
+            if ((int)((ex & 1) | ((ey & 1) << 1) | ((ez & 1) << 2)) != col) continue;
+            const double E = evals[e];
+            const int32_t* ed = &edof[e * 24];
+            double loc[24];
+            for (int j = 0; j < 24; j++) loc[j] = p[ed[j]];
+            for (int i = 0; i < 24; i++) {
+                double s = 0.0;
+                for (int j = 0; j < 24; j++) s += ke[i * 24 + j] * loc[j];
+                Ap[ed[i]] += E * s;
+            }
+        }
+    }
+}
+
 int or_applyA(int nelx, int nely, int nelz, const double* ke, const double* evals,
               const int32_t* active, int activeCount, const double* p, double* Ap) {
     const int64_t ndof = 3LL * (nelx + 1) * (nely + 1) * (nelz + 1);
@@ -222,6 +258,10 @@ int or_applyA(int nelx, int nely, int nelz, const double* ke, const double* eval
     std::vector<int32_t> edof(nel * 24);
     or_edofs(nelx, nely, nelz, edof.data());
     for (int64_t i = 0; i < ndof; i++) Ap[i] = 0.0;
+    if (g_threads > 1) {
+        ebe_colored(nelx, nely, ke, evals, active, activeCount, edof.data(), p, Ap);
+        return 0;
+    }
     double loc[24];
     for (int ai = 0; ai < activeCount; ai++) {
         int64_t e = active[ai];
@@ -400,6 +440,11 @@ static void level_applyA(const OrMG& mg, const OrLevel& l, const double* p, doub
         return;
     }
     for (int64_t i = 0; i < l.ndof; i++) Ap[i] = 0.0;
+    if (g_threads > 1) {
+        ebe_colored(l.nelx, l.nely, mg.ke.data(), l.E.data(), l.active.data(), (int64_t)l.active.size(),
+                    l.edof.data(), p, Ap);
+        return;
+    }
     double loc[24];
     for (int32_t e : l.active) {
         double E = l.E[e];
@@ -581,6 +626,7 @@ static int coarse_cholesky(OrMG& mg, OrLevel& c) {
         }
         double d = std::sqrt(s);
         A[j * nf + j] = d;
+#pragma omp parallel for num_threads(g_threads) if (g_threads > 1 && nf - j > 256) schedule(static)
         for (int64_t i = j + 1; i < nf; i++) {
             double t = A[i * nf + j];
             for (int64_t k = 0; k < j; k++) t -= A[i * nf + k] * A[j * nf + k];
@@ -610,7 +656,12 @@ static void coarse_solve_chol(const OrLevel& c, const double* b, double* x) {
 
 static void smooth(const OrMG& mg, OrLevel& l, const double* b, double* x, double omega) {  // mgSmooth :820-834
     level_applyA(mg, l, x, l.Au.data());
-    for (int32_t d : l.freed) x[d] += omega * l.invDiag[d] * (b[d] - l.Au[d]);
+    const int64_t nf = (int64_t)l.freed.size();
+#pragma omp parallel for num_threads(g_threads) if (g_threads > 1) schedule(static)
+    for (int64_t k = 0; k < nf; k++) {
+        const int32_t d = l.freed[k];
+        x[d] += omega * l.invDiag[d] * (b[d] - l.Au[d]);
+    }
 }
 
 // Build-only smoother safeguard (no reference counterpart): estimate
@@ -679,6 +730,7 @@ static void vcycle(OrMG& mg, int li, const double* b, double* x) {
     for (int64_t i = 0; i < c.ndof; i++) if (c.fixed[i]) c.b[i] = 0;
     std::fill(c.x.begin(), c.x.end(), 0.0);
     vcycle(mg, li + 1, c.b.data(), c.x.data());
+#pragma omp parallel for num_threads(g_threads) if (g_threads > 1) schedule(static)
     for (int fz = 0; fz < nzF; fz++) {
         int cz[2]; double wz[2]; pweights(fz, nzC, cz[0], cz[1], wz[0], wz[1]);
         for (int fy = 0; fy < nyF; fy++) {

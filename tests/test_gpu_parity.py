@@ -210,6 +210,49 @@ def test_energy_filters_oc_bitexact(gpu, oracle):
                               oracle.density_filter(nx, ny, nz, 1.5, v, transpose=True))
 
 
+@pytest.mark.parametrize("rmin", [3.0, 4.5])
+def test_wide_filters_bitexact(gpu, oracle, rmin):
+    """Sensitivity and density filters with radii up to the reference UI's
+    5 (prepareFilter takes any rmin, js/optimizer-worker-3d.js:3006-3050):
+    window +-floor(rmin) (sensitivity) and ceil(rmin) (density), bit-exact
+    with the oracle, and three SIMP steps of a 10x6x6 cantilever with the
+    wide sensitivity filter against the oracle loop."""
+    T = gpu
+    import topologyoptimization_amd.problems as P
+    rng = np.random.default_rng(int(rmin * 10))
+    nx, ny, nz = 12, 9, 8
+    nel, ndof = T.dims(nx, ny, nz)
+    fixed = np.zeros(ndof, np.uint8)
+    cfg = T.default_config(nx, ny, nz, linear_solver=T.SOLVER_PCG, rmin=rmin)
+    with T.Solver(cfg, fixed, np.zeros(ndof, np.float32)) as S:
+        x = rng.uniform(0, 1, nel).astype(np.float32)
+        dc = -rng.uniform(0, 3, nel).astype(np.float32)
+        assert np.array_equal(S.filter(T.FILTER_SENSITIVITY, x, dc), oracle.sens_filter(nx, ny, nz, rmin, x, dc))
+    cfg = T.default_config(nx, ny, nz, filter_type=T.FILTER_DENSITY, rmin=rmin)
+    with T.Solver(cfg, fixed, np.zeros(ndof, np.float32)) as S:
+        v = rng.uniform(0, 1, nel).astype(np.float32)
+        assert np.array_equal(S.filter(T.FILTER_DENSITY, None, v), oracle.density_filter(nx, ny, nz, rmin, v))
+        assert np.array_equal(S.filter_adjoint(T.FILTER_DENSITY, v),
+                              oracle.density_filter(nx, ny, nz, rmin, v, transpose=True))
+    cfg, fixed, F, pres, void, x0 = P.cantilever(10, 6, 6, linear_solver=T.SOLVER_PCG, rmin=rmin, cg_tol=1e-8,
+                                                 cg_max_iter=4000, volfrac=0.3)
+    ref = oracle.simp_run(cfg, fixed, F, None, None, 3, x0)
+    with T.Solver(cfg, fixed, F) as S:
+        comp = np.array([S.step().compliance for _ in range(3)])
+        xg = S.design()
+    assert np.abs(comp - ref["compliance"]).max() <= 1e-6 * np.abs(ref["compliance"]).max()
+    assert np.abs(xg - ref["x"]).max() <= 1e-3
+
+
+def test_filter_radius_beyond_window_rejected(gpu):
+    T = gpu
+    cfg = T.default_config(8, 4, 4, rmin=6.0)
+    _, ndof = T.dims(8, 4, 4)
+    with pytest.raises(T.TopoError) as e:
+        T.Solver(cfg, np.zeros(ndof, np.uint8), np.zeros(ndof, np.float32))
+    assert e.value.code == -8
+
+
 def test_simp_c1_trajectory(gpu, oracle):
     """5x5x5 reference cube, 30 SIMP iterations, Jacobi-PCG FP64 tol 1e-8:
     compliance within 1e-6 relative at every iteration, final densities

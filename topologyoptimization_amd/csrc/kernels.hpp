@@ -1299,11 +1299,13 @@ __global__ void __launch_bounds__(256) k_energy_sens(const double* __restrict__ 
 // over the prepareFilter neighbourhood :3006-3050): for each element the
 // neighbours are visited x-major, then y, then z, accumulating in Float32
 // with FP64 products; Hs is precomputed the same way.
+constexpr int kFilterMaxR = 5;   // window half width up to 5 (filterRadius up to 5 in the reference UI)
+constexpr int kFilterMaxCount = (2 * kFilterMaxR + 1) * (2 * kFilterMaxR + 1) * (2 * kFilterMaxR + 1);
 struct FilterStencil {
     int r;            // window half width
     int count;        // number of (dx,dy,dz) with dist <= rmin
-    int off[125][3];
-    double w[125];    // rmin - dist
+    int off[kFilterMaxCount][3];
+    double w[kFilterMaxCount];    // rmin - dist
 };
 
 __global__ void k_filter_hs(float* __restrict__ Hs, Grid g, const FilterStencil* __restrict__ st) {

@@ -2355,7 +2355,8 @@ struct Solver {
         if (cfg.filter_type == TOPO_FILTER_DENSITY) {
             // applyDensityFilter window ceil(r), d^2 <= r^2, order z, y, x
             const int r = (int)std::ceil(rmin);
-            TOPO_CHECK(r <= 2, TOPO_E_UNSUPPORTED, "density filter radius > 2 not supported");
+            TOPO_CHECK(rmin > 0 && r <= kFilterMaxR, TOPO_E_UNSUPPORTED,
+                       "density filter radius %g (0 < rmin <= %d)", rmin, kFilterMaxR);
             st.r = r;
             const double r2 = rmin * rmin;
             for (int kz = -r; kz <= r; kz++)
@@ -2371,7 +2372,8 @@ struct Solver {
         } else {
             // prepareFilter window +-floor(rmin), dist <= rmin, order x, y, z
             const int r = (int)std::floor(rmin);
-            TOPO_CHECK(r <= 2, TOPO_E_UNSUPPORTED, "sensitivity filter radius >= 3 not supported");
+            TOPO_CHECK(rmin > 0 && r <= kFilterMaxR, TOPO_E_UNSUPPORTED,
+                       "sensitivity filter radius %g (0 < rmin < %d)", rmin, kFilterMaxR + 1);
             st.r = r;
             for (int kx = -r; kx <= r; kx++)
                 for (int ky = -r; ky <= r; ky++)

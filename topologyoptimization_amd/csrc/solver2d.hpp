@@ -120,7 +120,8 @@ struct Solver2D {
         FilterStencil st;
         std::memset(&st, 0, sizeof st);
         const int r = (int)std::floor(cfg.rmin);
-        TOPO_CHECK(cfg.rmin > 0 && r <= 5, TOPO_E_UNSUPPORTED, "2-D filter radius %g (0 < rmin < 6)", cfg.rmin);
+        TOPO_CHECK(cfg.rmin > 0 && r <= kFilterMaxR, TOPO_E_UNSUPPORTED, "2-D filter radius %g (0 < rmin < %d)",
+                   cfg.rmin, kFilterMaxR + 1);
         st.r = r;
         for (int kx = -r; kx <= r; kx++)
             for (int ky = -r; ky <= r; ky++) {
