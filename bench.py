@@ -535,9 +535,10 @@ def simp2d_leg(T, grid=(1024, 512), steps=5, warmup=1, cpu=True):
 
 
 def profile_kernels(args):
-    """Fixed kernel workload for rocprofv3 counter runs: 20 fine applyA in
-    FP32, 20 in FP64 and 20 plain FP64 copies (the calibration kernel) at the
-    benchmark grid."""
+    """Fixed kernel workload for rocprofv3 counter runs at the benchmark
+    grid, 20 launches each: the fine applyA in FP32 (mode 0), in FP64
+    (mode 0), the CG's FP64 operator (masked, p.Ap partials: mode 1), and a
+    plain FP64 copy of one vector (the calibration kernel)."""
     import topologyoptimization_amd as T
     import topologyoptimization_amd.problems as P
     grid = tuple(int(v) for v in args.grid.split(","))
@@ -550,6 +551,7 @@ def profile_kernels(args):
     S.set_density(x0, 3.0)
     out = {}
     for name, op, prec in (("applyA_fp32", T.TIME_APPLYA, T.PREC_MIXED), ("applyA_fp64", T.TIME_APPLYA, T.PREC_FP64),
+                           ("applyA_fp64_masked", T.TIME_APPLYA_MASKED, T.PREC_FP64),
                            ("copy_fp64", T.TIME_COPY, T.PREC_FP64)):
         ms, by = S.time_op(op, prec, 0, 20)
         out[name] = {"avg_ms": ms, "algorithmic_bytes": by}

@@ -122,23 +122,26 @@ def test_compliance_gradient_is_reference_sensitivity(gpu, oracle):
     assert np.allclose(dJ, ref, rtol=1e-6, atol=0)
 
 
-@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("prec", [0, 1, 2])
 def test_inverter_simp_steps(gpu, prec):
-    """Five SIMP steps of the inverter (forward + adjoint solve, damped OC):
-    the first step drives the output displacement down (5e-3 -> ~1e-8), every
-    solve converges, and the design volume never exceeds volfrac (with
-    positive sensitivities the damped OC cannot always reach it from below).
-    Once the objective is near zero its gradient is at the level of the
-    solves' rounding; the FP64 trajectory stays there, while the one with an
-    FP32 inner CG (PREC_MIXED) wanders with any change of summation order,
-    so only the FP64 run is held to ending below its start."""
+    """Thirty SIMP steps of the quarter-model force inverter (forward +
+    adjoint solve, damped OC, move 0.1) at 32x16x16: the first step lowers
+    the output displacement, every solve converges, the design volume never
+    exceeds volfrac (with positive sensitivities the damped OC cannot always
+    reach it from below). With the FP32 V-cycle (2) or the FP32 inner CG (1)
+    the inverting lever forms (u_out well below zero) within the 30 steps;
+    the FP64 run hovers at |u_out| ~ 1e-9 .. 1e-4, where the gradient is at
+    the level of the solves' rounding, so which way it leaves that plateau
+    depends on summation order: it is held to never rising above its start."""
     T = gpu
     cfg, fixed, F, _, _, x0 = P.inverter(32, 16, 16, precision=prec, cg_max_iter=2000)
     with T.Solver(cfg, fixed, F) as S:
-        infos = [S.step() for _ in range(5)]
+        infos = [S.step() for _ in range(30)]
     J = [i.compliance for i in infos]
-    assert J[1] < 1e-3 * J[0], J
-    if prec == T.PREC_FP64:
-        assert J[-1] < J[0], J
+    print("u_out:", " ".join("%.2e" % j for j in J))
+    assert J[1] < J[0], J
+    assert max(J) <= J[0]
+    if prec != T.PREC_FP64:
+        assert min(J) < -1e-2, J
     assert all(i.volume <= cfg.volfrac + 1e-3 for i in infos)
     assert all(i.cg_iters > 0 and i.residual <= 1e-6 for i in infos)

@@ -389,6 +389,7 @@ struct MGLevel {
     int tnrb = 0;                  // row blocks of the transposed FP16 product (column tiles of Z16)
     double* d0c = nullptr;         // original diagonal (pivot test)
     double* ztmp = nullptr;        // 64 x nfp scratch of the Z row blocks
+    double *gj_d0 = nullptr, *gj_c = nullptr, *gj_r = nullptr;   // Gauss-Jordan fallback scratch (kept)
     double* work = nullptr;
     // nested-dissection order (nd): a separator tree in postorder over the
     // free DOFs, one dense front per node; Z = L^-1 is zero left of the
@@ -1339,7 +1340,7 @@ struct MGH {
         if (!done) {
             // not positive definite (or too small a block to band): dense
             // Gauss-Jordan inverse with pivot dropping
-            gauss_jordan(c, l.Kinv, n);
+            gauss_jordan(c, l, l.Kinv, n);
             if (l.Kinv32)
                 hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(n * n)), dim3(256), 0, c.s, l.Kinv32,
                                    (const double*)l.Kinv, n * n);
@@ -1528,17 +1529,21 @@ struct MGH {
 
     // in-place inverse of the n x n block A (row-major, ld n) by blocked
     // Gauss-Jordan with pivot dropping
-    void gauss_jordan(Ctx& c, double* A, int64_t nf) {
-        double* d0 = c.alloc<double>(nf);
+    // the scratch (original diagonal, panel copies) is allocated on the
+    // level's first fallback and kept for later ones
+    void gauss_jordan(Ctx& c, MGLevel& l, double* A, int64_t nf) {
+        if (!l.gj_d0) l.gj_d0 = c.alloc<double>(nf);
+        double* d0 = l.gj_d0;
         hipLaunchKernelGGL(k_diag_copy, dim3(nblk(nf)), dim3(256), 0, c.s, (const double*)A, nf, d0);
         const int nt = (int)((nf + kGJB - 1) / kGJB);
         // large blocks: the rank-64 update as a plain rocBLAS DGEMM
         const bool use_blas = nf > 1024;
-        double *Cp = nullptr, *Rp = nullptr;
-        if (use_blas) {
-            Cp = c.alloc<double>(nf * kGJB);
-            Rp = c.alloc<double>(nf * kGJB);
+        if (use_blas && !l.gj_c) {
+            l.gj_c = c.alloc<double>(nf * kGJB);
+            l.gj_r = c.alloc<double>(nf * kGJB);
         }
+        double* Cp = use_blas ? l.gj_c : nullptr;
+        double* Rp = use_blas ? l.gj_r : nullptr;
         for (int64_t k0 = 0; k0 < nf; k0 += kGJB) {
             const int kb = (int)std::min<int64_t>(kGJB, nf - k0);
             hipLaunchKernelGGL(k_bgj_diag, dim3(1), dim3(256), 0, c.s, A, nf, k0, kb, (const double*)d0);
@@ -1558,11 +1563,6 @@ struct MGH {
             hipLaunchKernelGGL(k_bgj_col, dim3(nt), dim3(256), 0, c.s, A, nf, k0, kb);
         }
         c.sync();
-        if (use_blas) {
-            c.free_one(Cp);
-            c.free_one(Rp);
-        }
-        c.free_one(d0);
     }
 
     // Box partition of the structured grid (KSPSolver._buildSubdomains
