@@ -304,14 +304,16 @@ int topo_solver_get_field(topo_solver* s, int32_t field, void* out);
  * operation carries all of them, the coarsest-level inverse is read once per
  * group, each case keeps its own CG scalars and stops on its own). rhs_tile
  * 0 autotunes at the first solve: the x-march element-tile shape of the two
- * finest levels (x-planes per workgroup CX in {auto, 2, 3, 4, 5, 6, 8, 10,
- * 12, 16}; results are bit-identical for every CX) and, with several load
- * cases, the group size R in {1, 2, 4} with the least time per right-hand
- * side for the fine operator plus one V-cycle. This reports the choice
- * (rhs_tile 1 when no batching applies, NT/CX of the finest level) and the
- * timings: log receives max_entries x {what, NT, CX, ms}, what = the MG level
- * of a tile timing (ms per launch) or -R for a batch timing (ms per
- * right-hand side). Returns the number of tuning entries. */
+ * finest levels (x-planes per workgroup CX in {auto, 2, 4, 6, 8, 10, 12, 16,
+ * 18, 20, 24, 32}; results are bit-identical for every CX), in the mixed
+ * precisions also that of the FP64 CG operator on the finest level, and, with
+ * several load cases, the group size R in {1, 2, 4} with the least time per
+ * right-hand side for the fine operator plus one V-cycle. This reports the
+ * choice (rhs_tile 1 when no batching applies, NT/CX of the finest level)
+ * and the timings: log receives max_entries x {what, NT, CX, ms}, what = the
+ * MG level of a tile timing (ms per launch), 1000 for a timing of the FP64
+ * CG operator (ms per launch) or -R for a batch timing (ms per right-hand
+ * side). Returns the number of tuning entries. */
 int topo_solver_tuning(topo_solver* s, int32_t* rhs_tile, int32_t* xm_nt, int32_t* xm_cx, double* log,
                        int32_t max_entries);
 

@@ -130,9 +130,13 @@ def test_inverter_simp_steps(gpu, prec):
     exceeds volfrac (with positive sensitivities the damped OC cannot always
     reach it from below). With the FP32 V-cycle (2) or the FP32 inner CG (1)
     the inverting lever forms (u_out well below zero) within the 30 steps;
-    the FP64 run hovers at |u_out| ~ 1e-9 .. 1e-4, where the gradient is at
-    the level of the solves' rounding, so which way it leaves that plateau
-    depends on summation order: it is held to never rising above its start."""
+    once formed, the damped OC at move 0.1 may settle into a two-cycle
+    between an inverting design and its neighbour (u_out about -2e-2 /
+    +2e-2 at this size, depending on rounding), so later steps are not held
+    below the start. The FP64 run hovers at |u_out| ~ 1e-9 .. 1e-4, where the
+    gradient is at the level of the solves' rounding, so which way it leaves
+    that plateau depends on summation order: it is held to never rising above
+    its start."""
     T = gpu
     cfg, fixed, F, _, _, x0 = P.inverter(32, 16, 16, precision=prec, cg_max_iter=2000)
     with T.Solver(cfg, fixed, F) as S:
@@ -140,8 +144,9 @@ def test_inverter_simp_steps(gpu, prec):
     J = [i.compliance for i in infos]
     print("u_out:", " ".join("%.2e" % j for j in J))
     assert J[1] < J[0], J
-    assert max(J) <= J[0]
     if prec != T.PREC_FP64:
         assert min(J) < -1e-2, J
+    else:
+        assert max(J) <= J[0], J
     assert all(i.volume <= cfg.volfrac + 1e-3 for i in infos)
     assert all(i.cg_iters > 0 and i.residual <= 1e-6 for i in infos)

@@ -110,9 +110,8 @@ __device__ __forceinline__ void had_block(T (&A)[12], T (&Bq)[12], const HadCoef
     }
 }
 
-// dynamic LDS: s_in[(NT + H) * 3] | s_q[NT * 9]
-template <int NT>
-__host__ __device__ inline size_t xm_lds_bytes(int nnz, size_t elt) {
+// dynamic LDS of an NT-thread workgroup: s_in[(NT + H) * 3] | s_q[NT * 9]
+inline size_t xm_lds_bytes(int nnz, size_t elt, int NT) {
     return ((size_t)(NT + nnz + 1) * 3 + (size_t)NT * 9) * elt;
 }
 
@@ -249,13 +248,19 @@ __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __
         yz_forward<T>(own, n01, n10, n11, f);
     };
 
-    T fprev[12], carry[12];
+    // face transforms of the two node planes around the current element
+    // plane and the carried trailing-face outputs, in two register sets that
+    // trade roles every step (the march is unrolled by two below, so no
+    // register copies at the loop edge)
+    T fa[12], fb[12], ca[12], cb[12];
     double cpair = 0.0;   // this thread's share of the current plane pair (MODE 1, 5)
 #pragma unroll
-    for (int k = 0; k < 12; k++) carry[k] = T(0);
+    for (int k = 0; k < 12; k++) ca[k] = T(0);
     fetch(X0 - 1);
-    stage_face(X0, fprev);
-    for (int ex = X0 - 1; ex < X1; ex++) {
+    stage_face(X0, fa);
+    // one x step: element plane ex between node planes ex (face fprev) and
+    // ex+1 (staged here into fnew); completes output node plane ex
+    auto step = [&](const int ex, const T (&fprev)[12], T (&fnew)[12], const T (&cin)[12], T (&cout)[12]) {
         // operands of this step's output epilogue (node plane ex), loaded
         // before the element work so their latency overlaps it
         const bool outstep = is_out && ex >= X0;
@@ -278,19 +283,15 @@ __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __
             }
         }
         T A[12], Bq[12];
-        {
-            T fnew[12];
-            // reads of s_in for the previous plane finished before the last
-            // barrier of the previous step; only the first step follows the
-            // pre-loop staging directly
-            if (ex == X0 - 1) __syncthreads();
-            stage_face(ex + 2, fnew);
+        // reads of s_in for the previous plane finished before the last
+        // barrier of the previous step; only the first step follows the
+        // pre-loop staging directly
+        if (ex == X0 - 1) __syncthreads();
+        stage_face(ex + 2, fnew);
 #pragma unroll
-            for (int k = 0; k < 12; k++) {
-                A[k] = fprev[k] + fnew[k];
-                Bq[k] = fprev[k] - fnew[k];
-                fprev[k] = fnew[k];
-            }
+        for (int k = 0; k < 12; k++) {
+            A[k] = fprev[k] + fnew[k];
+            Bq[k] = fprev[k] - fnew[k];
         }
         T Ee = T(0);
         if (elem_yz && ex >= 0 && ex < g.nelx) {
@@ -304,8 +305,8 @@ __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __
         for (int k = 0; k < 12; k++) {
             const T f0 = A[k] + Bq[k];
             const T f1 = A[k] - Bq[k];
-            nv[k] = carry[k] + f0;
-            carry[k] = f1;
+            nv[k] = cin[k] + f0;
+            cout[k] = f1;
         }
         // inverse y/z butterflies of the face at node plane ex -> 4 node values
         T q00[3];
@@ -356,7 +357,13 @@ __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __
                 cpair = 0.0;
             }
         }
+    };
+    int ex = X0 - 1;
+    for (; ex + 1 < X1; ex += 2) {
+        step(ex, fa, fb, ca, cb);
+        step(ex + 1, fb, fa, cb, ca);
     }
+    if (ex < X1) step(ex, fa, fb, ca, cb);
     if ((MODE == 1 || MODE == 5) && part) {
         __syncthreads();
         for (int k = t; 2 * k < X1 - X0; k += NT) {

@@ -920,8 +920,12 @@ struct MGH {
     bool exact = false;   // reference-order kernel everywhere
     int xm_nt = 0;        // x-march workgroup size for every level (0 auto, 512, 1024)
     int xm_cx = 0;        // x-march planes per workgroup for every level (0 auto)
+    int xm_cx64 = 0;      // x-march planes per workgroup of every FP64 operator (0: autotuned / as FP32)
     bool rz_fuse = true;  // r.z partials from the V-cycle's last sweep (else a separate kernel)
     std::vector<int> lv_nt, lv_cx;   // per-level choices of the autotuner (0 auto)
+    // x-chunk of the FP64 operator where the autotuner timed it apart from
+    // the level's FP32 one (the mixed solvers' CG operator; 0: lv_cx)
+    std::vector<int> lv_cx64;
     DevKe* K0 = nullptr;
     Springs springs{};    // mechanism ports on the finest level
     int nrhs = 1;         // right-hand sides a batched V-cycle can carry (level vectors are nrhs x ndof)
@@ -968,10 +972,12 @@ struct MGH {
                 (l.Z32 ? 32 : 0) | (l.Kinv32 ? 64 : 0));
             put(li < lv_nt.size() ? lv_nt[li] : -1);
             put(li < lv_cx.size() ? lv_cx[li] : -1);
+            put(li < lv_cx64.size() ? lv_cx64[li] : -1);
         }
         put((exact ? 1 : 0) | (rz_fuse ? 2 : 0) | (bddc ? 4 : 0));
         put(xm_nt);
         put(xm_cx);
+        put(xm_cx64);
         put(cg_check);
         put(jcg_check);
         put(coarse_fp32);
@@ -1635,14 +1641,29 @@ struct MGH {
         return z;
     }
 
-    // whether level li's operator is the x-march kernel (mode 4 needs it)
+    // x-march workgroup size of level li's operator in precision T
+    template <typename T>
+    int xm_pick_nt(int li) const {
+        const int H = L[li].g.nnz() + 1;
+        const int nt_sel = xm_nt ? xm_nt : (li < (int)lv_nt.size() ? lv_nt[li] : 0);
+        return (nt_sel == 1024 || (nt_sel == 0 && 4 * H > 512)) ? 1024 : 512;
+    }
+
+    // x-planes per workgroup of level li's x-march operator in precision T (0 auto)
+    template <typename T>
+    int xm_pick_cx(int li) const {
+        if (sizeof(T) == 8 && xm_cx64) return xm_cx64;
+        if (xm_cx) return xm_cx;
+        if (sizeof(T) == 8 && li < (int)lv_cx64.size() && lv_cx64[li]) return lv_cx64[li];
+        return li < (int)lv_cx.size() ? lv_cx[li] : 0;
+    }
+
+    // whether level li's operator in precision T is the x-march kernel (mode 4 needs it)
+    template <typename T>
     bool xm_level(int li) const {
         const MGLevel& l = L[li];
         if (l.dense || exact || !K0->had_ok) return false;
-        const int H = l.g.nnz() + 1;
-        const int nt_sel = xm_nt ? xm_nt : (li < (int)lv_nt.size() ? lv_nt[li] : 0);
-        const int NT = (nt_sel == 1024 || (nt_sel == 0 && 4 * H > 512)) ? 1024 : 512;
-        return 2 * H <= NT;
+        return 2 * (l.g.nnz() + 1) <= xm_pick_nt<T>(li);
     }
 
     // out = A in (mode 0), masked (1), b - A in (2), Jacobi sweep (3), the
@@ -1656,9 +1677,9 @@ struct MGH {
               double* pap = nullptr, const double* rdot = nullptr) {
         int npart = 0;
         MGLevel& l = L[li];
-        if (mode == 5 && !(rdot && pap && xm_level(li) && !(li == 0 && springs.n > 0))) mode = 3;
+        if (mode == 5 && !(rdot && pap && xm_level<T>(li) && !(li == 0 && springs.n > 0))) mode = 3;
         const double* om = mode >= 3 ? omp(li) : nullptr;
-        TOPO_CHECK(mode != 4 || (xm_level(li) && !(li == 0 && springs.n > 0)), TOPO_E_UNSUPPORTED,
+        TOPO_CHECK(mode != 4 || (xm_level<T>(li) && !(li == 0 && springs.n > 0)), TOPO_E_UNSUPPORTED,
                    "fused first sweep on a level without the x-march operator");
         if (l.dense) {
             hipLaunchKernelGGL(k_dense_apply<T>, dim3(nblk(l.ndof * 64), nr), dim3(256), 0, c.s, Sel<T>::K(l), in, out, l.ndof,
@@ -1666,9 +1687,8 @@ struct MGH {
         } else {
             typedef typename Sel<T>::TE TE;
             const int H = l.g.nnz() + 1;
-            const int nt_sel = xm_nt ? xm_nt : (li < (int)lv_nt.size() ? lv_nt[li] : 0);
-            const int cx_sel = xm_cx ? xm_cx : (li < (int)lv_cx.size() ? lv_cx[li] : 0);
-            const int NT = (nt_sel == 1024 || (nt_sel == 0 && 4 * H > 512)) ? 1024 : 512;
+            const int cx_sel = xm_pick_cx<T>(li);
+            const int NT = xm_pick_nt<T>(li);
             if (!exact && K0->had_ok && 2 * H <= NT) {
                 const int nyz = l.g.nny() * l.g.nnz();
                 const int gx = (nyz + (NT - H) - 1) / (NT - H);
@@ -1683,8 +1703,7 @@ struct MGH {
                                 !(li == 0 && springs.n > 0)) ? pap : nullptr;
                 if (mode == 5 && !part) mode = 3;
                 if (part) npart = (int)nplane;
-                const size_t lds = NT == 512 ? xm_lds_bytes<512>(l.g.nnz(), sizeof(T))
-                                             : xm_lds_bytes<1024>(l.g.nnz(), sizeof(T));
+                const size_t lds = xm_lds_bytes(l.g.nnz(), sizeof(T), NT);
 #define TOPO_XM_LAUNCH(NTV, MV)                                                                                  \
     hipLaunchKernelGGL((k_applyA_xm<T, TE, NTV, MV>), grid, dim3(NTV), lds, c.s, in, out,                      \
                        (const TE*)Sel<T>::E(l), thr, l.g, (const uint8_t*)l.fixed, b, (const T*)Sel<T>::invD(l), \
@@ -1800,7 +1819,7 @@ struct MGH {
         }
         const int nsweep_pre = coarsest ? nu1 + coarse_sweeps : nu1;
         int s0 = 1;
-        if (nsweep_pre >= 2 && xm_level(li) && !(li == 0 && springs.n > 0)) {
+        if (nsweep_pre >= 2 && xm_level<T>(li) && !(li == 0 && springs.n > 0)) {
             // the first two sweeps in one pass (x0 = omega invD b formed on the fly)
             apply<T>(c, li, (const T*)b, x2, 4, b, gate, nr);
             std::swap(x, x2);
@@ -2291,6 +2310,7 @@ struct Solver {
         mg.exact = cfg.exact_apply != 0;
         if (const char* e = getenv("TOPO_XM_NT")) mg.xm_nt = atoi(e);
         if (const char* e = getenv("TOPO_XM_CX")) mg.xm_cx = atoi(e);
+        if (const char* e = getenv("TOPO_XM_CX64")) mg.xm_cx64 = atoi(e);
         if (const char* e = getenv("TOPO_RZ_FUSE")) mg.rz_fuse = atoi(e) != 0;
         if (const char* e = getenv("TOPO_ND_DEPTH")) mg.nd_depth = atoi(e);
         if (const char* e = getenv("TOPO_ND_MIN")) mg.nd_min = atoll(e);
@@ -2519,10 +2539,14 @@ struct Solver {
             return (double)ms / reps;
         };
         tune_log.clear();
-        const int cx_c[8] = {0, 2, 4, 6, 8, 10, 12, 16};   // even (the CG partials cover node-plane pairs)
+        // even (the CG partials cover node-plane pairs); the large ones let a
+        // launch fit the chip in one round of workgroups
+        constexpr int kNcx = 12;
+        const int cx_c[kNcx] = {0, 2, 4, 6, 8, 10, 12, 16, 18, 20, 24, 32};
         const int nl = (int)mg.L.size();
         mg.lv_nt.assign(nl, 0);
         mg.lv_cx.assign(nl, 0);
+        mg.lv_cx64.assign(nl, 0);
         if (!mg.xm_nt && !mg.xm_cx && !mg.exact) {
             for (int li = 0; li < std::min(nl, 2); li++) {
                 MGLevel& l = mg.L[li];
@@ -2536,7 +2560,7 @@ struct Solver {
                 double best = 1e300;
                 const int H = l.g.nnz() + 1;
                 const int nt_rule = 4 * H > 512 ? 1024 : 512;
-                for (int b = 0; b < 8; b++) {
+                for (int b = 0; b < kNcx; b++) {
                     mg.lv_nt[li] = 0;
                     mg.lv_cx[li] = cx_c[b];
                     const double ms = time_it([&]() {
@@ -2548,6 +2572,21 @@ struct Solver {
                 }
                 mg.lv_nt[li] = 0;
                 mg.lv_cx[li] = tuned_cx[li];
+            }
+            // the mixed solvers' CG operator (FP64, masked, finest level):
+            // its own x-chunk, as FP64 arithmetic and registers give it
+            // another best tile than the FP32 smoothers (log entries 1000)
+            MGLevel& l = mg.L[0];
+            if (f32 && !mg.xm_cx64 && !l.dense && mg.xm_level<double>(0)) {
+                double best = 1e300;
+                const int nt = mg.xm_pick_nt<double>(0);
+                for (int b = 0; b < kNcx; b++) {
+                    mg.lv_cx64[0] = cx_c[b];
+                    const double ms = time_it([&]() { mg.apply<double>(C, 0, l.x, l.x2, 1, nullptr, nullptr); }, 5);
+                    tune_log.push_back({1000, nt, cx_c[b], ms});
+                    if (ms < best) { best = ms; tuned_cx64 = cx_c[b]; }
+                }
+                mg.lv_cx64[0] = tuned_cx64;
             }
         }
         rhs_tile = 1;
@@ -2570,7 +2609,7 @@ struct Solver {
         C.graph_clear();
         tuned = true;
     }
-    int tuned_nt[2] = {0, 0}, tuned_cx[2] = {0, 0};
+    int tuned_nt[2] = {0, 0}, tuned_cx[2] = {0, 0}, tuned_cx64 = 0;
     struct TuneEntry {
         int what;          // >= 0: tile timing of that MG level; < 0: -R, batch of R right-hand sides
         int nt, cx;
