@@ -196,6 +196,8 @@ def main():
     ms64, by64 = S.time_op(T.TIME_APPLYA, T.PREC_FP64, 0, 30)
     # the CG's FP64 operator (fixed DOFs masked, p.Ap partials in the epilogue)
     ms64m, by64m = S.time_op(T.TIME_APPLYA_MASKED, T.PREC_FP64, 0, 30)
+    # the same FP32 applyA with the element product on the matrix cores
+    msmf, bymf = S.time_op(T.TIME_APPLYA_MFMA, T.PREC_MIXED, 0, 50)
     msc, byc = S.time_op(T.TIME_COPY, T.PREC_FP64, 0, 30)
     msv, _ = S.time_op(T.TIME_VCYCLE, T.PREC_MIXED, 0, 5)
     S.close()
@@ -251,9 +253,14 @@ def main():
         },
         "gdof_matvec_per_s": {"fp32": ndof / (ms32 * 1e-3) / 1e9, "fp64": ndof / (ms64 * 1e-3) / 1e9,
                               "fp64_masked_pap": ndof / (ms64m * 1e-3) / 1e9},
-        "applyA_ms": {"fp32": ms32, "fp64": ms64, "fp64_masked_pap": ms64m},
+        "applyA_ms": {"fp32": ms32, "fp64": ms64, "fp64_masked_pap": ms64m, "fp32_mfma": msmf},
         "applyA_hbm_frac": {"fp32": by32 / (ms32 * 1e-3) / 8e12, "fp64": by64 / (ms64 * 1e-3) / 8e12,
-                            "fp64_masked_pap": by64m / (ms64m * 1e-3) / 8e12},
+                            "fp64_masked_pap": by64m / (ms64m * 1e-3) / 8e12, "fp32_mfma": bymf / (msmf * 1e-3) / 8e12},
+        # matrix-core work of the split-bfloat16 element products: 3 products
+        # of the 32x32 padded K0 with each element's 32-vector
+        "mfma_apply": {"matrix_flops_per_launch": 6144.0 * nel,
+                       "tflops": 6144.0 * nel / (msmf * 1e-3) / 1e12,
+                       "frac_of_bf16_dense_peak": 6144.0 * nel / (msmf * 1e-3) / 2.5e15},
         "vcycle_ms_fp32": msv,
         "copy_GBps": byc / (msc * 1e-3) / 1e9,
         "fine_applyA_per_step": fine_calls,
@@ -536,9 +543,10 @@ def simp2d_leg(T, grid=(1024, 512), steps=5, warmup=1, cpu=True):
 
 def profile_kernels(args):
     """Fixed kernel workload for rocprofv3 counter runs at the benchmark
-    grid, 20 launches each: the fine applyA in FP32 (mode 0), in FP64
-    (mode 0), the CG's FP64 operator (masked, p.Ap partials: mode 1), and a
-    plain FP64 copy of one vector (the calibration kernel)."""
+    grid, 20 launches each: the fine applyA in FP32 (mode 0, VALU and
+    matrix-core kernels), in FP64 (mode 0), the CG's FP64 operator (masked,
+    p.Ap partials: mode 1), and a plain FP64 copy of one vector (the
+    calibration kernel)."""
     import topologyoptimization_amd as T
     import topologyoptimization_amd.problems as P
     grid = tuple(int(v) for v in args.grid.split(","))
@@ -548,9 +556,12 @@ def profile_kernels(args):
         cfg, fixed, F, pres, void, x0 = P.cantilever(*grid, precision=T.PREC_MIXED)
         x0 = P.splitmix64_uniform(T.dims(*grid)[0])
     S = T.Solver(cfg, fixed, F)
-    S.set_density(x0, 3.0)
+    # one FE solve at the design: the autotuner picks the element tiles the
+    # solves use
+    S.fe(x0, 3.0)
     out = {}
-    for name, op, prec in (("applyA_fp32", T.TIME_APPLYA, T.PREC_MIXED), ("applyA_fp64", T.TIME_APPLYA, T.PREC_FP64),
+    for name, op, prec in (("applyA_fp32", T.TIME_APPLYA, T.PREC_MIXED), ("applyA_fp32_mfma", T.TIME_APPLYA_MFMA, T.PREC_MIXED),
+                           ("applyA_fp64", T.TIME_APPLYA, T.PREC_FP64),
                            ("applyA_fp64_masked", T.TIME_APPLYA_MASKED, T.PREC_FP64),
                            ("copy_fp64", T.TIME_COPY, T.PREC_FP64)):
         ms, by = S.time_op(op, prec, 0, 20)

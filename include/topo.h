@@ -121,7 +121,13 @@ typedef struct topo_config {
                                    makes `change` NaN and ends the loop; 1: energies
                                    below 0 are set to 0 (K0 is positive semidefinite, so
                                    a negative sum is rounding)                        */
-    int32_t reserved[6];
+    int32_t mfma_apply;         /* FP32 operator (V-cycle smoothers, FP32 CG) on the x-march
+                                   levels: 0 (default) the FP32 VALU kernel (Hadamard
+                                   form); 1 the element product on the matrix cores,
+                                   operands split into two bfloat16 parts (about 16
+                                   significant bits) with FP32 accumulation (slower at
+                                   192x96x96: 28 against 21 us, bound by its LDS traffic) */
+    int32_t reserved[5];
 } topo_config;
 
 /* Fill *cfg with the reference defaults (js/optimizer-worker-3d.js:20-64,
@@ -368,7 +374,8 @@ int topo_op_oc(topo_solver* s, const float* x, const float* dcn, float* xnew,
 /* ---- timing (HIP events on the stream the kernels run on) ------------ */
 enum { TOPO_TIME_APPLYA = 0, TOPO_TIME_VCYCLE = 1, TOPO_TIME_ENERGY = 2,
        TOPO_TIME_FILTER = 3, TOPO_TIME_OC = 4, TOPO_TIME_COPY = 5,
-       TOPO_TIME_APPLYA_MASKED = 6 /* the CG's operator: fixed DOFs masked, p.Ap partials */ };
+       TOPO_TIME_APPLYA_MASKED = 6 /* the CG's operator: fixed DOFs masked, p.Ap partials */,
+       TOPO_TIME_APPLYA_MFMA = 7   /* FP32 applyA on the matrix cores (as mfma_apply = 1) */ };
 /* run `op` reps times back to back on device-resident data and return the
  * average duration of one launch in ms (and the algorithmic bytes of one). */
 int topo_time_op(topo_solver* s, int32_t op, int32_t prec, int32_t level, int32_t reps,

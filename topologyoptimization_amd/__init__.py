@@ -23,7 +23,7 @@ COARSE_SWEEPS, COARSE_CHOLESKY = 0, 1
 FILTER_SENSITIVITY, FILTER_DENSITY, FILTER_HELMHOLTZ = 0, 1, 2
 OBJ_COMPLIANCE, OBJ_MECHANISM = 0, 1
 FIELD_U, FIELD_XPHYS, FIELD_ENERGY, FIELD_DC, FIELD_DCN, FIELD_STRESS, FIELD_ADJOINT = range(7)
-TIME_APPLYA, TIME_VCYCLE, TIME_ENERGY, TIME_FILTER, TIME_OC, TIME_COPY, TIME_APPLYA_MASKED = range(7)
+TIME_APPLYA, TIME_VCYCLE, TIME_ENERGY, TIME_FILTER, TIME_OC, TIME_COPY, TIME_APPLYA_MASKED, TIME_APPLYA_MFMA = range(8)
 
 ERRORS = {0: "OK", -1: "E_ARG", -2: "E_HIP", -3: "E_OOM", -4: "E_BREAKDOWN", -5: "E_UNSTRUCTURED",
           -6: "E_NODEVICE", -7: "E_STATE", -8: "E_UNSUPPORTED", -9: "E_CANCELLED"}
@@ -54,7 +54,8 @@ class Config(C.Structure):
         ("mg_omega_cap", C.c_double),
         ("coarse_fp32", C.c_int32),
         ("energy_clamp", C.c_int32),
-        ("reserved", C.c_int32 * 6),
+        ("mfma_apply", C.c_int32),
+        ("reserved", C.c_int32 * 5),
     ]
 
     def copy(self):

@@ -185,6 +185,7 @@ int topo_config_default(topo_config* c, int32_t nelx, int32_t nely, int32_t nelz
     c->rhs_tile = 0;
     c->use_graphs = 1;
     c->skip_factor = 1000.0;          // skipThreshold = Emin * 1000
+    c->mfma_apply = 0;               // the VALU kernel is the faster one at these sizes
     return TOPO_OK;
 }
 

@@ -18,6 +18,7 @@
 #include <vector>
 #include "kernels.hpp"
 #include "kernels_xm.hpp"
+#include "kernels_mf.hpp"
 #include "kernels_adj.hpp"
 #include "kernels_chol.hpp"
 #include "kernels_bddc.hpp"
@@ -330,10 +331,69 @@ static HadCoef<TD> had_cast(const HadCoef<double>& c) {
     return r;
 }
 
-// K0 in device memory (FP64 + FP32 copies) and its Hadamard-block coefficients
+// bfloat16 bits of x, round to nearest even (finite x)
+static uint16_t bf16_bits(double xd) {
+    const float x = (float)xd;
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+static double bf16_value(uint16_t b) {
+    const uint32_t u = (uint32_t)b << 16;
+    float x;
+    std::memcpy(&x, &u, 4);
+    return x;
+}
+
+// A-operand fragments of K0 for k_applyA_mf: the 24x24 K0 permuted to the
+// 32x32 order 16 dx + 8 dy + 4 dz + c (local node m at offset off[m], the
+// element numbering of the reference, `:1448-1494`; c = 3 rows and columns
+// zero), split K = hi + lo in bfloat16; frag[(rb*2 + part)*64 + lane][i] =
+// part of row 16 rb + lane % 16, column 8 (lane / 16) + i.
+static void mf_k0_frags(const double* K, uint16_t (*frag)[8]) {
+    static const int off[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+    int ln[2][2][2];
+    for (int m = 0; m < 8; m++) ln[off[m][0]][off[m][1]][off[m][2]] = m;
+    auto kval = [&](int r, int k) {
+        const int c = r & 3, c2 = k & 3;
+        if (c == 3 || c2 == 3) return 0.0;
+        const int m = ln[r >> 4][(r >> 3) & 1][(r >> 2) & 1];
+        const int m2 = ln[k >> 4][(k >> 3) & 1][(k >> 2) & 1];
+        return K[(3 * m + c) * 24 + 3 * m2 + c2];
+    };
+    for (int rb = 0; rb < 2; rb++)
+        for (int lane = 0; lane < 64; lane++)
+            for (int i = 0; i < 8; i++) {
+                const double v = kval(16 * rb + lane % 16, 8 * (lane / 16) + i);
+                const uint16_t hi = bf16_bits(v);
+                frag[(rb * 2 + 0) * 64 + lane][i] = hi;
+                frag[(rb * 2 + 1) * 64 + lane][i] = bf16_bits(v - bf16_value(hi));
+            }
+}
+
+// Lets the k_applyA_mf variants use their dynamic LDS (up to ~82 KB with
+// NT = 512, above the 64 KB a launch gets without asking); false when the
+// device refuses, and the VALU kernel is used instead.
+static bool mf_allow_lds() {
+    const int bytes = (int)mf_lds_bytes(255, 512);
+    const void* fns[6] = {(const void*)k_applyA_mf<512, 0>, (const void*)k_applyA_mf<512, 1>,
+                          (const void*)k_applyA_mf<512, 2>, (const void*)k_applyA_mf<512, 3>,
+                          (const void*)k_applyA_mf<512, 4>, (const void*)k_applyA_mf<512, 5>};
+    for (const void* f : fns)
+        if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
+    return true;
+}
+
+// K0 in device memory (FP64 + FP32 copies), its Hadamard-block coefficients
+// and its matrix-core fragments
 struct DevKe {
     double* ke = nullptr;
     float* ke32 = nullptr;
+    uint4* kf = nullptr;   // 256 x 16 B, see mf_k0_frags
     HadCoef<double> hc64;
     HadCoef<float> hc32;
     bool had_ok = false;
@@ -343,11 +403,15 @@ struct DevKe {
         if (!ke) {
             ke = c.alloc<double>(576);
             ke32 = c.alloc<float>(576);
+            kf = c.alloc<uint4>(256);
         }
         float k32[576];
         for (int i = 0; i < 576; i++) k32[i] = (float)hke[i];
+        uint16_t frag[256][8];
+        mf_k0_frags(hke, frag);
         TOPO_HIP(hipMemcpyAsync(ke, hke, 576 * sizeof(double), hipMemcpyHostToDevice, c.s));
         TOPO_HIP(hipMemcpyAsync(ke32, k32, 576 * sizeof(float), hipMemcpyHostToDevice, c.s));
+        TOPO_HIP(hipMemcpyAsync(kf, frag, sizeof(frag), hipMemcpyHostToDevice, c.s));
         c.sync();
     }
 };
@@ -921,6 +985,9 @@ struct MGH {
     int xm_nt = 0;        // x-march workgroup size for every level (0 auto, 512, 1024)
     int xm_cx = 0;        // x-march planes per workgroup for every level (0 auto)
     int xm_cx64 = 0;      // x-march planes per workgroup of every FP64 operator (0: autotuned / as FP32)
+    // FP32 operator on the x-march levels with the element product on the
+    // matrix cores (k_applyA_mf) instead of the VALU Hadamard kernel
+    bool mfma = true;
     bool rz_fuse = true;  // r.z partials from the V-cycle's last sweep (else a separate kernel)
     std::vector<int> lv_nt, lv_cx;   // per-level choices of the autotuner (0 auto)
     // x-chunk of the FP64 operator where the autotuner timed it apart from
@@ -974,7 +1041,7 @@ struct MGH {
             put(li < lv_cx.size() ? lv_cx[li] : -1);
             put(li < lv_cx64.size() ? lv_cx64[li] : -1);
         }
-        put((exact ? 1 : 0) | (rz_fuse ? 2 : 0) | (bddc ? 4 : 0));
+        put((exact ? 1 : 0) | (rz_fuse ? 2 : 0) | (bddc ? 4 : 0) | (mfma ? 8 : 0));
         put(xm_nt);
         put(xm_cx);
         put(xm_cx64);
@@ -1703,6 +1770,27 @@ struct MGH {
                                 !(li == 0 && springs.n > 0)) ? pap : nullptr;
                 if (mode == 5 && !part) mode = 3;
                 if (part) npart = (int)nplane;
+                bool on_mfma = false;
+                if constexpr (std::is_same<T, float>::value) {
+                    if (mfma && NT == 512) {
+                        on_mfma = true;
+                        const size_t lds = mf_lds_bytes(l.g.nnz(), 512);
+#define TOPO_MF_LAUNCH(MV)                                                                                        \
+    hipLaunchKernelGGL((k_applyA_mf<512, MV>), grid, dim3(512), lds, c.s, in, out, (const float*)Sel<T>::E(l), thr, \
+                       l.g, (const uint8_t*)l.fixed, b, (const float*)Sel<T>::invD(l), om, gate, (const uint4*)K0->kf, \
+                       CX, gx, gy, part, rdot)
+                        switch (mode) {
+                        case 0: TOPO_MF_LAUNCH(0); break;
+                        case 1: TOPO_MF_LAUNCH(1); break;
+                        case 2: TOPO_MF_LAUNCH(2); break;
+                        case 3: TOPO_MF_LAUNCH(3); break;
+                        case 5: TOPO_MF_LAUNCH(5); break;
+                        default: TOPO_MF_LAUNCH(4); break;
+                        }
+#undef TOPO_MF_LAUNCH
+                    }
+                }
+                if (!on_mfma) {
                 const size_t lds = xm_lds_bytes(l.g.nnz(), sizeof(T), NT);
 #define TOPO_XM_LAUNCH(NTV, MV)                                                                                  \
     hipLaunchKernelGGL((k_applyA_xm<T, TE, NTV, MV>), grid, dim3(NTV), lds, c.s, in, out,                      \
@@ -1724,6 +1812,7 @@ struct MGH {
                 }
 #undef TOPO_XM_MODES
 #undef TOPO_XM_LAUNCH
+                }
             } else {
                 hipLaunchKernelGGL((k_applyA_exact<T, TE>), dim3(nblk(l.g.nnodes()), nr), dim3(256), 0, c.s, in, out,
                                    (const TE*)Sel<T>::E(l), thr, l.g, l.fixed, mode, b, Sel<T>::invD(l), om, gate,
@@ -2308,6 +2397,10 @@ struct Solver {
         mg.thr = cfg.skip_factor > 0 ? cfg.Emin * cfg.skip_factor : -1.0;
         mg.omega_cap = cfg.mg_omega_cap;
         mg.exact = cfg.exact_apply != 0;
+        mf_ok = mf_allow_lds();
+        mg.mfma = cfg.mfma_apply != 0 && mf_ok;
+        if (cfg.mfma_apply && !mg.mfma && std::getenv("TOPO_VERBOSE"))
+            std::fprintf(stderr, "[topo] matrix-core apply unavailable (LDS limit): FP32 VALU kernel\n");
         if (const char* e = getenv("TOPO_XM_NT")) mg.xm_nt = atoi(e);
         if (const char* e = getenv("TOPO_XM_CX")) mg.xm_cx = atoi(e);
         if (const char* e = getenv("TOPO_XM_CX64")) mg.xm_cx64 = atoi(e);
@@ -2610,6 +2703,7 @@ struct Solver {
         tuned = true;
     }
     int tuned_nt[2] = {0, 0}, tuned_cx[2] = {0, 0}, tuned_cx64 = 0;
+    bool mf_ok = false;   // the matrix-core applyA may run (its LDS granted)
     struct TuneEntry {
         int what;          // >= 0: tile timing of that MG level; < 0: -R, batch of R right-hand sides
         int nt, cx;
@@ -3346,6 +3440,14 @@ int topo_time_op(topo_solver* h, int32_t op, int32_t prec, int32_t level, int32_
             if (f32) S.mg.apply<float>(C, level, l.x32, l.x232, 1, nullptr, nullptr, 1, C.partials);
             else S.mg.apply<double>(C, level, l.x, l.x2, 1, nullptr, nullptr, 1, C.partials);
             break;
+        case TOPO_TIME_APPLYA_MFMA: {
+            TOPO_CHECK(f32 && S.mf_ok, TOPO_E_UNSUPPORTED, "matrix-core applyA needs a mixed solver and its LDS");
+            const bool keep = S.mg.mfma;
+            S.mg.mfma = true;
+            S.mg.apply<float>(C, level, l.x32, l.x232, 0, nullptr, nullptr);
+            S.mg.mfma = keep;
+            break;
+        }
         case TOPO_TIME_VCYCLE:
             if (f32) S.mg.vcycle<float>(C, 0, S.mg.L[0].b32, nullptr);
             else S.mg.vcycle<double>(C, 0, S.mg.L[0].b, nullptr);
@@ -3369,7 +3471,7 @@ int topo_time_op(topo_solver* h, int32_t op, int32_t prec, int32_t level, int32_
     };
     // algorithmic bytes of one launch
     const double sv = f32 ? 4.0 : 8.0;
-    if (op == TOPO_TIME_APPLYA) by = 2.0 * l.ndof * sv + (double)l.g.nel() * sv;
+    if (op == TOPO_TIME_APPLYA || op == TOPO_TIME_APPLYA_MFMA) by = 2.0 * l.ndof * sv + (double)l.g.nel() * sv;
     else if (op == TOPO_TIME_APPLYA_MASKED) by = 2.0 * l.ndof * sv + (double)l.g.nel() * sv + (double)l.ndof;
     else if (op == TOPO_TIME_COPY) by = 2.0 * l.ndof * 8.0;
     else if (op == TOPO_TIME_ENERGY) by = (double)S.ndof * 8 + (double)S.nel * 4 * 4;

@@ -159,6 +159,7 @@ function toTopoConfig(addon, model, config) {
     if (num(config.coarseFp32)) c.coarse_fp32 = config.coarseFp32;
     if (num(config.rhsTile)) c.rhs_tile = config.rhsTile;
     if (config.energyClamp != null) c.energy_clamp = config.energyClamp ? 1 : 0;
+    if (config.mfmaApply != null) c.mfma_apply = config.mfmaApply ? 1 : 0;
     return c;
 }
 

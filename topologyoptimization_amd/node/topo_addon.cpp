@@ -122,7 +122,7 @@ napi_value DeviceInfo(napi_env env, napi_callback_info) {
     X(d, mg_omega) X(i, coarse_solver) X(i, coarse_sweeps) X(i, galerkin_max_ndof) X(i, cg_max_iter)  \
     X(d, cg_tol) X(i, warm_start) X(d, move) X(i, objective) X(i, mech_in_dof) X(i, mech_out_dof)     \
     X(d, mech_k_in) X(d, mech_k_out) X(i, rhs_tile) X(i, use_graphs) X(d, skip_factor)                \
-    X(i, exact_apply) X(d, mg_omega_cap) X(i, coarse_fp32) X(i, energy_clamp)
+    X(i, exact_apply) X(d, mg_omega_cap) X(i, coarse_fp32) X(i, energy_clamp) X(i, mfma_apply)
 
 napi_value cfg_to_js(napi_env env, const topo_config& c) {
     napi_value o;
