@@ -137,8 +137,12 @@ def matched_leg(T, P, grid, prec, designs, first_iter, levels, plain=False, nste
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    # timed SIMP steps and untimed warm-up steps; defaults per workload:
+    # c3 20 after 5 (the timed window reaches the late, harder designs), c2 20
+    # after 1, c4 5 after 2, c5 5 after 15 (the inverting lever has formed),
+    # 2d 5 after 1
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--grid", default="192,96,96")
     ap.add_argument("--levels", type=int, default=4)
     ap.add_argument("--no-fp64", action="store_true")
@@ -162,11 +166,16 @@ def main():
     ap.add_argument("--omega-cap", type=float, default=OMEGA_CAP,
                     help="cap c of the damped-Jacobi weight, omega_eff = min(0.6, c / rho(D^-1 A)) per level")
     args = ap.parse_args()
+    dsteps, dwarm = {"c3": (20, 5), "c2": (20, 1), "c4": (5, 2), "c5": (5, 15), "2d": (5, 1)}[args.workload]
+    if args.steps is None:
+        args.steps = dsteps
+    if args.warmup is None:
+        args.warmup = dwarm
     globals()["OMEGA_CAP"] = args.omega_cap
     if args.profile_kernel:
         return profile_kernels(args)
     if args.gpus != 1:
-        print("bench.py runs on one GPU: this build has no multi-GPU path (see DESIGN.md, 'Sharding').",
+        print("bench.py runs on one GPU: the solver has no multi-GPU path (one GPU holds the whole grid).",
               file=sys.stderr)
         sys.exit(2)
     import topologyoptimization_amd as T
@@ -286,7 +295,7 @@ def c2_workload(T, P, args):
     log("device:", name)
     grid = (96, 48, 48)
     nel, ndof = T.dims(*grid)
-    steps = args.steps if args.steps != 3 else 20
+    steps = args.steps
 
     def make(prec):
         cfg, fixed, F, _, _, x0 = P.cantilever_c2(precision=prec)
