@@ -110,9 +110,18 @@ __device__ __forceinline__ void had_block(T (&A)[12], T (&Bq)[12], const HadCoef
     }
 }
 
-// dynamic LDS of an NT-thread workgroup: s_in[(NT + H) * 3] | s_q[NT * 9]
+// Whether the kernel keeps the staged node planes double-buffered in LDS and
+// recomputes the previous plane's face transform from there instead of
+// carrying it in registers: the FP64 kernels, whose 24 fewer live registers
+// let two 512-thread workgroups share a CU (and keep the 1024-thread one
+// within its 128 registers). The FP32 ones measured faster with the carry.
+template <typename T, int NT>
+constexpr bool xm_recomp() { return sizeof(T) == 8; }
+
+// dynamic LDS of an NT-thread workgroup: s_in[nbuf][(NT + H) * 3] | s_q[NT * 9]
 inline size_t xm_lds_bytes(int nnz, size_t elt, int NT) {
-    return ((size_t)(NT + nnz + 1) * 3 + (size_t)NT * 9) * elt;
+    const size_t nbuf = elt == 8 ? 2 : 1;
+    return ((size_t)(NT + nnz + 1) * 3 * nbuf + (size_t)NT * 9) * elt;
 }
 
 // MODE as k_applyA_exact: 0 Ap, 1 masked, 2 b - Ap (free) / 0 (fixed),
@@ -160,8 +169,11 @@ __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __
     const int nnz = g.nnz(), nny = g.nny(), nnx = g.nnx();
     const int nyz = nny * nnz;
     const int H = nnz + 1;
+    constexpr bool RECOMP = xm_recomp<T, NT>();
     T* s_in = reinterpret_cast<T*>(xm_smem);
-    T* s_q = s_in + (NT + H) * 3;       // per thread: q01[3] q10[3] q11[3]
+    T* s_q = s_in + (NT + H) * 3 * (RECOMP ? 2 : 1);   // per thread: q01[3] q10[3] q11[3]
+    // LDS buffer of node plane ix
+    auto sbuf = [&](int ix) -> T* { return RECOMP ? s_in + (ix & 1) * (NT + H) * 3 : s_in; };
     const int t = threadIdx.x;
     const int outs = NT - H;
     const int jbase = bx * outs - H;
@@ -223,6 +235,7 @@ __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __
     // stage the prefetched plane in LDS, prefetch plane ixn, and return this
     // thread's face transform of the staged plane
     auto stage_face = [&](int ixn, T (&f)[12]) {
+        T* sb = sbuf(ixn - 1);
         T own[3] = {pf[0], pf[1], pf[2]};
 #pragma unroll
         for (int d = 0; d < 3; d++) stg[d] = own[d];
@@ -231,19 +244,31 @@ __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __
             for (int d = 0; d < 3; d++) { stgb[d] = pfb[d]; stgd[d] = pfd[d]; }
         }
 #pragma unroll
-        for (int d = 0; d < 3; d++) s_in[3 * t + d] = own[d];
+        for (int d = 0; d < 3; d++) sb[3 * t + d] = own[d];
         if (t < H) {
 #pragma unroll
-            for (int d = 0; d < 3; d++) s_in[3 * (NT + t) + d] = ph[d];
+            for (int d = 0; d < 3; d++) sb[3 * (NT + t) + d] = ph[d];
         }
         fetch(ixn);
         __syncthreads();
         T n01[3], n10[3], n11[3];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            n01[d] = s_in[3 * (t + 1) + d];
-            n10[d] = s_in[3 * (t + nnz) + d];
-            n11[d] = s_in[3 * (t + nnz + 1) + d];
+            n01[d] = sb[3 * (t + 1) + d];
+            n10[d] = sb[3 * (t + nnz) + d];
+            n11[d] = sb[3 * (t + nnz + 1) + d];
+        }
+        yz_forward<T>(own, n01, n10, n11, f);
+    };
+    // face transform of node plane ix from its LDS buffer (RECOMP)
+    auto face_lds = [&](int ix, const T (&own)[3], T (&f)[12]) {
+        const T* sb = sbuf(ix);
+        T n01[3], n10[3], n11[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            n01[d] = sb[3 * (t + 1) + d];
+            n10[d] = sb[3 * (t + nnz) + d];
+            n11[d] = sb[3 * (t + nnz + 1) + d];
         }
         yz_forward<T>(own, n01, n10, n11, f);
     };
@@ -285,13 +310,25 @@ __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __
         T A[12], Bq[12];
         // reads of s_in for the previous plane finished before the last
         // barrier of the previous step; only the first step follows the
-        // pre-loop staging directly
-        if (ex == X0 - 1) __syncthreads();
+        // pre-loop staging directly (with double-buffered planes it writes
+        // the other buffer)
+        if (!RECOMP && ex == X0 - 1) __syncthreads();
         stage_face(ex + 2, fnew);
+        if constexpr (RECOMP) {
+            // plane ex was staged one step earlier (a barrier ago)
+            T fp[12];
+            face_lds(ex, ep, fp);
 #pragma unroll
-        for (int k = 0; k < 12; k++) {
-            A[k] = fprev[k] + fnew[k];
-            Bq[k] = fprev[k] - fnew[k];
+            for (int k = 0; k < 12; k++) {
+                A[k] = fp[k] + fnew[k];
+                Bq[k] = fp[k] - fnew[k];
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 12; k++) {
+                A[k] = fprev[k] + fnew[k];
+                Bq[k] = fprev[k] - fnew[k];
+            }
         }
         T Ee = T(0);
         if (elem_yz && ex >= 0 && ex < g.nelx) {
@@ -358,12 +395,17 @@ __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __
             }
         }
     };
-    int ex = X0 - 1;
-    for (; ex + 1 < X1; ex += 2) {
-        step(ex, fa, fb, ca, cb);
-        step(ex + 1, fb, fa, cb, ca);
+    if constexpr (RECOMP) {
+        // the previous face comes from LDS: one carry, updated in place
+        for (int ex = X0 - 1; ex < X1; ex++) step(ex, fa, fb, ca, ca);
+    } else {
+        int ex = X0 - 1;
+        for (; ex + 1 < X1; ex += 2) {
+            step(ex, fa, fb, ca, cb);
+            step(ex + 1, fb, fa, cb, ca);
+        }
+        if (ex < X1) step(ex, fa, fb, ca, cb);
     }
-    if (ex < X1) step(ex, fa, fb, ca, cb);
     if ((MODE == 1 || MODE == 5) && part) {
         __syncthreads();
         for (int k = t; 2 * k < X1 - X0; k += NT) {

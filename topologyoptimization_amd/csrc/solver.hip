@@ -388,6 +388,27 @@ static bool mf_allow_lds() {
     return true;
 }
 
+// The FP64 x-march kernels keep two node planes in LDS (up to ~74 KB with
+// 512 threads, nnz <= 255; ~147 KB with 1024 threads, nnz <= 511): grant
+// it; a device that refuses cannot run them.
+static void xm_allow_lds() {
+    static bool done = false;
+    if (done) return;
+    const int b512 = (int)xm_lds_bytes(255, sizeof(double), 512);
+    const int b1024 = (int)xm_lds_bytes(511, sizeof(double), 1024);
+    const void* f512[6] = {(const void*)k_applyA_xm<double, double, 512, 0>, (const void*)k_applyA_xm<double, double, 512, 1>,
+                           (const void*)k_applyA_xm<double, double, 512, 2>, (const void*)k_applyA_xm<double, double, 512, 3>,
+                           (const void*)k_applyA_xm<double, double, 512, 4>, (const void*)k_applyA_xm<double, double, 512, 5>};
+    const void* f1024[6] = {(const void*)k_applyA_xm<double, double, 1024, 0>, (const void*)k_applyA_xm<double, double, 1024, 1>,
+                            (const void*)k_applyA_xm<double, double, 1024, 2>, (const void*)k_applyA_xm<double, double, 1024, 3>,
+                            (const void*)k_applyA_xm<double, double, 1024, 4>, (const void*)k_applyA_xm<double, double, 1024, 5>};
+    for (int i = 0; i < 6; i++) {
+        TOPO_HIP(hipFuncSetAttribute(f512[i], hipFuncAttributeMaxDynamicSharedMemorySize, b512));
+        TOPO_HIP(hipFuncSetAttribute(f1024[i], hipFuncAttributeMaxDynamicSharedMemorySize, b1024));
+    }
+    done = true;
+}
+
 // K0 in device memory (FP64 + FP32 copies), its Hadamard-block coefficients
 // and its matrix-core fragments
 struct DevKe {
@@ -1054,6 +1075,7 @@ struct MGH {
     // (js/optimizer-worker-3d.js:408-442); nlevels <= 0 = reference rule
     void build(Ctx& c, Grid g0, const uint8_t* dfixed0, int nlevels, bool fp32) {
         want32 = fp32;
+        xm_allow_lds();
         // the transfer kernels index nodes in 32 bits
         TOPO_CHECK(g0.nnodes() < ((int64_t)1 << 32), TOPO_E_ARG, "grid of %lld nodes too large",
                    (long long)g0.nnodes());
