@@ -94,10 +94,11 @@ def test_autotuner_and_simp_steps(gpu):
     assert np.all(fine[:, 2] % 2 == 0)   # even x-chunks (0 = the size rule, made even)
     best = fine[np.argmin(fine[:, 3])]
     assert (nt, cx) == (int(best[1]), int(best[2]))
-    # the mixed solver's FP64 CG operator is tuned on its own (same candidates)
+    # the mixed solver's FP64 CG operator is tuned on its own (same candidates,
+    # the size rule logged as the x-chunk it gives)
     cg64 = tiles[tiles[:, 0] == 1000]
     assert len(cg64) == 12 and np.all(cg64[:, 3] > 0)
-    assert sorted(cg64[:, 2]) == sorted(fine[:, 2])
+    assert np.all(cg64[:, 2] >= 2) and np.all(cg64[:, 2] % 2 == 0)
     print("autotuned: R=%d NT=%d CX=%d, fine operator %.4f ms" % (R, nt, cx, best[3]))
     cfg1, fixed, F, _, _, _ = _bridge(T, precision=2, rhs_tile=1, cg_tol=1e-6, cg_max_iter=400)
     with T.Solver(cfg1, fixed, F) as S:

@@ -69,6 +69,7 @@ __global__ void __launch_bounds__(NT) k_applyA_mf(const float* __restrict__ p, f
                                                   const double* __restrict__ omp, const int* gate,
                                                   const uint4* __restrict__ kf, int CX, int gx, int gy,
                                                   double* __restrict__ part, const double* __restrict__ rdot = nullptr) {
+#pragma clang fp contract(off)
     gate = batch_gate(gate);
     if (gate && *gate) return;
     {
@@ -141,12 +142,12 @@ __global__ void __launch_bounds__(NT) k_applyA_mf(const float* __restrict__ p, f
                     for (int d = 0; d < 3; d++) {
                         pfb[d] = bvec[o + 3 * j + d];
                         pfd[d] = invD[o + 3 * j + d];
-                        pf[d] = 0.f + omega * pfd[d] * pfb[d];
+                        pf[d] = xfma(omega * pfd[d], pfb[d], 0.f);
                     }
                 }
                 if (jhv) {
 #pragma unroll
-                    for (int d = 0; d < 3; d++) ph[d] = 0.f + omega * invD[o + 3 * jh + d] * bvec[o + 3 * jh + d];
+                    for (int d = 0; d < 3; d++) ph[d] = xfma(omega * invD[o + 3 * jh + d], bvec[o + 3 * jh + d], 0.f);
                 }
             } else {
                 if (jv) {
@@ -288,10 +289,10 @@ __global__ void __launch_bounds__(NT) k_applyA_mf(const float* __restrict__ p, f
                         v = ef[d] ? 0.f : eb[d] - v;
                     } else if (MODE == 3 || MODE == 4 || MODE == 5) {
                         const float r = eb[d] - v;
-                        v = ef[d] ? ep[d] : ep[d] + omega * ed[d] * r;
+                        v = ef[d] ? ep[d] : xfma(omega * ed[d], r, ep[d]);
                     }
-                    if (MODE == 1) cpl += (double)ep[d] * (double)v;
-                    if (MODE == 5) cpl += er[d] * (double)v;
+                    if (MODE == 1) cpl = xfma((double)ep[d], (double)v, cpl);
+                    if (MODE == 5) cpl = xfma(er[d], (double)v, cpl);
                     out[obase + d] = v;
                 }
             }

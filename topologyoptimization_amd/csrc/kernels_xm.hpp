@@ -37,12 +37,20 @@ struct HadCoef {
     T tf[3];                // class ~e_k: scalar (1,1,1) member
 };
 
+// Floating-point contraction is off in these kernels and the fused
+// multiply-adds are written out, so every inlined copy of the march step
+// (it is unrolled) forms each value with the same operations: the results do
+// not depend on the x-chunking or on which copy computed a plane.
+__device__ __forceinline__ float xfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double xfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 // face arrays f[qy][qz][d] flattened as f[(qy*2 + qz)*3 + d]
 #define FI(qy, qz, d) (((qy) * 2 + (qz)) * 3 + (d))
 
 template <typename T>
 __device__ __forceinline__ void yz_forward(const T (&n00)[3], const T (&n01)[3], const T (&n10)[3], const T (&n11)[3],
                                            T (&f)[12]) {
+#pragma clang fp contract(off)
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         const T s0 = n00[d] + n01[d], d0 = n00[d] - n01[d];
@@ -55,58 +63,60 @@ __device__ __forceinline__ void yz_forward(const T (&n00)[3], const T (&n01)[3],
 }
 
 // A = u^[qx=0], Bq = u^[qx=1] (in: transformed displacements; out: the
-// block-diagonal operator applied, scaled by E; translation members -> 0)
+// block-diagonal operator applied; translation members -> 0). The element
+// modulus is applied by the caller after the inverse x butterfly.
 template <typename T>
-__device__ __forceinline__ void had_block(T (&A)[12], T (&Bq)[12], const HadCoef<T>& c, T E) {
+__device__ __forceinline__ void had_block(T (&A)[12], T (&Bq)[12], const HadCoef<T>& c) {
+#pragma clang fp contract(off)
     // class (0,0,0)
     {
         const T v0 = Bq[FI(0, 0, 0)], v1 = A[FI(1, 0, 1)], v2 = A[FI(0, 1, 2)];
-        const T s = c.b0 * (v0 + v1 + v2);
-        Bq[FI(0, 0, 0)] = E * (c.am0 * v0 + s);
-        A[FI(1, 0, 1)] = E * (c.am0 * v1 + s);
-        A[FI(0, 1, 2)] = E * (c.am0 * v2 + s);
+        const T s = c.b0 * ((v0 + v1) + v2);
+        Bq[FI(0, 0, 0)] = xfma(c.am0, v0, s);
+        A[FI(1, 0, 1)] = xfma(c.am0, v1, s);
+        A[FI(0, 1, 2)] = xfma(c.am0, v2, s);
     }
     // class (1,1,1)
     {
         const T v0 = A[FI(1, 1, 0)], v1 = Bq[FI(0, 1, 1)], v2 = Bq[FI(1, 0, 2)];
-        const T s = c.b7 * (v0 + v1 + v2);
-        A[FI(1, 1, 0)] = E * (c.am7 * v0 + s);
-        Bq[FI(0, 1, 1)] = E * (c.am7 * v1 + s);
-        Bq[FI(1, 0, 2)] = E * (c.am7 * v2 + s);
+        const T s = c.b7 * ((v0 + v1) + v2);
+        A[FI(1, 1, 0)] = xfma(c.am7, v0, s);
+        Bq[FI(0, 1, 1)] = xfma(c.am7, v1, s);
+        Bq[FI(1, 0, 2)] = xfma(c.am7, v2, s);
     }
     // classes e_k: members (d ascending, d != k); translation -> 0
     {
         T v1 = Bq[FI(1, 0, 1)], v2 = Bq[FI(0, 1, 2)];
-        Bq[FI(1, 0, 1)] = E * (c.oa[0] * v1 + c.oc[0] * v2);
-        Bq[FI(0, 1, 2)] = E * (c.oc[0] * v1 + c.ob[0] * v2);
+        Bq[FI(1, 0, 1)] = xfma(c.oa[0], v1, c.oc[0] * v2);
+        Bq[FI(0, 1, 2)] = xfma(c.oc[0], v1, c.ob[0] * v2);
         v1 = Bq[FI(1, 0, 0)];
         v2 = A[FI(1, 1, 2)];
-        Bq[FI(1, 0, 0)] = E * (c.oa[1] * v1 + c.oc[1] * v2);
-        A[FI(1, 1, 2)] = E * (c.oc[1] * v1 + c.ob[1] * v2);
+        Bq[FI(1, 0, 0)] = xfma(c.oa[1], v1, c.oc[1] * v2);
+        A[FI(1, 1, 2)] = xfma(c.oc[1], v1, c.ob[1] * v2);
         v1 = Bq[FI(0, 1, 0)];
         v2 = A[FI(1, 1, 1)];
-        Bq[FI(0, 1, 0)] = E * (c.oa[2] * v1 + c.oc[2] * v2);
-        A[FI(1, 1, 1)] = E * (c.oc[2] * v1 + c.ob[2] * v2);
+        Bq[FI(0, 1, 0)] = xfma(c.oa[2], v1, c.oc[2] * v2);
+        A[FI(1, 1, 1)] = xfma(c.oc[2], v1, c.ob[2] * v2);
         A[FI(0, 0, 0)] = T(0);
         A[FI(0, 0, 1)] = T(0);
         A[FI(0, 0, 2)] = T(0);
     }
     // classes ~e_k: scalar (1,1,1) member d = k, shear pair (d ascending)
     {
-        Bq[FI(1, 1, 0)] = E * (c.tf[0] * Bq[FI(1, 1, 0)]);
+        Bq[FI(1, 1, 0)] = c.tf[0] * Bq[FI(1, 1, 0)];
         T v1 = A[FI(0, 1, 1)], v2 = A[FI(1, 0, 2)];
-        A[FI(0, 1, 1)] = E * (c.ta[0] * v1 + c.tc[0] * v2);
-        A[FI(1, 0, 2)] = E * (c.tc[0] * v1 + c.tb[0] * v2);
-        Bq[FI(1, 1, 1)] = E * (c.tf[1] * Bq[FI(1, 1, 1)]);
+        A[FI(0, 1, 1)] = xfma(c.ta[0], v1, c.tc[0] * v2);
+        A[FI(1, 0, 2)] = xfma(c.tc[0], v1, c.tb[0] * v2);
+        Bq[FI(1, 1, 1)] = c.tf[1] * Bq[FI(1, 1, 1)];
         v1 = A[FI(0, 1, 0)];
         v2 = Bq[FI(0, 0, 2)];
-        A[FI(0, 1, 0)] = E * (c.ta[1] * v1 + c.tc[1] * v2);
-        Bq[FI(0, 0, 2)] = E * (c.tc[1] * v1 + c.tb[1] * v2);
-        Bq[FI(1, 1, 2)] = E * (c.tf[2] * Bq[FI(1, 1, 2)]);
+        A[FI(0, 1, 0)] = xfma(c.ta[1], v1, c.tc[1] * v2);
+        Bq[FI(0, 0, 2)] = xfma(c.tc[1], v1, c.tb[1] * v2);
+        Bq[FI(1, 1, 2)] = c.tf[2] * Bq[FI(1, 1, 2)];
         v1 = A[FI(1, 0, 0)];
         v2 = Bq[FI(0, 0, 1)];
-        A[FI(1, 0, 0)] = E * (c.ta[2] * v1 + c.tc[2] * v2);
-        Bq[FI(0, 0, 1)] = E * (c.tc[2] * v1 + c.tb[2] * v2);
+        A[FI(1, 0, 0)] = xfma(c.ta[2], v1, c.tc[2] * v2);
+        Bq[FI(0, 0, 1)] = xfma(c.tc[2], v1, c.tb[2] * v2);
     }
 }
 
@@ -138,14 +148,17 @@ inline size_t xm_lds_bytes(int nnz, size_t elt, int NT) {
 // depend on the autotuned x-chunking); MODE 5 is the mode-3 sweep that
 // leaves the shares of r.z the same way (r = rdot, the CG's FP64 residual;
 // z = the sweep's output, the V-cycle's last). CX must be even.
+// (The FP64 512-thread variants up to MODE 2 -- the CG operator and the
+// residual -- are held to four waves per SIMD, two workgroups per CU.)
 template <typename T, typename TE, int NT, int MODE>
-__global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __restrict__ out,
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 && NT == 512 && MODE <= 2 ? 4 : 1))) k_applyA_xm(const T* __restrict__ p, T* __restrict__ out,
                                                   const TE* __restrict__ E, double thr, Grid g,
                                                   const uint8_t* __restrict__ fixed,
                                                   const T* __restrict__ bvec, const T* __restrict__ invD,
                                                   const double* __restrict__ omp, const int* gate, HadCoef<T> co,
                                                   int CX, int gx, int gy, double* __restrict__ part,
                                                   const double* __restrict__ rdot = nullptr) {
+#pragma clang fp contract(off)
     gate = batch_gate(gate);
     if (gate && *gate) return;
     {
@@ -207,13 +220,13 @@ __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __
                     for (int d = 0; d < 3; d++) {
                         pfb[d] = bvec[o + 3 * j + d];
                         pfd[d] = invD[o + 3 * j + d];
-                        pf[d] = T(0) + omega * pfd[d] * pfb[d];
+                        pf[d] = xfma(omega * pfd[d], pfb[d], T(0));
                     }
                 }
                 if (jhv) {
 #pragma unroll
                     for (int d = 0; d < 3; d++)
-                        ph[d] = T(0) + omega * invD[o + 3 * jh + d] * bvec[o + 3 * jh + d];
+                        ph[d] = xfma(omega * invD[o + 3 * jh + d], bvec[o + 3 * jh + d], T(0));
                 }
             } else {
                 const T* src = p + (int64_t)ix * plane;
@@ -335,15 +348,16 @@ __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __
             const TE ev = E[(int64_t)ex * eplane + erow];
             if ((double)ev > thr) Ee = (T)ev;
         }
-        had_block<T>(A, Bq, co, Ee);
-        // inverse x butterfly: face at x=0 (plane ex) and x=1 (plane ex+1)
+        had_block<T>(A, Bq, co);
+        // inverse x butterfly: face at x=0 (plane ex) and x=1 (plane ex+1),
+        // scaled by the element modulus
         T nv[12];
 #pragma unroll
         for (int k = 0; k < 12; k++) {
             const T f0 = A[k] + Bq[k];
             const T f1 = A[k] - Bq[k];
-            nv[k] = cin[k] + f0;
-            cout[k] = f1;
+            nv[k] = xfma(Ee, f0, cin[k]);
+            cout[k] = Ee * f1;
         }
         // inverse y/z butterflies of the face at node plane ex -> 4 node values
         T q00[3];
@@ -373,10 +387,10 @@ __global__ void __launch_bounds__(NT) k_applyA_xm(const T* __restrict__ p, T* __
                     v = ef[d] ? T(0) : eb[d] - v;
                 } else if (MODE == 3 || MODE == 4 || MODE == 5) {
                     const T r = eb[d] - v;
-                    v = ef[d] ? ep[d] : ep[d] + omega * ed[d] * r;
+                    v = ef[d] ? ep[d] : xfma(omega * ed[d], r, ep[d]);
                 }
-                if (MODE == 1) cpl += (double)ep[d] * (double)v;
-                if (MODE == 5) cpl += er[d] * (double)v;
+                if (MODE == 1) cpl = xfma((double)ep[d], (double)v, cpl);
+                if (MODE == 5) cpl = xfma(er[d], (double)v, cpl);
                 out[obase + d] = v;
             }
         }

@@ -1738,6 +1738,17 @@ struct MGH {
         return (nt_sel == 1024 || (nt_sel == 0 && 4 * H > 512)) ? 1024 : 512;
     }
 
+    // the size rule for the x-planes per workgroup of level li's x-march
+    // operator with NT-thread workgroups (before clamping to an even 2..32)
+    int xm_auto_cx(int li, int NT) const {
+        const Grid& g = L[li].g;
+        const int H = g.nnz() + 1;
+        const int gx = (g.nny() * g.nnz() + (NT - H) - 1) / (NT - H);
+        const int cx = (int)((int64_t)g.nnx() * gx * (NT / 512) / 1024);
+        const int c2 = std::max(2, std::min(32, cx));
+        return c2 + (c2 & 1);
+    }
+
     // x-planes per workgroup of level li's x-march operator in precision T (0 auto)
     template <typename T>
     int xm_pick_cx(int li) const {
@@ -1781,17 +1792,19 @@ struct MGH {
             if (!exact && K0->had_ok && 2 * H <= NT) {
                 const int nyz = l.g.nny() * l.g.nnz();
                 const int gx = (nyz + (NT - H) - 1) / (NT - H);
-                int CX = cx_sel > 0 ? cx_sel : (int)((int64_t)l.g.nnx() * gx * (NT / 512) / 1024);
-                CX = std::max(2, std::min(32, CX));
-                CX += CX & 1;   // even: the CG partials cover whole node-plane pairs
-                const int gy = (l.g.nnx() + CX - 1) / CX;
-                const dim3 grid(8 * ((gx * gy + 7) / 8), nr);
+                int CX = cx_sel > 0 ? cx_sel : xm_auto_cx(li, NT);
+                CX = std::max(1, std::min(32, CX));
                 // CG partials: one per (yz tile, node-plane pair), whatever CX
                 const int64_t nplane = (int64_t)gx * ((l.g.nnx() + 1) / 2);
                 double* part = ((mode == 1 || mode == 5) && pap && nplane <= kCgStride &&
                                 !(li == 0 && springs.n > 0)) ? pap : nullptr;
                 if (mode == 5 && !part) mode = 3;
-                if (part) npart = (int)nplane;
+                if (part) {
+                    npart = (int)nplane;
+                    CX = std::max(2, CX + (CX & 1));   // even: the partials cover whole node-plane pairs
+                }
+                const int gy = (l.g.nnx() + CX - 1) / CX;
+                const dim3 grid(8 * ((gx * gy + 7) / 8), nr);
                 bool on_mfma = false;
                 if constexpr (std::is_same<T, float>::value) {
                     if (mfma && NT == 512) {
@@ -2658,49 +2671,73 @@ struct Solver {
         // launch fit the chip in one round of workgroups
         constexpr int kNcx = 12;
         const int cx_c[kNcx] = {0, 2, 4, 6, 8, 10, 12, 16, 18, 20, 24, 32};
+        // the coarser levels (no CG partials there) also try odd chunks: one
+        // or three planes per workgroup, for latency-bound small grids
+        const int cx_s[kNcx] = {0, 1, 2, 3, 4, 6, 8, 10, 12, 16, 20, 32};
+        // per candidate the least of two timings of 5 launches, the two
+        // rounds over all candidates one after the other (clock and cache
+        // drift between candidates evens out)
+        auto sweep_cx = [&](double (&ms)[kNcx], const int (&cand)[kNcx], auto&& launch) {
+            for (int b = 0; b < kNcx; b++) ms[b] = 1e300;
+            for (int round = 0; round < 2; round++)
+                for (int b = 0; b < kNcx; b++)
+                    ms[b] = std::min(ms[b], time_it([&]() { launch(cand[b]); }, 5));
+        };
         const int nl = (int)mg.L.size();
         mg.lv_nt.assign(nl, 0);
         mg.lv_cx.assign(nl, 0);
         mg.lv_cx64.assign(nl, 0);
         if (!mg.xm_nt && !mg.xm_cx && !mg.exact) {
-            for (int li = 0; li < std::min(nl, 2); li++) {
+            for (int li = 0; li < std::min(nl, 3); li++) {
                 MGLevel& l = mg.L[li];
-                if (l.dense) continue;
+                if (l.dense || l.chol) continue;
+                const int (&cand)[kNcx] = li == 0 ? cx_c : cx_s;
                 // operands: the level's own vectors, filled from the rhs copy
                 hipLaunchKernelGGL((k_convert<double, double>), dim3(nblk(l.ndof)), dim3(256), 0, C.s, l.x,
                                    (const double*)v64.r, l.ndof);
                 if (f32)
                     hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(l.ndof)), dim3(256), 0, C.s, l.x32,
                                        (const double*)v64.r, l.ndof);
-                double best = 1e300;
                 const int H = l.g.nnz() + 1;
                 const int nt_rule = 4 * H > 512 ? 1024 : 512;
-                for (int b = 0; b < kNcx; b++) {
-                    mg.lv_nt[li] = 0;
-                    mg.lv_cx[li] = cx_c[b];
-                    const double ms = time_it([&]() {
-                        if (f32) mg.apply<float>(C, li, l.x32, l.x232, 1, nullptr, nullptr);
-                        else mg.apply<double>(C, li, l.x, l.x2, 1, nullptr, nullptr);
-                    }, 5);
-                    tune_log.push_back({li, nt_rule, cx_c[b], ms});
-                    if (ms < best) { best = ms; tuned_nt[li] = nt_rule; tuned_cx[li] = cx_c[b]; }
-                }
                 mg.lv_nt[li] = 0;
-                mg.lv_cx[li] = tuned_cx[li];
+                double ms[kNcx];
+                sweep_cx(ms, cand, [&](int cx) {
+                    mg.lv_cx[li] = cx;
+                    if (f32) mg.apply<float>(C, li, l.x32, l.x232, 1, nullptr, nullptr);
+                    else mg.apply<double>(C, li, l.x, l.x2, 1, nullptr, nullptr);
+                });
+                int bi = 0;
+                for (int b = 0; b < kNcx; b++) {
+                    tune_log.push_back({li, nt_rule, cand[b], ms[b]});
+                    if (ms[b] < ms[bi]) bi = b;
+                }
+                if (li < 2) {
+                    tuned_nt[li] = nt_rule;
+                    tuned_cx[li] = cand[bi];
+                }
+                mg.lv_cx[li] = cand[bi];
             }
             // the mixed solvers' CG operator (FP64, masked, finest level):
             // its own x-chunk, as FP64 arithmetic and registers give it
             // another best tile than the FP32 smoothers (log entries 1000)
             MGLevel& l = mg.L[0];
             if (f32 && !mg.xm_cx64 && !l.dense && mg.xm_level<double>(0)) {
-                double best = 1e300;
                 const int nt = mg.xm_pick_nt<double>(0);
+                double ms[kNcx];
+                // candidate 0 is the size rule, made explicit (0 in lv_cx64
+                // would mean "as the FP32 operator")
+                const int cx_rule = mg.xm_auto_cx(0, nt);
+                sweep_cx(ms, cx_c, [&](int cx) {
+                    mg.lv_cx64[0] = cx ? cx : cx_rule;
+                    mg.apply<double>(C, 0, l.x, l.x2, 1, nullptr, nullptr);
+                });
+                int bi = 0;
                 for (int b = 0; b < kNcx; b++) {
-                    mg.lv_cx64[0] = cx_c[b];
-                    const double ms = time_it([&]() { mg.apply<double>(C, 0, l.x, l.x2, 1, nullptr, nullptr); }, 5);
-                    tune_log.push_back({1000, nt, cx_c[b], ms});
-                    if (ms < best) { best = ms; tuned_cx64 = cx_c[b]; }
+                    tune_log.push_back({1000, nt, cx_c[b] ? cx_c[b] : cx_rule, ms[b]});
+                    if (ms[b] < ms[bi]) bi = b;
                 }
+                tuned_cx64 = cx_c[bi] ? cx_c[bi] : cx_rule;
                 mg.lv_cx64[0] = tuned_cx64;
             }
         }
