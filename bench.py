@@ -5,10 +5,12 @@ MG-PCG in FP64 and against the plain FP64 method (Jacobi-PCG), at matched
 compliance, plus GDOF-matvec/s of the fine applyA.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 192,96,96]
+                [--workload c3|c2|c4|c5|2d]
 
 A step is one SIMP iteration (projection/filter, FE solve to tol 1e-6,
 energies, sensitivities, filter, OC) on device-resident data. Prints one
-JSON line.
+JSON line. The default (c3) line also carries the 2-D leg and the c2, c4
+and c5 workloads at their default step counts (`other_workloads`).
 """
 import argparse
 import json
@@ -26,6 +28,8 @@ sys.path.insert(0, ROOT)
 # at 192x96x96 over the timed steps: 1.2 / 1.4 / 1.6 / 1.8 / 1.9 take 1.45 / 1.18 / 1.06 / 0.94 / 0.95
 # times the CG iterations of 1.6 ... (relative to 1.6), 2.0 stalls a solve at the 400-iteration cap
 OMEGA_CAP = 1.8
+# timed SIMP steps and untimed warm-up steps per workload
+DEFAULT_STEPS = {"c3": (20, 5), "c2": (20, 1), "c4": (5, 2), "c5": (5, 15), "2d": (5, 1)}
 
 
 def log(*a):
@@ -163,10 +167,12 @@ def main():
                          "around an FP32 inner MG-PCG")
     ap.add_argument("--profile-kernel", action="store_true")
     ap.add_argument("--no-2d", action="store_true", help="skip the 2-D SIMP leg")
+    ap.add_argument("--no-others", action="store_true",
+                    help="c3 only: skip the c2, c4 and c5 workloads reported beside the headline")
     ap.add_argument("--omega-cap", type=float, default=OMEGA_CAP,
                     help="cap c of the damped-Jacobi weight, omega_eff = min(0.6, c / rho(D^-1 A)) per level")
     args = ap.parse_args()
-    dsteps, dwarm = {"c3": (20, 5), "c2": (20, 1), "c4": (5, 2), "c5": (5, 15), "2d": (5, 1)}[args.workload]
+    dsteps, dwarm = DEFAULT_STEPS[args.workload]
     if args.steps is None:
         args.steps = dsteps
     if args.warmup is None:
@@ -184,9 +190,11 @@ def main():
         print(json.dumps(simp2d_leg(T, steps=args.steps, warmup=args.warmup, cpu=not args.no_cpu)))
         return
     if args.workload == "c2":
-        return c2_workload(T, P, args)
+        print(json.dumps(c2_workload(T, P, args)))
+        return
     if args.workload != "c3":
-        return other_workload(T, P, args)
+        print(json.dumps(other_workload(T, P, args)))
+        return
     grid = tuple(int(v) for v in args.grid.split(","))
     n, name = T.device_info()
     assert n > 0, "no HIP device"
@@ -229,6 +237,21 @@ def main():
         cpu = cpu_baseline(grid, infos, args.levels)
     fine_calls = np.mean([i["applyA_fine"] for i in infos])
     simp2d = None if args.no_2d else simp2d_leg(T, cpu=not args.no_cpu)
+    # the other BASELINE workloads at their default step counts (the CPU
+    # legs of the bridge and the inverter are left out: each would build a
+    # full-size hierarchy on the host)
+    others = None
+    if not args.no_others:
+        others = {}
+        for wl in ("c2", "c4", "c5"):
+            a = argparse.Namespace(**vars(args))
+            a.workload = wl
+            a.steps, a.warmup = DEFAULT_STEPS[wl]
+            a.no_cpu = args.no_cpu or wl != "c2"
+            try:
+                others[wl] = c2_workload(T, P, a) if wl == "c2" else other_workload(T, P, a)
+            except Exception as e:   # reported, never fatal to the headline
+                others[wl] = {"error": repr(e)}
     out = {
         "metric": "SIMP iters/s + GDOF-matvec/s at 5.4M DOF, mixed-FP32 vs FP64 MGPCG",
         "value": value,
@@ -279,6 +302,7 @@ def main():
         "autotuned_fine_tile": {"xm_nt": tnt, "xm_cx": tcx},
         "autotune_candidates": [[int(a), int(b), int(c), float(d)] for a, b, c, d in tlog],
         "simp2d": simp2d,
+        "other_workloads": others,
         "device": name,
     }
     print(json.dumps(out))
@@ -347,7 +371,7 @@ def c2_workload(T, P, args):
                      "avg_ms": ms32},
         "cpu_baseline": None if args.no_cpu else cpu_baseline_c2(infos), "device": name,
     }
-    print(json.dumps(out))
+    return out
 
 
 def other_workload(T, P, args):
@@ -401,7 +425,7 @@ def other_workload(T, P, args):
         "cpu_baseline": cpu, "device": name,
     }
     out.update(extra)
-    print(json.dumps(out))
+    return out
 
 
 def _rel(a, b):
@@ -531,6 +555,9 @@ def simp2d_leg(T, grid=(1024, 512), steps=5, warmup=1, cpu=True):
     out = {"workload": "2-D cantilever %dx%d quad4, %d DOF, Jacobi-PCG FP64, steps %d..%d" %
                        (nelx, nely, ndof, warmup + 1, warmup + steps),
            "value": steps / dt, "unit": "SIMP iters/s", "ms_per_step": dt / steps * 1e3, "cg_iters": cg,
+           # the 2-D worker's own cap (js/optimizer-worker.js solveCG): at
+           # this size its absolute tolerance is not reached within it
+           "cg_max_iter": cfg.cg_max_iter, "solves_at_cap": int(sum(c >= cfg.cg_max_iter for c in cg)),
            "us_per_cg_iteration": solve_ms * 1e3 / max(1, sum(cg)),
            "compliance": infos[-1].compliance, "cpu_baseline": None}
     if cpu:

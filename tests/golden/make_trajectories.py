@@ -3,7 +3,10 @@
 tests in tests/test_gpu_loops.py compare against. Run from the repository
 root after building the oracle (make -C oracle):
 
-    python tests/golden/make_trajectories.py
+    python tests/golden/make_trajectories.py [case ...]
+
+With case names, only those cases are (re)computed and the others are kept
+from the existing file.
 
 Each case stores its compliance per iteration, the change per iteration,
 the CG iteration counts and the final design (reference element order).
@@ -24,9 +27,17 @@ from oracle import oracle as O  # noqa: E402
 import trajectory_cases as TC  # noqa: E402
 
 
-def main():
+def main(argv):
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "trajectories.npz")
     out = {}
-    for name in TC.CASES:
+    names = TC.CASES
+    if argv:
+        names = argv
+        for n in names:
+            assert n in TC.CASES, n
+        with np.load(path) as old:
+            out = {k: old[k] for k in old.files}
+    for name in names:
         cfg, fixed, F, pres, void, x0, niter = TC.build(name, "oracle")
         t0 = time.time()
         r = O.simp_run(cfg, fixed, F, pres, void, niter, x0)
@@ -37,8 +48,8 @@ def main():
         out[name + "/change"] = r["change"]
         out[name + "/cg"] = r["cg"]
         out[name + "/x"] = r["x"]
-    np.savez_compressed(os.path.join(os.path.dirname(os.path.abspath(__file__)), "trajectories.npz"), **out)
+    np.savez_compressed(path, **out)
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -5,6 +5,7 @@ tests/trajectory_cases.py):
 
 * the 192x96x96 benchmark recipe at 32x16x16, mixed precision (FP64 MG-PCG
   with the FP32 V-cycle reading the FP16 coarse copy), 10 iterations;
+* the full 4-level recipe at 48x24x24, mixed precision, 10 iterations;
 * the same recipe at 48x24x24 with 3 levels and the coarsest block factored
   in nested-dissection order (forced by TOPO_ND_MIN / TOPO_ND_LEAF), with
   the FP16 and the FP32 copies of its inverse factor, 6 iterations;
@@ -67,6 +68,16 @@ def test_c3_recipe_mixed_vs_oracle(gpu):
     0.12 once reached, bisection of normal length, no NaN."""
     infos, x = _run(gpu, "c3_recipe_32")
     _check("c3_recipe_32", infos, x, 1e-4, cg_slack=2)
+    assert np.isfinite(x).all()
+    for i in infos[3:]:
+        assert abs(i.volume - 0.12) <= 1e-3 and i.oc_passes < 100, (i.volume, i.oc_passes)
+
+
+def test_c3_recipe_48_mixed_vs_oracle(gpu):
+    """The full recipe (4 levels, FP64 Cholesky of the 6x3x3 coarsest
+    block, FP16 copy in the FP32 V-cycle) at 48x24x24, 10 iterations."""
+    infos, x = _run(gpu, "c3_recipe_48")
+    _check("c3_recipe_48", infos, x, 1e-4, cg_slack=3)
     assert np.isfinite(x).all()
     for i in infos[3:]:
         assert abs(i.volume - 0.12) <= 1e-3 and i.oc_passes < 100, (i.volume, i.oc_passes)

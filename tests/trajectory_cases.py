@@ -10,7 +10,7 @@ import numpy as np
 import topologyoptimization_amd as T
 import topologyoptimization_amd.problems as P
 
-CASES = ("c3_recipe_32", "c3_recipe_48_3lev", "reference_default_16", "c2_recipe_24")
+CASES = ("c3_recipe_32", "c3_recipe_48", "c3_recipe_48_3lev", "reference_default_16", "c2_recipe_24")
 
 
 def build(name, side):
@@ -20,6 +20,15 @@ def build(name, side):
         # kept, energies clamped) on a 32x16x16 grid, 10 iterations: volume
         # reaches 0.12 in the third
         nx, ny, nz = 32, 16, 16
+        cfg, fixed, F, pres, void, _ = P.cantilever(nx, ny, nz, linear_solver=T.SOLVER_MGPCG, mg_levels=4, mg_omega=0.6,
+                                                    coarse_solver=T.COARSE_CHOLESKY, galerkin_max_ndof=0,
+                                                    precision=T.PREC_FP64 if side == "oracle" else T.PREC_MIXED_PC,
+                                                    coarse_fp32=2, max_iterations=1000, mg_omega_cap=1.8)
+        return cfg, fixed, F, pres, void, P.splitmix64_uniform(T.dims(nx, ny, nz)[0]), 10
+    if name == "c3_recipe_48":
+        # the full recipe (4 levels: 48x24x24 -> 24x12x12 -> 12x6x6 -> 6x3x3,
+        # FP64 Cholesky of the 6x3x3 block) at 48x24x24, 10 iterations
+        nx, ny, nz = 48, 24, 24
         cfg, fixed, F, pres, void, _ = P.cantilever(nx, ny, nz, linear_solver=T.SOLVER_MGPCG, mg_levels=4, mg_omega=0.6,
                                                     coarse_solver=T.COARSE_CHOLESKY, galerkin_max_ndof=0,
                                                     precision=T.PREC_FP64 if side == "oracle" else T.PREC_MIXED_PC,
