@@ -767,6 +767,62 @@ __global__ void __launch_bounds__(kRedThreads) k_cgp_pupdate(T* __restrict__ p, 
     }
 }
 
+// k_cgp_pupdate (with U, MASK false) that also keeps the books of the
+// iteration, so no k_cgp_finish follows: the iteration's state st is only
+// read, and block 0 writes the updated state into stn, which the next
+// iteration's kernels read (the two state arrays trade roles every
+// iteration; writing st in place would race with the blocks still reading
+// it). The books are k_cgp_finish's, from the same partial sums in the same
+// order, so the iterates are the same bits. A right-hand side that is done
+// is copied through unchanged.
+template <typename T, typename TZ, int NR, typename TU = T>
+__global__ void __launch_bounds__(kRedThreads) k_cgp_pupdate_books(T* __restrict__ p, const TZ* __restrict__ zz,
+                                                                  int64_t n, const CGState* st, CGState* stn,
+                                                                  const double* part, TU* __restrict__ U, int nb0,
+                                                                  int nb2) {
+    const bool books = blockIdx.x == 0;
+    if (all_done(st, NR)) {
+        if (books && threadIdx.x < NR) stn[threadIdx.x] = st[threadIdx.x];
+        return;
+    }
+    double rz[NR], pAp[NR];
+    sum_partials<NR>(part, 2, NR, rz, nb2);
+    sum_partials<NR>(part, 0, NR, pAp, nb0);
+#pragma unroll
+    for (int q = 0; q < NR; q++) {
+        if (st[q].done) continue;
+        const T beta = (T)(rz[q] / (st[q].rz + st[q].guard));
+        const double alpha = st[q].rz / (pAp[q] + st[q].guard);
+        T* pq = p + q * n;
+        const TZ* zq = zz + q * n;
+        TU* Uq = U + q * n;
+        for (int64_t i = gtid(); i < n; i += gstride()) {
+            const T pi = pq[i];
+            Uq[i] = (TU)((double)Uq[i] + alpha * (double)pi);
+            pq[i] = (T)zq[i] + beta * pi;
+        }
+    }
+    if (!books) return;
+    double rn[NR];
+    sum_partials<NR>(part, 1, NR, rn);
+    if (threadIdx.x != 0) return;
+    for (int q = 0; q < NR; q++) {
+        CGState s = st[q];
+        if (!s.done) {
+            s.pAp = pAp[q];
+            s.alpha = s.rz / (pAp[q] + s.guard);
+            s.rn = rn[q];
+            s.rz_new = rz[q];
+            s.beta = rz[q] / (s.rz + s.guard);
+            s.rz = rz[q];
+            s.iters += 1;
+            if (!isfinite(s.alpha) || !isfinite(rn[q]) || !isfinite(rz[q])) s.nonfinite = 1;
+            s.done = (rn[q] < s.tolSq) || (s.iters >= s.maxit) || s.nonfinite;
+        }
+        stn[q] = s;
+    }
+}
+
 // the books of one iteration (one block of kRedThreads): alpha, ||r||^2,
 // rz, beta, iteration count, non-finite flag, convergence
 template <int NR>

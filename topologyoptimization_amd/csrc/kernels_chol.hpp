@@ -493,7 +493,10 @@ __global__ void __launch_bounds__(256) k_tri_part(const TK* __restrict__ M, int6
 // part[(q nrb + rb) n + j] (0 for blocks above the diagonal), summed by
 // k_tri_sum. rfirst (optional): first nonzero column of each row, the same
 // for the rows of a wave; a wave whose rows are zero over the block's
-// columns does not read them.
+// columns does not read them. With ypart, y is not read but formed by the
+// block for its rows from the ynch chunk partials of k_tri_part, exactly as
+// k_tri_sum would write it (y_i = (ys ? ys_i : 1) * sum, rounded to TB), so
+// the row-chunk sums need no kernel of their own.
 __device__ __forceinline__ float fma_t(float a, float b, float c) { return fmaf(a, b, c); }
 __device__ __forceinline__ double fma_t(double a, double b, double c) { return fma(a, b, c); }
 constexpr int kTcRows = 256;      // rows per block
@@ -504,7 +507,9 @@ template <int NR, typename TB, typename TK = _Float16, int NW = 4>
 __global__ void __launch_bounds__(64 * NW) k_tri_colpart(const TK* __restrict__ M, int64_t n,
                                                          const TB* __restrict__ y, double* __restrict__ part, int nrb,
                                                          const int* gate, int upper = 0,
-                                                         const int32_t* __restrict__ rfirst = nullptr) {
+                                                         const int32_t* __restrict__ rfirst = nullptr,
+                                                         const double* __restrict__ ypart = nullptr, int ynch = 0,
+                                                         const double* __restrict__ ys = nullptr) {
     constexpr int VN = 16 / (int)sizeof(TK);   // entries per 16-byte piece
     constexpr int kTcCols = 64 * VN;
     static_assert(NW == 4 || NW == 8, "4 or 8 waves per block");
@@ -527,6 +532,30 @@ __global__ void __launch_bounds__(64 * NW) k_tri_colpart(const TK* __restrict__ 
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int64_t j = J0 + VN * lane;
     const bool colok = j < n;
+    // y of the block's rows from the chunk partials: chunks sub, sub + 4, ...
+    // in ascending order for sub = 0..3, then (s0 + s1) + (s2 + s3), the
+    // order of k_tri_sum's four lanes
+    __shared__ TB ysh[NR][kTcRows];
+    if (ypart) {
+        const int t = threadIdx.x;
+        const int64_t i = I0 + t;
+        if (t < kTcRows && i < n) {
+#pragma unroll
+            for (int q = 0; q < NR; q++) {
+                double ss[4];
+#pragma unroll
+                for (int sub = 0; sub < 4; sub++) {
+                    double a = 0.0;
+                    const double* pq = ypart + (int64_t)q * ynch * n + i;
+                    for (int c = sub; c < ynch; c += 4) a += pq[(int64_t)c * n];
+                    ss[sub] = a;
+                }
+                const double sum = (ss[0] + ss[1]) + (ss[2] + ss[3]);
+                ysh[q][t] = (TB)(ys ? ys[i] * sum : sum);
+            }
+        }
+        __syncthreads();
+    }
     TA acc[NR][VN];
 #pragma unroll
     for (int q = 0; q < NR; q++)
@@ -552,7 +581,7 @@ __global__ void __launch_bounds__(64 * NW) k_tri_colpart(const TK* __restrict__ 
             }
 #pragma unroll
             for (int q = 0; q < NR; q++) {
-                const TA yv = (TA)y[q * n + i];
+                const TA yv = ypart ? (TA)ysh[q][i - I0] : (TA)y[q * n + i];
 #pragma unroll
                 for (int u = 0; u < VN; u++) acc[q][u] = fma_t(av[u], yv, acc[q][u]);
             }

@@ -59,6 +59,7 @@ struct Ctx {
     double* partials = nullptr;   // reduction partials (last-block and fence-free kernels)
     unsigned* ticket = nullptr;
     CGState* st = nullptr;        // device CG state
+    CGState* st_alt = nullptr;    // second device CG state (MG-PCG books kept by the p update, by iteration parity)
     CGState* h_st = nullptr;      // pinned mirror
     OCState* oc = nullptr;
     OCState* h_oc = nullptr;
@@ -142,6 +143,7 @@ struct Ctx {
         partials = alloc<double>(std::max(32 * kRedBlocks, 3 * kCgRows * kCgStride));
         ticket = alloc<unsigned>(4);
         st = alloc<CGState>(kMaxRHS);
+        st_alt = alloc<CGState>(kMaxRHS);
         oc = alloc<OCState>(1);
         dsc = alloc<double>(16);
         TOPO_HIP(hipHostMalloc((void**)&h_st, kMaxRHS * sizeof(CGState), hipHostMallocDefault));
@@ -469,6 +471,7 @@ struct MGLevel {
     double* Linv = nullptr;        // inverses of the diagonal blocks of L
     double* ytmp = nullptr;        // Z b (nrhs x nfp)
     double* tpart = nullptr;       // chunk partials of the triangular products (nrhs x max(tnch, tnrb) x nfp)
+    double* tpart1 = nullptr;      // row-chunk partials of Z b read by the column-tile pass (nrhs x tnch x nfp)
     int tcw = 0, tnch = 0;         // column chunk width (0: one wave per row quad) and count
     int trw = 8;                   // rows per wave of the chunked product (4 or 8)
     int tnrb = 0;                  // row blocks of the transposed FP16 product (column tiles of Z16)
@@ -581,20 +584,25 @@ static std::vector<int32_t> nd_order(const Grid& g, const std::vector<uint8_t>& 
 // One triangular product of the coarse solve (k_tri_apply semantics): by
 // column chunks (k_tri_part + k_tri_sum) when the level has a chunk width,
 // else one wave per row quad.
+// With sums = false and the chunked form, only the chunk partials are
+// written (into l.tpart1) and true is returned: the caller's column-tile pass
+// forms y from them (k_tri_colpart's ypart).
 template <typename T, typename TK, typename TB, typename TY>
-static void tri_product(Ctx& c, const MGLevel& l, const TK* M, int upper, const TB* bf, TY* y, T* x, const int* gate,
-                        const double* ys, int nr) {
+static bool tri_product(Ctx& c, const MGLevel& l, const TK* M, int upper, const TB* bf, TY* y, T* x, const int* gate,
+                        const double* ys, int nr, bool sums = true) {
     if (l.tcw) {
         const int32_t* rf = upper ? (const int32_t*)nullptr : (const int32_t*)l.rfirst;
+        double* part = sums ? l.tpart : l.tpart1;
         if (l.trw == 8) {
             const dim3 gp((unsigned)l.tnch, (unsigned)(l.nfp / 32));
             TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_part<NR, TK, TB, 8>), gp, dim3(256), 0, c.s, M, l.nfp, upper,
-                                                  bf, l.tpart, l.tcw, l.tnch, gate, rf));
+                                                  bf, part, l.tcw, l.tnch, gate, rf));
         } else {
             const dim3 gp((unsigned)l.tnch, (unsigned)(l.nfp / 16));
             TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_part<NR, TK, TB, 4>), gp, dim3(256), 0, c.s, M, l.nfp, upper,
-                                                  bf, l.tpart, l.tcw, l.tnch, gate, rf));
+                                                  bf, part, l.tcw, l.tnch, gate, rf));
         }
+        if (!sums) return true;
         TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_sum<T, NR, TY>), dim3(nblk(4 * l.nfp)), dim3(256), 0, c.s,
                                               (const double*)l.tpart, l.tnch, l.nfp, y, x, (const int32_t*)l.freeIdx,
                                               l.nf, l.ndof, gate, ys));
@@ -603,19 +611,24 @@ static void tri_product(Ctx& c, const MGLevel& l, const TK* M, int upper, const 
         TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_apply<T, NR, TK, TB, TY>), gt, dim3(256), 0, c.s, M, l.nfp, upper,
                                               bf, y, x, (const int32_t*)l.freeIdx, l.nf, l.ndof, gate, ys));
     }
+    return false;
 }
 
 // Column-tile product Z^T y of the coarse solve: nw = 8 waves per tile (32
 // rows each) or 4 (64 rows each), the same for every batch width, so a
-// batched solve gives the bits of one right-hand side at a time
+// batched solve gives the bits of one right-hand side at a time. With ypart
+// (the ynch row-chunk partials of Z b), y is formed by the tiles themselves.
 template <int NR, typename TB, typename TK>
 static void tri_colpart(hipStream_t s, const TK* M, int64_t n, const TB* y, double* part, int nrb, const int* gate,
-                        const int32_t* rfirst, int nw) {
+                        const int32_t* rfirst, int nw, const double* ypart = nullptr, int ynch = 0,
+                        const double* ys = nullptr) {
     const dim3 gc((unsigned)((n + tc_cols<TK>() - 1) / tc_cols<TK>()), (unsigned)nrb);
     if (nw == 8)
-        hipLaunchKernelGGL((k_tri_colpart<NR, TB, TK, 8>), gc, dim3(512), 0, s, M, n, y, part, nrb, gate, 0, rfirst);
+        hipLaunchKernelGGL((k_tri_colpart<NR, TB, TK, 8>), gc, dim3(512), 0, s, M, n, y, part, nrb, gate, 0, rfirst,
+                           ypart, ynch, ys);
     else
-        hipLaunchKernelGGL((k_tri_colpart<NR, TB, TK, 4>), gc, dim3(256), 0, s, M, n, y, part, nrb, gate, 0, rfirst);
+        hipLaunchKernelGGL((k_tri_colpart<NR, TB, TK, 4>), gc, dim3(256), 0, s, M, n, y, part, nrb, gate, 0, rfirst,
+                           ypart, ynch, ys);
 }
 
 template <typename T> struct Sel;
@@ -1329,6 +1342,7 @@ struct MGH {
                 l.tnch = l.tcw ? (int)((l.nfp + l.tcw - 1) / l.tcw) : 0;
                 l.tnrb = (int)((l.nfp + kTcRows - 1) / kTcRows);
                 l.tpart = c.alloc<double>((int64_t)nrhs * std::max(l.tnch, l.tnrb) * l.nfp);
+                if (l.tnch) l.tpart1 = c.alloc<double>((int64_t)nrhs * l.tnch * l.nfp);
             }
             l.d0c = c.alloc<double>(l.nfp);
             if (want32 && coarse_fp32) l.Kinv32 = c.alloc<float>(l.nfp * l.nfp);
@@ -1894,33 +1908,40 @@ struct MGH {
                     // S^2 Zq b by row chunks, then Zq^T y by column tiles of
                     // the same matrix (just read: the second pass finds much
                     // of it in the memory-side cache)
-                    tri_product<T, _Float16, float, float>(c, l, l.Z16, 0, (const float*)l.bf, (float*)l.ytmp,
-                                                           (T*)nullptr, gate, l.zs2, nr);
+                    // (the column tiles sum the row chunks of S^2 Zq b themselves)
+                    const bool yp = tri_product<T, _Float16, float, float>(c, l, l.Z16, 0, (const float*)l.bf,
+                                                                           (float*)l.ytmp, (T*)nullptr, gate, l.zs2,
+                                                                           nr, false);
                     TOPO_NR_SWITCH(nr, (tri_colpart<NR, float, _Float16>(c.s, (const _Float16*)l.Z16, l.nfp,
                                                                          (const float*)l.ytmp, l.tpart, l.tnrb, gate,
-                                                                         (const int32_t*)l.rfirst, tc_nw)));
+                                                                         (const int32_t*)l.rfirst, tc_nw,
+                                                                         yp ? l.tpart1 : nullptr, l.tnch, l.zs2)));
                     TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_sum<T, NR, float>), dim3(nblk(4 * l.nfp)), dim3(256),
                                                           0, c.s, (const double*)l.tpart, l.tnrb, l.nfp,
                                                           (float*)nullptr, x, (const int32_t*)l.freeIdx, l.nf, l.ndof,
                                                           gate, (const double*)nullptr));
                 } else if (f32 && l.Z32) {
                     // Z b by rows, Z^T y by column tiles of the same FP32 copy
-                    tri_product<T, float, double, double>(c, l, l.Z32, 0, (const double*)l.bf, l.ytmp, (T*)nullptr,
-                                                          gate, nullptr, nr);
+                    const bool yp = tri_product<T, float, double, double>(c, l, l.Z32, 0, (const double*)l.bf,
+                                                                          l.ytmp, (T*)nullptr, gate, nullptr, nr,
+                                                                          false);
                     TOPO_NR_SWITCH(nr, (tri_colpart<NR, double, float>(c.s, (const float*)l.Z32, l.nfp,
                                                                         (const double*)l.ytmp, l.tpart, l.tnrb, gate,
-                                                                        (const int32_t*)l.rfirst, tc_nw)));
+                                                                        (const int32_t*)l.rfirst, tc_nw,
+                                                                        yp ? l.tpart1 : nullptr, l.tnch, nullptr)));
                     TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_sum<T, NR, double>), dim3(nblk(4 * l.nfp)), dim3(256),
                                                           0, c.s, (const double*)l.tpart, l.tnrb, l.nfp,
                                                           (double*)nullptr, x, (const int32_t*)l.freeIdx, l.nf, l.ndof,
                                                           gate, (const double*)nullptr));
                 } else {
                     // FP64: Z b by rows, Z^T y by column tiles of Z (FP64 sums)
-                    tri_product<T, double, double, double>(c, l, l.Z, 0, (const double*)l.bf, l.ytmp, (T*)nullptr,
-                                                           gate, nullptr, nr);
+                    const bool yp = tri_product<T, double, double, double>(c, l, l.Z, 0, (const double*)l.bf,
+                                                                           l.ytmp, (T*)nullptr, gate, nullptr, nr,
+                                                                           false);
                     TOPO_NR_SWITCH(nr, (tri_colpart<NR, double, double>(c.s, (const double*)l.Z, l.nfp,
                                                                          (const double*)l.ytmp, l.tpart, l.tnrb, gate,
-                                                                         (const int32_t*)l.rfirst, tc_nw)));
+                                                                         (const int32_t*)l.rfirst, tc_nw,
+                                                                         yp ? l.tpart1 : nullptr, l.tnch, nullptr)));
                     TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_tri_sum<T, NR, double>), dim3(nblk(4 * l.nfp)), dim3(256),
                                                           0, c.s, (const double*)l.tpart, l.tnrb, l.nfp,
                                                           (double*)nullptr, x, (const int32_t*)l.freeIdx, l.nf, l.ndof,
@@ -2192,17 +2213,25 @@ static CGResult pcg_mg(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVecs& 
     TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_cg_start<T, NR>), dim3(kRedBlocks), dim3(kRedThreads), 0, c.s,
                                            (const T*)r, (const T*)z, n, c.st, c.partials, c.ticket));
     c.check_launch();
-    const int* gate = &c.st->done;
-    const int check = mg.cg_check;
+    // The p update keeps the books (k_cgp_pupdate_books) where the context
+    // has the second state: iteration k reads state k & 1 and leaves the
+    // next in the other, so a block of iterations is even (the state is
+    // back in c.st for the host).
+    const bool books = c.st_alt != nullptr && std::is_same<T, TU>::value;
+    const int check = books ? (mg.cg_check + 1) & ~1 : mg.cg_check;
     auto body = [&]() {
         for (int k = 0; k < check; k++) {
+            CGState* sk = (books && (k & 1)) ? c.st_alt : c.st;
+            CGState* sn = (k & 1) ? c.st : c.st_alt;
+            const int* gate = &sk->done;
             // fence-free iteration: each reduction's partials are summed by
-            // the kernel that needs them, k_cgp_finish keeps the books
-            // the applyA leaves the p.Ap partials itself when it can
+            // the kernel that needs them, the p update (or k_cgp_finish)
+            // keeps the books; the applyA leaves the p.Ap partials itself
+            // when it can
             int nb0 = mg.apply<T>(c, 0, p, Ap, 1, nullptr, gate, nr, c.partials);
             if (nb0 == 0) {
                 TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_cgp_pAp<T, NR>), dim3(kCgBlocks), dim3(kRedThreads), 0, c.s,
-                                                       (const T*)p, (const T*)Ap, n, c.st, c.partials));
+                                                       (const T*)p, (const T*)Ap, n, (const CGState*)sk, c.partials));
                 nb0 = kCgBlocks;
             }
             // r_lo: the V-cycle's FP32 residual, written by the update itself
@@ -2210,7 +2239,7 @@ static CGResult pcg_mg(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVecs& 
             // (U += alpha p is done by the p update below)
             TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_cgp_update_ur<T, TU, NR, PT>), dim3(kCgBlocks),
                                                    dim3(kRedThreads), 0, c.s, (TU*)nullptr, r, (const T*)p,
-                                                   (const T*)Ap, n, c.st, c.partials, r_lo, nb0));
+                                                   (const T*)Ap, n, (const CGState*)sk, c.partials, r_lo, nb0));
             // z is never formed: r.z and p = z + beta p read the V-cycle's
             // output, which is zero at the fixed DOFs (the mask of ebeMGPCG
             // :1170-1172 is a no-op on it)
@@ -2220,15 +2249,22 @@ static CGResult pcg_mg(Ctx& c, MGH& mg, const T* rhs, TU* U, bool warm, CGVecs& 
                                                    mg.rz_fuse ? c.partials + cg_row(2, 0) : nullptr, &nb2);
             if (nb2 == 0) {
                 TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_cgp_rz<T, PT, NR, false>), dim3(kCgBlocks), dim3(kRedThreads),
-                                                       0, c.s, (const T*)r, zz, (const uint8_t*)l.fixed, n, c.st,
-                                                       c.partials));
+                                                       0, c.s, (const T*)r, zz, (const uint8_t*)l.fixed, n,
+                                                       (const CGState*)sk, c.partials));
                 nb2 = kCgBlocks;
             }
-            TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_cgp_pupdate<T, PT, NR, false, TU>), dim3(kCgBlocks),
-                                                   dim3(kRedThreads), 0, c.s, p, zz, (const uint8_t*)l.fixed, n,
-                                                   c.st, (const double*)c.partials, U, nb0, nb2));
-            TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_cgp_finish<NR>), dim3(1), dim3(kRedThreads), 0, c.s, c.st,
-                                                   (const double*)c.partials, nb0, nb2));
+            if (books) {
+                if constexpr (std::is_same<T, TU>::value)
+                    TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_cgp_pupdate_books<T, PT, NR, TU>), dim3(kCgBlocks),
+                                                           dim3(kRedThreads), 0, c.s, p, zz, n, (const CGState*)sk, sn,
+                                                           (const double*)c.partials, U, nb0, nb2));
+            } else {
+                TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_cgp_pupdate<T, PT, NR, false, TU>), dim3(kCgBlocks),
+                                                       dim3(kRedThreads), 0, c.s, p, zz, (const uint8_t*)l.fixed, n,
+                                                       c.st, (const double*)c.partials, U, nb0, nb2));
+                TOPO_NR_SWITCH(nr, hipLaunchKernelGGL((k_cgp_finish<NR>), dim3(1), dim3(kRedThreads), 0, c.s, c.st,
+                                                       (const double*)c.partials, nb0, nb2));
+            }
         }
     };
     hipGraphExec_t ex = nullptr;
