@@ -130,24 +130,31 @@ __global__ void __launch_bounds__(NT) k_applyA_mf(const float* __restrict__ p, f
 
     // register prefetch of one node plane (own node + halo node)
     float pf[3], ph[3], pfb[3], pfd[3];
+    float phb[3], phd[3];   // MODE 4: b and invD of the halo node
     auto fetch = [&](int ix) {
 #pragma unroll
         for (int d = 0; d < 3; d++) { pf[d] = 0.f; ph[d] = 0.f; pfb[d] = 0.f; pfd[d] = 0.f; }
+        if constexpr (MODE == 4) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) { phb[d] = 0.f; phd[d] = 0.f; }
+        }
         if (ix >= 0 && ix < nnx) {
             const int64_t o = (int64_t)ix * plane;
             if constexpr (MODE == 4) {
-                // x0 = 0 + omega invD b, as the separate first sweep forms it
+                // b and invD only: x0 = omega invD b is formed at staging
                 if (jv) {
 #pragma unroll
                     for (int d = 0; d < 3; d++) {
                         pfb[d] = bvec[o + 3 * j + d];
                         pfd[d] = invD[o + 3 * j + d];
-                        pf[d] = xfma(omega * pfd[d], pfb[d], 0.f);
                     }
                 }
                 if (jhv) {
 #pragma unroll
-                    for (int d = 0; d < 3; d++) ph[d] = xfma(omega * invD[o + 3 * jh + d], bvec[o + 3 * jh + d], 0.f);
+                    for (int d = 0; d < 3; d++) {
+                        phb[d] = bvec[o + 3 * jh + d];
+                        phd[d] = invD[o + 3 * jh + d];
+                    }
                 }
             } else {
                 if (jv) {
@@ -166,6 +173,14 @@ __global__ void __launch_bounds__(NT) k_applyA_mf(const float* __restrict__ p, f
     float stgb[3] = {0.f, 0.f, 0.f}, stgd[3] = {0.f, 0.f, 0.f};   // MODE 4
     // split the prefetched plane ix into its LDS buffer, prefetch ix + 1
     auto stage = [&](int ix) {
+        if constexpr (MODE == 4) {
+            // x0 = 0 + omega invD b, as the separate first sweep forms it
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                pf[d] = xfma(omega * pfd[d], pfb[d], 0.f);
+                ph[d] = xfma(omega * phd[d], phb[d], 0.f);
+            }
+        }
         uint2 hi, lo;
         mf_split3(pf, hi, lo);
         const int bo = (ix & 1) * NP;

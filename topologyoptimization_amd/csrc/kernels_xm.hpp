@@ -208,25 +208,32 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(
     // register prefetch of one node plane (own node + halo node)
     T pf[3], ph[3];
     T pfb[3], pfd[3];   // MODE 4: own b and invD of the prefetched plane
+    T phb[3], phd[3];   // MODE 4: the halo node's
     auto fetch = [&](int ix) {
 #pragma unroll
         for (int d = 0; d < 3; d++) { pf[d] = T(0); ph[d] = T(0); pfb[d] = T(0); pfd[d] = T(0); }
+        if constexpr (MODE == 4) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) { phb[d] = T(0); phd[d] = T(0); }
+        }
         if (ix >= 0 && ix < nnx) {
             if constexpr (MODE == 4) {
-                // x0 = 0 + omega invD b, as the separate first sweep forms it
+                // b and invD only: x0 = omega invD b is formed when the plane
+                // is staged, so the loads stay in flight over a plane step
                 const int64_t o = (int64_t)ix * plane;
                 if (jv) {
 #pragma unroll
                     for (int d = 0; d < 3; d++) {
                         pfb[d] = bvec[o + 3 * j + d];
                         pfd[d] = invD[o + 3 * j + d];
-                        pf[d] = xfma(omega * pfd[d], pfb[d], T(0));
                     }
                 }
                 if (jhv) {
 #pragma unroll
-                    for (int d = 0; d < 3; d++)
-                        ph[d] = xfma(omega * invD[o + 3 * jh + d], bvec[o + 3 * jh + d], T(0));
+                    for (int d = 0; d < 3; d++) {
+                        phb[d] = bvec[o + 3 * jh + d];
+                        phd[d] = invD[o + 3 * jh + d];
+                    }
                 }
             } else {
                 const T* src = p + (int64_t)ix * plane;
@@ -249,6 +256,14 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(
     // thread's face transform of the staged plane
     auto stage_face = [&](int ixn, T (&f)[12]) {
         T* sb = sbuf(ixn - 1);
+        if constexpr (MODE == 4) {
+            // x0 = 0 + omega invD b, as the separate first sweep forms it
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                pf[d] = xfma(omega * pfd[d], pfb[d], T(0));
+                ph[d] = xfma(omega * phd[d], phb[d], T(0));
+            }
+        }
         T own[3] = {pf[0], pf[1], pf[2]};
 #pragma unroll
         for (int d = 0; d < 3; d++) stg[d] = own[d];
