@@ -309,11 +309,19 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(
     double cpair = 0.0;   // this thread's share of the current plane pair (MODE 1, 5)
 #pragma unroll
     for (int k = 0; k < 12; k++) ca[k] = T(0);
+    // element modulus of element plane ex (0 outside the grid), read one
+    // step before it is used
+    auto emod = [&](int ex) -> TE {
+        return (elem_yz && ex >= 0 && ex < g.nelx) ? E[(int64_t)ex * eplane + erow] : TE(0);
+    };
+    TE enext = emod(X0 - 1);
     fetch(X0 - 1);
     stage_face(X0, fa);
     // one x step: element plane ex between node planes ex (face fprev) and
     // ex+1 (staged here into fnew); completes output node plane ex
     auto step = [&](const int ex, const T (&fprev)[12], T (&fnew)[12], const T (&cin)[12], T (&cout)[12]) {
+        const TE ev = enext;
+        enext = emod(ex + 1);
         // operands of this step's output epilogue (node plane ex), loaded
         // before the element work so their latency overlaps it
         const bool outstep = is_out && ex >= X0;
@@ -358,11 +366,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(
                 Bq[k] = fprev[k] - fnew[k];
             }
         }
-        T Ee = T(0);
-        if (elem_yz && ex >= 0 && ex < g.nelx) {
-            const TE ev = E[(int64_t)ex * eplane + erow];
-            if ((double)ev > thr) Ee = (T)ev;
-        }
+        const T Ee = (double)ev > thr ? (T)ev : T(0);
         had_block<T>(A, Bq, co);
         // inverse x butterfly: face at x=0 (plane ex) and x=1 (plane ex+1),
         // scaled by the element modulus
