@@ -984,10 +984,12 @@ __device__ __forceinline__ void rw1d(int c, int nF, int nC, int (&f)[3], T (&w)[
 // reference's visiting order (fine fz, fy, fx ascending; one corner each,
 // mgVcycle assembly/index.ts:868-961), then fixed coarse DOFs zeroed (:964-967).
 // The weights are products of 1/2 and 1, exact in T; a zero-weight slot adds
-// an exact zero, so the sums are those of the contributors alone.
+// an exact zero, so the sums are those of the contributors alone. The 27
+// fine node values are all loaded before the sums (the compiler otherwise
+// waits for each load before issuing the next).
 template <typename T>
-__global__ void k_restrict(const T* __restrict__ res, T* __restrict__ bC, Grid gf, Grid gc,
-                           const uint8_t* __restrict__ fixedC, const int* gate) {
+__global__ void __launch_bounds__(256) k_restrict(const T* __restrict__ res, T* __restrict__ bC, Grid gf, Grid gc,
+                                                  const uint8_t* __restrict__ fixedC, const int* gate) {
 #pragma clang fp contract(off)
     gate = batch_gate(gate);
     if (gate && *gate) return;
@@ -1004,6 +1006,21 @@ __global__ void k_restrict(const T* __restrict__ res, T* __restrict__ bC, Grid g
         rw1d<T>((int)cx, nxF, nxC, fx, wx);
         rw1d<T>(cy, nyF, nyC, fy, wy);
         rw1d<T>(cz, nzF, nzC, fz, wz);
+        T v[27][3];
+#pragma unroll
+        for (int iz = 0; iz < 3; iz++)
+#pragma unroll
+            for (int iy = 0; iy < 3; iy++)
+#pragma unroll
+                for (int ix = 0; ix < 3; ix++) {
+                    const uint32_t nf = ((uint32_t)fx[ix] * (uint32_t)nyF + (uint32_t)fy[iy]) * (uint32_t)nzF +
+                                        (uint32_t)fz[iz];
+#pragma unroll
+                    for (int d = 0; d < 3; d++) v[(iz * 3 + iy) * 3 + ix][d] = res[3 * (int64_t)nf + d];
+                }
+        uint8_t fc[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) fc[d] = fixedC[3 * nc + d];
         T acc[3] = {T(0), T(0), T(0)};
 #pragma unroll
         for (int iz = 0; iz < 3; iz++)
@@ -1012,11 +1029,10 @@ __global__ void k_restrict(const T* __restrict__ res, T* __restrict__ bC, Grid g
 #pragma unroll
                 for (int ix = 0; ix < 3; ix++) {
                     const T w = (wx[ix] * wy[iy]) * wz[iz];
-                    const int64_t nf = ((int64_t)fx[ix] * nyF + fy[iy]) * nzF + fz[iz];
 #pragma unroll
-                    for (int d = 0; d < 3; d++) acc[d] += w * res[3 * nf + d];
+                    for (int d = 0; d < 3; d++) acc[d] += w * v[(iz * 3 + iy) * 3 + ix][d];
                 }
-        for (int d = 0; d < 3; d++) bC[3 * nc + d] = fixedC[3 * nc + d] ? T(0) : acc[d];
+        for (int d = 0; d < 3; d++) bC[3 * nc + d] = fc[d] ? T(0) : acc[d];
     }
 }
 

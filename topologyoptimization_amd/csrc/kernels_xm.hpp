@@ -205,46 +205,33 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(
     const int64_t erow = (int64_t)iy * g.nelz + iz;
     const int64_t eplane = (int64_t)g.nely * g.nelz;
 
-    // register prefetch of one node plane (own node + halo node)
+    // register prefetch of one node plane (own node + halo node). The loads
+    // are unconditional (a lane with no node reads entry 0, and its value is
+    // replaced by 0 when the plane is staged), so no branch surrounds them
+    // and the compiler's wait counts track each load exactly: a load is only
+    // waited for where its value is used, a plane step later.
     T pf[3], ph[3];
     T pfb[3], pfd[3];   // MODE 4: own b and invD of the prefetched plane
     T phb[3], phd[3];   // MODE 4: the halo node's
     auto fetch = [&](int ix) {
-#pragma unroll
-        for (int d = 0; d < 3; d++) { pf[d] = T(0); ph[d] = T(0); pfb[d] = T(0); pfd[d] = T(0); }
+        const bool xin = ix >= 0 && ix < nnx;
+        const int64_t aj = (xin && jv) ? (int64_t)ix * plane + 3 * j : 0;
+        const int64_t ah = (xin && jhv) ? (int64_t)ix * plane + 3 * jh : 0;
         if constexpr (MODE == 4) {
+            // b and invD only: x0 = omega invD b is formed when the plane is
+            // staged
 #pragma unroll
-            for (int d = 0; d < 3; d++) { phb[d] = T(0); phd[d] = T(0); }
-        }
-        if (ix >= 0 && ix < nnx) {
-            if constexpr (MODE == 4) {
-                // b and invD only: x0 = omega invD b is formed when the plane
-                // is staged, so the loads stay in flight over a plane step
-                const int64_t o = (int64_t)ix * plane;
-                if (jv) {
+            for (int d = 0; d < 3; d++) {
+                pfb[d] = bvec[aj + d];
+                pfd[d] = invD[aj + d];
+                phb[d] = bvec[ah + d];
+                phd[d] = invD[ah + d];
+            }
+        } else {
 #pragma unroll
-                    for (int d = 0; d < 3; d++) {
-                        pfb[d] = bvec[o + 3 * j + d];
-                        pfd[d] = invD[o + 3 * j + d];
-                    }
-                }
-                if (jhv) {
-#pragma unroll
-                    for (int d = 0; d < 3; d++) {
-                        phb[d] = bvec[o + 3 * jh + d];
-                        phd[d] = invD[o + 3 * jh + d];
-                    }
-                }
-            } else {
-                const T* src = p + (int64_t)ix * plane;
-                if (jv) {
-#pragma unroll
-                    for (int d = 0; d < 3; d++) pf[d] = src[3 * j + d];
-                }
-                if (jhv) {
-#pragma unroll
-                    for (int d = 0; d < 3; d++) ph[d] = src[3 * jh + d];
-                }
+            for (int d = 0; d < 3; d++) {
+                pf[d] = p[aj + d];
+                ph[d] = p[ah + d];
             }
         }
     };
@@ -256,12 +243,23 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(
     // thread's face transform of the staged plane
     auto stage_face = [&](int ixn, T (&f)[12]) {
         T* sb = sbuf(ixn - 1);
+        // the prefetched plane's values, 0 for a lane with no node there
+        const bool xin = ixn - 1 >= 0 && ixn - 1 < nnx;
+        const bool okj = xin && jv, okh = xin && jhv;
         if constexpr (MODE == 4) {
             // x0 = 0 + omega invD b, as the separate first sweep forms it
 #pragma unroll
             for (int d = 0; d < 3; d++) {
+                pfb[d] = okj ? pfb[d] : T(0);
+                pfd[d] = okj ? pfd[d] : T(0);
                 pf[d] = xfma(omega * pfd[d], pfb[d], T(0));
-                ph[d] = xfma(omega * phd[d], phb[d], T(0));
+                ph[d] = okh ? xfma(omega * phd[d], phb[d], T(0)) : T(0);
+            }
+        } else {
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                pf[d] = okj ? pf[d] : T(0);
+                ph[d] = okh ? ph[d] : T(0);
             }
         }
         T own[3] = {pf[0], pf[1], pf[2]};
@@ -309,10 +307,12 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(
     double cpair = 0.0;   // this thread's share of the current plane pair (MODE 1, 5)
 #pragma unroll
     for (int k = 0; k < 12; k++) ca[k] = T(0);
-    // element modulus of element plane ex (0 outside the grid), read one
-    // step before it is used
+    // element modulus of element plane ex, read one step before it is used
+    // (unconditionally, entry 0 for a lane with no element there; eok says
+    // whether the value is the element's)
     auto emod = [&](int ex) -> TE {
-        return (elem_yz && ex >= 0 && ex < g.nelx) ? E[(int64_t)ex * eplane + erow] : TE(0);
+        const bool ok = elem_yz && ex >= 0 && ex < g.nelx;
+        return E[ok ? (int64_t)ex * eplane + erow : 0];
     };
     TE enext = emod(X0 - 1);
     fetch(X0 - 1);
@@ -321,6 +321,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(
     // ex+1 (staged here into fnew); completes output node plane ex
     auto step = [&](const int ex, const T (&fprev)[12], T (&fnew)[12], const T (&cin)[12], T (&cout)[12]) {
         const TE ev = enext;
+        const bool eok = elem_yz && ex >= 0 && ex < g.nelx;
         enext = emod(ex + 1);
         // operands of this step's output epilogue (node plane ex), loaded
         // before the element work so their latency overlaps it
@@ -332,15 +333,18 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(
 #pragma unroll
             for (int d = 0; d < 3; d++) { eb[d] = stgb[d]; ed[d] = stgd[d]; }
         }
+        // (unconditional, entry 0 for a lane with no output node: only the
+        // output nodes' values are used)
         uint8_t ef[3] = {0, 0, 0};
         double er[3] = {0.0, 0.0, 0.0};
-        if (outstep && MODE >= 1) {
+        if constexpr (MODE >= 1) {
+            const int64_t ob = outstep ? obase : 0;
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                ef[d] = fixed[obase + d];
-                if (MODE == 2 || MODE == 3 || MODE == 5) eb[d] = bvec[obase + d];
-                if (MODE == 3 || MODE == 5) ed[d] = invD[obase + d];
-                if (MODE == 5) er[d] = rdot[obase + d];
+                ef[d] = fixed[ob + d];
+                if (MODE == 2 || MODE == 3 || MODE == 5) eb[d] = bvec[ob + d];
+                if (MODE == 3 || MODE == 5) ed[d] = invD[ob + d];
+                if (MODE == 5) er[d] = rdot[ob + d];
             }
         }
         T A[12], Bq[12];
@@ -366,7 +370,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(
                 Bq[k] = fprev[k] - fnew[k];
             }
         }
-        const T Ee = (double)ev > thr ? (T)ev : T(0);
+        const T Ee = (eok && (double)ev > thr) ? (T)ev : T(0);
         had_block<T>(A, Bq, co);
         // inverse x butterfly: face at x=0 (plane ex) and x=1 (plane ex+1),
         // scaled by the element modulus
