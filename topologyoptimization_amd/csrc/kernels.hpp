@@ -592,9 +592,17 @@ __device__ __forceinline__ void sum_partials(const double* part, int k0, int nr,
 #pragma unroll
             for (int u = 0; u < 8; u++) a[u] += row[b + u * kRedThreads];
         }
+        // the tail's loads are issued together (a column past the end reads
+        // entry 0 and adds an exact +0: the accumulators start at +0 and
+        // never hold -0)
+        double v[8];
 #pragma unroll
-        for (int u = 0; u < 8; u++)
-            if (b + u * kRedThreads < nb) a[u] += row[b + u * kRedThreads];
+        for (int u = 0; u < 8; u++) {
+            const int c = b + u * kRedThreads;
+            v[u] = row[c < nb ? c : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) a[u] += (b + u * kRedThreads < nb) ? v[u] : 0.0;
         w[i] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
     }
     block_reduce<NV>(w, sh);

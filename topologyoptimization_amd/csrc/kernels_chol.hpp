@@ -542,13 +542,19 @@ __global__ void __launch_bounds__(64 * NW) k_tri_colpart(const TK* __restrict__ 
         if (t < kTcRows && i < n) {
 #pragma unroll
             for (int q = 0; q < NR; q++) {
-                double ss[4];
+                double ss[4] = {0.0, 0.0, 0.0, 0.0};
+                const double* pq = ypart + (int64_t)q * ynch * n + i;
+                for (int c0 = 0; c0 < ynch; c0 += 16) {
+                    // 16 chunks' loads together, then added in ascending
+                    // order per sub (past the end: entry 0, an exact +0)
+                    double v[16];
 #pragma unroll
-                for (int sub = 0; sub < 4; sub++) {
-                    double a = 0.0;
-                    const double* pq = ypart + (int64_t)q * ynch * n + i;
-                    for (int c = sub; c < ynch; c += 4) a += pq[(int64_t)c * n];
-                    ss[sub] = a;
+                    for (int u = 0; u < 16; u++) {
+                        const int c = c0 + u;
+                        v[u] = pq[(int64_t)(c < ynch ? c : 0) * n];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 16; u++) ss[u & 3] += (c0 + u < ynch) ? v[u] : 0.0;
                 }
                 const double sum = (ss[0] + ss[1]) + (ss[2] + ss[3]);
                 ysh[q][t] = (TB)(ys ? ys[i] * sum : sum);
@@ -620,8 +626,19 @@ __global__ void k_tri_sum(const double* __restrict__ part, int nch, int64_t n, T
         if (gate && gate[(int64_t)q * kGateStride]) continue;
         double s = 0.0;
         if (ok) {
+            // chunks sub, sub + 4, ... in ascending order, loaded four at a
+            // time (past the end: entry 0, adding an exact +0)
             const double* pq = part + (int64_t)q * nch * n + i;
-            for (int c = sub; c < nch; c += 4) s += pq[(int64_t)c * n];
+            for (int c0 = sub; c0 < nch; c0 += 16) {
+                double v[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int c = c0 + 4 * u;
+                    v[u] = pq[(int64_t)(c < nch ? c : 0) * n];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++) s += (c0 + 4 * u < nch) ? v[u] : 0.0;
+            }
         }
         s += __shfl_xor(s, 1, 64);
         s += __shfl_xor(s, 2, 64);
