@@ -1051,7 +1051,7 @@ __global__ void __launch_bounds__(256) k_restrict(const T* __restrict__ res, T* 
 // and min(C+1, nC-1) per axis (pw1d of f gives c0 = C), so 8 coarse values
 // serve two fine nodes, and consecutive threads update consecutive z-pairs.
 template <typename T>
-__global__ void k_prolong_add(T* __restrict__ x, const T* __restrict__ xC, Grid gf, Grid gc,
+__global__ void __launch_bounds__(256) k_prolong_add(T* __restrict__ x, const T* __restrict__ xC, Grid gf, Grid gc,
                               const uint8_t* __restrict__ fixedF, const int* gate) {
 #pragma clang fp contract(off)
     gate = batch_gate(gate);
@@ -1069,9 +1069,11 @@ __global__ void k_prolong_add(T* __restrict__ x, const T* __restrict__ xC, Grid 
         const int cz = (int)(u - t * (uint32_t)nzC), cy = (int)(t - ux * (uint32_t)nyC), cx = (int)ux;
         const int dxf = q & 1, dyf = q >> 1;
         const int fx = 2 * cx + dxf, fy = 2 * cy + dyf;
-        if (fx >= nxF || fy >= nyF) continue;
+        const bool inxy = fx < nxF && fy < nyF;
         const int c1[3] = {min(cx + 1, nxC - 1), min(cy + 1, nyC - 1), min(cz + 1, nzC - 1)};
         const int c0[3] = {cx, cy, cz};
+        // every load first, none behind a branch (a fine node outside the
+        // grid reads entry 0 and is not written)
         T v8[8][3];
 #pragma unroll
         for (int k = 0; k < 8; k++) {
@@ -1080,25 +1082,36 @@ __global__ void k_prolong_add(T* __restrict__ x, const T* __restrict__ xC, Grid 
 #pragma unroll
             for (int d = 0; d < 3; d++) v8[k][d] = xC[3 * n + d];
         }
+        bool ok[2];
+        int64_t i0[2];
+        T xv[2][3];
+        uint8_t fv[2][3];
+#pragma unroll
+        for (int dzf = 0; dzf < 2; dzf++) {
+            const int fz = 2 * cz + dzf;
+            ok[dzf] = inxy && fz < nzF;
+            i0[dzf] = ok[dzf] ? 3 * (((int64_t)fx * nyF + fy) * nzF + fz) : 0;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                xv[dzf][d] = x[i0[dzf] + d];
+                fv[dzf][d] = fixedF[i0[dzf] + d];
+            }
+        }
         const double tx = (c0[0] != c1[0] && dxf) ? 0.5 : 0.0;
         const double ty = (c0[1] != c1[1] && dyf) ? 0.5 : 0.0;
         const double wx[2] = {1 - tx, tx}, wy[2] = {1 - ty, ty};
 #pragma unroll
         for (int dzf = 0; dzf < 2; dzf++) {
-            const int fz = 2 * cz + dzf;
-            if (fz >= nzF) continue;
             const double tz = (c0[2] != c1[2] && dzf) ? 0.5 : 0.0;
             const double wz[2] = {1 - tz, tz};
-            const int64_t nf = ((int64_t)fx * nyF + fy) * nzF + fz;
             // no branch on the mask: a fixed DOF adds an exact zero
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                const int64_t i = 3 * nf + d;
                 T v = v8[0][d] * (T)((wx[0] * wy[0]) * wz[0]);
 #pragma unroll
                 for (int k = 1; k < 8; k++) v = v + v8[k][d] * (T)((wx[k & 1] * wy[(k >> 1) & 1]) * wz[k >> 2]);
-                const T xi = x[i];
-                x[i] = fixedF[i] ? xi : xi + v;
+                const T xi = xv[dzf][d];
+                if (ok[dzf]) x[i0[dzf] + d] = fv[dzf][d] ? xi : xi + v;
             }
         }
     }

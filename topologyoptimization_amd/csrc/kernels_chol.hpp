@@ -536,10 +536,16 @@ __global__ void __launch_bounds__(64 * NW) k_tri_colpart(const TK* __restrict__ 
     // in ascending order for sub = 0..3, then (s0 + s1) + (s2 + s3), the
     // order of k_tri_sum's four lanes
     __shared__ TB ysh[NR][kTcRows];
+    // this wave's rows and the first nonzero column of its rows, read before
+    // the row sums below so the two loads overlap
+    const int64_t r0 = I0 + (int64_t)w * (kTcRows / NW);
+    const int64_t r1 = r0 + kTcRows / NW < n ? r0 + kTcRows / NW : n;
+    const int64_t rf0 = rfirst ? (int64_t)rfirst[r0 < n ? r0 : 0] : 0;
     if (ypart) {
         const int t = threadIdx.x;
         const int64_t i = I0 + t;
         if (t < kTcRows && i < n) {
+            const double ysv = ys ? ys[i] : 1.0;
 #pragma unroll
             for (int q = 0; q < NR; q++) {
                 double ss[4] = {0.0, 0.0, 0.0, 0.0};
@@ -557,7 +563,7 @@ __global__ void __launch_bounds__(64 * NW) k_tri_colpart(const TK* __restrict__ 
                     for (int u = 0; u < 16; u++) ss[u & 3] += (c0 + u < ynch) ? v[u] : 0.0;
                 }
                 const double sum = (ss[0] + ss[1]) + (ss[2] + ss[3]);
-                ysh[q][t] = (TB)(ys ? ys[i] * sum : sum);
+                ysh[q][t] = (TB)(ys ? ysv * sum : sum);
             }
         }
         __syncthreads();
@@ -567,9 +573,7 @@ __global__ void __launch_bounds__(64 * NW) k_tri_colpart(const TK* __restrict__ 
     for (int q = 0; q < NR; q++)
 #pragma unroll
         for (int u = 0; u < VN; u++) acc[q][u] = TA(0);
-    const int64_t r0 = I0 + (int64_t)w * (kTcRows / NW);
-    const int64_t r1 = r0 + kTcRows / NW < n ? r0 + kTcRows / NW : n;
-    const bool wz = rfirst && r0 < n && (int64_t)rfirst[r0] > J0 + kTcCols - 1;   // wave-uniform
+    const bool wz = rfirst && r0 < n && rf0 > J0 + kTcCols - 1;   // wave-uniform
     if (colok && !wz) {
 #pragma unroll 8
         for (int64_t i = r0; i < r1; i++) {
