@@ -107,6 +107,41 @@ def test_autotuner_and_simp_steps(gpu):
         assert abs(a.compliance - b.compliance) <= 1e-6 * abs(b.compliance)
 
 
+@pytest.mark.parametrize("prec", [0, 2])
+def test_workgroup_size_does_not_change_results(gpu, monkeypatch, prec):
+    """256-, 512- and 1024-thread x-march workgroups (every level, fixed
+    x-chunk) give the same bits for the operator on every level and for the
+    V-cycle, FP64 and FP32: each node's sums are formed in the same order
+    whatever the workgroup. (The CG's partial sums are per yz tile, so the
+    fine level keeps one workgroup size; the autotuner varies it on the
+    coarser levels only.)"""
+    T = gpu
+    nel, ndof = T.dims(40, 20, 20)
+    x = P.splitmix64_uniform(nel, lo=0.2)
+    r = np.random.default_rng(7).standard_normal(ndof)
+    out = []
+    for nt in (256, 512, 1024):
+        monkeypatch.setenv("TOPO_XM_NT", str(nt))
+        monkeypatch.setenv("TOPO_XM_CX", "4")
+        cfg, fixed, F, _, _, _ = P.cantilever(40, 20, 20, linear_solver=T.SOLVER_MGPCG, precision=prec,
+                                              mg_levels=3, cg_tol=1e-8, galerkin_max_ndof=0)
+        with T.Solver(cfg, fixed, F) as S:
+            S.fe(x, 3.0)   # builds the hierarchy for the design
+            res = [S.vcycle(r)]
+            precs = [T.PREC_FP64] + ([T.PREC_MIXED] if prec else [])
+            for pr in precs:
+                if pr != T.PREC_FP64:
+                    res.append(S.vcycle(r, prec=pr))
+                for lev, dims in enumerate(((40, 20, 20), (20, 10, 10), (10, 5, 5))):
+                    p = np.random.default_rng(lev).standard_normal(T.dims(*dims)[1])
+                    res.append(S.applyA(p, level=lev, prec=pr))
+            out.append(res)
+    for res in out[1:]:
+        assert len(res) == len(out[0])
+        for a, b in zip(out[0], res):
+            assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("prec", [0, 1])
 def test_tile_shape_does_not_change_results(gpu, prec):
     """The autotuned element tiles give the same bits as the default ones:

@@ -1016,7 +1016,7 @@ struct MGH {
     double thr = 1e-6;   // skip threshold 1000 * Emin
     bool want32 = false;
     bool exact = false;   // reference-order kernel everywhere
-    int xm_nt = 0;        // x-march workgroup size for every level (0 auto, 512, 1024)
+    int xm_nt = 0;        // x-march workgroup size for every level (0 auto, 256, 512, 1024)
     int xm_cx = 0;        // x-march planes per workgroup for every level (0 auto)
     int xm_cx64 = 0;      // x-march planes per workgroup of every FP64 operator (0: autotuned / as FP32)
     // FP32 operator on the x-march levels with the element product on the
@@ -1749,6 +1749,7 @@ struct MGH {
     int xm_pick_nt(int li) const {
         const int H = L[li].g.nnz() + 1;
         const int nt_sel = xm_nt ? xm_nt : (li < (int)lv_nt.size() ? lv_nt[li] : 0);
+        if (nt_sel == 256 && 2 * H <= 256) return 256;
         return (nt_sel == 1024 || (nt_sel == 0 && 4 * H > 512)) ? 1024 : 512;
     }
 
@@ -1856,6 +1857,8 @@ struct MGH {
     }
                 if (NT == 512) {
                     TOPO_XM_MODES(512)
+                } else if (NT == 256) {
+                    TOPO_XM_MODES(256)
                 } else {
                     TOPO_XM_MODES(1024)
                 }
@@ -2736,23 +2739,36 @@ struct Solver {
                                        (const double*)v64.r, l.ndof);
                 const int H = l.g.nnz() + 1;
                 const int nt_rule = 4 * H > 512 ? 1024 : 512;
-                mg.lv_nt[li] = 0;
-                double ms[kNcx];
-                sweep_cx(ms, cand, [&](int cx) {
-                    mg.lv_cx[li] = cx;
-                    if (f32) mg.apply<float>(C, li, l.x32, l.x232, 1, nullptr, nullptr);
-                    else mg.apply<double>(C, li, l.x, l.x2, 1, nullptr, nullptr);
-                });
-                int bi = 0;
-                for (int b = 0; b < kNcx; b++) {
-                    tune_log.push_back({li, nt_rule, cand[b], ms[b]});
-                    if (ms[b] < ms[bi]) bi = b;
+                // the coarser levels also try 256-thread workgroups (more
+                // workgroups on grids too small to fill the chip; the sums
+                // are the same bits for every workgroup size)
+                const int nnt = (li > 0 && 2 * H <= 256) ? 2 : 1;
+                double best = 1e300;
+                int best_nt = 0, best_cx = 0;
+                for (int ti = 0; ti < nnt; ti++) {
+                    const int ntv = ti == 0 ? nt_rule : 256;
+                    mg.lv_nt[li] = ti == 0 ? 0 : 256;
+                    double ms[kNcx];
+                    sweep_cx(ms, cand, [&](int cx) {
+                        mg.lv_cx[li] = cx;
+                        if (f32) mg.apply<float>(C, li, l.x32, l.x232, 1, nullptr, nullptr);
+                        else mg.apply<double>(C, li, l.x, l.x2, 1, nullptr, nullptr);
+                    });
+                    for (int b = 0; b < kNcx; b++) {
+                        tune_log.push_back({li, ntv, cand[b], ms[b]});
+                        if (ms[b] < best) {
+                            best = ms[b];
+                            best_nt = mg.lv_nt[li];
+                            best_cx = cand[b];
+                        }
+                    }
                 }
+                mg.lv_nt[li] = best_nt;
                 if (li < 2) {
-                    tuned_nt[li] = nt_rule;
-                    tuned_cx[li] = cand[bi];
+                    tuned_nt[li] = best_nt ? best_nt : nt_rule;
+                    tuned_cx[li] = best_cx;
                 }
-                mg.lv_cx[li] = cand[bi];
+                mg.lv_cx[li] = best_cx;
             }
             // the mixed solvers' CG operator (FP64, masked, finest level):
             // its own x-chunk, as FP64 arithmetic and registers give it
