@@ -1204,19 +1204,29 @@ struct MGH {
         }
         omegaL.assign(L.size(), omega);
         const int nl = (int)std::min<size_t>(L.size(), 16);
+        // the power iterations of every level run on the second stream,
+        // beside the coarse factorization (which does not depend on the
+        // smoother weights): the second stream waits for the work issued so
+        // far, and their launches are issued once the factorization's are
+        // (issued first, their 60 launches per level held the start of the
+        // factorization's chain back by their issue time); their results
+        // are read after it
+        std::function<void()> power;
         if (omega_cap > 0) {
             if (!d_psc) d_psc = c.alloc<double>(4 * 16);
-            // the power iterations of every level run on the second stream,
-            // overlapping the coarse factorization (which does not depend on
-            // the smoother weights); their results are read after it
             c.side_ready(NB_EV_MIN);
             c.order(c.s, c.s2, 0);
-            std::swap(c.s, c.s2);
-            for (int li = 0; li < nl; li++) power_launch(c, li, d_psc + 4 * li);
-            std::swap(c.s, c.s2);
-            c.check_launch();
+            power = [this, &c, nl] {
+                std::swap(c.s, c.s2);
+                for (int li = 0; li < nl; li++) power_launch(c, li, d_psc + 4 * li);
+                std::swap(c.s, c.s2);
+                c.check_launch();
+            };
         }
-        if (coarse_solver == TOPO_COARSE_CHOLESKY) factor_coarsest(c);
+        if (coarse_solver == TOPO_COARSE_CHOLESKY)
+            factor_coarsest(c, power ? &power : nullptr);
+        else if (power)
+            power();
         if (omega_cap > 0) {
             TOPO_HIP(hipStreamSynchronize(c.s2));
             double h[64];
@@ -1277,8 +1287,18 @@ struct MGH {
                                l.freeIdx, l.freeOf, nf, ld, (const double*)K0->ke, l.Kinv);
         hipLaunchKernelGGL(k_regularize, dim3((unsigned)ld), dim3(256), 0, c.s, l.Kinv, ld);
     }
-    void factor_coarsest(Ctx& c) {
+    // `overlap` (may be null) is called once, after the nested-dissection
+    // factorization's launches are issued and before the host waits for
+    // it; on the other paths (which use the side streams) before they start
+    void factor_coarsest(Ctx& c, const std::function<void()>* overlap = nullptr) {
         MGLevel& l = L.back();
+        bool ov_done = false;
+        auto run_overlap = [&] {
+            if (overlap && !ov_done) {
+                ov_done = true;
+                (*overlap)();
+            }
+        };
         if (!l.freeIdx) {
             std::vector<uint8_t> hf(l.ndof);
             TOPO_HIP(hipMemcpy(hf.data(), l.fixed, l.ndof, hipMemcpyDeviceToHost));
@@ -1372,6 +1392,7 @@ struct MGH {
             hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, c.s, dflag, 0, (int64_t)1, (const int*)nullptr);
             nd_factor(c, l, dflag);
             c.check_launch();
+            run_overlap();
             TOPO_HIP(hipMemcpyAsync(&c.h_sc[0], dflag, sizeof(int), hipMemcpyDeviceToHost, c.s));
             c.sync();
             int flag = 0;
@@ -1382,6 +1403,7 @@ struct MGH {
                 l.tri = true;
             }
         } else if (NB > 2 * wb) {
+            run_overlap();
             if (!l.Z) {
                 l.Z = c.alloc<double>(n * n);
                 l.Linv = c.alloc<double>((int64_t)NB * kCB * kCB);
@@ -1446,6 +1468,7 @@ struct MGH {
                 coarse_block(c, l);   // the factorization overwrote the block
             }
         }
+        run_overlap();
         if (!done) {
             // not positive definite (or too small a block to band): dense
             // Gauss-Jordan inverse with pivot dropping
