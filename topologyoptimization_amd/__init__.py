@@ -477,8 +477,8 @@ class Solver:
     def tuning(self):
         """(rhs_tile, xm_nt, xm_cx, log[n x (R, NT, CX, ms per RHS)]) of the batched-solve autotuner"""
         r, nt, cx = C.c_int32(), C.c_int32(), C.c_int32()
-        log = np.zeros((64, 4))
-        n = _check(lib().topo_solver_tuning(self.h, C.byref(r), C.byref(nt), C.byref(cx), _p(log), 64))
+        log = np.zeros((256, 4))
+        n = _check(lib().topo_solver_tuning(self.h, C.byref(r), C.byref(nt), C.byref(cx), _p(log), 256))
         return r.value, nt.value, cx.value, log[:n]
 
     def helmholtz(self, v):

@@ -557,8 +557,8 @@ __global__ void k_cg_check(CGState* st, int nr) {
 // (an L2 write-back per block) serialises, ~20 ns per block.
 // Quantities: 0 = p.Ap, 1 = ||r||^2, 2 = r.z; row (k, q) of the partials
 // holds quantity k of right-hand side q for each block of its producer
-// (kCgBlocks for the vector kernels; p.Ap may also come from the x-march
-// applyA, one partial per workgroup, up to kCgStride).
+// (kCgBlocks for the vector kernels; p.Ap and r.z may also come from the
+// x-march kernels, one partial per (yz tile, node plane), up to kCgStride).
 constexpr int kCgBlocks = 1024;
 constexpr int kCgStride = 8192;
 constexpr int kCgRows = 4;   // right-hand sides per quantity

@@ -84,7 +84,7 @@ __global__ void __launch_bounds__(NT) k_applyA_mf(const float* __restrict__ p, f
     const int w = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
     if (w >= nwork) return;   // padding workgroup
     const int bx = w / gy, by = w - bx * gy;
-    __shared__ double s_red[16 * (NT / 64)];
+    __shared__ double s_red[32 * (NT / 64)];
     const float omega = omp ? (float)*omp : 0.f;
     extern __shared__ __align__(16) unsigned char mf_smem[];
     constexpr int SD = mf_sd<NT>();
@@ -201,7 +201,6 @@ __global__ void __launch_bounds__(NT) k_applyA_mf(const float* __restrict__ p, f
     };
 
     float carry[3] = {0.f, 0.f, 0.f};   // x = 1 face sums of the previous element plane at this node
-    double cpair = 0.0;
     fetch(X0 - 1);
     stage(X0 - 1);
     for (int ex = X0 - 1; ex < X1; ex++) {
@@ -314,22 +313,18 @@ __global__ void __launch_bounds__(NT) k_applyA_mf(const float* __restrict__ p, f
 #pragma unroll
             for (int d = 0; d < 3; d++) carry[d] = cn[d];
         }
-        // CG partials per (yz tile, node-plane pair), as k_applyA_xm
+        // CG partials per (yz tile, node plane), as k_applyA_xm
         if ((MODE == 1 || MODE == 5) && part && ex >= X0) {
-            cpair += cpl;
-            if (((ex - X0) & 1) || ex == X1 - 1) {
-                const double ws = wave_sum_dpp(cpair);
-                if ((t & 63) == 63) s_red[((ex - X0) >> 1) * (NT / 64) + (t >> 6)] = ws;
-                cpair = 0.0;
-            }
+            const double ws = wave_sum_dpp(cpl);
+            if ((t & 63) == 63) s_red[(ex - X0) * (NT / 64) + (t >> 6)] = ws;
         }
     }
     if ((MODE == 1 || MODE == 5) && part) {
         __syncthreads();
-        for (int k = t; 2 * k < X1 - X0; k += NT) {
+        for (int k = t; k < X1 - X0; k += NT) {
             double sum = 0.0;
             for (int wv = 0; wv < NT / 64; wv++) sum += s_red[k * (NT / 64) + wv];
-            part[(int64_t)blockIdx.y * kCgStride + (int64_t)bx * ((nnx + 1) / 2) + (X0 >> 1) + k] = sum;
+            part[(int64_t)blockIdx.y * kCgStride + (int64_t)bx * nnx + X0 + k] = sum;
         }
     }
 }

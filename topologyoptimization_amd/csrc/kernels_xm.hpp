@@ -141,13 +141,13 @@ inline size_t xm_lds_bytes(int nnz, size_t elt, int NT) {
 // output is the mode-3 sweep from x0 (the staged own b, invD and x0 serve the
 // epilogue). A template parameter so each variant holds only the registers
 // its epilogue needs. With `part` (MODE 1), each workgroup also leaves, per
-// pair k of output node planes (2k, 2k+1) of its yz tile bx, the pair's share
-// of p.Ap (own p of each output node times its output, FP64, fixed order) at
-// part[blockIdx.y * kCgStride + bx * ceil(nnx/2) + k] for the CG update
-// (gx * ceil(nnx/2) partials whatever the even CX, so the CG scalars do not
-// depend on the autotuned x-chunking); MODE 5 is the mode-3 sweep that
-// leaves the shares of r.z the same way (r = rdot, the CG's FP64 residual;
-// z = the sweep's output, the V-cycle's last). CX must be even.
+// output node plane ix of its yz tile bx, the plane's share of p.Ap (own p
+// of each output node times its output, FP64, fixed order) at
+// part[blockIdx.y * kCgStride + bx * nnx + ix] for the CG update (gx * nnx
+// partials whatever the CX, so the CG scalars do not depend on the
+// autotuned x-chunking); MODE 5 is the mode-3 sweep that leaves the shares
+// of r.z the same way (r = rdot, the CG's FP64 residual; z = the sweep's
+// output, the V-cycle's last). CX <= 32.
 // (The FP64 512-thread variants up to MODE 2 -- the CG operator and the
 // residual -- are held to four waves per SIMD, two workgroups per CU.)
 template <typename T, typename TE, int NT, int MODE>
@@ -176,7 +176,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(
     const int w = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
     if (w >= nwork) return;   // padding workgroup
     const int bx = w / gy, by = w - bx * gy;
-    __shared__ double s_red[16 * (NT / 64)];   // per-plane-pair wave sums (MODE 1, 5 with part; CX <= 32)
+    __shared__ double s_red[32 * (NT / 64)];   // per-plane wave sums (MODE 1, 5 with part; CX <= 32)
     const T omega = omp ? (T)*omp : T(0);
     extern __shared__ __align__(16) unsigned char xm_smem[];
     const int nnz = g.nnz(), nny = g.nny(), nnx = g.nnx();
@@ -304,7 +304,6 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(
     // trade roles every step (the march is unrolled by two below, so no
     // register copies at the loop edge)
     T fa[12], fb[12], ca[12], cb[12];
-    double cpair = 0.0;   // this thread's share of the current plane pair (MODE 1, 5)
 #pragma unroll
     for (int k = 0; k < 12; k++) ca[k] = T(0);
     // element modulus of element plane ex, read one step before it is used
@@ -417,19 +416,13 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(
                 out[obase + d] = v;
             }
         }
-        // one partial per (yz tile, node-plane pair 2k, 2k+1): the same
-        // sums whatever the (even) x-chunking, so the CG scalars do not
-        // depend on the element tiles. Each thread adds the pair's two
-        // planes, then each wave sums the pair (DPP, lane 63) into its own
-        // LDS slot of the pair (no barrier here); the slots are added after
-        // the march.
+        // one partial per (yz tile, node plane): the same sums whatever the
+        // x-chunking, so the CG scalars do not depend on the element tiles.
+        // Each wave sums the plane (DPP, lane 63) into its own LDS slot of
+        // the plane (no barrier here); the slots are added after the march.
         if ((MODE == 1 || MODE == 5) && part && ex >= X0) {
-            cpair += cpl;
-            if (((ex - X0) & 1) || ex == X1 - 1) {
-                const double ws = wave_sum_dpp(cpair);
-                if ((t & 63) == 63) s_red[((ex - X0) >> 1) * (NT / 64) + (t >> 6)] = ws;
-                cpair = 0.0;
-            }
+            const double ws = wave_sum_dpp(cpl);
+            if ((t & 63) == 63) s_red[(ex - X0) * (NT / 64) + (t >> 6)] = ws;
         }
     };
     if constexpr (RECOMP) {
@@ -445,10 +438,10 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(
     }
     if ((MODE == 1 || MODE == 5) && part) {
         __syncthreads();
-        for (int k = t; 2 * k < X1 - X0; k += NT) {
+        for (int k = t; k < X1 - X0; k += NT) {
             double sum = 0.0;
             for (int wv = 0; wv < NT / 64; wv++) sum += s_red[k * (NT / 64) + wv];
-            part[(int64_t)blockIdx.y * kCgStride + (int64_t)bx * ((nnx + 1) / 2) + (X0 >> 1) + k] = sum;
+            part[(int64_t)blockIdx.y * kCgStride + (int64_t)bx * nnx + X0 + k] = sum;
         }
     }
 }

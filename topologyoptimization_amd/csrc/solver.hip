@@ -1019,6 +1019,7 @@ struct MGH {
     int xm_nt = 0;        // x-march workgroup size for every level (0 auto, 256, 512, 1024)
     int xm_cx = 0;        // x-march planes per workgroup for every level (0 auto)
     int xm_cx64 = 0;      // x-march planes per workgroup of every FP64 operator (0: autotuned / as FP32)
+    int ncu = 256;        // compute units of the device (the x-chunk size rule)
     // FP32 operator on the x-march levels with the element product on the
     // matrix cores (k_applyA_mf) instead of the VALU Hadamard kernel
     bool mfma = true;
@@ -1088,6 +1089,12 @@ struct MGH {
     // (js/optimizer-worker-3d.js:408-442); nlevels <= 0 = reference rule
     void build(Ctx& c, Grid g0, const uint8_t* dfixed0, int nlevels, bool fp32) {
         want32 = fp32;
+        {
+            int dev = 0, n = 0;
+            if (hipGetDevice(&dev) == hipSuccess &&
+                hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+                ncu = n;
+        }
         xm_allow_lds();
         // the transfer kernels index nodes in 32 bits
         TOPO_CHECK(g0.nnodes() < ((int64_t)1 << 32), TOPO_E_ARG, "grid of %lld nodes too large",
@@ -1777,14 +1784,16 @@ struct MGH {
     }
 
     // the size rule for the x-planes per workgroup of level li's x-march
-    // operator with NT-thread workgroups (before clamping to an even 2..32)
+    // operator with NT-thread workgroups (before clamping to 1..32): as many
+    // x-chunks per yz tile as fit the chip in one round of workgroups at
+    // four waves per SIMD (ncu * 1024 / NT of them), the planes split evenly
     int xm_auto_cx(int li, int NT) const {
         const Grid& g = L[li].g;
         const int H = g.nnz() + 1;
         const int gx = (g.nny() * g.nnz() + (NT - H) - 1) / (NT - H);
-        const int cx = (int)((int64_t)g.nnx() * gx * (NT / 512) / 1024);
-        const int c2 = std::max(2, std::min(32, cx));
-        return c2 + (c2 & 1);
+        const int gy = std::max(1, (int)((int64_t)ncu * 1024 / NT / gx));
+        const int cx = (g.nnx() + gy - 1) / gy;
+        return std::max(1, std::min(32, cx));
     }
 
     // x-planes per workgroup of level li's x-march operator in precision T (0 auto)
@@ -1832,15 +1841,12 @@ struct MGH {
                 const int gx = (nyz + (NT - H) - 1) / (NT - H);
                 int CX = cx_sel > 0 ? cx_sel : xm_auto_cx(li, NT);
                 CX = std::max(1, std::min(32, CX));
-                // CG partials: one per (yz tile, node-plane pair), whatever CX
-                const int64_t nplane = (int64_t)gx * ((l.g.nnx() + 1) / 2);
+                // CG partials: one per (yz tile, node plane), whatever CX
+                const int64_t nplane = (int64_t)gx * l.g.nnx();
                 double* part = ((mode == 1 || mode == 5) && pap && nplane <= kCgStride &&
                                 !(li == 0 && springs.n > 0)) ? pap : nullptr;
                 if (mode == 5 && !part) mode = 3;
-                if (part) {
-                    npart = (int)nplane;
-                    CX = std::max(2, CX + (CX & 1));   // even: the partials cover whole node-plane pairs
-                }
+                if (part) npart = (int)nplane;
                 const int gy = (l.g.nnx() + CX - 1) / CX;
                 const dim3 grid(8 * ((gx * gy + 7) / 8), nr);
                 bool on_mfma = false;
@@ -2729,13 +2735,16 @@ struct Solver {
             return (double)ms / reps;
         };
         tune_log.clear();
-        // even (the CG partials cover node-plane pairs); the large ones let a
-        // launch fit the chip in one round of workgroups
-        constexpr int kNcx = 12;
-        const int cx_c[kNcx] = {0, 2, 4, 6, 8, 10, 12, 16, 18, 20, 24, 32};
-        // the coarser levels (no CG partials there) also try odd chunks: one
-        // or three planes per workgroup, for latency-bound small grids
-        const int cx_s[kNcx] = {0, 1, 2, 3, 4, 6, 8, 10, 12, 16, 20, 32};
+        // candidate 0 is the size rule (one round of workgroups); around it
+        // every chunk from 4 to 12 planes (a launch's time goes with its
+        // largest chunk, and one plane more or less moves the round's fill),
+        // and the large ones that fit the chip in one round at fewer
+        // workgroups per CU
+        constexpr int kNcx = 14;
+        const int cx_c[kNcx] = {0, 4, 5, 6, 7, 8, 9, 10, 11, 12, 16, 20, 24, 32};
+        // the coarser levels: also one to three planes per workgroup, for
+        // latency-bound small grids
+        const int cx_s[kNcx] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 16, 20, 32};
         // per candidate the least of two timings of 5 launches, the two
         // rounds over all candidates one after the other (clock and cache
         // drift between candidates evens out)
