@@ -391,13 +391,19 @@ static bool mf_allow_lds() {
 }
 
 // The FP64 x-march kernels keep two node planes in LDS (up to ~74 KB with
-// 512 threads, nnz <= 255; ~147 KB with 1024 threads, nnz <= 511): grant
-// it; a device that refuses cannot run them.
+// 512 threads, nnz <= 255; ~147 KB with 1024 threads, nnz <= 511), and the
+// FP32 1024-thread ones need up to ~74 KB (nnz <= 511): grant it; a device
+// that refuses cannot run them.
 static void xm_allow_lds() {
     static bool done = false;
     if (done) return;
     const int b512 = (int)xm_lds_bytes(255, sizeof(double), 512);
     const int b1024 = (int)xm_lds_bytes(511, sizeof(double), 1024);
+    const int s1024 = (int)xm_lds_bytes(511, sizeof(float), 1024);
+    const void* g1024[6] = {(const void*)k_applyA_xm<float, float, 1024, 0>, (const void*)k_applyA_xm<float, float, 1024, 1>,
+                            (const void*)k_applyA_xm<float, float, 1024, 2>, (const void*)k_applyA_xm<float, float, 1024, 3>,
+                            (const void*)k_applyA_xm<float, float, 1024, 4>, (const void*)k_applyA_xm<float, float, 1024, 5>};
+    for (const void* f : g1024) TOPO_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, s1024));
     const void* f512[6] = {(const void*)k_applyA_xm<double, double, 512, 0>, (const void*)k_applyA_xm<double, double, 512, 1>,
                            (const void*)k_applyA_xm<double, double, 512, 2>, (const void*)k_applyA_xm<double, double, 512, 3>,
                            (const void*)k_applyA_xm<double, double, 512, 4>, (const void*)k_applyA_xm<double, double, 512, 5>};
