@@ -41,6 +41,16 @@ __device__ __forceinline__ double wave_sum_dpp(double v) {
 // small helpers
 __device__ __forceinline__ int64_t gtid() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
 __device__ __forceinline__ int64_t gstride() { return (int64_t)gridDim.x * blockDim.x; }
+// gtid() in XCD-aware block order, for a grid of a multiple of 8 blocks (the
+// identity otherwise): the hardware places block L on XCD L % 8, and block L
+// takes block (L % 8) * (gridDim.x / 8) + L / 8 of the items, so each XCD
+// works on one contiguous range and neighbouring items, which share operands
+// (the transfers' stencils), meet in one L2. Same stride as gtid().
+__device__ __forceinline__ int64_t xcd_gtid() {
+    const unsigned G = gridDim.x, L = blockIdx.x;
+    const unsigned vb = (G & 7u) ? L : (L & 7u) * (G >> 3) + (L >> 3);
+    return (int64_t)vb * blockDim.x + threadIdx.x;
+}
 
 // local index of the element node with offsets (a,b,c) in {0,1}^3
 // (order (0,0,0),(1,0,0),(1,1,0),(0,1,0),(0,0,1),(1,0,1),(1,1,1),(0,1,1))
@@ -1005,7 +1015,7 @@ __global__ void __launch_bounds__(256) k_restrict(const T* __restrict__ res, T* 
     bC += batch_off(gc.ndof());
     const int nxF = gf.nnx(), nyF = gf.nny(), nzF = gf.nnz();
     const int nxC = gc.nnx(), nyC = gc.nny(), nzC = gc.nnz();
-    for (int64_t nc = gtid(); nc < gc.nnodes(); nc += gstride()) {
+    for (int64_t nc = xcd_gtid(); nc < gc.nnodes(); nc += gstride()) {
         // 32-bit index arithmetic (node counts below 2^32)
         const uint32_t u = (uint32_t)nc, t = u / (uint32_t)nzC, cx = t / (uint32_t)nyC;
         const int cz = (int)(u - t * (uint32_t)nzC), cy = (int)(t - cx * (uint32_t)nyC);
@@ -1046,10 +1056,11 @@ __global__ void __launch_bounds__(256) k_restrict(const T* __restrict__ res, T* 
 
 // x += P xC on free fine DOFs (mgVcycle :977-1042): the 8 corner terms are
 // summed left to right in the order 000,100,010,110,001,101,011,111.
-// Thread (q, C), q = (dx, dy) in {0,1}^2, C a coarse node: the fine z-pair
-// f = (2Cx + dx, 2Cy + dy, 2Cz + {0,1}) (inside the grid) has the corners C
-// and min(C+1, nC-1) per axis (pw1d of f gives c0 = C), so 8 coarse values
-// serve two fine nodes, and consecutive threads update consecutive z-pairs.
+// Thread C, a coarse node: the fine nodes f = (2Cx + dx, 2Cy + dy, 2Cz + dz),
+// d* in {0,1} (inside the grid), have the corners C and min(C+1, nC-1) per
+// axis (pw1d of f gives c0 = C), so one load of the 8 coarse values serves
+// all eight; consecutive threads update consecutive z-pairs of each of the
+// four (dx, dy) rows.
 template <typename T>
 __global__ void __launch_bounds__(256) k_prolong_add(T* __restrict__ x, const T* __restrict__ xC, Grid gf, Grid gc,
                               const uint8_t* __restrict__ fixedF, const int* gate) {
@@ -1061,15 +1072,10 @@ __global__ void __launch_bounds__(256) k_prolong_add(T* __restrict__ x, const T*
     const int nxF = gf.nnx(), nyF = gf.nny(), nzF = gf.nnz();
     const int nxC = gc.nnx(), nyC = gc.nny(), nzC = gc.nnz();
     const int64_t ncn = gc.nnodes();
-    for (int64_t tq = gtid(); tq < 4 * ncn; tq += gstride()) {
-        const int q = (int)(tq / ncn);
-        const int64_t nc = tq - (int64_t)q * ncn;
+    for (int64_t nc = xcd_gtid(); nc < ncn; nc += gstride()) {
         // 32-bit index arithmetic (node counts below 2^32)
         const uint32_t u = (uint32_t)nc, t = u / (uint32_t)nzC, ux = t / (uint32_t)nyC;
         const int cz = (int)(u - t * (uint32_t)nzC), cy = (int)(t - ux * (uint32_t)nyC), cx = (int)ux;
-        const int dxf = q & 1, dyf = q >> 1;
-        const int fx = 2 * cx + dxf, fy = 2 * cy + dyf;
-        const bool inxy = fx < nxF && fy < nyF;
         const int c1[3] = {min(cx + 1, nxC - 1), min(cy + 1, nyC - 1), min(cz + 1, nzC - 1)};
         const int c0[3] = {cx, cy, cz};
         // every load first, none behind a branch (a fine node outside the
@@ -1082,36 +1088,46 @@ __global__ void __launch_bounds__(256) k_prolong_add(T* __restrict__ x, const T*
 #pragma unroll
             for (int d = 0; d < 3; d++) v8[k][d] = xC[3 * n + d];
         }
-        bool ok[2];
-        int64_t i0[2];
-        T xv[2][3];
-        uint8_t fv[2][3];
+        bool ok[4][2];
+        int64_t i0[4][2];
+        T xv[4][2][3];
+        uint8_t fv[4][2][3];
 #pragma unroll
-        for (int dzf = 0; dzf < 2; dzf++) {
-            const int fz = 2 * cz + dzf;
-            ok[dzf] = inxy && fz < nzF;
-            i0[dzf] = ok[dzf] ? 3 * (((int64_t)fx * nyF + fy) * nzF + fz) : 0;
+        for (int q = 0; q < 4; q++) {
+            const int fx = 2 * cx + (q & 1), fy = 2 * cy + (q >> 1);
+            const bool inxy = fx < nxF && fy < nyF;
 #pragma unroll
-            for (int d = 0; d < 3; d++) {
-                xv[dzf][d] = x[i0[dzf] + d];
-                fv[dzf][d] = fixedF[i0[dzf] + d];
+            for (int dzf = 0; dzf < 2; dzf++) {
+                const int fz = 2 * cz + dzf;
+                ok[q][dzf] = inxy && fz < nzF;
+                i0[q][dzf] = ok[q][dzf] ? 3 * (((int64_t)fx * nyF + fy) * nzF + fz) : 0;
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    xv[q][dzf][d] = x[i0[q][dzf] + d];
+                    fv[q][dzf][d] = fixedF[i0[q][dzf] + d];
+                }
             }
         }
-        const double tx = (c0[0] != c1[0] && dxf) ? 0.5 : 0.0;
-        const double ty = (c0[1] != c1[1] && dyf) ? 0.5 : 0.0;
-        const double wx[2] = {1 - tx, tx}, wy[2] = {1 - ty, ty};
 #pragma unroll
-        for (int dzf = 0; dzf < 2; dzf++) {
-            const double tz = (c0[2] != c1[2] && dzf) ? 0.5 : 0.0;
-            const double wz[2] = {1 - tz, tz};
-            // no branch on the mask: a fixed DOF adds an exact zero
+        for (int q = 0; q < 4; q++) {
+            const int dxf = q & 1, dyf = q >> 1;
+            const double tx = (c0[0] != c1[0] && dxf) ? 0.5 : 0.0;
+            const double ty = (c0[1] != c1[1] && dyf) ? 0.5 : 0.0;
+            const double wx[2] = {1 - tx, tx}, wy[2] = {1 - ty, ty};
 #pragma unroll
-            for (int d = 0; d < 3; d++) {
-                T v = v8[0][d] * (T)((wx[0] * wy[0]) * wz[0]);
+            for (int dzf = 0; dzf < 2; dzf++) {
+                const double tz = (c0[2] != c1[2] && dzf) ? 0.5 : 0.0;
+                const double wz[2] = {1 - tz, tz};
+                // no branch on the mask: a fixed DOF adds an exact zero
 #pragma unroll
-                for (int k = 1; k < 8; k++) v = v + v8[k][d] * (T)((wx[k & 1] * wy[(k >> 1) & 1]) * wz[k >> 2]);
-                const T xi = xv[dzf][d];
-                if (ok[dzf]) x[i0[dzf] + d] = fv[dzf][d] ? xi : xi + v;
+                for (int d = 0; d < 3; d++) {
+                    T v = v8[0][d] * (T)((wx[0] * wy[0]) * wz[0]);
+#pragma unroll
+                    for (int k = 1; k < 8; k++)
+                        v = v + v8[k][d] * (T)((wx[k & 1] * wy[(k >> 1) & 1]) * wz[k >> 2]);
+                    const T xi = xv[q][dzf][d];
+                    if (ok[q][dzf]) x[i0[q][dzf] + d] = fv[q][dzf][d] ? xi : xi + v;
+                }
             }
         }
     }

@@ -32,6 +32,8 @@ static inline int nblk(int64_t n, int bs = 256) {
     if (b > (1 << 20)) b = 1 << 20;
     return (int)b;
 }
+// nblk rounded up to a multiple of 8 blocks (the kernels using xcd_gtid)
+static inline int nblk8(int64_t n, int bs = 256) { return (nblk(n, bs) + 7) & ~7; }
 
 
 // ---------------------------------------------------------------------------
@@ -2020,10 +2022,10 @@ struct MGH {
         if (coarsest) return x;
         apply<T>(c, li, x, Sel<T>::res(l), 2, b, gate, nr);
         MGLevel& lc = L[li + 1];
-        hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(lc.g.nnodes()), nr), dim3(256), 0, c.s, (const T*)Sel<T>::res(l),
+        hipLaunchKernelGGL(k_restrict<T>, dim3(nblk8(lc.g.nnodes()), nr), dim3(256), 0, c.s, (const T*)Sel<T>::res(l),
                            Sel<T>::b(lc), l.g, lc.g, lc.fixed, gate);
         T* xc = vcycle<T>(c, li + 1, Sel<T>::b(lc), gate, nr);
-        hipLaunchKernelGGL(k_prolong_add<T>, dim3(nblk(4 * lc.g.nnodes()), nr), dim3(256), 0, c.s, x, (const T*)xc, l.g,
+        hipLaunchKernelGGL(k_prolong_add<T>, dim3(nblk8(lc.g.nnodes()), nr), dim3(256), 0, c.s, x, (const T*)xc, l.g,
                            lc.g, l.fixed, gate);
         for (int s = 0; s < nu2; s++) {
             const bool last = rdot && li == 0 && s == nu2 - 1;
