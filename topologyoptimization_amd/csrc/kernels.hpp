@@ -570,7 +570,7 @@ __global__ void k_cg_check(CGState* st, int nr) {
 // (kCgBlocks for the vector kernels; p.Ap and r.z may also come from the
 // x-march kernels, one partial per (yz tile, node plane), up to kCgStride).
 constexpr int kCgBlocks = 1024;
-constexpr int kCgStride = 8192;
+constexpr int kCgStride = 16384;
 constexpr int kCgRows = 4;   // right-hand sides per quantity
 __host__ __device__ __forceinline__ int64_t cg_row(int k, int q) { return (int64_t)(k * kCgRows + q) * kCgStride; }
 
