@@ -1005,14 +1005,21 @@ __device__ __forceinline__ void rw1d(int c, int nF, int nC, int (&f)[3], T (&w)[
 // an exact zero, so the sums are those of the contributors alone. The 27
 // fine node values are all loaded before the sums (the compiler otherwise
 // waits for each load before issuing the next).
-template <typename T>
+// With bf (the coarsest level's direct solve), the free coarse DOFs are
+// written there instead, at their positions freeOf[] of the solve's order
+// and rounded to TB as the solve's gather of bC would (bC is not written;
+// the positions without a DOF are left as they are: zero).
+template <typename T, typename TB = T>
 __global__ void __launch_bounds__(256) k_restrict(const T* __restrict__ res, T* __restrict__ bC, Grid gf, Grid gc,
-                                                  const uint8_t* __restrict__ fixedC, const int* gate) {
+                                                  const uint8_t* __restrict__ fixedC, const int* gate,
+                                                  TB* __restrict__ bf = nullptr,
+                                                  const int32_t* __restrict__ freeOf = nullptr, int64_t nfp = 0) {
 #pragma clang fp contract(off)
     gate = batch_gate(gate);
     if (gate && *gate) return;
     res += batch_off(gf.ndof());
     bC += batch_off(gc.ndof());
+    if (bf) bf += batch_off(nfp);
     const int nxF = gf.nnx(), nyF = gf.nny(), nzF = gf.nnz();
     const int nxC = gc.nnx(), nyC = gc.nny(), nzC = gc.nnz();
     for (int64_t nc = xcd_gtid(); nc < gc.nnodes(); nc += gstride()) {
@@ -1050,7 +1057,14 @@ __global__ void __launch_bounds__(256) k_restrict(const T* __restrict__ res, T* 
 #pragma unroll
                     for (int d = 0; d < 3; d++) acc[d] += w * v[(iz * 3 + iy) * 3 + ix][d];
                 }
-        for (int d = 0; d < 3; d++) bC[3 * nc + d] = fc[d] ? T(0) : acc[d];
+        if (bf) {
+            for (int d = 0; d < 3; d++) {
+                const int32_t pos = freeOf[3 * nc + d];
+                if (pos >= 0) bf[pos] = (TB)(fc[d] ? T(0) : acc[d]);
+            }
+        } else {
+            for (int d = 0; d < 3; d++) bC[3 * nc + d] = fc[d] ? T(0) : acc[d];
+        }
     }
 }
 

@@ -1028,6 +1028,10 @@ struct MGH {
     int xm_cx = 0;        // x-march planes per workgroup for every level (0 auto)
     int xm_cx64 = 0;      // x-march planes per workgroup of every FP64 operator (0: autotuned / as FP32)
     int ncu = 256;        // compute units of the device (the x-chunk size rule)
+    // the coarsest level's gathered right-hand side (bf) is zero at its
+    // padding positions and its solution vectors at the fixed DOFs, so the
+    // restriction may write bf directly (set by build once they are zeroed)
+    bool bf_direct = false;
     // FP32 operator on the x-march levels with the element product on the
     // matrix cores (k_applyA_mf) instead of the VALU Hadamard kernel
     bool mfma = true;
@@ -1253,6 +1257,24 @@ struct MGH {
             }
         }
         upload_omega(c);
+        // the coarsest direct solve's buffers: bf all zero (only free
+        // positions are written later), x and x32 zero (the solve writes the
+        // free DOFs; the power iterations used x as scratch)
+        bf_direct = false;
+        {
+            MGLevel& lc = L.back();
+            if (L.size() > 1 && lc.chol && lc.bf && lc.freeOf) {
+                hipLaunchKernelGGL(k_fill<double>, dim3(nblk(lc.nfp * nrhs)), dim3(256), 0, c.s, lc.bf, 0.0,
+                                   lc.nfp * nrhs, (const int*)nullptr);
+                hipLaunchKernelGGL(k_fill<double>, dim3(nblk(lc.ndof * nrhs)), dim3(256), 0, c.s, lc.x, 0.0,
+                                   lc.ndof * nrhs, (const int*)nullptr);
+                if (lc.x32)
+                    hipLaunchKernelGGL(k_fill<float>, dim3(nblk(lc.ndof * nrhs)), dim3(256), 0, c.s, lc.x32, 0.0f,
+                                       lc.ndof * nrhs, (const int*)nullptr);
+                c.check_launch();
+                bf_direct = true;
+            }
+        }
         if (bddc) bddc_build(c);
     }
 
@@ -1921,7 +1943,7 @@ struct MGH {
     // when it could not).
     template <typename T>
     T* vcycle(Ctx& c, int li, const T* b, const int* gate, int nr = 1, const double* rdot = nullptr,
-              double* rzpart = nullptr, int* nrz = nullptr) {
+              double* rzpart = nullptr, int* nrz = nullptr, bool b_in_bf = false) {
         MGLevel& l = L[li];
         T* x = Sel<T>::x(l);
         T* x2 = Sel<T>::x2(l);
@@ -1931,7 +1953,10 @@ struct MGH {
             // ytmp); the gather also zeroes x (b is another buffer)
             const bool f16 = std::is_same<T, float>::value && l.Z16 && l.tri;
             const int gn = nblk(std::max(l.nfp, l.ndof));
-            if (f16)
+            // (b_in_bf: the restriction wrote bf itself, and x is zero at
+            // the fixed DOFs from the build)
+            if (b_in_bf) {
+            } else if (f16)
                 hipLaunchKernelGGL((k_gather_free<T, float>), dim3(gn, nr), dim3(256), 0, c.s, b, l.freeIdx, l.nf,
                                    l.nfp, (float*)l.bf, gate, l.ndof, x);
             else
@@ -2022,9 +2047,21 @@ struct MGH {
         if (coarsest) return x;
         apply<T>(c, li, x, Sel<T>::res(l), 2, b, gate, nr);
         MGLevel& lc = L[li + 1];
-        hipLaunchKernelGGL(k_restrict<T>, dim3(nblk8(lc.g.nnodes()), nr), dim3(256), 0, c.s, (const T*)Sel<T>::res(l),
-                           Sel<T>::b(lc), l.g, lc.g, lc.fixed, gate);
-        T* xc = vcycle<T>(c, li + 1, Sel<T>::b(lc), gate, nr);
+        // into the coarsest direct solve's gathered right-hand side when the
+        // next level is that solve (its gather kernel is then not needed)
+        const bool to_bf = bf_direct && li + 1 == (int)L.size() - 1 && lc.chol && lc.bf && lc.freeOf;
+        if (to_bf && std::is_same<T, float>::value && lc.Z16 && lc.tri)
+            hipLaunchKernelGGL((k_restrict<T, float>), dim3(nblk8(lc.g.nnodes()), nr), dim3(256), 0, c.s,
+                               (const T*)Sel<T>::res(l), Sel<T>::b(lc), l.g, lc.g, lc.fixed, gate, (float*)lc.bf,
+                               (const int32_t*)lc.freeOf, lc.nfp);
+        else if (to_bf)
+            hipLaunchKernelGGL((k_restrict<T, double>), dim3(nblk8(lc.g.nnodes()), nr), dim3(256), 0, c.s,
+                               (const T*)Sel<T>::res(l), Sel<T>::b(lc), l.g, lc.g, lc.fixed, gate, lc.bf,
+                               (const int32_t*)lc.freeOf, lc.nfp);
+        else
+            hipLaunchKernelGGL(k_restrict<T>, dim3(nblk8(lc.g.nnodes()), nr), dim3(256), 0, c.s,
+                               (const T*)Sel<T>::res(l), Sel<T>::b(lc), l.g, lc.g, lc.fixed, gate);
+        T* xc = vcycle<T>(c, li + 1, Sel<T>::b(lc), gate, nr, nullptr, nullptr, nullptr, to_bf);
         hipLaunchKernelGGL(k_prolong_add<T>, dim3(nblk8(lc.g.nnodes()), nr), dim3(256), 0, c.s, x, (const T*)xc, l.g,
                            lc.g, l.fixed, gate);
         for (int s = 0; s < nu2; s++) {
