@@ -623,6 +623,48 @@ __device__ __forceinline__ void sum_partials(const double* part, int k0, int nr,
     for (int i = 0; i < NV; i++) tot[i] = res[i];
 }
 
+// sum_partials of two quantities at once (rows (ka + i / nr, i % nr) over
+// nba columns into ta, rows (kb + i / nr, i % nr) over nbb into tb), each
+// row summed exactly as sum_partials sums it; one pass, so the loads of all
+// the rows are in flight together
+template <int NR>
+__device__ __forceinline__ void sum_partials2(const double* part, int nr, int ka, int nba, double (&ta)[NR],
+                                              int kb, int nbb, double (&tb)[NR]) {
+    constexpr int NV = 2 * NR;
+    __shared__ double sh[NV * (kRedThreads / 64)];
+    __shared__ double res[NV];
+    double w[NV];
+#pragma unroll
+    for (int i = 0; i < NV; i++) {
+        const int nb = i < NR ? nba : nbb;
+        const double* row = part + (i < NR ? cg_row(ka + i / nr, i % nr) : cg_row(kb + (i - NR) / nr, (i - NR) % nr));
+        double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        int b = threadIdx.x;
+        for (; b + 7 * kRedThreads < nb; b += 8 * kRedThreads) {
+#pragma unroll
+            for (int u = 0; u < 8; u++) a[u] += row[b + u * kRedThreads];
+        }
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int c = b + u * kRedThreads;
+            v[u] = row[c < nb ? c : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) a[u] += (b + u * kRedThreads < nb) ? v[u] : 0.0;
+        w[i] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+    }
+    block_reduce<NV>(w, sh);
+    if (threadIdx.x == 0)
+        for (int i = 0; i < NV; i++) res[i] = w[i];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NR; i++) {
+        ta[i] = res[i];
+        tb[i] = res[NR + i];
+    }
+}
+
 // p.Ap partials (quantity 0)
 template <typename T, int NR>
 __global__ void __launch_bounds__(kRedThreads) k_cgp_pAp(const T* __restrict__ p, const T* __restrict__ Ap, int64_t n,
@@ -652,6 +694,17 @@ __global__ void __launch_bounds__(kRedThreads) k_cgp_update_ur(TU* __restrict__ 
                                                               int64_t n, const CGState* st, double* part,
                                                               TL* __restrict__ r_lo, int nb0) {
     if (all_done(st, NR)) return;
+    // the first KP elements of r and Ap of right-hand side 0 are loaded
+    // before the partial sums (the U update, when here, reads as it goes)
+    constexpr int KP = 4;
+    T rp[KP], ap[KP];
+#pragma unroll
+    for (int k = 0; k < KP; k++) {
+        const int64_t i = gtid() + k * gstride();
+        const int64_t ii = i < n ? i : 0;
+        rp[k] = r[ii];
+        ap[k] = Ap[ii];
+    }
     double pAp[NR];
     sum_partials<NR>(part, 0, NR, pAp, nb0);
     double v[NR];
@@ -665,7 +718,19 @@ __global__ void __launch_bounds__(kRedThreads) k_cgp_update_ur(TU* __restrict__ 
             const T* pq = p + q * n;
             const T* aq = Ap + q * n;
             TL* lq = r_lo ? r_lo + q * n : nullptr;
-            for (int64_t i = gtid(); i < n; i += gstride()) {
+            int64_t i = gtid();
+            if (q == 0) {
+#pragma unroll
+                for (int k = 0; k < KP; k++, i += gstride())
+                    if (i < n) {
+                        if (U) Uq[i] = (TU)((double)Uq[i] + alpha * (double)pq[i]);
+                        const T ri = (T)((double)rp[k] - alpha * (double)ap[k]);
+                        rq[i] = ri;
+                        if (lq) lq[i] = (TL)ri;
+                        a += (double)ri * (double)ri;
+                    }
+            }
+            for (; i < n; i += gstride()) {
                 if (U) Uq[i] = (TU)((double)Uq[i] + alpha * (double)pq[i]);
                 const T ri = (T)((double)rq[i] - alpha * (double)aq[i]);
                 rq[i] = ri;
@@ -803,9 +868,23 @@ __global__ void __launch_bounds__(kRedThreads) k_cgp_pupdate_books(T* __restrict
         if (books && threadIdx.x < NR) stn[threadIdx.x] = st[threadIdx.x];
         return;
     }
+    // the first KP elements of this thread's share of right-hand side 0 are
+    // loaded before the partial sums, so the streaming starts under their
+    // latency
+    constexpr int KP = 4;
+    T pp[KP];
+    TZ zp[KP];
+    TU up[KP];
+#pragma unroll
+    for (int k = 0; k < KP; k++) {
+        const int64_t i = gtid() + k * gstride();
+        const int64_t ii = i < n ? i : 0;
+        pp[k] = p[ii];
+        zp[k] = zz[ii];
+        up[k] = U[ii];
+    }
     double rz[NR], pAp[NR];
-    sum_partials<NR>(part, 2, NR, rz, nb2);
-    sum_partials<NR>(part, 0, NR, pAp, nb0);
+    sum_partials2<NR>(part, NR, 2, nb2, rz, 0, nb0, pAp);
 #pragma unroll
     for (int q = 0; q < NR; q++) {
         if (st[q].done) continue;
@@ -814,7 +893,17 @@ __global__ void __launch_bounds__(kRedThreads) k_cgp_pupdate_books(T* __restrict
         T* pq = p + q * n;
         const TZ* zq = zz + q * n;
         TU* Uq = U + q * n;
-        for (int64_t i = gtid(); i < n; i += gstride()) {
+        int64_t i = gtid();
+        if (q == 0) {
+#pragma unroll
+            for (int k = 0; k < KP; k++, i += gstride())
+                if (i < n) {
+                    const T pi = pp[k];
+                    Uq[i] = (TU)((double)up[k] + alpha * (double)pi);
+                    pq[i] = (T)zp[k] + beta * pi;
+                }
+        }
+        for (; i < n; i += gstride()) {
             const T pi = pq[i];
             Uq[i] = (TU)((double)Uq[i] + alpha * (double)pi);
             pq[i] = (T)zq[i] + beta * pi;
